@@ -1,0 +1,85 @@
+// admx_common.hpp -- device helpers shared by the CDNA4 kernels of libadmixhip.
+//
+// Data layout in HBM (identical to the reference's Julia arrays, column-major):
+//   BED   : SNP-major 2-bit genotypes, SNP j occupies `pitch` bytes starting at
+//           bed + pitch*j (pitch = ceil(I/4) rounded up to 16 bytes so 16-byte
+//           vector loads of a SNP column are aligned); individual i sits in
+//           byte i>>2 at bit offset 2*(i&3) (src/loops.jl:529-534).
+//   q     : K x I  (q[k,i] at k + K*i),  p : K x J  (p[k,j] at k + K*j)
+//   x     : K x (I+J), q = x[:, 0:I], p = x[:, I:I+J]  (src/structs.jl:120-137)
+//   XtX   : K x K x n,  Xtz : K x n
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace admx {
+
+constexpr int kMaxK = 16;
+constexpr int kMaxQ = 8;  // quasi-Newton history columns supported on device
+
+// PLINK-1 code -> (m*g, m*(2-g)) with m the nonmissing flag:
+//   00 -> (0,2)  01 (missing) -> (0,0)  10 -> (1,1)  11 -> (2,0)
+// i.e. g_map = [0,0,1,2], nonmissing_map = [1,0,1,1] of src/loops.jl:2-5.
+__device__ __forceinline__ int code_g(uint32_t c) { return (0x02010000u >> (c * 8u)) & 0xffu; }
+__device__ __forceinline__ int code_h(uint32_t c) { return (0x00010002u >> (c * 8u)) & 0xffu; }
+
+// Reciprocal: hardware estimate refined by Newton steps (2 for FP64, 1 for FP32).
+__device__ __forceinline__ double rcp(double x) {
+  double r = __builtin_amdgcn_rcp(x);
+  double e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  e = __builtin_fma(-x, r, 1.0);
+  r = __builtin_fma(r, e, r);
+  return r;
+}
+__device__ __forceinline__ float rcp(float x) {
+  float r = __builtin_amdgcn_rcpf(x);
+  float e = __builtin_fmaf(-x, r, 1.0f);
+  return __builtin_fmaf(r, e, r);
+}
+
+template <typename T> __device__ __forceinline__ T tlog(T x);
+template <> __device__ __forceinline__ double tlog<double>(double x) { return log(x); }
+template <> __device__ __forceinline__ float tlog<float>(float x) { return logf(x); }
+
+// Number of upper-triangle entries of a K x K symmetric matrix.
+template <int K> struct Tri { static constexpr int N = K * (K + 1) / 2; };
+
+// Packed index of (a <= b) in row-major upper-triangle order.
+__host__ __device__ constexpr int tri_idx(int a, int b, int K) {
+  return a * K - a * (a - 1) / 2 + (b - a);
+}
+
+// Deterministic in-block tree sum of one double per thread (blockDim.x = NT).
+template <int NT>
+__device__ __forceinline__ double block_sum(double v, double* sh) {
+  const int t = threadIdx.x;
+  sh[t] = v;
+  __syncthreads();
+#pragma unroll
+  for (int s = NT / 2; s > 0; s >>= 1) {
+    if (t < s) sh[t] += sh[t + s];
+    __syncthreads();
+  }
+  const double r = sh[0];
+  __syncthreads();
+  return r;
+}
+
+}  // namespace admx
+
+// Host-side dispatch over K = 1..16: CALL is an expression that uses the
+// compile-time constant KK.
+#define ADMX_KCASE(k, CALL) \
+  case k: {                 \
+    constexpr int KK = k;   \
+    CALL;                   \
+  } break;
+#define ADMX_DISPATCH_K(Kval, CALL)                                                   \
+  switch (Kval) {                                                                     \
+    ADMX_KCASE(1, CALL) ADMX_KCASE(2, CALL) ADMX_KCASE(3, CALL) ADMX_KCASE(4, CALL)     \
+    ADMX_KCASE(5, CALL) ADMX_KCASE(6, CALL) ADMX_KCASE(7, CALL) ADMX_KCASE(8, CALL)     \
+    ADMX_KCASE(9, CALL) ADMX_KCASE(10, CALL) ADMX_KCASE(11, CALL) ADMX_KCASE(12, CALL)  \
+    ADMX_KCASE(13, CALL) ADMX_KCASE(14, CALL) ADMX_KCASE(15, CALL) ADMX_KCASE(16, CALL) \
+    default: break;                                                                   \
+  }

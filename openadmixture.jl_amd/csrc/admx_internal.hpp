@@ -1,0 +1,89 @@
+// admx_internal.hpp -- host-side declarations shared by the translation units
+// of libadmixhip (launchers of the CDNA4 kernels).  Not part of the public ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace admx {
+
+enum PassMode : int { MODE_UPDATE = 0, MODE_EM = 1, MODE_LL = 2 };
+
+// Arguments of one genotype pass over [i0,i1) x [j0,j1).
+struct PassArgs {
+  const uint8_t* bed;
+  int64_t pitch;   // bytes per SNP column (multiple of 16)
+  int64_t I;       // total individuals (bound for clamped loads)
+  const void* q;   // K x I
+  const void* p;   // K x J
+  int64_t i0, i1, j0, j1;
+  int64_t chunk;   // SNPs per chunk (Q side) / individuals per chunk, multiple of 16 (P side)
+  void* part;      // partial sums [nchunk][NA][n] (UPDATE/EM) or per-block doubles (LL)
+};
+
+// Per-column reduction of pass partials, optional QP + projection.
+struct FinArgs {
+  const void* part;
+  int64_t n;        // columns in this pass (Ilen or Jlen)
+  int nchunk;
+  int64_t off;      // global index of the first column (i0 or j0)
+  int64_t J;        // total SNPs (EM-q divisor 2J)
+  const void* x_in; // K x ncols parameters of the pass (q for Q side, p for P side)
+  void* x_out;      // K x ncols output (q_next / p_next) or EM output
+  void* aux;        // EM-p: p_tmp output (loop API)
+  void* XtX;        // K x K x ncols (nullable)
+  void* Xtz;        // K x ncols (nullable)
+  int accumulate;   // loop API: outputs += partial sums
+  int neg_xtz;      // store -Xtz (update_q!/update_p! negate Xtz before the QP)
+  int32_t* status;  // QP return codes (nullable)
+};
+
+// Number of per-column accumulators of a pass.
+inline int pass_na(int mode, bool qside, int K, bool f64) {
+  const int tri = K * (K + 1) / 2;
+  if (mode == MODE_EM) return qside ? K : 2 * K;
+  if (mode == MODE_UPDATE) {
+    if (qside) return f64 ? tri + 2 * K + 2 : tri + K;
+    return tri + K;
+  }
+  return 1;
+}
+
+template <typename T> void launch_qside(int mode, int K, const PassArgs& a, int nchunk, hipStream_t s);
+template <typename T> void launch_pside(int mode, int K, const PassArgs& a, int nchunk, hipStream_t s);
+
+// Q side: update finalize (+QP simplex +project_q); EM-q finalize.
+template <typename T> void launch_fin_q_update(int K, const FinArgs& f, bool do_qp, hipStream_t s);
+template <typename T> void launch_fin_q_em(int K, const FinArgs& f, hipStream_t s);
+// P side: update finalize (+QP box +project_p); EM-p finalize.
+template <typename T> void launch_fin_p_update(int K, const FinArgs& f, bool do_qp, hipStream_t s);
+template <typename T> void launch_fin_p_em(int K, const FinArgs& f, hipStream_t s);
+
+// Batched QP on host-provided tableau inputs (XtX, negated Xtz, x): delta out.
+template <typename T> void launch_qp_batch(int K, bool simplex, int64_t n, const void* XtX, const void* Xtz_neg,
+                                           const void* x, void* delta, int32_t* status, hipStream_t s);
+template <typename T> void launch_project_q(int K, int64_t n, void* b, hipStream_t s);
+template <typename T> void launch_project_p(int64_t n, void* b, hipStream_t s);
+
+// LL partial reduction: sum nparts doubles (fixed order) into *out (device).
+void launch_reduce_sum(const double* parts, int64_t nparts, double* out, int accumulate, hipStream_t s);
+
+// ZAL quasi-Newton (src/quasi_newton.jl:5-31) on flat x of length N.
+template <typename T> void launch_zal(int K, int64_t I, int64_t J, const void* x, const void* xn, const void* xn2,
+                                      void* U, void* V, void* x_out, int col, int m, int Qcols,
+                                      double* scratch, int64_t scratch_len, hipStream_t s);
+// x <- (lls[prev] < lls[qn]) ? x_tmp : x_next2 ; lls[new] accordingly.
+template <typename T> void launch_accept(int64_t N, void* x, const void* x_tmp, const void* x_next2, double* lls,
+                                         hipStream_t s);
+
+// BED decode / counts.
+void launch_decode(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int8_t* g, hipStream_t s);
+void launch_counts(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, unsigned long long* snp_nm,
+                   unsigned long long* snp_alt, unsigned long long* ind_nm, hipStream_t s);
+// Synthetic genotypes from (Q,P) (double), seeded counter hash.
+void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, int K, uint64_t seed,
+                    double miss_rate, uint8_t* bed, int64_t pitch, hipStream_t s);
+
+// Slots of the device log-likelihood vector.
+enum LLSlot : int { LL_PREV = 0, LL_NEW = 1, LL_BASIC = 2, LL_QN = 3, LL_TMP = 4, LL_NSLOTS = 8 };
+
+}  // namespace admx

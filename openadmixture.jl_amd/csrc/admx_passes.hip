@@ -1,0 +1,275 @@
+// admx_passes.hip -- the genotype passes: log-likelihood, FRAPPE-EM and the
+// per-genotype gradient + K x K Hessian accumulation, decoded straight from
+// the 2-bit SNP-major BED in HBM.
+//
+// Q side (log-likelihood, em_q, update_q; src/loops.jl:517-539, 610-632,
+// 867-894): one lane per individual i, a workgroup streams a chunk of SNPs.
+// For each SNP every lane needs the same p[:,j] -- a wave-uniform address --
+// so P arrives through scalar loads into SGPRs and feeds the FMAs as the
+// scalar operand; the lane's own q[:,i] and all accumulators live in VGPRs.
+// P side (em_p, update_p; src/loops.jl:700-721, 923-950): one lane per SNP j,
+// streaming individuals; q[:,i] is wave-uniform (SGPRs) and each lane reads
+// 16 genotypes of its own SNP column per 32-bit load.
+//
+// Every chunk writes its partial sums to HBM; a second kernel reduces them in
+// a fixed order, so results are bitwise reproducible run to run (the
+// reference's CUDA path uses float atomics).
+#include "admx_common.hpp"
+#include "admx_internal.hpp"
+
+namespace admx {
+
+// ------------------------------------------------------------------ Q side
+// Per-genotype algebra, with s = q_i . p_j, t = 1 - s, gm = m*g, hm = m*(2-g):
+//   c = gm/s, d = hm/t, a = c/s, b = d/t
+//   Xtz_q[k,i]    += c p_k + d (1 - p_k)
+//   XtX_q[k2,k,i] += a p_k p_k2 + b (1 - p_k)(1 - p_k2)
+// FP64 accumulates the equivalent rank-1 form
+//   XtX = sum (a+b) p p' + (sum b) 11' - 1 (sum b p)' - (sum b p) 1'
+//   Xtz = sum (c-d) p + (sum d) 1
+// (one FMA per Hessian entry instead of two; its cancellation costs at most a
+// factor 1/min(p,1-p)^2 <= 1e10 on a 1e-16 unit roundoff).  FP32 keeps the
+// direct two-term form, where that factor would matter.
+template <typename T, int K, int MODE, bool TRICK>
+__global__ __launch_bounds__(256) void k_qside(PassArgs a) {
+  constexpr int NT = Tri<K>::N;
+  const T* __restrict__ q = static_cast<const T*>(a.q);
+  const T* __restrict__ p = static_cast<const T*>(a.p);
+  const int64_t Ilen = a.i1 - a.i0;
+  const int64_t il = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool valid = il < Ilen;
+  const int64_t i = a.i0 + (valid ? il : 0);
+  const int64_t jb = a.j0 + (int64_t)blockIdx.y * a.chunk;
+  const int64_t je = (a.j1 < jb + a.chunk) ? a.j1 : jb + a.chunk;
+
+  T qi[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) qi[k] = q[k + (int64_t)K * i];
+  const uint8_t* __restrict__ gp = a.bed + (i >> 2);
+  const uint32_t sh = (uint32_t)(i & 3) * 2u;
+
+  T H[MODE == MODE_UPDATE ? NT : 1];
+  T z[K];
+  T hb[(MODE == MODE_UPDATE && TRICK) ? K : 1];
+  T sb = 0, sd = 0;
+  double acc = 0.0, prod = 1.0;
+  int esum = 0;
+#pragma unroll
+  for (int n = 0; n < (MODE == MODE_UPDATE ? NT : 1); ++n) H[n] = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) z[k] = 0;
+#pragma unroll
+  for (int k = 0; k < ((MODE == MODE_UPDATE && TRICK) ? K : 1); ++k) hb[k] = 0;
+
+  for (int64_t j = jb; j < je; ++j) {
+    const uint32_t c = ((uint32_t)gp[a.pitch * j] >> sh) & 3u;
+    const T* __restrict__ pj = p + (int64_t)K * j;
+    T pv[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) pv[k] = pj[k];
+    T s = qi[0] * pv[0];
+#pragma unroll
+    for (int k = 1; k < K; ++k) s = __builtin_fma(qi[k], pv[k], s);
+    const T t = (T)1 - s;
+    if constexpr (MODE == MODE_LL) {
+      if constexpr (sizeof(T) == 8) {
+        // prod_j s^g (1-s)^(2-g), renormalised every 8 SNPs; one log per lane.
+        const double x = (c >= 2u) ? (double)s : (double)t;
+        const double y = (c == 3u) ? (double)s : (double)t;
+        prod *= (c == 1u) ? 1.0 : x * y;
+        if (((j - jb) & 7) == 7) {
+          int e;
+          prod = __builtin_frexp(prod, &e);
+          esum += e;
+        }
+      } else {
+        // Terms in T, accumulated in Float64 (src/loops.jl:523, 535).
+        const T gm = (T)code_g(c), hm = (T)code_h(c);
+        const T term = gm * tlog<T>(s) + hm * tlog<T>(t);
+        acc += (double)term;
+      }
+    } else {
+      const T gm = (T)code_g(c), hm = (T)code_h(c);
+      const T r = rcp(s * t);
+      const T is = t * r, it = s * r;
+      const T cc = gm * is, dd = hm * it;
+      if constexpr (MODE == MODE_EM) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) z[k] = __builtin_fma(cc, pv[k], __builtin_fma(dd, (T)1 - pv[k], z[k]));
+      } else if constexpr (TRICK) {
+        const T aa = cc * is, bb = dd * it;
+        const T w = aa + bb, u = cc - dd;
+        sb += bb;
+        sd += dd;
+        int n = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const T wp = w * pv[k];
+          z[k] = __builtin_fma(u, pv[k], z[k]);
+          hb[k] = __builtin_fma(bb, pv[k], hb[k]);
+#pragma unroll
+          for (int k2 = k; k2 < K; ++k2, ++n) H[n] = __builtin_fma(wp, pv[k2], H[n]);
+        }
+      } else {
+        const T aa = cc * is, bb = dd * it;
+        int n = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const T mk = (T)1 - pv[k];
+          const T ap = aa * pv[k], bm = bb * mk;
+          z[k] = __builtin_fma(cc, pv[k], __builtin_fma(dd, mk, z[k]));
+#pragma unroll
+          for (int k2 = k; k2 < K; ++k2, ++n) H[n] = __builtin_fma(ap, pv[k2], __builtin_fma(bm, (T)1 - pv[k2], H[n]));
+        }
+      }
+    }
+  }
+
+  if constexpr (MODE == MODE_LL) {
+    __shared__ double sh_red[256];
+    if constexpr (sizeof(T) == 8) acc = log(prod) + (double)esum * 0.69314718055994530942;
+    const double tot = block_sum<256>(valid ? acc : 0.0, sh_red);
+    if (threadIdx.x == 0) static_cast<double*>(a.part)[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = tot;
+    return;
+  } else {
+    if (!valid) return;
+    T* __restrict__ part = static_cast<T*>(a.part);
+    const int NA = (MODE == MODE_EM) ? K : (TRICK ? NT + 2 * K + 2 : NT + K);
+    T* out = part + (int64_t)blockIdx.y * NA * Ilen + il;
+    int o = 0;
+    if constexpr (MODE == MODE_UPDATE) {
+#pragma unroll
+      for (int n = 0; n < NT; ++n) out[(o++) * Ilen] = H[n];
+    }
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[(o++) * Ilen] = z[k];
+    if constexpr (MODE == MODE_UPDATE && TRICK) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) out[(o++) * Ilen] = hb[k];
+      out[(o++) * Ilen] = sb;
+      out[(o++) * Ilen] = sd;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ P side
+//   Xtz_p[k,j]    += (c - d) q_k               (exact dLL/dp)
+//   XtX_p[k2,k,j] += (a + b) q_k q_k2          (exact -d2LL/dp2)
+//   EM: p_tmp += c q_k p_k ; p_next += d q_k (1 - p_k)
+template <typename T, int K, int MODE>
+__global__ __launch_bounds__(256) void k_pside(PassArgs a) {
+  constexpr int NT = Tri<K>::N;
+  const T* __restrict__ q = static_cast<const T*>(a.q);
+  const T* __restrict__ p = static_cast<const T*>(a.p);
+  const int64_t Jlen = a.j1 - a.j0;
+  const int64_t jl = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const bool valid = jl < Jlen;
+  const int64_t j = a.j0 + (valid ? jl : 0);
+  const int64_t ia0 = (a.i0 & ~(int64_t)15) + (int64_t)blockIdx.y * a.chunk;
+  const int64_t ia1 = (a.i1 < ia0 + a.chunk) ? a.i1 : ia0 + a.chunk;
+
+  T pj[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) pj[k] = p[k + (int64_t)K * j];
+  const uint8_t* __restrict__ col = a.bed + a.pitch * j;
+
+  T H[MODE == MODE_UPDATE ? NT : 1];
+  T z[K], e2[MODE == MODE_EM ? K : 1];
+#pragma unroll
+  for (int n = 0; n < (MODE == MODE_UPDATE ? NT : 1); ++n) H[n] = 0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) z[k] = 0;
+#pragma unroll
+  for (int k = 0; k < (MODE == MODE_EM ? K : 1); ++k) e2[k] = 0;
+
+  for (int64_t ib = ia0; ib < ia1; ib += 16) {
+    uint32_t w = *reinterpret_cast<const uint32_t*>(col + (ib >> 2));
+    if (ib < a.i0 || ib + 16 > a.i1) {  // uniform: mark out-of-range individuals missing
+      uint32_t keep = 0;
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+        if (ib + t >= a.i0 && ib + t < a.i1) keep |= 3u << (2 * t);
+      w = (w & keep) | (0x55555555u & ~keep);
+    }
+#pragma unroll
+    for (int t = 0; t < 16; ++t) {
+      const int64_t i = (ib + t < a.I) ? ib + t : a.I - 1;
+      const uint32_t c = (w >> (2 * t)) & 3u;
+      const T* __restrict__ qi = q + (int64_t)K * i;
+      T qv[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) qv[k] = qi[k];
+      T s = qv[0] * pj[0];
+#pragma unroll
+      for (int k = 1; k < K; ++k) s = __builtin_fma(qv[k], pj[k], s);
+      const T tt = (T)1 - s;
+      const T gm = (T)code_g(c), hm = (T)code_h(c);
+      const T r = rcp(s * tt);
+      const T is = tt * r, it = s * r;
+      const T cc = gm * is, dd = hm * it;
+      if constexpr (MODE == MODE_EM) {
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          z[k] = __builtin_fma(cc, qv[k], z[k]);
+          e2[k] = __builtin_fma(dd, qv[k], e2[k]);
+        }
+      } else {
+        const T w2 = cc * is + dd * it, u = cc - dd;
+        int n = 0;
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          const T wq = w2 * qv[k];
+          z[k] = __builtin_fma(u, qv[k], z[k]);
+#pragma unroll
+          for (int k2 = k; k2 < K; ++k2, ++n) H[n] = __builtin_fma(wq, qv[k2], H[n]);
+        }
+      }
+    }
+  }
+  if (!valid) return;
+  T* __restrict__ part = static_cast<T*>(a.part);
+  const int NA = (MODE == MODE_EM) ? 2 * K : NT + K;
+  T* out = part + (int64_t)blockIdx.y * NA * Jlen + jl;
+  int o = 0;
+  if constexpr (MODE == MODE_UPDATE) {
+#pragma unroll
+    for (int n = 0; n < NT; ++n) out[(o++) * Jlen] = H[n];
+  }
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[(o++) * Jlen] = z[k];
+  if constexpr (MODE == MODE_EM) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) out[(o++) * Jlen] = e2[k];
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+template <typename T, int K>
+static void qside_k(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
+  const int64_t n = a.i1 - a.i0;
+  dim3 grid((unsigned)((n + 255) / 256), (unsigned)nchunk);
+  constexpr bool trick = sizeof(T) == 8;
+  if (mode == MODE_UPDATE) hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick>), grid, dim3(256), 0, s, a);
+  else if (mode == MODE_EM) hipLaunchKernelGGL((k_qside<T, K, MODE_EM, false>), grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_qside<T, K, MODE_LL, false>), grid, dim3(256), 0, s, a);
+}
+
+template <typename T, int K>
+static void pside_k(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
+  const int64_t n = a.j1 - a.j0;
+  dim3 grid((unsigned)((n + 255) / 256), (unsigned)nchunk);
+  if (mode == MODE_UPDATE) hipLaunchKernelGGL((k_pside<T, K, MODE_UPDATE>), grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_pside<T, K, MODE_EM>), grid, dim3(256), 0, s, a);
+}
+
+template <typename T> void launch_qside(int mode, int K, const PassArgs& a, int nchunk, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (qside_k<T, KK>(mode, a, nchunk, s)));
+}
+template <typename T> void launch_pside(int mode, int K, const PassArgs& a, int nchunk, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (pside_k<T, KK>(mode, a, nchunk, s)));
+}
+
+template void launch_qside<ADMX_T>(int, int, const PassArgs&, int, hipStream_t);
+template void launch_pside<ADMX_T>(int, int, const PassArgs&, int, hipStream_t);
+
+}  // namespace admx

@@ -1,0 +1,550 @@
+// admx_qp.hip -- per-column reduction of the pass partials fused with the
+// batched K-dim QPs and the projections:
+//   update_q!: Xtz .*= -1; per individual create_tableau!(simplex=true) +
+//              quadratic_program!(c=1); q_next = q + delta; project_q!
+//              (src/algorithms_inner.jl:199-227, src/qp.jl:8-76, 171-317,
+//              src/projections.jl:1-63)
+//   update_p!: per SNP create_tableau!(simplex=false) + quadratic_program!(c=0);
+//              p_next = p + delta; project_p!  (src/algorithms_inner.jl:275-305)
+//
+// One lane owns one problem.  Its (K+c+1)^2 sweep tableau, diagonal copy,
+// sweep column, step and parameters sit in LDS, interleaved across the
+// workgroup (element e of lane t at e*NT + t), so every LDS access of a wave
+// touches consecutive addresses.  The active-set loop is the reference's
+// Jennrich-Sampson method step for step; arithmetic is evaluated without FMA
+// contraction, as Julia evaluates it outside @turbo.
+#include "admx_common.hpp"
+#include "admx_internal.hpp"
+
+namespace admx {
+
+template <typename T, int K, int C, int NT>
+struct QPLayout {
+  static constexpr int SZ = K + C + 1;
+  static constexpr int TAB = 0;
+  static constexpr int DG = SZ * SZ;
+  static constexpr int TMP = DG + SZ;
+  static constexpr int DEL = TMP + SZ;
+  static constexpr int PAR = DEL + K;
+  static constexpr int TOTAL = PAR + K;
+};
+
+// Lanes per workgroup of the QP kernels: the LDS footprint grows as K^2.
+template <int K> struct QPLanes { static constexpr int N = K <= 12 ? 64 : 32; };
+
+#define QP_TB(r, c) L[((r) + SZ * (c)) * NT + t]
+#define QP_DG(i) L[(LY::DG + (i)) * NT + t]
+#define QP_TMP(i) L[(LY::TMP + (i)) * NT + t]
+#define QP_DEL(i) L[(LY::DEL + (i)) * NT + t]
+#define QP_PAR(i) L[(LY::PAR + (i)) * NT + t]
+
+// sweep!(A, k, tmp, inverse)  (src/qp.jl:297-317).  The saved column lives in
+// registers (constant index, inner loop) and in LDS (runtime index, outer loop).
+template <typename T, int K, int C, int NT>
+__device__ __forceinline__ void qp_sweep(T* __restrict__ L, int t, int k, bool inverse) {
+#pragma clang fp contract(off)
+  using LY = QPLayout<T, K, C, NT>;
+  constexpr int SZ = K + C + 1;
+  const T pk = (T)1 / QP_TB(k, k);
+  T tmp[SZ];
+#pragma unroll
+  for (int i = 0; i < SZ; ++i) tmp[i] = QP_TB(i, k);
+#pragma unroll
+  for (int i = 0; i < SZ; ++i) {
+    QP_TB(i, k) = 0;
+    QP_TB(k, i) = 0;
+  }
+  const T sg = inverse ? (T)1 : (T)-1;
+#pragma unroll
+  for (int i = 0; i < SZ; ++i) {
+    tmp[i] = (i == k) ? sg : tmp[i];
+    QP_TMP(i) = tmp[i];
+  }
+  for (int i = 0; i < SZ; ++i) {
+    const T pv = pk * QP_TMP(i);
+#pragma unroll
+    for (int j = 0; j < SZ; ++j) QP_TB(j, i) = QP_TB(j, i) - pv * tmp[j];
+  }
+}
+
+// quadratic_program!(delta, tableau, par, pmin=0, pmax=1, p=K, c)  (src/qp.jl:171-292)
+template <typename T, int K, int C, int NT>
+__device__ int qp_solve(T* __restrict__ L, int t) {
+#pragma clang fp contract(off)
+  using LY = QPLayout<T, K, C, NT>;
+  constexpr int SZ = LY::SZ, E = SZ - 1;
+  const T small = (T)1e-5, tol = (T)1e-8;
+  for (int i = 0; i < K; ++i) QP_DEL(i) = 0;
+  for (int i = 0; i < SZ; ++i) QP_DG(i) = QP_TB(i, i);
+  for (int i = 0; i < K; ++i) {
+    if (QP_DG(i) <= (T)0 || QP_TB(i, i) < QP_DG(i) * tol) return 0;
+    qp_sweep<T, K, C, NT>(L, t, i, false);
+  }
+  uint32_t swept = (K >= 32) ? 0xffffffffu : ((1u << K) - 1u);
+  for (int i = K; i < K + C; ++i) {
+    if (QP_TB(i, i) >= (T)0) return 0;
+    qp_sweep<T, K, C, NT>(L, t, i, false);
+  }
+  for (int iteration = 1; iteration <= 1000; ++iteration) {
+    T a = 1;
+    for (int i = 0; i < K; ++i) {
+      if ((swept >> i) & 1u) {
+        const T ui = QP_TB(i, E);
+        T ai;
+        if (ui > (T)0) ai = (T)1 - QP_PAR(i) - QP_DEL(i);
+        else ai = (T)0 - QP_PAR(i) - QP_DEL(i);
+        if ((T)fabs((double)ui) > (T)1e-10) {
+          const T r = ai / ui;
+          a = r < a ? r : a;
+        }
+      }
+    }
+    for (int i = 0; i < K; ++i) {
+      if ((swept >> i) & 1u) {
+        const T ui = QP_TB(i, E);
+        QP_DEL(i) = QP_DEL(i) + a * ui;
+        QP_TB(i, E) = ((T)1 - a) * ui;
+        QP_TB(E, i) = QP_TB(i, E);
+      }
+    }
+    int pick = -1;
+    for (int i = 0; i < K; ++i) {
+      const T xi = QP_PAR(i) + QP_DEL(i);
+      const bool critical = ((T)0 >= xi - small) || ((T)1 <= xi + small);
+      if (((swept >> i) & 1u) && fabs((double)QP_TB(i, i)) > 1e-10 && critical) {
+        pick = i;
+        break;
+      }
+    }
+    if (pick >= 0) {
+      qp_sweep<T, K, C, NT>(L, t, pick, true);
+      swept &= ~(1u << pick);
+      continue;
+    }
+    for (int i = 0; i < K; ++i) {
+      const T ui = QP_TB(i, E);
+      const T xi = QP_PAR(i) + QP_DEL(i);
+      const bool violation = (ui > (T)0 && (T)0 >= xi - small) || (ui < (T)0 && (T)1 <= xi + small);
+      if (!((swept >> i) & 1u) && violation) {
+        pick = i;
+        break;
+      }
+    }
+    if (pick >= 0) {
+      qp_sweep<T, K, C, NT>(L, t, pick, false);
+      swept |= 1u << pick;
+      continue;
+    }
+    return iteration;
+  }
+  return 0;
+}
+
+// project_q!(b; pseudocount=1e-5) on one column held in registers
+// (src/projections.jl:27-63).  Only the sorted values enter the threshold, so
+// an odd-even transposition network replaces the reference's index quicksort.
+template <typename T, int K>
+__device__ __forceinline__ void project_simplex(T (&v)[K]) {
+#pragma clang fp contract(off)
+  const T pc = (T)1e-5;
+  const T tau = (T)1 - (T)K * pc;
+  T s[K];
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    v[i] -= pc;
+    s[i] = v[i];
+  }
+#pragma unroll
+  for (int pass = 0; pass < K; ++pass) {
+#pragma unroll
+    for (int i = pass & 1; i + 1 < K; i += 2) {
+      const T hi = s[i] < s[i + 1] ? s[i + 1] : s[i];
+      const T lo = s[i] < s[i + 1] ? s[i] : s[i + 1];
+      s[i] = hi;
+      s[i + 1] = lo;
+    }
+  }
+  T tsum = 0, tmax = 0;
+  bool found = false;
+#pragma unroll
+  for (int i = 0; i + 1 < K; ++i) {
+    if (!found) {
+      tsum += s[i];
+      const T tm = (tsum - tau) / (T)(i + 1);
+      if (tm >= s[i + 1]) {
+        found = true;
+        tmax = tm;
+      }
+    }
+  }
+  if (!found) tmax = (tsum + s[K - 1] - tau) / (T)K;
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    const T d = v[i] - tmax;
+    v[i] = (d > (T)0 ? d : (T)0) + pc;
+  }
+}
+
+// project_p!(b; pseudocount=1e-5)  (src/projections.jl:69-74)
+template <typename T>
+__device__ __forceinline__ T project_box(T b) {
+  const T pc = (T)1e-5;
+  const T v = b > pc ? b : pc;
+  const T hi = (T)1 - pc;
+  return v < hi ? v : hi;
+}
+
+// Load a full K x K Hessian + gradient into the tableau of lane t, then solve.
+// create_tableau!  (src/qp.jl:8-76).  The Gerschgorin term mu of qp.jl:54-57 is
+// computed through a mul! whose output aliases its input; BLAS gemv with
+// beta = 0 clears y before reading x, so the reference executes mu = 0.
+template <typename T, int K, int C, int NT>
+__device__ __forceinline__ int qp_from_hessian(T* __restrict__ L, int t, const T (&Hf)[K * K], const T (&xtz)[K],
+                                               const T (&par)[K]) {
+#pragma clang fp contract(off)
+  using LY = QPLayout<T, K, C, NT>;
+  constexpr int SZ = LY::SZ;
+#pragma unroll
+  for (int c = 0; c < SZ; ++c)
+#pragma unroll
+    for (int r = 0; r < SZ; ++r) {
+      T v = 0;
+      if (r < K && c < K) v = Hf[r + K * c];
+      QP_TB(r, c) = v;
+    }
+#pragma unroll
+  for (int i = 0; i < K; ++i) {
+    QP_TB(SZ - 1, i) = xtz[i];  // -r with r = -Xtz
+    QP_TB(i, SZ - 1) = xtz[i];
+    QP_PAR(i) = par[i];
+  }
+  if constexpr (C == 1) {
+#pragma unroll
+    for (int i = 0; i < K; ++i) {
+      QP_TB(K, i) = 1;
+      QP_TB(i, K) = 1;
+    }
+    T sx = 0;
+#pragma unroll
+    for (int i = 0; i < K; ++i) sx += par[i];
+    QP_TB(K + 1, K) = (T)1 - sx;
+    QP_TB(K, K + 1) = (T)1 - sx;
+  }
+  return qp_solve<T, K, C, NT>(L, t);
+}
+
+// ------------------------------------------------------------------ Q side finalize
+template <typename T, int K, bool TRICK, bool DO_QP>
+__global__ __launch_bounds__(QPLanes<K>::N) void k_fin_q_update(FinArgs f) {
+  constexpr int NT = QPLanes<K>::N;
+  constexpr int NTRI = Tri<K>::N;
+  constexpr int NA = TRICK ? NTRI + 2 * K + 2 : NTRI + K;
+  using LY = QPLayout<T, K, 1, NT>;
+  __shared__ T lds[DO_QP ? LY::TOTAL * NT : 1];
+  const int t = threadIdx.x;
+  const int64_t col = (int64_t)blockIdx.x * NT + t;
+  if (col >= f.n) return;
+  const T* __restrict__ part = static_cast<const T*>(f.part);
+  T acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) acc[a] = 0;
+  for (int c = 0; c < f.nchunk; ++c) {
+    const T* src = part + (int64_t)c * NA * f.n + col;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) acc[a] += src[(int64_t)a * f.n];
+  }
+  T Hf[K * K], xtz[K];
+  {
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int k2 = k; k2 < K; ++k2, ++n) {
+        T h = acc[n];
+        if constexpr (TRICK) h = h + acc[NTRI + 2 * K] - acc[NTRI + K + k] - acc[NTRI + K + k2];
+        Hf[k2 + K * k] = h;
+        Hf[k + K * k2] = h;
+      }
+#pragma unroll
+    for (int k = 0; k < K; ++k) xtz[k] = TRICK ? acc[NTRI + k] + acc[NTRI + 2 * K + 1] : acc[NTRI + k];
+  }
+  const int64_t g = f.off + col;
+  if (f.XtX) {
+    T* X = static_cast<T*>(f.XtX) + (int64_t)K * K * g;
+#pragma unroll
+    for (int e = 0; e < K * K; ++e) X[e] = f.accumulate ? X[e] + Hf[e] : Hf[e];
+  }
+  if (f.Xtz) {
+    T* Z = static_cast<T*>(f.Xtz) + (int64_t)K * g;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const T v = f.neg_xtz ? -xtz[k] : xtz[k];
+      Z[k] = f.accumulate ? Z[k] + v : v;
+    }
+  }
+  if constexpr (DO_QP) {
+    const T* __restrict__ xin = static_cast<const T*>(f.x_in) + (int64_t)K * g;
+    T par[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) par[k] = xin[k];
+    T* L = lds;
+    const int rc = qp_from_hessian<T, K, 1, NT>(L, t, Hf, xtz, par);
+    if (f.status) f.status[g] = rc;
+    T v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = par[k] + QP_DEL(k);
+    project_simplex<T, K>(v);
+    T* xo = static_cast<T*>(f.x_out) + (int64_t)K * g;
+#pragma unroll
+    for (int k = 0; k < K; ++k) xo[k] = v[k];
+  }
+}
+
+// em_q! finalize (src/algorithms_inner.jl:76-89): q_next = q .* E / 2J, or the
+// tile loop's q_next += q .* E.
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_fin_q_em(FinArgs f) {
+  const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (col >= f.n) return;
+  const T* __restrict__ part = static_cast<const T*>(f.part);
+  T acc[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) acc[k] = 0;
+  for (int c = 0; c < f.nchunk; ++c) {
+    const T* src = part + (int64_t)c * K * f.n + col;
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] += src[(int64_t)k * f.n];
+  }
+  const int64_t g = f.off + col;
+  const T* q = static_cast<const T*>(f.x_in) + (int64_t)K * g;
+  T* o = static_cast<T*>(f.x_out) + (int64_t)K * g;
+  const T div = (T)(2 * f.J);
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const T v = q[k] * acc[k];
+    o[k] = f.accumulate ? o[k] + v : v / div;
+  }
+}
+
+// ------------------------------------------------------------------ P side finalize
+template <typename T, int K, bool DO_QP>
+__global__ __launch_bounds__(QPLanes<K>::N) void k_fin_p_update(FinArgs f) {
+  constexpr int NT = QPLanes<K>::N;
+  constexpr int NTRI = Tri<K>::N;
+  constexpr int NA = NTRI + K;
+  using LY = QPLayout<T, K, 0, NT>;
+  __shared__ T lds[DO_QP ? LY::TOTAL * NT : 1];
+  const int t = threadIdx.x;
+  const int64_t col = (int64_t)blockIdx.x * NT + t;
+  if (col >= f.n) return;
+  const T* __restrict__ part = static_cast<const T*>(f.part);
+  T acc[NA];
+#pragma unroll
+  for (int a = 0; a < NA; ++a) acc[a] = 0;
+  for (int c = 0; c < f.nchunk; ++c) {
+    const T* src = part + (int64_t)c * NA * f.n + col;
+#pragma unroll
+    for (int a = 0; a < NA; ++a) acc[a] += src[(int64_t)a * f.n];
+  }
+  T Hf[K * K], xtz[K];
+  {
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int k2 = k; k2 < K; ++k2, ++n) {
+        Hf[k2 + K * k] = acc[n];
+        Hf[k + K * k2] = acc[n];
+      }
+#pragma unroll
+    for (int k = 0; k < K; ++k) xtz[k] = acc[NTRI + k];
+  }
+  const int64_t g = f.off + col;
+  if (f.XtX) {
+    T* X = static_cast<T*>(f.XtX) + (int64_t)K * K * g;
+#pragma unroll
+    for (int e = 0; e < K * K; ++e) X[e] = f.accumulate ? X[e] + Hf[e] : Hf[e];
+  }
+  if (f.Xtz) {
+    T* Z = static_cast<T*>(f.Xtz) + (int64_t)K * g;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const T v = f.neg_xtz ? -xtz[k] : xtz[k];
+      Z[k] = f.accumulate ? Z[k] + v : v;
+    }
+  }
+  if constexpr (DO_QP) {
+    const T* __restrict__ xin = static_cast<const T*>(f.x_in) + (int64_t)K * g;
+    T par[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) par[k] = xin[k];
+    T* L = lds;
+    const int rc = qp_from_hessian<T, K, 0, NT>(L, t, Hf, xtz, par);
+    if (f.status) f.status[g] = rc;
+    T* xo = static_cast<T*>(f.x_out) + (int64_t)K * g;
+#pragma unroll
+    for (int k = 0; k < K; ++k) xo[k] = project_box<T>(par[k] + QP_DEL(k));
+  }
+}
+
+// em_p! finalize (src/algorithms_inner.jl:119-137): p_tmp = p .* A, p_next =
+// (1-p) .* B, then p_next = p_tmp / (p_tmp + p_next); or the tile loop's +=.
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_fin_p_em(FinArgs f) {
+  const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (col >= f.n) return;
+  const T* __restrict__ part = static_cast<const T*>(f.part);
+  T A[K], B[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) A[k] = B[k] = 0;
+  for (int c = 0; c < f.nchunk; ++c) {
+    const T* src = part + (int64_t)c * 2 * K * f.n + col;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      A[k] += src[(int64_t)k * f.n];
+      B[k] += src[(int64_t)(K + k) * f.n];
+    }
+  }
+  const int64_t g = f.off + col;
+  const T* p = static_cast<const T*>(f.x_in) + (int64_t)K * g;
+  T* o = static_cast<T*>(f.x_out) + (int64_t)K * g;
+  if (f.accumulate) {
+    T* pt = static_cast<T*>(f.aux) + (int64_t)K * g;
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      pt[k] = pt[k] + p[k] * A[k];
+      o[k] = o[k] + ((T)1 - p[k]) * B[k];
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const T pt = p[k] * A[k], pn = ((T)1 - p[k]) * B[k];
+      o[k] = pt / (pt + pn);
+    }
+  }
+}
+
+// ------------------------------------------------------------------ standalone batched QP
+// One problem per lane from host-shaped inputs: XtX K x K x n, Xtz (already
+// negated, as the reference stores it after `Xtz .*= -1`), x K x n.
+template <typename T, int K, int C>
+__global__ __launch_bounds__(QPLanes<K>::N) void k_qp_batch(int64_t n, const T* __restrict__ XtX,
+                                                            const T* __restrict__ Xtz_neg, const T* __restrict__ x,
+                                                            T* __restrict__ delta, int32_t* __restrict__ status) {
+  constexpr int NT = QPLanes<K>::N;
+  using LY = QPLayout<T, K, C, NT>;
+  __shared__ T lds[LY::TOTAL * NT];
+  const int t = threadIdx.x;
+  const int64_t col = (int64_t)blockIdx.x * NT + t;
+  if (col >= n) return;
+  T Hf[K * K], xtz[K], par[K];
+#pragma unroll
+  for (int e = 0; e < K * K; ++e) Hf[e] = XtX[(int64_t)K * K * col + e];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    xtz[k] = -Xtz_neg[(int64_t)K * col + k];
+    par[k] = x[(int64_t)K * col + k];
+  }
+  T* L = lds;
+  const int rc = qp_from_hessian<T, K, C, NT>(L, t, Hf, xtz, par);
+  if (status) status[col] = rc;
+#pragma unroll
+  for (int k = 0; k < K; ++k) delta[(int64_t)K * col + k] = QP_DEL(k);
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_project_q(int64_t n, T* __restrict__ b) {
+  const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (col >= n) return;
+  T v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = b[(int64_t)K * col + k];
+  project_simplex<T, K>(v);
+#pragma unroll
+  for (int k = 0; k < K; ++k) b[(int64_t)K * col + k] = v[k];
+}
+
+template <typename T>
+__global__ __launch_bounds__(256) void k_project_p(int64_t n, T* __restrict__ b) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256)
+    b[e] = project_box<T>(b[e]);
+}
+
+// ------------------------------------------------------------------ launchers
+static inline unsigned nblk(int64_t n, int nt) { return (unsigned)((n + nt - 1) / nt); }
+
+template <typename T, int K>
+static void fin_q_update_k(const FinArgs& f, bool do_qp, hipStream_t s) {
+  constexpr bool trick = sizeof(T) == 8;
+  constexpr int NT = QPLanes<K>::N;
+  if (do_qp) hipLaunchKernelGGL((k_fin_q_update<T, K, trick, true>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
+  else hipLaunchKernelGGL((k_fin_q_update<T, K, trick, false>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
+}
+template <typename T, int K>
+static void fin_p_update_k(const FinArgs& f, bool do_qp, hipStream_t s) {
+  constexpr int NT = QPLanes<K>::N;
+  if (do_qp) hipLaunchKernelGGL((k_fin_p_update<T, K, true>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
+  else hipLaunchKernelGGL((k_fin_p_update<T, K, false>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
+}
+template <typename T, int K>
+static void fin_q_em_k(const FinArgs& f, hipStream_t s) {
+  hipLaunchKernelGGL((k_fin_q_em<T, K>), dim3(nblk(f.n, 256)), dim3(256), 0, s, f);
+}
+template <typename T, int K>
+static void fin_p_em_k(const FinArgs& f, hipStream_t s) {
+  hipLaunchKernelGGL((k_fin_p_em<T, K>), dim3(nblk(f.n, 256)), dim3(256), 0, s, f);
+}
+template <typename T, int K>
+static void qp_batch_k(bool simplex, int64_t n, const void* XtX, const void* Xtz, const void* x, void* delta,
+                       int32_t* status, hipStream_t s) {
+  constexpr int NT = QPLanes<K>::N;
+  if (simplex)
+    hipLaunchKernelGGL((k_qp_batch<T, K, 1>), dim3(nblk(n, NT)), dim3(NT), 0, s, n, (const T*)XtX, (const T*)Xtz,
+                       (const T*)x, (T*)delta, status);
+  else
+    hipLaunchKernelGGL((k_qp_batch<T, K, 0>), dim3(nblk(n, NT)), dim3(NT), 0, s, n, (const T*)XtX, (const T*)Xtz,
+                       (const T*)x, (T*)delta, status);
+}
+template <typename T, int K>
+static void project_q_k(int64_t n, void* b, hipStream_t s) {
+  hipLaunchKernelGGL((k_project_q<T, K>), dim3(nblk(n, 256)), dim3(256), 0, s, n, (T*)b);
+}
+
+template <typename T> void launch_fin_q_update(int K, const FinArgs& f, bool do_qp, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (fin_q_update_k<T, KK>(f, do_qp, s)));
+}
+template <typename T> void launch_fin_p_update(int K, const FinArgs& f, bool do_qp, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (fin_p_update_k<T, KK>(f, do_qp, s)));
+}
+template <typename T> void launch_fin_q_em(int K, const FinArgs& f, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (fin_q_em_k<T, KK>(f, s)));
+}
+template <typename T> void launch_fin_p_em(int K, const FinArgs& f, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (fin_p_em_k<T, KK>(f, s)));
+}
+template <typename T>
+void launch_qp_batch(int K, bool simplex, int64_t n, const void* XtX, const void* Xtz_neg, const void* x, void* delta,
+                     int32_t* status, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (qp_batch_k<T, KK>(simplex, n, XtX, Xtz_neg, x, delta, status, s)));
+}
+template <typename T> void launch_project_q(int K, int64_t n, void* b, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (project_q_k<T, KK>(n, b, s)));
+}
+template <typename T> void launch_project_p(int64_t n, void* b, hipStream_t s) {
+  const int64_t nb = (n + 255) / 256;
+  hipLaunchKernelGGL((k_project_p<T>), dim3((unsigned)(nb < 65536 ? (nb > 0 ? nb : 1) : 65536)), dim3(256), 0, s, n,
+                     (T*)b);
+}
+
+#define ADMX_INST(T)                                                                                         \
+  template void launch_fin_q_update<T>(int, const FinArgs&, bool, hipStream_t);                               \
+  template void launch_fin_p_update<T>(int, const FinArgs&, bool, hipStream_t);                               \
+  template void launch_fin_q_em<T>(int, const FinArgs&, hipStream_t);                                         \
+  template void launch_fin_p_em<T>(int, const FinArgs&, hipStream_t);                                         \
+  template void launch_qp_batch<T>(int, bool, int64_t, const void*, const void*, const void*, void*, int32_t*, \
+                                   hipStream_t);                                                             \
+  template void launch_project_q<T>(int, int64_t, void*, hipStream_t);                                        \
+  template void launch_project_p<T>(int64_t, void*, hipStream_t);
+ADMX_INST(ADMX_T)
+
+}  // namespace admx

@@ -1,0 +1,289 @@
+"""ctypes front-end of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY.  Imported by tests/, by ``__graft_entry__.smoke()``
+and by the CPU-baseline leg of ``bench.py``; never by the product package.
+
+Arrays follow the reference's Julia layout: column-major (``order='F'``)
+K x I / K x J matrices, K x K x n Hessian stacks, SNP-major BED bytes with a
+row pitch (bytes per SNP).  Ranges are 0-based half-open.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+c_i64 = C.c_int64
+c_dbl = C.c_double
+c_vp = C.c_void_p
+
+
+def build() -> None:
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _declare(_lib)
+    return _lib
+
+
+def _sfx(dtype) -> str:
+    return "_f64" if np.dtype(dtype) == np.float64 else "_f32"
+
+
+def _declare(L):
+    L.orc_decode.argtypes = [c_vp, c_i64, c_i64, c_i64, c_vp]
+    L.orc_counts.argtypes = [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp]
+    L.orc_gen_bed.argtypes = [c_vp, c_vp, c_i64, c_i64, c_i64, C.c_uint64, c_dbl, c_vp, c_i64]
+    for s in ("_f64", "_f32"):
+        getattr(L, "orc_loglikelihood" + s).argtypes = [c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, C.c_int]
+        getattr(L, "orc_loglikelihood" + s).restype = c_dbl
+        getattr(L, "orc_em_q_loop" + s).argtypes = [c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, C.c_int]
+        getattr(L, "orc_em_p_loop" + s).argtypes = [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, C.c_int]
+        for n in ("orc_update_q_loop", "orc_update_p_loop"):
+            getattr(L, n + s).argtypes = [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, C.c_int]
+        getattr(L, "orc_sweep" + s).argtypes = [c_vp, c_i64, c_i64, c_vp, C.c_int]
+        fl = c_dbl if s == "_f64" else C.c_float
+        getattr(L, "orc_create_tableau" + s).argtypes = [c_vp, c_vp, c_vp, c_vp, c_i64, C.c_int, fl]
+        getattr(L, "orc_quadratic_program" + s).argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, c_vp, c_vp]
+        getattr(L, "orc_quadratic_program" + s).restype = C.c_int
+        getattr(L, "orc_project_q" + s).argtypes = [c_vp, c_i64, c_i64]
+        getattr(L, "orc_project_p" + s).argtypes = [c_vp, c_i64]
+        getattr(L, "orc_update_UV" + s).argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64]
+        getattr(L, "orc_update_QN" + s).argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64]
+        getattr(L, "orc_em_q" + s).argtypes = [c_vp]
+        getattr(L, "orc_em_p" + s).argtypes = [c_vp]
+        getattr(L, "orc_init_em" + s).argtypes = [c_vp, c_i64, c_vp]
+        getattr(L, "orc_update_q" + s).argtypes = [c_vp, C.c_int, c_vp]
+        getattr(L, "orc_update_p" + s).argtypes = [c_vp, C.c_int, c_vp]
+        getattr(L, "orc_qn_iteration" + s).argtypes = [c_vp, c_i64, C.c_int, C.c_int, c_vp]
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(c_vp)
+
+
+def _chk(a, dtype=None):
+    assert a.flags.f_contiguous or a.ndim <= 1, "arrays must be column-major"
+    if dtype is not None:
+        assert a.dtype == np.dtype(dtype), (a.dtype, dtype)
+    return a
+
+
+def _nthreads(n):
+    if n is not None:
+        return int(n)
+    return int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1))
+
+
+# ---------------------------------------------------------------- decode / counts
+def decode(bed, pitch, I, J):
+    g = np.empty((I, J), dtype=np.int8, order="F")
+    lib().orc_decode(_p(bed), pitch, I, J, _p(g))
+    return g
+
+
+def counts(bed, pitch, I, J):
+    snm = np.zeros(J, np.int64)
+    salt = np.zeros(J, np.int64)
+    inm = np.zeros(I, np.int64)
+    lib().orc_counts(_p(bed), pitch, I, J, _p(snm), _p(salt), _p(inm))
+    return snm, salt, inm
+
+
+def gen_bed(Qt, Pt, seed, miss_rate, pitch=None):
+    K, I = Qt.shape
+    J = Pt.shape[1]
+    Qt = np.asfortranarray(Qt, dtype=np.float64)
+    Pt = np.asfortranarray(Pt, dtype=np.float64)
+    if pitch is None:
+        pitch = (I + 3) // 4
+    bed = np.zeros(pitch * J, dtype=np.uint8)
+    lib().orc_gen_bed(_p(Qt), _p(Pt), I, J, K, C.c_uint64(seed), miss_rate, _p(bed), pitch)
+    return bed
+
+
+# ---------------------------------------------------------------- tile loops
+def loglikelihood(bed, pitch, q, p, irange=None, jrange=None, nthreads=None):
+    K, I = q.shape
+    J = p.shape[1]
+    i0, i1 = irange if irange else (0, I)
+    j0, j1 = jrange if jrange else (0, J)
+    f = getattr(lib(), "orc_loglikelihood" + _sfx(q.dtype))
+    return f(_p(bed), pitch, _p(_chk(q)), _p(_chk(p, q.dtype)), K, i0, i1, j0, j1, _nthreads(nthreads))
+
+
+def em_q_loop(q_next, bed, pitch, q, p, irange=None, jrange=None, nthreads=None):
+    K, I = q.shape
+    J = p.shape[1]
+    i0, i1 = irange if irange else (0, I)
+    j0, j1 = jrange if jrange else (0, J)
+    getattr(lib(), "orc_em_q_loop" + _sfx(q.dtype))(_p(_chk(q_next, q.dtype)), _p(bed), pitch, _p(q), _p(p), K, i0, i1, j0, j1, _nthreads(nthreads))
+
+
+def em_p_loop(p_next, p_tmp, bed, pitch, q, p, irange=None, jrange=None, nthreads=None):
+    K, I = q.shape
+    J = p.shape[1]
+    i0, i1 = irange if irange else (0, I)
+    j0, j1 = jrange if jrange else (0, J)
+    getattr(lib(), "orc_em_p_loop" + _sfx(q.dtype))(_p(_chk(p_next, q.dtype)), _p(_chk(p_tmp, q.dtype)), _p(bed), pitch, _p(q), _p(p), K, i0, i1, j0, j1, _nthreads(nthreads))
+
+
+def update_q_loop(XtX, Xtz, bed, pitch, q, p, irange=None, jrange=None, nthreads=None):
+    K, I = q.shape
+    J = p.shape[1]
+    i0, i1 = irange if irange else (0, I)
+    j0, j1 = jrange if jrange else (0, J)
+    getattr(lib(), "orc_update_q_loop" + _sfx(q.dtype))(_p(_chk(XtX, q.dtype)), _p(_chk(Xtz, q.dtype)), _p(bed), pitch, _p(q), _p(p), K, i0, i1, j0, j1, _nthreads(nthreads))
+
+
+def update_p_loop(XtX, Xtz, bed, pitch, q, p, irange=None, jrange=None, nthreads=None):
+    K, I = q.shape
+    J = p.shape[1]
+    i0, i1 = irange if irange else (0, I)
+    j0, j1 = jrange if jrange else (0, J)
+    getattr(lib(), "orc_update_p_loop" + _sfx(q.dtype))(_p(_chk(XtX, q.dtype)), _p(_chk(Xtz, q.dtype)), _p(bed), pitch, _p(q), _p(p), K, i0, i1, j0, j1, _nthreads(nthreads))
+
+
+# ---------------------------------------------------------------- QP / projections / QN
+def sweep(A, k, inverse=False):
+    n = A.shape[0]
+    tmp = np.empty(n, A.dtype)
+    getattr(lib(), "orc_sweep" + _sfx(A.dtype))(_p(_chk(A)), n, k, _p(tmp), int(inverse))
+    return A
+
+
+def create_tableau(H, r, x, simplex, mu=0.0):
+    K = H.shape[0]
+    sz = K + 2 if simplex else K + 1
+    tab = np.zeros((sz, sz), dtype=H.dtype, order="F")
+    getattr(lib(), "orc_create_tableau" + _sfx(H.dtype))(_p(tab), _p(_chk(H)), _p(r), _p(x), K, int(simplex), mu)
+    return tab
+
+
+def quadratic_program(tab, par, pmin, pmax, c):
+    """Returns (delta, status).  ``tab`` is modified in place."""
+    p = par.shape[0]
+    delta = np.zeros(p, tab.dtype)
+    d = np.zeros(tab.shape[0], tab.dtype)
+    tmp = np.zeros(tab.shape[0], tab.dtype)
+    swept = np.zeros(p, np.uint8)
+    f = getattr(lib(), "orc_quadratic_program" + _sfx(tab.dtype))
+    r = f(_p(delta), _p(_chk(tab)), _p(par), _p(pmin), _p(pmax), p, c, _p(d), _p(tmp), _p(swept))
+    return delta, r
+
+
+def project_q(b):
+    K, n = b.shape
+    getattr(lib(), "orc_project_q" + _sfx(b.dtype))(_p(_chk(b)), K, n)
+    return b
+
+
+def project_p(b):
+    getattr(lib(), "orc_project_p" + _sfx(b.dtype))(_p(_chk(b)), b.size)
+    return b
+
+
+def update_UV(U, V, x, x_next, x_next2, it, Q):
+    N = x.size
+    getattr(lib(), "orc_update_UV" + _sfx(x.dtype))(_p(U), _p(V), _p(x), _p(x_next), _p(x_next2), N, it, Q)
+
+
+def update_QN(x_out, x_mapped, x, U, V, m):
+    N = x.size
+    getattr(lib(), "orc_update_QN" + _sfx(x.dtype))(_p(x_out), _p(x_mapped), _p(x), _p(U), _p(V), N, m)
+
+
+# ---------------------------------------------------------------- whole algorithm
+class _StateF64(C.Structure):
+    _fields_ = [("I", c_i64), ("J", c_i64), ("K", c_i64), ("Q", c_i64),
+                ("bed", c_vp), ("pitch", c_i64),
+                ("x", c_vp), ("x_next", c_vp), ("x_next2", c_vp), ("x_tmp", c_vp),
+                ("XtX_q", c_vp), ("Xtz_q", c_vp), ("XtX_p", c_vp), ("Xtz_p", c_vp),
+                ("U", c_vp), ("V", c_vp),
+                ("ll_prev", c_dbl), ("ll_new", c_dbl), ("nthreads", C.c_int)]
+
+
+class OracleAdmix:
+    """Host-array mirror of AdmixData{T} (src/structs.jl:9-215) driven by the oracle."""
+
+    def __init__(self, bed, pitch, I, J, K, Q, x0, U0=None, V0=None, dtype=np.float64, nthreads=None):
+        self.dtype = np.dtype(dtype)
+        self.I, self.J, self.K, self.Q = I, J, K, Q
+        self.bed = np.ascontiguousarray(bed, dtype=np.uint8)
+        self.pitch = pitch
+        N = K * (I + J)
+        self.x = np.asfortranarray(x0, dtype=self.dtype).copy(order="F")
+        self.x_next = np.zeros((K, I + J), self.dtype, order="F")
+        self.x_next2 = np.zeros((K, I + J), self.dtype, order="F")
+        self.x_tmp = np.zeros((K, I + J), self.dtype, order="F")
+        self.XtX_q = np.zeros((K, K, I), self.dtype, order="F")
+        self.Xtz_q = np.zeros((K, I), self.dtype, order="F")
+        self.XtX_p = np.zeros((K, K, J), self.dtype, order="F")
+        self.Xtz_p = np.zeros((K, J), self.dtype, order="F")
+        self.U = np.zeros((N, Q), self.dtype, order="F") if U0 is None else np.asfortranarray(U0, dtype=self.dtype).copy(order="F")
+        self.V = np.zeros((N, Q), self.dtype, order="F") if V0 is None else np.asfortranarray(V0, dtype=self.dtype).copy(order="F")
+        self.st = _StateF64(I, J, K, Q, _p(self.bed), pitch, _p(self.x), _p(self.x_next), _p(self.x_next2), _p(self.x_tmp),
+                            _p(self.XtX_q), _p(self.Xtz_q), _p(self.XtX_p), _p(self.Xtz_p), _p(self.U), _p(self.V),
+                            float("nan"), float("nan"), _nthreads(nthreads))
+        self.sfx = _sfx(self.dtype)
+
+    @property
+    def ll_new(self):
+        return self.st.ll_new
+
+    @ll_new.setter
+    def ll_new(self, v):
+        self.st.ll_new = v
+
+    @property
+    def q(self):
+        return self.x[:, : self.I]
+
+    @property
+    def p(self):
+        return self.x[:, self.I:]
+
+    def _f(self, n):
+        return getattr(lib(), n + self.sfx)
+
+    def loglikelihood(self, x=None):
+        x = self.x if x is None else x
+        return loglikelihood(self.bed, self.pitch, np.asfortranarray(x[:, : self.I]), np.asfortranarray(x[:, self.I:]), nthreads=self.st.nthreads)
+
+    def em_q(self):
+        self._f("orc_em_q")(C.byref(self.st))
+
+    def em_p(self):
+        self._f("orc_em_p")(C.byref(self.st))
+
+    def init_em(self, iters):
+        lls = np.zeros(max(iters, 1))
+        self._f("orc_init_em")(C.byref(self.st), iters, _p(lls))
+        return lls[:iters]
+
+    def update_q(self, update2=False):
+        status = np.zeros(self.I, np.int32)
+        self._f("orc_update_q")(C.byref(self.st), int(update2), _p(status))
+        return status
+
+    def update_p(self, update2=False):
+        status = np.zeros(self.J, np.int32)
+        self._f("orc_update_p")(C.byref(self.st), int(update2), _p(status))
+        return status
+
+    def qn_iteration(self, it, fix_q=False, fix_p=False):
+        out = np.zeros(5)
+        self._f("orc_qn_iteration")(C.byref(self.st), it, int(fix_q), int(fix_p), _p(out))
+        return dict(prev=out[0], basic=out[1], qn=out[2], new=out[3], reldiff=out[4])
