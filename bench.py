@@ -1,0 +1,190 @@
+"""Benchmark of the admixture_qn! iteration on one MI355X.
+
+One step = one admixture_qn! iteration (mode=:ZAL, Q=3) on the device-resident
+state: 2 update_q! + 2 update_p! (gradient/Hessian passes over all I x J
+genotypes + batched QPs + projections), 2 log-likelihood passes, the ZAL
+extrapolation and the accept rule.  Workload (default): I=100,000
+individuals, J=100,000 SNPs, K=8, synthetic cohort (Q ~ Dirichlet(0.3),
+P ~ Beta(0.5,0.5) clipped to [0.01,0.99], 2% missing) generated on the device
+into the 2-bit BED (2.5 GB in HBM), random init + 5 FRAPPE-EM priming steps
+before timing.  value = I*J*steps / elapsed seconds (genotype-iterations/s).
+
+Prints one JSON line.  --gpus above 1 is not supported (one GPU per box).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+# Algorithmic FLOPs per genotype and pass (FMA = 2, divide = 1, log = 1, the
+# reference's full K^2 Hessian): Q-side update 4K^2+8K+6, P-side update
+# 2K^2+5K+8, log-likelihood 2K+7.
+FLOPS = {
+    "qside_update": lambda K: 4 * K * K + 8 * K + 6,
+    "pside_update": lambda K: 2 * K * K + 5 * K + 8,
+    "qside_ll": lambda K: 2 * K + 7,
+}
+PEAK_TFLOPS = {"f64": 78.6, "f32": 157.3}  # MI355X dense FP64 / FP32
+
+WORKLOADS = {
+    "c3": dict(I=100_000, J=100_000, K=8, alpha=0.3, p_dist="beta", p_lo=0.01, p_hi=0.99, miss_rate=0.02),
+    "c2": dict(I=10_000, J=100_000, K=5, alpha=0.3, p_dist="uniform", p_lo=0.05, p_hi=0.95, miss_rate=0.02),
+    "c1": dict(I=1_000, J=10_000, K=3, alpha=0.5, p_dist="uniform", p_lo=0.05, p_hi=0.95, miss_rate=0.01),
+}
+
+
+def host_cores() -> int:
+    n = os.environ.get("OMP_NUM_THREADS")
+    if n:
+        return max(1, int(n))
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def cpu_baseline(wl, dtype, seed, budget_s=20.0):
+    """The CPU oracle (C/OpenMP restatement of the reference loops) on a bounded
+    sample of the same workload: all I individuals, the first Js SNPs (same
+    truth, same generator), one EM priming step, then admixture_qn!
+    iterations until about budget_s seconds of CPU work are spent."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as O
+
+    import __graft_entry__ as g
+
+    oa = g.load_package()
+    from importlib import import_module
+
+    sim = import_module("openadmixture_jl_amd.sim")
+    I, K = wl["I"], wl["K"]
+    Js = max(64, min(wl["J"], int(2e8 // I)))
+    Qt, Pt = sim.draw_truth(I, wl["J"], K, wl["alpha"], wl["p_dist"], wl["p_lo"], wl["p_hi"], seed=seed)
+    Pt = np.asfortranarray(Pt[:, :Js])
+    pitch = (I + 3) // 4
+    bed = O.gen_bed(Qt, Pt, seed, wl["miss_rate"], pitch)
+    T = np.float64 if dtype == "f64" else np.float32
+    rng = np.random.default_rng(seed + 1)
+    x0 = rng.random((K, I + Js))
+    x0[:, :I] /= x0[:, :I].sum(0)
+    N = K * (I + Js)
+    cores = host_cores()
+    st = O.OracleAdmix(bed, pitch, I, Js, K, 3, x0, rng.random((N, 3)), rng.random((N, 3)), dtype=T,
+                       nthreads=cores)
+    st.init_em(1)
+    its, t0 = 0, time.perf_counter()
+    while True:
+        st.qn_iteration(its + 1)
+        its += 1
+        el = time.perf_counter() - t0
+        if el > budget_s or its >= 50 or el * (its + 1) / its > 1.5 * budget_s:
+            break
+    return {
+        "value": I * Js * its / el,
+        "unit": "genotype*iter/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": f"oracle (C/OpenMP restatement of src/loops.jl etc.), I={I}, first J={Js} SNPs of the same "
+                  f"cohort, K={K}, {dtype}, {its} admixture_qn! iterations after 1 EM step, {el:.1f} s",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
+    ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--seed", type=int, default=3)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
+              "is a design note, not a measured path", file=sys.stderr)
+        sys.exit(2)
+
+    import __graft_entry__ as g
+
+    oa = g.load_package()
+    from importlib import import_module
+
+    sim = import_module("openadmixture_jl_amd.sim")
+    wl = WORKLOADS[args.workload]
+    I, J, K = wl["I"], wl["J"], wl["K"]
+    T = np.float64 if args.dtype == "f64" else np.float32
+
+    ctx = oa.HIPAdmixData(I, J, K, 3, T)
+    t0 = time.perf_counter()
+    sim.cohort_on_device(ctx, wl["alpha"], wl["p_dist"], wl["p_lo"], wl["p_hi"], wl["miss_rate"], seed=args.seed)
+    t_gen = time.perf_counter() - t0
+    d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(args.seed + 1), T=T)
+    oa.init_em(d, None, 5, d_cu=ctx, log=None)
+    ctx.set_uv(d.U, d.V)
+    it = 0
+    for _ in range(args.warmup):
+        it += 1
+        ctx.qn_iteration(it)
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    lls = []
+    for _ in range(args.steps):
+        it += 1
+        lls.append(ctx.qn_iteration(it)["new"])
+    ctx.synchronize()
+    el = time.perf_counter() - t0
+    rep = ctx.profile_report()
+    ctx.profile(False)
+
+    value = I * J * args.steps / el
+    # dominant kernel and its achieved algorithmic FLOP rate
+    dom = max((k for k in rep if k in FLOPS), key=lambda k: rep[k][1])
+    n_launch, ms = rep[dom]
+    per_launch_s = ms / n_launch / 1e3
+    achieved = FLOPS[dom](K) * I * J / per_launch_s / 1e12
+    peak = PEAK_TFLOPS[args.dtype]
+    kernels = {k: {"launches": n, "avg_ms": round(t / n, 4), "share": round(t / (el * 1e3), 4)}
+               for k, (n, t) in sorted(rep.items(), key=lambda kv: -kv[1][1])}
+    out = {
+        "metric": "genotype*iterations/sec at K=8 (admixture_qn! ZAL iteration)",
+        "value": value,
+        "unit": "genotype*iter/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": el / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": args.dtype,
+        "data": "synthetic cohort generated on device (Dirichlet Q, Beta/Uniform P, Binomial(2, q.p) genotypes)",
+        "config": {"workload": f"{args.workload}: I={I}, J={J}, K={K}, Q=3, ZAL, plain {args.dtype}",
+                   "I": I, "J": J, "K": K, "genotypes": I * J, "bed_bytes": ((I + 3) // 4) * J},
+        "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+                     "frac": achieved / peak, "traffic": None,
+                     "note": f"{FLOPS[dom](K)} algorithmic flops/genotype x {I * J} genotypes per launch; "
+                             f"{args.dtype} vector ALU (MI355X dense FP64 vector = matrix = 78.6 TF)"},
+        "kernels": kernels,
+        "ll_last": lls[-1],
+        "gen_seconds": t_gen,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(wl, args.dtype, args.seed, args.cpu_budget)
+        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+    ctx.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

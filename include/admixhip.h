@@ -1,0 +1,154 @@
+/*
+ * admixhip.h -- C ABI of libadmixhip, the MI355X (gfx950) engine for the
+ * likelihood / EM / SQP inner loops of OpenADMIXTURE.jl.
+ *
+ * Conventions (identical to the reference's Julia arrays):
+ *   - every matrix is column-major: q[k,i] at k + K*i (K x I), p[k,j] at
+ *     k + K*j (K x J), x = [q p] (K x (I+J)), XtX[k2,k,n] at k2 + K*k + K*K*n,
+ *     U/V at n + K*(I+J)*c (K(I+J) x Q);
+ *   - element type is the context's dtype: double (ADMX_F64) or float (ADMX_F32);
+ *     log-likelihoods are always double;
+ *   - index ranges are 0-based half-open [i0,i1) x [j0,j1); the reference's
+ *     1-based UnitRange a:b is [a-1, b);
+ *   - host pointers are read or written only during the call (the caller keeps
+ *     them alive, e.g. GC.@preserve); device memory is owned by the context;
+ *   - every function returns ADMX_OK (0) or an error code, with a message in
+ *     admx_last_error() (thread-local).  A QP that fails its pivot test is not
+ *     an error: it yields delta = 0 and status 0, as in the reference;
+ *   - a context holds one HIP stream and is not re-entrant: calls on the same
+ *     context are serialised by an internal mutex.
+ */
+#ifndef ADMIXHIP_H
+#define ADMIXHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define ADMX_OK 0
+#define ADMX_EINVAL 1 /* bad argument (shape, range, dtype, K > 16, Q > 8) */
+#define ADMX_EHIP 2   /* HIP runtime error */
+#define ADMX_ESTATE 3 /* missing state (e.g. no genotypes uploaded) */
+
+#define ADMX_F64 0
+#define ADMX_F32 1
+
+/* which state vector (AdmixData fields x, x_next, x_next2, x_tmp; src/structs.jl:16-19) */
+#define ADMX_X 0
+#define ADMX_X_NEXT 1
+#define ADMX_X_NEXT2 2
+#define ADMX_X_TMP 3
+
+#define ADMX_SIDE_Q 0
+#define ADMX_SIDE_P 1
+
+/* tuning keys for admx_set_tuning */
+#define ADMX_TUNE_TARGET_WAVES 0 /* waves per genotype pass (sets the chunk count) */
+#define ADMX_TUNE_KEEP_HESSIANS 1 /* 1: update_q/update_p also store XtX and Xtz */
+
+typedef struct admx_ctx admx_ctx;
+
+const char *admx_version(void);
+const char *admx_last_error(void);
+/* Number of visible HIP devices. */
+int admx_device_count(int *n);
+
+/* ---- context: replaces CuAdmixData{T} + _cu_admixture_base
+ * (src/cuda/structs.jl:1-38, stub src/driver.jl:204-206). */
+int admx_create(admx_ctx **out, int64_t I, int64_t J, int64_t K, int64_t Q, int dtype);
+int admx_destroy(admx_ctx *ctx);
+/* Upload the raw SNP-major BED payload: J columns of bytes_per_snp bytes
+ * (bytes_per_snp >= ceil(I/4); the 3-byte PLINK magic already stripped),
+ * i.e. g.s.data of SnpArrays (src/cuda/structs.jl:35-37). */
+int admx_upload_bed(admx_ctx *ctx, const uint8_t *bed, int64_t bytes_per_snp);
+/* Download the BED payload as ceil(I/4) x J bytes. */
+int admx_download_bed(admx_ctx *ctx, uint8_t *bed);
+/* Synthetic genotypes on device from true Q (K x I) and P (K x J) in double:
+ * g ~ Binomial(2, q_i.p_j), missing with probability miss_rate, seeded
+ * counter hash.  No reference counterpart (workload generator). */
+int admx_generate_bed(admx_ctx *ctx, const double *Qt, const double *Pt, uint64_t seed, double miss_rate);
+
+/* ---- BED decode and counts (decode tables src/loops.jl:2-5).  No count
+ * function exists in the reference; these are exact integer counts. */
+int admx_decode(admx_ctx *ctx, int8_t *g); /* I x J, -1 = missing */
+int admx_counts(admx_ctx *ctx, int64_t *snp_nonmissing, int64_t *snp_alt, int64_t *ind_nonmissing);
+
+/* ---- tile loops of src/loops.jl (SnpLinAlg, skipmissing variants), host
+ * arrays, accumulate (+=) semantics into the caller's outputs. */
+/* loglikelihood_loop_skipmissing(g, q, p, qp_small, irange, jrange, K)::Float64  (loops.jl:517) */
+int admx_loglikelihood_loop(admx_ctx *ctx, const void *q, const void *p, int64_t i0, int64_t i1, int64_t j0,
+                            int64_t j1, double *out);
+/* em_q_loop_skipmissing!(q_next, g, q, p, qp_small, irange, jrange, K)  (loops.jl:610) */
+int admx_em_q_loop(admx_ctx *ctx, void *q_next, const void *q, const void *p, int64_t i0, int64_t i1, int64_t j0,
+                   int64_t j1);
+/* em_p_loop_skipmissing!(p_next, g, q, p, p_tmp, qp_small, irange, jrange, K)  (loops.jl:700) */
+int admx_em_p_loop(admx_ctx *ctx, void *p_next, void *p_tmp, const void *q, const void *p, int64_t i0, int64_t i1,
+                   int64_t j0, int64_t j1);
+/* update_q_loop_skipmissing!(XtX, Xtz, g, q, p, qp_small, irange, jrange, K)  (loops.jl:867) */
+int admx_update_q_loop(admx_ctx *ctx, void *XtX, void *Xtz, const void *q, const void *p, int64_t i0, int64_t i1,
+                       int64_t j0, int64_t j1);
+/* update_p_loop_skipmissing!(XtX, Xtz, g, q, p, qp_small, irange, jrange, K)  (loops.jl:923) */
+int admx_update_p_loop(admx_ctx *ctx, void *XtX, void *Xtz, const void *q, const void *p, int64_t i0, int64_t i1,
+                       int64_t j0, int64_t j1);
+
+/* ---- batched QP / projections / quasi-Newton on host arrays. */
+/* create_tableau! + quadratic_program! for n problems (src/qp.jl:8-292,
+ * batched as in src/algorithms_inner.jl:205-221 (simplex) and :281-298 (box)).
+ * Xtz_neg is the negated gradient as stored after `Xtz .*= -1`. */
+int admx_qp_batch(admx_ctx *ctx, int simplex, int64_t n, const void *XtX, const void *Xtz_neg, const void *x,
+                  void *delta, int32_t *status);
+/* project_q!(b) on n columns of K (src/projections.jl:1-63) */
+int admx_project_q(admx_ctx *ctx, void *b, int64_t n);
+/* project_p!(b) on n entries (src/projections.jl:69-74) */
+int admx_project_p(admx_ctx *ctx, void *b, int64_t n);
+/* update_UV! + update_QN! + project_p!/project_q! of one ZAL step
+ * (src/quasi_newton.jl:5-9, 29-31; src/algorithms_outer.jl:127-142): iter is
+ * the 1-based outer index; U, V are updated in place, x_tmp receives the step. */
+int admx_zal_step(admx_ctx *ctx, int64_t iter, const void *x, const void *x_next, const void *x_next2, void *U,
+                  void *V, void *x_tmp);
+
+/* ---- device-resident state (AdmixData mirrored in HBM) */
+int admx_set_x(admx_ctx *ctx, int which, const void *host);
+int admx_get_x(admx_ctx *ctx, int which, void *host);
+int admx_set_uv(admx_ctx *ctx, const void *U, const void *V);
+int admx_get_uv(admx_ctx *ctx, void *U, void *V);
+int admx_set_ll(admx_ctx *ctx, double ll_prev, double ll_new);
+int admx_get_ll(admx_ctx *ctx, double *ll_prev, double *ll_new);
+/* XtX/Xtz of the last update on side ADMX_SIDE_Q / ADMX_SIDE_P (Xtz negated,
+ * as the reference leaves it). */
+int admx_get_hessians(admx_ctx *ctx, int side, void *XtX, void *Xtz);
+
+/* ---- algorithms on the device-resident state */
+/* loglikelihood(g, q, p, ...) of state vector `which` (src/algorithms_inner.jl:27-35) */
+int admx_loglikelihood(admx_ctx *ctx, int which, double *out);
+/* em_q!(d, g): x_next.q from x  (src/algorithms_inner.jl:68-92) */
+int admx_em_q(admx_ctx *ctx);
+/* em_p!(d, g): x_next.p from x  (src/algorithms_inner.jl:108-141) */
+int admx_em_p(admx_ctx *ctx);
+/* init_em!(d, g, iters): lls[it] = LL after EM step it; sets ll_new
+ * (src/algorithms_outer.jl:12-45) */
+int admx_init_em(admx_ctx *ctx, int64_t iters, double *lls);
+/* update_q!(d, g, update2) / update_p!(d, g, update2)
+ * (src/algorithms_inner.jl:167-231, 245-310); status (nullable) receives the
+ * QP return codes, I or J entries. */
+int admx_update_q(admx_ctx *ctx, int update2, int32_t *status);
+int admx_update_p(admx_ctx *ctx, int update2, int32_t *status);
+/* One admixture_qn! iteration with mode=:ZAL (src/algorithms_outer.jl:92-159),
+ * iter 1-based.  out = {ll_prev, ll_basic, ll_qn, ll_new, reldiff}. */
+int admx_qn_iteration(admx_ctx *ctx, int64_t iter, int fix_q, int fix_p, double *out);
+int admx_synchronize(admx_ctx *ctx);
+
+/* ---- measurement: HIP-event timing of every kernel launch on the context's
+ * stream while enabled.  Report lines: "<kernel> <launches> <total_ms>\n". */
+int admx_profile_enable(admx_ctx *ctx, int on);
+int admx_profile_reset(admx_ctx *ctx);
+int admx_profile_report(admx_ctx *ctx, char *buf, size_t len);
+int admx_set_tuning(admx_ctx *ctx, int key, int64_t value);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* ADMIXHIP_H */

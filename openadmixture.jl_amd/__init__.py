@@ -1,0 +1,38 @@
+"""MI355X-native likelihood / EM / SQP engine with the API of OpenADMIXTURE.jl.
+
+Compute runs in hand-written gfx950 HIP kernels (libadmixhip.so, C ABI in
+include/admixhip.h); this package is the host-side mirror of the reference's
+interface for that path:
+
+  AdmixData                          src/structs.jl
+  *_loop_skipmissing                 src/loops.jl (tile loops, += semantics)
+  loglikelihood, em_q, em_p,         src/algorithms_inner.jl
+  update_q, update_p
+  init_em, admixture_qn              src/algorithms_outer.jl
+  run_admixture                      src/driver.jl
+  HIPAdmixData, hip_admixture_base   src/cuda/structs.jl (device mirror)
+
+The directory name contains a dot, so load it by path, e.g.
+``importlib.util.spec_from_file_location("openadmixture_jl_amd", ".../__init__.py",
+submodule_search_locations=[...])``.
+"""
+from . import _native
+from ._native import AdmixHIPError, device_count
+from .algorithms import admixture_qn, em_p, em_q, init_em, loglikelihood, update_p, update_q
+from .driver import _admixture_base, hip_admixture_base, run_admixture
+from .hip import HIPAdmixData
+from .loops import (UnitRange, em_p_loop_skipmissing, em_q_loop_skipmissing, loglikelihood_loop_skipmissing,
+                    update_p_loop_skipmissing, update_q_loop_skipmissing)
+from .plink import SnpArray, SnpLinAlg, write_plink
+from .structs import AdmixData
+
+# Fail loudly at import when the native library is absent.
+_native.lib()
+
+__all__ = [
+    "AdmixData", "HIPAdmixData", "hip_admixture_base", "run_admixture", "_admixture_base",
+    "init_em", "admixture_qn", "loglikelihood", "em_q", "em_p", "update_q", "update_p",
+    "loglikelihood_loop_skipmissing", "em_q_loop_skipmissing", "em_p_loop_skipmissing",
+    "update_q_loop_skipmissing", "update_p_loop_skipmissing", "UnitRange",
+    "SnpArray", "SnpLinAlg", "write_plink", "AdmixHIPError", "device_count",
+]

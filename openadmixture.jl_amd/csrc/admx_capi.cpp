@@ -1,0 +1,1025 @@
+// admx_capi.cpp -- host runtime of libadmixhip: the context (device mirror of
+// AdmixData{T}), the C ABI declared in include/admixhip.h, and the
+// device-resident orchestration of update_q!/update_p!/init_em!/admixture_qn!
+// on one HIP stream.  Only plumbing lives here; all arithmetic on genotypes,
+// QPs, projections and quasi-Newton steps runs in the HIP kernels.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "admixhip.h"
+#include "admx_internal.hpp"
+
+using namespace admx;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                   \
+  do {                                                                                 \
+    hipError_t e_ = (expr);                                                            \
+    if (e_ != hipSuccess) return fail(ADMX_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+#define CHK(expr)            \
+  do {                       \
+    int r_ = (expr);         \
+    if (r_ != ADMX_OK) return r_; \
+  } while (0)
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t n = 0;
+  int grow(size_t bytes) {
+    if (bytes <= n) return ADMX_OK;
+    if (p) HIPCHK(hipFree(p));
+    p = nullptr;
+    n = 0;
+    const size_t want = bytes < 256 ? 256 : bytes;
+    HIPCHK(hipMalloc(&p, want));
+    n = want;
+    return ADMX_OK;
+  }
+  void release() {
+    if (p) (void)hipFree(p);
+    p = nullptr;
+    n = 0;
+  }
+};
+
+struct ProfEv {
+  const char* name;
+  hipEvent_t a, b;
+};
+
+struct Prof {
+  bool on = false;
+  std::vector<ProfEv> pend;
+  std::vector<hipEvent_t> pool;
+  std::map<std::string, std::pair<long, double>> acc;
+};
+
+}  // namespace
+
+struct admx_ctx {
+  int64_t I = 0, J = 0, K = 0, Q = 0;
+  int dtype = ADMX_F64;
+  size_t esz = 8;
+  int64_t ibytes = 0, pitch = 0;
+  int device = 0;
+  hipStream_t s = nullptr;
+  uint8_t* bed = nullptr;
+  bool have_bed = false;
+  void* x[4] = {nullptr, nullptr, nullptr, nullptr};
+  void *XtX_q = nullptr, *Xtz_q = nullptr, *XtX_p = nullptr, *Xtz_p = nullptr;
+  void *U = nullptr, *V = nullptr;
+  double* lls = nullptr;
+  double* h_lls = nullptr;
+  int32_t* status = nullptr;
+  DevBuf part, llpart, zal;
+  DevBuf st[8];  // staging for host-array entry points
+  int64_t target_waves = 8192;
+  int keep_hess = 1;
+  std::mutex mu;
+  Prof prof;
+
+  int64_t N() const { return K * (I + J); }
+  void* qp(void* v) const { return v; }
+  void* pp(void* v) const { return static_cast<char*>(v) + (size_t)(K * I) * esz; }
+};
+
+namespace {
+
+hipEvent_t ev_get(admx_ctx* c) {
+  if (!c->prof.pool.empty()) {
+    hipEvent_t e = c->prof.pool.back();
+    c->prof.pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+
+struct ProfScope {
+  admx_ctx* c;
+  const char* name;
+  hipEvent_t a = nullptr, b = nullptr;
+  ProfScope(admx_ctx* c_, const char* n) : c(c_), name(n) {
+    if (c->prof.on) {
+      a = ev_get(c);
+      b = ev_get(c);
+      (void)hipEventRecord(a, c->s);
+    }
+  }
+  ~ProfScope() {
+    if (c->prof.on) {
+      (void)hipEventRecord(b, c->s);
+      c->prof.pend.push_back({name, a, b});
+    }
+  }
+};
+
+int prof_collect(admx_ctx* c) {
+  if (c->prof.pend.empty()) return ADMX_OK;
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (auto& e : c->prof.pend) {
+    float ms = 0.f;
+    HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
+    auto& slot = c->prof.acc[e.name];
+    slot.first += 1;
+    slot.second += ms;
+    c->prof.pool.push_back(e.a);
+    c->prof.pool.push_back(e.b);
+  }
+  c->prof.pend.clear();
+  return ADMX_OK;
+}
+
+int check_launch() {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(ADMX_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  return ADMX_OK;
+}
+
+int bad(const char* msg) { return fail(ADMX_EINVAL, msg); }
+
+int check_ctx(admx_ctx* c) {
+  if (!c) return bad("null context");
+  HIPCHK(hipSetDevice(c->device));
+  return ADMX_OK;
+}
+
+int check_ranges(admx_ctx* c, int64_t i0, int64_t i1, int64_t j0, int64_t j1) {
+  if (i0 < 0 || i1 < i0 || i1 > c->I || j0 < 0 || j1 < j0 || j1 > c->J)
+    return bad("range outside [0,I) x [0,J) or reversed");
+  return ADMX_OK;
+}
+
+int need_bed(admx_ctx* c) {
+  if (!c->have_bed) return fail(ADMX_ESTATE, "no genotypes: call admx_upload_bed or admx_generate_bed first");
+  return ADMX_OK;
+}
+
+// ---- chunking of the genotype passes
+int qside_chunks(admx_ctx* c, int64_t Ilen, int64_t Jlen, int64_t* chunk) {
+  const int64_t waves = (Ilen + 63) / 64;
+  int64_t n = (c->target_waves + waves - 1) / std::max<int64_t>(waves, 1);
+  n = std::max<int64_t>(1, std::min<int64_t>(n, std::max<int64_t>(1, Jlen / 64)));
+  *chunk = std::max<int64_t>(1, (Jlen + n - 1) / n);
+  return (int)std::max<int64_t>(1, (Jlen + *chunk - 1) / *chunk);
+}
+
+int pside_chunks(admx_ctx* c, int64_t i0, int64_t i1, int64_t Jlen, int64_t* chunk) {
+  const int64_t waves = (Jlen + 63) / 64;
+  const int64_t span = i1 - (i0 & ~(int64_t)15);
+  int64_t n = (c->target_waves + waves - 1) / std::max<int64_t>(waves, 1);
+  n = std::max<int64_t>(1, std::min<int64_t>(n, std::max<int64_t>(1, span / 64)));
+  int64_t ch = (span + n - 1) / n;
+  ch = (ch + 15) & ~(int64_t)15;
+  *chunk = std::max<int64_t>(16, ch);
+  return (int)std::max<int64_t>(1, (span + *chunk - 1) / *chunk);
+}
+
+const char* mode_name(bool qside, int mode) {
+  if (qside) return mode == MODE_UPDATE ? "qside_update" : (mode == MODE_EM ? "qside_em" : "qside_ll");
+  return mode == MODE_UPDATE ? "pside_update" : "pside_em";
+}
+
+// Run a Q-side pass; returns the chunk count (partials in c->part or c->llpart).
+template <typename T>
+int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0, int64_t j1,
+           int* nchunk, int64_t* nparts) {
+  const int64_t Ilen = i1 - i0, Jlen = j1 - j0;
+  *nchunk = 0;
+  *nparts = 0;
+  if (Ilen <= 0 || Jlen <= 0) return ADMX_OK;
+  int64_t chunk = 0;
+  const int n = qside_chunks(c, Ilen, Jlen, &chunk);
+  PassArgs a{c->bed, c->pitch, c->I, q, p, i0, i1, j0, j1, chunk, nullptr};
+  if (mode == MODE_LL) {
+    const int64_t np = ((Ilen + 255) / 256) * n;
+    CHK(c->llpart.grow((size_t)np * sizeof(double)));
+    a.part = c->llpart.p;
+    *nparts = np;
+  } else {
+    const int na = pass_na(mode, true, (int)c->K, sizeof(T) == 8);
+    CHK(c->part.grow((size_t)n * na * Ilen * sizeof(T)));
+    a.part = c->part.p;
+  }
+  {
+    ProfScope ps(c, mode_name(true, mode));
+    launch_qside<T>(mode, (int)c->K, a, n, c->s);
+  }
+  *nchunk = n;
+  return check_launch();
+}
+
+template <typename T>
+int pass_p(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0, int64_t j1,
+           int* nchunk) {
+  const int64_t Ilen = i1 - i0, Jlen = j1 - j0;
+  *nchunk = 0;
+  if (Ilen <= 0 || Jlen <= 0) return ADMX_OK;
+  int64_t chunk = 0;
+  const int n = pside_chunks(c, i0, i1, Jlen, &chunk);
+  const int na = pass_na(mode, false, (int)c->K, sizeof(T) == 8);
+  CHK(c->part.grow((size_t)n * na * Jlen * sizeof(T)));
+  PassArgs a{c->bed, c->pitch, c->I, q, p, i0, i1, j0, j1, chunk, c->part.p};
+  {
+    ProfScope ps(c, mode_name(false, mode));
+    launch_pside<T>(mode, (int)c->K, a, n, c->s);
+  }
+  *nchunk = n;
+  return check_launch();
+}
+
+// LL of (q, p) over a range into device slot lls[slot] (accumulate optional).
+template <typename T>
+int ll_to_slot(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0, int64_t j1, int slot) {
+  int nchunk = 0;
+  int64_t np = 0;
+  CHK(pass_q<T>(c, MODE_LL, q, p, i0, i1, j0, j1, &nchunk, &np));
+  ProfScope ps(c, "ll_reduce");
+  launch_reduce_sum(static_cast<const double*>(c->llpart.p), np, c->lls + slot, 0, c->s);
+  return check_launch();
+}
+
+template <typename T>
+int dev_update_q(admx_ctx* c, bool update2, int32_t* status_host) {
+  void* X = c->x[ADMX_X];
+  void* XN = c->x[ADMX_X_NEXT];
+  void* XN2 = c->x[ADMX_X_NEXT2];
+  void* q_next = update2 ? XN2 : XN;
+  void* q = update2 ? XN : X;
+  void* p = c->pp(update2 ? XN : X);
+  int nchunk = 0;
+  int64_t np = 0;
+  CHK(pass_q<T>(c, MODE_UPDATE, q, p, 0, c->I, 0, c->J, &nchunk, &np));
+  FinArgs f{c->part.p, c->I, nchunk, 0, c->J, q, q_next, nullptr,
+            c->keep_hess ? c->XtX_q : nullptr, c->keep_hess ? c->Xtz_q : nullptr, 0, 1,
+            status_host ? c->status : nullptr};
+  {
+    ProfScope ps(c, "fin_q_update_qp");
+    launch_fin_q_update<T>((int)c->K, f, true, c->s);
+  }
+  CHK(check_launch());
+  if (status_host) {
+    HIPCHK(hipMemcpyAsync(status_host, c->status, sizeof(int32_t) * c->I, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+  }
+  return ADMX_OK;
+}
+
+template <typename T>
+int dev_update_p(admx_ctx* c, bool update2, int32_t* status_host) {
+  void* X = c->x[ADMX_X];
+  void* XN = c->x[ADMX_X_NEXT];
+  void* XN2 = c->x[ADMX_X_NEXT2];
+  void* p_next = c->pp(update2 ? XN2 : XN);
+  void* q = update2 ? XN2 : XN;
+  void* p = c->pp(update2 ? XN : X);
+  int nchunk = 0;
+  CHK(pass_p<T>(c, MODE_UPDATE, q, p, 0, c->I, 0, c->J, &nchunk));
+  FinArgs f{c->part.p, c->J, nchunk, 0, c->J, p, p_next, nullptr,
+            c->keep_hess ? c->XtX_p : nullptr, c->keep_hess ? c->Xtz_p : nullptr, 0, 1,
+            status_host ? c->status : nullptr};
+  {
+    ProfScope ps(c, "fin_p_update_qp");
+    launch_fin_p_update<T>((int)c->K, f, true, c->s);
+  }
+  CHK(check_launch());
+  if (status_host) {
+    HIPCHK(hipMemcpyAsync(status_host, c->status, sizeof(int32_t) * c->J, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+  }
+  return ADMX_OK;
+}
+
+template <typename T>
+int dev_em_q(admx_ctx* c) {
+  void* X = c->x[ADMX_X];
+  int nchunk = 0;
+  int64_t np = 0;
+  CHK(pass_q<T>(c, MODE_EM, X, c->pp(X), 0, c->I, 0, c->J, &nchunk, &np));
+  FinArgs f{c->part.p, c->I, nchunk, 0, c->J, X, c->x[ADMX_X_NEXT], nullptr, nullptr, nullptr, 0, 0, nullptr};
+  ProfScope ps(c, "fin_q_em");
+  launch_fin_q_em<T>((int)c->K, f, c->s);
+  return check_launch();
+}
+
+template <typename T>
+int dev_em_p(admx_ctx* c) {
+  void* X = c->x[ADMX_X];
+  int nchunk = 0;
+  CHK(pass_p<T>(c, MODE_EM, X, c->pp(X), 0, c->I, 0, c->J, &nchunk));
+  FinArgs f{c->part.p, c->J, nchunk, 0, c->J, c->pp(X), c->pp(c->x[ADMX_X_NEXT]), nullptr, nullptr, nullptr, 0, 0,
+            nullptr};
+  ProfScope ps(c, "fin_p_em");
+  launch_fin_p_em<T>((int)c->K, f, c->s);
+  return check_launch();
+}
+
+template <typename T>
+int dev_zal(admx_ctx* c, int64_t iter, const void* x, const void* xn, const void* xn2, void* U, void* V, void* out) {
+  const int col = (int)((iter - 1) % c->Q);
+  const int m = (int)std::min<int64_t>(iter, c->Q);
+  constexpr int NP = kMaxQ * kMaxQ + kMaxQ;
+  const int64_t len = 512 * NP + kMaxQ;
+  CHK(c->zal.grow((size_t)len * sizeof(double)));
+  ProfScope ps(c, "zal");
+  launch_zal<T>((int)c->K, c->I, c->J, x, xn, xn2, U, V, out, col, m, (int)c->Q, static_cast<double*>(c->zal.p), len,
+                c->s);
+  return check_launch();
+}
+
+template <typename T>
+int dev_qn_iteration(admx_ctx* c, int64_t iter, int fix_q, int fix_p, double* out) {
+  const size_t qbytes = (size_t)(c->K * c->I) * c->esz, pbytes = (size_t)(c->K * c->J) * c->esz;
+  void* X = c->x[ADMX_X];
+  void* XN = c->x[ADMX_X_NEXT];
+  void* XN2 = c->x[ADMX_X_NEXT2];
+  void* XT = c->x[ADMX_X_TMP];
+  HIPCHK(hipMemcpyAsync(c->lls + LL_PREV, c->lls + LL_NEW, sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  if (!fix_q) CHK(dev_update_q<T>(c, false, nullptr));
+  else HIPCHK(hipMemcpyAsync(XN, X, qbytes, hipMemcpyDeviceToDevice, c->s));
+  if (!fix_p) CHK(dev_update_p<T>(c, false, nullptr));
+  else HIPCHK(hipMemcpyAsync(c->pp(XN), c->pp(X), pbytes, hipMemcpyDeviceToDevice, c->s));
+  if (!fix_q) CHK(dev_update_q<T>(c, true, nullptr));
+  else HIPCHK(hipMemcpyAsync(XN2, X, qbytes, hipMemcpyDeviceToDevice, c->s));
+  if (!fix_p) CHK(dev_update_p<T>(c, true, nullptr));
+  else HIPCHK(hipMemcpyAsync(c->pp(XN2), c->pp(X), pbytes, hipMemcpyDeviceToDevice, c->s));
+  CHK(ll_to_slot<T>(c, XN2, c->pp(XN2), 0, c->I, 0, c->J, LL_BASIC));
+  CHK(dev_zal<T>(c, iter, X, XN, XN2, c->U, c->V, XT));
+  CHK(ll_to_slot<T>(c, XT, c->pp(XT), 0, c->I, 0, c->J, LL_QN));
+  {
+    ProfScope ps(c, "accept");
+    launch_accept<T>(c->N(), X, XT, XN2, c->lls, c->s);
+  }
+  CHK(check_launch());
+  HIPCHK(hipMemcpyAsync(c->h_lls, c->lls, sizeof(double) * LL_NSLOTS, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  out[0] = c->h_lls[LL_PREV];
+  out[1] = c->h_lls[LL_BASIC];
+  out[2] = c->h_lls[LL_QN];
+  out[3] = c->h_lls[LL_NEW];
+  out[4] = std::fabs((out[3] - out[0]) / out[0]);
+  return ADMX_OK;
+}
+
+template <typename T>
+int dev_init_em(admx_ctx* c, int64_t iters, double* lls) {
+  const size_t qbytes = (size_t)(c->K * c->I) * c->esz, pbytes = (size_t)(c->K * c->J) * c->esz;
+  void* X = c->x[ADMX_X];
+  void* XN = c->x[ADMX_X_NEXT];
+  for (int64_t it = 0; it < iters; ++it) {
+    CHK(dev_em_q<T>(c));
+    CHK(dev_em_p<T>(c));
+    HIPCHK(hipMemcpyAsync(c->pp(X), c->pp(XN), pbytes, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(X, XN, qbytes, hipMemcpyDeviceToDevice, c->s));
+    if (lls) {
+      CHK(ll_to_slot<T>(c, X, c->pp(X), 0, c->I, 0, c->J, LL_TMP));
+      HIPCHK(hipMemcpyAsync(lls + it, c->lls + LL_TMP, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    }
+  }
+  CHK(ll_to_slot<T>(c, X, c->pp(X), 0, c->I, 0, c->J, LL_NEW));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+// ---- host-array tile loops (stage, run, accumulate on device, copy back)
+int stage_in(admx_ctx* c, int slot, const void* host, size_t bytes, size_t cap) {
+  CHK(c->st[slot].grow(cap));
+  if (bytes) HIPCHK(hipMemcpyAsync(c->st[slot].p, host, bytes, hipMemcpyHostToDevice, c->s));
+  return ADMX_OK;
+}
+
+template <typename T>
+int loop_ll(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0, int64_t j1, double* out) {
+  const size_t qb = (size_t)(c->K * c->I) * sizeof(T), pb = (size_t)(c->K * c->J) * sizeof(T);
+  CHK(stage_in(c, 0, q, qb, qb));
+  CHK(stage_in(c, 1, p, pb, pb));
+  if (i1 > i0 && j1 > j0) {
+    CHK(ll_to_slot<T>(c, c->st[0].p, c->st[1].p, i0, i1, j0, j1, LL_TMP));
+    HIPCHK(hipMemcpyAsync(out, c->lls + LL_TMP, sizeof(double), hipMemcpyDeviceToHost, c->s));
+  } else {
+    *out = 0.0;
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+template <typename T>
+int loop_update(admx_ctx* c, bool qside, void* XtX, void* Xtz, const void* q, const void* p, int64_t i0, int64_t i1,
+                int64_t j0, int64_t j1) {
+  const int64_t K = c->K;
+  const size_t qb = (size_t)(K * c->I) * sizeof(T), pb = (size_t)(K * c->J) * sizeof(T);
+  const int64_t ncol = qside ? c->I : c->J;
+  const int64_t c0 = qside ? i0 : j0, c1 = qside ? i1 : j1;
+  const size_t hb = (size_t)(K * K) * sizeof(T), zb = (size_t)K * sizeof(T);
+  CHK(stage_in(c, 0, q, qb, qb));
+  CHK(stage_in(c, 1, p, pb, pb));
+  CHK(c->st[2].grow(hb * ncol));
+  CHK(c->st[3].grow(zb * ncol));
+  const size_t nb = (size_t)(c1 - c0);
+  char* dX = static_cast<char*>(c->st[2].p) + hb * c0;
+  char* dZ = static_cast<char*>(c->st[3].p) + zb * c0;
+  if (nb) {
+    HIPCHK(hipMemcpyAsync(dX, static_cast<char*>(XtX) + hb * c0, hb * nb, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(dZ, static_cast<char*>(Xtz) + zb * c0, zb * nb, hipMemcpyHostToDevice, c->s));
+  }
+  int nchunk = 0;
+  int64_t np = 0;
+  if (qside) CHK(pass_q<T>(c, MODE_UPDATE, c->st[0].p, c->st[1].p, i0, i1, j0, j1, &nchunk, &np));
+  else CHK(pass_p<T>(c, MODE_UPDATE, c->st[0].p, c->st[1].p, i0, i1, j0, j1, &nchunk));
+  if (nb) {
+    FinArgs f{c->part.p, (int64_t)nb, nchunk, c0, c->J, qside ? c->st[0].p : c->st[1].p, nullptr, nullptr,
+              c->st[2].p, c->st[3].p, 1, 0, nullptr};
+    {
+      ProfScope ps(c, qside ? "fin_q_update" : "fin_p_update");
+      if (qside) launch_fin_q_update<T>((int)K, f, false, c->s);
+      else launch_fin_p_update<T>((int)K, f, false, c->s);
+    }
+    CHK(check_launch());
+    HIPCHK(hipMemcpyAsync(static_cast<char*>(XtX) + hb * c0, dX, hb * nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(static_cast<char*>(Xtz) + zb * c0, dZ, zb * nb, hipMemcpyDeviceToHost, c->s));
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+template <typename T>
+int loop_em_q(admx_ctx* c, void* q_next, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0,
+              int64_t j1) {
+  const int64_t K = c->K;
+  const size_t qb = (size_t)(K * c->I) * sizeof(T), pb = (size_t)(K * c->J) * sizeof(T), cb = K * sizeof(T);
+  CHK(stage_in(c, 0, q, qb, qb));
+  CHK(stage_in(c, 1, p, pb, pb));
+  CHK(c->st[2].grow(qb));
+  const size_t nb = (size_t)(i1 - i0);
+  char* dO = static_cast<char*>(c->st[2].p) + cb * i0;
+  if (nb) HIPCHK(hipMemcpyAsync(dO, static_cast<char*>(q_next) + cb * i0, cb * nb, hipMemcpyHostToDevice, c->s));
+  int nchunk = 0;
+  int64_t np = 0;
+  CHK(pass_q<T>(c, MODE_EM, c->st[0].p, c->st[1].p, i0, i1, j0, j1, &nchunk, &np));
+  if (nb) {
+    FinArgs f{c->part.p, (int64_t)nb, nchunk, i0, c->J, c->st[0].p, c->st[2].p, nullptr, nullptr, nullptr, 1, 0,
+              nullptr};
+    {
+      ProfScope ps(c, "fin_q_em");
+      launch_fin_q_em<T>((int)K, f, c->s);
+    }
+    CHK(check_launch());
+    HIPCHK(hipMemcpyAsync(static_cast<char*>(q_next) + cb * i0, dO, cb * nb, hipMemcpyDeviceToHost, c->s));
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+template <typename T>
+int loop_em_p(admx_ctx* c, void* p_next, void* p_tmp, const void* q, const void* p, int64_t i0, int64_t i1,
+              int64_t j0, int64_t j1) {
+  const int64_t K = c->K;
+  const size_t qb = (size_t)(K * c->I) * sizeof(T), pb = (size_t)(K * c->J) * sizeof(T), cb = K * sizeof(T);
+  CHK(stage_in(c, 0, q, qb, qb));
+  CHK(stage_in(c, 1, p, pb, pb));
+  CHK(c->st[2].grow(pb));
+  CHK(c->st[3].grow(pb));
+  const size_t nb = (size_t)(j1 - j0);
+  char* dO = static_cast<char*>(c->st[2].p) + cb * j0;
+  char* dT = static_cast<char*>(c->st[3].p) + cb * j0;
+  if (nb) {
+    HIPCHK(hipMemcpyAsync(dO, static_cast<char*>(p_next) + cb * j0, cb * nb, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(dT, static_cast<char*>(p_tmp) + cb * j0, cb * nb, hipMemcpyHostToDevice, c->s));
+  }
+  int nchunk = 0;
+  CHK(pass_p<T>(c, MODE_EM, c->st[0].p, c->st[1].p, i0, i1, j0, j1, &nchunk));
+  if (nb) {
+    FinArgs f{c->part.p, (int64_t)nb, nchunk, j0, c->J, c->st[1].p, c->st[2].p, c->st[3].p, nullptr, nullptr, 1, 0,
+              nullptr};
+    {
+      ProfScope ps(c, "fin_p_em");
+      launch_fin_p_em<T>((int)K, f, c->s);
+    }
+    CHK(check_launch());
+    HIPCHK(hipMemcpyAsync(static_cast<char*>(p_next) + cb * j0, dO, cb * nb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(static_cast<char*>(p_tmp) + cb * j0, dT, cb * nb, hipMemcpyDeviceToHost, c->s));
+  }
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+template <typename T>
+int host_qp_batch(admx_ctx* c, int simplex, int64_t n, const void* XtX, const void* Xtz, const void* x, void* delta,
+                  int32_t* status) {
+  const int64_t K = c->K;
+  const size_t hb = (size_t)(K * K * n) * sizeof(T), zb = (size_t)(K * n) * sizeof(T);
+  CHK(stage_in(c, 0, XtX, hb, hb));
+  CHK(stage_in(c, 1, Xtz, zb, zb));
+  CHK(stage_in(c, 2, x, zb, zb));
+  CHK(c->st[3].grow(zb));
+  CHK(c->st[4].grow(sizeof(int32_t) * n));
+  {
+    ProfScope ps(c, "qp_batch");
+    launch_qp_batch<T>((int)K, simplex != 0, n, c->st[0].p, c->st[1].p, c->st[2].p, c->st[3].p,
+                       static_cast<int32_t*>(c->st[4].p), c->s);
+  }
+  CHK(check_launch());
+  HIPCHK(hipMemcpyAsync(delta, c->st[3].p, zb, hipMemcpyDeviceToHost, c->s));
+  if (status) HIPCHK(hipMemcpyAsync(status, c->st[4].p, sizeof(int32_t) * n, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+template <typename T>
+int host_project(admx_ctx* c, bool simplex, void* b, int64_t n) {
+  const size_t bytes = (size_t)(simplex ? c->K * n : n) * sizeof(T);
+  CHK(stage_in(c, 0, b, bytes, bytes));
+  {
+    ProfScope ps(c, simplex ? "project_q" : "project_p");
+    if (simplex) launch_project_q<T>((int)c->K, n, c->st[0].p, c->s);
+    else launch_project_p<T>(n, c->st[0].p, c->s);
+  }
+  CHK(check_launch());
+  HIPCHK(hipMemcpyAsync(b, c->st[0].p, bytes, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+template <typename T>
+int host_zal(admx_ctx* c, int64_t iter, const void* x, const void* xn, const void* xn2, void* U, void* V,
+             void* x_tmp) {
+  const size_t nb = (size_t)c->N() * sizeof(T), ub = nb * (size_t)c->Q;
+  CHK(stage_in(c, 0, x, nb, nb));
+  CHK(stage_in(c, 1, xn, nb, nb));
+  CHK(stage_in(c, 2, xn2, nb, nb));
+  CHK(stage_in(c, 3, U, ub, ub));
+  CHK(stage_in(c, 4, V, ub, ub));
+  CHK(c->st[5].grow(nb));
+  CHK(dev_zal<T>(c, iter, c->st[0].p, c->st[1].p, c->st[2].p, c->st[3].p, c->st[4].p, c->st[5].p));
+  HIPCHK(hipMemcpyAsync(U, c->st[3].p, ub, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(V, c->st[4].p, ub, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(x_tmp, c->st[5].p, nb, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+#define BY_DTYPE(c, fn, ...) ((c)->dtype == ADMX_F64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
+
+struct Guard {
+  std::unique_lock<std::mutex> lk;
+  explicit Guard(admx_ctx* c) : lk(c->mu) {}
+};
+
+}  // namespace
+
+// =============================================================================== C ABI
+extern "C" {
+
+const char* admx_version(void) { return "libadmixhip 0.1.0 (gfx950)"; }
+
+const char* admx_last_error(void) { return g_err.c_str(); }
+
+int admx_device_count(int* n) {
+  if (!n) return bad("null pointer");
+  int c = 0;
+  HIPCHK(hipGetDeviceCount(&c));
+  *n = c;
+  return ADMX_OK;
+}
+
+int admx_create(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int dtype) {
+  if (!out) return bad("null output pointer");
+  *out = nullptr;
+  if (I < 1 || J < 1) return bad("I and J must be >= 1");
+  if (K < 1 || K > kMaxK) return bad("K must be in 1..16");
+  if (Q < 1 || Q > kMaxQ) return bad("Q must be in 1..8");
+  if (dtype != ADMX_F64 && dtype != ADMX_F32) return bad("dtype must be ADMX_F64 or ADMX_F32");
+  admx_ctx* c = new admx_ctx();
+  c->I = I;
+  c->J = J;
+  c->K = K;
+  c->Q = Q;
+  c->dtype = dtype;
+  c->esz = dtype == ADMX_F64 ? 8 : 4;
+  c->ibytes = (I + 3) / 4;
+  c->pitch = (c->ibytes + 63) & ~(int64_t)63;
+  auto cleanup = [&](int rc) {
+    admx_destroy(c);
+    return rc;
+  };
+  hipError_t e = hipGetDevice(&c->device);
+  if (e != hipSuccess) return cleanup(fail(ADMX_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e)));
+  if ((e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking)) != hipSuccess)
+    return cleanup(fail(ADMX_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
+  const size_t xb = (size_t)c->N() * c->esz;
+  auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, bytes < 256 ? 256 : bytes); };
+  if (alloc((void**)&c->bed, (size_t)c->pitch * J) != hipSuccess || alloc(&c->x[0], xb) != hipSuccess ||
+      alloc(&c->x[1], xb) != hipSuccess || alloc(&c->x[2], xb) != hipSuccess || alloc(&c->x[3], xb) != hipSuccess ||
+      alloc(&c->XtX_q, (size_t)(K * K * I) * c->esz) != hipSuccess ||
+      alloc(&c->Xtz_q, (size_t)(K * I) * c->esz) != hipSuccess ||
+      alloc(&c->XtX_p, (size_t)(K * K * J) * c->esz) != hipSuccess ||
+      alloc(&c->Xtz_p, (size_t)(K * J) * c->esz) != hipSuccess || alloc(&c->U, xb * Q) != hipSuccess ||
+      alloc(&c->V, xb * Q) != hipSuccess || alloc((void**)&c->lls, sizeof(double) * LL_NSLOTS) != hipSuccess ||
+      alloc((void**)&c->status, sizeof(int32_t) * std::max(I, J)) != hipSuccess)
+    return cleanup(fail(ADMX_EHIP, "hipMalloc failed for the context buffers"));
+  if (hipHostMalloc((void**)&c->h_lls, sizeof(double) * LL_NSLOTS, hipHostMallocDefault) != hipSuccess)
+    return cleanup(fail(ADMX_EHIP, "hipHostMalloc failed"));
+  double nanv[LL_NSLOTS];
+  for (int s = 0; s < LL_NSLOTS; ++s) nanv[s] = std::nan("");
+  if (hipMemcpy(c->lls, nanv, sizeof(nanv), hipMemcpyHostToDevice) != hipSuccess)
+    return cleanup(fail(ADMX_EHIP, "hipMemcpy failed"));
+  *out = c;
+  return ADMX_OK;
+}
+
+int admx_destroy(admx_ctx* c) {
+  if (!c) return ADMX_OK;
+  (void)hipSetDevice(c->device);
+  if (c->s) (void)hipStreamSynchronize(c->s);
+  for (auto& e : c->prof.pend) {
+    (void)hipEventDestroy(e.a);
+    (void)hipEventDestroy(e.b);
+  }
+  for (auto e : c->prof.pool) (void)hipEventDestroy(e);
+  void* bufs[] = {c->bed, c->x[0], c->x[1], c->x[2], c->x[3], c->XtX_q, c->Xtz_q, c->XtX_p,
+                  c->Xtz_p, c->U,   c->V,    c->lls,  c->status};
+  for (void* b : bufs)
+    if (b) (void)hipFree(b);
+  if (c->h_lls) (void)hipHostFree(c->h_lls);
+  c->part.release();
+  c->llpart.release();
+  c->zal.release();
+  for (auto& b : c->st) b.release();
+  if (c->s) (void)hipStreamDestroy(c->s);
+  delete c;
+  return ADMX_OK;
+}
+
+int admx_upload_bed(admx_ctx* c, const uint8_t* bed, int64_t bytes_per_snp) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  if (!bed) return bad("null BED pointer");
+  if (bytes_per_snp < c->ibytes) return bad("bytes_per_snp < ceil(I/4)");
+  HIPCHK(hipMemsetAsync(c->bed, 0, (size_t)c->pitch * c->J, c->s));
+  HIPCHK(hipMemcpy2DAsync(c->bed, c->pitch, bed, bytes_per_snp, c->ibytes, c->J, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  c->have_bed = true;
+  return ADMX_OK;
+}
+
+int admx_download_bed(admx_ctx* c, uint8_t* bed) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  if (!bed) return bad("null BED pointer");
+  HIPCHK(hipMemcpy2DAsync(bed, c->ibytes, c->bed, c->pitch, c->ibytes, c->J, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_generate_bed(admx_ctx* c, const double* Qt, const double* Pt, uint64_t seed, double miss_rate) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  if (!Qt || !Pt) return bad("null Q or P");
+  const size_t qb = (size_t)(c->K * c->I) * sizeof(double), pb = (size_t)(c->K * c->J) * sizeof(double);
+  CHK(stage_in(c, 0, Qt, qb, qb));
+  CHK(stage_in(c, 1, Pt, pb, pb));
+  {
+    ProfScope ps(c, "gen_bed");
+    launch_gen_bed(static_cast<const double*>(c->st[0].p), static_cast<const double*>(c->st[1].p), c->I, c->J,
+                   (int)c->K, seed, miss_rate, c->bed, c->pitch, c->s);
+  }
+  CHK(check_launch());
+  HIPCHK(hipStreamSynchronize(c->s));
+  c->have_bed = true;
+  return ADMX_OK;
+}
+
+int admx_decode(admx_ctx* c, int8_t* gout) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  if (!gout) return bad("null output");
+  const size_t bytes = (size_t)(c->I * c->J);
+  CHK(c->st[0].grow(bytes));
+  {
+    ProfScope ps(c, "decode");
+    launch_decode(c->bed, c->pitch, c->I, c->J, static_cast<int8_t*>(c->st[0].p), c->s);
+  }
+  CHK(check_launch());
+  HIPCHK(hipMemcpyAsync(gout, c->st[0].p, bytes, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_counts(admx_ctx* c, int64_t* snp_nm, int64_t* snp_alt, int64_t* ind_nm) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  if (!snp_nm || !snp_alt || !ind_nm) return bad("null output");
+  const size_t jb = sizeof(unsigned long long) * c->J, ib = sizeof(unsigned long long) * c->I;
+  CHK(c->st[0].grow(jb));
+  CHK(c->st[1].grow(jb));
+  CHK(c->st[2].grow(ib));
+  HIPCHK(hipMemsetAsync(c->st[2].p, 0, ib, c->s));
+  {
+    ProfScope ps(c, "counts");
+    launch_counts(c->bed, c->pitch, c->I, c->J, static_cast<unsigned long long*>(c->st[0].p),
+                  static_cast<unsigned long long*>(c->st[1].p), static_cast<unsigned long long*>(c->st[2].p), c->s);
+  }
+  CHK(check_launch());
+  HIPCHK(hipMemcpyAsync(snp_nm, c->st[0].p, jb, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(snp_alt, c->st[1].p, jb, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(ind_nm, c->st[2].p, ib, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_loglikelihood_loop(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0,
+                            int64_t j1, double* out) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  CHK(check_ranges(c, i0, i1, j0, j1));
+  if (!q || !p || !out) return bad("null pointer");
+  return BY_DTYPE(c, loop_ll, c, q, p, i0, i1, j0, j1, out);
+}
+
+int admx_em_q_loop(admx_ctx* c, void* q_next, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0,
+                   int64_t j1) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  CHK(check_ranges(c, i0, i1, j0, j1));
+  if (!q_next || !q || !p) return bad("null pointer");
+  return BY_DTYPE(c, loop_em_q, c, q_next, q, p, i0, i1, j0, j1);
+}
+
+int admx_em_p_loop(admx_ctx* c, void* p_next, void* p_tmp, const void* q, const void* p, int64_t i0, int64_t i1,
+                   int64_t j0, int64_t j1) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  CHK(check_ranges(c, i0, i1, j0, j1));
+  if (!p_next || !p_tmp || !q || !p) return bad("null pointer");
+  return BY_DTYPE(c, loop_em_p, c, p_next, p_tmp, q, p, i0, i1, j0, j1);
+}
+
+int admx_update_q_loop(admx_ctx* c, void* XtX, void* Xtz, const void* q, const void* p, int64_t i0, int64_t i1,
+                       int64_t j0, int64_t j1) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  CHK(check_ranges(c, i0, i1, j0, j1));
+  if (!XtX || !Xtz || !q || !p) return bad("null pointer");
+  return BY_DTYPE(c, loop_update, c, true, XtX, Xtz, q, p, i0, i1, j0, j1);
+}
+
+int admx_update_p_loop(admx_ctx* c, void* XtX, void* Xtz, const void* q, const void* p, int64_t i0, int64_t i1,
+                       int64_t j0, int64_t j1) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  CHK(check_ranges(c, i0, i1, j0, j1));
+  if (!XtX || !Xtz || !q || !p) return bad("null pointer");
+  return BY_DTYPE(c, loop_update, c, false, XtX, Xtz, q, p, i0, i1, j0, j1);
+}
+
+int admx_qp_batch(admx_ctx* c, int simplex, int64_t n, const void* XtX, const void* Xtz_neg, const void* x,
+                  void* delta, int32_t* status) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  if (n < 0) return bad("n < 0");
+  if (n == 0) return ADMX_OK;
+  if (!XtX || !Xtz_neg || !x || !delta) return bad("null pointer");
+  return BY_DTYPE(c, host_qp_batch, c, simplex, n, XtX, Xtz_neg, x, delta, status);
+}
+
+int admx_project_q(admx_ctx* c, void* b, int64_t n) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  if (n < 0 || (n > 0 && !b)) return bad("bad buffer");
+  if (n == 0) return ADMX_OK;
+  return BY_DTYPE(c, host_project, c, true, b, n);
+}
+
+int admx_project_p(admx_ctx* c, void* b, int64_t n) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  if (n < 0 || (n > 0 && !b)) return bad("bad buffer");
+  if (n == 0) return ADMX_OK;
+  return BY_DTYPE(c, host_project, c, false, b, n);
+}
+
+int admx_zal_step(admx_ctx* c, int64_t iter, const void* x, const void* xn, const void* xn2, void* U, void* V,
+                  void* x_tmp) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  if (iter < 1) return bad("iter must be >= 1");
+  if (!x || !xn || !xn2 || !U || !V || !x_tmp) return bad("null pointer");
+  return BY_DTYPE(c, host_zal, c, iter, x, xn, xn2, U, V, x_tmp);
+}
+
+int admx_set_x(admx_ctx* c, int which, const void* host) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  if (which < 0 || which > 3 || !host) return bad("bad state vector");
+  HIPCHK(hipMemcpyAsync(c->x[which], host, (size_t)c->N() * c->esz, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_get_x(admx_ctx* c, int which, void* host) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  if (which < 0 || which > 3 || !host) return bad("bad state vector");
+  HIPCHK(hipMemcpyAsync(host, c->x[which], (size_t)c->N() * c->esz, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_set_uv(admx_ctx* c, const void* U, const void* V) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  if (!U || !V) return bad("null pointer");
+  const size_t b = (size_t)c->N() * c->Q * c->esz;
+  HIPCHK(hipMemcpyAsync(c->U, U, b, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipMemcpyAsync(c->V, V, b, hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_get_uv(admx_ctx* c, void* U, void* V) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  if (!U || !V) return bad("null pointer");
+  const size_t b = (size_t)c->N() * c->Q * c->esz;
+  HIPCHK(hipMemcpyAsync(U, c->U, b, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(V, c->V, b, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_set_ll(admx_ctx* c, double ll_prev, double ll_new) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  double v[2] = {ll_prev, ll_new};
+  HIPCHK(hipMemcpyAsync(c->lls + LL_PREV, v, sizeof(v), hipMemcpyHostToDevice, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_get_ll(admx_ctx* c, double* ll_prev, double* ll_new) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  HIPCHK(hipMemcpyAsync(c->h_lls, c->lls, sizeof(double) * LL_NSLOTS, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  if (ll_prev) *ll_prev = c->h_lls[LL_PREV];
+  if (ll_new) *ll_new = c->h_lls[LL_NEW];
+  return ADMX_OK;
+}
+
+int admx_get_hessians(admx_ctx* c, int side, void* XtX, void* Xtz) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  if (!XtX || !Xtz) return bad("null pointer");
+  const int64_t n = side == ADMX_SIDE_Q ? c->I : c->J;
+  HIPCHK(hipMemcpyAsync(XtX, side == ADMX_SIDE_Q ? c->XtX_q : c->XtX_p, (size_t)(c->K * c->K * n) * c->esz,
+                        hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(Xtz, side == ADMX_SIDE_Q ? c->Xtz_q : c->Xtz_p, (size_t)(c->K * n) * c->esz,
+                        hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_loglikelihood(admx_ctx* c, int which, double* out) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  if (which < 0 || which > 3 || !out) return bad("bad state vector");
+  void* X = c->x[which];
+  CHK(BY_DTYPE(c, ll_to_slot, c, X, c->pp(X), 0, c->I, 0, c->J, LL_TMP));
+  HIPCHK(hipMemcpyAsync(out, c->lls + LL_TMP, sizeof(double), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_em_q(admx_ctx* c) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  CHK(BY_DTYPE(c, dev_em_q, c));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_em_p(admx_ctx* c) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  CHK(BY_DTYPE(c, dev_em_p, c));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_init_em(admx_ctx* c, int64_t iters, double* lls) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  if (iters < 0) return bad("iters < 0");
+  return BY_DTYPE(c, dev_init_em, c, iters, lls);
+}
+
+int admx_update_q(admx_ctx* c, int update2, int32_t* status) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  CHK(BY_DTYPE(c, dev_update_q, c, update2 != 0, status));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_update_p(admx_ctx* c, int update2, int32_t* status) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  CHK(BY_DTYPE(c, dev_update_p, c, update2 != 0, status));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_qn_iteration(admx_ctx* c, int64_t iter, int fix_q, int fix_p, double* out) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(need_bed(c));
+  if (iter < 1 || !out) return bad("iter must be >= 1 and out non-null");
+  return BY_DTYPE(c, dev_qn_iteration, c, iter, fix_q, fix_p, out);
+}
+
+int admx_synchronize(admx_ctx* c) {
+  CHK(check_ctx(c));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+int admx_profile_enable(admx_ctx* c, int on) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(prof_collect(c));
+  c->prof.on = on != 0;
+  return ADMX_OK;
+}
+
+int admx_profile_reset(admx_ctx* c) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(prof_collect(c));
+  c->prof.acc.clear();
+  return ADMX_OK;
+}
+
+int admx_profile_report(admx_ctx* c, char* buf, size_t len) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  CHK(prof_collect(c));
+  std::string s;
+  char line[256];
+  for (auto& kv : c->prof.acc) {
+    std::snprintf(line, sizeof(line), "%s %ld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
+    s += line;
+  }
+  if (!buf || len == 0) return bad("null buffer");
+  if (s.size() + 1 > len) return bad("buffer too small");
+  std::memcpy(buf, s.c_str(), s.size() + 1);
+  return ADMX_OK;
+}
+
+int admx_set_tuning(admx_ctx* c, int key, int64_t value) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  if (key == ADMX_TUNE_TARGET_WAVES) {
+    if (value < 1) return bad("target waves must be >= 1");
+    c->target_waves = value;
+  } else if (key == ADMX_TUNE_KEEP_HESSIANS) {
+    c->keep_hess = value != 0;
+  } else {
+    return bad("unknown tuning key");
+  }
+  return ADMX_OK;
+}
+
+}  // extern "C"

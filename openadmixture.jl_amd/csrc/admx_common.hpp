@@ -14,8 +14,6 @@
 
 namespace admx {
 
-constexpr int kMaxK = 16;
-constexpr int kMaxQ = 8;  // quasi-Newton history columns supported on device
 
 // PLINK-1 code -> (m*g, m*(2-g)) with m the nonmissing flag:
 //   00 -> (0,2)  01 (missing) -> (0,0)  10 -> (1,1)  11 -> (2,0)

@@ -6,6 +6,9 @@
 
 namespace admx {
 
+constexpr int kMaxK = 16;  // populations supported (templated kernels)
+constexpr int kMaxQ = 8;   // quasi-Newton history columns supported on device
+
 enum PassMode : int { MODE_UPDATE = 0, MODE_EM = 1, MODE_LL = 2 };
 
 // Arguments of one genotype pass over [i0,i1) x [j0,j1).

@@ -1,0 +1,364 @@
+// admx_misc.hip -- the smaller device kernels around the passes:
+//   * fixed-order reduction of per-block log-likelihood partials;
+//   * ZAL quasi-Newton extrapolation (src/quasi_newton.jl:5-31) with the
+//     projections of src/algorithms_outer.jl:141-142 fused into its last step;
+//   * the QN accept rule (src/algorithms_outer.jl:150-156) on device;
+//   * BED decode and missing / allele counts (HBM-bound);
+//   * the synthetic-cohort genotype generator.
+#include "admx_common.hpp"
+#include "admx_internal.hpp"
+#include "admx_project.hpp"
+
+namespace admx {
+
+// ------------------------------------------------------------------ reductions
+__global__ __launch_bounds__(256) void k_reduce_sum(const double* __restrict__ parts, int64_t n,
+                                                    double* __restrict__ out, int accumulate) {
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int64_t e = threadIdx.x; e < n; e += 256) s += parts[e];
+  const double tot = block_sum<256>(s, sh);
+  if (threadIdx.x == 0) *out = accumulate ? *out + tot : tot;
+}
+
+void launch_reduce_sum(const double* parts, int64_t nparts, double* out, int accumulate, hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, s, parts, nparts, out, accumulate);
+}
+
+// ------------------------------------------------------------------ ZAL quasi-Newton
+// update_UV!: U[:,col] = x_next - x, V[:,col] = x_next2 - x_next, and the
+// partial inner products A[a,b] = U[:,a]'(U[:,b] - V[:,b]) and
+// r[a] = U[:,a]'(x - x_next) over the first m columns, accumulated in double.
+template <typename T>
+__global__ __launch_bounds__(256) void k_zal_dots(int64_t N, const T* __restrict__ x, const T* __restrict__ xn,
+                                                  const T* __restrict__ xn2, T* __restrict__ U, T* __restrict__ V,
+                                                  int col, int m, double* __restrict__ part) {
+  __shared__ double sh[256];
+  constexpr int NP = kMaxQ * kMaxQ + kMaxQ;
+  double acc[NP];
+#pragma unroll
+  for (int a = 0; a < NP; ++a) acc[a] = 0.0;
+  for (int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256) {
+    const T xv = x[n], xnv = xn[n], xn2v = xn2[n];
+    const T u = xnv - xv, v = xn2v - xnv;
+    U[n + N * col] = u;
+    V[n + N * col] = v;
+    T uu[kMaxQ], vv[kMaxQ];
+#pragma unroll
+    for (int a = 0; a < kMaxQ; ++a) {
+      if (a < m) {
+        uu[a] = (a == col) ? u : U[n + N * a];
+        vv[a] = (a == col) ? v : V[n + N * a];
+      } else {
+        uu[a] = 0;
+        vv[a] = 0;
+      }
+    }
+    const T dx = xv - xnv;
+#pragma unroll
+    for (int a = 0; a < kMaxQ; ++a) {
+      acc[kMaxQ * kMaxQ + a] += (double)uu[a] * (double)dx;
+#pragma unroll
+      for (int b = 0; b < kMaxQ; ++b) acc[a + kMaxQ * b] += (double)uu[a] * (double)(uu[b] - vv[b]);
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < NP; ++a) {
+    const double tot = block_sum<256>(acc[a], sh);
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * NP + a] = tot;
+  }
+}
+
+// Sum the block partials in block order, then coef = inv(A) * r with the
+// inverse by Gauss-Jordan elimination with partial pivoting (Julia's inv).
+__global__ __launch_bounds__(64) void k_zal_solve(const double* __restrict__ part, int nblocks, int m,
+                                                  double* __restrict__ coef) {
+  constexpr int NP = kMaxQ * kMaxQ + kMaxQ;
+  __shared__ double tot[NP];
+  for (int a = threadIdx.x; a < NP; a += 64) {
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part[(int64_t)b * NP + a];
+    tot[a] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  double W[kMaxQ][2 * kMaxQ];
+  const int w = 2 * m;
+  for (int r = 0; r < m; ++r)
+    for (int c = 0; c < m; ++c) {
+      W[r][c] = tot[r + kMaxQ * c];
+      W[r][m + c] = (r == c) ? 1.0 : 0.0;
+    }
+  for (int c = 0; c < m; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < m; ++r)
+      if (fabs(W[r][c]) > fabs(W[piv][c])) piv = r;
+    if (piv != c)
+      for (int t = 0; t < w; ++t) {
+        const double tt = W[c][t];
+        W[c][t] = W[piv][t];
+        W[piv][t] = tt;
+      }
+    const double inv = 1.0 / W[c][c];
+    for (int t = 0; t < w; ++t) W[c][t] *= inv;
+    for (int r = 0; r < m; ++r)
+      if (r != c) {
+        const double f = W[r][c];
+        if (f != 0.0)
+          for (int t = 0; t < w; ++t) W[r][t] -= f * W[c][t];
+      }
+  }
+  for (int a = 0; a < m; ++a) {
+    double s = 0.0;
+    for (int b = 0; b < m; ++b) s += W[a][m + b] * tot[kMaxQ * kMaxQ + b];
+    coef[a] = s;
+  }
+}
+
+// x_tmp = x_next - V coef, then project_q! on the Q columns ...
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_zal_apply_q(int64_t I, int64_t N, const T* __restrict__ xn,
+                                                     const T* __restrict__ V, const double* __restrict__ coef, int m,
+                                                     T* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= I) return;
+  T cf[kMaxQ];
+#pragma unroll
+  for (int a = 0; a < kMaxQ; ++a) cf[a] = (a < m) ? (T)coef[a] : (T)0;
+  T v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t n = (int64_t)K * i + k;
+    T s = 0;
+    for (int a = 0; a < m; ++a) s += V[n + N * a] * cf[a];
+    v[k] = xn[n] - s;
+  }
+  project_simplex<T, K>(v);
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[(int64_t)K * i + k] = v[k];
+}
+
+// ... and project_p! on the P entries.
+template <typename T>
+__global__ __launch_bounds__(256) void k_zal_apply_p(int64_t n0, int64_t N, const T* __restrict__ xn,
+                                                     const T* __restrict__ V, const double* __restrict__ coef, int m,
+                                                     T* __restrict__ out) {
+  T cf[kMaxQ];
+#pragma unroll
+  for (int a = 0; a < kMaxQ; ++a) cf[a] = (a < m) ? (T)coef[a] : (T)0;
+  for (int64_t n = n0 + (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256) {
+    T s = 0;
+    for (int a = 0; a < m; ++a) s += V[n + N * a] * cf[a];
+    out[n] = project_box<T>(xn[n] - s);
+  }
+}
+
+template <typename T, int K>
+static void zal_apply_q_k(int64_t I, int64_t N, const void* xn, const void* V, const double* coef, int m, void* out,
+                          hipStream_t s) {
+  hipLaunchKernelGGL((k_zal_apply_q<T, K>), dim3((unsigned)((I + 255) / 256)), dim3(256), 0, s, I, N, (const T*)xn,
+                     (const T*)V, coef, m, (T*)out);
+}
+
+static unsigned grid_for(int64_t n, int64_t cap) {
+  int64_t b = (n + 255) / 256;
+  if (b < 1) b = 1;
+  return (unsigned)(b < cap ? b : cap);
+}
+
+template <typename T>
+void launch_zal(int K, int64_t I, int64_t J, const void* x, const void* xn, const void* xn2, void* U, void* V,
+                void* x_out, int col, int m, int Qcols, double* scratch, int64_t scratch_len, hipStream_t s) {
+  (void)Qcols;
+  const int64_t N = (int64_t)K * (I + J);
+  constexpr int NP = kMaxQ * kMaxQ + kMaxQ;
+  int64_t nb = grid_for(N, 512);
+  if ((nb * NP + kMaxQ) > scratch_len) nb = (scratch_len - kMaxQ) / NP;
+  double* part = scratch;
+  double* coef = scratch + nb * NP;
+  hipLaunchKernelGGL((k_zal_dots<T>), dim3((unsigned)nb), dim3(256), 0, s, N, (const T*)x, (const T*)xn,
+                     (const T*)xn2, (T*)U, (T*)V, col, m, part);
+  hipLaunchKernelGGL(k_zal_solve, dim3(1), dim3(64), 0, s, part, (int)nb, m, coef);
+  ADMX_DISPATCH_K(K, (zal_apply_q_k<T, KK>(I, N, xn, V, coef, m, x_out, s)));
+  const int64_t np = N - (int64_t)K * I;
+  hipLaunchKernelGGL((k_zal_apply_p<T>), dim3(grid_for(np, 4096)), dim3(256), 0, s, (int64_t)K * I, N,
+                     (const T*)xn, (const T*)V, coef, m, (T*)x_out);
+}
+
+// ------------------------------------------------------------------ accept rule
+template <typename T>
+__global__ __launch_bounds__(256) void k_accept(int64_t N, T* __restrict__ x, const T* __restrict__ x_tmp,
+                                                const T* __restrict__ x_next2, double* __restrict__ lls) {
+  const bool take_qn = lls[LL_PREV] < lls[LL_QN];
+  const T* src = take_qn ? x_tmp : x_next2;
+  for (int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256) x[n] = src[n];
+  if (blockIdx.x == 0 && threadIdx.x == 0) lls[LL_NEW] = take_qn ? lls[LL_QN] : lls[LL_BASIC];
+}
+
+template <typename T>
+void launch_accept(int64_t N, void* x, const void* x_tmp, const void* x_next2, double* lls, hipStream_t s) {
+  hipLaunchKernelGGL((k_accept<T>), dim3(grid_for(N, 4096)), dim3(256), 0, s, N, (T*)x, (const T*)x_tmp,
+                     (const T*)x_next2, lls);
+}
+
+template void launch_zal<double>(int, int64_t, int64_t, const void*, const void*, const void*, void*, void*, void*,
+                                 int, int, int, double*, int64_t, hipStream_t);
+template void launch_zal<float>(int, int64_t, int64_t, const void*, const void*, const void*, void*, void*, void*,
+                                int, int, int, double*, int64_t, hipStream_t);
+template void launch_accept<double>(int64_t, void*, const void*, const void*, double*, hipStream_t);
+template void launch_accept<float>(int64_t, void*, const void*, const void*, double*, hipStream_t);
+
+// ------------------------------------------------------------------ BED decode / counts
+// Dense int8 genotypes (I x J column-major, -1 = missing), one lane per BED byte.
+__global__ __launch_bounds__(256) void k_decode(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int64_t J,
+                                                int8_t* __restrict__ g) {
+  const int64_t nb = (I + 3) >> 2;
+  const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (b >= nb) return;
+  for (int64_t j = blockIdx.y; j < J; j += gridDim.y) {
+    const uint32_t byte = bed[pitch * j + b];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int64_t i = 4 * b + t;
+      if (i < I) {
+        const uint32_t c = (byte >> (2 * t)) & 3u;
+        g[i + I * j] = (int8_t)(c == 1u ? -1 : code_g(c));
+      }
+    }
+  }
+}
+
+void launch_decode(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int8_t* g, hipStream_t s) {
+  const int64_t nb = (I + 3) >> 2;
+  dim3 grid((unsigned)((nb + 255) / 256), (unsigned)(J < 65535 ? J : 65535));
+  hipLaunchKernelGGL(k_decode, grid, dim3(256), 0, s, bed, pitch, I, J, g);
+}
+
+// Valid-individual mask (2 bits per individual) of 32-bit word d of a column.
+__device__ __forceinline__ uint32_t valid_mask(int64_t d, int64_t I) {
+  const int64_t rem = I - 16 * d;
+  if (rem >= 16) return 0xffffffffu;
+  if (rem <= 0) return 0u;
+  return (1u << (2 * rem)) - 1u;
+}
+
+// Per-SNP nonmissing count and A2 allele count: one wave per SNP column,
+// 16 genotypes per 32-bit load, popcounts, wave reduction.
+__global__ __launch_bounds__(256) void k_counts_snp(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I,
+                                                    int64_t J, unsigned long long* __restrict__ snp_nm,
+                                                    unsigned long long* __restrict__ snp_alt) {
+  const int lane = threadIdx.x & 63;
+  const int64_t j = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (j >= J) return;
+  const uint32_t* col = reinterpret_cast<const uint32_t*>(bed + pitch * j);
+  const int64_t ndw = (I + 15) >> 4;
+  unsigned long long nm = 0, alt = 0;
+  for (int64_t d = lane; d < ndw; d += 64) {
+    const uint32_t vm = valid_mask(d, I);
+    const uint32_t w = col[d];
+    const uint32_t lo = w & 0x55555555u & vm, hi = (w >> 1) & 0x55555555u & vm;
+    const uint32_t nvalid = __popc(vm) >> 1;
+    nm += nvalid - __popc(lo & ~hi);
+    alt += __popc(hi) + __popc(hi & lo);
+  }
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) {
+    nm += __shfl_xor(nm, off, 64);
+    alt += __shfl_xor(alt, off, 64);
+  }
+  if (lane == 0) {
+    snp_nm[j] = nm;
+    snp_alt[j] = alt;
+  }
+}
+
+// Per-individual nonmissing count: one lane per 32-bit word position (16
+// individuals) streaming a chunk of SNPs, integer atomics (exact) at the end.
+__global__ __launch_bounds__(256) void k_counts_ind(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I,
+                                                    int64_t J, int64_t jchunk, unsigned long long* __restrict__ ind_nm) {
+  const int64_t d = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t ndw = (I + 15) >> 4;
+  if (d >= ndw) return;
+  const int64_t jb = (int64_t)blockIdx.y * jchunk;
+  const int64_t je = (jb + jchunk < J) ? jb + jchunk : J;
+  uint32_t cnt[16];
+#pragma unroll
+  for (int t = 0; t < 16; ++t) cnt[t] = 0;
+  for (int64_t j = jb; j < je; ++j) {
+    const uint32_t w = *reinterpret_cast<const uint32_t*>(bed + pitch * j + 4 * d);
+    const uint32_t miss = w & ~(w >> 1) & 0x55555555u;
+#pragma unroll
+    for (int t = 0; t < 16; ++t) cnt[t] += 1u - ((miss >> (2 * t)) & 1u);
+  }
+#pragma unroll
+  for (int t = 0; t < 16; ++t) {
+    const int64_t i = 16 * d + t;
+    if (i < I && je > jb) atomicAdd(&ind_nm[i], (unsigned long long)cnt[t]);
+  }
+}
+
+void launch_counts(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, unsigned long long* snp_nm,
+                   unsigned long long* snp_alt, unsigned long long* ind_nm, hipStream_t s) {
+  hipLaunchKernelGGL(k_counts_snp, dim3((unsigned)((J + 3) / 4)), dim3(256), 0, s, bed, pitch, I, J, snp_nm, snp_alt);
+  const int64_t ndw = (I + 15) >> 4;
+  int64_t nchunk = (J + 255) / 256;
+  if (nchunk > 1024) nchunk = 1024;
+  if (nchunk < 1) nchunk = 1;
+  const int64_t jchunk = (J + nchunk - 1) / nchunk;
+  nchunk = (J + jchunk - 1) / jchunk;
+  if (nchunk < 1) nchunk = 1;
+  hipLaunchKernelGGL(k_counts_ind, dim3((unsigned)((ndw + 255) / 256), (unsigned)nchunk), dim3(256), 0, s, bed, pitch,
+                     I, J, jchunk, ind_nm);
+}
+
+// ------------------------------------------------------------------ synthetic genotypes
+__device__ __forceinline__ uint64_t smix(uint64_t z) {
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t h) { return (double)(h >> 11) * 0x1.0p-53; }
+
+// g_ij ~ Binomial(2, q_i . p_j) from two hashed uniforms, missing with
+// probability miss_rate from a third; one lane per BED byte (4 individuals).
+__global__ __launch_bounds__(256) void k_gen_bed(const double* __restrict__ Qt, const double* __restrict__ Pt,
+                                                 int64_t I, int64_t J, int K, uint64_t base, double miss_rate,
+                                                 uint8_t* __restrict__ bed, int64_t pitch) {
+  const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (b >= pitch) return;
+  for (int64_t j = blockIdx.y; j < J; j += gridDim.y) {
+    uint32_t byte = 0;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int64_t i = 4 * b + t;
+      if (i < I) {
+        double s = Qt[(int64_t)K * i] * Pt[(int64_t)K * j];
+        for (int k = 1; k < K; ++k) s = __builtin_fma(Qt[k + (int64_t)K * i], Pt[k + (int64_t)K * j], s);
+        const uint64_t ctr = (uint64_t)j * (uint64_t)I + (uint64_t)i;
+        const double u1 = u01(smix(smix(3 * ctr + 0) ^ base));
+        const double u2 = u01(smix(smix(3 * ctr + 1) ^ base));
+        const double u3 = u01(smix(smix(3 * ctr + 2) ^ base));
+        const int g = (u1 < s) + (u2 < s);
+        uint32_t code = g == 0 ? 0u : (g == 1 ? 2u : 3u);
+        if (u3 < miss_rate) code = 1u;
+        byte |= code << (2 * t);
+      }
+    }
+    bed[pitch * j + b] = (uint8_t)byte;
+  }
+}
+
+void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, int K, uint64_t seed, double miss_rate,
+                    uint8_t* bed, int64_t pitch, hipStream_t s) {
+  uint64_t base = seed ^ 0x5851F42D4C957F2DULL;
+  // smix on the host: identical arithmetic to the device/CPU definition.
+  base += 0x9E3779B97F4A7C15ULL;
+  base = (base ^ (base >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  base = (base ^ (base >> 27)) * 0x94D049BB133111EBULL;
+  base = base ^ (base >> 31);
+  dim3 grid((unsigned)((pitch + 255) / 256), (unsigned)(J < 65535 ? J : 65535));
+  hipLaunchKernelGGL(k_gen_bed, grid, dim3(256), 0, s, Qt, Pt, I, J, K, base, miss_rate, bed, pitch);
+}
+
+}  // namespace admx

@@ -1,0 +1,253 @@
+"""Device context: the MI355X counterpart of ``CuAdmixData{T}`` and ``g_cu``
+(src/cuda/structs.jl:1-38).
+
+A ``HIPAdmixData`` owns, in HBM, the 2-bit BED payload, the four state vectors
+x, x_next, x_next2, x_tmp (K x (I+J) each, q = first I columns), the Hessian
+and gradient stacks of both sides, the quasi-Newton history U, V and the
+log-likelihood slots.  Every method is one call into libadmixhip.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _native as N
+
+
+def _f(a, dtype, shape=None):
+    a = np.asarray(a)
+    if a.dtype != dtype or not (a.flags.f_contiguous or a.ndim <= 1):
+        raise TypeError(f"expected a column-major {np.dtype(dtype)} array, got {a.dtype} "
+                        f"(f_contiguous={a.flags.f_contiguous})")
+    if shape is not None and a.shape != tuple(shape):
+        raise ValueError(f"expected shape {tuple(shape)}, got {a.shape}")
+    return a
+
+
+class HIPAdmixData:
+    """Device mirror of AdmixData{T} plus the device-resident BED."""
+
+    def __init__(self, I: int, J: int, K: int, Q: int = 3, T=np.float64):
+        self.I, self.J, self.K, self.Q = int(I), int(J), int(K), int(Q)
+        self.T = np.dtype(T)
+        self._h = C.c_void_p()
+        N.check(N.lib().admx_create(C.byref(self._h), self.I, self.J, self.K, self.Q, N.dtype_code(self.T)))
+
+    # ------------------------------------------------------------ lifetime
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            N.lib().admx_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    @property
+    def handle(self):
+        return self._h
+
+    # ------------------------------------------------------------ genotypes
+    def upload_bed(self, data):
+        """Raw SNP-major payload: a (bytes_per_snp x J) column-major uint8 array
+        (SnpArray.data), or flat bytes with bytes_per_snp = ceil(I/4)."""
+        data = np.asarray(data, dtype=np.uint8)
+        if data.ndim == 2:
+            if data.shape[1] != self.J:
+                raise ValueError("BED payload must have J columns")
+            bps = data.shape[0]
+            data = np.asfortranarray(data)
+        else:
+            bps = (self.I + 3) // 4
+            if data.size != bps * self.J:
+                raise ValueError("flat BED payload must hold ceil(I/4)*J bytes")
+        N.check(N.lib().admx_upload_bed(self._h, N.ptr(data), bps))
+
+    def download_bed(self):
+        out = np.empty(((self.I + 3) // 4, self.J), dtype=np.uint8, order="F")
+        N.check(N.lib().admx_download_bed(self._h, N.ptr(out)))
+        return out
+
+    def generate_bed(self, Qt, Pt, seed: int, miss_rate: float):
+        Qt = np.asfortranarray(Qt, dtype=np.float64)
+        Pt = np.asfortranarray(Pt, dtype=np.float64)
+        if Qt.shape != (self.K, self.I) or Pt.shape != (self.K, self.J):
+            raise ValueError("Q must be K x I and P must be K x J")
+        N.check(N.lib().admx_generate_bed(self._h, N.ptr(Qt), N.ptr(Pt), C.c_uint64(seed), float(miss_rate)))
+
+    def decode(self):
+        g = np.empty((self.I, self.J), dtype=np.int8, order="F")
+        N.check(N.lib().admx_decode(self._h, N.ptr(g)))
+        return g
+
+    def counts(self):
+        snm = np.zeros(self.J, np.int64)
+        salt = np.zeros(self.J, np.int64)
+        inm = np.zeros(self.I, np.int64)
+        N.check(N.lib().admx_counts(self._h, N.ptr(snm), N.ptr(salt), N.ptr(inm)))
+        return snm, salt, inm
+
+    # ------------------------------------------------------------ state
+    def set_x(self, which: int, x):
+        x = _f(x, self.T, (self.K, self.I + self.J))
+        N.check(N.lib().admx_set_x(self._h, which, N.ptr(x)))
+
+    def get_x(self, which: int, out=None):
+        if out is None:
+            out = np.empty((self.K, self.I + self.J), self.T, order="F")
+        _f(out, self.T, (self.K, self.I + self.J))
+        N.check(N.lib().admx_get_x(self._h, which, N.ptr(out)))
+        return out
+
+    def set_uv(self, U, V):
+        shp = (self.K * (self.I + self.J), self.Q)
+        N.check(N.lib().admx_set_uv(self._h, N.ptr(_f(U, self.T, shp)), N.ptr(_f(V, self.T, shp))))
+
+    def get_uv(self, U=None, V=None):
+        shp = (self.K * (self.I + self.J), self.Q)
+        U = np.empty(shp, self.T, order="F") if U is None else _f(U, self.T, shp)
+        V = np.empty(shp, self.T, order="F") if V is None else _f(V, self.T, shp)
+        N.check(N.lib().admx_get_uv(self._h, N.ptr(U), N.ptr(V)))
+        return U, V
+
+    def set_ll(self, ll_prev: float, ll_new: float):
+        N.check(N.lib().admx_set_ll(self._h, float(ll_prev), float(ll_new)))
+
+    def get_ll(self):
+        a, b = C.c_double(), C.c_double()
+        N.check(N.lib().admx_get_ll(self._h, C.byref(a), C.byref(b)))
+        return a.value, b.value
+
+    def hessians(self, side: int):
+        n = self.I if side == N.ADMX_SIDE_Q else self.J
+        XtX = np.empty((self.K, self.K, n), self.T, order="F")
+        Xtz = np.empty((self.K, n), self.T, order="F")
+        N.check(N.lib().admx_get_hessians(self._h, side, N.ptr(XtX), N.ptr(Xtz)))
+        return XtX, Xtz
+
+    # ------------------------------------------------------------ device-resident algorithms
+    def loglikelihood(self, which: int = N.ADMX_X) -> float:
+        out = C.c_double()
+        N.check(N.lib().admx_loglikelihood(self._h, which, C.byref(out)))
+        return out.value
+
+    def em_q(self):
+        N.check(N.lib().admx_em_q(self._h))
+
+    def em_p(self):
+        N.check(N.lib().admx_em_p(self._h))
+
+    def init_em(self, iters: int):
+        lls = np.zeros(max(int(iters), 1))
+        N.check(N.lib().admx_init_em(self._h, int(iters), N.ptr(lls)))
+        return lls[: int(iters)]
+
+    def update_q(self, update2=False, status=False):
+        st = np.zeros(self.I, np.int32) if status else None
+        N.check(N.lib().admx_update_q(self._h, int(bool(update2)), N.ptr(st)))
+        return st
+
+    def update_p(self, update2=False, status=False):
+        st = np.zeros(self.J, np.int32) if status else None
+        N.check(N.lib().admx_update_p(self._h, int(bool(update2)), N.ptr(st)))
+        return st
+
+    def qn_iteration(self, it: int, fix_q=False, fix_p=False):
+        out = np.zeros(5)
+        N.check(N.lib().admx_qn_iteration(self._h, int(it), int(bool(fix_q)), int(bool(fix_p)), N.ptr(out)))
+        return dict(prev=out[0], basic=out[1], qn=out[2], new=out[3], reldiff=out[4])
+
+    def synchronize(self):
+        N.check(N.lib().admx_synchronize(self._h))
+
+    # ------------------------------------------------------------ host-array tile loops
+    def _qp(self, q, p):
+        return _f(q, self.T, (self.K, self.I)), _f(p, self.T, (self.K, self.J))
+
+    def loglikelihood_loop(self, q, p, i0, i1, j0, j1) -> float:
+        q, p = self._qp(q, p)
+        out = C.c_double()
+        N.check(N.lib().admx_loglikelihood_loop(self._h, N.ptr(q), N.ptr(p), i0, i1, j0, j1, C.byref(out)))
+        return out.value
+
+    def em_q_loop(self, q_next, q, p, i0, i1, j0, j1):
+        q, p = self._qp(q, p)
+        _f(q_next, self.T, (self.K, self.I))
+        N.check(N.lib().admx_em_q_loop(self._h, N.ptr(q_next), N.ptr(q), N.ptr(p), i0, i1, j0, j1))
+
+    def em_p_loop(self, p_next, p_tmp, q, p, i0, i1, j0, j1):
+        q, p = self._qp(q, p)
+        _f(p_next, self.T, (self.K, self.J))
+        _f(p_tmp, self.T, (self.K, self.J))
+        N.check(N.lib().admx_em_p_loop(self._h, N.ptr(p_next), N.ptr(p_tmp), N.ptr(q), N.ptr(p), i0, i1, j0, j1))
+
+    def update_q_loop(self, XtX, Xtz, q, p, i0, i1, j0, j1):
+        q, p = self._qp(q, p)
+        _f(XtX, self.T, (self.K, self.K, self.I))
+        _f(Xtz, self.T, (self.K, self.I))
+        N.check(N.lib().admx_update_q_loop(self._h, N.ptr(XtX), N.ptr(Xtz), N.ptr(q), N.ptr(p), i0, i1, j0, j1))
+
+    def update_p_loop(self, XtX, Xtz, q, p, i0, i1, j0, j1):
+        q, p = self._qp(q, p)
+        _f(XtX, self.T, (self.K, self.K, self.J))
+        _f(Xtz, self.T, (self.K, self.J))
+        N.check(N.lib().admx_update_p_loop(self._h, N.ptr(XtX), N.ptr(Xtz), N.ptr(q), N.ptr(p), i0, i1, j0, j1))
+
+    def qp_batch(self, simplex: bool, XtX, Xtz_neg, x):
+        n = x.shape[1]
+        _f(XtX, self.T, (self.K, self.K, n))
+        _f(Xtz_neg, self.T, (self.K, n))
+        _f(x, self.T, (self.K, n))
+        delta = np.zeros((self.K, n), self.T, order="F")
+        status = np.zeros(n, np.int32)
+        N.check(N.lib().admx_qp_batch(self._h, int(bool(simplex)), n, N.ptr(XtX), N.ptr(Xtz_neg), N.ptr(x),
+                                      N.ptr(delta), N.ptr(status)))
+        return delta, status
+
+    def project_q(self, b):
+        _f(b, self.T)
+        N.check(N.lib().admx_project_q(self._h, N.ptr(b), b.size // self.K))
+        return b
+
+    def project_p(self, b):
+        _f(b, self.T)
+        N.check(N.lib().admx_project_p(self._h, N.ptr(b), b.size))
+        return b
+
+    def zal_step(self, it, x, x_next, x_next2, U, V, x_tmp):
+        shp = (self.K, self.I + self.J)
+        args = [_f(a, self.T, shp) for a in (x, x_next, x_next2)]
+        ushp = (self.K * (self.I + self.J), self.Q)
+        _f(U, self.T, ushp)
+        _f(V, self.T, ushp)
+        _f(x_tmp, self.T, shp)
+        N.check(N.lib().admx_zal_step(self._h, int(it), *[N.ptr(a) for a in args], N.ptr(U), N.ptr(V),
+                                      N.ptr(x_tmp)))
+
+    # ------------------------------------------------------------ measurement / tuning
+    def profile(self, on: bool = True):
+        N.check(N.lib().admx_profile_enable(self._h, int(bool(on))))
+
+    def profile_reset(self):
+        N.check(N.lib().admx_profile_reset(self._h))
+
+    def profile_report(self) -> dict:
+        buf = C.create_string_buffer(1 << 16)
+        N.check(N.lib().admx_profile_report(self._h, buf, len(buf)))
+        rep = {}
+        for line in buf.value.decode().splitlines():
+            name, n, ms = line.split()
+            rep[name] = (int(n), float(ms))
+        return rep
+
+    def set_tuning(self, key: int, value: int):
+        N.check(N.lib().admx_set_tuning(self._h, key, int(value)))

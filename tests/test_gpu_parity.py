@@ -1,0 +1,328 @@
+"""GPU parity: every entry point of libadmixhip, called through the C ABI,
+against the CPU oracle on the same seeded inputs.
+
+Tolerances (written per test): decode / counts / generator bit-exact;
+log-likelihood 1e-10 relative in Float64 and 1e-5 in Float32; gradients and
+Hessians 1e-9 (F64) / 2e-4 (F32) relative to the column's largest entry;
+QP steps and projected iterates 1e-9 max-abs in Float64.
+"""
+import numpy as np
+import pytest
+
+from conftest import make_cohort, random_state
+
+pytestmark = pytest.mark.gpu
+
+F64, F32 = np.float64, np.float32
+
+
+def note(*a):
+    print(*a, flush=True)
+
+
+def ctx_for(oa, bed, pitch, I, J, K, Q=3, T=F64):
+    c = oa.HIPAdmixData(I, J, K, Q, T)
+    c.upload_bed(np.asarray(bed).reshape((pitch, J), order="F"))
+    return c
+
+
+# ------------------------------------------------------------------ genotypes
+@pytest.mark.parametrize("I,J", [(1, 1), (13, 7), (1000, 300), (4099, 257)])
+def test_decode_counts_bit_exact(oa, O, I, J):
+    note("decode/counts", I, J)
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, 3, miss=0.05, seed=I + J)
+    c = ctx_for(oa, bed, pitch, I, J, 3)
+    g = c.decode()
+    assert np.array_equal(g, O.decode(bed, pitch, I, J))
+    snm, salt, inm = c.counts()
+    rs, ra, ri = O.counts(bed, pitch, I, J)
+    assert np.array_equal(snm, rs) and np.array_equal(salt, ra) and np.array_equal(inm, ri)
+    # round trip of the payload through HBM (pitch-padded on device)
+    assert np.array_equal(c.download_bed().reshape(-1, order="F"), bed)
+    c.close()
+
+
+def test_decode_known_bytes(oa):
+    # 0xE4 = 11 10 01 00 (low bits first): individuals 0..3 -> 0, missing, 1, 2
+    I, J = 4, 2
+    bed = np.array([[0xE4, 0x1B]], dtype=np.uint8).reshape((1, 2), order="F")
+    c = oa.HIPAdmixData(I, J, 2)
+    c.upload_bed(bed)
+    g = c.decode()
+    assert g[:, 0].tolist() == [0, -1, 1, 2]
+    assert g[:, 1].tolist() == [2, 1, -1, 0]
+    snm, salt, inm = c.counts()
+    assert snm.tolist() == [3, 3] and salt.tolist() == [3, 3] and inm.tolist() == [2, 1, 1, 2]
+    c.close()
+
+
+@pytest.mark.parametrize("I,J,K", [(999, 500, 3), (64, 64, 8)])
+def test_generator_matches_oracle(oa, O, I, J, K):
+    note("generator", I, J, K)
+    rng = np.random.default_rng(4)
+    Qt = np.asfortranarray(rng.dirichlet(0.5 * np.ones(K), size=I).T)
+    Pt = np.asfortranarray(rng.uniform(0.05, 0.95, size=(K, J)))
+    c = oa.HIPAdmixData(I, J, K)
+    c.generate_bed(Qt, Pt, 77, 0.02)
+    ref = O.gen_bed(Qt, Pt, 77, 0.02)
+    assert np.array_equal(c.download_bed().reshape(-1, order="F"), ref)
+    c.close()
+
+
+# ------------------------------------------------------------------ tile loops
+CASES = [(700, 3000, 4, 0), (333, 1025, 3, 1), (257, 515, 8, 2), (100, 200, 1, 3), (130, 300, 12, 4), (40, 90, 16, 5)]
+
+
+@pytest.mark.parametrize("T", [F64, F32])
+@pytest.mark.parametrize("I,J,K,seed", CASES)
+def test_loglikelihood_loop(oa, O, I, J, K, seed, T):
+    note("ll loop", I, J, K, T)
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, seed=seed)
+    q, p = random_state(I, J, K, seed, T)
+    c = ctx_for(oa, bed, pitch, I, J, K, T=T)
+    rtol = 1e-10 if T == F64 else 1e-5
+    for (i0, i1, j0, j1) in [(0, I, 0, J), (3, I - 5, 7, J - 2), (I // 2, I // 2 + 1, 0, J)]:
+        got = c.loglikelihood_loop(q, p, i0, i1, j0, j1)
+        ref = O.loglikelihood(bed, pitch, q, p, (i0, i1), (j0, j1))
+        assert abs(got - ref) <= rtol * abs(ref), (i0, i1, j0, j1, got, ref)
+    assert c.loglikelihood_loop(q, p, 5, 5, 0, J) == 0.0
+    c.close()
+
+
+def _colmax_close(got, ref, rtol):
+    scale = np.abs(ref).reshape(-1, ref.shape[-1]).max(axis=0) + 1e-300
+    err = np.abs(got - ref).reshape(-1, ref.shape[-1]).max(axis=0)
+    return np.all(err <= rtol * scale), (err / scale).max()
+
+
+@pytest.mark.parametrize("T", [F64, F32])
+@pytest.mark.parametrize("I,J,K,seed", CASES)
+def test_update_loops(oa, O, I, J, K, seed, T):
+    note("update loops", I, J, K, T)
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, seed=seed)
+    q, p = random_state(I, J, K, seed, T)
+    c = ctx_for(oa, bed, pitch, I, J, K, T=T)
+    rtol = 1e-9 if T == F64 else 2e-4
+    rng = np.random.default_rng(seed)
+    i0, i1, j0, j1 = 2, I - 1, 5, J - 3
+    # Q side: accumulate into non-zero outputs over a sub-range
+    XtX = np.asfortranarray(rng.random((K, K, I)).astype(T))
+    Xtz = np.asfortranarray(rng.random((K, I)).astype(T))
+    gX, gZ = XtX.copy(order="F"), Xtz.copy(order="F")
+    c.update_q_loop(gX, gZ, q, p, i0, i1, j0, j1)
+    rX, rZ = XtX.copy(order="F"), Xtz.copy(order="F")
+    O.update_q_loop(rX, rZ, bed, pitch, q, p, (i0, i1), (j0, j1))
+    ok, e = _colmax_close(gX, rX, rtol)
+    assert ok, e
+    ok, e = _colmax_close(gZ, rZ, rtol)
+    assert ok, e
+    # P side
+    XtX = np.asfortranarray(rng.random((K, K, J)).astype(T))
+    Xtz = np.asfortranarray(rng.random((K, J)).astype(T))
+    gX, gZ = XtX.copy(order="F"), Xtz.copy(order="F")
+    c.update_p_loop(gX, gZ, q, p, i0, i1, j0, j1)
+    rX, rZ = XtX.copy(order="F"), Xtz.copy(order="F")
+    O.update_p_loop(rX, rZ, bed, pitch, q, p, (i0, i1), (j0, j1))
+    ok, e = _colmax_close(gX, rX, rtol)
+    assert ok, e
+    ok, e = _colmax_close(gZ, rZ, rtol)
+    assert ok, e
+    c.close()
+
+
+@pytest.mark.parametrize("T", [F64, F32])
+@pytest.mark.parametrize("I,J,K,seed", CASES[:3])
+def test_em_loops(oa, O, I, J, K, seed, T):
+    note("em loops", I, J, K, T)
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, seed=seed)
+    q, p = random_state(I, J, K, seed, T)
+    c = ctx_for(oa, bed, pitch, I, J, K, T=T)
+    rtol = 1e-10 if T == F64 else 1e-4
+    i0, i1, j0, j1 = 1, I - 2, 3, J - 1
+    gq = np.zeros((K, I), T, order="F")
+    rq = np.zeros((K, I), T, order="F")
+    c.em_q_loop(gq, q, p, i0, i1, j0, j1)
+    O.em_q_loop(rq, bed, pitch, q, p, (i0, i1), (j0, j1))
+    ok, e = _colmax_close(gq, rq, rtol)
+    assert ok, e
+    gpn, gpt = np.zeros((K, J), T, order="F"), np.zeros((K, J), T, order="F")
+    rpn, rpt = np.zeros((K, J), T, order="F"), np.zeros((K, J), T, order="F")
+    c.em_p_loop(gpn, gpt, q, p, i0, i1, j0, j1)
+    O.em_p_loop(rpn, rpt, bed, pitch, q, p, (i0, i1), (j0, j1))
+    for a, b in ((gpn, rpn), (gpt, rpt)):
+        ok, e = _colmax_close(a, b, rtol)
+        assert ok, e
+    c.close()
+
+
+# ------------------------------------------------------------------ QP / projections / QN
+@pytest.mark.parametrize("K", [2, 3, 5, 8, 12, 16])
+@pytest.mark.parametrize("simplex", [True, False])
+def test_qp_batch_matches_oracle(oa, O, K, simplex):
+    note("qp batch", K, simplex)
+    I, J = 300, 400
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, seed=K)
+    q, p = random_state(I, J, K, K)
+    # realistic tableaux: the Hessians of an update pass
+    if simplex:
+        XtX = np.zeros((K, K, I), order="F")
+        Xtz = np.zeros((K, I), order="F")
+        O.update_q_loop(XtX, Xtz, bed, pitch, q, p)
+        x = q
+    else:
+        XtX = np.zeros((K, K, J), order="F")
+        Xtz = np.zeros((K, J), order="F")
+        O.update_p_loop(XtX, Xtz, bed, pitch, q, p)
+        x = p
+    Xtz = np.asfortranarray(-Xtz)
+    c = oa.HIPAdmixData(I, J, K)
+    delta, status = c.qp_batch(simplex, XtX, Xtz, x)
+    n = x.shape[1]
+    for col in range(n):
+        tab = O.create_tableau(np.asfortranarray(XtX[:, :, col]), Xtz[:, col].copy(), x[:, col].copy(), simplex)
+        rd, rs = O.quadratic_program(tab, x[:, col].copy(), np.zeros(K), np.ones(K), 1 if simplex else 0)
+        assert status[col] == rs, (col, status[col], rs)
+        assert np.abs(delta[:, col] - rd).max() <= 1e-9, (col, delta[:, col], rd)
+    c.close()
+
+
+@pytest.mark.parametrize("K", [1, 2, 3, 7, 16])
+def test_projections(oa, O, K):
+    note("projections", K)
+    rng = np.random.default_rng(K)
+    n = 2000
+    b = np.asfortranarray(rng.normal(0.3, 0.6, size=(K, n)))
+    c = oa.HIPAdmixData(4, 4, K)
+    got = c.project_q(b.copy(order="F"))
+    ref = O.project_q(b.copy(order="F"))
+    assert np.array_equal(got, ref)
+    v = rng.normal(0.5, 0.7, size=5000)
+    assert np.array_equal(c.project_p(v.copy()), O.project_p(v.copy()))
+    c.close()
+
+
+@pytest.mark.parametrize("it,Q", [(1, 3), (2, 3), (3, 3), (7, 3), (5, 1), (4, 8)])
+def test_zal_step(oa, O, it, Q):
+    note("zal", it, Q)
+    I, J, K = 300, 700, 4
+    rng = np.random.default_rng(it * 10 + Q)
+    x = np.asfortranarray(rng.random((K, I + J)))
+    xn = np.asfortranarray(x + 0.01 * rng.normal(size=x.shape))
+    xn2 = np.asfortranarray(xn + 0.008 * rng.normal(size=x.shape))
+    U0 = np.asfortranarray(0.01 * rng.normal(size=(K * (I + J), Q)))
+    V0 = np.asfortranarray(0.01 * rng.normal(size=(K * (I + J), Q)))
+    c = oa.HIPAdmixData(I, J, K, Q)
+    gU, gV = U0.copy(order="F"), V0.copy(order="F")
+    gx = np.zeros_like(x, order="F")
+    c.zal_step(it, x, xn, xn2, gU, gV, gx)
+    rU, rV = U0.copy(order="F"), V0.copy(order="F")
+    O.update_UV(rU, rV, x, xn, xn2, it, Q)
+    rx = np.zeros_like(x, order="F")
+    O.update_QN(rx, xn, x, rU, rV, min(it, Q))
+    O.project_p(rx[:, I:])
+    rxq = np.asfortranarray(rx[:, :I])
+    O.project_q(rxq)
+    rx[:, :I] = rxq
+    assert np.array_equal(gU, rU) and np.array_equal(gV, rV)
+    assert np.abs(gx - rx).max() < 1e-9
+    c.close()
+
+
+# ------------------------------------------------------------------ algorithms on device state
+def _pair(oa, O, I, J, K, Q=3, T=F64, seed=0, miss=0.01, beta=False):
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, seed=seed, miss=miss, beta=beta)
+    d = oa.AdmixData(I, J, K, Q, rng=np.random.default_rng(seed + 100), T=T)
+    c = ctx_for(oa, bed, pitch, I, J, K, Q, T)
+    ref = O.OracleAdmix(bed, pitch, I, J, K, Q, d.x, d.U, d.V, dtype=T)
+    return d, c, ref
+
+
+def test_update_q_p_device(oa, O):
+    note("update_q!/update_p!")
+    I, J, K = 600, 2000, 4
+    d, c, ref = _pair(oa, O, I, J, K)
+    ref.init_em(2)
+    d.x[...] = ref.x
+    s_q = oa.update_q(d, None, False, d_cu=c)
+    r_q = ref.update_q(False)
+    assert np.array_equal(s_q, r_q)
+    assert np.abs(d.q_next - ref.x_next[:, :I]).max() < 1e-9
+    ok, e = _colmax_close(d.XtX_q, ref.XtX_q, 1e-9)
+    assert ok, e
+    ok, e = _colmax_close(d.Xtz_q, ref.Xtz_q, 1e-9)
+    assert ok, e
+    d.x_next[:, :I] = ref.x_next[:, :I]
+    s_p = oa.update_p(d, None, False, d_cu=c)
+    r_p = ref.update_p(False)
+    assert np.array_equal(s_p, r_p)
+    assert np.abs(d.p_next - ref.x_next[:, I:]).max() < 1e-9
+    c.close()
+
+
+@pytest.mark.parametrize("I,J,K", [(500, 3000, 3), (1000, 2000, 5)])
+def test_em_and_qn_iterations_f64(oa, O, I, J, K):
+    note("qn iterations f64", I, J, K)
+    d, c, ref = _pair(oa, O, I, J, K, seed=I)
+    lls = oa.init_em(d, None, 5, d_cu=c, log=None)
+    rlls = ref.init_em(5)
+    np.testing.assert_allclose(lls, rlls, rtol=1e-10, atol=0)
+    assert np.abs(d.x - ref.x).max() < 1e-10
+    d_ll = []
+    for it in range(1, 9):
+        r = c.qn_iteration(it)
+        rr = ref.qn_iteration(it)
+        d_ll.append(r["new"])
+        for key in ("basic", "qn", "new"):
+            assert abs(r[key] - rr[key]) <= 1e-10 * abs(rr[key]), (it, key, r[key], rr[key])
+    x = c.get_x(0)
+    assert np.abs(x - ref.x).max() < 1e-6
+    assert all(b >= a - 1e-9 * abs(a) for a, b in zip(d_ll, d_ll[1:]))
+    c.close()
+
+
+def test_qn_iterations_f32(oa, O):
+    note("qn iterations f32")
+    I, J, K = 800, 3000, 3
+    d, c, ref = _pair(oa, O, I, J, K, T=F32, seed=9)
+    lls = oa.init_em(d, None, 3, d_cu=c, log=None)
+    rlls = ref.init_em(3)
+    np.testing.assert_allclose(lls, rlls, rtol=1e-5)
+    for it in range(1, 4):
+        r = c.qn_iteration(it)
+        rr = ref.qn_iteration(it)
+        assert abs(r["new"] - rr["new"]) <= 1e-5 * abs(rr["new"]), (it, r, rr)
+    c.close()
+
+
+def test_fix_q_fix_p(oa, O):
+    note("fix_q/fix_p")
+    I, J, K = 300, 900, 3
+    d, c, ref = _pair(oa, O, I, J, K, seed=3)
+    oa.init_em(d, None, 2, d_cu=c, log=None)
+    ref.init_em(2)
+    q0 = c.get_x(0)[:, :I].copy()
+    r = c.qn_iteration(1, fix_q=True)
+    rr = ref.qn_iteration(1, fix_q=True)
+    assert abs(r["new"] - rr["new"]) <= 1e-10 * abs(rr["new"])
+    if r["qn"] <= r["prev"]:
+        assert np.array_equal(c.get_x(0)[:, :I], q0)
+    r = c.qn_iteration(2, fix_p=True)
+    rr = ref.qn_iteration(2, fix_p=True)
+    assert abs(r["new"] - rr["new"]) <= 1e-10 * abs(rr["new"])
+    c.close()
+
+
+def test_reproducible_bitwise(oa, O):
+    note("reproducibility")
+    I, J, K = 700, 1500, 4
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, seed=21)
+    outs = []
+    for _ in range(2):
+        d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(5))
+        c = ctx_for(oa, bed, pitch, I, J, K)
+        oa.init_em(d, None, 2, d_cu=c, log=None)
+        r = [c.qn_iteration(it)["new"] for it in (1, 2, 3)]
+        outs.append((r, c.get_x(0)))
+        c.close()
+    assert outs[0][0] == outs[1][0]
+    assert np.array_equal(outs[0][1], outs[1][1])
