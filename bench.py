@@ -97,6 +97,24 @@ def cpu_baseline(wl, dtype, seed, budget_s=20.0):
     }
 
 
+def calibrate_copy(nbytes=1 << 30):
+    """A plain device-to-device copy of known size (HIP runtime copy kernel), used
+    to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
+    import ctypes as C
+
+    hip = C.CDLL("libamdhip64.so")
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), C.c_size_t(nbytes)) == 0
+    assert hip.hipMalloc(C.byref(b), C.c_size_t(nbytes)) == 0
+    assert hip.hipMemset(a, 1, C.c_size_t(nbytes)) == 0
+    for _ in range(3):
+        assert hip.hipMemcpy(b, a, C.c_size_t(nbytes), 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 3}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -107,7 +125,12 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (counter scaling)")
+    ap.add_argument("--target-waves", type=int, default=0, help="waves per genotype pass (0: library default)")
     args = ap.parse_args()
+    if args.calibrate_copy:
+        calibrate_copy()
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
               "is a design note, not a measured path", file=sys.stderr)
@@ -124,6 +147,8 @@ def main():
     T = np.float64 if args.dtype == "f64" else np.float32
 
     ctx = oa.HIPAdmixData(I, J, K, 3, T)
+    if args.target_waves:
+        ctx.set_tuning(oa._native.ADMX_TUNE_TARGET_WAVES, args.target_waves)
     t0 = time.perf_counter()
     sim.cohort_on_device(ctx, wl["alpha"], wl["p_dist"], wl["p_lo"], wl["p_hi"], wl["miss_rate"], seed=args.seed)
     t_gen = time.perf_counter() - t0

@@ -91,7 +91,7 @@ struct admx_ctx {
   int32_t* status = nullptr;
   DevBuf part, llpart, zal;
   DevBuf st[8];  // staging for host-array entry points
-  int64_t target_waves = 8192;
+  int64_t target_waves = 65536;
   int keep_hess = 1;
   std::mutex mu;
   Prof prof;

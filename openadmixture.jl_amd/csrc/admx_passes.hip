@@ -61,12 +61,26 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
 #pragma unroll
   for (int k = 0; k < ((MODE == MODE_UPDATE && TRICK) ? K : 1); ++k) hb[k] = 0;
 
+  // Software pipeline: the genotype byte and the P column of SNP j+1 are in
+  // flight while SNP j is computed.
+  uint32_t byte_n = 0;
+  T pn[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) pn[k] = 0;
+  if (jb < je) {
+    byte_n = gp[a.pitch * jb];
+#pragma unroll
+    for (int k = 0; k < K; ++k) pn[k] = p[(int64_t)K * jb + k];
+  }
   for (int64_t j = jb; j < je; ++j) {
-    const uint32_t c = ((uint32_t)gp[a.pitch * j] >> sh) & 3u;
-    const T* __restrict__ pj = p + (int64_t)K * j;
+    const uint32_t c = (byte_n >> sh) & 3u;
     T pv[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) pv[k] = pj[k];
+    for (int k = 0; k < K; ++k) pv[k] = pn[k];
+    const int64_t jn = (j + 1 < je) ? j + 1 : j;
+    byte_n = gp[a.pitch * jn];
+#pragma unroll
+    for (int k = 0; k < K; ++k) pn[k] = p[(int64_t)K * jn + k];
     T s = qi[0] * pv[0];
 #pragma unroll
     for (int k = 1; k < K; ++k) s = __builtin_fma(qi[k], pv[k], s);
@@ -182,8 +196,43 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
 #pragma unroll
   for (int k = 0; k < (MODE == MODE_EM ? K : 1); ++k) e2[k] = 0;
 
+  // Per-genotype work; qi is wave-uniform (scalar loads).
+  auto body = [&](uint32_t c, const T* __restrict__ qi) {
+    T qv[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) qv[k] = qi[k];
+    T s = qv[0] * pj[0];
+#pragma unroll
+    for (int k = 1; k < K; ++k) s = __builtin_fma(qv[k], pj[k], s);
+    const T tt = (T)1 - s;
+    const T gm = (T)code_g(c), hm = (T)code_h(c);
+    const T r = rcp(s * tt);
+    const T is = tt * r, it = s * r;
+    const T cc = gm * is, dd = hm * it;
+    if constexpr (MODE == MODE_EM) {
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        z[k] = __builtin_fma(cc, qv[k], z[k]);
+        e2[k] = __builtin_fma(dd, qv[k], e2[k]);
+      }
+    } else {
+      const T w2 = cc * is + dd * it, u = cc - dd;
+      int n = 0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        const T wq = w2 * qv[k];
+        z[k] = __builtin_fma(u, qv[k], z[k]);
+#pragma unroll
+        for (int k2 = k; k2 < K; ++k2, ++n) H[n] = __builtin_fma(wq, qv[k2], H[n]);
+      }
+    }
+  };
+
+  // 16 genotypes of the lane's SNP column per 32-bit load, next word prefetched.
+  uint32_t w_n = (ia0 < ia1) ? *reinterpret_cast<const uint32_t*>(col + (ia0 >> 2)) : 0u;
   for (int64_t ib = ia0; ib < ia1; ib += 16) {
-    uint32_t w = *reinterpret_cast<const uint32_t*>(col + (ib >> 2));
+    uint32_t w = w_n;
+    if (ib + 16 < ia1) w_n = *reinterpret_cast<const uint32_t*>(col + ((ib + 16) >> 2));
     if (ib < a.i0 || ib + 16 > a.i1) {  // uniform: mark out-of-range individuals missing
       uint32_t keep = 0;
 #pragma unroll
@@ -191,38 +240,15 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
         if (ib + t >= a.i0 && ib + t < a.i1) keep |= 3u << (2 * t);
       w = (w & keep) | (0x55555555u & ~keep);
     }
+    if (ib + 16 <= a.I) {  // interior word: constant offsets from one base pointer
+      const T* __restrict__ qb = q + (int64_t)K * ib;
 #pragma unroll
-    for (int t = 0; t < 16; ++t) {
-      const int64_t i = (ib + t < a.I) ? ib + t : a.I - 1;
-      const uint32_t c = (w >> (2 * t)) & 3u;
-      const T* __restrict__ qi = q + (int64_t)K * i;
-      T qv[K];
+      for (int t = 0; t < 16; ++t) body((w >> (2 * t)) & 3u, qb + K * t);
+    } else {  // last word of the cohort: clamp the (masked) individuals past I
 #pragma unroll
-      for (int k = 0; k < K; ++k) qv[k] = qi[k];
-      T s = qv[0] * pj[0];
-#pragma unroll
-      for (int k = 1; k < K; ++k) s = __builtin_fma(qv[k], pj[k], s);
-      const T tt = (T)1 - s;
-      const T gm = (T)code_g(c), hm = (T)code_h(c);
-      const T r = rcp(s * tt);
-      const T is = tt * r, it = s * r;
-      const T cc = gm * is, dd = hm * it;
-      if constexpr (MODE == MODE_EM) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          z[k] = __builtin_fma(cc, qv[k], z[k]);
-          e2[k] = __builtin_fma(dd, qv[k], e2[k]);
-        }
-      } else {
-        const T w2 = cc * is + dd * it, u = cc - dd;
-        int n = 0;
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-          const T wq = w2 * qv[k];
-          z[k] = __builtin_fma(u, qv[k], z[k]);
-#pragma unroll
-          for (int k2 = k; k2 < K; ++k2, ++n) H[n] = __builtin_fma(wq, qv[k2], H[n]);
-        }
+      for (int t = 0; t < 16; ++t) {
+        const int64_t i = (ib + t < a.I) ? ib + t : a.I - 1;
+        body((w >> (2 * t)) & 3u, q + (int64_t)K * i);
       }
     }
   }

@@ -90,6 +90,7 @@ def test_loglikelihood_loop(oa, O, I, J, K, seed, T):
 
 
 def _colmax_close(got, ref, rtol):
+    got, ref = np.asarray(got, np.float64), np.asarray(ref, np.float64)
     scale = np.abs(ref).reshape(-1, ref.shape[-1]).max(axis=0) + 1e-300
     err = np.abs(got - ref).reshape(-1, ref.shape[-1]).max(axis=0)
     return np.all(err <= rtol * scale), (err / scale).max()
