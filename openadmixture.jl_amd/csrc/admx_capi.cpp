@@ -175,16 +175,16 @@ int need_bed(admx_ctx* c) {
 }
 
 // ---- chunking of the genotype passes
-int qside_chunks(admx_ctx* c, int64_t Ilen, int64_t Jlen, int64_t* chunk) {
-  const int64_t waves = (Ilen + 63) / 64;
+int qside_chunks(admx_ctx* c, int64_t Ilen, int64_t Jlen, int64_t* chunk, int rows_per_wave = 64) {
+  const int64_t waves = (Ilen + rows_per_wave - 1) / rows_per_wave;
   int64_t n = (c->target_waves + waves - 1) / std::max<int64_t>(waves, 1);
   n = std::max<int64_t>(1, std::min<int64_t>(n, std::max<int64_t>(1, Jlen / 64)));
   *chunk = std::max<int64_t>(1, (Jlen + n - 1) / n);
   return (int)std::max<int64_t>(1, (Jlen + *chunk - 1) / *chunk);
 }
 
-int pside_chunks(admx_ctx* c, int64_t i0, int64_t i1, int64_t Jlen, int64_t* chunk) {
-  const int64_t waves = (Jlen + 63) / 64;
+int pside_chunks(admx_ctx* c, int64_t i0, int64_t i1, int64_t Jlen, int64_t* chunk, int rows_per_wave = 64) {
+  const int64_t waves = (Jlen + rows_per_wave - 1) / rows_per_wave;
   const int64_t span = i1 - (i0 & ~(int64_t)15);
   int64_t n = (c->target_waves + waves - 1) / std::max<int64_t>(waves, 1);
   n = std::max<int64_t>(1, std::min<int64_t>(n, std::max<int64_t>(1, span / 64)));
@@ -1016,6 +1016,7 @@ int admx_set_tuning(admx_ctx* c, int key, int64_t value) {
     c->target_waves = value;
   } else if (key == ADMX_TUNE_KEEP_HESSIANS) {
     c->keep_hess = value != 0;
+
   } else {
     return bad("unknown tuning key");
   }

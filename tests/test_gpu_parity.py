@@ -327,3 +327,4 @@ def test_reproducible_bitwise(oa, O):
         c.close()
     assert outs[0][0] == outs[1][0]
     assert np.array_equal(outs[0][1], outs[1][1])
+
