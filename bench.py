@@ -41,6 +41,28 @@ WORKLOADS = {
 }
 
 
+KERNEL_SYMBOLS = {  # bench kernel name -> rocprof symbol prefix of its FP64 K=8 instance
+    "qside_update": "void admx::k_qside<double, 8, 0, true>",
+    "pside_update": "void admx::k_pside<double, 8, 0>",
+    "qside_ll": "void admx::k_qside<double, 8, 2, false>",
+}
+
+
+def measured_traffic(kernel: str, workload: str, dtype: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
+    WRITE_SIZE summary of this workload (already scaled by the copy calibration);
+    None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{dtype}.txt")
+    sym = KERNEL_SYMBOLS.get(kernel)
+    if sym is None or not os.path.exists(path):
+        return None
+    for line in open(path):
+        parts = [x.strip() for x in line.split("|")]
+        if len(parts) == 5 and parts[0].startswith(sym):
+            return (float(parts[3]) + float(parts[4])) * 1e9
+    return None
+
+
 def host_cores() -> int:
     n = os.environ.get("OMP_NUM_THREADS")
     if n:
@@ -197,7 +219,7 @@ def main():
         "config": {"workload": f"{args.workload}: I={I}, J={J}, K={K}, Q=3, ZAL, plain {args.dtype}",
                    "I": I, "J": J, "K": K, "genotypes": I * J, "bed_bytes": ((I + 3) // 4) * J},
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": None,
+                     "frac": achieved / peak, "traffic": measured_traffic(dom, args.workload, args.dtype),
                      "note": f"{FLOPS[dom](K)} algorithmic flops/genotype x {I * J} genotypes per launch; "
                              f"{args.dtype} vector ALU (MI355X dense FP64 vector = matrix = 78.6 TF)"},
         "kernels": kernels,
