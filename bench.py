@@ -144,6 +144,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="c3", choices=sorted(WORKLOADS))
     ap.add_argument("--dtype", default="f64", choices=["f64", "f32"])
+    ap.add_argument("--form", default="plain", choices=["plain", "f32", "bf16"],
+                    help="update-pass precision of a FP64 run: plain FP64, FP32 vector ALU, or BF16 matrix cores")
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
@@ -171,6 +173,10 @@ def main():
     ctx = oa.HIPAdmixData(I, J, K, 3, T)
     if args.target_waves:
         ctx.set_tuning(oa._native.ADMX_TUNE_TARGET_WAVES, args.target_waves)
+    if args.form != "plain":
+        if args.dtype != "f64":
+            sys.exit("--form f32/bf16 runs on a FP64 context (FP64 log-likelihood, QP and convergence test)")
+        ctx.set_precision(args.form)
     t0 = time.perf_counter()
     sim.cohort_on_device(ctx, wl["alpha"], wl["p_dist"], wl["p_lo"], wl["p_hi"], wl["miss_rate"], seed=args.seed)
     t_gen = time.perf_counter() - t0
@@ -196,11 +202,14 @@ def main():
 
     value = I * J * args.steps / el
     # dominant kernel and its achieved algorithmic FLOP rate
-    dom = max((k for k in rep if k in FLOPS), key=lambda k: rep[k][1])
+    def base(k):  # qside_update_bf16 -> qside_update
+        return next((f for f in FLOPS if k.startswith(f)), None)
+
+    dom = max((k for k in rep if base(k)), key=lambda k: rep[k][1])
     n_launch, ms = rep[dom]
     per_launch_s = ms / n_launch / 1e3
-    achieved = FLOPS[dom](K) * I * J / per_launch_s / 1e12
-    peak = PEAK_TFLOPS[args.dtype]
+    achieved = FLOPS[base(dom)](K) * I * J / per_launch_s / 1e12
+    peak = PEAK_TFLOPS["f64" if (args.dtype == "f64" and args.form == "plain") else "f32"]
     kernels = {k: {"launches": n, "avg_ms": round(t / n, 4), "share": round(t / (el * 1e3), 4)}
                for k, (n, t) in sorted(rep.items(), key=lambda kv: -kv[1][1])}
     out = {
@@ -214,13 +223,13 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": args.dtype,
+        "dtype": args.dtype if args.form == "plain" else f"{args.form} passes, f64 LL/QP",
         "data": "synthetic cohort generated on device (Dirichlet Q, Beta/Uniform P, Binomial(2, q.p) genotypes)",
-        "config": {"workload": f"{args.workload}: I={I}, J={J}, K={K}, Q=3, ZAL, plain {args.dtype}",
+        "config": {"workload": f"{args.workload}: I={I}, J={J}, K={K}, Q=3, ZAL, {args.form} {args.dtype}",
                    "I": I, "J": J, "K": K, "genotypes": I * J, "bed_bytes": ((I + 3) // 4) * J},
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak, "traffic": measured_traffic(dom, args.workload, args.dtype),
-                     "note": f"{FLOPS[dom](K)} algorithmic flops/genotype x {I * J} genotypes per launch; "
+                     "note": f"{FLOPS[base(dom)](K)} algorithmic flops/genotype x {I * J} genotypes per launch; "
                              f"{args.dtype} vector ALU (MI355X dense FP64 vector = matrix = 78.6 TF)"},
         "kernels": kernels,
         "ll_last": lls[-1],

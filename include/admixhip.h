@@ -48,6 +48,13 @@ extern "C" {
 /* tuning keys for admx_set_tuning */
 #define ADMX_TUNE_TARGET_WAVES 0 /* waves per genotype pass (sets the chunk count) */
 #define ADMX_TUNE_KEEP_HESSIANS 1 /* 1: update_q/update_p also store XtX and Xtz */
+#define ADMX_TUNE_PRECISION 2     /* FP64 contexts: precision of the device-resident update passes */
+
+/* values of ADMX_TUNE_PRECISION.  Log-likelihood, QP, projections, quasi-Newton
+ * and the convergence test stay FP64 in every mode. */
+#define ADMX_PREC_F64 0  /* plain: everything FP64 */
+#define ADMX_PREC_F32 1  /* gradient + Hessian accumulated in FP32 (vector ALU) */
+#define ADMX_PREC_BF16 2 /* gradient in FP32, Hessian on BF16 matrix cores with FP32 accumulation */
 
 typedef struct admx_ctx admx_ctx;
 

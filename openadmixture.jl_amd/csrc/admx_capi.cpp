@@ -93,6 +93,8 @@ struct admx_ctx {
   DevBuf st[8];  // staging for host-array entry points
   int64_t target_waves = 65536;
   int keep_hess = 1;
+  int precision = ADMX_PREC_F64;
+  DevBuf fq, fp, ppt, mmt, qqt;  // mixed-precision pass inputs
   std::mutex mu;
   Prof prof;
 
@@ -258,6 +260,77 @@ int ll_to_slot(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1
   return check_launch();
 }
 
+// Mixed-precision update passes of a FP64 context: FP32 copies of (q, p), the
+// BF16 pair tables, the FP32/BF16 pass, then the FP64 finalize + QP + projection.
+int mixed_update(admx_ctx* c, bool qside, const double* q, const double* p, const double* x_in, double* x_out,
+                 int32_t* status_dev) {
+  const int64_t I = c->I, J = c->J, K = c->K;
+  CHK(c->fq.grow(sizeof(float) * K * I));
+  if (c->fp.n < sizeof(float) * 2 * K * (J + 17)) {
+    CHK(c->fp.grow(sizeof(float) * 2 * K * (J + 17)));
+    std::vector<float> half(2 * K * 17, 0.5f);  // padding SNP columns: s stays inside (0, 1)
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(static_cast<float*>(c->fp.p) + 2 * K * J, half.data(), sizeof(float) * half.size(),
+                     hipMemcpyHostToDevice));
+  }
+  float* fq = static_cast<float*>(c->fq.p);
+  float* fp = static_cast<float*>(c->fp.p);
+  const bool bf16 = c->precision == ADMX_PREC_BF16;
+  const int rows = pair_table_rows((int)K);
+  const int64_t ldj = (J + 15) & ~(int64_t)15, ldi = (I + 15) & ~(int64_t)15;
+  {
+    ProfScope ps(c, "mixed_prep");
+    launch_to_float(q, K * I, fq, c->s);
+    if (bf16 && qside) launch_to_float_pm(p, J, (int)K, fp, c->s);  // [p, 1-p] per SNP
+    else launch_to_float(p, K * J, fp, c->s);
+    if (bf16) {
+      if (qside) {
+        CHK(c->ppt.grow((size_t)rows * ldj * 2));
+        CHK(c->mmt.grow((size_t)rows * ldj * 2));
+        launch_pair_tables((int)K, p, J, ldj, c->ppt.p, c->mmt.p, c->s);
+      } else {
+        CHK(c->qqt.grow((size_t)rows * ldi * 2));
+        launch_pair_tables((int)K, q, I, ldi, c->qqt.p, nullptr, c->s);
+      }
+    }
+  }
+  CHK(check_launch());
+  const int na = (int)(K * (K + 1) / 2 + K);  // direct form: Hessian triangle + gradient
+  int64_t chunk = 0;
+  int n = 0;
+  if (qside) {
+    n = qside_chunks(c, I, J, &chunk);
+    chunk = (chunk + 15) & ~(int64_t)15;
+    n = (int)((J + chunk - 1) / chunk);
+    CHK(c->part.grow((size_t)n * na * I * sizeof(float)));
+  } else {
+    n = pside_chunks(c, 0, I, J, &chunk);
+    CHK(c->part.grow((size_t)n * na * J * sizeof(float)));
+  }
+  PassArgs a{c->bed, c->pitch, I, fq, fp, 0, I, 0, J, chunk, c->part.p};
+  {
+    ProfScope ps(c, qside ? (bf16 ? "qside_update_bf16" : "qside_update_f32")
+                          : (bf16 ? "pside_update_bf16" : "pside_update_f32"));
+    if (qside) {
+      if (bf16) launch_qside_bf16((int)K, a, n, fq, fp, c->ppt.p, c->mmt.p, ldj, c->s);
+      else launch_qside<float>(MODE_UPDATE, (int)K, a, n, c->s);
+    } else {
+      if (bf16) launch_pside_bf16((int)K, a, n, fq, fp, c->qqt.p, ldi, c->s);
+      else launch_pside<float>(MODE_UPDATE, (int)K, a, n, c->s);
+    }
+  }
+  CHK(check_launch());
+  FinArgs f{c->part.p, qside ? I : J, n, 0, J, x_in, x_out, nullptr,
+            c->keep_hess ? (qside ? c->XtX_q : c->XtX_p) : nullptr, c->keep_hess ? (qside ? c->Xtz_q : c->Xtz_p) : nullptr,
+            0, 1, status_dev};
+  {
+    ProfScope ps(c, qside ? "fin_q_update_qp" : "fin_p_update_qp");
+    if (qside) launch_fin_q_update_mixed((int)K, f, c->s);
+    else launch_fin_p_update_mixed((int)K, f, c->s);
+  }
+  return check_launch();
+}
+
 template <typename T>
 int dev_update_q(admx_ctx* c, bool update2, int32_t* status_host) {
   void* X = c->x[ADMX_X];
@@ -266,6 +339,15 @@ int dev_update_q(admx_ctx* c, bool update2, int32_t* status_host) {
   void* q_next = update2 ? XN2 : XN;
   void* q = update2 ? XN : X;
   void* p = c->pp(update2 ? XN : X);
+  if (sizeof(T) == 8 && c->precision != ADMX_PREC_F64) {
+    CHK(mixed_update(c, true, (const double*)q, (const double*)p, (const double*)q, (double*)q_next,
+                     status_host ? c->status : nullptr));
+    if (status_host) {
+      HIPCHK(hipMemcpyAsync(status_host, c->status, sizeof(int32_t) * c->I, hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+    }
+    return ADMX_OK;
+  }
   int nchunk = 0;
   int64_t np = 0;
   CHK(pass_q<T>(c, MODE_UPDATE, q, p, 0, c->I, 0, c->J, &nchunk, &np));
@@ -292,6 +374,15 @@ int dev_update_p(admx_ctx* c, bool update2, int32_t* status_host) {
   void* p_next = c->pp(update2 ? XN2 : XN);
   void* q = update2 ? XN2 : XN;
   void* p = c->pp(update2 ? XN : X);
+  if (sizeof(T) == 8 && c->precision != ADMX_PREC_F64) {
+    CHK(mixed_update(c, false, (const double*)q, (const double*)p, (const double*)p, (double*)p_next,
+                     status_host ? c->status : nullptr));
+    if (status_host) {
+      HIPCHK(hipMemcpyAsync(status_host, c->status, sizeof(int32_t) * c->J, hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipStreamSynchronize(c->s));
+    }
+    return ADMX_OK;
+  }
   int nchunk = 0;
   CHK(pass_p<T>(c, MODE_UPDATE, q, p, 0, c->I, 0, c->J, &nchunk));
   FinArgs f{c->part.p, c->J, nchunk, 0, c->J, p, p_next, nullptr,
@@ -627,7 +718,7 @@ int admx_create(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int 
     return cleanup(fail(ADMX_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
   const size_t xb = (size_t)c->N() * c->esz;
   auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, bytes < 256 ? 256 : bytes); };
-  if (alloc((void**)&c->bed, (size_t)c->pitch * J) != hipSuccess || alloc(&c->x[0], xb) != hipSuccess ||
+  if (alloc((void**)&c->bed, (size_t)c->pitch * (J + 16)) != hipSuccess || alloc(&c->x[0], xb) != hipSuccess ||
       alloc(&c->x[1], xb) != hipSuccess || alloc(&c->x[2], xb) != hipSuccess || alloc(&c->x[3], xb) != hipSuccess ||
       alloc(&c->XtX_q, (size_t)(K * K * I) * c->esz) != hipSuccess ||
       alloc(&c->Xtz_q, (size_t)(K * I) * c->esz) != hipSuccess ||
@@ -664,6 +755,11 @@ int admx_destroy(admx_ctx* c) {
   c->llpart.release();
   c->zal.release();
   for (auto& b : c->st) b.release();
+  c->fq.release();
+  c->fp.release();
+  c->ppt.release();
+  c->mmt.release();
+  c->qqt.release();
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
   return ADMX_OK;
@@ -674,7 +770,7 @@ int admx_upload_bed(admx_ctx* c, const uint8_t* bed, int64_t bytes_per_snp) {
   Guard g(c);
   if (!bed) return bad("null BED pointer");
   if (bytes_per_snp < c->ibytes) return bad("bytes_per_snp < ceil(I/4)");
-  HIPCHK(hipMemsetAsync(c->bed, 0, (size_t)c->pitch * c->J, c->s));
+  HIPCHK(hipMemsetAsync(c->bed, 0, (size_t)c->pitch * (c->J + 16), c->s));
   HIPCHK(hipMemcpy2DAsync(c->bed, c->pitch, bed, bytes_per_snp, c->ibytes, c->J, hipMemcpyHostToDevice, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   c->have_bed = true;
@@ -1016,6 +1112,10 @@ int admx_set_tuning(admx_ctx* c, int key, int64_t value) {
     c->target_waves = value;
   } else if (key == ADMX_TUNE_KEEP_HESSIANS) {
     c->keep_hess = value != 0;
+  } else if (key == ADMX_TUNE_PRECISION) {
+    if (value < ADMX_PREC_F64 || value > ADMX_PREC_BF16) return bad("precision must be ADMX_PREC_F64/F32/BF16");
+    if (value != ADMX_PREC_F64 && c->dtype != ADMX_F64) return bad("mixed precision needs a FP64 context");
+    c->precision = (int)value;
 
   } else {
     return bad("unknown tuning key");

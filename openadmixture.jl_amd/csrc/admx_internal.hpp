@@ -61,6 +61,20 @@ template <typename T> void launch_fin_q_em(int K, const FinArgs& f, hipStream_t 
 template <typename T> void launch_fin_p_update(int K, const FinArgs& f, bool do_qp, hipStream_t s);
 template <typename T> void launch_fin_p_em(int K, const FinArgs& f, hipStream_t s);
 
+// Mixed precision: FP64 finalize (+QP) of FP32 partials in the direct Hessian form.
+void launch_fin_q_update_mixed(int K, const FinArgs& f, hipStream_t s);
+void launch_fin_p_update_mixed(int K, const FinArgs& f, hipStream_t s);
+// Mixed precision passes (FP32 weights/gradient, BF16 MFMA Hessian) and their inputs.
+void launch_to_float(const double* x, int64_t n, float* y, hipStream_t s);
+// [p, 1-p] per column in FP32 (2K floats), the Q-side BF16 pass operand.
+void launch_to_float_pm(const double* x, int64_t ncol, int K, float* y, hipStream_t s);
+void launch_pair_tables(int K, const double* x, int64_t n, int64_t ld, void* PP, void* MM, hipStream_t s);
+int pair_table_rows(int K);
+void launch_qside_bf16(int K, const PassArgs& a, int nchunk, const float* fq, const float* fp, const void* PPt,
+                       const void* MMt, int64_t ld, hipStream_t s);
+void launch_pside_bf16(int K, const PassArgs& a, int nchunk, const float* fq, const float* fp, const void* QQt,
+                       int64_t ld, hipStream_t s);
+
 // Batched QP on host-provided tableau inputs (XtX, negated Xtz, x): delta out.
 template <typename T> void launch_qp_batch(int K, bool simplex, int64_t n, const void* XtX, const void* Xtz_neg,
                                            const void* x, void* delta, int32_t* status, hipStream_t s);

@@ -1,0 +1,383 @@
+// admx_mixed.hip -- mixed-precision gradient + Hessian passes: per-genotype
+// weights and gradient in FP32 on the VALU, the K x K Hessian on the matrix
+// cores in BF16 (v_mfma_f32_32x32x16_bf16) accumulating in FP32.  Everything
+// downstream (chunk sums, QP, projections, log-likelihood, quasi-Newton,
+// convergence test) stays FP64.
+//
+// Q side (update_q!, src/loops.jl:867-894), per individual i:
+//   XtX_q = sum_j a_ij pp_j + b_ij mm_j,  pp_j = p_j p_j', mm_j = (1-p_j)(1-p_j)'
+//   Xtz_q = sum_j c_ij p_j + d_ij (1-p_j)           (FP32 VALU)
+// the direct two-term form: every term is non-negative, so BF16 rounding
+// (2^-8 relative) cannot be amplified by cancellation.  One lane owns one
+// individual and streams SNPs with p_j wave-uniform (scalar loads), exactly as
+// the FP64 kernel; every 16 SNPs the lane's 16 weights a (and b) are packed to
+// BF16 and one v_permlane32_swap per register pair turns the 64 lanes' rows
+// into the A operands of two 32-row tiles:
+//   A[i][k] lives in lane i + 32*(k/8), element k%8;  B[k][n] in lane n + 32*(k/8);
+//   D[i][n] in lane n + 32*((i/4)%2), register (i%4) + 4*(i/8)   (checked on gfx950).
+// The reduction dimension stacks the two terms: k in [0,16) pairs weight a with
+// pp, k in [16,32) pairs b with mm, into the same accumulator.  B comes from
+// BF16 pair tables PPt/MMt [pair][SNP] built once per pass.
+// P side (update_p!, src/loops.jl:923-950): XtX_p = sum_i w_ij qq_i (w = a+b),
+// Xtz_p = sum_i (c-d) q_i; one lane per SNP, individuals streamed 16 at a time
+// (one 32-bit genotype word), q_i wave-uniform.
+#include <hip/hip_bf16.h>
+
+#include "admx_common.hpp"
+#include "admx_internal.hpp"
+
+namespace admx {
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x16 __attribute__((ext_vector_type(16)));
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <int K> struct Tiles32 {
+  static constexpr int NP = K * (K + 1) / 2;
+  static constexpr int NT = (NP + 31) / 32;  // 32-wide pair tiles
+  static constexpr int NW = NT * 32;
+};
+
+__device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
+  const __bf16 a = (__bf16)lo, b = (__bf16)hi;
+  return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+}
+
+// 8 consecutive BF16 of row `n` of a [NW][ld] table starting at column c (16-byte aligned).
+__device__ __forceinline__ bf16x8 load_b8(const __bf16* __restrict__ t, int64_t ld, int n, int64_t c) {
+  const u32x4 v = *reinterpret_cast<const u32x4*>(t + (int64_t)n * ld + c);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+// Swap the lanes' halves so that lo/hi (weights of SNPs 0-7 / 8-15 of the
+// lane's own row) become the A operands of row tiles 0 (lanes 0-31) and 1.
+__device__ __forceinline__ void to_tiles(const uint32_t (&lo)[4], const uint32_t (&hi)[4], bf16x8& t0, bf16x8& t1) {
+  u32x4 a, b;
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const auto r = __builtin_amdgcn_permlane32_swap(lo[e], hi[e], false, false);
+    a[e] = r[0];
+    b[e] = r[1];
+  }
+  t0 = __builtin_bit_cast(bf16x8, a);
+  t1 = __builtin_bit_cast(bf16x8, b);
+}
+
+// ------------------------------------------------------------------ pair tables
+// PP[n][c] = x_a x_b and (optionally) MM[n][c] = (1-x_a)(1-x_b) for the
+// upper-triangle pair n = (a <= b) of column c; zero for padding columns/pairs.
+template <int K>
+__global__ __launch_bounds__(256) void k_pair_tables(const double* __restrict__ x, int64_t n, int64_t ld,
+                                                     __bf16* __restrict__ PP, __bf16* __restrict__ MM) {
+  using TL = Tiles32<K>;
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= ld) return;
+  double v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) v[k] = (c < n) ? x[(int64_t)K * c + k] : 0.0;
+  int pn = 0;
+#pragma unroll
+  for (int a = 0; a < K; ++a)
+#pragma unroll
+    for (int b = a; b < K; ++b, ++pn) {
+      PP[(int64_t)pn * ld + c] = (__bf16)(c < n ? (float)(v[a] * v[b]) : 0.0f);
+      if (MM) MM[(int64_t)pn * ld + c] = (__bf16)(c < n ? (float)((1.0 - v[a]) * (1.0 - v[b])) : 0.0f);
+    }
+  for (; pn < TL::NW; ++pn) {
+    PP[(int64_t)pn * ld + c] = (__bf16)0.0f;
+    if (MM) MM[(int64_t)pn * ld + c] = (__bf16)0.0f;
+  }
+}
+
+__global__ __launch_bounds__(256) void k_to_float(const double* __restrict__ x, int64_t n, float* __restrict__ y) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) y[e] = (float)x[e];
+}
+
+// FP32 [p, 1-p] per column (2K floats), the wave-uniform operands of the Q side.
+__global__ __launch_bounds__(256) void k_to_float_pm(const double* __restrict__ x, int64_t ncol, int K,
+                                                     float* __restrict__ y) {
+  const int64_t n = ncol * K;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    const int64_t col = e / K;
+    const int k = (int)(e - col * K);
+    const double v = x[e];
+    y[2 * K * col + k] = (float)v;
+    y[2 * K * col + K + k] = (float)(1.0 - v);
+  }
+}
+
+// Store the 16 accumulators of one 32 x 32 tile: rows are the 32 individuals
+// (or SNPs) of row tile T, columns pairs 32t .. 32t+31.
+template <int NP>
+__device__ __forceinline__ void store_tile(float* __restrict__ part, int64_t n, int64_t row0, int t,
+                                           const f32x16& acc) {
+  const int lane = threadIdx.x & 63;
+  const int pn = 32 * t + (lane & 31);
+  if (pn >= NP) return;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int64_t row = row0 + (r & 3) + 4 * (lane >> 5) + 8 * (r >> 2);
+    if (row < n) part[(int64_t)pn * n + row] = acc[r];
+  }
+}
+
+// ------------------------------------------------------------------ Q side
+template <int K>
+__global__ __launch_bounds__(256) void k_qside_bf16(PassArgs a, const float* __restrict__ fq,
+                                                    const float* __restrict__ fpm, const __bf16* __restrict__ PPt,
+                                                    const __bf16* __restrict__ MMt, int64_t ld) {
+  using TL = Tiles32<K>;
+  constexpr int NP = TL::NP, NT = TL::NT;
+  const int lane = threadIdx.x & 63;
+  const int64_t Ilen = a.i1 - a.i0;
+  const int64_t il = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t wrow0 = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63);  // first row of this wave
+  const bool valid = il < Ilen;
+  const int64_t i = a.i0 + (valid ? il : 0);
+  const int64_t jb = a.j0 + (int64_t)blockIdx.y * a.chunk;
+  const int64_t je = (a.j1 < jb + a.chunk) ? a.j1 : jb + a.chunk;
+
+  float qi[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) qi[k] = fq[k + (int64_t)K * i];
+  const uint8_t* __restrict__ gp = a.bed + (i >> 2);
+  const uint32_t sh = (uint32_t)(i & 3) * 2u;
+  float z[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) z[k] = 0.f;
+  f32x16 acc[2][NT];
+#pragma unroll
+  for (int T = 0; T < 2; ++T)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[T][t] = (f32x16){};
+  const int kh = lane >> 5, cn = lane & 31;
+
+  // Double-buffered: the genotype bytes and B tiles of step js+16 are in flight
+  // while step js is computed; the P column of SNP j+1 is loaded while SNP j is
+  // computed.  The BED and the FP32 P copy carry 16 padding SNP rows, so no load
+  // of a partial last step is clamped (padding genotypes are masked as missing).
+  const uint32_t ioff = (uint32_t)(i >> 2);
+  uint32_t bytes_n[16];
+  bf16x8 bpp_n[NT], bmm_n[NT];
+  auto fetch = [&](int64_t js_) {
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      bpp_n[t] = load_b8(PPt, ld, 32 * t + cn, js_ + 8 * kh);
+      bmm_n[t] = load_b8(MMt, ld, 32 * t + cn, js_ + 8 * kh);
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) {
+      const uint8_t* __restrict__ row = a.bed + a.pitch * (js_ + u);
+      bytes_n[u] = row[ioff];
+    }
+  };
+  float pn[2 * K];
+  if (jb < je) {
+    fetch(jb);
+#pragma unroll
+    for (int k = 0; k < 2 * K; ++k) pn[k] = fpm[(int64_t)2 * K * jb + k];
+  }
+  for (int64_t js = jb; js < je; js += 16) {
+    bf16x8 bpp[NT], bmm[NT];
+    uint32_t codes[16];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      bpp[t] = bpp_n[t];
+      bmm[t] = bmm_n[t];
+    }
+#pragma unroll
+    for (int u = 0; u < 16; ++u) codes[u] = (valid && js + u < je) ? ((bytes_n[u] >> sh) & 3u) : 1u;
+    if (js + 16 < je) fetch(js + 16);
+    uint32_t wa[8], wb[8];
+#pragma unroll
+    for (int u = 0; u < 16; u += 2) {
+      float av[2], bv[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        float pv[K], mv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          pv[k] = pn[k];
+          mv[k] = pn[K + k];
+        }
+        const float* __restrict__ pj1 = fpm + (int64_t)2 * K * (js + u + h + 1);  // padded past J
+#pragma unroll
+        for (int k = 0; k < 2 * K; ++k) pn[k] = pj1[k];
+        float s = qi[0] * pv[0];
+#pragma unroll
+        for (int k = 1; k < K; ++k) s = __builtin_fmaf(qi[k], pv[k], s);
+        const float t1 = 1.f - s;
+        const uint32_t c = codes[u + h];
+        const float gm = (float)code_g(c), hm = (float)code_h(c);
+        const float r = rcp(s * t1);
+        const float is = t1 * r, it = s * r;
+        const float cc = gm * is, dd = hm * it;
+        av[h] = cc * is;
+        bv[h] = dd * it;
+#pragma unroll
+        for (int k = 0; k < K; ++k) z[k] = __builtin_fmaf(cc, pv[k], __builtin_fmaf(dd, mv[k], z[k]));
+      }
+      wa[u >> 1] = pack_bf16x2(av[0], av[1]);
+      wb[u >> 1] = pack_bf16x2(bv[0], bv[1]);
+    }
+    bf16x8 a0, a1, b0, b1;
+    to_tiles({wa[0], wa[1], wa[2], wa[3]}, {wa[4], wa[5], wa[6], wa[7]}, a0, a1);
+    to_tiles({wb[0], wb[1], wb[2], wb[3]}, {wb[4], wb[5], wb[6], wb[7]}, b0, b1);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bpp[t], acc[0][t], 0, 0, 0);
+      acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bpp[t], acc[1][t], 0, 0, 0);
+      acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b0, bmm[t], acc[0][t], 0, 0, 0);
+      acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(b1, bmm[t], acc[1][t], 0, 0, 0);
+    }
+  }
+  constexpr int NA = NP + K;
+  float* __restrict__ part = static_cast<float*>(a.part) + (int64_t)blockIdx.y * NA * Ilen;
+#pragma unroll
+  for (int T = 0; T < 2; ++T)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) store_tile<NP>(part, Ilen, wrow0 + 32 * T, t, acc[T][t]);
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) part[(int64_t)(NP + k) * Ilen + il] = z[k];
+  }
+}
+
+// ------------------------------------------------------------------ P side
+template <int K>
+__global__ __launch_bounds__(256) void k_pside_bf16(PassArgs a, const float* __restrict__ fq,
+                                                    const float* __restrict__ fp, const __bf16* __restrict__ QQt,
+                                                    int64_t ld) {
+  using TL = Tiles32<K>;
+  constexpr int NP = TL::NP, NT = TL::NT;
+  const int lane = threadIdx.x & 63;
+  const int64_t Jlen = a.j1 - a.j0;
+  const int64_t jl = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t wrow0 = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63);
+  const bool valid = jl < Jlen;
+  const int64_t j = a.j0 + (valid ? jl : 0);
+  const int64_t ia0 = (int64_t)blockIdx.y * a.chunk;  // i0 = 0 in the device pipeline
+  const int64_t ia1 = (a.i1 < ia0 + a.chunk) ? a.i1 : ia0 + a.chunk;
+
+  float pj[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) pj[k] = fp[k + (int64_t)K * j];
+  const uint32_t* __restrict__ col = reinterpret_cast<const uint32_t*>(a.bed + a.pitch * j);
+  float z[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) z[k] = 0.f;
+  f32x16 acc[2][NT];
+#pragma unroll
+  for (int T = 0; T < 2; ++T)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[T][t] = (f32x16){};
+  const int kh = lane >> 5, cn = lane & 31;
+
+  for (int64_t ib = ia0; ib < ia1; ib += 16) {
+    bf16x8 bqq[NT];
+#pragma unroll
+    for (int t = 0; t < NT; ++t) bqq[t] = load_b8(QQt, ld, 32 * t + cn, ib + 8 * kh);
+    uint32_t w = col[ib >> 4];
+    if (ib + 16 > ia1) {  // uniform: individuals past the chunk end are missing
+      uint32_t keep = 0;
+#pragma unroll
+      for (int t = 0; t < 16; ++t)
+        if (ib + t < ia1) keep |= 3u << (2 * t);
+      w = (w & keep) | (0x55555555u & ~keep);
+    }
+    if (!valid) w = 0x55555555u;
+    uint32_t wv[8];
+#pragma unroll
+    for (int u = 0; u < 16; u += 2) {
+      float wt[2];
+#pragma unroll
+      for (int h = 0; h < 2; ++h) {
+        const int64_t ii = (ib + u + h < ia1) ? ib + u + h : ia1 - 1;
+        const float* __restrict__ qv = fq + (int64_t)K * ii;
+        float qq[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) qq[k] = qv[k];
+        float s = qq[0] * pj[0];
+#pragma unroll
+        for (int k = 1; k < K; ++k) s = __builtin_fmaf(qq[k], pj[k], s);
+        const float t1 = 1.f - s;
+        const uint32_t c = (w >> (2 * (u + h))) & 3u;
+        const float gm = (float)code_g(c), hm = (float)code_h(c);
+        const float r = rcp(s * t1);
+        const float is = t1 * r, it = s * r;
+        const float cc = gm * is, dd = hm * it;
+        wt[h] = cc * is + dd * it;
+        const float uu = cc - dd;
+#pragma unroll
+        for (int k = 0; k < K; ++k) z[k] = __builtin_fmaf(uu, qq[k], z[k]);
+      }
+      wv[u >> 1] = pack_bf16x2(wt[0], wt[1]);
+    }
+    bf16x8 a0, a1;
+    to_tiles({wv[0], wv[1], wv[2], wv[3]}, {wv[4], wv[5], wv[6], wv[7]}, a0, a1);
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      acc[0][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a0, bqq[t], acc[0][t], 0, 0, 0);
+      acc[1][t] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a1, bqq[t], acc[1][t], 0, 0, 0);
+    }
+  }
+  constexpr int NA = NP + K;
+  float* __restrict__ part = static_cast<float*>(a.part) + (int64_t)blockIdx.y * NA * Jlen;
+#pragma unroll
+  for (int T = 0; T < 2; ++T)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) store_tile<NP>(part, Jlen, wrow0 + 32 * T, t, acc[T][t]);
+  if (valid) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) part[(int64_t)(NP + k) * Jlen + jl] = z[k];
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+static unsigned grid_cap(int64_t n) {
+  int64_t b = (n + 255) / 256;
+  return (unsigned)(b < 1 ? 1 : (b > 4096 ? 4096 : b));
+}
+
+void launch_to_float(const double* x, int64_t n, float* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_float, dim3(grid_cap(n)), dim3(256), 0, s, x, n, y);
+}
+void launch_to_float_pm(const double* x, int64_t ncol, int K, float* y, hipStream_t s) {
+  hipLaunchKernelGGL(k_to_float_pm, dim3(grid_cap(ncol * K)), dim3(256), 0, s, x, ncol, K, y);
+}
+
+template <int K>
+static void pair_tables_k(const double* x, int64_t n, int64_t ld, __bf16* PP, __bf16* MM, hipStream_t s) {
+  hipLaunchKernelGGL((k_pair_tables<K>), dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, s, x, n, ld, PP, MM);
+}
+void launch_pair_tables(int K, const double* x, int64_t n, int64_t ld, void* PP, void* MM, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (pair_tables_k<KK>(x, n, ld, (__bf16*)PP, (__bf16*)MM, s)));
+}
+int pair_table_rows(int K) {
+  const int np = K * (K + 1) / 2;
+  return ((np + 31) / 32) * 32;
+}
+
+template <int K>
+static void qside_bf16_k(const PassArgs& a, int nchunk, const float* fq, const float* fp, const void* PPt,
+                         const void* MMt, int64_t ld, hipStream_t s) {
+  dim3 grid((unsigned)((a.i1 - a.i0 + 255) / 256), (unsigned)nchunk);
+  hipLaunchKernelGGL((k_qside_bf16<K>), grid, dim3(256), 0, s, a, fq, fp, (const __bf16*)PPt, (const __bf16*)MMt,
+                     ld);
+}
+template <int K>
+static void pside_bf16_k(const PassArgs& a, int nchunk, const float* fq, const float* fp, const void* QQt, int64_t ld,
+                         hipStream_t s) {
+  dim3 grid((unsigned)((a.j1 - a.j0 + 255) / 256), (unsigned)nchunk);
+  hipLaunchKernelGGL((k_pside_bf16<K>), grid, dim3(256), 0, s, a, fq, fp, (const __bf16*)QQt, ld);
+}
+void launch_qside_bf16(int K, const PassArgs& a, int nchunk, const float* fq, const float* fp, const void* PPt,
+                       const void* MMt, int64_t ld, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (qside_bf16_k<KK>(a, nchunk, fq, fp, PPt, MMt, ld, s)));
+}
+void launch_pside_bf16(int K, const PassArgs& a, int nchunk, const float* fq, const float* fp, const void* QQt,
+                       int64_t ld, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (pside_bf16_k<KK>(a, nchunk, fq, fp, QQt, ld, s)));
+}
+
+}  // namespace admx

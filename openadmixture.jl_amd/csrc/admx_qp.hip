@@ -181,7 +181,7 @@ __device__ __forceinline__ int qp_from_hessian(T* __restrict__ L, int t, const T
 }
 
 // ------------------------------------------------------------------ Q side finalize
-template <typename T, int K, bool TRICK, bool DO_QP>
+template <typename T, int K, bool TRICK, bool DO_QP, typename PT = T>
 __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_q_update(FinArgs f) {
   constexpr int NT = QPLanes<K>::N;
   constexpr int NTRI = Tri<K>::N;
@@ -191,14 +191,14 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_q_update(FinArgs f) {
   const int t = threadIdx.x;
   const int64_t col = (int64_t)blockIdx.x * NT + t;
   if (col >= f.n) return;
-  const T* __restrict__ part = static_cast<const T*>(f.part);
+  const PT* __restrict__ part = static_cast<const PT*>(f.part);
   T acc[NA];
 #pragma unroll
   for (int a = 0; a < NA; ++a) acc[a] = 0;
   for (int c = 0; c < f.nchunk; ++c) {
-    const T* src = part + (int64_t)c * NA * f.n + col;
+    const PT* src = part + (int64_t)c * NA * f.n + col;
 #pragma unroll
-    for (int a = 0; a < NA; ++a) acc[a] += src[(int64_t)a * f.n];
+    for (int a = 0; a < NA; ++a) acc[a] += (T)src[(int64_t)a * f.n];
   }
   T Hf[K * K], xtz[K];
   {
@@ -274,7 +274,7 @@ __global__ __launch_bounds__(256) void k_fin_q_em(FinArgs f) {
 }
 
 // ------------------------------------------------------------------ P side finalize
-template <typename T, int K, bool DO_QP>
+template <typename T, int K, bool DO_QP, typename PT = T>
 __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_p_update(FinArgs f) {
   constexpr int NT = QPLanes<K>::N;
   constexpr int NTRI = Tri<K>::N;
@@ -284,14 +284,14 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_p_update(FinArgs f) {
   const int t = threadIdx.x;
   const int64_t col = (int64_t)blockIdx.x * NT + t;
   if (col >= f.n) return;
-  const T* __restrict__ part = static_cast<const T*>(f.part);
+  const PT* __restrict__ part = static_cast<const PT*>(f.part);
   T acc[NA];
 #pragma unroll
   for (int a = 0; a < NA; ++a) acc[a] = 0;
   for (int c = 0; c < f.nchunk; ++c) {
-    const T* src = part + (int64_t)c * NA * f.n + col;
+    const PT* src = part + (int64_t)c * NA * f.n + col;
 #pragma unroll
-    for (int a = 0; a < NA; ++a) acc[a] += src[(int64_t)a * f.n];
+    for (int a = 0; a < NA; ++a) acc[a] += (T)src[(int64_t)a * f.n];
   }
   T Hf[K * K], xtz[K];
   {
@@ -493,5 +493,21 @@ template <typename T> void launch_project_p(int64_t n, void* b, hipStream_t s) {
   template void launch_project_q<T>(int, int64_t, void*, hipStream_t);                                        \
   template void launch_project_p<T>(int64_t, void*, hipStream_t);
 ADMX_INST(ADMX_T)
+
+#ifdef ADMX_WITH_MIXED
+// FP64 finalize of FP32 partials (mixed-precision passes, direct Hessian form).
+template <int K>
+static void fin_q_mixed_k(const FinArgs& f, hipStream_t s) {
+  constexpr int NT = QPLanes<K>::N;
+  hipLaunchKernelGGL((k_fin_q_update<double, K, false, true, float>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
+}
+template <int K>
+static void fin_p_mixed_k(const FinArgs& f, hipStream_t s) {
+  constexpr int NT = QPLanes<K>::N;
+  hipLaunchKernelGGL((k_fin_p_update<double, K, true, float>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
+}
+void launch_fin_q_update_mixed(int K, const FinArgs& f, hipStream_t s) { ADMX_DISPATCH_K(K, (fin_q_mixed_k<KK>(f, s))); }
+void launch_fin_p_update_mixed(int K, const FinArgs& f, hipStream_t s) { ADMX_DISPATCH_K(K, (fin_p_mixed_k<KK>(f, s))); }
+#endif
 
 }  // namespace admx

@@ -249,5 +249,10 @@ class HIPAdmixData:
             rep[name] = (int(n), float(ms))
         return rep
 
+    def set_precision(self, mode):
+        """'f64' (plain), 'f32' or 'bf16' for the device-resident update passes."""
+        codes = {"f64": N.ADMX_PREC_F64, "f32": N.ADMX_PREC_F32, "bf16": N.ADMX_PREC_BF16}
+        self.set_tuning(N.ADMX_TUNE_PRECISION, codes[mode] if isinstance(mode, str) else int(mode))
+
     def set_tuning(self, key: int, value: int):
         N.check(N.lib().admx_set_tuning(self._h, key, int(value)))

@@ -1,0 +1,76 @@
+"""Mixed-precision forms of the device-resident update passes (FP64 context,
+FP32 or BF16-MFMA passes, FP64 QP / log-likelihood / convergence test).
+
+Per-output error bars against the plain FP64 path on the same state, written
+per test:
+  Hessian (XtX)   FP32: 1e-5, BF16: 5e-3   relative to the column maximum
+  gradient (Xtz)  FP32: 1e-5, BF16: 1e-5   relative to the column maximum
+  converged LL    |LL_form - LL_f64| / |LL_f64| < 1e-6 (the benchmark bar is 1e-4)
+"""
+import numpy as np
+import pytest
+
+from conftest import make_cohort
+
+pytestmark = pytest.mark.gpu
+
+TOL = {"f32": (1e-5, 1e-5), "bf16": (5e-3, 1e-5)}
+
+
+def _colrel(got, ref):
+    got = np.asarray(got, np.float64).reshape(-1, got.shape[-1])
+    ref = np.asarray(ref, np.float64).reshape(-1, ref.shape[-1])
+    return (np.abs(got - ref).max(0) / (np.abs(ref).max(0) + 1e-300)).max()
+
+
+def _ctx(oa, O, I, J, K, seed, beta=True):
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, alpha=0.3, miss=0.02, seed=seed, p_lo=0.01, p_hi=0.99, beta=beta)
+    c = oa.HIPAdmixData(I, J, K, 3)
+    c.upload_bed(bed.reshape((pitch, J), order="F"))
+    d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(seed))
+    return c, d
+
+
+@pytest.mark.parametrize("K", [2, 3, 8, 12])
+@pytest.mark.parametrize("form", ["f32", "bf16"])
+def test_mixed_hessians_match_f64(oa, O, K, form):
+    print("mixed hessians", K, form, flush=True)
+    I, J = 1500, 4100
+    c, d = _ctx(oa, O, I, J, K, seed=K)
+    oa.init_em(d, None, 3, d_cu=c, log=None)
+    out = {}
+    for mode in ("f64", form):
+        c.set_precision(mode)
+        c.update_q(False)
+        XtXq, Xtzq = c.hessians(0)
+        c.set_x(1, d.x)  # same q_next (= q) for both modes' update_p
+        c.update_p(False)
+        XtXp, Xtzp = c.hessians(1)
+        out[mode] = (XtXq, Xtzq, XtXp, Xtzp, c.get_x(1))
+    htol, gtol = TOL[form]
+    f, m = out["f64"], out[form]
+    eq, gq, ep, gp = _colrel(m[0], f[0]), _colrel(m[1], f[1]), _colrel(m[2], f[2]), _colrel(m[3], f[3])
+    print(f"K={K} {form}: XtX_q {eq:.2e} Xtz_q {gq:.2e} XtX_p {ep:.2e} Xtz_p {gp:.2e}", flush=True)
+    assert eq < htol and ep < htol, (eq, ep)
+    assert gq < gtol and gp < gtol, (gq, gp)
+    c.close()
+
+
+@pytest.mark.parametrize("form", ["f32", "bf16"])
+def test_mixed_converges_to_f64_optimum(oa, O, form):
+    print("mixed convergence", form, flush=True)
+    I, J, K = 1200, 6000, 4
+    res = {}
+    for mode in ("f64", form):
+        c, d = _ctx(oa, O, I, J, K, seed=77)
+        c.set_precision(mode)
+        oa.init_em(d, None, 5, d_cu=c, log=None)
+        hist = oa.admixture_qn(d, None, 200, 1e-10, d_cu=c, log=None)
+        res[mode] = (hist[-1], d.q.copy(), d.p.copy(), len(hist) - 1)
+        c.close()
+    llf, ll = res["f64"][0], res[form][0]
+    dq = np.abs(res[form][1] - res["f64"][1]).max()
+    dp = np.abs(res[form][2] - res["f64"][2]).max()
+    print(f"{form}: dLL {abs(ll - llf) / abs(llf):.2e}, max|dQ| {dq:.2e}, max|dP| {dp:.2e}, "
+          f"iterations {res[form][3]} vs {res['f64'][3]}", flush=True)
+    assert abs(ll - llf) <= 1e-6 * abs(llf)
