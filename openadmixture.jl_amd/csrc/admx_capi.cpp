@@ -151,6 +151,13 @@ int prof_collect(admx_ctx* c) {
   return ADMX_OK;
 }
 
+// The 16 SNP rows after the last one hold the missing code (01 in every field):
+// passes that step 16 SNPs at a time read them in the cohort's last step.
+int pad_bed(admx_ctx* c) {
+  HIPCHK(hipMemsetAsync(c->bed + (size_t)c->pitch * c->J, 0x55, (size_t)c->pitch * 16, c->s));
+  return ADMX_OK;
+}
+
 int check_launch() {
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(ADMX_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
@@ -265,12 +272,18 @@ int ll_to_slot(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1
 int mixed_update(admx_ctx* c, bool qside, const double* q, const double* p, const double* x_in, double* x_out,
                  int32_t* status_dev) {
   const int64_t I = c->I, J = c->J, K = c->K;
-  CHK(c->fq.grow(sizeof(float) * K * I));
-  if (c->fp.n < sizeof(float) * 2 * K * (J + 17)) {
-    CHK(c->fp.grow(sizeof(float) * 2 * K * (J + 17)));
-    std::vector<float> half(2 * K * 17, 0.5f);  // padding SNP columns: s stays inside (0, 1)
+  if (c->fq.n < sizeof(float) * K * (I + 16)) {
+    CHK(c->fq.grow(sizeof(float) * K * (I + 16)));
+    std::vector<float> pad(K * 16, 0.5f / (float)K);  // padding individuals: s stays inside (0, 1)
     HIPCHK(hipStreamSynchronize(c->s));
-    HIPCHK(hipMemcpy(static_cast<float*>(c->fp.p) + 2 * K * J, half.data(), sizeof(float) * half.size(),
+    HIPCHK(hipMemcpy(static_cast<float*>(c->fq.p) + K * I, pad.data(), sizeof(float) * pad.size(),
+                     hipMemcpyHostToDevice));
+  }
+  if (c->fp.n < sizeof(float) * K * (J + 17)) {
+    CHK(c->fp.grow(sizeof(float) * K * (J + 17)));
+    std::vector<float> half(K * 17, 0.5f);  // padding SNP columns: s stays inside (0, 1)
+    HIPCHK(hipStreamSynchronize(c->s));
+    HIPCHK(hipMemcpy(static_cast<float*>(c->fp.p) + K * J, half.data(), sizeof(float) * half.size(),
                      hipMemcpyHostToDevice));
   }
   float* fq = static_cast<float*>(c->fq.p);
@@ -281,8 +294,7 @@ int mixed_update(admx_ctx* c, bool qside, const double* q, const double* p, cons
   {
     ProfScope ps(c, "mixed_prep");
     launch_to_float(q, K * I, fq, c->s);
-    if (bf16 && qside) launch_to_float_pm(p, J, (int)K, fp, c->s);  // [p, 1-p] per SNP
-    else launch_to_float(p, K * J, fp, c->s);
+    launch_to_float(p, K * J, fp, c->s);
     if (bf16) {
       if (qside) {
         CHK(c->ppt.grow((size_t)rows * ldj * 2));
@@ -770,7 +782,8 @@ int admx_upload_bed(admx_ctx* c, const uint8_t* bed, int64_t bytes_per_snp) {
   Guard g(c);
   if (!bed) return bad("null BED pointer");
   if (bytes_per_snp < c->ibytes) return bad("bytes_per_snp < ceil(I/4)");
-  HIPCHK(hipMemsetAsync(c->bed, 0, (size_t)c->pitch * (c->J + 16), c->s));
+  HIPCHK(hipMemsetAsync(c->bed, 0, (size_t)c->pitch * c->J, c->s));
+  CHK(pad_bed(c));
   HIPCHK(hipMemcpy2DAsync(c->bed, c->pitch, bed, bytes_per_snp, c->ibytes, c->J, hipMemcpyHostToDevice, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   c->have_bed = true;
@@ -799,6 +812,7 @@ int admx_generate_bed(admx_ctx* c, const double* Qt, const double* Pt, uint64_t 
     launch_gen_bed(static_cast<const double*>(c->st[0].p), static_cast<const double*>(c->st[1].p), c->I, c->J,
                    (int)c->K, seed, miss_rate, c->bed, c->pitch, c->s);
   }
+  CHK(pad_bed(c));
   CHK(check_launch());
   HIPCHK(hipStreamSynchronize(c->s));
   c->have_bed = true;

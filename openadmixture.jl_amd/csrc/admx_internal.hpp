@@ -67,7 +67,6 @@ void launch_fin_p_update_mixed(int K, const FinArgs& f, hipStream_t s);
 // Mixed precision passes (FP32 weights/gradient, BF16 MFMA Hessian) and their inputs.
 void launch_to_float(const double* x, int64_t n, float* y, hipStream_t s);
 // [p, 1-p] per column in FP32 (2K floats), the Q-side BF16 pass operand.
-void launch_to_float_pm(const double* x, int64_t ncol, int K, float* y, hipStream_t s);
 void launch_pair_tables(int K, const double* x, int64_t n, int64_t ld, void* PP, void* MM, hipStream_t s);
 int pair_table_rows(int K);
 void launch_qside_bf16(int K, const PassArgs& a, int nchunk, const float* fq, const float* fp, const void* PPt,

@@ -38,10 +38,20 @@ template <int K> struct Tiles32 {
   static constexpr int NW = NT * 32;
 };
 
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef __bf16 bf16x2 __attribute__((ext_vector_type(2)));
+
+// Two FP32 values rounded to BF16 (round to nearest even) in one register.
 __device__ __forceinline__ uint32_t pack_bf16x2(float lo, float hi) {
-  const __bf16 a = (__bf16)lo, b = (__bf16)hi;
-  return (uint32_t)__builtin_bit_cast(uint16_t, a) | ((uint32_t)__builtin_bit_cast(uint16_t, b) << 16);
+  return __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){lo, hi}, bf16x2));
 }
+
+// Genotype counts of four 2-bit codes held in the low bits of the four bytes of
+// `codes`: byte x of the result is g (table 0,0,1,2 for codes 00,01,10,11) or
+// 2-g on observed entries (2,0,1,0), one v_perm_b32 byte lookup each.
+__device__ __forceinline__ uint32_t g_bytes(uint32_t codes) { return __builtin_amdgcn_perm(0u, 0x02010000u, codes); }
+__device__ __forceinline__ uint32_t h_bytes(uint32_t codes) { return __builtin_amdgcn_perm(0u, 0x00010002u, codes); }
+__device__ __forceinline__ float byte_f(uint32_t w, int x) { return (float)((w >> (8 * x)) & 0xffu); }
 
 // 8 consecutive BF16 of row `n` of a [NW][ld] table starting at column c (16-byte aligned).
 __device__ __forceinline__ bf16x8 load_b8(const __bf16* __restrict__ t, int64_t ld, int n, int64_t c) {
@@ -93,19 +103,6 @@ __global__ __launch_bounds__(256) void k_to_float(const double* __restrict__ x, 
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) y[e] = (float)x[e];
 }
 
-// FP32 [p, 1-p] per column (2K floats), the wave-uniform operands of the Q side.
-__global__ __launch_bounds__(256) void k_to_float_pm(const double* __restrict__ x, int64_t ncol, int K,
-                                                     float* __restrict__ y) {
-  const int64_t n = ncol * K;
-  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
-    const int64_t col = e / K;
-    const int k = (int)(e - col * K);
-    const double v = x[e];
-    y[2 * K * col + k] = (float)v;
-    y[2 * K * col + K + k] = (float)(1.0 - v);
-  }
-}
-
 // Store the 16 accumulators of one 32 x 32 tile: rows are the 32 individuals
 // (or SNPs) of row tile T, columns pairs 32t .. 32t+31.
 template <int NP>
@@ -122,12 +119,25 @@ __device__ __forceinline__ void store_tile(float* __restrict__ part, int64_t n, 
 }
 
 // ------------------------------------------------------------------ Q side
+// The gradient is accumulated as z_k = sum (c-d) p_k + sum d (the ADMIXTURE
+// gradient plus the constant column), K+1 FMAs per genotype instead of 2K; all
+// terms are O(1) and the two sums do not cancel.
+//
+// Genotype bytes: each step of 16 SNPs needs, per lane, the byte of its
+// individual in 16 SNP rows (16 x 16 bytes per wave).  Lane l fetches one
+// 32-bit word, bytes 4(l%4)..4(l%4)+3 of the wave's 16-byte span of row l/4, and
+// the 16 x 16 byte block is transposed through 256 bytes of LDS per wave, so
+// that every lane reads its 16 bytes (one per SNP) with one 128-bit load.  The
+// device pipeline starts at individual 0, so every wave's span is 16-byte
+// aligned and lies inside the padded BED pitch.
 template <int K>
 __global__ __launch_bounds__(256) void k_qside_bf16(PassArgs a, const float* __restrict__ fq,
-                                                    const float* __restrict__ fpm, const __bf16* __restrict__ PPt,
+                                                    const float* __restrict__ fp, const __bf16* __restrict__ PPt,
                                                     const __bf16* __restrict__ MMt, int64_t ld) {
   using TL = Tiles32<K>;
   constexpr int NP = TL::NP, NT = TL::NT;
+  __shared__ __align__(16) uint8_t tbuf[4][256];
+  uint8_t* __restrict__ tb = tbuf[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
   const int64_t Ilen = a.i1 - a.i0;
   const int64_t il = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -140,9 +150,8 @@ __global__ __launch_bounds__(256) void k_qside_bf16(PassArgs a, const float* __r
   float qi[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) qi[k] = fq[k + (int64_t)K * i];
-  const uint8_t* __restrict__ gp = a.bed + (i >> 2);
-  const uint32_t sh = (uint32_t)(i & 3) * 2u;
-  float z[K];
+  const uint32_t sh = (uint32_t)(lane & 3) * 2u;  // i0 = 0: lane l is individual wrow0 + l
+  float z[K], dsum = 0.f;
 #pragma unroll
   for (int k = 0; k < K; ++k) z[k] = 0.f;
   f32x16 acc[2][NT];
@@ -152,70 +161,68 @@ __global__ __launch_bounds__(256) void k_qside_bf16(PassArgs a, const float* __r
     for (int t = 0; t < NT; ++t) acc[T][t] = (f32x16){};
   const int kh = lane >> 5, cn = lane & 31;
 
-  // Double-buffered: the genotype bytes and B tiles of step js+16 are in flight
-  // while step js is computed; the P column of SNP j+1 is loaded while SNP j is
-  // computed.  The BED and the FP32 P copy carry 16 padding SNP rows, so no load
-  // of a partial last step is clamped (padding genotypes are masked as missing).
-  const uint32_t ioff = (uint32_t)(i >> 2);
-  uint32_t bytes_n[16];
-  bf16x8 bpp_n[NT], bmm_n[NT];
-  auto fetch = [&](int64_t js_) {
-#pragma unroll
-    for (int t = 0; t < NT; ++t) {
-      bpp_n[t] = load_b8(PPt, ld, 32 * t + cn, js_ + 8 * kh);
-      bmm_n[t] = load_b8(MMt, ld, 32 * t + cn, js_ + 8 * kh);
-    }
-#pragma unroll
-    for (int u = 0; u < 16; ++u) {
-      const uint8_t* __restrict__ row = a.bed + a.pitch * (js_ + u);
-      bytes_n[u] = row[ioff];
-    }
-  };
-  float pn[2 * K];
+  // The genotype word of step js+16 is in flight while step js is computed, and
+  // the B tiles of step js are loaded at its start and consumed by the MFMAs
+  // at its end; the P column of SNP j+1 is loaded while SNP j is computed.  Only
+  // the cohort's last step can run past je = J: the BED's 16 padding rows hold
+  // the missing code and the FP32 P copy's padding columns hold 0.5, so such
+  // SNPs add nothing.  Lanes past the last individual compute on individual 0
+  // and store nothing.
+  const uint8_t* __restrict__ gbase = a.bed + ((a.i0 + wrow0) >> 2) + 4 * (lane & 3) + a.pitch * (lane >> 2);
+  uint32_t gw_n = 0;
+  float pn[K];
   if (jb < je) {
-    fetch(jb);
+    gw_n = *reinterpret_cast<const uint32_t*>(gbase + a.pitch * jb);
 #pragma unroll
-    for (int k = 0; k < 2 * K; ++k) pn[k] = fpm[(int64_t)2 * K * jb + k];
+    for (int k = 0; k < K; ++k) pn[k] = fp[(int64_t)K * jb + k];
   }
   for (int64_t js = jb; js < je; js += 16) {
     bf16x8 bpp[NT], bmm[NT];
-    uint32_t codes[16];
 #pragma unroll
     for (int t = 0; t < NT; ++t) {
-      bpp[t] = bpp_n[t];
-      bmm[t] = bmm_n[t];
+      bpp[t] = load_b8(PPt, ld, 32 * t + cn, js + 8 * kh);
+      bmm[t] = load_b8(MMt, ld, 32 * t + cn, js + 8 * kh);
     }
+    // byte e of this lane's word is byte 4(l%4)+e of SNP row l/4 -> tb[byte][row]
 #pragma unroll
-    for (int u = 0; u < 16; ++u) codes[u] = (valid && js + u < je) ? ((bytes_n[u] >> sh) & 3u) : 1u;
-    if (js + 16 < je) fetch(js + 16);
+    for (int e = 0; e < 4; ++e) tb[(4 * (lane & 3) + e) * 16 + (lane >> 2)] = (uint8_t)(gw_n >> (8 * e));
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    const u32x4 cur = *reinterpret_cast<const u32x4*>(tb + 16 * (lane >> 2));  // 16 SNPs of this lane's byte
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (js + 16 < je) gw_n = *reinterpret_cast<const uint32_t*>(gbase + a.pitch * (js + 16));
+    uint32_t gb[4], hb[4];
+#pragma unroll
+    for (int w = 0; w < 4; ++w) {
+      const uint32_t cd = (cur[w] >> sh) & 0x03030303u;
+      gb[w] = g_bytes(cd);
+      hb[w] = h_bytes(cd);
+    }
     uint32_t wa[8], wb[8];
 #pragma unroll
     for (int u = 0; u < 16; u += 2) {
       float av[2], bv[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        float pv[K], mv[K];
+        float pv[K];
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
-          pv[k] = pn[k];
-          mv[k] = pn[K + k];
-        }
-        const float* __restrict__ pj1 = fpm + (int64_t)2 * K * (js + u + h + 1);  // padded past J
+        for (int k = 0; k < K; ++k) pv[k] = pn[k];
+        const float* __restrict__ pj1 = fp + (int64_t)K * (js + u + h + 1);  // padded past J
 #pragma unroll
-        for (int k = 0; k < 2 * K; ++k) pn[k] = pj1[k];
+        for (int k = 0; k < K; ++k) pn[k] = pj1[k];
         float s = qi[0] * pv[0];
 #pragma unroll
         for (int k = 1; k < K; ++k) s = __builtin_fmaf(qi[k], pv[k], s);
         const float t1 = 1.f - s;
-        const uint32_t c = codes[u + h];
-        const float gm = (float)code_g(c), hm = (float)code_h(c);
+        const float gm = byte_f(gb[(u + h) >> 2], (u + h) & 3), hm = byte_f(hb[(u + h) >> 2], (u + h) & 3);
         const float r = rcp(s * t1);
         const float is = t1 * r, it = s * r;
         const float cc = gm * is, dd = hm * it;
         av[h] = cc * is;
         bv[h] = dd * it;
+        const float uu = cc - dd;
+        dsum += dd;
 #pragma unroll
-        for (int k = 0; k < K; ++k) z[k] = __builtin_fmaf(cc, pv[k], __builtin_fmaf(dd, mv[k], z[k]));
+        for (int k = 0; k < K; ++k) z[k] = __builtin_fmaf(uu, pv[k], z[k]);
       }
       wa[u >> 1] = pack_bf16x2(av[0], av[1]);
       wb[u >> 1] = pack_bf16x2(bv[0], bv[1]);
@@ -239,7 +246,7 @@ __global__ __launch_bounds__(256) void k_qside_bf16(PassArgs a, const float* __r
     for (int t = 0; t < NT; ++t) store_tile<NP>(part, Ilen, wrow0 + 32 * T, t, acc[T][t]);
   if (valid) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) part[(int64_t)(NP + k) * Ilen + il] = z[k];
+    for (int k = 0; k < K; ++k) part[(int64_t)(NP + k) * Ilen + il] = z[k] + dsum;
   }
 }
 
@@ -286,14 +293,22 @@ __global__ __launch_bounds__(256) void k_pside_bf16(PassArgs a, const float* __r
       w = (w & keep) | (0x55555555u & ~keep);
     }
     if (!valid) w = 0x55555555u;
+    // plane t holds, in byte b, the code of individual ib + 4b + t
+    uint32_t gb[4], hb[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const uint32_t cd = (w >> (2 * t)) & 0x03030303u;
+      gb[t] = g_bytes(cd);
+      hb[t] = h_bytes(cd);
+    }
     uint32_t wv[8];
 #pragma unroll
     for (int u = 0; u < 16; u += 2) {
       float wt[2];
 #pragma unroll
       for (int h = 0; h < 2; ++h) {
-        const int64_t ii = (ib + u + h < ia1) ? ib + u + h : ia1 - 1;
-        const float* __restrict__ qv = fq + (int64_t)K * ii;
+        const int v = u + h;
+        const float* __restrict__ qv = fq + (int64_t)K * (ib + v);  // padded by 16 individuals
         float qq[K];
 #pragma unroll
         for (int k = 0; k < K; ++k) qq[k] = qv[k];
@@ -301,8 +316,7 @@ __global__ __launch_bounds__(256) void k_pside_bf16(PassArgs a, const float* __r
 #pragma unroll
         for (int k = 1; k < K; ++k) s = __builtin_fmaf(qq[k], pj[k], s);
         const float t1 = 1.f - s;
-        const uint32_t c = (w >> (2 * (u + h))) & 3u;
-        const float gm = (float)code_g(c), hm = (float)code_h(c);
+        const float gm = byte_f(gb[v & 3], v >> 2), hm = byte_f(hb[v & 3], v >> 2);
         const float r = rcp(s * t1);
         const float is = t1 * r, it = s * r;
         const float cc = gm * is, dd = hm * it;
@@ -341,9 +355,6 @@ static unsigned grid_cap(int64_t n) {
 
 void launch_to_float(const double* x, int64_t n, float* y, hipStream_t s) {
   hipLaunchKernelGGL(k_to_float, dim3(grid_cap(n)), dim3(256), 0, s, x, n, y);
-}
-void launch_to_float_pm(const double* x, int64_t ncol, int K, float* y, hipStream_t s) {
-  hipLaunchKernelGGL(k_to_float_pm, dim3(grid_cap(ncol * K)), dim3(256), 0, s, x, ncol, K, y);
 }
 
 template <int K>
