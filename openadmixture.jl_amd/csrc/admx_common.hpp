@@ -14,6 +14,7 @@
 
 namespace admx {
 
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 // PLINK-1 code -> (m*g, m*(2-g)) with m the nonmissing flag:
 //   00 -> (0,2)  01 (missing) -> (0,0)  10 -> (1,1)  11 -> (2,0)

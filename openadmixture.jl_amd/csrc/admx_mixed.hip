@@ -30,7 +30,6 @@ namespace admx {
 
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
-typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
 template <int K> struct Tiles32 {
   static constexpr int NP = K * (K + 1) / 2;

@@ -45,9 +45,6 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
   T qi[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) qi[k] = q[k + (int64_t)K * i];
-  const uint8_t* __restrict__ gp = a.bed + (i >> 2);
-  const uint32_t sh = (uint32_t)(i & 3) * 2u;
-
   T H[MODE == MODE_UPDATE ? NT : 1];
   T z[K];
   T hb[(MODE == MODE_UPDATE && TRICK) ? K : 1];
@@ -61,26 +58,8 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
 #pragma unroll
   for (int k = 0; k < ((MODE == MODE_UPDATE && TRICK) ? K : 1); ++k) hb[k] = 0;
 
-  // Software pipeline: the genotype byte and the P column of SNP j+1 are in
-  // flight while SNP j is computed.
-  uint32_t byte_n = 0;
-  T pn[K];
-#pragma unroll
-  for (int k = 0; k < K; ++k) pn[k] = 0;
-  if (jb < je) {
-    byte_n = gp[a.pitch * jb];
-#pragma unroll
-    for (int k = 0; k < K; ++k) pn[k] = p[(int64_t)K * jb + k];
-  }
-  for (int64_t j = jb; j < je; ++j) {
-    const uint32_t c = (byte_n >> sh) & 3u;
-    T pv[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) pv[k] = pn[k];
-    const int64_t jn = (j + 1 < je) ? j + 1 : j;
-    byte_n = gp[a.pitch * jn];
-#pragma unroll
-    for (int k = 0; k < K; ++k) pn[k] = p[(int64_t)K * jn + k];
+  // Per-genotype accumulation for genotype code c and P column pv.
+  auto step = [&](uint32_t c, const T (&pv)[K]) {
     T s = qi[0] * pv[0];
 #pragma unroll
     for (int k = 1; k < K; ++k) s = __builtin_fma(qi[k], pv[k], s);
@@ -91,11 +70,6 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
         const double x = (c >= 2u) ? (double)s : (double)t;
         const double y = (c == 3u) ? (double)s : (double)t;
         prod *= (c == 1u) ? 1.0 : x * y;
-        if (((j - jb) & 7) == 7) {
-          int e;
-          prod = __builtin_frexp(prod, &e);
-          esum += e;
-        }
       } else {
         // Terms in T, accumulated in Float64 (src/loops.jl:523, 535).
         const T gm = (T)code_g(c), hm = (T)code_h(c);
@@ -136,6 +110,82 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
           for (int k2 = k; k2 < K; ++k2, ++n) H[n] = __builtin_fma(ap, pv[k2], __builtin_fma(bm, (T)1 - pv[k2], H[n]));
         }
       }
+    }
+  };
+  auto renorm = [&]() {
+    if constexpr (MODE == MODE_LL && sizeof(T) == 8) {
+      int e;
+      prod = __builtin_frexp(prod, &e);
+      esum += e;
+    }
+  };
+
+  // Genotype codes, 16 SNPs per step.  The wave's 64 individuals occupy at most
+  // 17 bytes of an SNP row, inside the 32-byte window starting at the 16-byte
+  // boundary `wbase`.  Lane l fetches the two 32-bit words 4(l%4) and 16+4(l%4)
+  // of that window in SNP row l/4, and the 16 rows x 32 bytes are transposed
+  // through 512 bytes of LDS per wave, after which each lane reads the bytes of
+  // its own individual in the 16 SNPs with one 128-bit load.  The window can
+  // run up to 31 bytes past a row's last genotype byte, into the next row or
+  // the BED's padding rows; those bytes belong to lanes past the last
+  // individual, whose sums are discarded.  The P column of SNP j+1 is loaded
+  // while SNP j is computed; the words of step js+16 while step js is.
+  __shared__ __align__(16) uint8_t tbuf[4][512];
+  uint8_t* __restrict__ tb = tbuf[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
+  const int64_t wrow0 = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63);
+  const bool wave_live = wrow0 < Ilen;  // wave-uniform
+  const int64_t ifirst = a.i0 + wrow0;
+  const int64_t wbase = (ifirst >> 2) & ~(int64_t)15;
+  const int bsel = (int)(((ifirst + lane) >> 2) - wbase);  // 0..31
+  const uint32_t shw = (uint32_t)((ifirst + lane) & 3) * 2u;
+  const uint8_t* __restrict__ wrow = a.bed + wbase + 4 * (lane & 3) + a.pitch * (lane >> 2);
+
+  int64_t js = jb;
+  if (wave_live && je - jb >= 16) {
+    uint32_t w0 = *reinterpret_cast<const uint32_t*>(wrow + a.pitch * jb);
+    uint32_t w1 = *reinterpret_cast<const uint32_t*>(wrow + a.pitch * jb + 16);
+    T pn[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) pn[k] = p[(int64_t)K * jb + k];
+    for (; js + 16 <= je; js += 16) {
+      // byte e of word w0 (w1) is window byte 4(l%4)+e (+16) of SNP row l/4 -> tb[byte][row]
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        tb[(4 * (lane & 3) + e) * 16 + (lane >> 2)] = (uint8_t)(w0 >> (8 * e));
+        tb[(16 + 4 * (lane & 3) + e) * 16 + (lane >> 2)] = (uint8_t)(w1 >> (8 * e));
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const u32x4 cur = *reinterpret_cast<const u32x4*>(tb + 16 * bsel);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (js + 32 <= je) {
+        w0 = *reinterpret_cast<const uint32_t*>(wrow + a.pitch * (js + 16));
+        w1 = *reinterpret_cast<const uint32_t*>(wrow + a.pitch * (js + 16) + 16);
+      }
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        T pv[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) pv[k] = pn[k];
+        const int64_t jn = (u < 15) ? js + u + 1 : ((js + 16 < je) ? js + 16 : js + 15);
+#pragma unroll
+        for (int k = 0; k < K; ++k) pn[k] = p[(int64_t)K * jn + k];
+        step((cur[u >> 2] >> (8 * (u & 3) + shw)) & 3u, pv);
+        if ((u & 7) == 7) renorm();
+      }
+    }
+  }
+  // The chunk's last < 16 SNPs: one byte per SNP (lanes past the last
+  // individual read individual i0's byte).
+  if (wave_live) {
+    const uint8_t* __restrict__ gp = a.bed + (i >> 2);
+    const uint32_t sh = (uint32_t)(i & 3) * 2u;
+    for (; js < je; ++js) {
+      T pv[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) pv[k] = p[(int64_t)K * js + k];
+      step((gp[a.pitch * js] >> sh) & 3u, pv);
+      renorm();
     }
   }
 
