@@ -7,7 +7,17 @@ extrapolation and the accept rule.  Workload (default): I=100,000
 individuals, J=100,000 SNPs, K=8, synthetic cohort (Q ~ Dirichlet(0.3),
 P ~ Beta(0.5,0.5) clipped to [0.01,0.99], 2% missing) generated on the device
 into the 2-bit BED (2.5 GB in HBM), random init + 5 FRAPPE-EM priming steps
-before timing.  value = I*J*steps / elapsed seconds (genotype-iterations/s).
+before timing.  value = I*J*steps / elapsed seconds (genotype-iterations/s)
+of the plain FP64 method.
+
+Time to converge: from the same random start, each form (plain FP64, and the
+BF16-matrix-core update passes under FP64 log-likelihood/QP/convergence) runs
+EM(5) priming plus admixture_qn! iterations; its time to converge is the wall
+time from the start of priming until the first iteration whose log-likelihood
+is within 1e-4 (relative) of LL*, the final log-likelihood of the plain FP64
+run (rtol 1e-7).  The CPU figure multiplies the oracle's measured seconds per
+EM step and per iteration (bounded sample, scaled to all SNPs) by the plain
+run's step counts.
 
 Prints one JSON line.  --gpus above 1 is not supported (one GPU per box).
 """
@@ -101,7 +111,9 @@ def cpu_baseline(wl, dtype, seed, budget_s=20.0):
     cores = host_cores()
     st = O.OracleAdmix(bed, pitch, I, Js, K, 3, x0, rng.random((N, 3)), rng.random((N, 3)), dtype=T,
                        nthreads=cores)
+    t0 = time.perf_counter()
     st.init_em(1)
+    em_s = time.perf_counter() - t0
     its, t0 = 0, time.perf_counter()
     while True:
         st.qn_iteration(its + 1)
@@ -116,7 +128,43 @@ def cpu_baseline(wl, dtype, seed, budget_s=20.0):
         "kind": "port",
         "sample": f"oracle (C/OpenMP restatement of src/loops.jl etc.), I={I}, first J={Js} SNPs of the same "
                   f"cohort, K={K}, {dtype}, {its} admixture_qn! iterations after 1 EM step, {el:.1f} s",
+        "em_step_seconds_full": em_s * wl["J"] / Js,
+        "iteration_seconds_full": el / its * wl["J"] / Js,
     }
+
+
+def converge_run(oa, ctx, wl, T, seed, form, max_iter, rtol=1e-7, stop_at=None):
+    """EM(5) priming + admixture_qn! iterations from the seeded random start with
+    the update passes in `form`; returns [(seconds since start, LL)] per step
+    (priming as one entry) and the number of QN iterations run.  With stop_at =
+    (LL*, tol) the run ends at the first iteration within tol of LL*."""
+    I, J, K = wl["I"], wl["J"], wl["K"]
+    ctx.set_precision("f64" if form == "plain" else form)
+    d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(seed + 1), T=T)
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    oa.init_em(d, None, 5, d_cu=ctx, log=None)
+    ctx.set_uv(d.U, d.V)
+    ctx.set_ll(d.ll_prev, d.ll_new)
+    hist = [(time.perf_counter() - t0, d.ll_new)]
+    its = 0
+    for it in range(1, max_iter + 1):
+        r = ctx.qn_iteration(it)
+        hist.append((time.perf_counter() - t0, r["new"]))
+        its = it
+        if r["reldiff"] < rtol:
+            break
+        if stop_at is not None and abs(r["new"] - stop_at[0]) <= stop_at[1] * abs(stop_at[0]):
+            break
+    return hist, its
+
+
+def first_within(hist, ll_star, tol):
+    """(index, seconds, LL) of the first entry of hist within tol of ll_star."""
+    for n, (t, ll) in enumerate(hist):
+        if abs(ll - ll_star) <= tol * abs(ll_star):
+            return n, t, ll
+    return None
 
 
 def calibrate_copy(nbytes=1 << 30):
@@ -151,6 +199,9 @@ def main():
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (counter scaling)")
     ap.add_argument("--target-waves", type=int, default=0, help="waves per genotype pass (0: library default)")
+    ap.add_argument("--no-ttc", action="store_true", help="skip the time-to-converge runs")
+    ap.add_argument("--ttc-forms", default="bf16", help="forms timed to convergence beside plain FP64")
+    ap.add_argument("--ttc-max-iter", type=int, default=200)
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -235,9 +286,44 @@ def main():
         "ll_last": lls[-1],
         "gen_seconds": t_gen,
     }
+    if not args.no_ttc and args.dtype == "f64":
+        tol = 1e-4
+        hist, its = converge_run(oa, ctx, wl, T, args.seed, "f64", args.ttc_max_iter)
+        ll_star = hist[-1][1]
+        ttc = {"criterion": f"|LL - LL*| <= {tol} |LL*|; LL* = final LL of the plain FP64 run (rtol 1e-7, "
+                            f"at most {args.ttc_max_iter} iterations)",
+               "ll_star": ll_star, "plain_iterations_run": its,
+               "plain_converged": its < args.ttc_max_iter}
+        hit = first_within(hist, ll_star, tol)
+        ttc["plain"] = {"seconds": hit[1], "qn_iterations": hit[0], "ll": hit[2],
+                        "ms_per_iteration": (hist[-1][0] - hist[0][0]) / max(its, 1) * 1e3,
+                        "priming_seconds": hist[0][0]}
+        for form in [f for f in args.ttc_forms.split(",") if f]:
+            h2, its2 = converge_run(oa, ctx, wl, T, args.seed, form, args.ttc_max_iter, stop_at=(ll_star, tol))
+            hit2 = first_within(h2, ll_star, tol)
+            ttc[form] = ({"seconds": hit2[1], "qn_iterations": hit2[0], "ll": hit2[2],
+                          "ms_per_iteration": (h2[-1][0] - h2[0][0]) / max(its2, 1) * 1e3,
+                          "priming_seconds": h2[0][0], "vs_plain_speedup": hit[1] / hit2[1]} if hit2 else
+                         {"seconds": None, "note": f"not within {tol} of LL* after {its2} iterations",
+                          "ll_last": h2[-1][1]})
+        ctx.set_precision("f64" if args.form == "plain" else args.form)
+        out["time_to_converge"] = ttc
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(wl, args.dtype, args.seed, args.cpu_budget)
-        out["gpu_over_cpu"] = value / out["cpu_baseline"]["value"]
+        cb = cpu_baseline(wl, args.dtype, args.seed, args.cpu_budget)
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        out["gpu_over_cpu"] = value / cb["value"]
+        if "time_to_converge" in out:
+            ttc = out["time_to_converge"]
+            n_it = ttc["plain"]["qn_iterations"]
+            cpu_s = 5 * cb["em_step_seconds_full"] + n_it * cb["iteration_seconds_full"]
+            ttc["cpu_extrapolated"] = {
+                "seconds": cpu_s,
+                "basis": f"oracle seconds per EM step and per iteration on {cb['cores']} host threads, scaled to "
+                         f"all {J} SNPs, x (5 EM steps + {n_it} iterations of the plain GPU run)"}
+            ttc["gpu_plain_vs_cpu"] = cpu_s / ttc["plain"]["seconds"]
+            for form in args.ttc_forms.split(","):
+                if form in ttc and ttc[form].get("seconds"):
+                    ttc[form]["vs_cpu"] = cpu_s / ttc[form]["seconds"]
     ctx.close()
     print(json.dumps(out), flush=True)
 
