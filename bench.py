@@ -48,21 +48,25 @@ WORKLOADS = {
     "c3": dict(I=100_000, J=100_000, K=8, alpha=0.3, p_dist="beta", p_lo=0.01, p_hi=0.99, miss_rate=0.02),
     "c2": dict(I=10_000, J=100_000, K=5, alpha=0.3, p_dist="uniform", p_lo=0.05, p_hi=0.95, miss_rate=0.02),
     "c1": dict(I=1_000, J=10_000, K=3, alpha=0.5, p_dist="uniform", p_lo=0.05, p_hi=0.95, miss_rate=0.01),
+    # K sweep K=2..12 with 5-fold CV on one cohort of true K=6
+    "c4": dict(I=20_000, J=200_000, K=6, alpha=0.5, p_dist="uniform", p_lo=0.05, p_hi=0.95, miss_rate=0.01),
 }
 
 
-KERNEL_SYMBOLS = {  # bench kernel name -> rocprof symbol prefix of its FP64 K=8 instance
+KERNEL_SYMBOLS = {  # bench kernel name -> rocprof symbol prefix of its K=8 instance
     "qside_update": "void admx::k_qside<double, 8, 0, true>",
     "pside_update": "void admx::k_pside<double, 8, 0>",
     "qside_ll": "void admx::k_qside<double, 8, 2, false>",
+    "qside_update_bf16": "void admx::k_qside_bf16<8>",
+    "pside_update_bf16": "void admx::k_pside_bf16<8>",
 }
 
 
-def measured_traffic(kernel: str, workload: str, dtype: str):
+def measured_traffic(kernel: str, workload: str, tag: str):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 FETCH_SIZE /
-    WRITE_SIZE summary of this workload (already scaled by the copy calibration);
-    None when no summary matches."""
-    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{dtype}.txt")
+    WRITE_SIZE summary of this workload and form (tag f64, f32 or bf16; already
+    scaled by the copy calibration); None when no summary matches."""
+    path = os.path.join(ROOT, "profiles", f"traffic_{workload}_{tag}.txt")
     sym = KERNEL_SYMBOLS.get(kernel)
     if sym is None or not os.path.exists(path):
         return None
@@ -167,6 +171,40 @@ def first_within(hist, ll_star, tol):
     return None
 
 
+def run_sweep(args):
+    """K = 2..12 sweep with 5-fold cross-validation on one device cohort; prints
+    one JSON line with the per-K time and CV error."""
+    import __graft_entry__ as g
+
+    oa = g.load_package()
+    from importlib import import_module
+
+    sim = import_module("openadmixture_jl_amd.sim")
+    wl = WORKLOADS[args.workload]
+    I, J, Kt = wl["I"], wl["J"], wl["K"]
+    src = oa.HIPAdmixData(I, J, Kt, 3)
+    sim.cohort_on_device(src, wl["alpha"], wl["p_dist"], wl["p_lo"], wl["p_hi"], wl["miss_rate"], seed=args.seed)
+    Ks = [int(k) for k in args.sweep_ks.split(",")] if args.sweep_ks else list(range(2, 13))
+    t0 = time.perf_counter()
+    res = oa.cv_sweep(src, Ks, nfolds=args.folds, fold_seed=args.seed, init_seed=args.seed + 1,
+                      max_iter=args.sweep_max_iter, precision="f64" if args.form == "plain" else args.form,
+                      log=lambda m: print(m, file=sys.stderr, flush=True))
+    el = time.perf_counter() - t0
+    src.close()
+    out = {
+        "metric": "K-sweep wall time and CV error (held-out binomial deviance per entry)",
+        "value": el, "unit": "s", "higher_is_better": False, "n_gpus": 1,
+        "dtype": "f64" if args.form == "plain" else f"{args.form} passes, f64 LL/QP",
+        "data": "synthetic cohort generated on device (Dirichlet Q, Uniform P, Binomial(2, q.p) genotypes)",
+        "config": {"workload": f"{args.workload}: I={I}, J={J}, true K={Kt}, K={Ks[0]}..{Ks[-1]}, {args.folds}-fold CV",
+                   "I": I, "J": J, "K_true": Kt, "folds": args.folds, "max_iter": args.sweep_max_iter},
+        "best_k": oa.best_k(res),
+        "per_k": [{k: r[k] for k in ("K", "seconds", "cv_error", "ll_full", "iterations_full", "converged_full",
+                                     "iterations_folds", "waves")} for r in res],
+    }
+    print(json.dumps(out), flush=True)
+
+
 def calibrate_copy(nbytes=1 << 30):
     """A plain device-to-device copy of known size (HIP runtime copy kernel), used
     to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -202,6 +240,9 @@ def main():
     ap.add_argument("--no-ttc", action="store_true", help="skip the time-to-converge runs")
     ap.add_argument("--ttc-forms", default="bf16", help="forms timed to convergence beside plain FP64")
     ap.add_argument("--ttc-max-iter", type=int, default=200)
+    ap.add_argument("--folds", type=int, default=5, help="CV folds of the K sweep (workload c4)")
+    ap.add_argument("--sweep-ks", default="", help="comma-separated K values of the sweep (default 2..12)")
+    ap.add_argument("--sweep-max-iter", type=int, default=300)
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -210,6 +251,10 @@ def main():
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
               "is a design note, not a measured path", file=sys.stderr)
         sys.exit(2)
+
+    if args.workload == "c4":
+        run_sweep(args)
+        return
 
     import __graft_entry__ as g
 
@@ -279,7 +324,8 @@ def main():
         "config": {"workload": f"{args.workload}: I={I}, J={J}, K={K}, Q=3, ZAL, {args.form} {args.dtype}",
                    "I": I, "J": J, "K": K, "genotypes": I * J, "bed_bytes": ((I + 3) // 4) * J},
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-                     "frac": achieved / peak, "traffic": measured_traffic(dom, args.workload, args.dtype),
+                     "frac": achieved / peak,
+                     "traffic": measured_traffic(dom, args.workload, args.dtype if args.form == "plain" else args.form),
                      "note": f"{FLOPS[base(dom)](K)} algorithmic flops/genotype x {I * J} genotypes per launch; "
                              f"{args.dtype} vector ALU (MI355X dense FP64 vector = matrix = 78.6 TF)"},
         "kernels": kernels,

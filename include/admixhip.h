@@ -78,6 +78,19 @@ int admx_download_bed(admx_ctx *ctx, uint8_t *bed);
  * counter hash.  No reference counterpart (workload generator). */
 int admx_generate_bed(admx_ctx *ctx, const double *Qt, const double *Pt, uint64_t seed, double miss_rate);
 
+/* ---- cross-validation holdout.  No reference counterpart (the reference has
+ * no cross-validation); these serve the K sweep with nfolds-fold CV.  Entry
+ * (i, j) belongs to fold floor(nfolds * hi32(smix(j*I + i ^ base(seed))) / 2^32),
+ * so the folds partition the genotype entries. */
+/* dst's genotypes = src's with the entries of fold `fold` set to missing;
+ * fold < 0 copies.  Same I, J and device; device-to-device. */
+int admx_bed_from(admx_ctx *dst, admx_ctx *src, int32_t fold, int32_t nfolds, uint64_t seed);
+/* Binomial deviance 2[g log(g/2s) + (2-g) log((2-g)/2(1-s))] of fit's state x
+ * (FP64 context) summed over the observed entries of fold `fold` of src's
+ * (unmasked) genotypes, and the number of those entries. */
+int admx_heldout_deviance(admx_ctx *fit, admx_ctx *src, int32_t fold, int32_t nfolds, uint64_t seed,
+                          double *deviance_sum, int64_t *count);
+
 /* ---- BED decode and counts (decode tables src/loops.jl:2-5).  No count
  * function exists in the reference; these are exact integer counts. */
 int admx_decode(admx_ctx *ctx, int8_t *g); /* I x J, -1 = missing */

@@ -11,6 +11,7 @@ interface for that path:
   init_em, admixture_qn              src/algorithms_outer.jl
   run_admixture                      src/driver.jl
   HIPAdmixData, hip_admixture_base   src/cuda/structs.jl (device mirror)
+  cv_sweep, autotune_waves           K sweep with n-fold CV (no reference counterpart)
 
 The directory name contains a dot, so load it by path, e.g.
 ``importlib.util.spec_from_file_location("openadmixture_jl_amd", ".../__init__.py",
@@ -19,12 +20,14 @@ submodule_search_locations=[...])``.
 from . import _native
 from ._native import AdmixHIPError, device_count
 from .algorithms import admixture_qn, em_p, em_q, init_em, loglikelihood, update_p, update_q
+from .cv import best_k, cv_sweep, fit_on_device
 from .driver import _admixture_base, hip_admixture_base, run_admixture
 from .hip import HIPAdmixData
 from .loops import (UnitRange, em_p_loop_skipmissing, em_q_loop_skipmissing, loglikelihood_loop_skipmissing,
                     update_p_loop_skipmissing, update_q_loop_skipmissing)
 from .plink import SnpArray, SnpLinAlg, write_plink
 from .structs import AdmixData
+from .tune import autotune_waves
 
 # Fail loudly at import when the native library is absent.
 _native.lib()
@@ -35,4 +38,5 @@ __all__ = [
     "loglikelihood_loop_skipmissing", "em_q_loop_skipmissing", "em_p_loop_skipmissing",
     "update_q_loop_skipmissing", "update_p_loop_skipmissing", "UnitRange",
     "SnpArray", "SnpLinAlg", "write_plink", "AdmixHIPError", "device_count",
+    "cv_sweep", "best_k", "fit_on_device", "autotune_waves",
 ]

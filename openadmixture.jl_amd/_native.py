@@ -33,6 +33,8 @@ SIGNATURES = {
     "admx_generate_bed": ([_vp, _vp, _vp, _u64, _dbl], _int),
     "admx_decode": ([_vp, _vp], _int),
     "admx_counts": ([_vp, _vp, _vp, _vp], _int),
+    "admx_bed_from": ([_vp, _vp, C.c_int32, C.c_int32, C.c_uint64], _int),
+    "admx_heldout_deviance": ([_vp, _vp, C.c_int32, C.c_int32, C.c_uint64, _vp, _vp], _int),
     "admx_loglikelihood_loop": ([_vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp], _int),
     "admx_em_q_loop": ([_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64], _int),
     "admx_em_p_loop": ([_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64], _int),

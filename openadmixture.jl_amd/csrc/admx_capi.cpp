@@ -687,6 +687,17 @@ struct Guard {
   explicit Guard(admx_ctx* c) : lk(c->mu) {}
 };
 
+// Both contexts' locks, taken in address order (or one lock when a == b).
+struct Guard2 {
+  std::unique_lock<std::mutex> l1, l2;
+  Guard2(admx_ctx* a, admx_ctx* b) {
+    admx_ctx* lo = a < b ? a : b;
+    admx_ctx* hi = a < b ? b : a;
+    l1 = std::unique_lock<std::mutex>(lo->mu);
+    if (hi != lo) l2 = std::unique_lock<std::mutex>(hi->mu);
+  }
+};
+
 }  // namespace
 
 // =============================================================================== C ABI
@@ -816,6 +827,63 @@ int admx_generate_bed(admx_ctx* c, const double* Qt, const double* Pt, uint64_t 
   CHK(check_launch());
   HIPCHK(hipStreamSynchronize(c->s));
   c->have_bed = true;
+  return ADMX_OK;
+}
+
+int admx_bed_from(admx_ctx* dst, admx_ctx* src, int32_t fold, int32_t nfolds, uint64_t seed) {
+  CHK(check_ctx(dst));
+  CHK(check_ctx(src));
+  if (dst == src) return bad("source and destination must be different contexts");
+  Guard2 g(dst, src);
+  CHK(need_bed(src));
+  if (dst->I != src->I || dst->J != src->J) return bad("contexts differ in I or J");
+  if (fold >= 0 && (nfolds < 2 || fold >= nfolds)) return bad("need 0 <= fold < nfolds and nfolds >= 2");
+  if (dst->device != src->device) return bad("contexts must live on the same device");
+  HIPCHK(hipStreamSynchronize(src->s));
+  HIPCHK(hipSetDevice(dst->device));
+  {
+    ProfScope ps(dst, "mask_fold");
+    launch_mask_fold(src->bed, dst->bed, dst->pitch, dst->I, dst->J, fold, fold >= 0 ? nfolds : 1, seed, dst->s);
+  }
+  CHK(check_launch());
+  CHK(pad_bed(dst));
+  HIPCHK(hipStreamSynchronize(dst->s));
+  dst->have_bed = true;
+  return ADMX_OK;
+}
+
+int admx_heldout_deviance(admx_ctx* fit, admx_ctx* src, int32_t fold, int32_t nfolds, uint64_t seed,
+                          double* deviance_sum, int64_t* count) {
+  CHK(check_ctx(fit));
+  CHK(check_ctx(src));
+  Guard2 g(fit, src);
+  CHK(need_bed(src));
+  if (!deviance_sum || !count) return bad("null output");
+  if (fit->I != src->I || fit->J != src->J) return bad("contexts differ in I or J");
+  if (fit->dtype != ADMX_F64) return bad("held-out deviance needs a FP64 fit context");
+  if (nfolds < 2 || fold < 0 || fold >= nfolds) return bad("need 0 <= fold < nfolds and nfolds >= 2");
+  if (fit->device != src->device) return bad("contexts must live on the same device");
+  if (src != fit) HIPCHK(hipStreamSynchronize(src->s));
+  HIPCHK(hipSetDevice(fit->device));
+  int64_t jchunk = 0;
+  const int64_t nb = heldout_dev_blocks(fit->I, fit->J, &jchunk);
+  CHK(fit->st[0].grow(sizeof(double) * 2 * nb + 2 * sizeof(double)));
+  double* part = static_cast<double*>(fit->st[0].p);
+  double* tot = part + 2 * nb;
+  const double* q = static_cast<const double*>(fit->x[ADMX_X]);
+  {
+    ProfScope ps(fit, "heldout_dev");
+    launch_heldout_dev(src->bed, src->pitch, fit->I, fit->J, (int)fit->K, q, q + fit->K * fit->I, fold, nfolds, seed,
+                       part, fit->s);
+    launch_reduce_sum(part, nb, tot, 0, fit->s);
+    launch_reduce_sum(part + nb, nb, tot + 1, 0, fit->s);
+  }
+  CHK(check_launch());
+  double h[2];
+  HIPCHK(hipMemcpyAsync(h, tot, sizeof(h), hipMemcpyDeviceToHost, fit->s));
+  HIPCHK(hipStreamSynchronize(fit->s));
+  *deviance_sum = h[0];
+  *count = (int64_t)h[1];
   return ADMX_OK;
 }
 

@@ -96,6 +96,11 @@ void launch_decode(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int8
 void launch_counts(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, unsigned long long* snp_nm,
                    unsigned long long* snp_alt, unsigned long long* ind_nm, hipStream_t s);
 // Synthetic genotypes from (Q,P) (double), seeded counter hash.
+void launch_mask_fold(const uint8_t* src, uint8_t* dst, int64_t pitch, int64_t I, int64_t J, int fold, int nfolds,
+                      uint64_t seed, hipStream_t s);
+int64_t heldout_dev_blocks(int64_t I, int64_t J, int64_t* jchunk);
+void launch_heldout_dev(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int K, const double* q,
+                        const double* p, int fold, int nfolds, uint64_t seed, double* part, hipStream_t s);
 void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, int K, uint64_t seed,
                     double miss_rate, uint8_t* bed, int64_t pitch, hipStream_t s);
 

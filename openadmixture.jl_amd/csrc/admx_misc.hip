@@ -361,4 +361,106 @@ void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, in
   hipLaunchKernelGGL(k_gen_bed, grid, dim3(256), 0, s, Qt, Pt, I, J, K, base, miss_rate, bed, pitch);
 }
 
+// ------------------------------------------------------------------ cross-validation folds
+// Fold of genotype entry (i, j) for an nfolds-fold holdout: the entry counter
+// j*I + i hashed with the fold seed, mapped to [0, nfolds) by a multiply-shift
+// of the top 32 hash bits.  Every entry belongs to exactly one fold.
+__device__ __forceinline__ int fold_of(uint64_t ctr, uint64_t base, int nfolds) {
+  return (int)(((smix(ctr ^ base) >> 32) * (uint64_t)nfolds) >> 32);
+}
+
+// dst = src with every entry of fold `fold` set to missing (01); fold < 0 is a
+// plain copy.  One lane per BED byte (4 individuals); both BEDs share the pitch.
+__global__ __launch_bounds__(256) void k_mask_fold(const uint8_t* __restrict__ src, uint8_t* __restrict__ dst,
+                                                   int64_t pitch, int64_t I, int64_t J, int fold, int nfolds,
+                                                   uint64_t base) {
+  const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (b >= pitch) return;
+  for (int64_t j = blockIdx.y; j < J; j += gridDim.y) {
+    uint32_t byte = src[pitch * j + b];
+    if (fold >= 0) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int64_t i = 4 * b + t;
+        if (i < I && fold_of((uint64_t)j * (uint64_t)I + (uint64_t)i, base, nfolds) == fold)
+          byte = (byte & ~(3u << (2 * t))) | (1u << (2 * t));
+      }
+    }
+    dst[pitch * j + b] = (uint8_t)byte;
+  }
+}
+
+// Held-out binomial deviance of the fit (q, p) on the observed entries of fold
+// `fold` of the unmasked BED:
+//   dev = 2 [ g log(g / 2s) + (2-g) log((2-g) / 2(1-s)) ],  0 log 0 = 0,
+// summed in FP64 with the entry count.  One lane per individual, a block per
+// (256 individuals, SNP chunk); per-block partials [2][nblocks].
+__global__ __launch_bounds__(256) void k_heldout_dev(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I,
+                                                     int64_t J, int K, const double* __restrict__ q,
+                                                     const double* __restrict__ p, int64_t jchunk, int fold,
+                                                     int nfolds, uint64_t base, double* __restrict__ part) {
+  __shared__ double red[256];
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t jb = (int64_t)blockIdx.y * jchunk;
+  const int64_t je = (jb + jchunk < J) ? jb + jchunk : J;
+  double dev = 0.0, cnt = 0.0;
+  if (i < I) {
+    const uint32_t sh = (uint32_t)(i & 3) * 2u;
+    for (int64_t j = jb; j < je; ++j) {
+      const uint32_t c = (bed[pitch * j + (i >> 2)] >> sh) & 3u;
+      if (c == 1u || fold_of((uint64_t)j * (uint64_t)I + (uint64_t)i, base, nfolds) != fold) continue;
+      double s = 0.0;
+      for (int k = 0; k < K; ++k) s = __builtin_fma(q[k + (int64_t)K * i], p[k + (int64_t)K * j], s);
+      const int g = code_g(c);
+      double d = 0.0;
+      if (g > 0) d += (double)g * log((double)g / (2.0 * s));
+      if (g < 2) d += (double)(2 - g) * log((double)(2 - g) / (2.0 * (1.0 - s)));
+      dev += 2.0 * d;
+      cnt += 1.0;
+    }
+  }
+  const int64_t nb = (int64_t)gridDim.x * gridDim.y;
+  const int64_t blk = (int64_t)blockIdx.y * gridDim.x + blockIdx.x;
+  const double tdev = block_sum<256>(dev, red);
+  const double tcnt = block_sum<256>(cnt, red);
+  if (threadIdx.x == 0) {
+    part[blk] = tdev;
+    part[nb + blk] = tcnt;
+  }
+}
+
+static uint64_t fold_base(uint64_t seed) {
+  uint64_t z = seed ^ 0x2545F4914F6CDD1DULL;  // smix on the host, as on the device
+  z += 0x9E3779B97F4A7C15ULL;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
+  return z ^ (z >> 31);
+}
+
+void launch_mask_fold(const uint8_t* src, uint8_t* dst, int64_t pitch, int64_t I, int64_t J, int fold, int nfolds,
+                      uint64_t seed, hipStream_t s) {
+  dim3 grid((unsigned)((pitch + 255) / 256), (unsigned)(J < 65535 ? J : 65535));
+  hipLaunchKernelGGL(k_mask_fold, grid, dim3(256), 0, s, src, dst, pitch, I, J, fold, nfolds, fold_base(seed));
+}
+
+int64_t heldout_dev_blocks(int64_t I, int64_t J, int64_t* jchunk) {
+  const int64_t bx = (I + 255) / 256;
+  int64_t ny = (16384 + bx - 1) / bx;
+  if (ny > J) ny = J;
+  if (ny < 1) ny = 1;
+  *jchunk = (J + ny - 1) / ny;
+  ny = (J + *jchunk - 1) / *jchunk;
+  return bx * ny;
+}
+
+void launch_heldout_dev(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int K, const double* q,
+                        const double* p, int fold, int nfolds, uint64_t seed, double* part, hipStream_t s) {
+  int64_t jchunk = 0;
+  const int64_t nb = heldout_dev_blocks(I, J, &jchunk);
+  const int64_t bx = (I + 255) / 256;
+  dim3 grid((unsigned)bx, (unsigned)(nb / bx));
+  hipLaunchKernelGGL(k_heldout_dev, grid, dim3(256), 0, s, bed, pitch, I, J, K, q, p, jchunk, fold, nfolds,
+                     fold_base(seed), part);
+}
+
 }  // namespace admx

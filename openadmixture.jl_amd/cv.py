@@ -1,0 +1,92 @@
+"""Uneven K sweep with n-fold cross-validation on one device-resident cohort.
+
+The reference fits one K per ``run_admixture`` call (src/driver.jl:35-75) and
+has no cross-validation; this module is the batched sweep built on the same
+device primitives.  For every K it fits the full data and each of the
+``nfolds`` holdout problems: fold f is the cohort with the entries of fold f
+(a seeded hash partition of all genotype entries, ~1/nfolds each) set to
+missing, fitted by EM priming + admixture_qn! like the full problem.  The CV
+error of K is the held-out binomial deviance per observed held-out entry,
+
+    CV(K) = sum_f sum_{(i,j) in fold f, observed} 2[g log(g/2s) + (2-g) log((2-g)/2(1-s))]
+            / (number of such entries),   s = q_i . p_j of fold f's fit,
+
+which is ADMIXTURE's deviance-residual CV criterion.  One context per K holds
+that K's state; the fold genotypes are re-derived on the device from the
+source context's BED (``admx_bed_from``), so the cohort is read from the host
+once.
+"""
+from __future__ import annotations
+
+import time
+
+import numpy as np
+
+from . import _native as N
+from .algorithms import admixture_qn, init_em
+from .hip import HIPAdmixData
+from .structs import AdmixData
+from .tune import autotune_waves
+
+
+def fit_on_device(ctx: HIPAdmixData, K: int, seed: int, em_iters: int = 5, max_iter: int = 1000,
+                  rtol: float = 1e-7, Q: int = 3):
+    """EM priming + admixture_qn! on ctx's genotypes from a seeded random start;
+    returns the host AdmixData (q, p, LL history, iteration count)."""
+    d = AdmixData(ctx.I, ctx.J, K, Q, rng=np.random.default_rng(seed), T=ctx.T)
+    init_em(d, None, em_iters, d_cu=ctx, log=None)
+    admixture_qn(d, None, max_iter, rtol, d_cu=ctx, log=None)
+    return d
+
+
+def cv_sweep(src: HIPAdmixData, Ks, nfolds: int = 5, fold_seed: int = 0, init_seed: int = 1, em_iters: int = 5,
+             max_iter: int = 1000, rtol: float = 1e-7, precision: str = "f64", Q: int = 3, autotune: bool = True,
+             log=None):
+    """Fit K in Ks on src's genotypes, each with nfolds holdout problems.
+
+    Returns one dict per K: K, seconds (all nfolds + 1 fits of that K), the full
+    fit's final LL and iteration count, the CV error and per-fold deviances,
+    and the chosen waves per pass."""
+    if nfolds < 2:
+        raise ValueError("nfolds must be >= 2")
+    out = []
+    for K in Ks:
+        t0 = time.perf_counter()
+        ctx = HIPAdmixData(src.I, src.J, int(K), Q, src.T)
+        try:
+            ctx.set_precision(precision)
+            ctx.bed_from(src, -1)
+            waves = None
+            if autotune:
+                ctx.set_x(N.ADMX_X, AdmixData(src.I, src.J, K, Q, rng=np.random.default_rng(init_seed), T=src.T).x)
+                waves = autotune_waves(ctx)
+            full = fit_on_device(ctx, K, init_seed + 1000 * K, em_iters, max_iter, rtol, Q)
+            devs, cnts, iters = [], [], []
+            for f in range(nfolds):
+                ctx.bed_from(src, f, nfolds, fold_seed)
+                d = fit_on_device(ctx, K, init_seed + 1000 * K + 1 + f, em_iters, max_iter, rtol, Q)
+                dev, cnt = ctx.heldout_deviance(src, f, nfolds, fold_seed)
+                devs.append(dev)
+                cnts.append(cnt)
+                iters.append(d.iterations)
+            ctx.synchronize()
+        finally:
+            ctx.close()
+        el = time.perf_counter() - t0
+        rec = dict(K=int(K), seconds=el, ll_full=float(full.ll_new), iterations_full=int(full.iterations),
+                   converged_full=bool(full.converged), iterations_folds=iters,
+                   cv_error=float(np.sum(devs) / max(np.sum(cnts), 1)),
+                   fold_deviance=[float(v) for v in devs], fold_entries=[int(v) for v in cnts], waves=waves)
+        out.append(rec)
+        if log:
+            log(f"K={K}: CV error {rec['cv_error']:.6f}, full LL {rec['ll_full']:.6e}, "
+                f"{rec['iterations_full']} iterations, {el:.2f} s")
+    return out
+
+
+def best_k(results):
+    """K with the smallest CV error."""
+    return min(results, key=lambda r: r["cv_error"])["K"]
+
+
+__all__ = ["cv_sweep", "fit_on_device", "best_k"]

@@ -96,6 +96,20 @@ class HIPAdmixData:
         N.check(N.lib().admx_counts(self._h, N.ptr(snm), N.ptr(salt), N.ptr(inm)))
         return snm, salt, inm
 
+    # ------------------------------------------------------------ cross-validation holdout
+    def bed_from(self, src: "HIPAdmixData", fold: int = -1, nfolds: int = 5, seed: int = 0):
+        """Genotypes of `src` (same I, J), with the entries of `fold` set to
+        missing (fold < 0: plain copy); device to device."""
+        N.check(N.lib().admx_bed_from(self._h, src.handle, int(fold), int(nfolds), C.c_uint64(seed)))
+
+    def heldout_deviance(self, src: "HIPAdmixData", fold: int, nfolds: int = 5, seed: int = 0):
+        """(deviance sum, entry count) of this context's state x on the observed
+        entries of `fold` of src's genotypes."""
+        dev, cnt = C.c_double(), C.c_int64()
+        N.check(N.lib().admx_heldout_deviance(self._h, src.handle, int(fold), int(nfolds), C.c_uint64(seed),
+                                              C.byref(dev), C.byref(cnt)))
+        return dev.value, cnt.value
+
     # ------------------------------------------------------------ state
     def set_x(self, which: int, x):
         x = _f(x, self.T, (self.K, self.I + self.J))
@@ -253,6 +267,7 @@ class HIPAdmixData:
         """'f64' (plain), 'f32' or 'bf16' for the device-resident update passes."""
         codes = {"f64": N.ADMX_PREC_F64, "f32": N.ADMX_PREC_F32, "bf16": N.ADMX_PREC_BF16}
         self.set_tuning(N.ADMX_TUNE_PRECISION, codes[mode] if isinstance(mode, str) else int(mode))
+        self._precision = mode
 
     def set_tuning(self, key: int, value: int):
         N.check(N.lib().admx_set_tuning(self._h, key, int(value)))

@@ -133,6 +133,52 @@ void orc_gen_bed(const double *Qt, const double *Pt, int64_t I, int64_t J, int64
   }
 }
 
+/* ---- Cross-validation holdout (not in the reference, which has no CV; the
+ * project's definition, parity unpinned against the reference).  Entry (i, j)
+ * belongs to fold floor(nfolds * hi32(smix(j*I + i ^ base)) / 2^32) with
+ * base = smix(seed ^ 0x2545F4914F6CDD1D). */
+int orc_fold_of(int64_t i, int64_t j, int64_t I, uint64_t seed, int nfolds) {
+  const uint64_t base = orc_smix(seed ^ 0x2545F4914F6CDD1DULL);
+  const uint64_t ctr = (uint64_t)j * (uint64_t)I + (uint64_t)i;
+  return (int)(((orc_smix(ctr ^ base) >> 32) * (uint64_t)nfolds) >> 32);
+}
+
+/* dst = src with the entries of `fold` set to missing (01). */
+void orc_mask_fold(const uint8_t *src, uint8_t *dst, int64_t pitch, int64_t I, int64_t J, int fold, int nfolds,
+                   uint64_t seed) {
+  memcpy(dst, src, (size_t)(pitch * J));
+  for (int64_t j = 0; j < J; ++j)
+    for (int64_t i = 0; i < I; ++i)
+      if (orc_fold_of(i, j, I, seed, nfolds) == fold) {
+        uint8_t *b = dst + (i >> 2) + pitch * j;
+        const int sh = (int)((i & 3) << 1);
+        *b = (uint8_t)((*b & ~(3 << sh)) | (1 << sh));
+      }
+}
+
+/* Held-out binomial deviance of (q, p) over the observed entries of `fold`:
+ * sum 2[g log(g/2s) + (2-g) log((2-g)/2(1-s))] with 0 log 0 = 0, and the count. */
+double orc_heldout_deviance(const uint8_t *bed, int64_t pitch, const double *q, const double *p, int64_t I,
+                            int64_t J, int64_t K, int fold, int nfolds, uint64_t seed, int64_t *count) {
+  double dev = 0.0;
+  int64_t n = 0;
+  for (int64_t j = 0; j < J; ++j)
+    for (int64_t i = 0; i < I; ++i) {
+      const int c = orc_code(bed, pitch, i, j);
+      if (c == 1 || orc_fold_of(i, j, I, seed, nfolds) != fold) continue;
+      double s = 0.0;
+      for (int64_t k = 0; k < K; ++k) s += q[k + K * i] * p[k + K * j];
+      const double g = G_MAP[c];
+      double d = 0.0;
+      if (g > 0.0) d += g * log(g / (2.0 * s));
+      if (g < 2.0) d += (2.0 - g) * log((2.0 - g) / (2.0 * (1.0 - s)));
+      dev += 2.0 * d;
+      ++n;
+    }
+  *count = n;
+  return dev;
+}
+
 /* ---- Per-precision bodies.  TILE_MAXITER = tile_maxiter(T): 512 bytes / sizeof(T)
  * (src/loops.jl:1, 7-10). */
 #define T double

@@ -46,6 +46,12 @@ def _declare(L):
     L.orc_decode.argtypes = [c_vp, c_i64, c_i64, c_i64, c_vp]
     L.orc_counts.argtypes = [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp, c_vp]
     L.orc_gen_bed.argtypes = [c_vp, c_vp, c_i64, c_i64, c_i64, C.c_uint64, c_dbl, c_vp, c_i64]
+    L.orc_fold_of.argtypes = [c_i64, c_i64, c_i64, C.c_uint64, C.c_int]
+    L.orc_fold_of.restype = C.c_int
+    L.orc_mask_fold.argtypes = [c_vp, c_vp, c_i64, c_i64, c_i64, C.c_int, C.c_int, C.c_uint64]
+    L.orc_heldout_deviance.argtypes = [c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, C.c_int, C.c_int, C.c_uint64,
+                                       c_vp]
+    L.orc_heldout_deviance.restype = c_dbl
     for s in ("_f64", "_f32"):
         getattr(L, "orc_loglikelihood" + s).argtypes = [c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, C.c_int]
         getattr(L, "orc_loglikelihood" + s).restype = c_dbl
@@ -112,6 +118,26 @@ def gen_bed(Qt, Pt, seed, miss_rate, pitch=None):
     bed = np.zeros(pitch * J, dtype=np.uint8)
     lib().orc_gen_bed(_p(Qt), _p(Pt), I, J, K, C.c_uint64(seed), miss_rate, _p(bed), pitch)
     return bed
+
+
+# ---------------------------------------------------------------- cross-validation holdout
+def fold_of(i, j, I, seed, nfolds):
+    return lib().orc_fold_of(int(i), int(j), int(I), C.c_uint64(seed), int(nfolds))
+
+
+def mask_fold(bed, pitch, I, J, fold, nfolds, seed):
+    out = np.empty_like(bed)
+    lib().orc_mask_fold(_p(bed), _p(out), pitch, I, J, int(fold), int(nfolds), C.c_uint64(seed))
+    return out
+
+
+def heldout_deviance(bed, pitch, q, p, fold, nfolds, seed):
+    K, I = q.shape
+    J = p.shape[1]
+    cnt = C.c_int64()
+    dev = lib().orc_heldout_deviance(_p(bed), pitch, _p(_chk(q, np.float64)), _p(_chk(p, np.float64)), I, J, K,
+                                     int(fold), int(nfolds), C.c_uint64(seed), C.byref(cnt))
+    return dev, cnt.value
 
 
 # ---------------------------------------------------------------- tile loops
