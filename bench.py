@@ -238,6 +238,7 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (counter scaling)")
     ap.add_argument("--target-waves", type=int, default=0, help="waves per genotype pass (0: library default)")
     ap.add_argument("--no-ttc", action="store_true", help="skip the time-to-converge runs")
+    ap.add_argument("--graphs", action="store_true", help="replay steady-state iterations from hipGraphs")
     ap.add_argument("--ttc-forms", default="bf16", help="forms timed to convergence beside plain FP64")
     ap.add_argument("--ttc-max-iter", type=int, default=200)
     ap.add_argument("--folds", type=int, default=5, help="CV folds of the K sweep (workload c4)")
@@ -269,6 +270,8 @@ def main():
     ctx = oa.HIPAdmixData(I, J, K, 3, T)
     if args.target_waves:
         ctx.set_tuning(oa._native.ADMX_TUNE_TARGET_WAVES, args.target_waves)
+    if args.graphs:
+        ctx.set_graphs(True)
     if args.form != "plain":
         if args.dtype != "f64":
             sys.exit("--form f32/bf16 runs on a FP64 context (FP64 log-likelihood, QP and convergence test)")

@@ -49,6 +49,8 @@ extern "C" {
 #define ADMX_TUNE_TARGET_WAVES 0 /* waves per genotype pass (sets the chunk count) */
 #define ADMX_TUNE_KEEP_HESSIANS 1 /* 1: update_q/update_p also store XtX and Xtz */
 #define ADMX_TUNE_PRECISION 2     /* FP64 contexts: precision of the device-resident update passes */
+#define ADMX_TUNE_GRAPHS 3        /* 1: admx_qn_iteration replays steady-state iterations (iter > Q) from
+                                   * captured hipGraphs (not while profiling); results are identical */
 
 /* values of ADMX_TUNE_PRECISION.  Log-likelihood, QP, projections, quasi-Newton
  * and the convergence test stay FP64 in every mode. */

@@ -17,7 +17,7 @@ ADMX_OK, ADMX_EINVAL, ADMX_EHIP, ADMX_ESTATE = 0, 1, 2, 3
 ADMX_F64, ADMX_F32 = 0, 1
 ADMX_X, ADMX_X_NEXT, ADMX_X_NEXT2, ADMX_X_TMP = 0, 1, 2, 3
 ADMX_SIDE_Q, ADMX_SIDE_P = 0, 1
-ADMX_TUNE_TARGET_WAVES, ADMX_TUNE_KEEP_HESSIANS, ADMX_TUNE_PRECISION = 0, 1, 2
+ADMX_TUNE_TARGET_WAVES, ADMX_TUNE_KEEP_HESSIANS, ADMX_TUNE_PRECISION, ADMX_TUNE_GRAPHS = 0, 1, 2, 3
 ADMX_PREC_F64, ADMX_PREC_F32, ADMX_PREC_BF16 = 0, 1, 2
 
 # Every exported symbol with its C signature (return type is int unless noted).

@@ -97,6 +97,18 @@ struct admx_ctx {
   DevBuf fq, fp, ppt, mmt, qqt;  // mixed-precision pass inputs
   std::mutex mu;
   Prof prof;
+  // hipGraph replay of steady-state quasi-Newton iterations (iter > Q): one
+  // executable graph per (history column, fix_q, fix_p), valid while the
+  // buffer layout signature graph_sig holds; graph_warm has bit (fix_q | 2 fix_p)
+  // set once an eager iteration of that kind ran under the signature.
+  int use_graphs = 0;
+  struct GraphEntry {
+    int64_t key;
+    hipGraphExec_t exec;
+  };
+  std::vector<GraphEntry> graphs;
+  uint64_t graph_sig = 0;
+  unsigned graph_warm = 0;
 
   int64_t N() const { return K * (I + J); }
   void* qp(void* v) const { return v; }
@@ -449,8 +461,10 @@ int dev_zal(admx_ctx* c, int64_t iter, const void* x, const void* xn, const void
   return check_launch();
 }
 
+// Everything of one admixture_qn! iteration up to the accept kernel, enqueued
+// on the context stream without host synchronisation (capturable).
 template <typename T>
-int dev_qn_iteration(admx_ctx* c, int64_t iter, int fix_q, int fix_p, double* out) {
+int enqueue_qn_iteration(admx_ctx* c, int64_t iter, int fix_q, int fix_p) {
   const size_t qbytes = (size_t)(c->K * c->I) * c->esz, pbytes = (size_t)(c->K * c->J) * c->esz;
   void* X = c->x[ADMX_X];
   void* XN = c->x[ADMX_X_NEXT];
@@ -472,7 +486,73 @@ int dev_qn_iteration(admx_ctx* c, int64_t iter, int fix_q, int fix_p, double* ou
     ProfScope ps(c, "accept");
     launch_accept<T>(c->N(), X, XT, XN2, c->lls, c->s);
   }
-  CHK(check_launch());
+  return check_launch();
+}
+
+// Signature of everything a captured iteration bakes in: scratch buffer
+// addresses and sizes and the tuning that shapes the launches.
+uint64_t layout_sig(const admx_ctx* c) {
+  uint64_t h = 1469598103934665603ULL;
+  auto mix = [&](uint64_t v) {
+    h ^= v;
+    h *= 1099511628211ULL;
+  };
+  const DevBuf* bufs[] = {&c->part, &c->llpart, &c->zal, &c->fq, &c->fp, &c->ppt, &c->mmt, &c->qqt};
+  for (const DevBuf* b : bufs) {
+    mix((uint64_t)(uintptr_t)b->p);
+    mix((uint64_t)b->n);
+  }
+  mix((uint64_t)c->target_waves);
+  mix((uint64_t)c->keep_hess);
+  mix((uint64_t)c->precision);
+  return h;
+}
+
+void drop_graphs(admx_ctx* c) {
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
+  c->graph_warm = 0;
+}
+
+template <typename T>
+int dev_qn_iteration(admx_ctx* c, int64_t iter, int fix_q, int fix_p, double* out) {
+  const unsigned kind = (unsigned)(fix_q ? 1 : 0) | (fix_p ? 2u : 0u);
+  if (layout_sig(c) != c->graph_sig) {
+    drop_graphs(c);
+    c->graph_sig = layout_sig(c);
+  }
+  if (!c->use_graphs || c->prof.on || iter <= c->Q || !(c->graph_warm & (1u << kind))) {
+    CHK(enqueue_qn_iteration<T>(c, iter, fix_q, fix_p));
+    // scratch buffers may have grown in this eager run: graphs need a run
+    // under an unchanged signature before capture (no allocation is captured)
+    if (layout_sig(c) != c->graph_sig) {
+      drop_graphs(c);
+      c->graph_sig = layout_sig(c);
+    } else {
+      c->graph_warm |= 1u << kind;
+    }
+  } else {
+    const int64_t key = ((iter - 1) % c->Q) | ((int64_t)kind << 8);
+    hipGraphExec_t exec = nullptr;
+    for (auto& g : c->graphs)
+      if (g.key == key) exec = g.exec;
+    if (!exec) {
+      HIPCHK(hipStreamBeginCapture(c->s, hipStreamCaptureModeThreadLocal));
+      const int rc = enqueue_qn_iteration<T>(c, iter, fix_q, fix_p);
+      hipGraph_t graph = nullptr;
+      const hipError_t e = hipStreamEndCapture(c->s, &graph);
+      if (rc != ADMX_OK) {
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc;
+      }
+      if (e != hipSuccess) return fail(ADMX_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(e));
+      const hipError_t e2 = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (e2 != hipSuccess) return fail(ADMX_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e2));
+      c->graphs.push_back({key, exec});
+    }
+    HIPCHK(hipGraphLaunch(exec, c->s));
+  }
   HIPCHK(hipMemcpyAsync(c->h_lls, c->lls, sizeof(double) * LL_NSLOTS, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   out[0] = c->h_lls[LL_PREV];
@@ -769,6 +849,8 @@ int admx_destroy(admx_ctx* c) {
     (void)hipEventDestroy(e.b);
   }
   for (auto e : c->prof.pool) (void)hipEventDestroy(e);
+  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+  c->graphs.clear();
   void* bufs[] = {c->bed, c->x[0], c->x[1], c->x[2], c->x[3], c->XtX_q, c->Xtz_q, c->XtX_p,
                   c->Xtz_p, c->U,   c->V,    c->lls,  c->status};
   for (void* b : bufs)
@@ -1198,7 +1280,9 @@ int admx_set_tuning(admx_ctx* c, int key, int64_t value) {
     if (value < ADMX_PREC_F64 || value > ADMX_PREC_BF16) return bad("precision must be ADMX_PREC_F64/F32/BF16");
     if (value != ADMX_PREC_F64 && c->dtype != ADMX_F64) return bad("mixed precision needs a FP64 context");
     c->precision = (int)value;
-
+  } else if (key == ADMX_TUNE_GRAPHS) {
+    c->use_graphs = value != 0;
+    if (!c->use_graphs) drop_graphs(c);
   } else {
     return bad("unknown tuning key");
   }

@@ -269,5 +269,9 @@ class HIPAdmixData:
         self.set_tuning(N.ADMX_TUNE_PRECISION, codes[mode] if isinstance(mode, str) else int(mode))
         self._precision = mode
 
+    def set_graphs(self, on: bool = True):
+        """Replay steady-state qn_iteration calls from captured hipGraphs."""
+        self.set_tuning(N.ADMX_TUNE_GRAPHS, int(bool(on)))
+
     def set_tuning(self, key: int, value: int):
         N.check(N.lib().admx_set_tuning(self._h, key, int(value)))
