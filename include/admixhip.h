@@ -93,6 +93,31 @@ int admx_bed_from(admx_ctx *dst, admx_ctx *src, int32_t fold, int32_t nfolds, ui
 int admx_heldout_deviance(admx_ctx *fit, admx_ctx *src, int32_t fold, int32_t nfolds, uint64_t seed,
                           double *deviance_sum, int64_t *count);
 
+/* ---- learned warm start.  No reference counterpart (the reference starts
+ * from a random point plus FRAPPE EM, src/algorithms_outer.jl:12-45).
+ * Features: per individual, (1/sqrt(Jp)) sum_{j < Jp observed} (g_ij - mu_j) isd_j R[j, :]
+ * over the first Jp SNPs of a reference panel (R: Jp x 64 row-major, FP32).
+ * Network: FP32 MLP in -> hidden -> hidden -> out, ReLU, softmax head, trained
+ * by Adam on the soft-target cross entropy; parameters flat
+ * [W1 (hidden x in), b1, W2, b2, W3 (out x hidden), b3], row-major arrays. */
+typedef struct admx_mlp admx_mlp;
+int admx_mlp_create(admx_mlp **out, int32_t in_dim, int32_t hidden, int32_t out_dim, uint64_t seed);
+int admx_mlp_destroy(admx_mlp *m);
+int admx_mlp_num_params(admx_mlp *m, int64_t *n);
+int admx_mlp_get_params(admx_mlp *m, float *w);
+int admx_mlp_set_params(admx_mlp *m, const float *w); /* also resets the Adam state */
+/* mean loss and its gradient (flat, like the parameters) over B rows X (B x in), T (B x out) */
+int admx_mlp_loss_grad(admx_mlp *m, const float *X, const float *T, int64_t B, float *loss, float *grad);
+int admx_mlp_predict(admx_mlp *m, const float *X, int64_t n, float *Y); /* Y: n x out probabilities */
+/* Adam (lr, 0.9, 0.999, 1e-8) over shuffled batches; epoch_loss[epochs] may be NULL */
+int admx_mlp_train(admx_mlp *m, const float *X, const float *T, int64_t n, int32_t epochs, int32_t batch, float lr,
+                   uint64_t seed, float *epoch_loss);
+/* features of ctx's genotypes into ctx (kept for admx_warm_q); F (I x 64) may be NULL */
+int admx_features(admx_ctx *ctx, const float *R, const float *mu, const float *isd, int64_t Jp, float *F);
+/* q of the state x <- simplex projection (pseudocount 1e-5) of the network's
+ * output on ctx's features; FP64 context, network out == K, in == 64 */
+int admx_warm_q(admx_ctx *ctx, admx_mlp *m);
+
 /* ---- BED decode and counts (decode tables src/loops.jl:2-5).  No count
  * function exists in the reference; these are exact integer counts. */
 int admx_decode(admx_ctx *ctx, int8_t *g); /* I x J, -1 = missing */

@@ -12,6 +12,7 @@ interface for that path:
   run_admixture                      src/driver.jl
   HIPAdmixData, hip_admixture_base   src/cuda/structs.jl (device mirror)
   cv_sweep, autotune_waves           K sweep with n-fold CV (no reference counterpart)
+  MLP, Panel, warm_started_fit       learned warm start (no reference counterpart)
 
 The directory name contains a dot, so load it by path, e.g.
 ``importlib.util.spec_from_file_location("openadmixture_jl_amd", ".../__init__.py",
@@ -28,6 +29,7 @@ from .loops import (UnitRange, em_p_loop_skipmissing, em_q_loop_skipmissing, log
 from .plink import SnpArray, SnpLinAlg, write_plink
 from .structs import AdmixData
 from .tune import autotune_waves
+from .warmstart import MLP, Panel, training_set, warm_started_fit, world_frequencies
 
 # Fail loudly at import when the native library is absent.
 _native.lib()
@@ -39,4 +41,5 @@ __all__ = [
     "update_q_loop_skipmissing", "update_p_loop_skipmissing", "UnitRange",
     "SnpArray", "SnpLinAlg", "write_plink", "AdmixHIPError", "device_count",
     "cv_sweep", "best_k", "fit_on_device", "autotune_waves",
+    "MLP", "Panel", "training_set", "warm_started_fit", "world_frequencies",
 ]

@@ -34,6 +34,16 @@ SIGNATURES = {
     "admx_decode": ([_vp, _vp], _int),
     "admx_counts": ([_vp, _vp, _vp, _vp], _int),
     "admx_bed_from": ([_vp, _vp, C.c_int32, C.c_int32, C.c_uint64], _int),
+    "admx_mlp_create": ([_vp, C.c_int32, C.c_int32, C.c_int32, C.c_uint64], _int),
+    "admx_mlp_destroy": ([_vp], _int),
+    "admx_mlp_num_params": ([_vp, _vp], _int),
+    "admx_mlp_get_params": ([_vp, _vp], _int),
+    "admx_mlp_set_params": ([_vp, _vp], _int),
+    "admx_mlp_loss_grad": ([_vp, _vp, _vp, _i64, _vp, _vp], _int),
+    "admx_mlp_predict": ([_vp, _vp, _i64, _vp], _int),
+    "admx_mlp_train": ([_vp, _vp, _vp, _i64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _vp], _int),
+    "admx_features": ([_vp, _vp, _vp, _vp, _i64, _vp], _int),
+    "admx_warm_q": ([_vp, _vp], _int),
     "admx_heldout_deviance": ([_vp, _vp, C.c_int32, C.c_int32, C.c_uint64, _vp, _vp], _int),
     "admx_loglikelihood_loop": ([_vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp], _int),
     "admx_em_q_loop": ([_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64], _int),

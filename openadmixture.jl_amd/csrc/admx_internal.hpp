@@ -8,6 +8,7 @@ namespace admx {
 
 constexpr int kMaxK = 16;  // populations supported (templated kernels)
 constexpr int kMaxQ = 8;   // quasi-Newton history columns supported on device
+constexpr int kFeat = 64;  // genotype-projection feature dimension of the warm start
 
 enum PassMode : int { MODE_UPDATE = 0, MODE_EM = 1, MODE_LL = 2 };
 
@@ -106,5 +107,18 @@ void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, in
 
 // Slots of the device log-likelihood vector.
 enum LLSlot : int { LL_PREV = 0, LL_NEW = 1, LL_BASIC = 2, LL_QN = 3, LL_TMP = 4, LL_NSLOTS = 8 };
+
+// learned warm start (admx_mlp.hip)
+void launch_features(const uint8_t* bed, int64_t pitch, int64_t I, int64_t Jp, const float* R, const float* mu,
+                     const float* isd, int nchunk, int64_t jchunk, float* part, float* F, hipStream_t s);
+void launch_sgemm(bool ta, bool tb, int M, int N, int Kd, const float* A, int lda, const float* B, int ldb, float* C,
+                  int ldc, const float* bias, bool relu, const float* mask, bool accumulate, hipStream_t s);
+void launch_colsum(const float* X, int M, int N, float* out, hipStream_t s);
+void launch_softmax_ce(const float* Z, int M, int K, const float* T, float* Y, float* dZ, float* rowloss,
+                       hipStream_t s);
+void launch_gather_rows(const float* X, int ld, const int32_t* idx, int n, float* Y, hipStream_t s);
+void launch_adam(int64_t n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float c1,
+                 float c2, float eps, hipStream_t s);
+void launch_probs_to_q(const float* Y, int64_t n, double* q, hipStream_t s);
 
 }  // namespace admx

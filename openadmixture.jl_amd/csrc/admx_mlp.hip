@@ -1,0 +1,246 @@
+// admx_mlp.hip -- the learned warm start: genotype-projection features and a
+// small FP32 multilayer perceptron (forward, backward, Adam) on the device.
+//
+// Features (per individual i, over the first Jp SNPs of a reference panel):
+//   f_i = (1/sqrt(Jp)) sum_{j observed} (g_ij - mu_j) * isd_j * R[j, :]      (64 values)
+// with the panel's mean mu_j = 2 f_j and isd_j = 1/sqrt(2 f_j (1-f_j)), R a
+// fixed random Jp x 64 matrix; missing entries contribute 0.
+//
+// Network (row-major activations, batch rows):
+//   H1 = relu(X W1' + b1), H2 = relu(H1 W2' + b2), Z = H2 W3' + b3, Y = softmax(Z)
+// with W_l stored out x in.  Loss: soft-target cross entropy
+//   L = -(1/B) sum_b sum_k T[b,k] log Y[b,k],  dZ = (Y - T)/B.
+// Parameters, gradients and Adam moments live in one flat float vector laid
+// out [W1, b1, W2, b2, W3, b3].  Every reduction is a fixed-order tree, so
+// training is bitwise reproducible.
+#include "admx_common.hpp"
+#include "admx_internal.hpp"
+
+namespace admx {
+
+// ------------------------------------------------------------------ features
+// One lane per individual; a block per (256 individuals, SNP chunk); partial
+// feature vectors per chunk [chunk][i][64], reduced in chunk order by
+// k_feat_reduce.  The panel row R[j, :] and (mu_j, isd_j) are wave-uniform.
+__global__ __launch_bounds__(256) void k_features(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I,
+                                                  int64_t Jp, const float* __restrict__ R,
+                                                  const float* __restrict__ mu, const float* __restrict__ isd,
+                                                  int64_t jchunk, float* __restrict__ part) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t jb = (int64_t)blockIdx.y * jchunk;
+  const int64_t je = (jb + jchunk < Jp) ? jb + jchunk : Jp;
+  const int64_t ii = i < I ? i : I - 1;
+  const uint8_t* __restrict__ gp = bed + (ii >> 2);
+  const uint32_t sh = (uint32_t)(ii & 3) * 2u;
+  float acc[kFeat];
+#pragma unroll
+  for (int e = 0; e < kFeat; ++e) acc[e] = 0.f;
+  for (int64_t j = jb; j < je; ++j) {
+    const uint32_t c = (gp[pitch * j] >> sh) & 3u;
+    const float z = (c == 1u) ? 0.f : ((float)code_g(c) - mu[j]) * isd[j];
+    const float* __restrict__ r = R + (int64_t)kFeat * j;
+#pragma unroll
+    for (int e = 0; e < kFeat; ++e) acc[e] = __builtin_fmaf(z, r[e], acc[e]);
+  }
+  if (i >= I) return;
+  float* __restrict__ out = part + ((int64_t)blockIdx.y * I + i) * kFeat;
+#pragma unroll
+  for (int e = 0; e < kFeat; e += 4)
+    *reinterpret_cast<float4*>(out + e) = make_float4(acc[e], acc[e + 1], acc[e + 2], acc[e + 3]);
+}
+
+__global__ __launch_bounds__(256) void k_feat_reduce(const float* __restrict__ part, int64_t n, int nchunk,
+                                                     float scale, float* __restrict__ F) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) {
+    float s = 0.f;
+    for (int c = 0; c < nchunk; ++c) s += part[(int64_t)c * n + e];
+    F[e] = s * scale;
+  }
+}
+
+// ------------------------------------------------------------------ FP32 GEMM
+// C[M x N] = op(A) op(B) (+ bias[n]) (relu) (* (mask > 0)) (+= C if accumulate),
+// row-major; op(A) is M x Kd (A is Kd x M when TA), op(B) is Kd x N (B is N x Kd
+// when TB).  64 x 64 tile per 256-lane block, 4 x 4 outputs per lane, Kd in
+// steps of 16 through LDS.
+template <bool TA, bool TB>
+__global__ __launch_bounds__(256) void k_sgemm(int M, int N, int Kd, const float* __restrict__ A, int lda,
+                                               const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
+                                               const float* __restrict__ bias, int relu,
+                                               const float* __restrict__ mask, int accumulate) {
+  __shared__ float As[16][64 + 4];
+  __shared__ float Bs[16][64 + 4];
+  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
+  float acc[4][4] = {};
+  for (int k0 = 0; k0 < Kd; k0 += 16) {
+    // 64 x 16 tiles of op(A) and op(B): 1024 values, 4 per lane
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int idx = threadIdx.x + 256 * r;
+      const int mm = idx >> 4, kk = idx & 15;  // A tile as [m][k]
+      const int gm = m0 + mm, gk = k0 + kk;
+      float av = 0.f;
+      if (gm < M && gk < Kd) av = TA ? A[(int64_t)gk * lda + gm] : A[(int64_t)gm * lda + gk];
+      As[kk][mm] = av;
+      const int nn = idx >> 4;  // B tile as [k][n]
+      const int gn = n0 + nn;
+      float bv = 0.f;
+      if (gn < N && gk < Kd) bv = TB ? B[(int64_t)gn * ldb + gk] : B[(int64_t)gk * ldb + gn];
+      Bs[kk][nn] = bv;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int kk = 0; kk < 16; ++kk) {
+      float a[4], b[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        a[u] = As[kk][ty + 16 * u];
+        b[u] = Bs[kk][tx + 16 * u];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = __builtin_fmaf(a[u], b[v], acc[u][v]);
+    }
+    __syncthreads();
+  }
+#pragma unroll
+  for (int u = 0; u < 4; ++u) {
+    const int gm = m0 + ty + 16 * u;
+    if (gm >= M) continue;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int gn = n0 + tx + 16 * v;
+      if (gn >= N) continue;
+      float val = acc[u][v];
+      if (bias) val += bias[gn];
+      if (relu) val = val > 0.f ? val : 0.f;
+      if (mask && !(mask[(int64_t)gm * ldc + gn] > 0.f)) val = 0.f;
+      float* cp = C + (int64_t)gm * ldc + gn;
+      *cp = accumulate ? *cp + val : val;
+    }
+  }
+}
+
+// db[n] = sum_b dZ[b, n]  (fixed-order per column)
+__global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ X, int M, int N, float* __restrict__ out) {
+  const int n = blockIdx.x * 256 + threadIdx.x;
+  if (n >= N) return;
+  float s = 0.f;
+  for (int m = 0; m < M; ++m) s += X[(int64_t)m * N + n];
+  out[n] = s;
+}
+
+// Softmax rows of Z (M x K, K <= 16) into Y; with targets T, dZ = (Y - T)/M and
+// per-row loss -sum_k T log Y into rowloss.
+__global__ __launch_bounds__(256) void k_softmax_ce(const float* __restrict__ Z, int M, int K,
+                                                    const float* __restrict__ T, float* __restrict__ Y,
+                                                    float* __restrict__ dZ, float* __restrict__ rowloss) {
+  const int m = blockIdx.x * 256 + threadIdx.x;
+  if (m >= M) return;
+  const float* z = Z + (int64_t)m * K;
+  float mx = z[0];
+  for (int k = 1; k < K; ++k) mx = fmaxf(mx, z[k]);
+  float e[16], s = 0.f;
+  for (int k = 0; k < K; ++k) {
+    e[k] = expf(z[k] - mx);
+    s += e[k];
+  }
+  const float is = 1.f / s, ls = logf(s);
+  float loss = 0.f;
+  for (int k = 0; k < K; ++k) {
+    const float y = e[k] * is;
+    if (Y) Y[(int64_t)m * K + k] = y;
+    if (T) {
+      const float t = T[(int64_t)m * K + k];
+      if (dZ) dZ[(int64_t)m * K + k] = (y - t) / (float)M;
+      if (t > 0.f) loss -= t * (z[k] - mx - ls);
+    }
+  }
+  if (rowloss) rowloss[m] = loss;
+}
+
+// Gather rows idx[0..n) of X (ld columns) into Y.
+__global__ __launch_bounds__(256) void k_gather_rows(const float* __restrict__ X, int ld,
+                                                     const int32_t* __restrict__ idx, int n, float* __restrict__ Y) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= (int64_t)n * ld) return;
+  const int64_t r = e / ld, c = e - r * ld;
+  Y[e] = X[(int64_t)idx[r] * ld + c];
+}
+
+// Adam (Kingma & Ba 2015) with bias correction: m = b1 m + (1-b1) g,
+// v = b2 v + (1-b2) g^2, w -= lr * (m/c1) / (sqrt(v/c2) + eps).
+__global__ __launch_bounds__(256) void k_adam(int64_t n, float* __restrict__ w, const float* __restrict__ g,
+                                              float* __restrict__ m, float* __restrict__ v, float lr, float b1,
+                                              float b2, float c1, float c2, float eps) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e >= n) return;
+  const float gv = g[e];
+  const float mv = b1 * m[e] + (1.f - b1) * gv;
+  const float vv = b2 * v[e] + (1.f - b2) * gv * gv;
+  m[e] = mv;
+  v[e] = vv;
+  w[e] -= lr * (mv / c1) / (sqrtf(vv / c2) + eps);
+}
+
+// Row-major I x K probabilities -> the FP64 q block of x (identical layout:
+// q[k,i] at k + K*i).
+__global__ __launch_bounds__(256) void k_probs_to_q(const float* __restrict__ Y, int64_t n, double* __restrict__ q) {
+  const int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (e < n) q[e] = (double)Y[e];
+}
+
+// ------------------------------------------------------------------ launchers
+static unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+void launch_features(const uint8_t* bed, int64_t pitch, int64_t I, int64_t Jp, const float* R, const float* mu,
+                     const float* isd, int nchunk, int64_t jchunk, float* part, float* F, hipStream_t s) {
+  dim3 grid(blocks_for(I), (unsigned)nchunk);
+  hipLaunchKernelGGL(k_features, grid, dim3(256), 0, s, bed, pitch, I, Jp, R, mu, isd, jchunk, part);
+  const int64_t n = I * kFeat;
+  const unsigned nb = blocks_for(n) < 8192u ? blocks_for(n) : 8192u;
+  hipLaunchKernelGGL(k_feat_reduce, dim3(nb), dim3(256), 0, s, part, n, nchunk, 1.0f / sqrtf((float)Jp), F);
+}
+
+void launch_sgemm(bool ta, bool tb, int M, int N, int Kd, const float* A, int lda, const float* B, int ldb, float* C,
+                  int ldc, const float* bias, bool relu, const float* mask, bool accumulate, hipStream_t s) {
+  dim3 grid((unsigned)((N + 63) / 64), (unsigned)((M + 63) / 64));
+  if (!ta && tb)
+    hipLaunchKernelGGL((k_sgemm<false, true>), grid, dim3(256), 0, s, M, N, Kd, A, lda, B, ldb, C, ldc, bias,
+                       (int)relu, mask, (int)accumulate);
+  else if (!ta && !tb)
+    hipLaunchKernelGGL((k_sgemm<false, false>), grid, dim3(256), 0, s, M, N, Kd, A, lda, B, ldb, C, ldc, bias,
+                       (int)relu, mask, (int)accumulate);
+  else if (ta && !tb)
+    hipLaunchKernelGGL((k_sgemm<true, false>), grid, dim3(256), 0, s, M, N, Kd, A, lda, B, ldb, C, ldc, bias,
+                       (int)relu, mask, (int)accumulate);
+  else
+    hipLaunchKernelGGL((k_sgemm<true, true>), grid, dim3(256), 0, s, M, N, Kd, A, lda, B, ldb, C, ldc, bias,
+                       (int)relu, mask, (int)accumulate);
+}
+
+void launch_colsum(const float* X, int M, int N, float* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_colsum, dim3(blocks_for(N)), dim3(256), 0, s, X, M, N, out);
+}
+
+void launch_softmax_ce(const float* Z, int M, int K, const float* T, float* Y, float* dZ, float* rowloss,
+                       hipStream_t s) {
+  hipLaunchKernelGGL(k_softmax_ce, dim3(blocks_for(M)), dim3(256), 0, s, Z, M, K, T, Y, dZ, rowloss);
+}
+
+void launch_gather_rows(const float* X, int ld, const int32_t* idx, int n, float* Y, hipStream_t s) {
+  hipLaunchKernelGGL(k_gather_rows, dim3(blocks_for((int64_t)n * ld)), dim3(256), 0, s, X, ld, idx, n, Y);
+}
+
+void launch_adam(int64_t n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float c1,
+                 float c2, float eps, hipStream_t s) {
+  hipLaunchKernelGGL(k_adam, dim3(blocks_for(n)), dim3(256), 0, s, n, w, g, m, v, lr, b1, b2, c1, c2, eps);
+}
+
+void launch_probs_to_q(const float* Y, int64_t n, double* q, hipStream_t s) {
+  hipLaunchKernelGGL(k_probs_to_q, dim3(blocks_for(n)), dim3(256), 0, s, Y, n, q);
+}
+
+}  // namespace admx

@@ -110,6 +110,18 @@ class HIPAdmixData:
                                               C.byref(dev), C.byref(cnt)))
         return dev.value, cnt.value
 
+    # ------------------------------------------------------------ learned warm start
+    def features(self, R, mu, isd, out=None):
+        """Genotype-projection features over the first len(mu) SNPs (kept on the
+        device for warm_q); returns them when `out` is given."""
+        Jp = int(np.asarray(mu).size)
+        N.check(N.lib().admx_features(self._h, N.ptr(R), N.ptr(mu), N.ptr(isd), Jp, N.ptr(out)))
+        return out
+
+    def warm_q(self, net):
+        """q of the state x <- projected network output on the stored features."""
+        N.check(N.lib().admx_warm_q(self._h, net.handle))
+
     # ------------------------------------------------------------ state
     def set_x(self, which: int, x):
         x = _f(x, self.T, (self.K, self.I + self.J))

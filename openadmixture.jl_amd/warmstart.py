@@ -1,0 +1,241 @@
+"""Learned warm start for admixture_qn! (no reference counterpart: the reference
+starts every fit from a random point plus FRAPPE EM, src/algorithms_outer.jl:12-45).
+
+Pieces, all computed by libadmixhip kernels:
+
+* ``Panel`` -- the reference panel the network is tied to: the first ``Jp``
+  SNPs of a cohort, their panel allele frequencies f_j and a fixed random
+  projection R (Jp x 64).  Per individual the feature vector is
+  (1/sqrt(Jp)) sum_j (g_ij - 2 f_j) / sqrt(2 f_j (1 - f_j)) R[j, :] over the
+  observed panel genotypes (``admx_features``).
+* ``MLP`` -- FP32 network 64 -> 256 -> 256 -> K with ReLU and a softmax head,
+  trained by Adam on the soft-target cross entropy between its output and the
+  converged Q that EM/SQP finds on simulated cohorts (``admx_mlp_*``).
+* ``training_set`` -- simulated cohorts drawn from a world of ``Kw`` panel
+  populations (each cohort admixes a random subset of K_c of them with
+  Dirichlet(alpha) proportions), each solved by the plain method; the fitted
+  components are matched to panel populations by their allele frequencies
+  (Hungarian assignment on correlation), so targets use the panel's labels.
+* ``warm_started_fit`` -- propose q from the network, set p to the observed
+  allele frequencies and refine it with Q held fixed, judge the proposal by
+  its log-likelihood against one FRAPPE EM step from the random start, fall
+  back to the random start + EM when the proposal is worse, then run
+  admixture_qn! to convergence.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import math
+import time
+
+import numpy as np
+
+from . import _native as N
+from .algorithms import init_em
+from .hip import HIPAdmixData
+from .structs import AdmixData
+
+FEAT = 64
+
+
+class MLP:
+    """Device MLP in -> hidden -> hidden -> out (FP32) owned by libadmixhip."""
+
+    def __init__(self, in_dim: int = FEAT, hidden: int = 256, out_dim: int = 10, seed: int = 0):
+        self.in_dim, self.hidden, self.out_dim = int(in_dim), int(hidden), int(out_dim)
+        self._h = C.c_void_p()
+        N.check(N.lib().admx_mlp_create(C.byref(self._h), self.in_dim, self.hidden, self.out_dim, C.c_uint64(seed)))
+        n = C.c_int64()
+        N.check(N.lib().admx_mlp_num_params(self._h, C.byref(n)))
+        self.num_params = n.value
+
+    @property
+    def handle(self):
+        return self._h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            N.lib().admx_mlp_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def params(self) -> np.ndarray:
+        w = np.empty(self.num_params, np.float32)
+        N.check(N.lib().admx_mlp_get_params(self._h, N.ptr(w)))
+        return w
+
+    def set_params(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.size != self.num_params:
+            raise ValueError(f"expected {self.num_params} parameters")
+        N.check(N.lib().admx_mlp_set_params(self._h, N.ptr(w)))
+
+    def _rows(self, X, width):
+        X = np.ascontiguousarray(X, dtype=np.float32)
+        if X.ndim != 2 or X.shape[1] != width:
+            raise ValueError(f"expected an (n, {width}) array")
+        return X
+
+    def loss_grad(self, X, T):
+        X, T = self._rows(X, self.in_dim), self._rows(T, self.out_dim)
+        g = np.empty(self.num_params, np.float32)
+        loss = C.c_float()
+        N.check(N.lib().admx_mlp_loss_grad(self._h, N.ptr(X), N.ptr(T), X.shape[0], C.byref(loss), N.ptr(g)))
+        return loss.value, g
+
+    def predict(self, X):
+        X = self._rows(X, self.in_dim)
+        Y = np.empty((X.shape[0], self.out_dim), np.float32)
+        N.check(N.lib().admx_mlp_predict(self._h, N.ptr(X), X.shape[0], N.ptr(Y)))
+        return Y
+
+    def train(self, X, T, epochs: int = 20, batch: int = 512, lr: float = 1e-3, seed: int = 0):
+        X, T = self._rows(X, self.in_dim), self._rows(T, self.out_dim)
+        if X.shape[0] != T.shape[0]:
+            raise ValueError("X and T must have the same number of rows")
+        hist = np.zeros(max(int(epochs), 1), np.float32)
+        N.check(N.lib().admx_mlp_train(self._h, N.ptr(X), N.ptr(T), X.shape[0], int(epochs), int(batch),
+                                       C.c_float(lr), C.c_uint64(seed), N.ptr(hist)))
+        return hist[: int(epochs)]
+
+
+class Panel:
+    """Reference panel of the features: Jp SNPs, their allele frequencies, R."""
+
+    def __init__(self, freqs, seed: int = 0):
+        f = np.clip(np.asarray(freqs, np.float64), 1e-3, 1 - 1e-3)
+        self.Jp = f.size
+        self.mu = np.ascontiguousarray(2.0 * f, dtype=np.float32)
+        self.isd = np.ascontiguousarray(1.0 / np.sqrt(2.0 * f * (1.0 - f)), dtype=np.float32)
+        self.R = np.ascontiguousarray(np.random.default_rng(seed).standard_normal((self.Jp, FEAT)), dtype=np.float32)
+
+    def features(self, ctx: HIPAdmixData, to_host: bool = True):
+        """Features of ctx's genotypes (kept on the device for warm_q)."""
+        F = np.empty((ctx.I, FEAT), np.float32) if to_host else None
+        return ctx.features(self.R, self.mu, self.isd, F)
+
+
+def world_frequencies(Kw: int, J: int, fst: float = 0.05, seed: int = 0):
+    """Panel populations by the Balding-Nichols model: ancestral f_j ~ U(0.05, 0.95),
+    P[k, j] ~ Beta(f_j (1-F)/F, (1-f_j)(1-F)/F) clipped to [0.01, 0.99]."""
+    rng = np.random.default_rng(seed)
+    anc = rng.uniform(0.05, 0.95, size=J)
+    a = anc * (1 - fst) / fst
+    b = (1 - anc) * (1 - fst) / fst
+    P = np.clip(rng.beta(a, b, size=(Kw, J)), 0.01, 0.99)
+    return np.asfortranarray(P), anc
+
+
+def match_components(p_fit, p_ref):
+    """Permutation perm with fitted component k <-> reference row perm[k],
+    maximising the summed correlation of allele frequencies."""
+    from scipy.optimize import linear_sum_assignment
+
+    a = p_fit - p_fit.mean(1, keepdims=True)
+    b = p_ref - p_ref.mean(1, keepdims=True)
+    corr = (a @ b.T) / np.outer(np.linalg.norm(a, axis=1), np.linalg.norm(b, axis=1))
+    rows, cols = linear_sum_assignment(-corr)
+    perm = np.empty(len(rows), np.int64)
+    perm[rows] = cols
+    return perm, corr[rows, cols]
+
+
+def training_set(P_world, panel: Panel, n_cohorts: int, I: int = 5000, J: int = 20000, Ks=range(4, 11),
+                 alpha_range=(0.1, 1.0), miss_rate: float = 0.01, seed: int = 0, max_iter: int = 500,
+                 log=None):
+    """Simulate cohorts from the world, solve each by the plain method and
+    return (X, T, info): X the 64 features per individual, T the converged Q in
+    the panel's population labels (zero outside the cohort's populations)."""
+    from .cv import fit_on_device
+
+    Kw = P_world.shape[0]
+    rng = np.random.default_rng(seed)
+    Ks = list(Ks)
+    X, T, info = [], [], []
+    for c in range(n_cohorts):
+        Kc = Ks[c % len(Ks)]
+        alpha = float(rng.uniform(*alpha_range))
+        pops = np.sort(rng.choice(Kw, Kc, replace=False))
+        Qt = np.asfortranarray(rng.dirichlet(alpha * np.ones(Kc), size=I).T)
+        Pt = np.asfortranarray(P_world[pops, :J])
+        ctx = HIPAdmixData(I, J, Kc, 3)
+        try:
+            ctx.generate_bed(Qt, Pt, seed * 1000 + c, miss_rate)
+            d = fit_on_device(ctx, Kc, seed * 1000 + c + 7, max_iter=max_iter)
+            F = panel.features(ctx)
+        finally:
+            ctx.close()
+        perm, corr = match_components(d.p, Pt)
+        Tc = np.zeros((I, Kw), np.float32)
+        Tc[:, pops[perm]] = d.q.T
+        X.append(F)
+        T.append(Tc)
+        info.append(dict(K=Kc, alpha=alpha, pops=pops.tolist(), iterations=d.iterations,
+                         min_match_corr=float(corr.min())))
+        if log:
+            log(f"cohort {c}: K={Kc} alpha={alpha:.2f} {d.iterations} iterations, match corr >= {corr.min():.3f}")
+    return np.concatenate(X), np.concatenate(T), info
+
+
+def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_iter: int = 1000,
+                     rtol: float = 1e-7, p_steps: int = 3, judge: bool = True, em_iters: int = 5, Q: int = 3,
+                     stop_at=None):
+    """admixture_qn! from the network's proposal (with the EM judge/fallback).
+
+    Returns dict(history=[(seconds, LL)], chosen='warm'|'em', ll_warm, ll_em,
+    iterations).  history[0] is the starting point; seconds count from the
+    start of the call (features, inference, P refinement and the judge
+    included).  With stop_at = (LL*, tol) the run ends at the first iteration
+    within tol of LL*."""
+    if net.out_dim != ctx.K:
+        raise ValueError("network output must equal the context's K")
+    I, J, K = ctx.I, ctx.J, ctx.K
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    d = AdmixData(I, J, K, Q, rng=np.random.default_rng(seed), T=ctx.T)
+    panel.features(ctx, to_host=False)
+    ctx.warm_q(net)
+    snm, salt, _ = ctx.counts()
+    f = np.clip(salt / np.maximum(2 * snm, 1), 1e-5, 1 - 1e-5)
+    x = ctx.get_x(N.ADMX_X)
+    x[:, I:] = f[None, :]
+    ctx.set_x(N.ADMX_X, x)
+    ctx.set_uv(d.U, d.V)
+    ctx.set_ll(float("nan"), ctx.loglikelihood(N.ADMX_X))
+    for it in range(1, p_steps + 1):
+        ctx.qn_iteration(it, fix_q=True)
+    ll_warm = ctx.get_ll()[1]
+    chosen, ll_em = "warm", None
+    if judge:
+        x_warm = ctx.get_x(N.ADMX_X)
+        init_em(d, None, 1, d_cu=ctx, log=None)
+        ll_em = d.ll_new
+        if not (ll_warm >= ll_em):
+            chosen = "em"
+            if em_iters > 1:
+                init_em(d, None, em_iters - 1, d_cu=ctx, log=None)
+        else:
+            ctx.set_x(N.ADMX_X, x_warm)
+    ll0 = ll_warm if chosen == "warm" else d.ll_new
+    ctx.set_uv(d.U, d.V)
+    ctx.set_ll(float("nan"), ll0)
+    hist = [(time.perf_counter() - t0, ll0)]
+    its = 0
+    for it in range(1, max_iter + 1):
+        r = ctx.qn_iteration(it)
+        hist.append((time.perf_counter() - t0, r["new"]))
+        its = it
+        if r["reldiff"] < rtol:
+            break
+        if stop_at is not None and abs(r["new"] - stop_at[0]) <= stop_at[1] * abs(stop_at[0]):
+            break
+    return dict(history=hist, chosen=chosen, ll_warm=ll_warm, ll_em=ll_em, iterations=its,
+                converged=its < max_iter and not math.isnan(hist[-1][1]))
+
+
+__all__ = ["MLP", "Panel", "world_frequencies", "match_components", "training_set", "warm_started_fit"]
