@@ -15,7 +15,10 @@ BF16-matrix-core update passes under FP64 log-likelihood/QP/convergence) runs
 EM(5) priming plus admixture_qn! iterations; its time to converge is the wall
 time from the start of priming until the first iteration whose log-likelihood
 is within 1e-4 (relative) of LL*, the final log-likelihood of the plain FP64
-run (rtol 1e-7).  The CPU figure multiplies the oracle's measured seconds per
+run (rtol 1e-7).  The warm-start forms (FP64 or BF16 passes) start from the
+learned proposal instead (features, network inference, P refinement and the
+EM judge are inside the timed span); their network is trained beforehand on
+cohorts simulated from this workload's populations (reported separately).  The CPU figure multiplies the oracle's measured seconds per
 EM step and per iteration (bounded sample, scaled to all SNPs) by the plain
 run's step counts.
 
@@ -48,6 +51,10 @@ WORKLOADS = {
     "c3": dict(I=100_000, J=100_000, K=8, alpha=0.3, p_dist="beta", p_lo=0.01, p_hi=0.99, miss_rate=0.02),
     "c2": dict(I=10_000, J=100_000, K=5, alpha=0.3, p_dist="uniform", p_lo=0.05, p_hi=0.95, miss_rate=0.02),
     "c1": dict(I=1_000, J=10_000, K=3, alpha=0.5, p_dist="uniform", p_lo=0.05, p_hi=0.95, miss_rate=0.01),
+    # learned warm start: 10-population world (Balding-Nichols, F=0.05), training
+    # cohorts I=5,000 x J=20,000 (the panel SNPs), evaluation cohort of all 10
+    "c5": dict(I=200_000, J=50_000, K=10, alpha=0.5, train_I=5_000, train_J=20_000, train_cohorts=14,
+               miss_rate=0.01),
     # K sweep K=2..12 with 5-fold CV on one cohort of true K=6
     "c4": dict(I=20_000, J=200_000, K=6, alpha=0.5, p_dist="uniform", p_lo=0.05, p_hi=0.95, miss_rate=0.01),
 }
@@ -205,6 +212,95 @@ def run_sweep(args):
     print(json.dumps(out), flush=True)
 
 
+def run_warmstart(args):
+    """Learned warm start: simulate and solve the training cohorts, train the
+    network on the device, then time plain / BF16 / warm / warm+BF16 starts to
+    |LL - LL*| <= 1e-4 |LL*| on the evaluation cohort; prints one JSON line."""
+    import __graft_entry__ as g
+
+    oa = g.load_package()
+    wl = WORKLOADS[args.workload]
+    I, J, K = wl["I"], wl["J"], wl["K"]
+    log = lambda m: print(m, file=sys.stderr, flush=True)  # noqa: E731
+    P, _ = oa.world_frequencies(K, J, fst=0.05, seed=args.seed)
+    panel = oa.Panel(P[:, :wl["train_J"]].mean(0), seed=args.seed)
+    t0 = time.perf_counter()
+    X, T, info = oa.training_set(P, panel, wl["train_cohorts"], I=wl["train_I"], J=wl["train_J"], Ks=range(4, 11),
+                                 alpha_range=(0.1, 1.0), miss_rate=wl["miss_rate"], seed=args.seed, log=log)
+    t_data = time.perf_counter() - t0
+    net = oa.MLP(64, 256, K, seed=args.seed)
+    t0 = time.perf_counter()
+    hist = net.train(X, T, epochs=args.train_epochs, batch=512, lr=2e-3, seed=args.seed)
+    t_train = time.perf_counter() - t0
+    log(f"training set {X.shape[0]} rows in {t_data:.1f} s; network trained in {t_train:.2f} s, "
+        f"loss {hist[0]:.4f} -> {hist[-1]:.4f}")
+
+    rng = np.random.default_rng(args.seed + 99)
+    Qt = np.asfortranarray(rng.dirichlet(wl["alpha"] * np.ones(K), size=I).T)
+    ctx = oa.HIPAdmixData(I, J, K, 3)
+    ctx.generate_bed(Qt, P, args.seed + 5, wl["miss_rate"])
+    tol = 1e-4
+    runs = {}
+
+    def plain(form, stop_at):
+        ctx.set_precision(form)
+        d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(args.seed + 1))
+        ctx.synchronize()
+        t_start = time.perf_counter()
+        oa.init_em(d, None, 5, d_cu=ctx, log=None)
+        ctx.set_uv(d.U, d.V)
+        ctx.set_ll(d.ll_prev, d.ll_new)
+        h = [(time.perf_counter() - t_start, d.ll_new)]
+        for it in range(1, args.ttc_max_iter + 1):
+            r = ctx.qn_iteration(it)
+            h.append((time.perf_counter() - t_start, r["new"]))
+            if r["reldiff"] < 1e-7:
+                break
+            if stop_at and abs(r["new"] - stop_at) <= tol * abs(stop_at):
+                break
+        return h, None
+
+    def warm(form, stop_at):
+        ctx.set_precision(form)
+        r = oa.warm_started_fit(ctx, net, panel, seed=args.seed + 1, max_iter=args.ttc_max_iter,
+                                stop_at=(stop_at, tol) if stop_at else None)
+        return r["history"], r["chosen"]
+
+    h, _ = plain("f64", None)
+    ll_star = h[-1][1]
+    runs["plain_f64"] = (h, None)
+    runs["bf16"] = plain("bf16", ll_star)
+    runs["warm_f64"] = warm("f64", ll_star)
+    runs["warm_bf16"] = warm("bf16", ll_star)
+    ctx.close()
+    net.close()
+    res = {}
+    for name, (h, chosen) in runs.items():
+        hit = first_within(h, ll_star, tol)
+        res[name] = {"seconds": hit[1] if hit else None, "qn_iterations": hit[0] if hit else None,
+                     "start_seconds": h[0][0], "start_ll": h[0][1], "ll_last": h[-1][1]}
+        if chosen:
+            res[name]["chosen_start"] = chosen
+    base = res["plain_f64"]["seconds"]
+    for name in res:
+        if res[name]["seconds"]:
+            res[name]["speedup_vs_plain"] = base / res[name]["seconds"]
+    out = {
+        "metric": "time to |LL - LL*| <= 1e-4 |LL*| with the learned warm start (LL* = plain FP64 final LL)",
+        "value": res["warm_bf16"]["seconds"], "unit": "s", "higher_is_better": False, "n_gpus": 1,
+        "dtype": "bf16 passes, f64 LL/QP; f32 network",
+        "data": "synthetic: 10-population Balding-Nichols world (F=0.05), Dirichlet(alpha) admixture",
+        "config": {"workload": f"{args.workload}: eval I={I}, J={J}, K={K}; train {wl['train_cohorts']} cohorts "
+                               f"I={wl['train_I']}, J={wl['train_J']}, K=4..10, alpha in [0.1, 1]",
+                   "I": I, "J": J, "K": K},
+        "ll_star": ll_star,
+        "training": {"rows": int(X.shape[0]), "cohorts": info, "data_seconds": t_data, "train_seconds": t_train,
+                     "epochs": args.train_epochs, "loss_first": float(hist[0]), "loss_last": float(hist[-1])},
+        "forms": res,
+    }
+    print(json.dumps(out), flush=True)
+
+
 def calibrate_copy(nbytes=1 << 30):
     """A plain device-to-device copy of known size (HIP runtime copy kernel), used
     to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -239,11 +335,13 @@ def main():
     ap.add_argument("--target-waves", type=int, default=0, help="waves per genotype pass (0: library default)")
     ap.add_argument("--no-ttc", action="store_true", help="skip the time-to-converge runs")
     ap.add_argument("--graphs", action="store_true", help="replay steady-state iterations from hipGraphs")
-    ap.add_argument("--ttc-forms", default="bf16", help="forms timed to convergence beside plain FP64")
+    ap.add_argument("--ttc-forms", default="bf16,warm_f64,warm_bf16",
+                    help="forms timed to convergence beside plain FP64 (bf16, f32, warm_f64, warm_bf16)")
     ap.add_argument("--ttc-max-iter", type=int, default=200)
     ap.add_argument("--folds", type=int, default=5, help="CV folds of the K sweep (workload c4)")
     ap.add_argument("--sweep-ks", default="", help="comma-separated K values of the sweep (default 2..12)")
     ap.add_argument("--sweep-max-iter", type=int, default=300)
+    ap.add_argument("--train-epochs", type=int, default=20, help="network training epochs (workload c5)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -255,6 +353,9 @@ def main():
 
     if args.workload == "c4":
         run_sweep(args)
+        return
+    if args.workload == "c5":
+        run_warmstart(args)
         return
 
     import __graft_entry__ as g
@@ -277,7 +378,8 @@ def main():
             sys.exit("--form f32/bf16 runs on a FP64 context (FP64 log-likelihood, QP and convergence test)")
         ctx.set_precision(args.form)
     t0 = time.perf_counter()
-    sim.cohort_on_device(ctx, wl["alpha"], wl["p_dist"], wl["p_lo"], wl["p_hi"], wl["miss_rate"], seed=args.seed)
+    Qt_true, Pt_true = sim.cohort_on_device(ctx, wl["alpha"], wl["p_dist"], wl["p_lo"], wl["p_hi"], wl["miss_rate"],
+                                            seed=args.seed)
     t_gen = time.perf_counter() - t0
     d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(args.seed + 1), T=T)
     oa.init_em(d, None, 5, d_cu=ctx, log=None)
@@ -347,14 +449,44 @@ def main():
         ttc["plain"] = {"seconds": hit[1], "qn_iterations": hit[0], "ll": hit[2],
                         "ms_per_iteration": (hist[-1][0] - hist[0][0]) / max(its, 1) * 1e3,
                         "priming_seconds": hist[0][0]}
-        for form in [f for f in args.ttc_forms.split(",") if f]:
-            h2, its2 = converge_run(oa, ctx, wl, T, args.seed, form, args.ttc_max_iter, stop_at=(ll_star, tol))
+        forms = [f for f in args.ttc_forms.split(",") if f]
+        net = panel = None
+        if any(f.startswith("warm") for f in forms):
+            # the network is trained once on cohorts simulated from the populations
+            # of this workload (subsets of K_c = 4..K of them, I=5,000 over the
+            # first 20,000 SNPs), each solved by the plain method
+            tw = time.perf_counter()
+            Jp = min(20_000, J)
+            panel = oa.Panel(Pt_true[:, :Jp].mean(0), seed=args.seed)
+            X, Tt, _ = oa.training_set(np.asfortranarray(Pt_true), panel, 10, I=5_000, J=Jp, Ks=range(4, K + 1),
+                                       alpha_range=(0.1, 1.0), miss_rate=wl["miss_rate"], seed=args.seed)
+            t_data = time.perf_counter() - tw
+            net = oa.MLP(64, 256, K, seed=args.seed)
+            tw = time.perf_counter()
+            lh = net.train(X, Tt, epochs=20, batch=512, lr=2e-3, seed=args.seed)
+            ttc["warm_start_training"] = {
+                "rows": int(X.shape[0]), "cohorts": 10, "simulate_and_solve_seconds": t_data,
+                "train_seconds": time.perf_counter() - tw, "loss_first": float(lh[0]), "loss_last": float(lh[-1]),
+                "note": "one-time cost, not part of any time to converge"}
+        for form in forms:
+            if form.startswith("warm"):
+                prec = form.split("_", 1)[1] if "_" in form else "f64"
+                ctx.set_precision(prec)
+                r = oa.warm_started_fit(ctx, net, panel, seed=args.seed + 1, max_iter=args.ttc_max_iter,
+                                        stop_at=(ll_star, tol))
+                h2, its2 = r["history"], r["iterations"]
+            else:
+                h2, its2 = converge_run(oa, ctx, wl, T, args.seed, form, args.ttc_max_iter, stop_at=(ll_star, tol))
             hit2 = first_within(h2, ll_star, tol)
             ttc[form] = ({"seconds": hit2[1], "qn_iterations": hit2[0], "ll": hit2[2],
                           "ms_per_iteration": (h2[-1][0] - h2[0][0]) / max(its2, 1) * 1e3,
                           "priming_seconds": h2[0][0], "vs_plain_speedup": hit[1] / hit2[1]} if hit2 else
                          {"seconds": None, "note": f"not within {tol} of LL* after {its2} iterations",
                           "ll_last": h2[-1][1]})
+            if form.startswith("warm"):
+                ttc[form]["chosen_start"] = r["chosen"]
+        if net is not None:
+            net.close()
         ctx.set_precision("f64" if args.form == "plain" else args.form)
         out["time_to_converge"] = ttc
     if not args.no_cpu_baseline:
