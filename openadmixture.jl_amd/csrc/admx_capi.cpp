@@ -229,14 +229,16 @@ int qside_chunks(admx_ctx* c, int64_t Ilen, int64_t Jlen, int64_t* chunk, int ro
   return (int)std::max<int64_t>(1, (Jlen + *chunk - 1) / *chunk);
 }
 
+// P-side passes stream individuals in tiles of 256 from a 256-aligned start
+// (the first tile may begin before i0; its leading individuals are masked).
 int pside_chunks(admx_ctx* c, int64_t i0, int64_t i1, int64_t Jlen, int64_t* chunk, int rows_per_wave = 64) {
   const int64_t waves = (Jlen + rows_per_wave - 1) / rows_per_wave;
-  const int64_t span = i1 - (i0 & ~(int64_t)15);
+  const int64_t span = i1 - (i0 & ~(int64_t)255);
   int64_t n = (c->target_waves + waves - 1) / std::max<int64_t>(waves, 1);
-  n = std::max<int64_t>(1, std::min<int64_t>(n, std::max<int64_t>(1, span / 64)));
+  n = std::max<int64_t>(1, std::min<int64_t>(n, std::max<int64_t>(1, span / 256)));
   int64_t ch = (span + n - 1) / n;
-  ch = (ch + 15) & ~(int64_t)15;
-  *chunk = std::max<int64_t>(16, ch);
+  ch = (ch + 255) & ~(int64_t)255;
+  *chunk = std::max<int64_t>(256, ch);
   return (int)std::max<int64_t>(1, (span + *chunk - 1) / *chunk);
 }
 

@@ -16,6 +16,22 @@ namespace admx {
 
 typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
+// (x, y) tile of a 1-D launch of ntot = nx * ny workgroups, ordered so that each
+// XCD receives a run of consecutive tiles: workgroup L is dispatched to XCD
+// L % 8 as that XCD's (L / 8)-th, and tile lid = first(L % 8) + L / 8.  Tiles
+// adjacent in x (neighbouring individuals, the same 128-byte BED lines) then
+// share one XCD's L2 instead of being fetched by two.
+struct Tile2 {
+  int64_t x, y;
+};
+__device__ __forceinline__ Tile2 xcd_tile(int64_t nx) {
+  const int64_t ntot = gridDim.x, L = blockIdx.x;
+  const int64_t q = ntot >> 3, r = ntot & 7;
+  const int64_t xcd = L & 7, slot = L >> 3;
+  const int64_t lid = (xcd < r ? xcd * (q + 1) : r * (q + 1) + (xcd - r) * q) + slot;
+  return {lid % nx, lid / nx};
+}
+
 // PLINK-1 code -> (m*g, m*(2-g)) with m the nonmissing flag:
 //   00 -> (0,2)  01 (missing) -> (0,0)  10 -> (1,1)  11 -> (2,0)
 // i.e. g_map = [0,0,1,2], nonmissing_map = [1,0,1,1] of src/loops.jl:2-5.

@@ -139,11 +139,12 @@ __global__ __launch_bounds__(256) void k_qside_bf16(PassArgs a, const float* __r
   uint8_t* __restrict__ tb = tbuf[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
   const int64_t Ilen = a.i1 - a.i0;
-  const int64_t il = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t wrow0 = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63);  // first row of this wave
+  const Tile2 tile = xcd_tile((Ilen + 255) / 256);
+  const int64_t il = tile.x * 256 + threadIdx.x;
+  const int64_t wrow0 = tile.x * 256 + (threadIdx.x & ~63);  // first row of this wave
   const bool valid = il < Ilen;
   const int64_t i = a.i0 + (valid ? il : 0);
-  const int64_t jb = a.j0 + (int64_t)blockIdx.y * a.chunk;
+  const int64_t jb = a.j0 + tile.y * a.chunk;
   const int64_t je = (a.j1 < jb + a.chunk) ? a.j1 : jb + a.chunk;
 
   float qi[K];
@@ -238,7 +239,7 @@ __global__ __launch_bounds__(256) void k_qside_bf16(PassArgs a, const float* __r
     }
   }
   constexpr int NA = NP + K;
-  float* __restrict__ part = static_cast<float*>(a.part) + (int64_t)blockIdx.y * NA * Ilen;
+  float* __restrict__ part = static_cast<float*>(a.part) + tile.y * NA * Ilen;
 #pragma unroll
   for (int T = 0; T < 2; ++T)
 #pragma unroll
@@ -268,7 +269,7 @@ __global__ __launch_bounds__(256) void k_pside_bf16(PassArgs a, const float* __r
   float pj[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) pj[k] = fp[k + (int64_t)K * j];
-  const uint32_t* __restrict__ col = reinterpret_cast<const uint32_t*>(a.bed + a.pitch * j);
+  const uint8_t* __restrict__ col = a.bed + a.pitch * j;
   float z[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) z[k] = 0.f;
@@ -279,11 +280,27 @@ __global__ __launch_bounds__(256) void k_pside_bf16(PassArgs a, const float* __r
     for (int t = 0; t < NT; ++t) acc[T][t] = (f32x16){};
   const int kh = lane >> 5, cn = lane & 31;
 
+  // Genotype tiles of 256 individuals: 64 bytes of the lane's SNP row with four
+  // 16-byte loads issued together, parked in LDS as [word][lane] and read back
+  // one word per step, as in the FP64 P side.  The tile is loaded when it is
+  // reached (no registers held across the tile keep the occupancy at 4 waves;
+  // the other waves cover the load).
+  __shared__ uint32_t tw[16][256];
   for (int64_t ib = ia0; ib < ia1; ib += 16) {
+    const int sw = (int)((ib - ia0) >> 4) & 15;
+    if (sw == 0) {  // uniform: a new tile
+      const u32x4* __restrict__ src = reinterpret_cast<const u32x4*>(col + (ib >> 2));
+#pragma unroll
+      for (int c = 0; c < 4; ++c) {
+        const u32x4 v = src[c];
+#pragma unroll
+        for (int e = 0; e < 4; ++e) tw[4 * c + e][threadIdx.x] = v[e];
+      }
+    }
     bf16x8 bqq[NT];
 #pragma unroll
     for (int t = 0; t < NT; ++t) bqq[t] = load_b8(QQt, ld, 32 * t + cn, ib + 8 * kh);
-    uint32_t w = col[ib >> 4];
+    uint32_t w = tw[sw][threadIdx.x];
     if (ib + 16 > ia1) {  // uniform: individuals past the chunk end are missing
       uint32_t keep = 0;
 #pragma unroll
@@ -371,7 +388,7 @@ int pair_table_rows(int K) {
 template <int K>
 static void qside_bf16_k(const PassArgs& a, int nchunk, const float* fq, const float* fp, const void* PPt,
                          const void* MMt, int64_t ld, hipStream_t s) {
-  dim3 grid((unsigned)((a.i1 - a.i0 + 255) / 256), (unsigned)nchunk);
+  dim3 grid((unsigned)(((a.i1 - a.i0 + 255) / 256) * nchunk));  // 1-D, tiles ordered per XCD (xcd_tile)
   hipLaunchKernelGGL((k_qside_bf16<K>), grid, dim3(256), 0, s, a, fq, fp, (const __bf16*)PPt, (const __bf16*)MMt,
                      ld);
 }

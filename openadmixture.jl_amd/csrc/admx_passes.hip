@@ -36,10 +36,12 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
   const T* __restrict__ q = static_cast<const T*>(a.q);
   const T* __restrict__ p = static_cast<const T*>(a.p);
   const int64_t Ilen = a.i1 - a.i0;
-  const int64_t il = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  const int64_t nbx = (Ilen + 255) / 256;
+  const Tile2 tile = xcd_tile(nbx);
+  const int64_t il = tile.x * 256 + threadIdx.x;
   const bool valid = il < Ilen;
   const int64_t i = a.i0 + (valid ? il : 0);
-  const int64_t jb = a.j0 + (int64_t)blockIdx.y * a.chunk;
+  const int64_t jb = a.j0 + tile.y * a.chunk;
   const int64_t je = (a.j1 < jb + a.chunk) ? a.j1 : jb + a.chunk;
 
   T qi[K];
@@ -133,7 +135,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
   __shared__ __align__(16) uint8_t tbuf[4][512];
   uint8_t* __restrict__ tb = tbuf[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63;
-  const int64_t wrow0 = (int64_t)blockIdx.x * 256 + (threadIdx.x & ~63);
+  const int64_t wrow0 = tile.x * 256 + (threadIdx.x & ~63);
   const bool wave_live = wrow0 < Ilen;  // wave-uniform
   const int64_t ifirst = a.i0 + wrow0;
   const int64_t wbase = (ifirst >> 2) & ~(int64_t)15;
@@ -193,13 +195,13 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
     __shared__ double sh_red[256];
     if constexpr (sizeof(T) == 8) acc = log(prod) + (double)esum * 0.69314718055994530942;
     const double tot = block_sum<256>(valid ? acc : 0.0, sh_red);
-    if (threadIdx.x == 0) static_cast<double*>(a.part)[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = tot;
+    if (threadIdx.x == 0) static_cast<double*>(a.part)[tile.y * nbx + tile.x] = tot;
     return;
   } else {
     if (!valid) return;
     T* __restrict__ part = static_cast<T*>(a.part);
     const int NA = (MODE == MODE_EM) ? K : (TRICK ? NT + 2 * K + 2 : NT + K);
-    T* out = part + (int64_t)blockIdx.y * NA * Ilen + il;
+    T* out = part + tile.y * NA * Ilen + il;
     int o = 0;
     if constexpr (MODE == MODE_UPDATE) {
 #pragma unroll
@@ -229,7 +231,7 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
   const int64_t jl = (int64_t)blockIdx.x * 256 + threadIdx.x;
   const bool valid = jl < Jlen;
   const int64_t j = a.j0 + (valid ? jl : 0);
-  const int64_t ia0 = (a.i0 & ~(int64_t)15) + (int64_t)blockIdx.y * a.chunk;
+  const int64_t ia0 = (a.i0 & ~(int64_t)255) + (int64_t)blockIdx.y * a.chunk;  // chunk: multiple of 256
   const int64_t ia1 = (a.i1 < ia0 + a.chunk) ? a.i1 : ia0 + a.chunk;
 
   T pj[K];
@@ -278,27 +280,49 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
     }
   };
 
-  // 16 genotypes of the lane's SNP column per 32-bit load, next word prefetched.
-  uint32_t w_n = (ia0 < ia1) ? *reinterpret_cast<const uint32_t*>(col + (ia0 >> 2)) : 0u;
-  for (int64_t ib = ia0; ib < ia1; ib += 16) {
-    uint32_t w = w_n;
-    if (ib + 16 < ia1) w_n = *reinterpret_cast<const uint32_t*>(col + ((ib + 16) >> 2));
-    if (ib < a.i0 || ib + 16 > a.i1) {  // uniform: mark out-of-range individuals missing
-      uint32_t keep = 0;
+  // Genotypes in tiles of 256 individuals: the 64 bytes of the lane's SNP row
+  // come with four 16-byte loads issued together, one tile ahead, so every
+  // 64-byte segment of a row is requested once, while it is needed (a lane that
+  // streamed its row word by word kept ~64 rows per wave live in L2 at once).
+  // The landed tile is parked in LDS as [word][lane] (each lane touches only
+  // its own column: no barrier) and read back one 32-bit word per 16 genotypes.
+  // Tiles stay inside the padded pitch: a row holds round_up(ceil(I/4), 64) bytes.
+  __shared__ uint32_t tw[16][256];
+  u32x4 nx[4];
+  auto fetch = [&](int64_t ib) {
+    const u32x4* __restrict__ src = reinterpret_cast<const u32x4*>(col + (ib >> 2));
 #pragma unroll
-      for (int t = 0; t < 16; ++t)
-        if (ib + t >= a.i0 && ib + t < a.i1) keep |= 3u << (2 * t);
-      w = (w & keep) | (0x55555555u & ~keep);
-    }
-    if (ib + 16 <= a.I) {  // interior word: constant offsets from one base pointer
-      const T* __restrict__ qb = q + (int64_t)K * ib;
+    for (int c = 0; c < 4; ++c) nx[c] = src[c];
+  };
+  if (ia0 < ia1) fetch(ia0);
+  for (int64_t it = ia0; it < ia1; it += 256) {
 #pragma unroll
-      for (int t = 0; t < 16; ++t) body((w >> (2 * t)) & 3u, qb + K * t);
-    } else {  // last word of the cohort: clamp the (masked) individuals past I
+    for (int c = 0; c < 4; ++c)
 #pragma unroll
-      for (int t = 0; t < 16; ++t) {
-        const int64_t i = (ib + t < a.I) ? ib + t : a.I - 1;
-        body((w >> (2 * t)) & 3u, q + (int64_t)K * i);
+      for (int e = 0; e < 4; ++e) tw[4 * c + e][threadIdx.x] = nx[c][e];
+    if (it + 256 < ia1) fetch(it + 256);
+    const int s0 = (it < a.i0) ? (int)((a.i0 - it) >> 4) : 0;
+    const int s1 = (int)((((ia1 - it) < 256 ? (ia1 - it) : 256) + 15) >> 4);
+    for (int sw = s0; sw < s1; ++sw) {
+      const int64_t ib = it + 16 * sw;
+      uint32_t w = tw[sw][threadIdx.x];
+      if (ib < a.i0 || ib + 16 > a.i1) {  // uniform: mark out-of-range individuals missing
+        uint32_t keep = 0;
+#pragma unroll
+        for (int t = 0; t < 16; ++t)
+          if (ib + t >= a.i0 && ib + t < a.i1) keep |= 3u << (2 * t);
+        w = (w & keep) | (0x55555555u & ~keep);
+      }
+      if (ib + 16 <= a.I) {  // interior word: constant offsets from one base pointer
+        const T* __restrict__ qb = q + (int64_t)K * ib;
+#pragma unroll
+        for (int t = 0; t < 16; ++t) body((w >> (2 * t)) & 3u, qb + K * t);
+      } else {  // last word of the cohort: clamp the (masked) individuals past I
+#pragma unroll
+        for (int t = 0; t < 16; ++t) {
+          const int64_t i = (ib + t < a.I) ? ib + t : a.I - 1;
+          body((w >> (2 * t)) & 3u, q + (int64_t)K * i);
+        }
       }
     }
   }
@@ -323,7 +347,7 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
 template <typename T, int K>
 static void qside_k(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
   const int64_t n = a.i1 - a.i0;
-  dim3 grid((unsigned)((n + 255) / 256), (unsigned)nchunk);
+  dim3 grid((unsigned)(((n + 255) / 256) * nchunk));  // 1-D, tiles ordered per XCD (xcd_tile)
   constexpr bool trick = sizeof(T) == 8;
   if (mode == MODE_UPDATE) hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick>), grid, dim3(256), 0, s, a);
   else if (mode == MODE_EM) hipLaunchKernelGGL((k_qside<T, K, MODE_EM, false>), grid, dim3(256), 0, s, a);
