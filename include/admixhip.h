@@ -51,6 +51,11 @@ extern "C" {
 #define ADMX_TUNE_PRECISION 2     /* FP64 contexts: precision of the device-resident update passes */
 #define ADMX_TUNE_GRAPHS 3        /* 1: admx_qn_iteration replays steady-state iterations (iter > Q) from
                                    * captured hipGraphs (not while profiling); results are identical */
+#define ADMX_TUNE_QN_MODE 4       /* quasi-Newton step of admx_qn_iteration: ADMX_QN_ZAL or ADMX_QN_LBQN */
+
+/* values of ADMX_TUNE_QN_MODE: admixture_qn!'s `mode` (src/algorithms_outer.jl:127-136) */
+#define ADMX_QN_ZAL 0
+#define ADMX_QN_LBQN 1
 
 /* values of ADMX_TUNE_PRECISION.  Log-likelihood, QP, projections, quasi-Newton
  * and the convergence test stay FP64 in every mode. */
@@ -156,6 +161,10 @@ int admx_project_p(admx_ctx *ctx, void *b, int64_t n);
  * the 1-based outer index; U, V are updated in place, x_tmp receives the step. */
 int admx_zal_step(admx_ctx *ctx, int64_t iter, const void *x, const void *x_next, const void *x_next2, void *U,
                   void *V, void *x_tmp);
+/* update_UV_LBQN! + update_QN_LBQN! + project_p!/project_q!  (src/quasi_newton.jl:15-58;
+ * src/algorithms_outer.jl:132-142), host arrays, same shapes as admx_zal_step */
+int admx_lbqn_step(admx_ctx *ctx, int64_t iter, const void *x, const void *x_next, const void *x_next2, void *U,
+                   void *V, void *x_tmp);
 
 /* ---- device-resident state (AdmixData mirrored in HBM) */
 int admx_set_x(admx_ctx *ctx, int which, const void *host);

@@ -18,6 +18,8 @@ ADMX_F64, ADMX_F32 = 0, 1
 ADMX_X, ADMX_X_NEXT, ADMX_X_NEXT2, ADMX_X_TMP = 0, 1, 2, 3
 ADMX_SIDE_Q, ADMX_SIDE_P = 0, 1
 ADMX_TUNE_TARGET_WAVES, ADMX_TUNE_KEEP_HESSIANS, ADMX_TUNE_PRECISION, ADMX_TUNE_GRAPHS = 0, 1, 2, 3
+ADMX_TUNE_QN_MODE = 4
+ADMX_QN_ZAL, ADMX_QN_LBQN = 0, 1
 ADMX_PREC_F64, ADMX_PREC_F32, ADMX_PREC_BF16 = 0, 1, 2
 
 # Every exported symbol with its C signature (return type is int unless noted).
@@ -53,6 +55,7 @@ SIGNATURES = {
     "admx_qp_batch": ([_vp, _int, _i64, _vp, _vp, _vp, _vp, _vp], _int),
     "admx_project_q": ([_vp, _vp, _i64], _int),
     "admx_project_p": ([_vp, _vp, _i64], _int),
+    "admx_lbqn_step": ([_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp], _int),
     "admx_zal_step": ([_vp, _i64, _vp, _vp, _vp, _vp, _vp, _vp], _int),
     "admx_set_x": ([_vp, _int, _vp], _int),
     "admx_get_x": ([_vp, _int, _vp], _int),

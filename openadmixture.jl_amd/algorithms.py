@@ -125,13 +125,16 @@ def admixture_qn(d: AdmixData, g=None, iter: int = 1000, rtol: float = 1e-7, d_c
     """admixture_qn!(d, g, iter, rtol; mode=:ZAL)  (src/algorithms_outer.jl:63-193).
 
     Each iteration: two SQP map applications (update_q!, update_p! twice),
-    LL of the basic point, ZAL extrapolation with Q history columns,
+    LL of the basic point, ZAL (mode="ZAL") or limited-memory Broyden
+    (mode="LBQN") extrapolation with Q history columns,
     projections, LL of the QN point, accept the QN point iff ll_prev < ll_qn,
     stop when |ll_new - ll_prev| / |ll_prev| < rtol.  Returns the LL history.
     """
-    if mode != "ZAL":
-        raise NotImplementedError("only mode=:ZAL is provided on device")
+    mode = str(mode).lstrip(":")
+    if mode not in ("ZAL", "LBQN"):
+        raise ValueError(f"Invalid mode {mode!r}")
     c = _dev(d, d_cu)
+    c.set_qn_mode(mode)
     c.set_x(N.ADMX_X, d.x)
     c.set_uv(d.U, d.V)
     if math.isnan(d.ll_new):

@@ -104,6 +104,7 @@ struct admx_ctx {
   // buffer layout signature graph_sig holds; graph_warm has bit (fix_q | 2 fix_p)
   // set once an eager iteration of that kind ran under the signature.
   int use_graphs = 0;
+  int qn_mode = ADMX_QN_ZAL;
   struct GraphEntry {
     int64_t key;
     hipGraphExec_t exec;
@@ -488,6 +489,28 @@ int dev_zal(admx_ctx* c, int64_t iter, const void* x, const void* xn, const void
   return check_launch();
 }
 
+// LBQN step (update_UV_LBQN! + update_QN_LBQN!, src/quasi_newton.jl:15-58, and
+// the projections): the history columns shift right by device copies, then
+// the new first column, the Gram inner products, the recursion and the apply.
+template <typename T>
+int dev_lbqn(admx_ctx* c, int64_t iter, const void* x, const void* xn, const void* xn2, void* U, void* V,
+             void* out) {
+  const int m = (int)std::min<int64_t>(iter, c->Q);
+  const size_t colb = (size_t)c->N() * sizeof(T);
+  for (int col = m - 1; col >= 1; --col) {
+    HIPCHK(hipMemcpyAsync(static_cast<char*>(U) + colb * col, static_cast<char*>(U) + colb * (col - 1), colb,
+                          hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(static_cast<char*>(V) + colb * col, static_cast<char*>(V) + colb * (col - 1), colb,
+                          hipMemcpyDeviceToDevice, c->s));
+  }
+  constexpr int NP = kMaxQ * kMaxQ + kMaxQ;
+  const int64_t len = 512 * NP + 2 * kMaxQ;
+  CHK(c->zal.grow((size_t)len * sizeof(double)));
+  ProfScope ps(c, "lbqn");
+  launch_lbqn<T>((int)c->K, c->I, c->J, x, xn, xn2, U, V, out, m, static_cast<double*>(c->zal.p), len, c->s);
+  return check_launch();
+}
+
 // Everything of one admixture_qn! iteration up to the accept kernel, enqueued
 // on the context stream without host synchronisation (capturable).
 template <typename T>
@@ -507,7 +530,8 @@ int enqueue_qn_iteration(admx_ctx* c, int64_t iter, int fix_q, int fix_p) {
   if (!fix_p) CHK(dev_update_p<T>(c, true, nullptr));
   else HIPCHK(hipMemcpyAsync(c->pp(XN2), c->pp(X), pbytes, hipMemcpyDeviceToDevice, c->s));
   CHK(ll_to_slot<T>(c, XN2, c->pp(XN2), 0, c->I, 0, c->J, LL_BASIC));
-  CHK(dev_zal<T>(c, iter, X, XN, XN2, c->U, c->V, XT));
+  if (c->qn_mode == ADMX_QN_LBQN) CHK(dev_lbqn<T>(c, iter, X, XN, XN2, c->U, c->V, XT));
+  else CHK(dev_zal<T>(c, iter, X, XN, XN2, c->U, c->V, XT));
   CHK(ll_to_slot<T>(c, XT, c->pp(XT), 0, c->I, 0, c->J, LL_QN));
   {
     ProfScope ps(c, "accept");
@@ -532,6 +556,7 @@ uint64_t layout_sig(const admx_ctx* c) {
   mix((uint64_t)c->target_waves);
   mix((uint64_t)c->keep_hess);
   mix((uint64_t)c->precision);
+  mix((uint64_t)c->qn_mode);
   return h;
 }
 
@@ -780,6 +805,24 @@ int host_zal(admx_ctx* c, int64_t iter, const void* x, const void* xn, const voi
   CHK(stage_in(c, 4, V, ub, ub));
   CHK(c->st[5].grow(nb));
   CHK(dev_zal<T>(c, iter, c->st[0].p, c->st[1].p, c->st[2].p, c->st[3].p, c->st[4].p, c->st[5].p));
+  HIPCHK(hipMemcpyAsync(U, c->st[3].p, ub, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(V, c->st[4].p, ub, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipMemcpyAsync(x_tmp, c->st[5].p, nb, hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  return ADMX_OK;
+}
+
+template <typename T>
+int host_lbqn(admx_ctx* c, int64_t iter, const void* x, const void* xn, const void* xn2, void* U, void* V,
+              void* x_tmp) {
+  const size_t nb = (size_t)c->N() * sizeof(T), ub = nb * (size_t)c->Q;
+  CHK(stage_in(c, 0, x, nb, nb));
+  CHK(stage_in(c, 1, xn, nb, nb));
+  CHK(stage_in(c, 2, xn2, nb, nb));
+  CHK(stage_in(c, 3, U, ub, ub));
+  CHK(stage_in(c, 4, V, ub, ub));
+  CHK(c->st[5].grow(nb));
+  CHK(dev_lbqn<T>(c, iter, c->st[0].p, c->st[1].p, c->st[2].p, c->st[3].p, c->st[4].p, c->st[5].p));
   HIPCHK(hipMemcpyAsync(U, c->st[3].p, ub, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipMemcpyAsync(V, c->st[4].p, ub, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipMemcpyAsync(x_tmp, c->st[5].p, nb, hipMemcpyDeviceToHost, c->s));
@@ -1195,6 +1238,15 @@ int admx_zal_step(admx_ctx* c, int64_t iter, const void* x, const void* xn, cons
   return BY_DTYPE(c, host_zal, c, iter, x, xn, xn2, U, V, x_tmp);
 }
 
+int admx_lbqn_step(admx_ctx* c, int64_t iter, const void* x, const void* xn, const void* xn2, void* U, void* V,
+                   void* x_tmp) {
+  CHK(check_ctx(c));
+  Guard g(c);
+  if (iter < 1) return bad("iter must be >= 1");
+  if (!x || !xn || !xn2 || !U || !V || !x_tmp) return bad("null pointer");
+  return BY_DTYPE(c, host_lbqn, c, iter, x, xn, xn2, U, V, x_tmp);
+}
+
 int admx_set_x(admx_ctx* c, int which, const void* host) {
   CHK(check_ctx(c));
   Guard g(c);
@@ -1381,6 +1433,9 @@ int admx_set_tuning(admx_ctx* c, int key, int64_t value) {
     if (value < ADMX_PREC_F64 || value > ADMX_PREC_BF16) return bad("precision must be ADMX_PREC_F64/F32/BF16");
     if (value != ADMX_PREC_F64 && c->dtype != ADMX_F64) return bad("mixed precision needs a FP64 context");
     c->precision = (int)value;
+  } else if (key == ADMX_TUNE_QN_MODE) {
+    if (value != ADMX_QN_ZAL && value != ADMX_QN_LBQN) return bad("quasi-Newton mode must be ADMX_QN_ZAL or ADMX_QN_LBQN");
+    c->qn_mode = (int)value;
   } else if (key == ADMX_TUNE_GRAPHS) {
     c->use_graphs = value != 0;
     if (!c->use_graphs) drop_graphs(c);

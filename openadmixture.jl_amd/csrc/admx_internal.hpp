@@ -89,6 +89,9 @@ template <typename T> void launch_zal(int K, int64_t I, int64_t J, const void* x
                                       void* U, void* V, void* x_out, int col, int m, int Qcols,
                                       double* scratch, int64_t scratch_len, hipStream_t s);
 // x <- (lls[prev] < lls[qn]) ? x_tmp : x_next2 ; lls[new] accordingly.
+template <typename T> void launch_lbqn(int K, int64_t I, int64_t J, const void* x, const void* xn, const void* xn2,
+                                       void* U, void* V, void* x_out, int m, double* scratch, int64_t scratch_len,
+                                       hipStream_t s);
 template <typename T> void launch_accept(int64_t N, void* x, const void* x_tmp, const void* x_next2, double* lls,
                                          hipStream_t s);
 

@@ -185,6 +185,138 @@ void launch_zal(int K, int64_t I, int64_t J, const void* x, const void* xn, cons
                      (const T*)xn, (const T*)V, coef, m, (T*)x_out);
 }
 
+// ------------------------------------------------------------------ LBQN quasi-Newton
+// update_UV_LBQN! (src/quasi_newton.jl:15-23) after the host has shifted the
+// history columns right: U[:,0] = x_next - x, V[:,0] = (x_next2 - x_next) - U[:,0],
+// and the partial inner products G[a,b] = V[:,a]'V[:,b] and r[a] = V[:,a]'U[:,0]
+// over the first m columns, in double.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lbqn_dots(int64_t N, const T* __restrict__ x, const T* __restrict__ xn,
+                                                   const T* __restrict__ xn2, T* __restrict__ U, T* __restrict__ V,
+                                                   int m, double* __restrict__ part) {
+  __shared__ double sh[256];
+  constexpr int NP = kMaxQ * kMaxQ + kMaxQ;
+  double acc[NP];
+#pragma unroll
+  for (int a = 0; a < NP; ++a) acc[a] = 0.0;
+  for (int64_t n = (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256) {
+    const T xv = x[n], xnv = xn[n], xn2v = xn2[n];
+    const T u = xnv - xv;
+    T v = xn2v - xnv;
+    v = v - u;
+    U[n] = u;
+    V[n] = v;
+    T vv[kMaxQ];
+#pragma unroll
+    for (int a = 0; a < kMaxQ; ++a) vv[a] = (a == 0) ? v : ((a < m) ? V[n + N * a] : (T)0);
+#pragma unroll
+    for (int a = 0; a < kMaxQ; ++a) {
+      acc[kMaxQ * kMaxQ + a] += (double)vv[a] * (double)u;
+#pragma unroll
+      for (int b = 0; b < kMaxQ; ++b) acc[a + kMaxQ * b] += (double)vv[a] * (double)vv[b];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < NP; ++a) {
+    const double tot = block_sum<256>(acc[a], sh);
+    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * NP + a] = tot;
+  }
+}
+
+// The two-loop recursion of update_QN_LBQN! (src/quasi_newton.jl:37-58) in
+// Gram form: with x_q^(i) = u - sum_{k<i} alpha_k V_k,
+//   alpha_i = (r_i - sum_{k<i} alpha_k G[i,k]) / G[i,i],  gamma = r_0 / G[0,0],
+// and x_next = x - gamma u + sum_i gamma alpha_i V_i - sum_i alpha_i U_i:
+// coef[0..kMaxQ) multiplies U, coef[kMaxQ..2kMaxQ) multiplies V.
+__global__ __launch_bounds__(64) void k_lbqn_solve(const double* __restrict__ part, int nblocks, int m,
+                                                   double* __restrict__ coef) {
+  constexpr int NP = kMaxQ * kMaxQ + kMaxQ;
+  __shared__ double tot[NP];
+  for (int a = threadIdx.x; a < NP; a += 64) {
+    double s = 0.0;
+    for (int b = 0; b < nblocks; ++b) s += part[(int64_t)b * NP + a];
+    tot[a] = s;
+  }
+  __syncthreads();
+  if (threadIdx.x != 0) return;
+  const double* G = tot;
+  const double* r = tot + kMaxQ * kMaxQ;
+  double alpha[kMaxQ];
+  for (int i = 0; i < m; ++i) {
+    double d = r[i];
+    for (int k = 0; k < i; ++k) d -= alpha[k] * G[i + kMaxQ * k];
+    alpha[i] = (1.0 / G[i + kMaxQ * i]) * d;
+  }
+  const double gamma = r[0] / G[0];
+  for (int a = 0; a < kMaxQ; ++a) {
+    coef[a] = (a < m) ? -alpha[a] - (a == 0 ? gamma : 0.0) : 0.0;
+    coef[kMaxQ + a] = (a < m) ? gamma * alpha[a] : 0.0;
+  }
+}
+
+// x_tmp = x + U cu + V cv, then project_q! on the Q columns ...
+template <typename T, int K>
+__global__ __launch_bounds__(256) void k_lbqn_apply_q(int64_t I, int64_t N, const T* __restrict__ x,
+                                                      const T* __restrict__ U, const T* __restrict__ V,
+                                                      const double* __restrict__ coef, int m, T* __restrict__ out) {
+  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (i >= I) return;
+  T v[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int64_t n = (int64_t)K * i + k;
+    T s = 0;
+    for (int a = 0; a < m; ++a) s += U[n + N * a] * (T)coef[a] + V[n + N * a] * (T)coef[kMaxQ + a];
+    v[k] = x[n] + s;
+  }
+  project_simplex<T, K>(v);
+#pragma unroll
+  for (int k = 0; k < K; ++k) out[(int64_t)K * i + k] = v[k];
+}
+
+// ... and project_p! on the P entries.
+template <typename T>
+__global__ __launch_bounds__(256) void k_lbqn_apply_p(int64_t n0, int64_t N, const T* __restrict__ x,
+                                                      const T* __restrict__ U, const T* __restrict__ V,
+                                                      const double* __restrict__ coef, int m, T* __restrict__ out) {
+  for (int64_t n = n0 + (int64_t)blockIdx.x * 256 + threadIdx.x; n < N; n += (int64_t)gridDim.x * 256) {
+    T s = 0;
+    for (int a = 0; a < m; ++a) s += U[n + N * a] * (T)coef[a] + V[n + N * a] * (T)coef[kMaxQ + a];
+    out[n] = project_box<T>(x[n] + s);
+  }
+}
+
+template <typename T, int K>
+static void lbqn_apply_q_k(int64_t I, int64_t N, const void* x, const void* U, const void* V, const double* coef,
+                           int m, void* out, hipStream_t s) {
+  hipLaunchKernelGGL((k_lbqn_apply_q<T, K>), dim3((unsigned)((I + 255) / 256)), dim3(256), 0, s, I, N, (const T*)x,
+                     (const T*)U, (const T*)V, coef, m, (T*)out);
+}
+
+// The history columns are shifted by the caller (device copies) before this.
+template <typename T>
+void launch_lbqn(int K, int64_t I, int64_t J, const void* x, const void* xn, const void* xn2, void* U, void* V,
+                 void* x_out, int m, double* scratch, int64_t scratch_len, hipStream_t s) {
+  const int64_t N = (int64_t)K * (I + J);
+  constexpr int NP = kMaxQ * kMaxQ + kMaxQ;
+  int64_t nb = grid_for(N, 512);
+  if ((nb * NP + 2 * kMaxQ) > scratch_len) nb = (scratch_len - 2 * kMaxQ) / NP;
+  double* part = scratch;
+  double* coef = scratch + nb * NP;
+  hipLaunchKernelGGL((k_lbqn_dots<T>), dim3((unsigned)nb), dim3(256), 0, s, N, (const T*)x, (const T*)xn,
+                     (const T*)xn2, (T*)U, (T*)V, m, part);
+  hipLaunchKernelGGL(k_lbqn_solve, dim3(1), dim3(64), 0, s, part, (int)nb, m, coef);
+  ADMX_DISPATCH_K(K, (lbqn_apply_q_k<T, KK>(I, N, x, U, V, coef, m, x_out, s)));
+  const int64_t np = N - (int64_t)K * I;
+  hipLaunchKernelGGL((k_lbqn_apply_p<T>), dim3(grid_for(np, 4096)), dim3(256), 0, s, (int64_t)K * I, N,
+                     (const T*)x, (const T*)U, (const T*)V, coef, m, (T*)x_out);
+}
+
+template void launch_lbqn<double>(int, int64_t, int64_t, const void*, const void*, const void*, void*, void*, void*,
+                                  int, double*, int64_t, hipStream_t);
+template void launch_lbqn<float>(int, int64_t, int64_t, const void*, const void*, const void*, void*, void*, void*,
+                                 int, double*, int64_t, hipStream_t);
+
 // ------------------------------------------------------------------ accept rule
 template <typename T>
 __global__ __launch_bounds__(256) void k_accept(int64_t N, T* __restrict__ x, const T* __restrict__ x_tmp,

@@ -259,6 +259,23 @@ class HIPAdmixData:
         N.check(N.lib().admx_zal_step(self._h, int(it), *[N.ptr(a) for a in args], N.ptr(U), N.ptr(V),
                                       N.ptr(x_tmp)))
 
+    def lbqn_step(self, it, x, x_next, x_next2, U, V, x_tmp):
+        shp = (self.K, self.I + self.J)
+        args = [_f(a, self.T, shp) for a in (x, x_next, x_next2)]
+        ushp = (self.K * (self.I + self.J), self.Q)
+        _f(U, self.T, ushp)
+        _f(V, self.T, ushp)
+        _f(x_tmp, self.T, shp)
+        N.check(N.lib().admx_lbqn_step(self._h, int(it), *[N.ptr(a) for a in args], N.ptr(U), N.ptr(V),
+                                       N.ptr(x_tmp)))
+
+    def set_qn_mode(self, mode: str):
+        """admixture_qn!'s quasi-Newton step: 'ZAL' or 'LBQN'."""
+        codes = {"ZAL": N.ADMX_QN_ZAL, "LBQN": N.ADMX_QN_LBQN}
+        if mode not in codes:
+            raise ValueError("mode must be 'ZAL' or 'LBQN'")
+        self.set_tuning(N.ADMX_TUNE_QN_MODE, codes[mode])
+
     # ------------------------------------------------------------ measurement / tuning
     def profile(self, on: bool = True):
         N.check(N.lib().admx_profile_enable(self._h, int(bool(on))))

@@ -429,6 +429,55 @@ void FN(orc_update_QN)(T *x_out, const T *x_mapped, const T *x, const T *U,
   }
 }
 
+/* ---- update_UV_LBQN!  (src/quasi_newton.jl:15-23): shift the first
+ * min(iter, Q) - 1 columns of U and V right by one, then
+ * U[:,1] = x_next - x and V[:,1] = (x_next2 - x_next) - U[:,1]. */
+void FN(orc_update_UV_LBQN)(T *U, T *V, const T *x, const T *x_next,
+                            const T *x_next2, int64_t N, int64_t iter, int64_t Q) {
+  const int64_t m = iter < Q ? iter : Q;
+  for (int64_t i = m - 1; i >= 1; --i) {
+    memcpy(U + N * i, U + N * (i - 1), sizeof(T) * N);
+    memcpy(V + N * i, V + N * (i - 1), sizeof(T) * N);
+  }
+  for (int64_t n = 0; n < N; ++n) {
+    U[n] = x_next[n] - x[n];
+    V[n] = x_next2[n] - x_next[n];
+    V[n] = V[n] - U[n];
+  }
+}
+
+/* ---- update_QN_LBQN!(x_next, x, x_q, x_r, U, V)  (src/quasi_newton.jl:37-58)
+ * with the first m columns: gamma = u'v / v'v (u, v the first columns),
+ * x_q = u; for i = 1..m: alpha_i = (V_i'x_q) / (V_i'V_i), x_q -= alpha_i V_i;
+ * x_r = gamma x_q + sum_i alpha_i U_i; x_next = x - x_r.  Inner products in
+ * double, vector updates in T in the reference's order. */
+void FN(orc_update_QN_LBQN)(T *x_out, const T *x, const T *U, const T *V, int64_t N, int64_t m) {
+  T *xq = (T *)malloc(sizeof(T) * N), *xr = (T *)malloc(sizeof(T) * N);
+  double uv = 0, vv = 0, alpha[16] = {0};
+  for (int64_t n = 0; n < N; ++n) {
+    uv += (double)U[n] * (double)V[n];
+    vv += (double)V[n] * (double)V[n];
+  }
+  const double gamma = uv / vv;
+  memcpy(xq, U, sizeof(T) * N);
+  for (int64_t i = 0; i < m; ++i) {
+    const T *vi = V + N * i;
+    double d2 = 0, dq = 0;
+    for (int64_t n = 0; n < N; ++n) {
+      d2 += (double)vi[n] * (double)vi[n];
+      dq += (double)vi[n] * (double)xq[n];
+    }
+    alpha[i] = (1.0 / d2) * dq;
+    for (int64_t n = 0; n < N; ++n) xq[n] = xq[n] - (T)alpha[i] * vi[n];
+  }
+  for (int64_t n = 0; n < N; ++n) xr[n] = (T)gamma * xq[n];
+  for (int64_t i = 0; i < m; ++i)
+    for (int64_t n = 0; n < N; ++n) xr[n] = xr[n] + (T)alpha[i] * U[n + N * i];
+  for (int64_t n = 0; n < N; ++n) x_out[n] = x[n] - xr[n];
+  free(xq);
+  free(xr);
+}
+
 /* ----------------------------------------------------------------------------
  * Whole-algorithm state (AdmixData{T}, src/structs.jl:9-91), host arrays only.
  */
@@ -537,9 +586,10 @@ void FN(orc_update_p)(FN(orc_state) * st, int update2, int32_t *status) {
   FN(orc_project_p)(p_next, K * J);
 }
 
-/* One iteration of admixture_qn!(d, g; mode=:ZAL)  (src/algorithms_outer.jl:92-159).
- * iter is the 1-based outer index i.  out = {prev, basic, qn, new, reldiff}. */
-void FN(orc_qn_iteration)(FN(orc_state) * st, int64_t iter, int fix_q, int fix_p, double *out) {
+/* One iteration of admixture_qn!(d, g; mode)  (src/algorithms_outer.jl:92-159),
+ * mode 0 = :ZAL, 1 = :LBQN (:127-136).  iter is the 1-based outer index i.
+ * out = {prev, basic, qn, new, reldiff}. */
+void FN(orc_qn_iteration_mode)(FN(orc_state) * st, int64_t iter, int fix_q, int fix_p, int mode, double *out) {
   const int64_t K = st->K, I = st->I, J = st->J, N = K * (I + J);
   st->ll_prev = st->ll_new;
   const double ll_prev = st->ll_prev;
@@ -554,9 +604,14 @@ void FN(orc_qn_iteration)(FN(orc_state) * st, int64_t iter, int fix_q, int fix_p
   /* update_q!/update_p! set d.ll_prev = d.ll_new; the outer value is the one compared */
   st->ll_prev = ll_prev;
   const double ll_basic = FN(st_ll)(st, st->x_next2);
-  FN(orc_update_UV)(st->U, st->V, st->x, st->x_next, st->x_next2, N, iter, st->Q);
   const int64_t m = iter < st->Q ? iter : st->Q;
-  FN(orc_update_QN)(st->x_tmp, st->x_next, st->x, st->U, st->V, N, m);
+  if (mode == 1) {
+    FN(orc_update_UV_LBQN)(st->U, st->V, st->x, st->x_next, st->x_next2, N, iter, st->Q);
+    FN(orc_update_QN_LBQN)(st->x_tmp, st->x, st->U, st->V, N, m);
+  } else {
+    FN(orc_update_UV)(st->U, st->V, st->x, st->x_next, st->x_next2, N, iter, st->Q);
+    FN(orc_update_QN)(st->x_tmp, st->x_next, st->x, st->U, st->V, N, m);
+  }
   FN(orc_project_p)(FN(ppart)(st, st->x_tmp), K * J);
   FN(orc_project_q)(FN(qpart)(st, st->x_tmp), K, I);
   const double ll_qn = FN(st_ll)(st, st->x_tmp);
@@ -572,6 +627,10 @@ void FN(orc_qn_iteration)(FN(orc_state) * st, int64_t iter, int fix_q, int fix_p
   out[2] = ll_qn;
   out[3] = st->ll_new;
   out[4] = fabs((st->ll_new - st->ll_prev) / st->ll_prev);
+}
+
+void FN(orc_qn_iteration)(FN(orc_state) * st, int64_t iter, int fix_q, int fix_p, double *out) {
+  FN(orc_qn_iteration_mode)(st, iter, fix_q, fix_p, 0, out);
 }
 
 #undef FN

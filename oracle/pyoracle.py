@@ -74,6 +74,9 @@ def _declare(L):
         getattr(L, "orc_update_q" + s).argtypes = [c_vp, C.c_int, c_vp]
         getattr(L, "orc_update_p" + s).argtypes = [c_vp, C.c_int, c_vp]
         getattr(L, "orc_qn_iteration" + s).argtypes = [c_vp, c_i64, C.c_int, C.c_int, c_vp]
+        getattr(L, "orc_qn_iteration_mode" + s).argtypes = [c_vp, c_i64, C.c_int, C.c_int, C.c_int, c_vp]
+        getattr(L, "orc_update_UV_LBQN" + s).argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64]
+        getattr(L, "orc_update_QN_LBQN" + s).argtypes = [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64]
 
 
 def _p(a):
@@ -231,6 +234,16 @@ def update_QN(x_out, x_mapped, x, U, V, m):
     getattr(lib(), "orc_update_QN" + _sfx(x.dtype))(_p(x_out), _p(x_mapped), _p(x), _p(U), _p(V), N, m)
 
 
+def update_UV_LBQN(U, V, x, x_next, x_next2, it, Q):
+    N = x.size
+    getattr(lib(), "orc_update_UV_LBQN" + _sfx(x.dtype))(_p(U), _p(V), _p(x), _p(x_next), _p(x_next2), N, it, Q)
+
+
+def update_QN_LBQN(x_out, x, U, V, m):
+    N = x.size
+    getattr(lib(), "orc_update_QN_LBQN" + _sfx(x.dtype))(_p(x_out), _p(x), _p(U), _p(V), N, m)
+
+
 # ---------------------------------------------------------------- whole algorithm
 class _StateF64(C.Structure):
     _fields_ = [("I", c_i64), ("J", c_i64), ("K", c_i64), ("Q", c_i64),
@@ -309,7 +322,8 @@ class OracleAdmix:
         self._f("orc_update_p")(C.byref(self.st), int(update2), _p(status))
         return status
 
-    def qn_iteration(self, it, fix_q=False, fix_p=False):
+    def qn_iteration(self, it, fix_q=False, fix_p=False, mode="ZAL"):
         out = np.zeros(5)
-        self._f("orc_qn_iteration")(C.byref(self.st), it, int(fix_q), int(fix_p), _p(out))
+        self._f("orc_qn_iteration_mode")(C.byref(self.st), it, int(fix_q), int(fix_p), 1 if mode == "LBQN" else 0,
+                                         _p(out))
         return dict(prev=out[0], basic=out[1], qn=out[2], new=out[3], reldiff=out[4])

@@ -377,3 +377,42 @@ def test_f32_oracle_tracks_f64(O):
     a = O.loglikelihood(bed, pitch, q, p)
     b = O.loglikelihood(bed, pitch, q.astype(np.float32), p.astype(np.float32))
     assert abs(a - b) <= 1e-5 * abs(a)
+
+
+# ---------------------------------------------------------------- LBQN (src/quasi_newton.jl:15-58)
+@pytest.mark.parametrize("m_hist", [1, 3])
+def test_lbqn_jumps_to_the_fixed_point_of_a_scalar_linear_map(O, m_hist):
+    """For F(x) = a x + b (any dimension), every V column is (a - 1) times its U
+    column, so the two-loop recursion gives x - u/(a-1) = b/(1-a) exactly."""
+    rng = np.random.default_rng(4)
+    N, Q, a = 50, 3, 0.7
+    b = rng.normal(size=N)
+    U = np.asfortranarray(rng.normal(size=(N, Q)))
+    V = np.asfortranarray(rng.normal(size=(N, Q)))
+    x = rng.normal(size=N)
+    for it in range(1, m_hist + 1):  # fill the history with iterates of the same map
+        x1 = a * x + b
+        x2 = a * x1 + b
+        O.update_UV_LBQN(U, V, x, x1, x2, it, Q)
+        if it < m_hist:
+            x = x2
+    m = min(m_hist, Q)
+    out = np.empty(N)
+    O.update_QN_LBQN(out, x, U, V, m)
+    assert np.allclose(out, b / (1 - a), rtol=0, atol=1e-12)
+
+
+def test_lbqn_history_shift(O):
+    rng = np.random.default_rng(5)
+    N, Q = 20, 3
+    U = np.asfortranarray(rng.normal(size=(N, Q)))
+    V = np.asfortranarray(rng.normal(size=(N, Q)))
+    U0, V0 = U.copy(order="F"), V.copy(order="F")
+    x, x1, x2 = rng.normal(size=(3, N))
+    O.update_UV_LBQN(U, V, x, x1, x2, 5, Q)  # iter >= Q: columns shift right, the last drops
+    assert np.array_equal(U[:, 1:], U0[:, :2]) and np.array_equal(V[:, 1:], V0[:, :2])
+    assert np.array_equal(U[:, 0], x1 - x)
+    assert np.array_equal(V[:, 0], (x2 - x1) - (x1 - x))
+    U, V = U0.copy(order="F"), V0.copy(order="F")
+    O.update_UV_LBQN(U, V, x, x1, x2, 2, Q)  # iter 2: only column 1 shifts into 2
+    assert np.array_equal(U[:, 1], U0[:, 0]) and np.array_equal(U[:, 2], U0[:, 2])
