@@ -52,6 +52,10 @@ extern "C" {
 #define ADMX_TUNE_GRAPHS 3        /* 1: admx_qn_iteration replays steady-state iterations (iter > Q) from
                                    * captured hipGraphs (not while profiling); results are identical */
 #define ADMX_TUNE_QN_MODE 4       /* quasi-Newton step of admx_qn_iteration: ADMX_QN_ZAL or ADMX_QN_LBQN */
+#define ADMX_TUNE_SKIPMISSING 5   /* 1 (default): the *_skipmissing loops; 0: the mean-imputation loops
+                                   * (src/loops.jl:493-514, 589-608, 679-698, 840-865, 896-921), where a
+                                   * missing genotype counts as the SNP's mean over its observed entries;
+                                   * every pass and entry point follows it (FP64/FP32 passes only) */
 
 /* values of ADMX_TUNE_QN_MODE: admixture_qn!'s `mode` (src/algorithms_outer.jl:127-136) */
 #define ADMX_QN_ZAL 0

@@ -24,8 +24,9 @@ from .algorithms import admixture_qn, em_p, em_q, init_em, loglikelihood, update
 from .cv import best_k, cv_sweep, fit_on_device
 from .driver import _admixture_base, hip_admixture_base, run_admixture
 from .hip import HIPAdmixData
-from .loops import (UnitRange, em_p_loop_skipmissing, em_q_loop_skipmissing, loglikelihood_loop_skipmissing,
-                    update_p_loop_skipmissing, update_q_loop_skipmissing)
+from .loops import (UnitRange, em_p_loop, em_p_loop_skipmissing, em_q_loop, em_q_loop_skipmissing,
+                    loglikelihood_loop, loglikelihood_loop_skipmissing, update_p_loop, update_p_loop_skipmissing,
+                    update_q_loop, update_q_loop_skipmissing)
 from .plink import SnpArray, SnpLinAlg, write_plink
 from .structs import AdmixData
 from .tune import autotune_waves
@@ -39,6 +40,7 @@ __all__ = [
     "init_em", "admixture_qn", "loglikelihood", "em_q", "em_p", "update_q", "update_p",
     "loglikelihood_loop_skipmissing", "em_q_loop_skipmissing", "em_p_loop_skipmissing",
     "update_q_loop_skipmissing", "update_p_loop_skipmissing", "UnitRange",
+    "loglikelihood_loop", "em_q_loop", "em_p_loop", "update_q_loop", "update_p_loop",
     "SnpArray", "SnpLinAlg", "write_plink", "AdmixHIPError", "device_count",
     "cv_sweep", "best_k", "fit_on_device", "autotune_waves",
     "MLP", "Panel", "training_set", "warm_started_fit", "world_frequencies",

@@ -19,6 +19,7 @@ ADMX_X, ADMX_X_NEXT, ADMX_X_NEXT2, ADMX_X_TMP = 0, 1, 2, 3
 ADMX_SIDE_Q, ADMX_SIDE_P = 0, 1
 ADMX_TUNE_TARGET_WAVES, ADMX_TUNE_KEEP_HESSIANS, ADMX_TUNE_PRECISION, ADMX_TUNE_GRAPHS = 0, 1, 2, 3
 ADMX_TUNE_QN_MODE = 4
+ADMX_TUNE_SKIPMISSING = 5
 ADMX_QN_ZAL, ADMX_QN_LBQN = 0, 1
 ADMX_PREC_F64, ADMX_PREC_F32, ADMX_PREC_BF16 = 0, 1, 2
 

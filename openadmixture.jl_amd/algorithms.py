@@ -28,19 +28,24 @@ def _dev(d: AdmixData, d_cu) -> HIPAdmixData:
     if not isinstance(c, HIPAdmixData):
         raise TypeError("a HIP device context is required (d_cu=..., from hip_admixture_base); "
                         "this package has no CPU path")
-    if not d.skipmissing:
-        raise ValueError("`skipmissing` must be true for GPU computation")
     if (c.I, c.J, c.K) != (d.I, d.J, d.K) or c.T != d.T:
         raise ValueError("device context does not match AdmixData (I, J, K, T)")
+    # the reference's CUDA path asserts skipmissing (src/algorithms_inner.jl:191, 267);
+    # this backend runs both loop families
+    c.set_skipmissing(d.skipmissing)
     return c
 
 
 def loglikelihood(g, q, p, qp_small=None, K=None, skipmissing=True) -> float:
-    """loglikelihood(g, q, p, qp_small, K, skipmissing)  (src/algorithms_inner.jl:27-35)."""
-    if not skipmissing:
-        raise NotImplementedError("mean-imputation loop variants are not provided")
+    """loglikelihood(g, q, p, qp_small, K, skipmissing)  (src/algorithms_inner.jl:27-35):
+    loglikelihood_loop_skipmissing or, with skipmissing=False, the mean-imputation
+    loglikelihood_loop (src/loops.jl:493-514)."""
     c = _ctx(g)
-    return c.loglikelihood_loop(np.asfortranarray(q), np.asfortranarray(p), 0, c.I, 0, c.J)
+    c.set_skipmissing(skipmissing)
+    try:
+        return c.loglikelihood_loop(np.asfortranarray(q), np.asfortranarray(p), 0, c.I, 0, c.J)
+    finally:
+        c.set_skipmissing(True)
 
 
 def em_q(d: AdmixData, g=None, mode="base", d_cu=None, verbose=False) -> None:

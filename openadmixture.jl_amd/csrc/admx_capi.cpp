@@ -96,6 +96,9 @@ struct admx_ctx {
   int precision = ADMX_PREC_F64;
   DevBuf fq, fp, ppt, mmt, qqt;  // mixed-precision pass inputs
   DevBuf feat, featpart;          // warm-start features (I x 64) and their chunk partials
+  int impute = 0;                 // 1: mean-imputation loops (skipmissing = false)
+  bool mu_valid = false;          // mu holds the per-SNP means of the current genotypes
+  DevBuf snp_mu, cnt;             // per-SNP means (T, J); count scratch
   int64_t feat_rows = 0;          // individuals with features in `feat` (0: none)
   std::mutex mu;
   Prof prof;
@@ -248,6 +251,24 @@ const char* mode_name(bool qside, int mode) {
   return mode == MODE_UPDATE ? "pside_update" : "pside_em";
 }
 
+// Per-SNP means of the current genotypes for the mean-imputation loops
+// (computed once per genotype upload).
+template <typename T>
+int ensure_mu(admx_ctx* c) {
+  if (!c->impute || c->mu_valid) return ADMX_OK;
+  CHK(c->snp_mu.grow(sizeof(T) * (size_t)c->J));
+  CHK(c->cnt.grow(sizeof(unsigned long long) * (size_t)(2 * c->J + c->I)));
+  auto* nm = static_cast<unsigned long long*>(c->cnt.p);
+  {
+    ProfScope ps(c, "snp_mean");
+    launch_counts(c->bed, c->pitch, c->I, c->J, nm, nm + c->J, nm + 2 * c->J, c->s);
+    launch_snp_mean<T>(nm, nm + c->J, c->J, static_cast<T*>(c->snp_mu.p), c->s);
+  }
+  CHK(check_launch());
+  c->mu_valid = true;
+  return ADMX_OK;
+}
+
 // Run a Q-side pass; returns the chunk count (partials in c->part or c->llpart).
 template <typename T>
 int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0, int64_t j1,
@@ -258,7 +279,8 @@ int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
   if (Ilen <= 0 || Jlen <= 0) return ADMX_OK;
   int64_t chunk = 0;
   const int n = qside_chunks(c, Ilen, Jlen, &chunk);
-  PassArgs a{c->bed, c->pitch, c->I, q, p, i0, i1, j0, j1, chunk, nullptr};
+  CHK(ensure_mu<T>(c));
+  PassArgs a{c->bed, c->pitch, c->I, q, p, i0, i1, j0, j1, chunk, nullptr, c->impute ? c->snp_mu.p : nullptr};
   if (mode == MODE_LL) {
     const int64_t np = ((Ilen + 255) / 256) * n;
     CHK(c->llpart.grow((size_t)np * sizeof(double)));
@@ -287,7 +309,8 @@ int pass_p(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
   const int n = pside_chunks(c, i0, i1, Jlen, &chunk);
   const int na = pass_na(mode, false, (int)c->K, sizeof(T) == 8);
   CHK(c->part.grow((size_t)n * na * Jlen * sizeof(T)));
-  PassArgs a{c->bed, c->pitch, c->I, q, p, i0, i1, j0, j1, chunk, c->part.p};
+  CHK(ensure_mu<T>(c));
+  PassArgs a{c->bed, c->pitch, c->I, q, p, i0, i1, j0, j1, chunk, c->part.p, c->impute ? c->snp_mu.p : nullptr};
   {
     ProfScope ps(c, mode_name(false, mode));
     launch_pside<T>(mode, (int)c->K, a, n, c->s);
@@ -311,6 +334,7 @@ int ll_to_slot(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1
 // BF16 pair tables, the FP32/BF16 pass, then the FP64 finalize + QP + projection.
 int mixed_update(admx_ctx* c, bool qside, const double* q, const double* p, const double* x_in, double* x_out,
                  int32_t* status_dev) {
+  if (c->impute) return bad("the mean-imputation loops run with the FP64/FP32 passes (ADMX_PREC_F64)");
   const int64_t I = c->I, J = c->J, K = c->K;
   if (c->fq.n < sizeof(float) * K * (I + 16)) {
     CHK(c->fq.grow(sizeof(float) * K * (I + 16)));
@@ -557,6 +581,8 @@ uint64_t layout_sig(const admx_ctx* c) {
   mix((uint64_t)c->keep_hess);
   mix((uint64_t)c->precision);
   mix((uint64_t)c->qn_mode);
+  mix((uint64_t)c->impute);
+  mix((uint64_t)(uintptr_t)c->snp_mu.p);
   return h;
 }
 
@@ -1009,6 +1035,8 @@ int admx_destroy(admx_ctx* c) {
   c->qqt.release();
   c->feat.release();
   c->featpart.release();
+  c->snp_mu.release();
+  c->cnt.release();
   if (c->s) (void)hipStreamDestroy(c->s);
   delete c;
   return ADMX_OK;
@@ -1024,6 +1052,7 @@ int admx_upload_bed(admx_ctx* c, const uint8_t* bed, int64_t bytes_per_snp) {
   HIPCHK(hipMemcpy2DAsync(c->bed, c->pitch, bed, bytes_per_snp, c->ibytes, c->J, hipMemcpyHostToDevice, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   c->have_bed = true;
+  c->mu_valid = false;
   return ADMX_OK;
 }
 
@@ -1053,6 +1082,7 @@ int admx_generate_bed(admx_ctx* c, const double* Qt, const double* Pt, uint64_t 
   CHK(check_launch());
   HIPCHK(hipStreamSynchronize(c->s));
   c->have_bed = true;
+  c->mu_valid = false;
   return ADMX_OK;
 }
 
@@ -1075,6 +1105,7 @@ int admx_bed_from(admx_ctx* dst, admx_ctx* src, int32_t fold, int32_t nfolds, ui
   CHK(pad_bed(dst));
   HIPCHK(hipStreamSynchronize(dst->s));
   dst->have_bed = true;
+  dst->mu_valid = false;
   return ADMX_OK;
 }
 
@@ -1433,6 +1464,8 @@ int admx_set_tuning(admx_ctx* c, int key, int64_t value) {
     if (value < ADMX_PREC_F64 || value > ADMX_PREC_BF16) return bad("precision must be ADMX_PREC_F64/F32/BF16");
     if (value != ADMX_PREC_F64 && c->dtype != ADMX_F64) return bad("mixed precision needs a FP64 context");
     c->precision = (int)value;
+  } else if (key == ADMX_TUNE_SKIPMISSING) {
+    c->impute = value == 0;
   } else if (key == ADMX_TUNE_QN_MODE) {
     if (value != ADMX_QN_ZAL && value != ADMX_QN_LBQN) return bad("quasi-Newton mode must be ADMX_QN_ZAL or ADMX_QN_LBQN");
     c->qn_mode = (int)value;

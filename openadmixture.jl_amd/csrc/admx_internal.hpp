@@ -22,6 +22,7 @@ struct PassArgs {
   int64_t i0, i1, j0, j1;
   int64_t chunk;   // SNPs per chunk (Q side) / individuals per chunk, multiple of 16 (P side)
   void* part;      // partial sums [nchunk][NA][n] (UPDATE/EM) or per-block doubles (LL)
+  const void* mu = nullptr;  // per-SNP means (T, J): mean-imputation loops; nullptr: skipmissing
 };
 
 // Per-column reduction of pass partials, optional QP + projection.
@@ -67,7 +68,7 @@ void launch_fin_q_update_mixed(int K, const FinArgs& f, hipStream_t s);
 void launch_fin_p_update_mixed(int K, const FinArgs& f, hipStream_t s);
 // Mixed precision passes (FP32 weights/gradient, BF16 MFMA Hessian) and their inputs.
 void launch_to_float(const double* x, int64_t n, float* y, hipStream_t s);
-// [p, 1-p] per column in FP32 (2K floats), the Q-side BF16 pass operand.
+// BF16 pair tables PP (and MM) [pair][column] of x, the B operands of the MFMA passes.
 void launch_pair_tables(int K, const double* x, int64_t n, int64_t ld, void* PP, void* MM, hipStream_t s);
 int pair_table_rows(int K);
 void launch_qside_bf16(int K, const PassArgs& a, int nchunk, const float* fq, const float* fp, const void* PPt,
@@ -88,10 +89,11 @@ void launch_reduce_sum(const double* parts, int64_t nparts, double* out, int acc
 template <typename T> void launch_zal(int K, int64_t I, int64_t J, const void* x, const void* xn, const void* xn2,
                                       void* U, void* V, void* x_out, int col, int m, int Qcols,
                                       double* scratch, int64_t scratch_len, hipStream_t s);
-// x <- (lls[prev] < lls[qn]) ? x_tmp : x_next2 ; lls[new] accordingly.
+// LBQN quasi-Newton (src/quasi_newton.jl:15-58); the caller shifts the history first.
 template <typename T> void launch_lbqn(int K, int64_t I, int64_t J, const void* x, const void* xn, const void* xn2,
                                        void* U, void* V, void* x_out, int m, double* scratch, int64_t scratch_len,
                                        hipStream_t s);
+// x <- (lls[prev] < lls[qn]) ? x_tmp : x_next2 ; lls[new] accordingly.
 template <typename T> void launch_accept(int64_t N, void* x, const void* x_tmp, const void* x_next2, double* lls,
                                          hipStream_t s);
 
@@ -99,12 +101,16 @@ template <typename T> void launch_accept(int64_t N, void* x, const void* x_tmp, 
 void launch_decode(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int8_t* g, hipStream_t s);
 void launch_counts(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, unsigned long long* snp_nm,
                    unsigned long long* snp_alt, unsigned long long* ind_nm, hipStream_t s);
-// Synthetic genotypes from (Q,P) (double), seeded counter hash.
+// Per-SNP mean genotype over the observed entries (mean-imputation loops).
+template <typename T> void launch_snp_mean(const unsigned long long* snp_nm, const unsigned long long* snp_alt,
+                                           int64_t J, T* mu, hipStream_t s);
+// Cross-validation folds: masked copy of a BED, held-out deviance partials.
 void launch_mask_fold(const uint8_t* src, uint8_t* dst, int64_t pitch, int64_t I, int64_t J, int fold, int nfolds,
                       uint64_t seed, hipStream_t s);
 int64_t heldout_dev_blocks(int64_t I, int64_t J, int64_t* jchunk);
 void launch_heldout_dev(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int K, const double* q,
                         const double* p, int fold, int nfolds, uint64_t seed, double* part, hipStream_t s);
+// Synthetic genotypes from (Q,P) (double), seeded counter hash.
 void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, int K, uint64_t seed,
                     double miss_rate, uint8_t* bed, int64_t pitch, hipStream_t s);
 

@@ -443,6 +443,25 @@ void launch_counts(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, unsi
                      I, J, jchunk, ind_nm);
 }
 
+// SnpLinAlg's mean genotype of SNP j over its observed entries, in double,
+// stored as T (0 for a SNP with no observed entry).
+template <typename T>
+__global__ __launch_bounds__(256) void k_snp_mean(const unsigned long long* __restrict__ nm,
+                                                  const unsigned long long* __restrict__ alt, int64_t J,
+                                                  T* __restrict__ mu) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j < J) mu[j] = nm[j] ? (T)((double)alt[j] / (double)nm[j]) : (T)0;
+}
+
+template <typename T>
+void launch_snp_mean(const unsigned long long* nm, const unsigned long long* alt, int64_t J, T* mu, hipStream_t s) {
+  hipLaunchKernelGGL((k_snp_mean<T>), dim3((unsigned)((J + 255) / 256)), dim3(256), 0, s, nm, alt, J, mu);
+}
+template void launch_snp_mean<double>(const unsigned long long*, const unsigned long long*, int64_t, double*,
+                                      hipStream_t);
+template void launch_snp_mean<float>(const unsigned long long*, const unsigned long long*, int64_t, float*,
+                                     hipStream_t);
+
 // ------------------------------------------------------------------ synthetic genotypes
 __device__ __forceinline__ uint64_t smix(uint64_t z) {
   z += 0x9E3779B97F4A7C15ULL;

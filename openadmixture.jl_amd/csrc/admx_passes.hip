@@ -11,6 +11,10 @@
 // streaming individuals; q[:,i] is wave-uniform (SGPRs) and each lane reads
 // 16 genotypes of its own SNP column per 32-bit load.
 //
+// IMP selects the mean-imputation loops (loops.jl:493-514, 589-608, 679-698,
+// 840-865, 896-921): a missing genotype counts as g = mu_j (the SNP's mean,
+// gm = mu_j, hm = 2 - mu_j) instead of being skipped.
+//
 // Every chunk writes its partial sums to HBM; a second kernel reduces them in
 // a fixed order, so results are bitwise reproducible run to run (the
 // reference's CUDA path uses float atomics).
@@ -30,7 +34,7 @@ namespace admx {
 // (one FMA per Hessian entry instead of two; its cancellation costs at most a
 // factor 1/min(p,1-p)^2 <= 1e10 on a 1e-16 unit roundoff).  FP32 keeps the
 // direct two-term form, where that factor would matter.
-template <typename T, int K, int MODE, bool TRICK>
+template <typename T, int K, int MODE, bool TRICK, bool IMP>
 __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
   constexpr int NT = Tri<K>::N;
   const T* __restrict__ q = static_cast<const T*>(a.q);
@@ -60,26 +64,41 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
 #pragma unroll
   for (int k = 0; k < ((MODE == MODE_UPDATE && TRICK) ? K : 1); ++k) hb[k] = 0;
 
-  // Per-genotype accumulation for genotype code c and P column pv.
-  auto step = [&](uint32_t c, const T (&pv)[K]) {
+  const T* __restrict__ mu = static_cast<const T*>(a.mu);
+  // (m*g, m*(2-g)) of code c in SNP j; imputed mean for a missing code under IMP.
+  auto weights = [&](uint32_t c, int64_t j, T& gm, T& hm) {
+    gm = (T)code_g(c);
+    hm = (T)code_h(c);
+    if constexpr (IMP) {
+      if (c == 1u) {
+        const T mj = mu[j];
+        gm = mj;
+        hm = (T)2 - mj;
+      }
+    }
+  };
+  // Per-genotype accumulation for genotype code c of SNP j and P column pv.
+  auto step = [&](uint32_t c, int64_t j, const T (&pv)[K]) {
     T s = qi[0] * pv[0];
 #pragma unroll
     for (int k = 1; k < K; ++k) s = __builtin_fma(qi[k], pv[k], s);
     const T t = (T)1 - s;
     if constexpr (MODE == MODE_LL) {
-      if constexpr (sizeof(T) == 8) {
+      if constexpr (sizeof(T) == 8 && !IMP) {
         // prod_j s^g (1-s)^(2-g), renormalised every 8 SNPs; one log per lane.
         const double x = (c >= 2u) ? (double)s : (double)t;
         const double y = (c == 3u) ? (double)s : (double)t;
         prod *= (c == 1u) ? 1.0 : x * y;
       } else {
         // Terms in T, accumulated in Float64 (src/loops.jl:523, 535).
-        const T gm = (T)code_g(c), hm = (T)code_h(c);
+        T gm, hm;
+        weights(c, j, gm, hm);
         const T term = gm * tlog<T>(s) + hm * tlog<T>(t);
         acc += (double)term;
       }
     } else {
-      const T gm = (T)code_g(c), hm = (T)code_h(c);
+      T gm, hm;
+      weights(c, j, gm, hm);
       const T r = rcp(s * t);
       const T is = t * r, it = s * r;
       const T cc = gm * is, dd = hm * it;
@@ -115,7 +134,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
     }
   };
   auto renorm = [&]() {
-    if constexpr (MODE == MODE_LL && sizeof(T) == 8) {
+    if constexpr (MODE == MODE_LL && sizeof(T) == 8 && !IMP) {
       int e;
       prod = __builtin_frexp(prod, &e);
       esum += e;
@@ -172,7 +191,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
         const int64_t jn = (u < 15) ? js + u + 1 : ((js + 16 < je) ? js + 16 : js + 15);
 #pragma unroll
         for (int k = 0; k < K; ++k) pn[k] = p[(int64_t)K * jn + k];
-        step((cur[u >> 2] >> (8 * (u & 3) + shw)) & 3u, pv);
+        step((cur[u >> 2] >> (8 * (u & 3) + shw)) & 3u, js + u, pv);
         if ((u & 7) == 7) renorm();
       }
     }
@@ -186,14 +205,14 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
       T pv[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) pv[k] = p[(int64_t)K * js + k];
-      step((gp[a.pitch * js] >> sh) & 3u, pv);
+      step((gp[a.pitch * js] >> sh) & 3u, js, pv);
       renorm();
     }
   }
 
   if constexpr (MODE == MODE_LL) {
     __shared__ double sh_red[256];
-    if constexpr (sizeof(T) == 8) acc = log(prod) + (double)esum * 0.69314718055994530942;
+    if constexpr (sizeof(T) == 8 && !IMP) acc = log(prod) + (double)esum * 0.69314718055994530942;
     const double tot = block_sum<256>(valid ? acc : 0.0, sh_red);
     if (threadIdx.x == 0) static_cast<double*>(a.part)[tile.y * nbx + tile.x] = tot;
     return;
@@ -222,7 +241,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
 //   Xtz_p[k,j]    += (c - d) q_k               (exact dLL/dp)
 //   XtX_p[k2,k,j] += (a + b) q_k q_k2          (exact -d2LL/dp2)
 //   EM: p_tmp += c q_k p_k ; p_next += d q_k (1 - p_k)
-template <typename T, int K, int MODE>
+template <typename T, int K, int MODE, bool IMP>
 __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
   constexpr int NT = Tri<K>::N;
   const T* __restrict__ q = static_cast<const T*>(a.q);
@@ -238,6 +257,8 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
 #pragma unroll
   for (int k = 0; k < K; ++k) pj[k] = p[k + (int64_t)K * j];
   const uint8_t* __restrict__ col = a.bed + a.pitch * j;
+  T mj = 0;
+  if constexpr (IMP) mj = static_cast<const T*>(a.mu)[j];
 
   T H[MODE == MODE_UPDATE ? NT : 1];
   T z[K], e2[MODE == MODE_EM ? K : 1];
@@ -249,7 +270,9 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
   for (int k = 0; k < (MODE == MODE_EM ? K : 1); ++k) e2[k] = 0;
 
   // Per-genotype work; qi is wave-uniform (scalar loads).
-  auto body = [&](uint32_t c, const T* __restrict__ qi) {
+  // inr = 0 marks an individual outside [i0, i1) whose code was set to missing:
+  // it adds nothing, also under IMP (only genuinely missing entries are imputed).
+  auto body = [&](uint32_t c, const T* __restrict__ qi, uint32_t inr) {
     T qv[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) qv[k] = qi[k];
@@ -257,7 +280,13 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
 #pragma unroll
     for (int k = 1; k < K; ++k) s = __builtin_fma(qv[k], pj[k], s);
     const T tt = (T)1 - s;
-    const T gm = (T)code_g(c), hm = (T)code_h(c);
+    T gm = (T)code_g(c), hm = (T)code_h(c);
+    if constexpr (IMP) {
+      if (c == 1u && inr) {
+        gm = mj;
+        hm = (T)2 - mj;
+      }
+    }
     const T r = rcp(s * tt);
     const T is = tt * r, it = s * r;
     const T cc = gm * is, dd = hm * it;
@@ -306,8 +335,9 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
     for (int sw = s0; sw < s1; ++sw) {
       const int64_t ib = it + 16 * sw;
       uint32_t w = tw[sw][threadIdx.x];
+      uint32_t keep = 0xffffffffu;
       if (ib < a.i0 || ib + 16 > a.i1) {  // uniform: mark out-of-range individuals missing
-        uint32_t keep = 0;
+        keep = 0;
 #pragma unroll
         for (int t = 0; t < 16; ++t)
           if (ib + t >= a.i0 && ib + t < a.i1) keep |= 3u << (2 * t);
@@ -316,12 +346,12 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
       if (ib + 16 <= a.I) {  // interior word: constant offsets from one base pointer
         const T* __restrict__ qb = q + (int64_t)K * ib;
 #pragma unroll
-        for (int t = 0; t < 16; ++t) body((w >> (2 * t)) & 3u, qb + K * t);
+        for (int t = 0; t < 16; ++t) body((w >> (2 * t)) & 3u, qb + K * t, (keep >> (2 * t)) & 1u);
       } else {  // last word of the cohort: clamp the (masked) individuals past I
 #pragma unroll
         for (int t = 0; t < 16; ++t) {
           const int64_t i = (ib + t < a.I) ? ib + t : a.I - 1;
-          body((w >> (2 * t)) & 3u, q + (int64_t)K * i);
+          body((w >> (2 * t)) & 3u, q + (int64_t)K * i, (keep >> (2 * t)) & 1u);
         }
       }
     }
@@ -344,22 +374,32 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
 }
 
 // ------------------------------------------------------------------ launchers
-template <typename T, int K>
-static void qside_k(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
+template <typename T, int K, bool IMP>
+static void qside_ki(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
   const int64_t n = a.i1 - a.i0;
   dim3 grid((unsigned)(((n + 255) / 256) * nchunk));  // 1-D, tiles ordered per XCD (xcd_tile)
   constexpr bool trick = sizeof(T) == 8;
-  if (mode == MODE_UPDATE) hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick>), grid, dim3(256), 0, s, a);
-  else if (mode == MODE_EM) hipLaunchKernelGGL((k_qside<T, K, MODE_EM, false>), grid, dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_qside<T, K, MODE_LL, false>), grid, dim3(256), 0, s, a);
+  if (mode == MODE_UPDATE) hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick, IMP>), grid, dim3(256), 0, s, a);
+  else if (mode == MODE_EM) hipLaunchKernelGGL((k_qside<T, K, MODE_EM, false, IMP>), grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_qside<T, K, MODE_LL, false, IMP>), grid, dim3(256), 0, s, a);
+}
+template <typename T, int K>
+static void qside_k(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
+  if (a.mu) qside_ki<T, K, true>(mode, a, nchunk, s);
+  else qside_ki<T, K, false>(mode, a, nchunk, s);
 }
 
-template <typename T, int K>
-static void pside_k(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
+template <typename T, int K, bool IMP>
+static void pside_ki(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
   const int64_t n = a.j1 - a.j0;
   dim3 grid((unsigned)((n + 255) / 256), (unsigned)nchunk);
-  if (mode == MODE_UPDATE) hipLaunchKernelGGL((k_pside<T, K, MODE_UPDATE>), grid, dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_pside<T, K, MODE_EM>), grid, dim3(256), 0, s, a);
+  if (mode == MODE_UPDATE) hipLaunchKernelGGL((k_pside<T, K, MODE_UPDATE, IMP>), grid, dim3(256), 0, s, a);
+  else hipLaunchKernelGGL((k_pside<T, K, MODE_EM, IMP>), grid, dim3(256), 0, s, a);
+}
+template <typename T, int K>
+static void pside_k(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
+  if (a.mu) pside_ki<T, K, true>(mode, a, nchunk, s);
+  else pside_ki<T, K, false>(mode, a, nchunk, s);
 }
 
 template <typename T> void launch_qside(int mode, int K, const PassArgs& a, int nchunk, hipStream_t s) {

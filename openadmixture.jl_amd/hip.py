@@ -269,6 +269,12 @@ class HIPAdmixData:
         N.check(N.lib().admx_lbqn_step(self._h, int(it), *[N.ptr(a) for a in args], N.ptr(U), N.ptr(V),
                                        N.ptr(x_tmp)))
 
+    def set_skipmissing(self, skip: bool = True):
+        """True: the *_skipmissing loops; False: the mean-imputation loops (a
+        missing genotype counts as its SNP's mean over the observed entries)."""
+        self.set_tuning(N.ADMX_TUNE_SKIPMISSING, int(bool(skip)))
+        self._skipmissing = bool(skip)
+
     def set_qn_mode(self, mode: str):
         """admixture_qn!'s quasi-Newton step: 'ZAL' or 'LBQN'."""
         codes = {"ZAL": N.ADMX_QN_ZAL, "LBQN": N.ADMX_QN_LBQN}

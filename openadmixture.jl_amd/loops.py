@@ -96,3 +96,47 @@ def update_p_loop_skipmissing(XtX, Xtz, g, q, p, qp_small, irange, jrange, K) ->
     i0, i1 = _bounds(irange)
     j0, j1 = _bounds(jrange)
     c.update_p_loop(XtX, Xtz, q, p, i0, i1, j0, j1)
+
+
+# ---------------------------------------------------------------- mean-imputation loops
+def _impute(c, fn, *args):
+    c.set_skipmissing(False)
+    try:
+        return fn(*args)
+    finally:
+        c.set_skipmissing(True)
+
+
+def loglikelihood_loop(g, q, p, qp_small, irange, jrange, K) -> float:
+    """src/loops.jl:493-514: sum of g log s + (2-g) log(1-s) with a missing g = mu_j."""
+    c = _ctx(g)
+    _check_k(c, K)
+    return _impute(c, c.loglikelihood_loop, q, p, *_bounds(irange), *_bounds(jrange))
+
+
+def em_q_loop(q_next, g, q, p, qp_small, irange, jrange, K) -> None:
+    """src/loops.jl:589-608: em_q_loop! (missing g = mu_j)."""
+    c = _ctx(g)
+    _check_k(c, K)
+    _impute(c, c.em_q_loop, q_next, q, p, *_bounds(irange), *_bounds(jrange))
+
+
+def em_p_loop(p_next, g, q, p, p_tmp, qp_small, irange, jrange, K) -> None:
+    """src/loops.jl:679-698: em_p_loop! (missing g = mu_j)."""
+    c = _ctx(g)
+    _check_k(c, K)
+    _impute(c, c.em_p_loop, p_next, p_tmp, q, p, *_bounds(irange), *_bounds(jrange))
+
+
+def update_q_loop(XtX, Xtz, g, q, p, qp_small, irange, jrange, K) -> None:
+    """src/loops.jl:840-865: update_q_loop! (missing g = mu_j)."""
+    c = _ctx(g)
+    _check_k(c, K)
+    _impute(c, c.update_q_loop, XtX, Xtz, q, p, *_bounds(irange), *_bounds(jrange))
+
+
+def update_p_loop(XtX, Xtz, g, q, p, qp_small, irange, jrange, K) -> None:
+    """src/loops.jl:896-921: update_p_loop! (missing g = mu_j)."""
+    c = _ctx(g)
+    _check_k(c, K)
+    _impute(c, c.update_p_loop, XtX, Xtz, q, p, *_bounds(irange), *_bounds(jrange))

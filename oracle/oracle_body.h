@@ -26,18 +26,33 @@ static void FN(qp_block)(T *qp, const T *q, const T *p, int64_t K, int64_t i0,
     }
 }
 
+/* Genotype weight g and nonmissing flag m of code c in SNP j.  mu == NULL: the
+ * _skipmissing loops (g_map, nonmissing_map; src/loops.jl:2-5).  Otherwise the
+ * mean-imputation loops: g = g_map[c] + (c == missing) * mu[j], m = 1
+ * (src/loops.jl:493-514, 589-608, 679-698, 840-865, 896-921). */
+static inline void FN(gw)(int c, const T *mu, int64_t j, T *g, T *m) {
+  if (mu) {
+    *g = (T)G_MAP[c] + (T)(c == 1) * mu[j];
+    *m = (T)1;
+  } else {
+    *g = (T)G_MAP[c];
+    *m = (T)NM_MAP[c];
+  }
+}
+
 /* ---- loglikelihood_loop_skipmissing(g::SnpLinAlg,...)  (src/loops.jl:517-539)
  * Terms are evaluated in T, accumulated in Float64 (loops.jl:523). */
 static double FN(ll_tile)(const uint8_t *bed, int64_t pitch, const T *q,
                           const T *p, int64_t K, int64_t i0, int64_t i1,
-                          int64_t j0, int64_t j1, T *qp) {
+                          int64_t j0, int64_t j1, const T *mu, T *qp) {
   FN(qp_block)(qp, q, p, K, i0, i1, j0, j1);
   const int64_t li = i1 - i0;
   double r = 0.0;
   for (int64_t j = j0; j < j1; ++j)
     for (int64_t i = i0; i < i1; ++i) {
       const int c = orc_code(bed, pitch, i, j);
-      const T g = (T)G_MAP[c], m = (T)NM_MAP[c];
+      T g, m;
+      FN(gw)(c, mu, j, &g, &m);
       const T s = qp[(i - i0) + li * (j - j0)];
       const T term = m * ((g * LOGT(s)) + (((T)2 - g) * LOGT((T)1 - s)));
       r += (double)term;
@@ -51,51 +66,52 @@ static double FN(ll_tile)(const uint8_t *bed, int64_t pitch, const T *q,
  * so this tree reproduces the reference's summation tree exactly. */
 static double FN(ll_tree)(const uint8_t *bed, int64_t pitch, const T *q,
                           const T *p, int64_t K, int64_t i0, int64_t i1,
-                          int64_t j0, int64_t j1, int depth) {
+                          int64_t j0, int64_t j1, const T *mu, int depth) {
   const int64_t ilen = i1 - i0, jlen = j1 - j0, maxL = TILE_MAXITER;
   if (ilen > maxL || jlen > maxL) {
     double r1 = 0, r2 = 0;
     if (ilen > jlen) {
       const int64_t m = orc_cleave(i0, i1, 4);
 #pragma omp task shared(r1) if (depth < 8)
-      r1 = FN(ll_tree)(bed, pitch, q, p, K, i0, m, j0, j1, depth + 1);
-      r2 = FN(ll_tree)(bed, pitch, q, p, K, m, i1, j0, j1, depth + 1);
+      r1 = FN(ll_tree)(bed, pitch, q, p, K, i0, m, j0, j1, mu, depth + 1);
+      r2 = FN(ll_tree)(bed, pitch, q, p, K, m, i1, j0, j1, mu, depth + 1);
 #pragma omp taskwait
     } else {
       const int64_t m = orc_cleave(j0, j1, 4);
 #pragma omp task shared(r1) if (depth < 8)
-      r1 = FN(ll_tree)(bed, pitch, q, p, K, i0, i1, j0, m, depth + 1);
-      r2 = FN(ll_tree)(bed, pitch, q, p, K, i0, i1, m, j1, depth + 1);
+      r1 = FN(ll_tree)(bed, pitch, q, p, K, i0, i1, j0, m, mu, depth + 1);
+      r2 = FN(ll_tree)(bed, pitch, q, p, K, i0, i1, m, j1, mu, depth + 1);
 #pragma omp taskwait
     }
     return r1 + r2;
   }
   T qp[TILE_MAXITER * TILE_MAXITER];
-  return FN(ll_tile)(bed, pitch, q, p, K, i0, i1, j0, j1, qp);
+  return FN(ll_tile)(bed, pitch, q, p, K, i0, i1, j0, j1, mu, qp);
 }
 
 /* loglikelihood(g::SnpLinAlg, q, p, qp_small, K, true)  (src/algorithms_inner.jl:27-35) */
 double FN(orc_loglikelihood)(const uint8_t *bed, int64_t pitch, const T *q,
                              const T *p, int64_t K, int64_t i0, int64_t i1,
-                             int64_t j0, int64_t j1, int nthreads) {
+                             int64_t j0, int64_t j1, const T *mu, int nthreads) {
   double r = 0;
   if (i1 <= i0 || j1 <= j0) return 0.0;
 #pragma omp parallel num_threads(nthreads > 0 ? nthreads : 1)
 #pragma omp single
-  r = FN(ll_tree)(bed, pitch, q, p, K, i0, i1, j0, j1, 0);
+  r = FN(ll_tree)(bed, pitch, q, p, K, i0, i1, j0, j1, mu, 0);
   return r;
 }
 
 /* ---- em_q_loop_skipmissing!(q_next, g::SnpLinAlg, ...)  (src/loops.jl:610-632) */
 static void FN(em_q_tile)(T *q_next, const uint8_t *bed, int64_t pitch,
                           const T *q, const T *p, int64_t K, int64_t i0,
-                          int64_t i1, int64_t j0, int64_t j1, T *qp) {
+                          int64_t i1, int64_t j0, int64_t j1, const T *mu, T *qp) {
   FN(qp_block)(qp, q, p, K, i0, i1, j0, j1);
   const int64_t li = i1 - i0;
   for (int64_t j = j0; j < j1; ++j)
     for (int64_t i = i0; i < i1; ++i) {
       const int c = orc_code(bed, pitch, i, j);
-      const T g = (T)G_MAP[c], m = (T)NM_MAP[c];
+      T g, m;
+      FN(gw)(c, mu, j, &g, &m);
       const T s = qp[(i - i0) + li * (j - j0)];
       for (int64_t k = 0; k < K; ++k) {
         const T qk = q[k + K * i], pk = p[k + K * j];
@@ -109,13 +125,14 @@ static void FN(em_q_tile)(T *q_next, const uint8_t *bed, int64_t pitch,
 static void FN(em_p_tile)(T *p_next, T *p_tmp, const uint8_t *bed,
                           int64_t pitch, const T *q, const T *p, int64_t K,
                           int64_t i0, int64_t i1, int64_t j0, int64_t j1,
-                          T *qp) {
+                          const T *mu, T *qp) {
   FN(qp_block)(qp, q, p, K, i0, i1, j0, j1);
   const int64_t li = i1 - i0;
   for (int64_t j = j0; j < j1; ++j)
     for (int64_t i = i0; i < i1; ++i) {
       const int c = orc_code(bed, pitch, i, j);
-      const T g = (T)G_MAP[c], m = (T)NM_MAP[c];
+      T g, m;
+      FN(gw)(c, mu, j, &g, &m);
       const T s = qp[(i - i0) + li * (j - j0)];
       for (int64_t k = 0; k < K; ++k) {
         const T qk = q[k + K * i], pk = p[k + K * j];
@@ -128,13 +145,14 @@ static void FN(em_p_tile)(T *p_next, T *p_tmp, const uint8_t *bed,
 /* ---- update_q_loop_skipmissing!(XtX, Xtz, g::SnpLinAlg, ...)  (src/loops.jl:867-894) */
 static void FN(update_q_tile)(T *XtX, T *Xtz, const uint8_t *bed, int64_t pitch,
                               const T *q, const T *p, int64_t K, int64_t i0,
-                              int64_t i1, int64_t j0, int64_t j1, T *qp) {
+                              int64_t i1, int64_t j0, int64_t j1, const T *mu, T *qp) {
   FN(qp_block)(qp, q, p, K, i0, i1, j0, j1);
   const int64_t li = i1 - i0;
   for (int64_t j = j0; j < j1; ++j)
     for (int64_t i = i0; i < i1; ++i) {
       const int c = orc_code(bed, pitch, i, j);
-      const T g = (T)G_MAP[c], m = (T)NM_MAP[c];
+      T g, m;
+      FN(gw)(c, mu, j, &g, &m);
       const T s = qp[(i - i0) + li * (j - j0)];
       for (int64_t k = 0; k < K; ++k) {
         const T pk = p[k + K * j];
@@ -152,13 +170,14 @@ static void FN(update_q_tile)(T *XtX, T *Xtz, const uint8_t *bed, int64_t pitch,
 /* ---- update_p_loop_skipmissing!(XtX, Xtz, g::SnpLinAlg, ...)  (src/loops.jl:923-950) */
 static void FN(update_p_tile)(T *XtX, T *Xtz, const uint8_t *bed, int64_t pitch,
                               const T *q, const T *p, int64_t K, int64_t i0,
-                              int64_t i1, int64_t j0, int64_t j1, T *qp) {
+                              int64_t i1, int64_t j0, int64_t j1, const T *mu, T *qp) {
   FN(qp_block)(qp, q, p, K, i0, i1, j0, j1);
   const int64_t li = i1 - i0;
   for (int64_t j = j0; j < j1; ++j)
     for (int64_t i = i0; i < i1; ++i) {
       const int c = orc_code(bed, pitch, i, j);
-      const T g = (T)G_MAP[c], m = (T)NM_MAP[c];
+      T g, m;
+      FN(gw)(c, mu, j, &g, &m);
       const T s = qp[(i - i0) + li * (j - j0)];
       for (int64_t k = 0; k < K; ++k) {
         const T qk = q[k + K * i];
@@ -179,53 +198,53 @@ static void FN(update_p_tile)(T *XtX, T *Xtz, const uint8_t *bed, int64_t pitch,
  * tile, so the result does not depend on the thread count. */
 void FN(orc_em_q_loop)(T *q_next, const uint8_t *bed, int64_t pitch, const T *q,
                        const T *p, int64_t K, int64_t i0, int64_t i1, int64_t j0,
-                       int64_t j1, int nthreads) {
+                       int64_t j1, const T *mu, int nthreads) {
   const int64_t L = TILE_MAXITER;
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
   for (int64_t ib = i0; ib < i1; ib += L) {
     T qp[TILE_MAXITER * TILE_MAXITER];
     const int64_t ie = ib + L < i1 ? ib + L : i1;
     for (int64_t jb = j0; jb < j1; jb += L)
-      FN(em_q_tile)(q_next, bed, pitch, q, p, K, ib, ie, jb, jb + L < j1 ? jb + L : j1, qp);
+      FN(em_q_tile)(q_next, bed, pitch, q, p, K, ib, ie, jb, jb + L < j1 ? jb + L : j1, mu, qp);
   }
 }
 
 void FN(orc_em_p_loop)(T *p_next, T *p_tmp, const uint8_t *bed, int64_t pitch,
                        const T *q, const T *p, int64_t K, int64_t i0, int64_t i1,
-                       int64_t j0, int64_t j1, int nthreads) {
+                       int64_t j0, int64_t j1, const T *mu, int nthreads) {
   const int64_t L = TILE_MAXITER;
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
   for (int64_t jb = j0; jb < j1; jb += L) {
     T qp[TILE_MAXITER * TILE_MAXITER];
     const int64_t je = jb + L < j1 ? jb + L : j1;
     for (int64_t ib = i0; ib < i1; ib += L)
-      FN(em_p_tile)(p_next, p_tmp, bed, pitch, q, p, K, ib, ib + L < i1 ? ib + L : i1, jb, je, qp);
+      FN(em_p_tile)(p_next, p_tmp, bed, pitch, q, p, K, ib, ib + L < i1 ? ib + L : i1, jb, je, mu, qp);
   }
 }
 
 void FN(orc_update_q_loop)(T *XtX, T *Xtz, const uint8_t *bed, int64_t pitch,
                            const T *q, const T *p, int64_t K, int64_t i0,
-                           int64_t i1, int64_t j0, int64_t j1, int nthreads) {
+                           int64_t i1, int64_t j0, int64_t j1, const T *mu, int nthreads) {
   const int64_t L = TILE_MAXITER;
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
   for (int64_t ib = i0; ib < i1; ib += L) {
     T qp[TILE_MAXITER * TILE_MAXITER];
     const int64_t ie = ib + L < i1 ? ib + L : i1;
     for (int64_t jb = j0; jb < j1; jb += L)
-      FN(update_q_tile)(XtX, Xtz, bed, pitch, q, p, K, ib, ie, jb, jb + L < j1 ? jb + L : j1, qp);
+      FN(update_q_tile)(XtX, Xtz, bed, pitch, q, p, K, ib, ie, jb, jb + L < j1 ? jb + L : j1, mu, qp);
   }
 }
 
 void FN(orc_update_p_loop)(T *XtX, T *Xtz, const uint8_t *bed, int64_t pitch,
                            const T *q, const T *p, int64_t K, int64_t i0,
-                           int64_t i1, int64_t j0, int64_t j1, int nthreads) {
+                           int64_t i1, int64_t j0, int64_t j1, const T *mu, int nthreads) {
   const int64_t L = TILE_MAXITER;
 #pragma omp parallel for schedule(dynamic) num_threads(nthreads > 0 ? nthreads : 1)
   for (int64_t jb = j0; jb < j1; jb += L) {
     T qp[TILE_MAXITER * TILE_MAXITER];
     const int64_t je = jb + L < j1 ? jb + L : j1;
     for (int64_t ib = i0; ib < i1; ib += L)
-      FN(update_p_tile)(XtX, Xtz, bed, pitch, q, p, K, ib, ib + L < i1 ? ib + L : i1, jb, je, qp);
+      FN(update_p_tile)(XtX, Xtz, bed, pitch, q, p, K, ib, ib + L < i1 ? ib + L : i1, jb, je, mu, qp);
   }
 }
 
@@ -490,6 +509,7 @@ typedef struct {
   T *U, *V; /* K(I+J) x Q */
   double ll_prev, ll_new;
   int nthreads;
+  const T *mu; /* NULL: skipmissing; else per-SNP means for the mean-imputation loops */
 } FN(orc_state);
 
 static T *FN(qpart)(FN(orc_state) * st, T *x) { (void)st; return x; }
@@ -497,7 +517,7 @@ static T *FN(ppart)(FN(orc_state) * st, T *x) { return x + st->K * st->I; }
 
 static double FN(st_ll)(FN(orc_state) * st, T *x) {
   return FN(orc_loglikelihood)(st->bed, st->pitch, FN(qpart)(st, x), FN(ppart)(st, x),
-                               st->K, 0, st->I, 0, st->J, st->nthreads);
+                               st->K, 0, st->I, 0, st->J, st->mu, st->nthreads);
 }
 
 /* em_q!(d, g)  (src/algorithms_inner.jl:68-92): q_next from (q, p), divided by 2J */
@@ -506,7 +526,7 @@ void FN(orc_em_q)(FN(orc_state) * st) {
   T *q_next = FN(qpart)(st, st->x_next);
   memset(q_next, 0, sizeof(T) * K * I);
   FN(orc_em_q_loop)(q_next, st->bed, st->pitch, FN(qpart)(st, st->x),
-                    FN(ppart)(st, st->x), K, 0, I, 0, J, st->nthreads);
+                    FN(ppart)(st, st->x), K, 0, I, 0, J, st->mu, st->nthreads);
   const T div = (T)(2 * J);
   for (int64_t a = 0; a < K * I; ++a) q_next[a] /= div;
 }
@@ -518,7 +538,7 @@ void FN(orc_em_p)(FN(orc_state) * st) {
   memset(p_next, 0, sizeof(T) * K * J);
   memset(p_tmp, 0, sizeof(T) * K * J);
   FN(orc_em_p_loop)(p_next, p_tmp, st->bed, st->pitch, FN(qpart)(st, st->x),
-                    FN(ppart)(st, st->x), K, 0, I, 0, J, st->nthreads);
+                    FN(ppart)(st, st->x), K, 0, I, 0, J, st->mu, st->nthreads);
   for (int64_t a = 0; a < K * J; ++a) p_next[a] = p_tmp[a] / (p_tmp[a] + p_next[a]);
 }
 
@@ -546,7 +566,7 @@ void FN(orc_update_q)(FN(orc_state) * st, int update2, int32_t *status) {
   st->ll_prev = st->ll_new;
   memset(st->XtX_q, 0, sizeof(T) * K * K * I);
   memset(st->Xtz_q, 0, sizeof(T) * K * I);
-  FN(orc_update_q_loop)(st->XtX_q, st->Xtz_q, st->bed, st->pitch, q, p, K, 0, I, 0, J, st->nthreads);
+  FN(orc_update_q_loop)(st->XtX_q, st->Xtz_q, st->bed, st->pitch, q, p, K, 0, I, 0, J, st->mu, st->nthreads);
   for (int64_t a = 0; a < K * I; ++a) st->Xtz_q[a] *= (T)-1;
 #pragma omp parallel for num_threads(st->nthreads > 0 ? st->nthreads : 1)
   for (int64_t i = 0; i < I; ++i) {
@@ -571,7 +591,7 @@ void FN(orc_update_p)(FN(orc_state) * st, int update2, int32_t *status) {
   st->ll_prev = st->ll_new;
   memset(st->XtX_p, 0, sizeof(T) * K * K * J);
   memset(st->Xtz_p, 0, sizeof(T) * K * J);
-  FN(orc_update_p_loop)(st->XtX_p, st->Xtz_p, st->bed, st->pitch, q, p, K, 0, I, 0, J, st->nthreads);
+  FN(orc_update_p_loop)(st->XtX_p, st->Xtz_p, st->bed, st->pitch, q, p, K, 0, I, 0, J, st->mu, st->nthreads);
   for (int64_t a = 0; a < K * J; ++a) st->Xtz_p[a] *= (T)-1;
 #pragma omp parallel for num_threads(st->nthreads > 0 ? st->nthreads : 1)
   for (int64_t j = 0; j < J; ++j) {

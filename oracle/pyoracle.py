@@ -53,12 +53,16 @@ def _declare(L):
                                        c_vp]
     L.orc_heldout_deviance.restype = c_dbl
     for s in ("_f64", "_f32"):
-        getattr(L, "orc_loglikelihood" + s).argtypes = [c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, C.c_int]
+        getattr(L, "orc_loglikelihood" + s).argtypes = [c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_vp,
+                                                        C.c_int]
         getattr(L, "orc_loglikelihood" + s).restype = c_dbl
-        getattr(L, "orc_em_q_loop" + s).argtypes = [c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, C.c_int]
-        getattr(L, "orc_em_p_loop" + s).argtypes = [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, C.c_int]
+        getattr(L, "orc_em_q_loop" + s).argtypes = [c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_vp,
+                                                    C.c_int]
+        getattr(L, "orc_em_p_loop" + s).argtypes = [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64,
+                                                    c_vp, C.c_int]
         for n in ("orc_update_q_loop", "orc_update_p_loop"):
-            getattr(L, n + s).argtypes = [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, C.c_int]
+            getattr(L, n + s).argtypes = [c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_i64, c_i64, c_i64, c_i64, c_vp,
+                                          C.c_int]
         getattr(L, "orc_sweep" + s).argtypes = [c_vp, c_i64, c_i64, c_vp, C.c_int]
         fl = c_dbl if s == "_f64" else C.c_float
         getattr(L, "orc_create_tableau" + s).argtypes = [c_vp, c_vp, c_vp, c_vp, c_i64, C.c_int, fl]
@@ -144,45 +148,62 @@ def heldout_deviance(bed, pitch, q, p, fold, nfolds, seed):
 
 
 # ---------------------------------------------------------------- tile loops
-def loglikelihood(bed, pitch, q, p, irange=None, jrange=None, nthreads=None):
+# mu: None for the reference's *_skipmissing loops; a length-J array of SNP
+# means selects the mean-imputation loops (missing g = mu[j]).
+def _mu(mu, dtype):
+    return None if mu is None else np.ascontiguousarray(mu, dtype=dtype)
+
+
+def snp_means(bed, pitch, I, J, dtype=np.float64):
+    """SnpLinAlg's mean genotype per SNP over the observed entries (additive model)."""
+    snm, salt, _ = counts(bed, pitch, I, J)
+    return (salt / np.maximum(snm, 1)).astype(dtype)
+
+
+def loglikelihood(bed, pitch, q, p, irange=None, jrange=None, nthreads=None, mu=None):
     K, I = q.shape
     J = p.shape[1]
     i0, i1 = irange if irange else (0, I)
     j0, j1 = jrange if jrange else (0, J)
+    mu_a = _mu(mu, q.dtype)
     f = getattr(lib(), "orc_loglikelihood" + _sfx(q.dtype))
-    return f(_p(bed), pitch, _p(_chk(q)), _p(_chk(p, q.dtype)), K, i0, i1, j0, j1, _nthreads(nthreads))
+    return f(_p(bed), pitch, _p(_chk(q)), _p(_chk(p, q.dtype)), K, i0, i1, j0, j1, _p(mu_a), _nthreads(nthreads))
 
 
-def em_q_loop(q_next, bed, pitch, q, p, irange=None, jrange=None, nthreads=None):
+def em_q_loop(q_next, bed, pitch, q, p, irange=None, jrange=None, nthreads=None, mu=None):
     K, I = q.shape
     J = p.shape[1]
     i0, i1 = irange if irange else (0, I)
     j0, j1 = jrange if jrange else (0, J)
-    getattr(lib(), "orc_em_q_loop" + _sfx(q.dtype))(_p(_chk(q_next, q.dtype)), _p(bed), pitch, _p(q), _p(p), K, i0, i1, j0, j1, _nthreads(nthreads))
+    mu_a = _mu(mu, q.dtype)
+    getattr(lib(), "orc_em_q_loop" + _sfx(q.dtype))(_p(_chk(q_next, q.dtype)), _p(bed), pitch, _p(q), _p(p), K, i0, i1, j0, j1, _p(mu_a), _nthreads(nthreads))
 
 
-def em_p_loop(p_next, p_tmp, bed, pitch, q, p, irange=None, jrange=None, nthreads=None):
+def em_p_loop(p_next, p_tmp, bed, pitch, q, p, irange=None, jrange=None, nthreads=None, mu=None):
     K, I = q.shape
     J = p.shape[1]
     i0, i1 = irange if irange else (0, I)
     j0, j1 = jrange if jrange else (0, J)
-    getattr(lib(), "orc_em_p_loop" + _sfx(q.dtype))(_p(_chk(p_next, q.dtype)), _p(_chk(p_tmp, q.dtype)), _p(bed), pitch, _p(q), _p(p), K, i0, i1, j0, j1, _nthreads(nthreads))
+    mu_a = _mu(mu, q.dtype)
+    getattr(lib(), "orc_em_p_loop" + _sfx(q.dtype))(_p(_chk(p_next, q.dtype)), _p(_chk(p_tmp, q.dtype)), _p(bed), pitch, _p(q), _p(p), K, i0, i1, j0, j1, _p(mu_a), _nthreads(nthreads))
 
 
-def update_q_loop(XtX, Xtz, bed, pitch, q, p, irange=None, jrange=None, nthreads=None):
+def update_q_loop(XtX, Xtz, bed, pitch, q, p, irange=None, jrange=None, nthreads=None, mu=None):
     K, I = q.shape
     J = p.shape[1]
     i0, i1 = irange if irange else (0, I)
     j0, j1 = jrange if jrange else (0, J)
-    getattr(lib(), "orc_update_q_loop" + _sfx(q.dtype))(_p(_chk(XtX, q.dtype)), _p(_chk(Xtz, q.dtype)), _p(bed), pitch, _p(q), _p(p), K, i0, i1, j0, j1, _nthreads(nthreads))
+    mu_a = _mu(mu, q.dtype)
+    getattr(lib(), "orc_update_q_loop" + _sfx(q.dtype))(_p(_chk(XtX, q.dtype)), _p(_chk(Xtz, q.dtype)), _p(bed), pitch, _p(q), _p(p), K, i0, i1, j0, j1, _p(mu_a), _nthreads(nthreads))
 
 
-def update_p_loop(XtX, Xtz, bed, pitch, q, p, irange=None, jrange=None, nthreads=None):
+def update_p_loop(XtX, Xtz, bed, pitch, q, p, irange=None, jrange=None, nthreads=None, mu=None):
     K, I = q.shape
     J = p.shape[1]
     i0, i1 = irange if irange else (0, I)
     j0, j1 = jrange if jrange else (0, J)
-    getattr(lib(), "orc_update_p_loop" + _sfx(q.dtype))(_p(_chk(XtX, q.dtype)), _p(_chk(Xtz, q.dtype)), _p(bed), pitch, _p(q), _p(p), K, i0, i1, j0, j1, _nthreads(nthreads))
+    mu_a = _mu(mu, q.dtype)
+    getattr(lib(), "orc_update_p_loop" + _sfx(q.dtype))(_p(_chk(XtX, q.dtype)), _p(_chk(Xtz, q.dtype)), _p(bed), pitch, _p(q), _p(p), K, i0, i1, j0, j1, _p(mu_a), _nthreads(nthreads))
 
 
 # ---------------------------------------------------------------- QP / projections / QN
@@ -251,13 +272,13 @@ class _StateF64(C.Structure):
                 ("x", c_vp), ("x_next", c_vp), ("x_next2", c_vp), ("x_tmp", c_vp),
                 ("XtX_q", c_vp), ("Xtz_q", c_vp), ("XtX_p", c_vp), ("Xtz_p", c_vp),
                 ("U", c_vp), ("V", c_vp),
-                ("ll_prev", c_dbl), ("ll_new", c_dbl), ("nthreads", C.c_int)]
+                ("ll_prev", c_dbl), ("ll_new", c_dbl), ("nthreads", C.c_int), ("mu", c_vp)]
 
 
 class OracleAdmix:
     """Host-array mirror of AdmixData{T} (src/structs.jl:9-215) driven by the oracle."""
 
-    def __init__(self, bed, pitch, I, J, K, Q, x0, U0=None, V0=None, dtype=np.float64, nthreads=None):
+    def __init__(self, bed, pitch, I, J, K, Q, x0, U0=None, V0=None, dtype=np.float64, nthreads=None, mu=None):
         self.dtype = np.dtype(dtype)
         self.I, self.J, self.K, self.Q = I, J, K, Q
         self.bed = np.ascontiguousarray(bed, dtype=np.uint8)
@@ -275,7 +296,10 @@ class OracleAdmix:
         self.V = np.zeros((N, Q), self.dtype, order="F") if V0 is None else np.asfortranarray(V0, dtype=self.dtype).copy(order="F")
         self.st = _StateF64(I, J, K, Q, _p(self.bed), pitch, _p(self.x), _p(self.x_next), _p(self.x_next2), _p(self.x_tmp),
                             _p(self.XtX_q), _p(self.Xtz_q), _p(self.XtX_p), _p(self.Xtz_p), _p(self.U), _p(self.V),
-                            float("nan"), float("nan"), _nthreads(nthreads))
+                            float("nan"), float("nan"), _nthreads(nthreads), None)
+        if mu is not None:
+            self.mu = np.ascontiguousarray(mu, dtype=self.dtype)
+            self.st.mu = _p(self.mu)
         self.sfx = _sfx(self.dtype)
 
     @property

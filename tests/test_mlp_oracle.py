@@ -94,6 +94,9 @@ def _scripted(oa, ll_warm, ll_em):
         def synchronize(self):
             pass
 
+        def set_skipmissing(self, skip=True):
+            pass
+
         def features(self, R, mu, isd, out=None):
             self.calls.append("features")
             return out

@@ -416,3 +416,59 @@ def test_lbqn_history_shift(O):
     U, V = U0.copy(order="F"), V0.copy(order="F")
     O.update_UV_LBQN(U, V, x, x1, x2, 2, Q)  # iter 2: only column 1 shifts into 2
     assert np.array_equal(U[:, 1], U0[:, 0]) and np.array_equal(U[:, 2], U0[:, 2])
+
+
+# ---------------------------------------------------------------- mean-imputation loops (loops.jl:493-921)
+def test_snp_means_from_decoded_genotypes(O):
+    I, J = 203, 77
+    _, _, bed, pitch = make_cohort(O, I, J, 3, miss=0.1, seed=31)
+    g = O.decode(bed, pitch, I, J).astype(np.float64)
+    obs = g >= 0
+    ref = np.where(obs, g, 0).sum(0) / obs.sum(0)
+    assert np.allclose(O.snp_means(bed, pitch, I, J), ref, rtol=1e-15, atol=0)
+
+
+def test_imputation_loops_equal_skipmissing_without_missing(O):
+    I, J, K = 90, 70, 3
+    _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.0, seed=32)
+    q, p = random_state(I, J, K, 2)
+    mu = O.snp_means(bed, pitch, I, J)
+    assert O.loglikelihood(bed, pitch, q, p, mu=mu) == O.loglikelihood(bed, pitch, q, p)
+    a = [np.zeros((K, K, I), order="F"), np.zeros((K, I), order="F")]
+    b = [np.zeros((K, K, I), order="F"), np.zeros((K, I), order="F")]
+    O.update_q_loop(*a, bed, pitch, q, p, mu=mu)
+    O.update_q_loop(*b, bed, pitch, q, p)
+    assert np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
+def test_imputation_ll_adds_the_missing_entries_at_their_mean(O):
+    I, J, K = 80, 60, 3
+    _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.15, seed=33)
+    q, p = random_state(I, J, K, 3)
+    mu = O.snp_means(bed, pitch, I, J)
+    g = O.decode(bed, pitch, I, J)
+    s = q.T @ p
+    miss = g < 0
+    extra = math.fsum((mu[None, :] * np.log(s) + (2 - mu[None, :]) * np.log(1 - s))[miss])
+    got = O.loglikelihood(bed, pitch, q, p, mu=mu, nthreads=1)
+    ref = O.loglikelihood(bed, pitch, q, p, nthreads=1) + extra
+    assert abs(got - ref) <= 1e-12 * abs(ref)
+
+
+def test_imputation_update_p_is_exact_gradient(O):
+    I, J, K = 60, 4, 3
+    _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.1, seed=34)
+    q, p = random_state(I, J, K, 5)
+    mu = O.snp_means(bed, pitch, I, J)
+    XtX = np.zeros((K, K, J), order="F")
+    Xtz = np.zeros((K, J), order="F")
+    O.update_p_loop(XtX, Xtz, bed, pitch, q, p, mu=mu)
+    h = 1e-5
+    for j in range(J):
+        for k in range(K):
+            pp, pm = p.copy(order="F"), p.copy(order="F")
+            pp[k, j] += h
+            pm[k, j] -= h
+            fd = (O.loglikelihood(bed, pitch, q, pp, mu=mu, nthreads=1)
+                  - O.loglikelihood(bed, pitch, q, pm, mu=mu, nthreads=1)) / (2 * h)
+            assert abs(Xtz[k, j] - fd) <= 1e-6 * (1 + abs(fd))
