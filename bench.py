@@ -335,6 +335,8 @@ def main():
     ap.add_argument("--target-waves", type=int, default=0, help="waves per genotype pass (0: library default)")
     ap.add_argument("--no-ttc", action="store_true", help="skip the time-to-converge runs")
     ap.add_argument("--graphs", action="store_true", help="replay steady-state iterations from hipGraphs")
+    ap.add_argument("--stream-snps", type=int, default=0,
+                    help="keep the BED in pinned host memory and stream it in chunks of this many SNPs")
     ap.add_argument("--ttc-forms", default="bf16,warm_f64,warm_bf16",
                     help="forms timed to convergence beside plain FP64 (bf16, f32, warm_f64, warm_bf16)")
     ap.add_argument("--ttc-max-iter", type=int, default=200)
@@ -368,7 +370,7 @@ def main():
     I, J, K = wl["I"], wl["J"], wl["K"]
     T = np.float64 if args.dtype == "f64" else np.float32
 
-    ctx = oa.HIPAdmixData(I, J, K, 3, T)
+    ctx = oa.HIPAdmixData(I, J, K, 3, T, stream_snps=args.stream_snps)
     if args.target_waves:
         ctx.set_tuning(oa._native.ADMX_TUNE_TARGET_WAVES, args.target_waves)
     if args.graphs:

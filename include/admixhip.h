@@ -77,6 +77,17 @@ int admx_device_count(int *n);
 /* ---- context: replaces CuAdmixData{T} + _cu_admixture_base
  * (src/cuda/structs.jl:1-38, stub src/driver.jl:204-206). */
 int admx_create(admx_ctx **out, int64_t I, int64_t J, int64_t K, int64_t Q, int dtype);
+/* J-chunk streaming: a context whose BED stays in pinned host memory (for
+ * cohorts larger than HBM).  Every genotype pass streams it in chunks of
+ * snps_per_chunk SNPs through two device buffers, the copy of the next chunk
+ * overlapping the kernels of the current one.  Same results as a resident
+ * context.  FP64/FP32 passes with the skipmissing loops; decode, counts, fold
+ * masking, the warm-start features, the mixed-precision passes and the
+ * mean-imputation loops need a resident BED.  No reference counterpart (its
+ * joffset/width hooks, src/cuda/kernels.jl:3,20 and src/cuda/structs.jl:13, are
+ * unfinished). */
+int admx_create_streamed(admx_ctx **out, int64_t I, int64_t J, int64_t K, int64_t Q, int dtype,
+                         int64_t snps_per_chunk);
 int admx_destroy(admx_ctx *ctx);
 /* Upload the raw SNP-major BED payload: J columns of bytes_per_snp bytes
  * (bytes_per_snp >= ceil(I/4); the 3-byte PLINK magic already stripped),

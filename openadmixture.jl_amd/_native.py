@@ -30,6 +30,7 @@ SIGNATURES = {
     "admx_last_error": ([], C.c_char_p),
     "admx_device_count": ([_vp], _int),
     "admx_create": ([_vp, _i64, _i64, _i64, _i64, _int], _int),
+    "admx_create_streamed": ([_vp, _i64, _i64, _i64, _i64, _int, _i64], _int),
     "admx_destroy": ([_vp], _int),
     "admx_upload_bed": ([_vp, _vp, _i64], _int),
     "admx_download_bed": ([_vp, _vp], _int),

@@ -96,6 +96,13 @@ struct admx_ctx {
   int precision = ADMX_PREC_F64;
   DevBuf fq, fp, ppt, mmt, qqt;  // mixed-precision pass inputs
   DevBuf feat, featpart;          // warm-start features (I x 64) and their chunk partials
+  // J-chunk streaming: the BED lives in pinned host memory and every pass
+  // streams it through two device chunk buffers on a copy stream.
+  int64_t stream_snps = 0;        // SNPs per chunk (0: the BED is resident in HBM)
+  uint8_t* hbed = nullptr;        // pinned host BED, pitch * (J + 16)
+  uint8_t* dchunk[2] = {nullptr, nullptr};
+  hipStream_t cs = nullptr;       // copy stream (default priority)
+  hipEvent_t ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
   int impute = 0;                 // 1: mean-imputation loops (skipmissing = false)
   bool mu_valid = false;          // mu holds the per-SNP means of the current genotypes
   DevBuf snp_mu, cnt;             // per-SNP means (T, J); count scratch
@@ -251,11 +258,52 @@ const char* mode_name(bool qside, int mode) {
   return mode == MODE_UPDATE ? "pside_update" : "pside_em";
 }
 
+// Genotype source of a pass over SNPs [j0, j1): fn(base, cj0, cj1) launches on
+// c->s with bed rows reachable as base + pitch * j for j in [cj0, cj1 + 16).
+// Resident context: one call with the HBM BED.  Streamed context: chunks of
+// stream_snps SNPs alternate between two device buffers; the copy of chunk
+// k+1 (copy stream) overlaps the kernels of chunk k, and a buffer is refilled
+// only after the kernels that read it have finished (events).  The base pointer
+// of a chunk is the buffer address minus pitch * cj0.
+template <typename F>
+int for_chunks(admx_ctx* c, int64_t j0, int64_t j1, F&& fn) {
+  if (!c->stream_snps) return fn(static_cast<const uint8_t*>(c->bed), j0, j1);
+  const int64_t S = c->stream_snps;
+  int k = 0;
+  for (int64_t cj0 = j0; cj0 < j1; cj0 += S, ++k) {
+    const int64_t cj1 = std::min(j1, cj0 + S);
+    const int b = k & 1;
+    const int64_t r1 = std::min(cj1 + 16, c->J + 16);
+    HIPCHK(hipStreamWaitEvent(c->cs, c->ev_free[b], 0));
+    HIPCHK(hipMemcpyAsync(c->dchunk[b], c->hbed + c->pitch * cj0, (size_t)(c->pitch * (r1 - cj0)),
+                          hipMemcpyHostToDevice, c->cs));
+    HIPCHK(hipEventRecord(c->ev_ready[b], c->cs));
+    HIPCHK(hipStreamWaitEvent(c->s, c->ev_ready[b], 0));
+    CHK(fn(static_cast<const uint8_t*>(c->dchunk[b]) - c->pitch * cj0, cj0, cj1));
+    HIPCHK(hipEventRecord(c->ev_free[b], c->s));
+  }
+  return ADMX_OK;
+}
+
+// SNP chunks of a pass over [j0, j1) (one for a resident context).
+std::vector<std::pair<int64_t, int64_t>> snp_chunks(const admx_ctx* c, int64_t j0, int64_t j1) {
+  std::vector<std::pair<int64_t, int64_t>> r;
+  const int64_t S = c->stream_snps ? c->stream_snps : std::max<int64_t>(j1 - j0, 1);
+  for (int64_t a = j0; a < j1; a += S) r.push_back({a, std::min(j1, a + S)});
+  return r;
+}
+
+int need_resident(admx_ctx* c, const char* what) {
+  if (c->stream_snps) return fail(ADMX_ESTATE, std::string(what) + " needs a resident BED (not a streamed context)");
+  return ADMX_OK;
+}
+
 // Per-SNP means of the current genotypes for the mean-imputation loops
 // (computed once per genotype upload).
 template <typename T>
 int ensure_mu(admx_ctx* c) {
   if (!c->impute || c->mu_valid) return ADMX_OK;
+  CHK(need_resident(c, "the mean-imputation loops"));
   CHK(c->snp_mu.grow(sizeof(T) * (size_t)c->J));
   CHK(c->cnt.grow(sizeof(unsigned long long) * (size_t)(2 * c->J + c->I)));
   auto* nm = static_cast<unsigned long long*>(c->cnt.p);
@@ -277,26 +325,43 @@ int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
   *nchunk = 0;
   *nparts = 0;
   if (Ilen <= 0 || Jlen <= 0) return ADMX_OK;
-  int64_t chunk = 0;
-  const int n = qside_chunks(c, Ilen, Jlen, &chunk);
   CHK(ensure_mu<T>(c));
-  PassArgs a{c->bed, c->pitch, c->I, q, p, i0, i1, j0, j1, chunk, nullptr, c->impute ? c->snp_mu.p : nullptr};
-  if (mode == MODE_LL) {
-    const int64_t np = ((Ilen + 255) / 256) * n;
-    CHK(c->llpart.grow((size_t)np * sizeof(double)));
-    a.part = c->llpart.p;
-    *nparts = np;
-  } else {
-    const int na = pass_na(mode, true, (int)c->K, sizeof(T) == 8);
-    CHK(c->part.grow((size_t)n * na * Ilen * sizeof(T)));
-    a.part = c->part.p;
+  // one launch per SNP chunk (one for a resident BED); their partial slots sit side by side
+  const auto chunks = snp_chunks(c, j0, j1);
+  std::vector<int> ns;
+  std::vector<int64_t> chs;
+  int total = 0;
+  for (const auto& r : chunks) {
+    int64_t ch = 0;
+    ns.push_back(qside_chunks(c, Ilen, r.second - r.first, &ch));
+    chs.push_back(ch);
+    total += ns.back();
   }
+  const int64_t nbx = (Ilen + 255) / 256;
+  const int na = pass_na(mode, true, (int)c->K, sizeof(T) == 8);
+  if (mode == MODE_LL) CHK(c->llpart.grow((size_t)(nbx * total) * sizeof(double)));
+  else CHK(c->part.grow((size_t)total * na * Ilen * sizeof(T)));
   {
     ProfScope ps(c, mode_name(true, mode));
-    launch_qside<T>(mode, (int)c->K, a, n, c->s);
+    size_t k = 0;
+    int64_t off = 0;
+    CHK(for_chunks(c, j0, j1, [&](const uint8_t* base, int64_t cj0, int64_t cj1) {
+      PassArgs a{base, c->pitch, c->I, q, p, i0, i1, cj0, cj1, chs[k], nullptr, c->impute ? c->snp_mu.p : nullptr};
+      if (mode == MODE_LL) {
+        a.part = static_cast<double*>(c->llpart.p) + off;
+        off += nbx * ns[k];
+      } else {
+        a.part = static_cast<T*>(c->part.p) + off;
+        off += (int64_t)ns[k] * na * Ilen;
+      }
+      launch_qside<T>(mode, (int)c->K, a, ns[k], c->s);
+      ++k;
+      return check_launch();
+    }));
   }
-  *nchunk = n;
-  return check_launch();
+  *nchunk = total;
+  if (mode == MODE_LL) *nparts = nbx * total;
+  return ADMX_OK;
 }
 
 template <typename T>
@@ -310,13 +375,17 @@ int pass_p(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
   const int na = pass_na(mode, false, (int)c->K, sizeof(T) == 8);
   CHK(c->part.grow((size_t)n * na * Jlen * sizeof(T)));
   CHK(ensure_mu<T>(c));
-  PassArgs a{c->bed, c->pitch, c->I, q, p, i0, i1, j0, j1, chunk, c->part.p, c->impute ? c->snp_mu.p : nullptr};
   {
     ProfScope ps(c, mode_name(false, mode));
-    launch_pside<T>(mode, (int)c->K, a, n, c->s);
+    CHK(for_chunks(c, j0, j1, [&](const uint8_t* base, int64_t cj0, int64_t cj1) {
+      PassArgs a{base, c->pitch, c->I, q, p, i0, i1, cj0, cj1, chunk, c->part.p, c->impute ? c->snp_mu.p : nullptr,
+                 Jlen, cj0 - j0};
+      launch_pside<T>(mode, (int)c->K, a, n, c->s);
+      return check_launch();
+    }));
   }
   *nchunk = n;
-  return check_launch();
+  return ADMX_OK;
 }
 
 // LL of (q, p) over a range into device slot lls[slot] (accumulate optional).
@@ -335,6 +404,7 @@ int ll_to_slot(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1
 int mixed_update(admx_ctx* c, bool qside, const double* q, const double* p, const double* x_in, double* x_out,
                  int32_t* status_dev) {
   if (c->impute) return bad("the mean-imputation loops run with the FP64/FP32 passes (ADMX_PREC_F64)");
+  CHK(need_resident(c, "the mixed-precision passes"));
   const int64_t I = c->I, J = c->J, K = c->K;
   if (c->fq.n < sizeof(float) * K * (I + 16)) {
     CHK(c->fq.grow(sizeof(float) * K * (I + 16)));
@@ -599,7 +669,7 @@ int dev_qn_iteration(admx_ctx* c, int64_t iter, int fix_q, int fix_p, double* ou
     drop_graphs(c);
     c->graph_sig = layout_sig(c);
   }
-  if (!c->use_graphs || c->prof.on || iter <= c->Q || !(c->graph_warm & (1u << kind))) {
+  if (!c->use_graphs || c->stream_snps || c->prof.on || iter <= c->Q || !(c->graph_warm & (1u << kind))) {
     CHK(enqueue_qn_iteration<T>(c, iter, fix_q, fix_p));
     // scratch buffers may have grown in this eager run: graphs need a run
     // under an unchanged signature before capture (no allocation is captured)
@@ -963,7 +1033,10 @@ int admx_device_count(int* n) {
   return ADMX_OK;
 }
 
-int admx_create(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int dtype) {
+}  // extern "C"
+
+namespace {
+int create_impl(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int dtype, int64_t stream_snps) {
   if (!out) return bad("null output pointer");
   *out = nullptr;
   if (I < 1 || J < 1) return bad("I and J must be >= 1");
@@ -979,6 +1052,7 @@ int admx_create(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int 
   c->esz = dtype == ADMX_F64 ? 8 : 4;
   c->ibytes = (I + 3) / 4;
   c->pitch = (c->ibytes + 63) & ~(int64_t)63;
+  c->stream_snps = stream_snps;
   auto cleanup = [&](int rc) {
     admx_destroy(c);
     return rc;
@@ -989,7 +1063,25 @@ int admx_create(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int 
     return cleanup(fail(ADMX_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
   const size_t xb = (size_t)c->N() * c->esz;
   auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, bytes < 256 ? 256 : bytes); };
-  if (alloc((void**)&c->bed, (size_t)c->pitch * (J + 16)) != hipSuccess || alloc(&c->x[0], xb) != hipSuccess ||
+  if (stream_snps) {
+    // BED in pinned host memory (zero, padding rows = missing code), two chunk buffers
+    const size_t hb = (size_t)c->pitch * (J + 16), cb = (size_t)c->pitch * (stream_snps + 16);
+    if (hipHostMalloc((void**)&c->hbed, hb, hipHostMallocDefault) != hipSuccess)
+      return cleanup(fail(ADMX_EHIP, "hipHostMalloc failed for the streamed BED"));
+    std::memset(c->hbed, 0, (size_t)c->pitch * J);
+    std::memset(c->hbed + (size_t)c->pitch * J, 0x55, (size_t)c->pitch * 16);
+    if (alloc((void**)&c->dchunk[0], cb) != hipSuccess || alloc((void**)&c->dchunk[1], cb) != hipSuccess)
+      return cleanup(fail(ADMX_EHIP, "hipMalloc failed for the BED chunk buffers"));
+    if (hipStreamCreateWithFlags(&c->cs, hipStreamNonBlocking) != hipSuccess)
+      return cleanup(fail(ADMX_EHIP, "hipStreamCreate failed (copy stream)"));
+    for (int b = 0; b < 2; ++b)
+      if (hipEventCreateWithFlags(&c->ev_ready[b], hipEventDisableTiming) != hipSuccess ||
+          hipEventCreateWithFlags(&c->ev_free[b], hipEventDisableTiming) != hipSuccess)
+        return cleanup(fail(ADMX_EHIP, "hipEventCreate failed"));
+  } else if (alloc((void**)&c->bed, (size_t)c->pitch * (J + 16)) != hipSuccess) {
+    return cleanup(fail(ADMX_EHIP, "hipMalloc failed for the BED"));
+  }
+  if (alloc(&c->x[0], xb) != hipSuccess ||
       alloc(&c->x[1], xb) != hipSuccess || alloc(&c->x[2], xb) != hipSuccess || alloc(&c->x[3], xb) != hipSuccess ||
       alloc(&c->XtX_q, (size_t)(K * K * I) * c->esz) != hipSuccess ||
       alloc(&c->Xtz_q, (size_t)(K * I) * c->esz) != hipSuccess ||
@@ -1006,6 +1098,19 @@ int admx_create(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int 
     return cleanup(fail(ADMX_EHIP, "hipMemcpy failed"));
   *out = c;
   return ADMX_OK;
+}
+}  // namespace
+
+extern "C" {
+
+int admx_create(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int dtype) {
+  return create_impl(out, I, J, K, Q, dtype, 0);
+}
+
+int admx_create_streamed(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int dtype,
+                         int64_t snps_per_chunk) {
+  if (snps_per_chunk < 16) return bad("a streamed context needs at least 16 SNPs per chunk");
+  return create_impl(out, I, J, K, Q, dtype, std::min<int64_t>(snps_per_chunk, J));
 }
 
 int admx_destroy(admx_ctx* c) {
@@ -1024,6 +1129,14 @@ int admx_destroy(admx_ctx* c) {
   for (void* b : bufs)
     if (b) (void)hipFree(b);
   if (c->h_lls) (void)hipHostFree(c->h_lls);
+  if (c->cs) (void)hipStreamSynchronize(c->cs);
+  if (c->hbed) (void)hipHostFree(c->hbed);
+  for (int b = 0; b < 2; ++b) {
+    if (c->dchunk[b]) (void)hipFree(c->dchunk[b]);
+    if (c->ev_ready[b]) (void)hipEventDestroy(c->ev_ready[b]);
+    if (c->ev_free[b]) (void)hipEventDestroy(c->ev_free[b]);
+  }
+  if (c->cs) (void)hipStreamDestroy(c->cs);
   c->part.release();
   c->llpart.release();
   c->zal.release();
@@ -1047,6 +1160,16 @@ int admx_upload_bed(admx_ctx* c, const uint8_t* bed, int64_t bytes_per_snp) {
   Guard g(c);
   if (!bed) return bad("null BED pointer");
   if (bytes_per_snp < c->ibytes) return bad("bytes_per_snp < ceil(I/4)");
+  if (c->stream_snps) {  // host copy into the pinned BED; the padding rows stay missing
+    for (int64_t j = 0; j < c->J; ++j) {
+      uint8_t* row = c->hbed + (size_t)c->pitch * j;
+      std::memcpy(row, bed + (size_t)bytes_per_snp * j, (size_t)c->ibytes);
+      std::memset(row + c->ibytes, 0, (size_t)(c->pitch - c->ibytes));
+    }
+    c->have_bed = true;
+    c->mu_valid = false;
+    return ADMX_OK;
+  }
   HIPCHK(hipMemsetAsync(c->bed, 0, (size_t)c->pitch * c->J, c->s));
   CHK(pad_bed(c));
   HIPCHK(hipMemcpy2DAsync(c->bed, c->pitch, bed, bytes_per_snp, c->ibytes, c->J, hipMemcpyHostToDevice, c->s));
@@ -1061,6 +1184,11 @@ int admx_download_bed(admx_ctx* c, uint8_t* bed) {
   Guard g(c);
   CHK(need_bed(c));
   if (!bed) return bad("null BED pointer");
+  if (c->stream_snps) {
+    for (int64_t j = 0; j < c->J; ++j)
+      std::memcpy(bed + (size_t)c->ibytes * j, c->hbed + (size_t)c->pitch * j, (size_t)c->ibytes);
+    return ADMX_OK;
+  }
   HIPCHK(hipMemcpy2DAsync(bed, c->ibytes, c->bed, c->pitch, c->ibytes, c->J, hipMemcpyDeviceToHost, c->s));
   HIPCHK(hipStreamSynchronize(c->s));
   return ADMX_OK;
@@ -1073,6 +1201,21 @@ int admx_generate_bed(admx_ctx* c, const double* Qt, const double* Pt, uint64_t 
   const size_t qb = (size_t)(c->K * c->I) * sizeof(double), pb = (size_t)(c->K * c->J) * sizeof(double);
   CHK(stage_in(c, 0, Qt, qb, qb));
   CHK(stage_in(c, 1, Pt, pb, pb));
+  if (c->stream_snps) {  // chunk by chunk through a device buffer into the pinned BED
+    const int64_t S = c->stream_snps;
+    for (int64_t j0 = 0; j0 < c->J; j0 += S) {
+      const int64_t j1 = std::min(c->J, j0 + S);
+      launch_gen_bed_rows(static_cast<const double*>(c->st[0].p), static_cast<const double*>(c->st[1].p), c->I, j0,
+                          j1, (int)c->K, seed, miss_rate, c->dchunk[0] - c->pitch * j0, c->pitch, c->s);
+      CHK(check_launch());
+      HIPCHK(hipMemcpyAsync(c->hbed + (size_t)c->pitch * j0, c->dchunk[0], (size_t)(c->pitch * (j1 - j0)),
+                            hipMemcpyDeviceToHost, c->s));
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->have_bed = true;
+    c->mu_valid = false;
+    return ADMX_OK;
+  }
   {
     ProfScope ps(c, "gen_bed");
     launch_gen_bed(static_cast<const double*>(c->st[0].p), static_cast<const double*>(c->st[1].p), c->I, c->J,
@@ -1092,6 +1235,8 @@ int admx_bed_from(admx_ctx* dst, admx_ctx* src, int32_t fold, int32_t nfolds, ui
   if (dst == src) return bad("source and destination must be different contexts");
   Guard2 g(dst, src);
   CHK(need_bed(src));
+  CHK(need_resident(src, "fold masking"));
+  CHK(need_resident(dst, "fold masking"));
   if (dst->I != src->I || dst->J != src->J) return bad("contexts differ in I or J");
   if (fold >= 0 && (nfolds < 2 || fold >= nfolds)) return bad("need 0 <= fold < nfolds and nfolds >= 2");
   if (dst->device != src->device) return bad("contexts must live on the same device");
@@ -1115,6 +1260,7 @@ int admx_heldout_deviance(admx_ctx* fit, admx_ctx* src, int32_t fold, int32_t nf
   CHK(check_ctx(src));
   Guard2 g(fit, src);
   CHK(need_bed(src));
+  CHK(need_resident(src, "the held-out deviance"));
   if (!deviance_sum || !count) return bad("null output");
   if (fit->I != src->I || fit->J != src->J) return bad("contexts differ in I or J");
   if (fit->dtype != ADMX_F64) return bad("held-out deviance needs a FP64 fit context");
@@ -1148,6 +1294,7 @@ int admx_decode(admx_ctx* c, int8_t* gout) {
   CHK(check_ctx(c));
   Guard g(c);
   CHK(need_bed(c));
+  CHK(need_resident(c, "decode"));
   if (!gout) return bad("null output");
   const size_t bytes = (size_t)(c->I * c->J);
   CHK(c->st[0].grow(bytes));
@@ -1165,6 +1312,7 @@ int admx_counts(admx_ctx* c, int64_t* snp_nm, int64_t* snp_alt, int64_t* ind_nm)
   CHK(check_ctx(c));
   Guard g(c);
   CHK(need_bed(c));
+  CHK(need_resident(c, "counts"));
   if (!snp_nm || !snp_alt || !ind_nm) return bad("null output");
   const size_t jb = sizeof(unsigned long long) * c->J, ib = sizeof(unsigned long long) * c->I;
   CHK(c->st[0].grow(jb));
@@ -1660,6 +1808,7 @@ int admx_features(admx_ctx* c, const float* R, const float* mu, const float* isd
   CHK(check_ctx(c));
   Guard g(c);
   CHK(need_bed(c));
+  CHK(need_resident(c, "the warm-start features"));
   if (!R || !mu || !isd) return bad("null panel array");
   if (Jp < 1 || Jp > c->J) return bad("panel SNP count must be in [1, J]");
   const size_t rb = sizeof(float) * (size_t)(Jp * kFeat), vb = sizeof(float) * (size_t)Jp;

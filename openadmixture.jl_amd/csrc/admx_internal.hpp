@@ -23,6 +23,8 @@ struct PassArgs {
   int64_t chunk;   // SNPs per chunk (Q side) / individuals per chunk, multiple of 16 (P side)
   void* part;      // partial sums [nchunk][NA][n] (UPDATE/EM) or per-block doubles (LL)
   const void* mu = nullptr;  // per-SNP means (T, J): mean-imputation loops; nullptr: skipmissing
+  int64_t ldp = 0;           // P side: column stride of the partials (0: j1 - j0)
+  int64_t col0 = 0;          // P side: column of SNP j0 in the partials
 };
 
 // Per-column reduction of pass partials, optional QP + projection.
@@ -113,6 +115,9 @@ void launch_heldout_dev(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J,
 // Synthetic genotypes from (Q,P) (double), seeded counter hash.
 void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, int K, uint64_t seed,
                     double miss_rate, uint8_t* bed, int64_t pitch, hipStream_t s);
+// Rows [j0, j1) only, into bed + pitch * j (a streamed context's chunk buffer).
+void launch_gen_bed_rows(const double* Qt, const double* Pt, int64_t I, int64_t j0, int64_t j1, int K,
+                         uint64_t seed, double miss_rate, uint8_t* bed, int64_t pitch, hipStream_t s);
 
 // Slots of the device log-likelihood vector.
 enum LLSlot : int { LL_PREV = 0, LL_NEW = 1, LL_BASIC = 2, LL_QN = 3, LL_TMP = 4, LL_NSLOTS = 8 };

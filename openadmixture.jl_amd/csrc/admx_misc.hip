@@ -474,11 +474,11 @@ __device__ __forceinline__ double u01(uint64_t h) { return (double)(h >> 11) * 0
 // g_ij ~ Binomial(2, q_i . p_j) from two hashed uniforms, missing with
 // probability miss_rate from a third; one lane per BED byte (4 individuals).
 __global__ __launch_bounds__(256) void k_gen_bed(const double* __restrict__ Qt, const double* __restrict__ Pt,
-                                                 int64_t I, int64_t J, int K, uint64_t base, double miss_rate,
-                                                 uint8_t* __restrict__ bed, int64_t pitch) {
+                                                 int64_t I, int64_t j0, int64_t J, int K, uint64_t base,
+                                                 double miss_rate, uint8_t* __restrict__ bed, int64_t pitch) {
   const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (b >= pitch) return;
-  for (int64_t j = blockIdx.y; j < J; j += gridDim.y) {
+  for (int64_t j = j0 + blockIdx.y; j < J; j += gridDim.y) {
     uint32_t byte = 0;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
@@ -500,16 +500,23 @@ __global__ __launch_bounds__(256) void k_gen_bed(const double* __restrict__ Qt, 
   }
 }
 
-void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, int K, uint64_t seed, double miss_rate,
-                    uint8_t* bed, int64_t pitch, hipStream_t s) {
+void launch_gen_bed_rows(const double* Qt, const double* Pt, int64_t I, int64_t j0, int64_t j1, int K,
+                         uint64_t seed, double miss_rate, uint8_t* bed, int64_t pitch, hipStream_t s) {
   uint64_t base = seed ^ 0x5851F42D4C957F2DULL;
   // smix on the host: identical arithmetic to the device/CPU definition.
   base += 0x9E3779B97F4A7C15ULL;
   base = (base ^ (base >> 30)) * 0xBF58476D1CE4E5B9ULL;
   base = (base ^ (base >> 27)) * 0x94D049BB133111EBULL;
   base = base ^ (base >> 31);
-  dim3 grid((unsigned)((pitch + 255) / 256), (unsigned)(J < 65535 ? J : 65535));
-  hipLaunchKernelGGL(k_gen_bed, grid, dim3(256), 0, s, Qt, Pt, I, J, K, base, miss_rate, bed, pitch);
+  const int64_t rows = j1 - j0;
+  if (rows <= 0) return;
+  dim3 grid((unsigned)((pitch + 255) / 256), (unsigned)(rows < 65535 ? rows : 65535));
+  hipLaunchKernelGGL(k_gen_bed, grid, dim3(256), 0, s, Qt, Pt, I, j0, j1, K, base, miss_rate, bed, pitch);
+}
+
+void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, int K, uint64_t seed, double miss_rate,
+                    uint8_t* bed, int64_t pitch, hipStream_t s) {
+  launch_gen_bed_rows(Qt, Pt, I, 0, J, K, seed, miss_rate, bed, pitch, s);
 }
 
 // ------------------------------------------------------------------ cross-validation folds

@@ -359,17 +359,18 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
   if (!valid) return;
   T* __restrict__ part = static_cast<T*>(a.part);
   const int NA = (MODE == MODE_EM) ? 2 * K : NT + K;
-  T* out = part + (int64_t)blockIdx.y * NA * Jlen + jl;
+  const int64_t ldp = a.ldp ? a.ldp : Jlen;
+  T* out = part + (int64_t)blockIdx.y * NA * ldp + a.col0 + jl;
   int o = 0;
   if constexpr (MODE == MODE_UPDATE) {
 #pragma unroll
-    for (int n = 0; n < NT; ++n) out[(o++) * Jlen] = H[n];
+    for (int n = 0; n < NT; ++n) out[(o++) * ldp] = H[n];
   }
 #pragma unroll
-  for (int k = 0; k < K; ++k) out[(o++) * Jlen] = z[k];
+  for (int k = 0; k < K; ++k) out[(o++) * ldp] = z[k];
   if constexpr (MODE == MODE_EM) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) out[(o++) * Jlen] = e2[k];
+    for (int k = 0; k < K; ++k) out[(o++) * ldp] = e2[k];
   }
 }
 

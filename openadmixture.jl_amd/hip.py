@@ -28,11 +28,19 @@ def _f(a, dtype, shape=None):
 class HIPAdmixData:
     """Device mirror of AdmixData{T} plus the device-resident BED."""
 
-    def __init__(self, I: int, J: int, K: int, Q: int = 3, T=np.float64):
+    def __init__(self, I: int, J: int, K: int, Q: int = 3, T=np.float64, stream_snps: int = 0):
+        """stream_snps > 0: the genotypes stay in pinned host memory and every
+        pass streams them through HBM in chunks of that many SNPs (cohorts
+        larger than HBM)."""
         self.I, self.J, self.K, self.Q = int(I), int(J), int(K), int(Q)
         self.T = np.dtype(T)
+        self.stream_snps = int(stream_snps)
         self._h = C.c_void_p()
-        N.check(N.lib().admx_create(C.byref(self._h), self.I, self.J, self.K, self.Q, N.dtype_code(self.T)))
+        if self.stream_snps:
+            N.check(N.lib().admx_create_streamed(C.byref(self._h), self.I, self.J, self.K, self.Q,
+                                                 N.dtype_code(self.T), self.stream_snps))
+        else:
+            N.check(N.lib().admx_create(C.byref(self._h), self.I, self.J, self.K, self.Q, N.dtype_code(self.T)))
 
     # ------------------------------------------------------------ lifetime
     def close(self):
