@@ -61,10 +61,10 @@ WORKLOADS = {
 
 
 KERNEL_SYMBOLS = {  # bench kernel name -> rocprof symbol prefix of its K=8 instance
-    "qside_update": "void admx::k_qside<double, 8, 0, true>",
-    "pside_update": "void admx::k_pside<double, 8, 0>",
-    "qside_ll": "void admx::k_qside<double, 8, 2, false>",
-    "qside_update_bf16": "void admx::k_qside_bf16<8>",
+    "qside_update": "void admx::k_qside<double, 8, 0, true, false>",
+    "pside_update": "void admx::k_pside<double, 8, 0, false>",
+    "qside_ll": "void admx::k_qside<double, 8, 2, false, false>",
+    "qside_update_bf16": "_ZN4admx12k_qside_bf16ILi8E",
     "pside_update_bf16": "void admx::k_pside_bf16<8>",
 }
 
