@@ -269,13 +269,23 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
 #pragma unroll
   for (int k = 0; k < (MODE == MODE_EM ? K : 1); ++k) e2[k] = 0;
 
-  // Per-genotype work; qi is wave-uniform (scalar loads).
+  // Per-genotype work on the wave-uniform q column qn of the current individual
+  // (scalar registers), while the next individual's column is being loaded:
+  // body() takes qn, issues the scalar loads of individual `next` into qn and
+  // then computes, so each wait covers one load issued a genotype earlier.
   // inr = 0 marks an individual outside [i0, i1) whose code was set to missing:
   // it adds nothing, also under IMP (only genuinely missing entries are imputed).
-  auto body = [&](uint32_t c, const T* __restrict__ qi, uint32_t inr) {
+  T qn[K];
+  auto loadq = [&](int64_t i) {
+    const T* __restrict__ src = q + (int64_t)K * (i < a.I ? i : a.I - 1);
+#pragma unroll
+    for (int k = 0; k < K; ++k) qn[k] = src[k];
+  };
+  auto body = [&](uint32_t c, int64_t next, uint32_t inr) {
     T qv[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) qv[k] = qi[k];
+    for (int k = 0; k < K; ++k) qv[k] = qn[k];
+    loadq(next);
     T s = qv[0] * pj[0];
 #pragma unroll
     for (int k = 1; k < K; ++k) s = __builtin_fma(qv[k], pj[k], s);
@@ -332,6 +342,7 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
     if (it + 256 < ia1) fetch(it + 256);
     const int s0 = (it < a.i0) ? (int)((a.i0 - it) >> 4) : 0;
     const int s1 = (int)((((ia1 - it) < 256 ? (ia1 - it) : 256) + 15) >> 4);
+    loadq(it + 16 * s0);
     for (int sw = s0; sw < s1; ++sw) {
       const int64_t ib = it + 16 * sw;
       uint32_t w = tw[sw][threadIdx.x];
@@ -343,17 +354,8 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
           if (ib + t >= a.i0 && ib + t < a.i1) keep |= 3u << (2 * t);
         w = (w & keep) | (0x55555555u & ~keep);
       }
-      if (ib + 16 <= a.I) {  // interior word: constant offsets from one base pointer
-        const T* __restrict__ qb = q + (int64_t)K * ib;
 #pragma unroll
-        for (int t = 0; t < 16; ++t) body((w >> (2 * t)) & 3u, qb + K * t, (keep >> (2 * t)) & 1u);
-      } else {  // last word of the cohort: clamp the (masked) individuals past I
-#pragma unroll
-        for (int t = 0; t < 16; ++t) {
-          const int64_t i = (ib + t < a.I) ? ib + t : a.I - 1;
-          body((w >> (2 * t)) & 3u, q + (int64_t)K * i, (keep >> (2 * t)) & 1u);
-        }
-      }
+      for (int t = 0; t < 16; ++t) body((w >> (2 * t)) & 3u, ib + t + 1, (keep >> (2 * t)) & 1u);
     }
   }
   if (!valid) return;
