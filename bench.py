@@ -344,6 +344,7 @@ def main():
     ap.add_argument("--sweep-ks", default="", help="comma-separated K values of the sweep (default 2..12)")
     ap.add_argument("--sweep-max-iter", type=int, default=300)
     ap.add_argument("--train-epochs", type=int, default=20, help="network training epochs (workload c5)")
+    ap.add_argument("--K", type=int, default=0, help="override the workload's K (throughput runs)")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -366,7 +367,9 @@ def main():
     from importlib import import_module
 
     sim = import_module("openadmixture_jl_amd.sim")
-    wl = WORKLOADS[args.workload]
+    wl = dict(WORKLOADS[args.workload])
+    if args.K:
+        wl["K"] = args.K
     I, J, K = wl["I"], wl["J"], wl["K"]
     T = np.float64 if args.dtype == "f64" else np.float32
 

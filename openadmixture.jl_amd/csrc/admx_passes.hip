@@ -34,9 +34,15 @@ namespace admx {
 // (one FMA per Hessian entry instead of two; its cancellation costs at most a
 // factor 1/min(p,1-p)^2 <= 1e10 on a 1e-16 unit roundoff).  FP32 keeps the
 // direct two-term form, where that factor would matter.
-template <typename T, int K, int MODE, bool TRICK, bool IMP>
+// PART (update mode): 0 accumulates everything; at large K the pass runs twice,
+// PART 1 with the first half of the Hessian triangle plus the vector sums and
+// PART 2 with the second half, so that each launch holds half the accumulators
+// (two waves per SIMD instead of one at K = 12).
+template <typename T, int K, int MODE, bool TRICK, bool IMP, int PART = 0>
 __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
   constexpr int NT = Tri<K>::N;
+  constexpr int N0 = (PART == 2) ? NT / 2 : 0, N1 = (PART == 1) ? NT / 2 : NT;  // owned triangle entries
+  constexpr bool VEC = PART != 2;                                               // owns z, hb, sb, sd
   const T* __restrict__ q = static_cast<const T*>(a.q);
   const T* __restrict__ p = static_cast<const T*>(a.p);
   const int64_t Ilen = a.i1 - a.i0;
@@ -51,14 +57,14 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
   T qi[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) qi[k] = q[k + (int64_t)K * i];
-  T H[MODE == MODE_UPDATE ? NT : 1];
+  T H[MODE == MODE_UPDATE ? N1 - N0 : 1];
   T z[K];
   T hb[(MODE == MODE_UPDATE && TRICK) ? K : 1];
   T sb = 0, sd = 0;
   double acc = 0.0, prod = 1.0;
   int esum = 0;
 #pragma unroll
-  for (int n = 0; n < (MODE == MODE_UPDATE ? NT : 1); ++n) H[n] = 0;
+  for (int n = 0; n < (MODE == MODE_UPDATE ? N1 - N0 : 1); ++n) H[n] = 0;
 #pragma unroll
   for (int k = 0; k < K; ++k) z[k] = 0;
 #pragma unroll
@@ -108,16 +114,21 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
       } else if constexpr (TRICK) {
         const T aa = cc * is, bb = dd * it;
         const T w = aa + bb, u = cc - dd;
-        sb += bb;
-        sd += dd;
+        if constexpr (VEC) {
+          sb += bb;
+          sd += dd;
+        }
         int n = 0;
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           const T wp = w * pv[k];
-          z[k] = __builtin_fma(u, pv[k], z[k]);
-          hb[k] = __builtin_fma(bb, pv[k], hb[k]);
+          if constexpr (VEC) {
+            z[k] = __builtin_fma(u, pv[k], z[k]);
+            hb[k] = __builtin_fma(bb, pv[k], hb[k]);
+          }
 #pragma unroll
-          for (int k2 = k; k2 < K; ++k2, ++n) H[n] = __builtin_fma(wp, pv[k2], H[n]);
+          for (int k2 = k; k2 < K; ++k2, ++n)
+            if (n >= N0 && n < N1) H[n - N0] = __builtin_fma(wp, pv[k2], H[n - N0]);
         }
       } else {
         const T aa = cc * is, bb = dd * it;
@@ -126,9 +137,11 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
         for (int k = 0; k < K; ++k) {
           const T mk = (T)1 - pv[k];
           const T ap = aa * pv[k], bm = bb * mk;
-          z[k] = __builtin_fma(cc, pv[k], __builtin_fma(dd, mk, z[k]));
+          if constexpr (VEC) z[k] = __builtin_fma(cc, pv[k], __builtin_fma(dd, mk, z[k]));
 #pragma unroll
-          for (int k2 = k; k2 < K; ++k2, ++n) H[n] = __builtin_fma(ap, pv[k2], __builtin_fma(bm, (T)1 - pv[k2], H[n]));
+          for (int k2 = k; k2 < K; ++k2, ++n)
+            if (n >= N0 && n < N1)
+              H[n - N0] = __builtin_fma(ap, pv[k2], __builtin_fma(bm, (T)1 - pv[k2], H[n - N0]));
         }
       }
     }
@@ -224,15 +237,18 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
     int o = 0;
     if constexpr (MODE == MODE_UPDATE) {
 #pragma unroll
-      for (int n = 0; n < NT; ++n) out[(o++) * Ilen] = H[n];
+      for (int n = N0; n < N1; ++n) out[(int64_t)n * Ilen] = H[n - N0];
+      o = NT;
     }
+    if constexpr (VEC) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) out[(o++) * Ilen] = z[k];
-    if constexpr (MODE == MODE_UPDATE && TRICK) {
+      for (int k = 0; k < K; ++k) out[(o++) * Ilen] = z[k];
+      if constexpr (MODE == MODE_UPDATE && TRICK) {
 #pragma unroll
-      for (int k = 0; k < K; ++k) out[(o++) * Ilen] = hb[k];
-      out[(o++) * Ilen] = sb;
-      out[(o++) * Ilen] = sd;
+        for (int k = 0; k < K; ++k) out[(o++) * Ilen] = hb[k];
+        out[(o++) * Ilen] = sb;
+        out[(o++) * Ilen] = sd;
+      }
     }
   }
 }
@@ -241,9 +257,13 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
 //   Xtz_p[k,j]    += (c - d) q_k               (exact dLL/dp)
 //   XtX_p[k2,k,j] += (a + b) q_k q_k2          (exact -d2LL/dp2)
 //   EM: p_tmp += c q_k p_k ; p_next += d q_k (1 - p_k)
-template <typename T, int K, int MODE, bool IMP>
+// PART as for k_qside: at large K the update pass runs twice, each launch owning
+// half of the Hessian triangle (PART 1 also the gradient).
+template <typename T, int K, int MODE, bool IMP, int PART = 0>
 __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
   constexpr int NT = Tri<K>::N;
+  constexpr int N0 = (PART == 2) ? NT / 2 : 0, N1 = (PART == 1) ? NT / 2 : NT;
+  constexpr bool VEC = PART != 2;
   const T* __restrict__ q = static_cast<const T*>(a.q);
   const T* __restrict__ p = static_cast<const T*>(a.p);
   const int64_t Jlen = a.j1 - a.j0;
@@ -260,10 +280,10 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
   T mj = 0;
   if constexpr (IMP) mj = static_cast<const T*>(a.mu)[j];
 
-  T H[MODE == MODE_UPDATE ? NT : 1];
+  T H[MODE == MODE_UPDATE ? N1 - N0 : 1];
   T z[K], e2[MODE == MODE_EM ? K : 1];
 #pragma unroll
-  for (int n = 0; n < (MODE == MODE_UPDATE ? NT : 1); ++n) H[n] = 0;
+  for (int n = 0; n < (MODE == MODE_UPDATE ? N1 - N0 : 1); ++n) H[n] = 0;
 #pragma unroll
   for (int k = 0; k < K; ++k) z[k] = 0;
 #pragma unroll
@@ -312,9 +332,10 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
 #pragma unroll
       for (int k = 0; k < K; ++k) {
         const T wq = w2 * qv[k];
-        z[k] = __builtin_fma(u, qv[k], z[k]);
+        if constexpr (VEC) z[k] = __builtin_fma(u, qv[k], z[k]);
 #pragma unroll
-        for (int k2 = k; k2 < K; ++k2, ++n) H[n] = __builtin_fma(wq, qv[k2], H[n]);
+        for (int k2 = k; k2 < K; ++k2, ++n)
+          if (n >= N0 && n < N1) H[n - N0] = __builtin_fma(wq, qv[k2], H[n - N0]);
       }
     }
   };
@@ -366,10 +387,13 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
   int o = 0;
   if constexpr (MODE == MODE_UPDATE) {
 #pragma unroll
-    for (int n = 0; n < NT; ++n) out[(o++) * ldp] = H[n];
+    for (int n = N0; n < N1; ++n) out[(int64_t)n * ldp] = H[n - N0];
+    o = NT;
   }
+  if constexpr (VEC) {
 #pragma unroll
-  for (int k = 0; k < K; ++k) out[(o++) * ldp] = z[k];
+    for (int k = 0; k < K; ++k) out[(o++) * ldp] = z[k];
+  }
   if constexpr (MODE == MODE_EM) {
 #pragma unroll
     for (int k = 0; k < K; ++k) out[(o++) * ldp] = e2[k];
@@ -382,7 +406,16 @@ static void qside_ki(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
   const int64_t n = a.i1 - a.i0;
   dim3 grid((unsigned)(((n + 255) / 256) * nchunk));  // 1-D, tiles ordered per XCD (xcd_tile)
   constexpr bool trick = sizeof(T) == 8;
-  if (mode == MODE_UPDATE) hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick, IMP>), grid, dim3(256), 0, s, a);
+  // FP64 from K = 11 on: the triangle in two launches (see k_qside)
+  constexpr bool split = sizeof(T) == 8 && K >= 11;
+  if (mode == MODE_UPDATE) {
+    if constexpr (split) {
+      hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick, IMP, 1>), grid, dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick, IMP, 2>), grid, dim3(256), 0, s, a);
+    } else {
+      hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick, IMP>), grid, dim3(256), 0, s, a);
+    }
+  }
   else if (mode == MODE_EM) hipLaunchKernelGGL((k_qside<T, K, MODE_EM, false, IMP>), grid, dim3(256), 0, s, a);
   else hipLaunchKernelGGL((k_qside<T, K, MODE_LL, false, IMP>), grid, dim3(256), 0, s, a);
 }
@@ -396,8 +429,17 @@ template <typename T, int K, bool IMP>
 static void pside_ki(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
   const int64_t n = a.j1 - a.j0;
   dim3 grid((unsigned)((n + 255) / 256), (unsigned)nchunk);
-  if (mode == MODE_UPDATE) hipLaunchKernelGGL((k_pside<T, K, MODE_UPDATE, IMP>), grid, dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_pside<T, K, MODE_EM, IMP>), grid, dim3(256), 0, s, a);
+  constexpr bool split = sizeof(T) == 8 && K >= 13;  // FP64: the triangle in two launches (see k_pside)
+  if (mode == MODE_UPDATE) {
+    if constexpr (split) {
+      hipLaunchKernelGGL((k_pside<T, K, MODE_UPDATE, IMP, 1>), grid, dim3(256), 0, s, a);
+      hipLaunchKernelGGL((k_pside<T, K, MODE_UPDATE, IMP, 2>), grid, dim3(256), 0, s, a);
+    } else {
+      hipLaunchKernelGGL((k_pside<T, K, MODE_UPDATE, IMP>), grid, dim3(256), 0, s, a);
+    }
+  } else {
+    hipLaunchKernelGGL((k_pside<T, K, MODE_EM, IMP>), grid, dim3(256), 0, s, a);
+  }
 }
 template <typename T, int K>
 static void pside_k(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
