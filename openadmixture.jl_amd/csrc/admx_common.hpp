@@ -2,9 +2,11 @@
 //
 // Data layout in HBM (identical to the reference's Julia arrays, column-major):
 //   BED   : SNP-major 2-bit genotypes, SNP j occupies `pitch` bytes starting at
-//           bed + pitch*j (pitch = ceil(I/4) rounded up to 16 bytes so 16-byte
-//           vector loads of a SNP column are aligned); individual i sits in
-//           byte i>>2 at bit offset 2*(i&3) (src/loops.jl:529-534).
+//           bed + pitch*j (pitch = ceil(I/4) rounded up to 64 bytes, so the
+//           16-byte loads of a SNP row and the 64-byte P-side tiles stay
+//           aligned and inside the row); J + 16 rows, the last 16 holding the
+//           missing code; individual i sits in byte i>>2 at bit offset 2*(i&3)
+//           (src/loops.jl:529-534).
 //   q     : K x I  (q[k,i] at k + K*i),  p : K x J  (p[k,j] at k + K*j)
 //   x     : K x (I+J), q = x[:, 0:I], p = x[:, I:I+J]  (src/structs.jl:120-137)
 //   XtX   : K x K x n,  Xtz : K x n

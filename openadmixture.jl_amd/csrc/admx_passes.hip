@@ -7,9 +7,10 @@
 // For each SNP every lane needs the same p[:,j] -- a wave-uniform address --
 // so P arrives through scalar loads into SGPRs and feeds the FMAs as the
 // scalar operand; the lane's own q[:,i] and all accumulators live in VGPRs.
+// Genotypes arrive 16 SNPs at a time through a per-wave LDS transpose.
 // P side (em_p, update_p; src/loops.jl:700-721, 923-950): one lane per SNP j,
-// streaming individuals; q[:,i] is wave-uniform (SGPRs) and each lane reads
-// 16 genotypes of its own SNP column per 32-bit load.
+// streaming individuals; q[:,i] is wave-uniform (SGPRs, loaded one genotype
+// ahead) and each lane's SNP row arrives in 64-byte tiles parked in LDS.
 //
 // IMP selects the mean-imputation loops (loops.jl:493-514, 589-608, 679-698,
 // 840-865, 896-921): a missing genotype counts as g = mu_j (the SNP's mean,
