@@ -47,7 +47,7 @@ struct FinArgs {
 // Number of per-column accumulators of a pass.
 inline int pass_na(int mode, bool qside, int K, bool f64) {
   const int tri = K * (K + 1) / 2;
-  if (mode == MODE_EM) return qside ? K : 2 * K;
+  if (mode == MODE_EM) return qside ? (f64 ? K + 1 : K) : 2 * K;  // FP64 EM-q: sum (c-d) p and sum d
   if (mode == MODE_UPDATE) {
     if (qside) return f64 ? tri + 2 * K + 2 : tri + K;
     return tri + K;

@@ -109,7 +109,13 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
       const T r = rcp(s * t);
       const T is = t * r, it = s * r;
       const T cc = gm * is, dd = hm * it;
-      if constexpr (MODE == MODE_EM) {
+      if constexpr (MODE == MODE_EM && sizeof(T) == 8) {
+        // FP64: sum c p + d (1-p) as sum (c-d) p + (sum d) 1, K+1 FMAs per genotype
+        const T u = cc - dd;
+        sd += dd;
+#pragma unroll
+        for (int k = 0; k < K; ++k) z[k] = __builtin_fma(u, pv[k], z[k]);
+      } else if constexpr (MODE == MODE_EM) {
 #pragma unroll
         for (int k = 0; k < K; ++k) z[k] = __builtin_fma(cc, pv[k], __builtin_fma(dd, (T)1 - pv[k], z[k]));
       } else if constexpr (TRICK) {
@@ -233,7 +239,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
   } else {
     if (!valid) return;
     T* __restrict__ part = static_cast<T*>(a.part);
-    const int NA = (MODE == MODE_EM) ? K : (TRICK ? NT + 2 * K + 2 : NT + K);
+    const int NA = (MODE == MODE_EM) ? (sizeof(T) == 8 ? K + 1 : K) : (TRICK ? NT + 2 * K + 2 : NT + K);
     T* out = part + tile.y * NA * Ilen + il;
     int o = 0;
     if constexpr (MODE == MODE_UPDATE) {
@@ -250,6 +256,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
         out[(o++) * Ilen] = sb;
         out[(o++) * Ilen] = sd;
       }
+      if constexpr (MODE == MODE_EM && sizeof(T) == 8) out[(o++) * Ilen] = sd;
     }
   }
 }

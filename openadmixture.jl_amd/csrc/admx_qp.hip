@@ -254,13 +254,20 @@ __global__ __launch_bounds__(256) void k_fin_q_em(FinArgs f) {
   const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (col >= f.n) return;
   const T* __restrict__ part = static_cast<const T*>(f.part);
-  T acc[K];
+  // FP64 partials carry sum (c-d) p_k and sum d (K+1 per chunk); FP32 the direct sums
+  constexpr int NA = sizeof(T) == 8 ? K + 1 : K;
+  T acc[K], dsum = 0;
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = 0;
   for (int c = 0; c < f.nchunk; ++c) {
-    const T* src = part + (int64_t)c * K * f.n + col;
+    const T* src = part + (int64_t)c * NA * f.n + col;
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] += src[(int64_t)k * f.n];
+    if constexpr (sizeof(T) == 8) dsum += src[(int64_t)K * f.n];
+  }
+  if constexpr (sizeof(T) == 8) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) acc[k] += dsum;
   }
   const int64_t g = f.off + col;
   const T* q = static_cast<const T*>(f.x_in) + (int64_t)K * g;
