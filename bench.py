@@ -60,10 +60,10 @@ WORKLOADS = {
 }
 
 
-KERNEL_SYMBOLS = {  # bench kernel name -> rocprof symbol prefix of its K=8 instance
-    "qside_update": "void admx::k_qside<double, 8, 0, true, false>",
-    "pside_update": "void admx::k_pside<double, 8, 0, false>",
-    "qside_ll": "void admx::k_qside<double, 8, 2, false, false>",
+KERNEL_SYMBOLS = {  # bench kernel name -> rocprof symbol prefix of its K=8 (skipmissing) instance
+    "qside_update": "void admx::k_qside<double, 8, 0, true, false",
+    "pside_update": "void admx::k_pside<double, 8, 0, false",
+    "qside_ll": "void admx::k_qside<double, 8, 2, false, false",
     "qside_update_bf16": "_ZN4admx12k_qside_bf16ILi8E",
     "pside_update_bf16": "void admx::k_pside_bf16<8>",
 }
@@ -463,7 +463,7 @@ def main():
             tw = time.perf_counter()
             Jp = min(20_000, J)
             panel = oa.Panel(Pt_true[:, :Jp].mean(0), seed=args.seed)
-            X, Tt, _ = oa.training_set(np.asfortranarray(Pt_true), panel, 10, I=5_000, J=Jp, Ks=range(4, K + 1),
+            X, Tt, _ = oa.training_set(np.asfortranarray(Pt_true), panel, 10, I=5_000, J=Jp, Ks=range(min(4, K), K + 1),
                                        alpha_range=(0.1, 1.0), miss_rate=wl["miss_rate"], seed=args.seed)
             t_data = time.perf_counter() - tw
             net = oa.MLP(64, 256, K, seed=args.seed)
