@@ -113,6 +113,37 @@ int admx_bed_from(admx_ctx *dst, admx_ctx *src, int32_t fold, int32_t nfolds, ui
 int admx_heldout_deviance(admx_ctx *fit, admx_ctx *src, int32_t fold, int32_t nfolds, uint64_t seed,
                           double *deviance_sum, int64_t *count);
 
+/* ---- sparse k-means feature ranking (SKFR): the AIM pre-selection of
+ * run_admixture(...; sparsity) (src/driver.jl:56-66, 80-101), where the
+ * reference calls SparseKmeansFeatureRanking.jl 0.1 (ImputedSnpMatrix,
+ * sparsekmeans1 / sparsekmeans2).  Genotypes are standardized by
+ * x = (g - mu_j) / sqrt(mu_j (1 - mu_j / 2)) with mu_j the mean over observed
+ * entries, missing entries x = 0.  Each iteration forms the cluster means,
+ * ranks the SNPs by sum_k n_k mean_kj^2 (global) or n_k mean_kj^2 per cluster
+ * (local), keeps the top `sparsity` (ties: lower SNP index first), zeroes the
+ * other center coordinates and moves every individual to its nearest nonempty
+ * center (ties: lower cluster); it stops when no label changes or after
+ * max_iter iterations.  Labels and SNP indices are 0-based.  nsplit (0: chosen
+ * from I and the column count, else 1..64) is the number of SNP slices whose
+ * distance partials are added in order; labels and AIMs are reproducible for
+ * a given nsplit.  Needs a resident BED; k <= 16. */
+#define ADMX_SKFR_GLOBAL 0 /* sparsekmeans1: one ranking over all clusters */
+#define ADMX_SKFR_LOCAL 1  /* sparsekmeans2: a ranking per cluster; AIMs = union */
+/* k-means++ start (the ImputedSnpMatrix constructor's initial clustering):
+ * centers drawn with probability proportional to the squared distance to the
+ * nearest chosen one, over all SNPs; clusters (I) = nearest center. */
+int admx_skfr_init(admx_ctx *ctx, int32_t k, uint64_t seed, int32_t nsplit, int32_t *clusters);
+/* sparsekmeans1 / sparsekmeans2 (ISM, sparsity; max_iter) from the labels in
+ * clusters (I, in/out).  aims receives the selected SNPs in decreasing score
+ * (capacity sparsity for ADMX_SKFR_GLOBAL, min(J, k * sparsity) for
+ * ADMX_SKFR_LOCAL, ranked by the largest per-cluster score), n_aims their
+ * number; centers (k x J column-major, nullable) the sparse centers in
+ * standardized units; wss (k, nullable) the within-cluster sums of squares
+ * over all SNPs and obj (nullable) their total; iters the iterations run. */
+int admx_skfr_run(admx_ctx *ctx, int32_t k, int32_t mode, int64_t sparsity, int32_t max_iter, int32_t nsplit,
+                  int32_t *clusters, int64_t *aims, int64_t *n_aims, double *centers, double *wss, double *obj,
+                  int32_t *iters);
+
 /* ---- learned warm start.  No reference counterpart (the reference starts
  * from a random point plus FRAPPE EM, src/algorithms_outer.jl:12-45).
  * Features: per individual, (1/sqrt(Jp)) sum_{j < Jp observed} (g_ij - mu_j) isd_j R[j, :]

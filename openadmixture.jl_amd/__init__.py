@@ -9,7 +9,9 @@ interface for that path:
   loglikelihood, em_q, em_p,         src/algorithms_inner.jl
   update_q, update_p
   init_em, admixture_qn              src/algorithms_outer.jl
-  run_admixture                      src/driver.jl
+  run_admixture, _filter_SKFR        src/driver.jl
+  ImputedSnpMatrix, sparsekmeans1/2, SKFR AIM pre-selection (SparseKmeansFeatureRanking
+  sparsekmeans_repeat/_path          calls of src/driver.jl:56-123)
   HIPAdmixData, hip_admixture_base   src/cuda/structs.jl (device mirror)
   cv_sweep, autotune_waves           K sweep with n-fold CV (no reference counterpart)
   MLP, Panel, warm_started_fit       learned warm start (no reference counterpart)
@@ -22,12 +24,13 @@ from . import _native
 from ._native import AdmixHIPError, device_count
 from .algorithms import admixture_qn, em_p, em_q, init_em, loglikelihood, update_p, update_q
 from .cv import best_k, cv_sweep, fit_on_device
-from .driver import _admixture_base, hip_admixture_base, run_admixture
+from .driver import _admixture_base, _filter_SKFR, hip_admixture_base, run_admixture
 from .hip import HIPAdmixData
 from .loops import (UnitRange, em_p_loop, em_p_loop_skipmissing, em_q_loop, em_q_loop_skipmissing,
                     loglikelihood_loop, loglikelihood_loop_skipmissing, update_p_loop, update_p_loop_skipmissing,
                     update_q_loop, update_q_loop_skipmissing)
-from .plink import SnpArray, SnpLinAlg, write_plink
+from .plink import SnpArray, SnpLinAlg, filter_plink, write_plink
+from .skfr import ImputedSnpMatrix, sparsekmeans1, sparsekmeans2, sparsekmeans_path, sparsekmeans_repeat
 from .structs import AdmixData
 from .tune import autotune_waves
 from .warmstart import MLP, Panel, training_set, warm_started_fit, world_frequencies
@@ -44,4 +47,6 @@ __all__ = [
     "SnpArray", "SnpLinAlg", "write_plink", "AdmixHIPError", "device_count",
     "cv_sweep", "best_k", "fit_on_device", "autotune_waves",
     "MLP", "Panel", "training_set", "warm_started_fit", "world_frequencies",
+    "_filter_SKFR", "filter_plink", "ImputedSnpMatrix", "sparsekmeans1", "sparsekmeans2", "sparsekmeans_repeat",
+    "sparsekmeans_path",
 ]

@@ -22,6 +22,7 @@ ADMX_TUNE_QN_MODE = 4
 ADMX_TUNE_SKIPMISSING = 5
 ADMX_QN_ZAL, ADMX_QN_LBQN = 0, 1
 ADMX_PREC_F64, ADMX_PREC_F32, ADMX_PREC_BF16 = 0, 1, 2
+ADMX_SKFR_GLOBAL, ADMX_SKFR_LOCAL = 0, 1
 
 # Every exported symbol with its C signature (return type is int unless noted).
 _i64, _int, _dbl, _vp, _u64 = C.c_int64, C.c_int, C.c_double, C.c_void_p, C.c_uint64
@@ -49,6 +50,9 @@ SIGNATURES = {
     "admx_features": ([_vp, _vp, _vp, _vp, _i64, _vp], _int),
     "admx_warm_q": ([_vp, _vp], _int),
     "admx_heldout_deviance": ([_vp, _vp, C.c_int32, C.c_int32, C.c_uint64, _vp, _vp], _int),
+    "admx_skfr_init": ([_vp, C.c_int32, C.c_uint64, C.c_int32, _vp], _int),
+    "admx_skfr_run": ([_vp, C.c_int32, C.c_int32, _i64, C.c_int32, C.c_int32, _vp, _vp, _vp, _vp, _vp, _vp, _vp],
+                      _int),
     "admx_loglikelihood_loop": ([_vp, _vp, _vp, _i64, _i64, _i64, _i64, _vp], _int),
     "admx_em_q_loop": ([_vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64], _int),
     "admx_em_p_loop": ([_vp, _vp, _vp, _vp, _vp, _i64, _i64, _i64, _i64], _int),

@@ -135,4 +135,39 @@ void launch_adam(int64_t n, float* w, const float* g, float* m, float* v, float 
                  float c2, float eps, hipStream_t s);
 void launch_probs_to_q(const float* Y, int64_t n, double* q, hipStream_t s);
 
+// sparse k-means feature ranking (admx_skfr.hip)
+void launch_skfr_masks(const int32_t* lab, int64_t I, int k, int64_t nwords, uint32_t* cm, unsigned* nk,
+                       hipStream_t s);
+int64_t skfr_slabs(int64_t J, int64_t ngroups, int64_t* gper);
+void launch_skfr_centers(int k, const uint8_t* bed, int64_t pitch, int64_t J, int64_t ngroups, int64_t gper,
+                         int64_t nslab, const uint32_t* cm, uint32_t* part, hipStream_t s);
+void launch_skfr_stats(const unsigned long long* nm, const unsigned long long* alt, int64_t J, double* mu,
+                       double* var, hipStream_t s);
+void launch_skfr_score(int k, int local, const uint32_t* part, int64_t nslab, const unsigned* nk, const double* mu,
+                       const double* var, int64_t J, double* E, double* key, int32_t* idx, hipStream_t s);
+size_t skfr_sort_bytes(int64_t n);
+void launch_skfr_sort(void* tmp, size_t bytes, const double* kin, double* kout, const int32_t* vin, int32_t* vout,
+                      int64_t n, hipStream_t s);
+void launch_skfr_flag(const int32_t* sorted, int64_t J, int64_t sp, int k, uint8_t* flag, hipStream_t s);
+void launch_skfr_union(const uint8_t* flag, const double* key, int64_t J, int k, double* ukey, int32_t* idx,
+                       unsigned long long* count, hipStream_t s);
+void launch_skfr_tables(const int32_t* cols, int64_t ns, int k, const uint8_t* flag, const unsigned* nk,
+                        const double* mu, const double* var, const double* E, int64_t J, double* T, hipStream_t s);
+void launch_skfr_pp_table(const uint8_t* bed, int64_t pitch, const int32_t* center, const double* mu,
+                          const double* var, int64_t J, double* T, hipStream_t s);
+int skfr_nsplit(int64_t I, int64_t ncols);
+void launch_skfr_tabsum(int R, const uint8_t* bed, int64_t pitch, int64_t I, const int32_t* cols, int64_t ncols,
+                        int nsplit, const double* T, double* P, hipStream_t s);
+void launch_skfr_assign(const double* P, int nsplit, int R, int64_t I, const unsigned* nk, int32_t* lab,
+                        double* dmin, unsigned long long* changes, hipStream_t s);
+void launch_skfr_combine(const double* P, int nsplit, int64_t I, double* out, double* D2, int first,
+                         hipStream_t s);
+void launch_skfr_pick(const double* D2, int64_t I, double u, int first, int32_t* center, hipStream_t s);
+void launch_skfr_argmin(const double* Dc, int k, int64_t I, int32_t* lab, hipStream_t s);
+void launch_skfr_wss(const int32_t* lab, const double* x2, const double* dmin, int64_t I, int k, double* out,
+                     hipStream_t s);
+void launch_skfr_mark(const int32_t* cols, int64_t ns, uint8_t* sel, hipStream_t s);
+void launch_skfr_center_out(const uint8_t* sel, const uint8_t* flag, const unsigned* nk, const double* var,
+                            const double* E, int64_t J, int k, double* out, hipStream_t s);
+
 }  // namespace admx

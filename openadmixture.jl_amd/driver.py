@@ -5,7 +5,8 @@ import numpy as np
 
 from .algorithms import admixture_qn, init_em
 from .hip import HIPAdmixData
-from .plink import SnpArray, SnpLinAlg
+from .plink import SnpArray, SnpLinAlg, filter_plink
+from .skfr import ImputedSnpMatrix, sparsekmeans1, sparsekmeans2, sparsekmeans_path, sparsekmeans_repeat
 from .structs import AdmixData
 
 
@@ -50,19 +51,64 @@ def _admixture_base(filename, K, n_iter=1000, rtol=1e-7, rng=None, em_iters=5, T
     return d
 
 
+def _filter_SKFR(filename, K, sparsity, rng=None, prefix=None, tries=10, max_inner_iter=50, ftn=sparsekmeans1,
+                 log=print, nsplit=0):
+    """_filter_SKFR(filename, K, sparsity; rng, prefix, tries, max_inner_iter, ftn)
+    (src/driver.jl:80-101; the vector-of-sparsities method :106-123): SKFR on
+    the device, then the PLINK trio reduced to the AIMs (sorted SNP order) as
+    prefix_K_{sparsity}aims.  Returns (bed path, clusters, aims) for one
+    sparsity, (bed paths, clusters per level, aims per level) for a list."""
+    if not filename.endswith(".bed"):
+        raise ValueError("filename should end with .bed")
+    prefix = filename[:-4] if prefix is None else prefix
+    X = ImputedSnpMatrix(filename, K, rng=rng, nsplit=nsplit)
+    if isinstance(sparsity, (list, tuple, np.ndarray)):
+        clusters, aims = sparsekmeans_path(X, sparsity, iter=tries, max_inner_iter=max_inner_iter, ftn=ftn)
+        outs = []
+        for s, a in zip(sparsity, aims):
+            des = f"{prefix}_{K}_{int(s)}aims"
+            filter_plink(filename, None, np.sort(a), des)
+            outs.append(des)
+        return outs, clusters, aims
+    if tries == 1:
+        clusters, _, aims, _, _ = ftn(X, int(sparsity), max_iter=max_inner_iter, squares=False)
+    else:
+        clusters, _, aims, _, _, _ = sparsekmeans_repeat(X, int(sparsity), iter=tries, max_inner_iter=max_inner_iter,
+                                                         ftn=ftn)
+    des = f"{prefix}_{K}_{int(sparsity)}aims"
+    if log:
+        log(des)
+    filter_plink(filename, None, np.sort(aims), des)
+    return des + ".bed", clusters, aims
+
+
 def run_admixture(filename, K, rng=None, sparsity=None, prefix=None, skfr_tries=1, skfr_max_inner_iter=50,
                   skfr_mode="global", admix_n_iter=1000, admix_rtol=1e-7, admix_n_em_iters=5, T=np.float64, Q=3,
                   use_gpu=True, verbose=False, progress_bar=False, fix_q=False, fix_p=False, init_q=None,
                   init_p=None, log=print, mode="ZAL", stream_snps=0):
     """run_admixture(filename, K; ...)  (src/driver.jl:35-75).  Returns
-    (d, clusters, aims); the SKFR pre-selection (sparsity) is not provided."""
+    (d, clusters, aims); with sparsity given, SKFR first selects that many
+    AIMs (skfr_mode "global": sparsekmeans1, "local": sparsekmeans2) and the
+    fit runs on the filtered PLINK file prefix_K_{sparsity}aims.bed.  Cluster
+    labels and AIM indices are 0-based."""
     if not filename.endswith(".bed"):
         raise ValueError("filename should end with .bed")
-    if sparsity is not None:
-        raise NotImplementedError("SKFR AIM pre-selection (sparsity=...) is outside this package")
     if log:
         log(f"Using {filename} as input.")
-    d = _admixture_base(filename, K, n_iter=admix_n_iter, rtol=admix_rtol, rng=rng, em_iters=admix_n_em_iters,
+    if sparsity is not None:
+        mode_s = str(skfr_mode).lstrip(":")
+        if mode_s == "global":
+            ftn = sparsekmeans1
+        elif mode_s == "local":
+            ftn = sparsekmeans2
+        else:
+            raise ValueError("skfr_mode can only be 'global' or 'local'")
+        admix_input, clusters, aims = _filter_SKFR(filename, K, sparsity, rng=rng, prefix=prefix,
+                                                   tries=skfr_tries, max_inner_iter=skfr_max_inner_iter, ftn=ftn,
+                                                   log=log)
+    else:
+        admix_input, clusters, aims = filename, None, None
+    d = _admixture_base(admix_input, K, n_iter=admix_n_iter, rtol=admix_rtol, rng=rng, em_iters=admix_n_em_iters,
                         T=T, Q=Q, use_gpu=use_gpu, verbose=verbose, progress_bar=progress_bar, fix_q=fix_q,
                         fix_p=fix_p, init_q=init_q, init_p=init_p, log=log, mode=mode, stream_snps=stream_snps)
-    return d, None, None
+    return d, clusters, aims

@@ -118,6 +118,35 @@ class HIPAdmixData:
                                               C.byref(dev), C.byref(cnt)))
         return dev.value, cnt.value
 
+    # ------------------------------------------------------------ SKFR AIM pre-selection
+    def skfr_init(self, k: int, seed: int = 0, nsplit: int = 0) -> np.ndarray:
+        """k-means++ initial clusters (I, 0-based) of the standardized genotypes."""
+        cl = np.zeros(self.I, np.int32)
+        N.check(N.lib().admx_skfr_init(self._h, int(k), C.c_uint64(seed), int(nsplit), N.ptr(cl)))
+        return cl
+
+    def skfr_run(self, k: int, clusters, sparsity: int, mode: str = "global", max_iter: int = 50,
+                 nsplit: int = 0, centers: bool = True):
+        """Sparse k-means by feature ranking from `clusters`; returns
+        (clusters, centers (k x J or None), aims (0-based, decreasing score),
+        WSS (k), objective, iterations)."""
+        modes = {"global": N.ADMX_SKFR_GLOBAL, "local": N.ADMX_SKFR_LOCAL}
+        if mode not in modes:
+            raise ValueError("skfr mode can only be 'global' or 'local'")
+        cl = np.ascontiguousarray(clusters, dtype=np.int32).copy()
+        if cl.shape != (self.I,):
+            raise ValueError(f"clusters must have {self.I} entries")
+        cap = min(self.J, int(k) * int(sparsity)) if mode == "local" else int(sparsity)
+        aims = np.zeros(max(cap, 1), np.int64)
+        n_aims = C.c_int64()
+        cen = np.zeros((int(k), self.J), np.float64, order="F") if centers else None
+        wss = np.zeros(int(k), np.float64)
+        obj, iters = C.c_double(), C.c_int32()
+        N.check(N.lib().admx_skfr_run(self._h, int(k), modes[mode], int(sparsity), int(max_iter), int(nsplit),
+                                      N.ptr(cl), N.ptr(aims), C.byref(n_aims), N.ptr(cen), N.ptr(wss),
+                                      C.byref(obj), C.byref(iters)))
+        return cl, cen, aims[: n_aims.value].copy(), wss, obj.value, iters.value
+
     # ------------------------------------------------------------ learned warm start
     def features(self, R, mu, isd, out=None):
         """Genotype-projection features over the first len(mu) SNPs (kept on the

@@ -75,3 +75,47 @@ def write_plink(prefix: str, data: np.ndarray, I: int, J: int) -> str:
         for j in range(J):
             f.write(f"1 snp{j} 0 {j + 1} A C\n")
     return prefix + ".bed"
+
+
+def _read_lines(path):
+    with open(path, "r") as f:
+        return [line.rstrip("\r\n") + "\n" for line in f if line.strip()]
+
+
+def filter_plink(src: str, rowmask=None, cols=None, des: str = "filtered") -> str:
+    """SnpArrays.filter(src, rowmask, colmask; des) as src/driver.jl:97 and
+    :120 use it: write des.bed/.bim/.fam holding the individuals where
+    ``rowmask`` is true (None: all) and the SNPs ``cols`` (0-based indices in
+    the order given; None: all), with the source's .fam/.bim lines.  Returns
+    the .bed path."""
+    stem = src[:-4] if src.endswith(".bed") else src
+    g = SnpArray(stem + ".bed")
+    I, J = g.shape
+    cols = np.arange(J) if cols is None else np.asarray(cols, dtype=np.int64)
+    if cols.size and (cols.min() < 0 or cols.max() >= J):
+        raise ValueError("SNP index out of range")
+    rows = np.ones(I, bool) if rowmask is None else np.asarray(rowmask, dtype=bool)
+    if rows.shape != (I,):
+        raise ValueError(f"rowmask must have {I} entries")
+    keep = np.flatnonzero(rows)
+    In = keep.size
+    nb_out = (In + 3) // 4
+    if In == I:
+        payload = np.asarray(g.data[:, cols])
+    else:
+        payload = np.zeros((nb_out, cols.size), np.uint8)
+        src_byte, src_shift = keep >> 2, 2 * (keep & 3)
+        dst_byte, dst_shift = np.arange(In) >> 2, 2 * (np.arange(In) & 3)
+        for t, j in enumerate(cols):
+            codes = (np.asarray(g.data[src_byte, j]) >> src_shift) & 3
+            np.bitwise_or.at(payload[:, t], dst_byte, (codes << dst_shift).astype(np.uint8))
+    with open(des + ".bed", "wb") as f:
+        f.write(MAGIC)
+        f.write(np.asfortranarray(payload).tobytes(order="F"))
+    fam = _read_lines(stem + ".fam")
+    bim = _read_lines(stem + ".bim")
+    with open(des + ".fam", "w") as f:
+        f.writelines(fam[i] for i in keep)
+    with open(des + ".bim", "w") as f:
+        f.writelines(bim[j] for j in cols)
+    return des + ".bed"

@@ -351,3 +351,78 @@ class OracleAdmix:
         self._f("orc_qn_iteration_mode")(C.byref(self.st), it, int(fix_q), int(fix_p), 1 if mode == "LBQN" else 0,
                                          _p(out))
         return dict(prev=out[0], basic=out[1], qn=out[2], new=out[3], reldiff=out[4])
+
+
+# ---------------------------------------------------------------- SKFR (oracle/skfr_oracle.c)
+def _declare_skfr(L):
+    L.orc_skfr_u.argtypes = [C.c_uint64, C.c_int]
+    L.orc_skfr_u.restype = c_dbl
+    L.orc_skfr_nsplit.argtypes = [c_i64, c_i64, C.c_int]
+    L.orc_skfr_nsplit.restype = C.c_int
+    L.orc_skfr_stats.argtypes = [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp]
+    L.orc_skfr_init.argtypes = [c_vp, c_i64, c_i64, c_i64, C.c_int, C.c_uint64, C.c_int, c_vp, c_vp]
+    L.orc_skfr_init.restype = C.c_int
+    L.orc_skfr_run.argtypes = [c_vp, c_i64, c_i64, c_i64, C.c_int, C.c_int, c_i64, C.c_int, C.c_int, c_vp, c_vp, c_vp,
+                               c_vp, c_vp, c_vp, c_vp]
+    L.orc_skfr_run.restype = C.c_int
+
+
+def _skfr_lib():
+    L = lib()
+    if not hasattr(L, "_skfr_declared"):
+        _declare_skfr(L)
+        L._skfr_declared = True
+    return L
+
+
+def _bedp(bed):
+    bed = np.asfortranarray(bed, dtype=np.uint8)
+    return bed, bed.shape[0]
+
+
+def skfr_u(seed: int, cc: int) -> float:
+    return _skfr_lib().orc_skfr_u(C.c_uint64(seed), cc)
+
+
+def skfr_nsplit(I: int, ncols: int, nsplit: int) -> int:
+    return _skfr_lib().orc_skfr_nsplit(I, ncols, nsplit)
+
+
+def skfr_stats(bed, I: int, J: int):
+    """(mu_j, v_j) of a pitch x J BED: mean over observed entries, mu (1 - mu/2)."""
+    bed, pitch = _bedp(bed)
+    mu, var = np.zeros(J), np.zeros(J)
+    _skfr_lib().orc_skfr_stats(bed.ctypes.data, pitch, I, J, mu.ctypes.data, var.ctypes.data)
+    return mu, var
+
+
+def skfr_init(bed, I: int, J: int, k: int, seed: int, nsplit: int):
+    """k-means++ start: (clusters, drawn center individuals)."""
+    bed, pitch = _bedp(bed)
+    cl = np.zeros(I, np.int32)
+    cen = np.zeros(k, np.int64)
+    rc = _skfr_lib().orc_skfr_init(bed.ctypes.data, pitch, I, J, k, C.c_uint64(seed), nsplit, cl.ctypes.data,
+                                   cen.ctypes.data)
+    if rc:
+        raise ValueError("orc_skfr_init: bad arguments")
+    return cl, cen
+
+
+def skfr_run(bed, I: int, J: int, k: int, clusters, sparsity: int, mode: str = "global", max_iter: int = 50,
+             nsplit: int = 1):
+    """Sparse k-means by feature ranking: (clusters, centers k x J, aims, WSS, obj, iters)."""
+    bed, pitch = _bedp(bed)
+    cl = np.ascontiguousarray(clusters, dtype=np.int32).copy()
+    local = 1 if mode == "local" else 0
+    cap = min(J, k * sparsity) if local else sparsity
+    aims = np.zeros(max(cap, 1), np.int64)
+    n_aims = C.c_int64()
+    cen = np.zeros((k, J), order="F")
+    wss = np.zeros(k)
+    obj, iters = C.c_double(), C.c_int32()
+    rc = _skfr_lib().orc_skfr_run(bed.ctypes.data, pitch, I, J, k, local, sparsity, max_iter, nsplit, cl.ctypes.data,
+                                  aims.ctypes.data, C.addressof(n_aims), cen.ctypes.data, wss.ctypes.data,
+                                  C.addressof(obj), C.addressof(iters))
+    if rc:
+        raise ValueError("orc_skfr_run: bad arguments")
+    return cl, cen, aims[: n_aims.value].copy(), wss, obj.value, iters.value

@@ -120,11 +120,11 @@ def test_plink_round_trip(oa, tmp_path):
         oa.SnpArray(str(tmp_path / "toy.txt"))
 
 
-def test_run_admixture_refuses_out_of_scope(oa, tmp_path):
-    with pytest.raises(NotImplementedError):
-        oa.run_admixture(str(tmp_path / "x.bed"), 3, sparsity=100, log=None)
+def test_run_admixture_argument_checks(oa, tmp_path):
     with pytest.raises(ValueError):
         oa.run_admixture(str(tmp_path / "x.txt"), 3, log=None)
+    with pytest.raises(ValueError):
+        oa.run_admixture(str(tmp_path / "x.bed"), 3, sparsity=100, skfr_mode="other", log=None)
 
 
 def test_sim_truth(oa):
