@@ -301,6 +301,68 @@ def run_warmstart(args):
     print(json.dumps(out), flush=True)
 
 
+def skfr_block(oa, ctx, wl, args):
+    """SKFR AIM pre-selection (src/driver.jl:56-101) on the same resident BED:
+    k = K clusters, sparsity = J/10 (the reference's documented example keeps
+    10,000 of 54,051 SNPs), k-means++ start, then up to --skfr-iters
+    iterations.  Kernel times come from the context's HIP-event profile; the
+    centers pass (one read of the whole BED per iteration) is HBM-bound, the
+    table-sum pass over the selected SNPs is FP64 vector-ALU bound.  The CPU
+    figure is the single-threaded oracle on a bounded sample of the same
+    cohort."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    I, J, K = wl["I"], wl["J"], wl["K"]
+    s = max(1, J // 10)
+    ctx.profile(True)
+    ctx.profile_reset()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    start = ctx.skfr_init(K, args.seed, 0)
+    t_init = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    cl, _, aims, _, obj, iters = ctx.skfr_run(K, start, s, max_iter=args.skfr_iters, centers=False)
+    t_run = time.perf_counter() - t0
+    rep = ctx.profile_report()
+    ctx.profile(False)
+    bed_bytes = ((I + 3) // 4) * J
+    n_c, ms_c = rep["skfr_centers"]
+    n_t, ms_t = rep["skfr_tabsum"]
+    per_c = ms_c / n_c / 1e3
+    per_t = ms_t / n_t / 1e3
+    blk = {
+        "config": f"I={I}, J={J}, k={K}, sparsity={s}, global ranking, k-means++ seed {args.seed}",
+        "iterations": iters, "init_seconds": t_init, "run_seconds": t_run,
+        "ms_per_iteration": t_run / iters * 1e3,
+        "genotype_iters_per_s": I * J * iters / t_run,
+        "kernels_ms": {k: round(t / n, 4) for k, (n, t) in rep.items() if k.startswith("skfr")},
+        "roofline_centers": {"bound": "hbm", "achieved": bed_bytes / per_c / 1e9, "peak": 8000.0, "unit": "GB/s",
+                             "frac": bed_bytes / per_c / 8e12,
+                             "note": f"algorithmic bytes = the BED, ceil(I/4)*J = {bed_bytes} per launch"},
+        "tabsum": {"lookups_per_s": I * s * K / per_t, "avg_ms": per_t * 1e3,
+                   "note": "one FP64 table lookup + add per (individual, selected SNP, cluster)"},
+        "objective": obj,
+    }
+    if not args.no_cpu_baseline:
+        import pyoracle as O
+
+        from importlib import import_module
+
+        sim = import_module("openadmixture_jl_amd.sim")
+        Is, Js = min(I, 4000), min(J, 4000)
+        Qt, Pt = sim.draw_truth(Is, Js, K, wl["alpha"], wl["p_dist"], wl["p_lo"], wl["p_hi"], seed=args.seed)
+        pitch = (Is + 3) // 4
+        bed = O.gen_bed(Qt, Pt, args.seed, wl["miss_rate"], pitch)
+        st, _ = O.skfr_init(bed, Is, Js, K, args.seed, 1)
+        t0 = time.perf_counter()
+        r = O.skfr_run(bed, Is, Js, K, st, max(1, Js // 10), "global", 3, 1)
+        el = time.perf_counter() - t0
+        blk["cpu_baseline"] = {"value": Is * Js * r[5] / el, "unit": "genotype*iter/s", "cores": 1, "kind": "port",
+                               "sample": f"SKFR oracle (C), I={Is}, J={Js}, k={K}, sparsity={Js // 10}, "
+                                         f"{r[5]} iterations, {el:.2f} s"}
+        blk["gpu_over_cpu"] = blk["genotype_iters_per_s"] / blk["cpu_baseline"]["value"]
+    return blk
+
+
 def calibrate_copy(nbytes=1 << 30):
     """A plain device-to-device copy of known size (HIP runtime copy kernel), used
     to scale the FETCH_SIZE / WRITE_SIZE counters of a profiled run."""
@@ -345,6 +407,8 @@ def main():
     ap.add_argument("--sweep-max-iter", type=int, default=300)
     ap.add_argument("--train-epochs", type=int, default=20, help="network training epochs (workload c5)")
     ap.add_argument("--K", type=int, default=0, help="override the workload's K (throughput runs)")
+    ap.add_argument("--skfr-iters", type=int, default=10, help="SKFR iterations of the AIM pre-selection block")
+    ap.add_argument("--no-skfr", action="store_true", help="skip the SKFR AIM pre-selection block")
     args = ap.parse_args()
     if args.calibrate_copy:
         calibrate_copy()
@@ -494,6 +558,8 @@ def main():
             net.close()
         ctx.set_precision("f64" if args.form == "plain" else args.form)
         out["time_to_converge"] = ttc
+    if not args.no_skfr and args.stream_snps == 0:
+        out["skfr"] = skfr_block(oa, ctx, wl, args)
     if not args.no_cpu_baseline:
         cb = cpu_baseline(wl, args.dtype, args.seed, args.cpu_budget)
         out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}

@@ -1472,9 +1472,15 @@ int admx_skfr_run(admx_ctx* c, int32_t k, int32_t mode, int64_t sparsity, int32_
   for (it = 1; it <= max_iter; ++it) {
     HIPCHK(hipMemsetAsync(v.nk, 0, sizeof(unsigned) * kMaxK, c->s));
     {
-      ProfScope ps(c, "skfr_centers");
+      ProfScope ps(c, "skfr_masks");
       launch_skfr_masks(v.lab, I, k, v.nwords, v.cm, v.nk, c->s);
+    }
+    {
+      ProfScope ps(c, "skfr_centers");
       launch_skfr_centers(k, c->bed, c->pitch, J, v.ngroups, v.gper, v.nslab, v.cm, v.part, c->s);
+    }
+    {
+      ProfScope ps(c, "skfr_score");
       launch_skfr_score(k, local, v.part, v.nslab, v.nk, v.mu, v.var, J, v.E, v.key, v.idx, c->s);
     }
     {
@@ -1504,9 +1510,15 @@ int admx_skfr_run(admx_ctx* c, int32_t k, int32_t mode, int64_t sparsity, int32_
     const int ns = skfr_split(c, nsplit, ncols);
     HIPCHK(hipMemsetAsync(v.changes, 0, sizeof(unsigned long long), c->s));
     {
-      ProfScope ps(c, "skfr_assign");
+      ProfScope ps(c, "skfr_tables");
       launch_skfr_tables(v.sidx, ncols, k, local ? v.flag : nullptr, v.nk, v.mu, v.var, v.E, J, v.T, c->s);
+    }
+    {
+      ProfScope ps(c, "skfr_tabsum");
       launch_skfr_tabsum(k, c->bed, c->pitch, I, v.sidx, ncols, ns, v.T, v.P, c->s);
+    }
+    {
+      ProfScope ps(c, "skfr_assign");
       launch_skfr_assign(v.P, ns, k, I, v.nk, v.lab, v.dmin, v.changes, c->s);
     }
     CHK(check_launch());

@@ -375,8 +375,16 @@ def _skfr_lib():
     return L
 
 
-def _bedp(bed):
-    bed = np.asfortranarray(bed, dtype=np.uint8)
+def _bedp(bed, I, J):
+    """(pitch x J Fortran BED, pitch) from a 2-D payload or a flat one of J equal rows."""
+    bed = np.asarray(bed, dtype=np.uint8)
+    if bed.ndim == 1:
+        if J < 1 or bed.size % J:
+            raise ValueError("flat BED size is not a multiple of J")
+        bed = bed.reshape((bed.size // J, J), order="F")
+    bed = np.asfortranarray(bed)
+    if bed.shape[1] != J or bed.shape[0] < (I + 3) // 4:
+        raise ValueError(f"BED of shape {bed.shape} does not hold I={I}, J={J}")
     return bed, bed.shape[0]
 
 
@@ -390,7 +398,7 @@ def skfr_nsplit(I: int, ncols: int, nsplit: int) -> int:
 
 def skfr_stats(bed, I: int, J: int):
     """(mu_j, v_j) of a pitch x J BED: mean over observed entries, mu (1 - mu/2)."""
-    bed, pitch = _bedp(bed)
+    bed, pitch = _bedp(bed, I, J)
     mu, var = np.zeros(J), np.zeros(J)
     _skfr_lib().orc_skfr_stats(bed.ctypes.data, pitch, I, J, mu.ctypes.data, var.ctypes.data)
     return mu, var
@@ -398,7 +406,7 @@ def skfr_stats(bed, I: int, J: int):
 
 def skfr_init(bed, I: int, J: int, k: int, seed: int, nsplit: int):
     """k-means++ start: (clusters, drawn center individuals)."""
-    bed, pitch = _bedp(bed)
+    bed, pitch = _bedp(bed, I, J)
     cl = np.zeros(I, np.int32)
     cen = np.zeros(k, np.int64)
     rc = _skfr_lib().orc_skfr_init(bed.ctypes.data, pitch, I, J, k, C.c_uint64(seed), nsplit, cl.ctypes.data,
@@ -411,8 +419,10 @@ def skfr_init(bed, I: int, J: int, k: int, seed: int, nsplit: int):
 def skfr_run(bed, I: int, J: int, k: int, clusters, sparsity: int, mode: str = "global", max_iter: int = 50,
              nsplit: int = 1):
     """Sparse k-means by feature ranking: (clusters, centers k x J, aims, WSS, obj, iters)."""
-    bed, pitch = _bedp(bed)
+    bed, pitch = _bedp(bed, I, J)
     cl = np.ascontiguousarray(clusters, dtype=np.int32).copy()
+    if cl.shape != (I,) or cl.min() < 0 or cl.max() >= k:
+        raise ValueError("clusters must hold I labels in [0, k)")
     local = 1 if mode == "local" else 0
     cap = min(J, k * sparsity) if local else sparsity
     aims = np.zeros(max(cap, 1), np.int64)

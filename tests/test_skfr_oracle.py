@@ -231,3 +231,21 @@ def test_nsplit_changes_only_rounding(O):
         np.testing.assert_array_equal(r[0], runs[0][0])
         np.testing.assert_array_equal(r[2], runs[0][2])
         assert r[4] == pytest.approx(runs[0][4], rel=1e-12)
+
+
+def test_flat_payload_and_shape_checks(O):
+    """A flat payload (as the oracle's generator returns it) is read with
+    pitch = size / J; payloads that cannot hold I x J are refused."""
+    G, lab = cohort(50, 30, 2, 12)
+    bed = pack(G)
+    start = lab.astype(np.int32)
+    a = O.skfr_run(bed, 50, 30, 2, start, 10, "global", 5, 1)
+    b = O.skfr_run(bed.ravel(order="F"), 50, 30, 2, start, 10, "global", 5, 1)
+    np.testing.assert_array_equal(a[0], b[0])
+    np.testing.assert_array_equal(a[2], b[2])
+    with pytest.raises(ValueError):
+        O.skfr_run(bed.ravel(order="F")[:-1], 50, 30, 2, start, 10)
+    with pytest.raises(ValueError):
+        O.skfr_init(bed[:5], 50, 30, 2, 0, 1)
+    with pytest.raises(ValueError):
+        O.skfr_run(bed, 50, 30, 2, start + 2, 10)
