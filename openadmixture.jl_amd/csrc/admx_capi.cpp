@@ -1312,7 +1312,7 @@ struct SkfrView {
   unsigned* nk;
   unsigned long long *changes, *count;
   int32_t* center;
-  unsigned long long *nm, *alt, *ind;
+  unsigned long long *nm, *alt;
   uint32_t* part;
   double *mu, *var, *ukey, *E, *key, *keys;
   int32_t *idx, *sidx;
@@ -1320,23 +1320,23 @@ struct SkfrView {
   double *T, *P, *x2, *dmin, *D2, *out, *Dc;
   void* tmp;
   size_t tmp_bytes;
-  int64_t ngroups, nwords, gper, nslab;
+  int64_t ntiles, nwords, tper, nslab;
 };
 
 // Sizes every SKFR buffer for k clusters, tables of up to ncols_max selected
-// SNPs and nsplit_max table-sum slices, and computes the SNP counts and
+// SNPs and p_elems table-sum partials, and computes the SNP counts and
 // (mu_j, v_j) of the context's genotypes.
-int skfr_prepare(admx_ctx* c, int k, int64_t ncols_max, int nsplit_max, bool need_dc, SkfrView* v) {
+int skfr_prepare(admx_ctx* c, int k, int64_t ncols_max, int64_t p_elems, bool need_dc, SkfrView* v) {
   const int64_t I = c->I, J = c->J;
   const int64_t nb = (I + 3) / 4;
-  v->ngroups = (nb + 15) / 16;
-  v->nwords = 4 * v->ngroups;
-  v->nslab = skfr_slabs(J, v->ngroups, &v->gper);
+  v->ntiles = (nb + 63) / 64;  // 64-byte tiles; the row pitch is a multiple of 64
+  v->nwords = 16 * v->ntiles;
+  v->nslab = skfr_slabs(J, v->ntiles, &v->tper);
   CHK(c->sk_lab.grow(sizeof(int32_t) * I));
-  CHK(c->sk_cm.grow(sizeof(uint32_t) * v->nwords * k));
+  CHK(c->sk_cm.grow(sizeof(uint32_t) * v->nwords * std::max(k - 1, 1)));
   CHK(c->sk_small.grow(256));
-  CHK(c->sk_cnt.grow(sizeof(unsigned long long) * (2 * J + I)));
-  CHK(c->sk_part.grow(sizeof(uint32_t) * v->nslab * 2 * k * J));
+  CHK(c->sk_cnt.grow(sizeof(unsigned long long) * 2 * J));
+  CHK(c->sk_part.grow(sizeof(uint32_t) * v->nslab * 2 * std::max(k - 1, 1) * J));
   CHK(c->sk_stat.grow(sizeof(double) * 3 * J));
   CHK(c->sk_E.grow(sizeof(double) * k * J));
   CHK(c->sk_key.grow(sizeof(double) * k * J));
@@ -1347,7 +1347,7 @@ int skfr_prepare(admx_ctx* c, int k, int64_t ncols_max, int nsplit_max, bool nee
   CHK(c->sk_sel.grow((size_t)J));
   const int64_t trows = std::max<int64_t>(std::max<int64_t>(4 * J, (int64_t)k * J), 4 * k * ncols_max);
   CHK(c->sk_T.grow(sizeof(double) * trows));
-  CHK(c->sk_P.grow(sizeof(double) * (size_t)nsplit_max * k * I));
+  CHK(c->sk_P.grow(sizeof(double) * (size_t)p_elems));
   CHK(c->sk_vec.grow(sizeof(double) * (3 * I + kMaxK + 1)));
   if (need_dc) CHK(c->sk_Dc.grow(sizeof(double) * k * I));
   const size_t tb = skfr_sort_bytes(J);
@@ -1361,7 +1361,6 @@ int skfr_prepare(admx_ctx* c, int k, int64_t ncols_max, int nsplit_max, bool nee
   v->center = reinterpret_cast<int32_t*>(sm + 80);
   v->nm = static_cast<unsigned long long*>(c->sk_cnt.p);
   v->alt = v->nm + J;
-  v->ind = v->alt + J;
   v->part = static_cast<uint32_t*>(c->sk_part.p);
   v->mu = static_cast<double*>(c->sk_stat.p);
   v->var = v->mu + J;
@@ -1383,17 +1382,16 @@ int skfr_prepare(admx_ctx* c, int k, int64_t ncols_max, int nsplit_max, bool nee
   v->tmp = c->sk_tmp.p;
   v->tmp_bytes = c->sk_tmp.n;
   HIPCHK(hipSetDevice(c->device));
-  HIPCHK(hipMemsetAsync(v->ind, 0, sizeof(unsigned long long) * I, c->s));
   {
     ProfScope ps(c, "skfr_stats");
-    launch_counts(c->bed, c->pitch, I, J, v->nm, v->alt, v->ind, c->s);
+    launch_counts_snp(c->bed, c->pitch, I, J, v->nm, v->alt, c->s);
     launch_skfr_stats(v->nm, v->alt, J, v->mu, v->var, c->s);
   }
   return check_launch();
 }
 
-int skfr_split(admx_ctx* c, int nsplit, int64_t ncols) {
-  if (nsplit <= 0) return skfr_nsplit(c->I, ncols);
+int skfr_split(admx_ctx* c, int nsplit, int64_t ncols, int R) {
+  if (nsplit <= 0) return skfr_nsplit(c->I, ncols, R);
   return (int)std::max<int64_t>(1, std::min<int64_t>(nsplit, ncols));
 }
 
@@ -1417,8 +1415,8 @@ int admx_skfr_init(admx_ctx* c, int32_t k, uint64_t seed, int32_t nsplit, int32_
   if (nsplit < 0 || nsplit > 64) return bad("need 0 <= nsplit <= 64");
   if (c->I >= (1LL << 31) || c->J >= (1LL << 31)) return bad("SKFR needs I, J < 2^31");
   SkfrView v;
-  const int ns = skfr_split(c, nsplit, c->J);
-  CHK(skfr_prepare(c, k, 0, ns, true, &v));
+  const int ns = skfr_split(c, nsplit, c->J, 1);
+  CHK(skfr_prepare(c, k, 0, (int64_t)ns * c->I, true, &v));
   {
     ProfScope ps(c, "skfr_kmeanspp");
     for (int cc = 0; cc < k; ++cc) {
@@ -1454,14 +1452,16 @@ int admx_skfr_run(admx_ctx* c, int32_t k, int32_t mode, int64_t sparsity, int32_
   const bool local = mode == ADMX_SKFR_LOCAL;
   const int64_t I = c->I, J = c->J;
   const int64_t ncols_max = local ? std::min<int64_t>(J, (int64_t)k * sparsity) : sparsity;
-  const int ns_max = skfr_split(c, nsplit, J);  // slices are nondecreasing in the column count
+  // slices are nondecreasing in the column count
+  const int64_t p_elems = std::max<int64_t>((int64_t)skfr_split(c, nsplit, J, 1) * I,
+                                            (int64_t)skfr_split(c, nsplit, ncols_max, k) * k * I);
   SkfrView v;
-  CHK(skfr_prepare(c, k, ncols_max, ns_max, false, &v));
+  CHK(skfr_prepare(c, k, ncols_max, p_elems, false, &v));
   HIPCHK(hipMemcpyAsync(v.lab, clusters, sizeof(int32_t) * I, hipMemcpyHostToDevice, c->s));
   {
     // sum_j x_ij^2 of every individual (the WSS term common to all clusters)
     ProfScope ps(c, "skfr_x2");
-    const int ns = skfr_split(c, nsplit, J);
+    const int ns = skfr_split(c, nsplit, J, 1);
     launch_skfr_pp_table(c->bed, c->pitch, nullptr, v.mu, v.var, J, v.T, c->s);
     launch_skfr_tabsum(1, c->bed, c->pitch, I, nullptr, J, ns, v.T, v.P, c->s);
     launch_skfr_combine(v.P, ns, I, v.x2, nullptr, 0, c->s);
@@ -1477,11 +1477,11 @@ int admx_skfr_run(admx_ctx* c, int32_t k, int32_t mode, int64_t sparsity, int32_
     }
     {
       ProfScope ps(c, "skfr_centers");
-      launch_skfr_centers(k, c->bed, c->pitch, J, v.ngroups, v.gper, v.nslab, v.cm, v.part, c->s);
+      launch_skfr_centers(k, c->bed, c->pitch, J, v.ntiles, v.tper, v.nslab, v.cm, v.part, c->s);
     }
     {
       ProfScope ps(c, "skfr_score");
-      launch_skfr_score(k, local, v.part, v.nslab, v.nk, v.mu, v.var, J, v.E, v.key, v.idx, c->s);
+      launch_skfr_score(k, local, v.part, v.nslab, v.nk, v.nm, v.alt, I, v.mu, v.var, J, v.E, v.key, v.idx, c->s);
     }
     {
       ProfScope ps(c, "skfr_select");
@@ -1507,7 +1507,7 @@ int admx_skfr_run(admx_ctx* c, int32_t k, int32_t mode, int64_t sparsity, int32_
       ncols = (int64_t)h;
       if (ncols < 1 || ncols > ncols_max) return fail(ADMX_EHIP, "SKFR: inconsistent union of local selections");
     }
-    const int ns = skfr_split(c, nsplit, ncols);
+    const int ns = skfr_split(c, nsplit, ncols, k);
     HIPCHK(hipMemsetAsync(v.changes, 0, sizeof(unsigned long long), c->s));
     {
       ProfScope ps(c, "skfr_tables");

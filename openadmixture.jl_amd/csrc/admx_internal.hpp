@@ -103,6 +103,9 @@ template <typename T> void launch_accept(int64_t N, void* x, const void* x_tmp, 
 void launch_decode(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int8_t* g, hipStream_t s);
 void launch_counts(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, unsigned long long* snp_nm,
                    unsigned long long* snp_alt, unsigned long long* ind_nm, hipStream_t s);
+// Per-SNP counts only.
+void launch_counts_snp(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, unsigned long long* snp_nm,
+                       unsigned long long* snp_alt, hipStream_t s);
 // Per-SNP mean genotype over the observed entries (mean-imputation loops).
 template <typename T> void launch_snp_mean(const unsigned long long* snp_nm, const unsigned long long* snp_alt,
                                            int64_t J, T* mu, hipStream_t s);
@@ -138,12 +141,13 @@ void launch_probs_to_q(const float* Y, int64_t n, double* q, hipStream_t s);
 // sparse k-means feature ranking (admx_skfr.hip)
 void launch_skfr_masks(const int32_t* lab, int64_t I, int k, int64_t nwords, uint32_t* cm, unsigned* nk,
                        hipStream_t s);
-int64_t skfr_slabs(int64_t J, int64_t ngroups, int64_t* gper);
-void launch_skfr_centers(int k, const uint8_t* bed, int64_t pitch, int64_t J, int64_t ngroups, int64_t gper,
+int64_t skfr_slabs(int64_t J, int64_t ntiles, int64_t* tper);
+void launch_skfr_centers(int k, const uint8_t* bed, int64_t pitch, int64_t J, int64_t ntiles, int64_t tper,
                          int64_t nslab, const uint32_t* cm, uint32_t* part, hipStream_t s);
 void launch_skfr_stats(const unsigned long long* nm, const unsigned long long* alt, int64_t J, double* mu,
                        double* var, hipStream_t s);
-void launch_skfr_score(int k, int local, const uint32_t* part, int64_t nslab, const unsigned* nk, const double* mu,
+void launch_skfr_score(int k, int local, const uint32_t* part, int64_t nslab, const unsigned* nk,
+                       const unsigned long long* nm, const unsigned long long* alt, int64_t I, const double* mu,
                        const double* var, int64_t J, double* E, double* key, int32_t* idx, hipStream_t s);
 size_t skfr_sort_bytes(int64_t n);
 void launch_skfr_sort(void* tmp, size_t bytes, const double* kin, double* kout, const int32_t* vin, int32_t* vout,
@@ -155,7 +159,7 @@ void launch_skfr_tables(const int32_t* cols, int64_t ns, int k, const uint8_t* f
                         const double* mu, const double* var, const double* E, int64_t J, double* T, hipStream_t s);
 void launch_skfr_pp_table(const uint8_t* bed, int64_t pitch, const int32_t* center, const double* mu,
                           const double* var, int64_t J, double* T, hipStream_t s);
-int skfr_nsplit(int64_t I, int64_t ncols);
+int skfr_nsplit(int64_t I, int64_t ncols, int R);
 void launch_skfr_tabsum(int R, const uint8_t* bed, int64_t pitch, int64_t I, const int32_t* cols, int64_t ncols,
                         int nsplit, const double* T, double* P, hipStream_t s);
 void launch_skfr_assign(const double* P, int nsplit, int R, int64_t I, const unsigned* nk, int32_t* lab,

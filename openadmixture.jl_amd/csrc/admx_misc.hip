@@ -429,6 +429,11 @@ __global__ __launch_bounds__(256) void k_counts_ind(const uint8_t* __restrict__ 
   }
 }
 
+void launch_counts_snp(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, unsigned long long* snp_nm,
+                       unsigned long long* snp_alt, hipStream_t s) {
+  hipLaunchKernelGGL(k_counts_snp, dim3((unsigned)((J + 3) / 4)), dim3(256), 0, s, bed, pitch, I, J, snp_nm, snp_alt);
+}
+
 void launch_counts(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, unsigned long long* snp_nm,
                    unsigned long long* snp_alt, unsigned long long* ind_nm, hipStream_t s) {
   hipLaunchKernelGGL(k_counts_snp, dim3((unsigned)((J + 3) / 4)), dim3(256), 0, s, bed, pitch, I, J, snp_nm, snp_alt);

@@ -30,8 +30,9 @@ namespace admx {
 namespace {
 
 // ------------------------------------------------------------------ cluster masks
-// cm[dw * k + c]: bit 2t set iff individual 16 dw + t (< I) is in cluster c
-// (the low-bit positions of the 2-bit codes of BED word dw); nk[c] += |C_c|.
+// cm[dw * (k-1) + c], c < k-1: 0b11 in the 2-bit field of every individual of
+// BED word dw (individuals 16 dw .. 16 dw + 15, < I) that is in cluster c; the
+// last cluster's counts follow from the SNP totals.  nk[c] += |C_c| for all c.
 __global__ __launch_bounds__(256) void k_skfr_masks(const int32_t* __restrict__ lab, int64_t I, int k, int64_t nwords,
                                                     uint32_t* __restrict__ cm, unsigned* __restrict__ nk) {
   const int64_t dw = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -45,7 +46,7 @@ __global__ __launch_bounds__(256) void k_skfr_masks(const int32_t* __restrict__ 
     uint32_t m = 0;
 #pragma unroll
     for (int t = 0; t < 16; ++t) m |= (uint32_t)(lb[t] == c) << (2 * t);
-    if (dw < nwords) cm[dw * k + c] = m;
+    if (dw < nwords && c < k - 1) cm[dw * (k - 1) + c] = m | (m << 1);
     unsigned n = __popc(m);
 #pragma unroll
     for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
@@ -54,47 +55,61 @@ __global__ __launch_bounds__(256) void k_skfr_masks(const int32_t* __restrict__ 
 }
 
 // ------------------------------------------------------------------ centers pass
-// One lane per SNP, a slab of 16-byte groups (64 individuals each) per
-// blockIdx.y.  Per 32-bit word (16 individuals), lo/hi are the code bit
-// planes: g = hi + (hi & lo) over observed entries and missing = lo & ~hi, so
-// per cluster mask m: gs += popc(hi & m) + popc(hi & lo & m), miss += popc(lo &
-// ~hi & m).  The masks are the same for every lane (scalar loads).  Slab
-// partials part[slab][2k][J] (gs rows, then missing rows) are summed exactly
-// by the score kernel.
-template <int KC>
+// One lane per SNP over a slab of 64-byte tiles (256 individuals) of its row,
+// the next tile's four 16-byte loads issued before the current tile is used.
+// Per 32-bit word (16 individuals) with code bit planes lo/hi: the observed
+// genotype is g = hi + (hi & lo) and missing = lo & ~hi, so with
+// comb = hi | (hi & lo) << 1 one popcount of comb & M gives the genotype sum
+// of the individuals in mask M (0b11 per member) and one of missing & M their
+// missing count.  The masks are the same for every lane (scalar loads).
+// Clusters 0 .. KA-1 only (KA = k - 1).  Slab partials part[slab][2 KA][J]
+// (genotype sums, then missing counts) are summed exactly by the score kernel.
+template <int KA>
 __global__ __launch_bounds__(256) void k_skfr_centers(const uint8_t* __restrict__ bed, int64_t pitch, int64_t J,
-                                                      int64_t ngroups, int64_t gper,
-                                                      const uint32_t* __restrict__ cm, uint32_t* __restrict__ part) {
+                                                      int64_t ntiles, int64_t tper, const uint32_t* __restrict__ cm,
+                                                      uint32_t* __restrict__ part) {
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  const int64_t g0 = (int64_t)blockIdx.y * gper;
-  const int64_t g1 = (g0 + gper < ngroups) ? g0 + gper : ngroups;
+  const int64_t t0 = (int64_t)blockIdx.y * tper;
+  const int64_t t1 = (t0 + tper < ntiles) ? t0 + tper : ntiles;
   const int64_t jr = j < J ? j : J - 1;
-  const u32x4* col = reinterpret_cast<const u32x4*>(bed + pitch * jr);
-  uint32_t gs[KC], ms[KC];
+  const u32x4* __restrict__ col = reinterpret_cast<const u32x4*>(bed + pitch * jr);
+  uint32_t gs[KA], ms[KA];
 #pragma unroll
-  for (int c = 0; c < KC; ++c) gs[c] = ms[c] = 0;
-  for (int64_t g = g0; g < g1; ++g) {
-    const u32x4 w4 = col[g];
-    const uint32_t* m = cm + 4 * g * KC;
+  for (int c = 0; c < KA; ++c) gs[c] = ms[c] = 0;
+  u32x4 nx[4];
+  if (t0 < t1) {
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const uint32_t w = w4[q];
-      const uint32_t lo = w & 0x55555555u, hi = (w >> 1) & 0x55555555u;
-      const uint32_t hl = hi & lo, mi = lo ^ hl;
+    for (int e = 0; e < 4; ++e) nx[e] = col[4 * t0 + e];
+  }
+  for (int64_t t = t0; t < t1; ++t) {
+    u32x4 cur[4];
 #pragma unroll
-      for (int c = 0; c < KC; ++c) {
-        const uint32_t mk = m[q * KC + c];
-        gs[c] += __popc(hi & mk) + __popc(hl & mk);
-        ms[c] += __popc(mi & mk);
+    for (int e = 0; e < 4; ++e) cur[e] = nx[e];
+    if (t + 1 < t1) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) nx[e] = col[4 * (t + 1) + e];
+    }
+    const uint32_t* __restrict__ m = cm + 16 * t * KA;
+#pragma unroll
+    for (int w = 0; w < 16; ++w) {
+      const uint32_t x = cur[w >> 2][w & 3];
+      const uint32_t lo = x & 0x55555555u, hi = (x >> 1) & 0x55555555u;
+      const uint32_t hl = hi & lo;
+      const uint32_t comb = hi | (hl << 1), mi = lo ^ hl;
+#pragma unroll
+      for (int c = 0; c < KA; ++c) {
+        const uint32_t M = m[w * KA + c];
+        gs[c] += __popc(comb & M);
+        ms[c] += __popc(mi & M);
       }
     }
   }
   if (j < J) {
-    uint32_t* o = part + (int64_t)blockIdx.y * 2 * KC * J + j;
+    uint32_t* o = part + (int64_t)blockIdx.y * 2 * KA * J + j;
 #pragma unroll
-    for (int c = 0; c < KC; ++c) {
+    for (int c = 0; c < KA; ++c) {
       o[c * J] = gs[c];
-      o[(KC + c) * J] = ms[c];
+      o[(KA + c) * J] = ms[c];
     }
   }
 }
@@ -111,24 +126,37 @@ __global__ void k_skfr_stats(const unsigned long long* __restrict__ nm, const un
 }
 
 // E[c][j] = e_cj and the sort keys: global key[j] = (sum_c e^2/n_c) / v_j;
-// local key[c][j] = (e^2/n_c) / v_j (0 for v_j = 0 or an empty cluster).
+// local key[c][j] = (e^2/n_c) / v_j (0 for v_j = 0 or an empty cluster).  The
+// last cluster's genotype sum and missing count are the SNP's totals minus
+// those of the other clusters.
 template <int KC>
 __global__ __launch_bounds__(256) void k_skfr_score(const uint32_t* __restrict__ part, int64_t nslab,
                                                     const unsigned* __restrict__ nk,
+                                                    const unsigned long long* __restrict__ nm,
+                                                    const unsigned long long* __restrict__ alt, int64_t I,
                                                     const double* __restrict__ mu_j,
                                                     const double* __restrict__ var_j, int64_t J, int local,
                                                     double* __restrict__ E, double* __restrict__ key,
                                                     int32_t* __restrict__ idx) {
+  constexpr int KA = KC - 1;
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (j >= J) return;
   const double mu = mu_j[j], v = var_j[j];
+  uint32_t gsum = 0, msum = 0;
   double acc = 0.0;
 #pragma unroll
   for (int c = 0; c < KC; ++c) {
     uint32_t gs = 0, ms = 0;
-    for (int64_t sl = 0; sl < nslab; ++sl) {
-      gs += part[(sl * 2 * KC + c) * J + j];
-      ms += part[(sl * 2 * KC + KC + c) * J + j];
+    if (c < KA) {
+      for (int64_t sl = 0; sl < nslab; ++sl) {
+        gs += part[(sl * 2 * KA + c) * J + j];
+        ms += part[(sl * 2 * KA + KA + c) * J + j];
+      }
+      gsum += gs;
+      msum += ms;
+    } else {
+      gs = (uint32_t)alt[j] - gsum;
+      ms = (uint32_t)((uint64_t)I - nm[j]) - msum;
     }
     const unsigned n = nk[c];
     const double ob = (double)(n - ms);
@@ -219,48 +247,70 @@ __global__ void k_skfr_pp_table(const uint8_t* __restrict__ bed, int64_t pitch, 
 }
 
 // Table sums over a column list: P[sl][r][i] = sum_{t in slice sl} T[t][r][code(i, cols[t])]
-// in increasing t from 0.0.  One lane per BED byte (4 individuals), slice
-// sl = blockIdx.y covers t in [sl L, min((sl+1) L, ncols)); cols == nullptr
-// means column t.
-template <int R>
+// in increasing t from 0.0; slice sl = blockIdx.y covers t in
+// [sl L, min((sl+1) L, ncols)); cols == nullptr means column t.  A lane owns B
+// bytes (4 B individuals) of every column.  The block stages the tables of
+// CT columns at a time in LDS, each lane issues the loads of its bytes of all
+// CT columns at once, then every (individual, row) adds one LDS entry indexed
+// by the 2-bit code.
+template <int R, int B>
 __global__ __launch_bounds__(256) void k_skfr_tabsum(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I,
                                                      int64_t nb, const int32_t* __restrict__ cols, int64_t ncols,
                                                      int64_t L, const double* __restrict__ T,
                                                      double* __restrict__ P) {
-  const int64_t b = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  constexpr int CT = 32;
+  constexpr int NI = 4 * B;
+  __shared__ double tl[CT * R * 4];
+  __shared__ int64_t off[CT];
+  const int64_t b0 = ((int64_t)blockIdx.x * 256 + threadIdx.x) * B;
+  const bool live = b0 < nb;
   const int64_t sl = blockIdx.y;
   const int64_t t0 = sl * L;
   const int64_t t1 = (t0 + L < ncols) ? t0 + L : ncols;
-  if (b >= nb) return;
-  double acc[R][4];
+  double acc[R][NI];
 #pragma unroll
   for (int r = 0; r < R; ++r)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) acc[r][q] = 0.0;
-  uint32_t nxt = 0;
-  if (t0 < t1) nxt = bed[pitch * (int64_t)(cols ? cols[t0] : t0) + b];
-  for (int64_t t = t0; t < t1; ++t) {
-    const uint32_t by = nxt;
-    if (t + 1 < t1) nxt = bed[pitch * (int64_t)(cols ? cols[t + 1] : t + 1) + b];
-    const double* tr = T + t * R * 4;
+    for (int e = 0; e < NI; ++e) acc[r][e] = 0.0;
+  for (int64_t tc = t0; tc < t1; tc += CT) {
+    const int n = (int)((t1 - tc < CT) ? t1 - tc : CT);
+    __syncthreads();
+    for (int e = threadIdx.x; e < n * R * 4; e += 256) tl[e] = T[tc * R * 4 + e];
+    for (int e = threadIdx.x; e < n; e += 256) off[e] = pitch * (int64_t)(cols ? cols[tc + e] : tc + e);
+    __syncthreads();
+    if (live) {
+      uint32_t by[CT];
 #pragma unroll
-    for (int r = 0; r < R; ++r) {
-      const double v0 = tr[r * 4 + 0], v1 = tr[r * 4 + 1], v2 = tr[r * 4 + 2], v3 = tr[r * 4 + 3];
+      for (int tt = 0; tt < CT; ++tt) {
+        by[tt] = 0;
+        if (tt < n) {
+          const uint8_t* src = bed + off[tt] + b0;
+          if constexpr (B == 1) by[tt] = *src;
+          if constexpr (B == 2) by[tt] = *reinterpret_cast<const uint16_t*>(src);
+          if constexpr (B == 4) by[tt] = *reinterpret_cast<const uint32_t*>(src);
+        }
+      }
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const uint32_t code = (by >> (2 * q)) & 3u;
-        const double lo = (code & 1u) ? v1 : v0;
-        const double hi = (code & 1u) ? v3 : v2;
-        acc[r][q] += (code & 2u) ? hi : lo;
+      for (int tt = 0; tt < CT; ++tt) {
+        if (tt < n) {
+          const double* row = tl + tt * R * 4;
+#pragma unroll
+          for (int e = 0; e < NI; ++e) {
+            const uint32_t code = (by[tt] >> (2 * e)) & 3u;
+#pragma unroll
+            for (int r = 0; r < R; ++r) acc[r][e] += row[r * 4 + code];
+          }
+        }
       }
     }
   }
+  if (!live) return;
 #pragma unroll
   for (int r = 0; r < R; ++r)
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
-      const int64_t i = 4 * b + q;
-      if (i < I) P[(sl * R + r) * I + i] = acc[r][q];
+    for (int e = 0; e < NI; ++e) {
+      const int64_t i = 4 * b0 + e;
+      if (i < I) P[(sl * R + r) * I + i] = acc[r][e];
     }
 }
 
@@ -435,19 +485,20 @@ void launch_skfr_masks(const int32_t* lab, int64_t I, int k, int64_t nwords, uin
   hipLaunchKernelGGL(k_skfr_masks, dim3(nblk(nwords)), dim3(256), 0, s, lab, I, k, nwords, cm, nk);
 }
 
-int64_t skfr_slabs(int64_t J, int64_t ngroups, int64_t* gper) {
-  const int64_t lanes_blocks = (J + 255) / 256;
-  int64_t ns = (2048 + lanes_blocks - 1) / lanes_blocks;  // about 8192 waves
-  if (ns > ngroups) ns = ngroups;
+int64_t skfr_slabs(int64_t J, int64_t ntiles, int64_t* tper) {
+  const int64_t blocks = (J + 255) / 256;
+  int64_t ns = (4096 + blocks - 1) / blocks;  // about 16384 waves
+  if (ns > ntiles) ns = ntiles;
   if (ns < 1) ns = 1;
-  *gper = (ngroups + ns - 1) / ns;
-  return (ngroups + *gper - 1) / *gper;
+  *tper = (ntiles + ns - 1) / ns;
+  return (ntiles + *tper - 1) / *tper;
 }
 
-void launch_skfr_centers(int k, const uint8_t* bed, int64_t pitch, int64_t J, int64_t ngroups, int64_t gper,
+void launch_skfr_centers(int k, const uint8_t* bed, int64_t pitch, int64_t J, int64_t ntiles, int64_t tper,
                          int64_t nslab, const uint32_t* cm, uint32_t* part, hipStream_t s) {
+  if (k < 2) return;  // one cluster: the SNP totals are its counts
   const dim3 grid(nblk(J), (unsigned)nslab);
-  ADMX_DISPATCH_K(k, (k_skfr_centers<KK><<<grid, dim3(256), 0, s>>>(bed, pitch, J, ngroups, gper, cm, part)));
+  ADMX_DISPATCH_K(k - 1, (k_skfr_centers<KK><<<grid, dim3(256), 0, s>>>(bed, pitch, J, ntiles, tper, cm, part)));
 }
 
 void launch_skfr_stats(const unsigned long long* nm, const unsigned long long* alt, int64_t J, double* mu,
@@ -455,10 +506,11 @@ void launch_skfr_stats(const unsigned long long* nm, const unsigned long long* a
   hipLaunchKernelGGL(k_skfr_stats, dim3(nblk(J)), dim3(256), 0, s, nm, alt, J, mu, var);
 }
 
-void launch_skfr_score(int k, int local, const uint32_t* part, int64_t nslab, const unsigned* nk, const double* mu,
+void launch_skfr_score(int k, int local, const uint32_t* part, int64_t nslab, const unsigned* nk,
+                       const unsigned long long* nm, const unsigned long long* alt, int64_t I, const double* mu,
                        const double* var, int64_t J, double* E, double* key, int32_t* idx, hipStream_t s) {
-  ADMX_DISPATCH_K(k, (k_skfr_score<KK><<<dim3(nblk(J)), dim3(256), 0, s>>>(part, nslab, nk, mu, var, J, local, E,
-                                                                            key, idx)));
+  ADMX_DISPATCH_K(k, (k_skfr_score<KK><<<dim3(nblk(J)), dim3(256), 0, s>>>(part, nslab, nk, nm, alt, I, mu, var, J,
+                                                                            local, E, key, idx)));
 }
 
 size_t skfr_sort_bytes(int64_t n) {
@@ -493,8 +545,14 @@ void launch_skfr_pp_table(const uint8_t* bed, int64_t pitch, const int32_t* cent
   hipLaunchKernelGGL(k_skfr_pp_table, dim3(nblk(J)), dim3(256), 0, s, bed, pitch, center, mu, var, J, T);
 }
 
-int skfr_nsplit(int64_t I, int64_t ncols) {
-  const int64_t blocks = ((I + 3) / 4 + 255) / 256;
+// Bytes per lane of the table-sum kernel: 16 individuals for one row, 8 for
+// up to four rows, 4 beyond (at most 64 FP64 accumulators per lane).
+static int tabsum_bytes(int R) { return R == 1 ? 4 : (R <= 4 ? 2 : 1); }
+
+int skfr_nsplit(int64_t I, int64_t ncols, int R) {
+  const int64_t nb = (I + 3) / 4;
+  const int64_t lanes = (nb + tabsum_bytes(R) - 1) / tabsum_bytes(R);
+  const int64_t blocks = (lanes + 255) / 256;
   int64_t n = (1024 + blocks - 1) / blocks;  // about 4096 waves
   if (n > 64) n = 64;
   if (n > ncols) n = ncols;
@@ -502,12 +560,20 @@ int skfr_nsplit(int64_t I, int64_t ncols) {
   return (int)n;
 }
 
-void launch_skfr_tabsum(int R, const uint8_t* bed, int64_t pitch, int64_t I, const int32_t* cols, int64_t ncols,
-                        int nsplit, const double* T, double* P, hipStream_t s) {
+template <int R>
+static void tabsum_r(const uint8_t* bed, int64_t pitch, int64_t I, const int32_t* cols, int64_t ncols, int nsplit,
+                     const double* T, double* P, hipStream_t s) {
   const int64_t nb = (I + 3) / 4;
   const int64_t L = (ncols + nsplit - 1) / nsplit;
-  const dim3 grid(nblk(nb), (unsigned)nsplit);
-  ADMX_DISPATCH_K(R, (k_skfr_tabsum<KK><<<grid, dim3(256), 0, s>>>(bed, pitch, I, nb, cols, ncols, L, T, P)));
+  constexpr int B = R == 1 ? 4 : (R <= 4 ? 2 : 1);
+  const int64_t lanes = (nb + B - 1) / B;
+  const dim3 grid(nblk(lanes), (unsigned)nsplit);
+  hipLaunchKernelGGL((k_skfr_tabsum<R, B>), grid, dim3(256), 0, s, bed, pitch, I, nb, cols, ncols, L, T, P);
+}
+
+void launch_skfr_tabsum(int R, const uint8_t* bed, int64_t pitch, int64_t I, const int32_t* cols, int64_t ncols,
+                        int nsplit, const double* T, double* P, hipStream_t s) {
+  ADMX_DISPATCH_K(R, (tabsum_r<KK>(bed, pitch, I, cols, ncols, nsplit, T, P, s)));
 }
 
 void launch_skfr_assign(const double* P, int nsplit, int R, int64_t I, const unsigned* nk, int32_t* lab,

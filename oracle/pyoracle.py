@@ -357,7 +357,7 @@ class OracleAdmix:
 def _declare_skfr(L):
     L.orc_skfr_u.argtypes = [C.c_uint64, C.c_int]
     L.orc_skfr_u.restype = c_dbl
-    L.orc_skfr_nsplit.argtypes = [c_i64, c_i64, C.c_int]
+    L.orc_skfr_nsplit.argtypes = [c_i64, c_i64, C.c_int, C.c_int]
     L.orc_skfr_nsplit.restype = C.c_int
     L.orc_skfr_stats.argtypes = [c_vp, c_i64, c_i64, c_i64, c_vp, c_vp]
     L.orc_skfr_init.argtypes = [c_vp, c_i64, c_i64, c_i64, C.c_int, C.c_uint64, C.c_int, c_vp, c_vp]
@@ -392,8 +392,8 @@ def skfr_u(seed: int, cc: int) -> float:
     return _skfr_lib().orc_skfr_u(C.c_uint64(seed), cc)
 
 
-def skfr_nsplit(I: int, ncols: int, nsplit: int) -> int:
-    return _skfr_lib().orc_skfr_nsplit(I, ncols, nsplit)
+def skfr_nsplit(I: int, ncols: int, nsplit: int, R: int = 1) -> int:
+    return _skfr_lib().orc_skfr_nsplit(I, ncols, nsplit, R)
 
 
 def skfr_stats(bed, I: int, J: int):

@@ -51,13 +51,16 @@ double orc_skfr_u(uint64_t seed, int cc) {
 }
 
 /* slices of the distance sums: requested nsplit clamped to [1, ncols], or the
- * automatic count for nsplit = 0 (about 4096 waves of 256-individual blocks) */
-int orc_skfr_nsplit(int64_t I, int64_t ncols, int nsplit) {
+ * automatic count for nsplit = 0 (about 4096 waves; a lane of the device
+ * kernel covers 16 individuals for R = 1 row, 8 for R <= 4, else 4) */
+int orc_skfr_nsplit(int64_t I, int64_t ncols, int nsplit, int R) {
   int64_t n;
   if (nsplit > 0) {
     n = nsplit;
   } else {
-    const int64_t blocks = ((I + 3) / 4 + 255) / 256;
+    const int64_t per = R == 1 ? 4 : (R <= 4 ? 2 : 1);
+    const int64_t lanes = ((I + 3) / 4 + per - 1) / per;
+    const int64_t blocks = (lanes + 255) / 256;
     n = (1024 + blocks - 1) / blocks;
     if (n > 64) n = 64;
   }
@@ -189,7 +192,7 @@ int orc_skfr_init(const uint8_t *bed, int64_t pitch, int64_t I, int64_t J, int k
   double *mu = malloc(sizeof(double) * J), *var = malloc(sizeof(double) * J);
   double *T = malloc(sizeof(double) * 4 * J), *Dc = malloc(sizeof(double) * k * I), *D2 = malloc(sizeof(double) * I);
   orc_skfr_stats(bed, pitch, I, J, mu, var);
-  const int ns = orc_skfr_nsplit(I, J, nsplit);
+  const int ns = orc_skfr_nsplit(I, J, nsplit, 1);
   for (int cc = 0; cc < k; ++cc) {
     const int64_t ci = sk_pick(D2, I, orc_skfr_u(seed, cc), cc == 0);
     if (centers_out) centers_out[cc] = ci;
@@ -250,7 +253,7 @@ int orc_skfr_run(const uint8_t *bed, int64_t pitch, int64_t I, int64_t J, int k,
   unsigned nk[SK_MAXK];
   orc_skfr_stats(bed, pitch, I, J, mu, var);
   sk_pp_table(bed, pitch, -1, mu, var, J, T);
-  sk_tabsum(bed, pitch, I, NULL, J, orc_skfr_nsplit(I, J, nsplit), 1, T, x2);
+  sk_tabsum(bed, pitch, I, NULL, J, orc_skfr_nsplit(I, J, nsplit, 1), 1, T, x2);
   int it;
   int64_t ncols = 0;
   for (it = 1; it <= max_iter; ++it) {
@@ -321,7 +324,7 @@ int orc_skfr_run(const uint8_t *bed, int64_t pitch, int64_t I, int64_t J, int k,
         o[3] = a * (b - 4.0);
       }
     }
-    sk_tabsum(bed, pitch, I, cols, ncols, orc_skfr_nsplit(I, ncols, nsplit), k, T, D);
+    sk_tabsum(bed, pitch, I, cols, ncols, orc_skfr_nsplit(I, ncols, nsplit, k), k, T, D);
     int64_t changes = 0;
     for (int64_t i = 0; i < I; ++i) {
       int best = -1;
