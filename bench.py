@@ -22,6 +22,10 @@ cohorts simulated from this workload's populations (reported separately).  The C
 EM step and per iteration (bounded sample, scaled to all SNPs) by the plain
 run's step counts.
 
+The "skfr" object times the SKFR AIM pre-selection (sparse k-means with
+feature ranking, k = K clusters, J/10 AIMs) on the same resident BED, with
+the roofline of its centers pass and the single-threaded oracle beside it.
+
 Prints one JSON line.  --gpus above 1 is not supported (one GPU per box).
 """
 from __future__ import annotations
