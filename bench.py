@@ -199,7 +199,7 @@ def run_sweep(args):
     t0 = time.perf_counter()
     res = oa.cv_sweep(src, Ks, nfolds=args.folds, fold_seed=args.seed, init_seed=args.seed + 1,
                       max_iter=args.sweep_max_iter, precision="f64" if args.form == "plain" else args.form,
-                      log=lambda m: print(m, file=sys.stderr, flush=True))
+                      fold_start=args.fold_start, log=lambda m: print(m, file=sys.stderr, flush=True))
     el = time.perf_counter() - t0
     src.close()
     out = {
@@ -208,7 +208,8 @@ def run_sweep(args):
         "dtype": "f64" if args.form == "plain" else f"{args.form} passes, f64 LL/QP",
         "data": "synthetic cohort generated on device (Dirichlet Q, Uniform P, Binomial(2, q.p) genotypes)",
         "config": {"workload": f"{args.workload}: I={I}, J={J}, true K={Kt}, K={Ks[0]}..{Ks[-1]}, {args.folds}-fold CV",
-                   "I": I, "J": J, "K_true": Kt, "folds": args.folds, "max_iter": args.sweep_max_iter},
+                   "I": I, "J": J, "K_true": Kt, "folds": args.folds, "max_iter": args.sweep_max_iter,
+                   "fold_start": args.fold_start},
         "best_k": oa.best_k(res),
         "per_k": [{k: r[k] for k in ("K", "seconds", "cv_error", "ll_full", "iterations_full", "converged_full",
                                      "iterations_folds", "waves")} for r in res],
@@ -409,6 +410,8 @@ def main():
     ap.add_argument("--folds", type=int, default=5, help="CV folds of the K sweep (workload c4)")
     ap.add_argument("--sweep-ks", default="", help="comma-separated K values of the sweep (default 2..12)")
     ap.add_argument("--sweep-max-iter", type=int, default=300)
+    ap.add_argument("--fold-start", default="full", choices=["full", "random"],
+                    help="CV fold fits start from the full fit's estimates or from their own random start")
     ap.add_argument("--train-epochs", type=int, default=20, help="network training epochs (workload c5)")
     ap.add_argument("--K", type=int, default=0, help="override the workload's K (throughput runs)")
     ap.add_argument("--skfr-iters", type=int, default=10, help="SKFR iterations of the AIM pre-selection block")

@@ -14,7 +14,11 @@ error of K is the held-out binomial deviance per observed held-out entry,
 which is ADMIXTURE's deviance-residual CV criterion.  One context per K holds
 that K's state; the fold genotypes are re-derived on the device from the
 source context's BED (``admx_bed_from``), so the cohort is read from the host
-once.
+once.  By default each fold fit starts from the full fit's estimates
+(``fold_start="full"``): a fold hides only 1/nfolds of the entries, so its
+optimum lies close to the full one and the fit needs a few iterations instead
+of a random start's EM priming and full descent; ``fold_start="random"`` gives
+every fold its own seeded random start and EM priming.
 """
 from __future__ import annotations
 
@@ -39,9 +43,18 @@ def fit_on_device(ctx: HIPAdmixData, K: int, seed: int, em_iters: int = 5, max_i
     return d
 
 
+def fit_from(ctx: HIPAdmixData, x, seed: int, max_iter: int = 1000, rtol: float = 1e-7, Q: int = 3):
+    """admixture_qn! on ctx's genotypes from the state x (K x (I+J)), without
+    EM priming; returns the host AdmixData."""
+    d = AdmixData(ctx.I, ctx.J, ctx.K, Q, rng=np.random.default_rng(seed), T=ctx.T)
+    d.x[...] = x
+    admixture_qn(d, None, max_iter, rtol, d_cu=ctx, log=None)
+    return d
+
+
 def cv_sweep(src: HIPAdmixData, Ks, nfolds: int = 5, fold_seed: int = 0, init_seed: int = 1, em_iters: int = 5,
              max_iter: int = 1000, rtol: float = 1e-7, precision: str = "f64", Q: int = 3, autotune: bool = True,
-             log=None):
+             fold_start: str = "full", log=None):
     """Fit K in Ks on src's genotypes, each with nfolds holdout problems.
 
     Returns one dict per K: K, seconds (all nfolds + 1 fits of that K), the full
@@ -49,6 +62,8 @@ def cv_sweep(src: HIPAdmixData, Ks, nfolds: int = 5, fold_seed: int = 0, init_se
     and the chosen waves per pass."""
     if nfolds < 2:
         raise ValueError("nfolds must be >= 2")
+    if fold_start not in ("full", "random"):
+        raise ValueError("fold_start must be 'full' or 'random'")
     out = []
     for K in Ks:
         t0 = time.perf_counter()
@@ -64,7 +79,10 @@ def cv_sweep(src: HIPAdmixData, Ks, nfolds: int = 5, fold_seed: int = 0, init_se
             devs, cnts, iters = [], [], []
             for f in range(nfolds):
                 ctx.bed_from(src, f, nfolds, fold_seed)
-                d = fit_on_device(ctx, K, init_seed + 1000 * K + 1 + f, em_iters, max_iter, rtol, Q)
+                if fold_start == "full":
+                    d = fit_from(ctx, full.x, init_seed + 1000 * K + 1 + f, max_iter, rtol, Q)
+                else:
+                    d = fit_on_device(ctx, K, init_seed + 1000 * K + 1 + f, em_iters, max_iter, rtol, Q)
                 dev, cnt = ctx.heldout_deviance(src, f, nfolds, fold_seed)
                 devs.append(dev)
                 cnts.append(cnt)
@@ -89,4 +107,4 @@ def best_k(results):
     return min(results, key=lambda r: r["cv_error"])["K"]
 
 
-__all__ = ["cv_sweep", "fit_on_device", "best_k"]
+__all__ = ["cv_sweep", "fit_on_device", "fit_from", "best_k"]

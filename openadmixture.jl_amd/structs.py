@@ -4,7 +4,11 @@ Layout is the reference's: x is K x (I+J) column-major, q = x[:, :I] and
 p = x[:, I:] are views of it (src/structs.jl:120-137), XtX is K x K x n, U and
 V are K(I+J) x Q.  Initialisation follows src/structs.jl:105-215 with numpy's
 Generator in place of Julia's RNG: x ~ U(0,1), columns of q normalised to
-sum 1, XtX/Xtz/U/V filled with U(0,1) draws, log-likelihoods NaN.
+sum 1, log-likelihoods NaN.  The reference also fills XtX/Xtz/U/V with U(0,1)
+draws; every use overwrites them first (update_q!/update_p! write XtX/Xtz,
+admixture_qn! reads only the U/V columns it has written,
+src/algorithms_outer.jl:125-133), so here they start zeroed, which allocates
+lazily instead of drawing ~10 K^2 (I + J) random numbers per fit.
 """
 from __future__ import annotations
 
@@ -24,12 +28,12 @@ class AdmixData:
         self.x_tmp = np.zeros((K, I + J), T, order="F")
         q = self.x[:, :I]
         q /= q.sum(axis=0, keepdims=True)
-        self.XtX_q = np.asfortranarray(rng.random((K, K, I)).astype(T))
-        self.Xtz_q = np.asfortranarray(rng.random((K, I)).astype(T))
-        self.XtX_p = np.asfortranarray(rng.random((K, K, J)).astype(T))
-        self.Xtz_p = np.asfortranarray(rng.random((K, J)).astype(T))
-        self.V = np.asfortranarray(rng.random((K * (I + J), Q)).astype(T))
-        self.U = np.asfortranarray(rng.random((K * (I + J), Q)).astype(T))
+        self.XtX_q = np.zeros((K, K, I), T, order="F")
+        self.Xtz_q = np.zeros((K, I), T, order="F")
+        self.XtX_p = np.zeros((K, K, J), T, order="F")
+        self.Xtz_p = np.zeros((K, J), T, order="F")
+        self.V = np.zeros((K * (I + J), Q), T, order="F")
+        self.U = np.zeros((K * (I + J), Q), T, order="F")
         self.ll_prev = float("nan")
         self.ll_new = float("nan")
 

@@ -188,16 +188,19 @@ def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_i
     """admixture_qn! from the network's proposal (with the EM judge/fallback).
 
     Returns dict(history=[(seconds, LL)], chosen='warm'|'em', ll_warm, ll_em,
-    iterations).  history[0] is the starting point; seconds count from the
-    start of the call (features, inference, P refinement and the judge
-    included).  With stop_at = (LL*, tol) the run ends at the first iteration
-    within tol of LL*."""
+    iterations).  history[0] is the starting point; seconds count from after
+    the host AdmixData allocation (features, inference, P refinement and the
+    judge included), the same origin as the plain run's timing.  With
+    stop_at = (LL*, tol) the run ends at the first iteration within tol of
+    LL*."""
     if net.out_dim != ctx.K:
         raise ValueError("network output must equal the context's K")
     I, J, K = ctx.I, ctx.J, ctx.K
+    # host state (random QN history and the judge's random start) is allocated
+    # before the clock starts, as for the plain run's AdmixData
+    d = AdmixData(I, J, K, Q, rng=np.random.default_rng(seed), T=ctx.T)
     ctx.synchronize()
     t0 = time.perf_counter()
-    d = AdmixData(I, J, K, Q, rng=np.random.default_rng(seed), T=ctx.T)
     panel.features(ctx, to_host=False)
     ctx.warm_q(net)
     snm, salt, _ = ctx.counts()

@@ -80,6 +80,26 @@ def test_cv_sweep_recovers_true_k(oa, O):
     src.close()
 
 
+def test_folds_from_full_fit_match_random_starts(oa, O):
+    """Fold fits started from the full fit's estimates reach the same CV
+    errors (to the convergence tolerance) as fold fits from their own random
+    starts, in fewer iterations."""
+    print("cv fold starts", flush=True)
+    I, J, Ktrue = 600, 4000, 3
+    src, *_ = _src(oa, O, I, J, Ktrue, seed=21, alpha=0.5)
+    warm = oa.cv_sweep(src, [2, 3, 4], nfolds=4, fold_seed=5, max_iter=300, autotune=False, fold_start="full")
+    cold = oa.cv_sweep(src, [2, 3, 4], nfolds=4, fold_seed=5, max_iter=300, autotune=False, fold_start="random")
+    assert oa.best_k(warm) == oa.best_k(cold) == Ktrue
+    for w, c in zip(warm, cold):
+        print(w["K"], w["cv_error"], c["cv_error"], w["iterations_folds"], c["iterations_folds"], flush=True)
+        assert sum(w["iterations_folds"]) < sum(c["iterations_folds"])
+        if w["K"] == Ktrue:  # a well-identified K: both reach the same optimum
+            assert abs(w["cv_error"] - c["cv_error"]) <= 1e-4 * c["cv_error"]
+    with pytest.raises(ValueError):
+        oa.cv_sweep(src, [2], fold_start="other")
+    src.close()
+
+
 def test_autotune_picks_a_candidate(oa, O):
     src, *_ = _src(oa, O, 900, 3000, 3, seed=2)
     c = oa.HIPAdmixData(900, 3000, 3, 3)
