@@ -475,7 +475,7 @@ int mixed_update(admx_ctx* c, bool qside, const double* q, const double* p, cons
   CHK(check_launch());
   FinArgs f{c->part.p, qside ? I : J, n, 0, J, x_in, x_out, nullptr,
             c->keep_hess ? (qside ? c->XtX_q : c->XtX_p) : nullptr, c->keep_hess ? (qside ? c->Xtz_q : c->Xtz_p) : nullptr,
-            0, 1, status_dev};
+            0, 1, status_dev, qside ? c->XtX_q : c->XtX_p, qside ? c->Xtz_q : c->Xtz_p};
   {
     ProfScope ps(c, qside ? "fin_q_update_qp" : "fin_p_update_qp");
     if (qside) launch_fin_q_update_mixed((int)K, f, c->s);
@@ -506,7 +506,7 @@ int dev_update_q(admx_ctx* c, bool update2, int32_t* status_host) {
   CHK(pass_q<T>(c, MODE_UPDATE, q, p, 0, c->I, 0, c->J, &nchunk, &np));
   FinArgs f{c->part.p, c->I, nchunk, 0, c->J, q, q_next, nullptr,
             c->keep_hess ? c->XtX_q : nullptr, c->keep_hess ? c->Xtz_q : nullptr, 0, 1,
-            status_host ? c->status : nullptr};
+            status_host ? c->status : nullptr, c->XtX_q, c->Xtz_q};
   {
     ProfScope ps(c, "fin_q_update_qp");
     launch_fin_q_update<T>((int)c->K, f, true, c->s);
@@ -540,7 +540,7 @@ int dev_update_p(admx_ctx* c, bool update2, int32_t* status_host) {
   CHK(pass_p<T>(c, MODE_UPDATE, q, p, 0, c->I, 0, c->J, &nchunk));
   FinArgs f{c->part.p, c->J, nchunk, 0, c->J, p, p_next, nullptr,
             c->keep_hess ? c->XtX_p : nullptr, c->keep_hess ? c->Xtz_p : nullptr, 0, 1,
-            status_host ? c->status : nullptr};
+            status_host ? c->status : nullptr, c->XtX_p, c->Xtz_p};
   {
     ProfScope ps(c, "fin_p_update_qp");
     launch_fin_p_update<T>((int)c->K, f, true, c->s);

@@ -42,6 +42,10 @@ struct FinArgs {
   int accumulate;   // loop API: outputs += partial sums
   int neg_xtz;      // store -Xtz (update_q!/update_p! negate Xtz before the QP)
   int32_t* status;  // QP return codes (nullable)
+  // K x K x ncols / K x ncols buffers the split reduce-then-QP path may use
+  // when XtX / Xtz are null (the context's own Hessian arrays)
+  void* XtX_scratch = nullptr;
+  void* Xtz_scratch = nullptr;
 };
 
 // Number of per-column accumulators of a pass.

@@ -406,6 +406,75 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_qp_batch(int64_t n, const T* 
   for (int k = 0; k < K; ++k) delta[(int64_t)K * col + k] = QP_DEL(k);
 }
 
+// QP + step + projection of the reduced Hessians: the second half of the
+// finalize when the QP tableau's LDS footprint would leave a fused reduce + QP
+// kernel at one or two waves per CU.  Reads XtX (full K x K) and Xtz as the
+// reduction stored them (negated when neg_xtz), so the tableau, and every
+// result, equals the fused kernel's bit for bit.
+template <typename T, int K, int C>
+__global__ __launch_bounds__(QPLanes<K>::N) void k_qp_apply(FinArgs f, const T* __restrict__ X,
+                                                            const T* __restrict__ Z) {
+  constexpr int NT = QPLanes<K>::N;
+  using LY = QPLayout<T, K, C, NT>;
+  __shared__ T lds[LY::TOTAL * NT];
+  const int t = threadIdx.x;
+  const int64_t col = (int64_t)blockIdx.x * NT + t;
+  if (col >= f.n) return;
+  const int64_t g = f.off + col;
+  const T* __restrict__ xin = static_cast<const T*>(f.x_in) + (int64_t)K * g;
+  T Hf[K * K], xtz[K], par[K];
+#pragma unroll
+  for (int e = 0; e < K * K; ++e) Hf[e] = X[(int64_t)K * K * g + e];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const T z = Z[(int64_t)K * g + k];
+    xtz[k] = f.neg_xtz ? -z : z;
+    par[k] = xin[k];
+  }
+  T* L = lds;
+  const int rc = qp_from_hessian<T, K, C, NT>(L, t, Hf, xtz, par);
+  if (f.status) f.status[g] = rc;
+  T* xo = static_cast<T*>(f.x_out) + (int64_t)K * g;
+  if constexpr (C == 1) {
+    T v[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) v[k] = par[k] + QP_DEL(k);
+    project_simplex<T, K>(v);
+#pragma unroll
+    for (int k = 0; k < K; ++k) xo[k] = v[k];
+  } else {
+#pragma unroll
+    for (int k = 0; k < K; ++k) xo[k] = project_box<T>(par[k] + QP_DEL(k));
+  }
+}
+
+// Split the finalize into a reduction kernel and k_qp_apply when the fused
+// kernel's tableau takes more than 80 KiB of LDS per 64 problems, i.e. when
+// only one fused workgroup fits a CU (I = J = 100,000, K = 12: fin_q 12.1 ->
+// 2.4 ms, fin_p 10 -> 1.3 ms; at K = 8, two workgroups per CU, the fused
+// kernel is faster: 0.73 vs 0.86 ms).
+template <typename T, int K, int C>
+constexpr bool qp_split() {
+  return (size_t)QPLayout<T, K, C, QPLanes<K>::N>::TOTAL * QPLanes<K>::N * sizeof(T) > 81920;
+}
+
+// Launch the split path if it applies; false leaves the fused kernel to the caller.
+template <typename T, int K, int C, typename RED>
+static bool fin_split(const FinArgs& f, hipStream_t s, RED reduce_kernel) {
+  if (!qp_split<T, K, C>() || f.accumulate) return false;
+  void* X = f.XtX ? f.XtX : f.XtX_scratch;
+  void* Z = f.Xtz ? f.Xtz : f.Xtz_scratch;
+  if (!X || !Z) return false;
+  constexpr int NT = QPLanes<K>::N;
+  FinArgs r = f;
+  r.XtX = X;
+  r.Xtz = Z;
+  hipLaunchKernelGGL(reduce_kernel, dim3((unsigned)((f.n + NT - 1) / NT)), dim3(NT), 0, s, r);
+  hipLaunchKernelGGL((k_qp_apply<T, K, C>), dim3((unsigned)((f.n + NT - 1) / NT)), dim3(NT), 0, s, r,
+                     static_cast<const T*>(X), static_cast<const T*>(Z));
+  return true;
+}
+
 template <typename T, int K>
 __global__ __launch_bounds__(256) void k_project_q(int64_t n, T* __restrict__ b) {
   const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
@@ -431,12 +500,14 @@ template <typename T, int K>
 static void fin_q_update_k(const FinArgs& f, bool do_qp, hipStream_t s) {
   constexpr bool trick = sizeof(T) == 8;
   constexpr int NT = QPLanes<K>::N;
+  if (do_qp && fin_split<T, K, 1>(f, s, k_fin_q_update<T, K, trick, false>)) return;
   if (do_qp) hipLaunchKernelGGL((k_fin_q_update<T, K, trick, true>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
   else hipLaunchKernelGGL((k_fin_q_update<T, K, trick, false>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
 }
 template <typename T, int K>
 static void fin_p_update_k(const FinArgs& f, bool do_qp, hipStream_t s) {
   constexpr int NT = QPLanes<K>::N;
+  if (do_qp && fin_split<T, K, 0>(f, s, k_fin_p_update<T, K, false>)) return;
   if (do_qp) hipLaunchKernelGGL((k_fin_p_update<T, K, true>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
   else hipLaunchKernelGGL((k_fin_p_update<T, K, false>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
 }
@@ -506,11 +577,13 @@ ADMX_INST(ADMX_T)
 template <int K>
 static void fin_q_mixed_k(const FinArgs& f, hipStream_t s) {
   constexpr int NT = QPLanes<K>::N;
+  if (fin_split<double, K, 1>(f, s, k_fin_q_update<double, K, false, false, float>)) return;
   hipLaunchKernelGGL((k_fin_q_update<double, K, false, true, float>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
 }
 template <int K>
 static void fin_p_mixed_k(const FinArgs& f, hipStream_t s) {
   constexpr int NT = QPLanes<K>::N;
+  if (fin_split<double, K, 0>(f, s, k_fin_p_update<double, K, false, float>)) return;
   hipLaunchKernelGGL((k_fin_p_update<double, K, true, float>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
 }
 void launch_fin_q_update_mixed(int K, const FinArgs& f, hipStream_t s) { ADMX_DISPATCH_K(K, (fin_q_mixed_k<KK>(f, s))); }

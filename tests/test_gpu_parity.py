@@ -229,6 +229,29 @@ def test_zal_step(oa, O, it, Q):
     c.close()
 
 
+@pytest.mark.parametrize("K", [5, 9, 10, 12])
+def test_update_steps_without_kept_hessians(oa, O, K):
+    """With ADMX_TUNE_KEEP_HESSIANS = 0 the split finalize (reduction, then QP)
+    keeps its Hessians in the context's own arrays: the steps and QP codes are
+    the same bit for bit."""
+    note("keep_hess", K)
+    I, J = 500, 1500
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, seed=K)
+    q, p = random_state(I, J, K, seed=K)
+    c = ctx_for(oa, bed, pitch, I, J, K)
+    x = np.asfortranarray(np.hstack([q, p]))
+    out = []
+    for keep in (1, 0):
+        c.set_tuning(oa._native.ADMX_TUNE_KEEP_HESSIANS, keep)
+        c.set_x(0, x)
+        sq = c.update_q(False, status=True)
+        sp = c.update_p(False, status=True)
+        out.append((c.get_x(1), sq, sp))
+    assert np.array_equal(out[0][0], out[1][0])
+    assert np.array_equal(out[0][1], out[1][1]) and np.array_equal(out[0][2], out[1][2])
+    c.close()
+
+
 # ------------------------------------------------------------------ algorithms on device state
 def _pair(oa, O, I, J, K, Q=3, T=F64, seed=0, miss=0.01, beta=False):
     Qt, Pt, bed, pitch = make_cohort(O, I, J, K, seed=seed, miss=miss, beta=beta)
@@ -238,9 +261,12 @@ def _pair(oa, O, I, J, K, Q=3, T=F64, seed=0, miss=0.01, beta=False):
     return d, c, ref
 
 
-def test_update_q_p_device(oa, O):
-    note("update_q!/update_p!")
-    I, J, K = 600, 2000, 4
+@pytest.mark.parametrize("K", [4, 8, 10, 12])
+def test_update_q_p_device(oa, O, K):
+    # K >= 9 (Q side) / 10 (P side) runs the finalize as a reduction kernel
+    # followed by the separate QP kernel
+    note("update_q!/update_p!", K)
+    I, J = 600, 2000
     d, c, ref = _pair(oa, O, I, J, K)
     ref.init_em(2)
     d.x[...] = ref.x
