@@ -61,11 +61,14 @@ extern "C" {
 #define ADMX_QN_ZAL 0
 #define ADMX_QN_LBQN 1
 
-/* values of ADMX_TUNE_PRECISION.  Log-likelihood, QP, projections, quasi-Newton
- * and the convergence test stay FP64 in every mode. */
+/* values of ADMX_TUNE_PRECISION.  QP, projections, quasi-Newton, the
+ * log-likelihood reduction and the convergence test stay FP64 in every mode. */
 #define ADMX_PREC_F64 0  /* plain: everything FP64 */
 #define ADMX_PREC_F32 1  /* gradient + Hessian accumulated in FP32 (vector ALU) */
-#define ADMX_PREC_BF16 2 /* gradient in FP32, Hessian on BF16 matrix cores with FP32 accumulation */
+#define ADMX_PREC_BF16 2 /* gradient in FP32, Hessian on BF16 matrix cores with FP32 accumulation; the
+                          * device state's log-likelihood takes s = q.p and 1 - s = q.(1-p) from the matrix
+                          * cores too (three-term BF16 splits, FP32 accumulation, relative error a few 1e-7),
+                          * with the per-genotype products and the reduction in FP64 */
 
 typedef struct admx_ctx admx_ctx;
 

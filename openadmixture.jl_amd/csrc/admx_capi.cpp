@@ -95,6 +95,7 @@ struct admx_ctx {
   int keep_hess = 1;
   int precision = ADMX_PREC_F64;
   DevBuf fq, fp, ppt, mmt, qqt;  // mixed-precision pass inputs
+  DevBuf lqt, lpt, lmt;           // BF16 split tables of the matrix-core log-likelihood
   DevBuf feat, featpart;          // warm-start features (I x 64) and their chunk partials
   // J-chunk streaming: the BED lives in pinned host memory and every pass
   // streams it through two device chunk buffers on a copy stream.
@@ -395,9 +396,43 @@ int pass_p(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
   return ADMX_OK;
 }
 
+// BF16 form: the log-likelihood of the whole cohort with s and t from the
+// matrix cores (split tables of q and p built per call).
+int ll_mixed(admx_ctx* c, const double* q, const double* p, int slot) {
+  const int64_t I = c->I, J = c->J, K = c->K;
+  const int KD = split_table_cols((int)K);
+  const int64_t rows_q = (I + 255) / 256 * 256, rows_p = J + 32;
+  CHK(c->lqt.grow(sizeof(uint16_t) * KD * rows_q));
+  CHK(c->lpt.grow(sizeof(uint16_t) * KD * rows_p));
+  CHK(c->lmt.grow(sizeof(uint16_t) * KD * rows_p));
+  const int64_t nbx = (I + 255) / 256;
+  int64_t nchunk = std::max<int64_t>(1, (c->target_waves / 4 + nbx - 1) / nbx);
+  nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, J / 256));
+  const int64_t jchunk = ((J + nchunk - 1) / nchunk + 31) / 32 * 32;
+  nchunk = (J + jchunk - 1) / jchunk;
+  CHK(c->llpart.grow(sizeof(double) * nbx * nchunk));
+  {
+    ProfScope ps(c, "qside_ll_mfma");
+    launch_split_tables((int)K, q, I, rows_q, 0, c->lqt.p, nullptr, c->s);
+    launch_split_tables((int)K, p, J, rows_p, 1, c->lpt.p, c->lmt.p, c->s);
+    launch_ll_mfma((int)K, c->bed, c->pitch, I, J, jchunk, (int)nchunk, c->lqt.p, c->lpt.p, c->lmt.p,
+                   static_cast<double*>(c->llpart.p), c->s);
+  }
+  CHK(check_launch());
+  ProfScope ps(c, "ll_reduce");
+  launch_reduce_sum(static_cast<const double*>(c->llpart.p), nbx * nchunk, c->lls + slot, 0, c->s);
+  return check_launch();
+}
+
 // LL of (q, p) over a range into device slot lls[slot] (accumulate optional).
+// The BF16 form takes the matrix-core path for the device state's full-cohort
+// LL (allow_mixed); the tile-loop entry point always runs the FP64/FP32 loop.
 template <typename T>
-int ll_to_slot(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0, int64_t j1, int slot) {
+int ll_to_slot(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0, int64_t j1, int slot,
+               bool allow_mixed = true) {
+  if (allow_mixed && sizeof(T) == 8 && c->precision == ADMX_PREC_BF16 && !c->impute && !c->stream_snps && i0 == 0 &&
+      i1 == c->I && j0 == 0 && j1 == c->J)
+    return ll_mixed(c, static_cast<const double*>(q), static_cast<const double*>(p), slot);
   int nchunk = 0;
   int64_t np = 0;
   CHK(pass_q<T>(c, MODE_LL, q, p, i0, i1, j0, j1, &nchunk, &np));
@@ -751,7 +786,7 @@ int loop_ll(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1, i
   CHK(stage_in(c, 0, q, qb, qb));
   CHK(stage_in(c, 1, p, pb, pb));
   if (i1 > i0 && j1 > j0) {
-    CHK(ll_to_slot<T>(c, c->st[0].p, c->st[1].p, i0, i1, j0, j1, LL_TMP));
+    CHK(ll_to_slot<T>(c, c->st[0].p, c->st[1].p, i0, i1, j0, j1, LL_TMP, false));
     HIPCHK(hipMemcpyAsync(out, c->lls + LL_TMP, sizeof(double), hipMemcpyDeviceToHost, c->s));
   } else {
     *out = 0.0;
@@ -1153,6 +1188,9 @@ int admx_destroy(admx_ctx* c) {
   c->ppt.release();
   c->mmt.release();
   c->qqt.release();
+  c->lqt.release();
+  c->lpt.release();
+  c->lmt.release();
   c->feat.release();
   c->featpart.release();
   c->snp_mu.release();

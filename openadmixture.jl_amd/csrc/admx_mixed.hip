@@ -377,6 +377,205 @@ template <int K>
 static void pair_tables_k(const double* x, int64_t n, int64_t ld, __bf16* PP, __bf16* MM, hipStream_t s) {
   hipLaunchKernelGGL((k_pair_tables<K>), dim3((unsigned)((ld + 255) / 256)), dim3(256), 0, s, x, n, ld, PP, MM);
 }
+// ------------------------------------------------------------------ log-likelihood terms on the matrix cores
+// s = q.p and t = q.(1-p) for a 32-SNP x 32-individual tile are two products
+// of BF16 matrices: every FP64 value v is split into three BF16 terms
+// v1 + v2 + v3 (v1 = bf16(v), v2 = bf16(v - v1), v3 = bf16(v - v1 - v2), a
+// residual of about 2^-24 |v|), and the six products of order < 3,
+//   s = q1 p1 + q1 p2 + q2 p1 + q1 p3 + q2 p2 + q3 p1,
+// stack along the MFMA's reduction dimension: row j of the SNP table holds
+// [p1 p2 p1 p3 p2 p1] (K values each), row i of the individual table
+// [q1 q1 q2 q1 q2 q3], zero-padded to KD = 16 ceil(6K / 16); a second SNP table
+// holds the same for m = 1 - p, so t is accurate relative to itself even where
+// s is close to 1.  The dropped products and the FP32 accumulation leave s and
+// t with relative errors of a few 1e-7.
+template <int K> struct SplitDim {
+  static constexpr int KD = ((6 * K + 15) / 16) * 16;
+  static constexpr int NM = KD / 16;
+};
+
+__device__ __forceinline__ void split3(double v, __bf16& h1, __bf16& h2, __bf16& h3) {
+  h1 = (__bf16)(float)v;
+  const double r1 = v - (double)(float)h1;
+  h2 = (__bf16)(float)r1;
+  const double r2 = r1 - (double)(float)h2;
+  h3 = (__bf16)(float)r2;
+}
+
+// Individual table (role 0) or SNP tables (role 1: T1 from p, T2 from 1 - p),
+// `rows` rows of KD BF16 each, zero past column n.
+template <int K>
+__global__ __launch_bounds__(256) void k_split_tables(const double* __restrict__ x, int64_t n, int64_t rows, int role,
+                                                      __bf16* __restrict__ T1, __bf16* __restrict__ T2) {
+  constexpr int KD = SplitDim<K>::KD;
+  const int64_t c = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (c >= rows) return;
+  // order of the three terms in the six product slots
+  constexpr int QO[6] = {0, 0, 1, 0, 1, 2};
+  constexpr int PO[6] = {0, 1, 0, 2, 1, 0};
+  __bf16 o1[KD], o2[KD];
+#pragma unroll
+  for (int e = 0; e < KD; ++e) o1[e] = o2[e] = (__bf16)0.0f;
+  if (c < n) {
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      const double v = x[(int64_t)K * c + k];
+      __bf16 h[3], g[3];
+      split3(v, h[0], h[1], h[2]);
+      split3(1.0 - v, g[0], g[1], g[2]);
+#pragma unroll
+      for (int slot = 0; slot < 6; ++slot) {
+        o1[slot * K + k] = role ? h[PO[slot]] : h[QO[slot]];
+        o2[slot * K + k] = g[PO[slot]];
+      }
+    }
+  }
+  u32x4* d1 = reinterpret_cast<u32x4*>(T1 + c * KD);
+#pragma unroll
+  for (int e = 0; e < KD / 8; ++e) {
+    u32x4 v;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) v[w] = __builtin_bit_cast(uint32_t, (bf16x2){o1[8 * e + 2 * w], o1[8 * e + 2 * w + 1]});
+    d1[e] = v;
+  }
+  if (role) {
+    u32x4* d2 = reinterpret_cast<u32x4*>(T2 + c * KD);
+#pragma unroll
+    for (int e = 0; e < KD / 8; ++e) {
+      u32x4 v;
+#pragma unroll
+      for (int w = 0; w < 4; ++w)
+        v[w] = __builtin_bit_cast(uint32_t, (bf16x2){o2[8 * e + 2 * w], o2[8 * e + 2 * w + 1]});
+      d2[e] = v;
+    }
+  }
+}
+
+// Skipmissing log-likelihood with matrix-core s and t.  A wave owns 64
+// individuals (two 32-column tiles) and walks its SNP chunk in tiles of 32:
+//   D[j][i] = sum_k A[j][k] B[k][i] on v_mfma_f32_32x32x16_bf16, A = SNP table
+//   rows (lane j + 32 (k/8)), B = individual table rows (lane i + 32 (k/8)),
+//   D[j][i] in lane i + 32 ((j/4) % 2), register (j % 4) + 4 (j / 8);
+// so lane l holds, for individual l % 32 of each tile, the 16 SNPs
+// 8a + 4 (l/32) + b (register 4a + b).  The tile's genotype bytes (32 rows x
+// 16 bytes) come with one 8-byte load per lane and are transposed through
+// 512 bytes of LDS into [byte][row], from which a lane reads the four SNPs
+// 8a + 4 (l/32) .. + 3 of its individual as one dword.  Per genotype the lane
+// forms s^g t^(2-g) in FP32 (1 if missing) and multiplies it into an FP64
+// product renormalised every 8 genotypes: one log per lane at the end.
+template <int K>
+__global__ __launch_bounds__(256) void k_ll_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int64_t J,
+                                                 int64_t jchunk, const __bf16* __restrict__ QT,
+                                                 const __bf16* __restrict__ PT, const __bf16* __restrict__ MT,
+                                                 double* __restrict__ part) {
+  constexpr int KD = SplitDim<K>::KD, NM = SplitDim<K>::NM;
+  __shared__ __align__(16) uint8_t tbuf[4][512];
+  __shared__ double sh_red[256];
+  uint8_t* __restrict__ tb = tbuf[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63, h = lane >> 5, cn = lane & 31;
+  const int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
+  const int64_t jb = (int64_t)blockIdx.y * jchunk;
+  const int64_t je = (jb + jchunk < J) ? jb + jchunk : J;
+
+  bf16x8 bq[2][NM];
+#pragma unroll
+  for (int T = 0; T < 2; ++T)
+#pragma unroll
+    for (int m = 0; m < NM; ++m) bq[T][m] = load_b8(QT, KD, (int)0, (i0 + 32 * T + cn) * KD + 16 * m + 8 * h);
+  const bool ok0 = i0 + cn < I, ok1 = i0 + 32 + cn < I;
+  const uint32_t shift = 2u * (uint32_t)(cn & 3);
+  const int bi0 = cn >> 2, bi1 = 8 + (cn >> 2);
+
+  const uint8_t* __restrict__ gbase = bed + (i0 >> 2) + 8 * (lane & 1);
+  auto gload = [&](int64_t j0) {
+    const int64_t j = j0 + (lane >> 1);
+    uint2 w = {0x55555555u, 0x55555555u};
+    if (j < je) w = *reinterpret_cast<const uint2*>(gbase + pitch * j);
+    return w;
+  };
+  double prod = 1.0;
+  int esum = 0;
+  uint2 wn = {0x55555555u, 0x55555555u};
+  if (jb < je) wn = gload(jb);
+  for (int64_t js = jb; js < je; js += 32) {
+    bf16x8 as[NM], at[NM];
+#pragma unroll
+    for (int m = 0; m < NM; ++m) {
+      as[m] = load_b8(PT, KD, (int)0, (js + cn) * KD + 16 * m + 8 * h);
+      at[m] = load_b8(MT, KD, (int)0, (js + cn) * KD + 16 * m + 8 * h);
+    }
+    // bytes e of this lane's 8 -> tb[byte 8 (l%2) + e][row l/2]
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+      tb[(8 * (lane & 1) + e) * 32 + (lane >> 1)] = (uint8_t)(wn.x >> (8 * e));
+      tb[(8 * (lane & 1) + 4 + e) * 32 + (lane >> 1)] = (uint8_t)(wn.y >> (8 * e));
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    uint32_t rd[2][4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      rd[0][a] = *reinterpret_cast<const uint32_t*>(tb + bi0 * 32 + 8 * a + 4 * h);
+      rd[1][a] = *reinterpret_cast<const uint32_t*>(tb + bi1 * 32 + 8 * a + 4 * h);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (js + 32 < je) wn = gload(js + 32);
+#pragma unroll
+    for (int T = 0; T < 2; ++T) {
+      f32x16 sa = (f32x16){}, ta = (f32x16){};
+#pragma unroll
+      for (int m = 0; m < NM; ++m) {
+        sa = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as[m], bq[T][m], sa, 0, 0, 0);
+        ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(at[m], bq[T][m], ta, 0, 0, 0);
+      }
+      const bool ok = T ? ok1 : ok0;
+#pragma unroll
+      for (int a = 0; a < 4; ++a) {
+#pragma unroll
+        for (int b = 0; b < 4; ++b) {
+          const uint32_t code = (rd[T][a] >> (8 * b + shift)) & 3u;
+          const float sv = sa[4 * a + b], tv = ta[4 * a + b];
+          const float x = (code >= 2u) ? sv : tv;
+          const float y = (code == 3u) ? sv : tv;
+          float f = x * y;
+          f = (code == 1u || !ok) ? 1.0f : f;
+          prod *= (double)f;
+        }
+        if (a & 1) {
+          int e;
+          prod = __builtin_frexp(prod, &e);
+          esum += e;
+        }
+      }
+    }
+  }
+  const double acc = log(prod) + (double)esum * 0.69314718055994530942;
+  const double tot = block_sum<256>(acc, sh_red);
+  if (threadIdx.x == 0) part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = tot;
+}
+
+template <int K>
+static void split_tables_k(const double* x, int64_t n, int64_t rows, int role, __bf16* T1, __bf16* T2, hipStream_t s) {
+  hipLaunchKernelGGL((k_split_tables<K>), dim3((unsigned)((rows + 255) / 256)), dim3(256), 0, s, x, n, rows, role, T1,
+                     T2);
+}
+void launch_split_tables(int K, const double* x, int64_t n, int64_t rows, int role, void* T1, void* T2,
+                         hipStream_t s) {
+  ADMX_DISPATCH_K(K, (split_tables_k<KK>(x, n, rows, role, (__bf16*)T1, (__bf16*)T2, s)));
+}
+int split_table_cols(int K) { return ((6 * K + 15) / 16) * 16; }
+
+template <int K>
+static void ll_mfma_k(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
+                      const __bf16* QT, const __bf16* PT, const __bf16* MT, double* part, hipStream_t s) {
+  const dim3 grid((unsigned)((I + 255) / 256), (unsigned)nchunk);
+  hipLaunchKernelGGL((k_ll_mfma<K>), grid, dim3(256), 0, s, bed, pitch, I, J, jchunk, QT, PT, MT, part);
+}
+void launch_ll_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
+                    const void* QT, const void* PT, const void* MT, double* part, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (ll_mfma_k<KK>(bed, pitch, I, J, jchunk, nchunk, (const __bf16*)QT, (const __bf16*)PT,
+                                    (const __bf16*)MT, part, s)));
+}
+
 void launch_pair_tables(int K, const double* x, int64_t n, int64_t ld, void* PP, void* MM, hipStream_t s) {
   ADMX_DISPATCH_K(K, (pair_tables_k<KK>(x, n, ld, (__bf16*)PP, (__bf16*)MM, s)));
 }

@@ -74,3 +74,34 @@ def test_mixed_converges_to_f64_optimum(oa, O, form):
     print(f"{form}: dLL {abs(ll - llf) / abs(llf):.2e}, max|dQ| {dq:.2e}, max|dP| {dp:.2e}, "
           f"iterations {res[form][3]} vs {res['f64'][3]}", flush=True)
     assert abs(ll - llf) <= 1e-6 * abs(llf)
+
+
+@pytest.mark.parametrize("I,J,K", [(1, 1, 1), (37, 45, 3), (1500, 4100, 8), (700, 333, 12), (300, 260, 16)])
+def test_matrix_core_loglikelihood(oa, O, I, J, K):
+    """BF16 form: the device state's LL with s and 1 - s from the matrix cores
+    (three-term BF16 splits) against the FP64 oracle.  Each term's s and t
+    carry a relative error of a few 1e-7 (FP32-level), so the bar is 1e-6
+    relative to the LL; over many genotypes the errors average out (a
+    6-million-genotype cohort lands near 1e-10).  The tile-loop entry point
+    stays on the FP64 loop (1e-10)."""
+    print("mfma ll", I, J, K, flush=True)
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, alpha=0.3, miss=0.03, seed=I + K, p_lo=0.01, p_hi=0.99, beta=True)
+    rng = np.random.default_rng(K)
+    q = rng.dirichlet(0.5 * np.ones(K), size=I).T
+    p = rng.uniform(1e-5, 1 - 1e-5, size=(K, J))
+    p[:, : J // 4] = np.clip(1 - 10 ** rng.uniform(-5, -2, size=(K, J // 4)), 1e-5, 1 - 1e-5)  # s close to 1
+    q, p = np.asfortranarray(q), np.asfortranarray(p)
+    c = oa.HIPAdmixData(I, J, K, 3)
+    c.upload_bed(bed.reshape((pitch, J), order="F"))
+    c.set_precision("bf16")
+    c.set_x(0, np.asfortranarray(np.hstack([q, p])))
+    got = c.loglikelihood(0)
+    ref = O.loglikelihood(bed, pitch, q, p)
+    rel = abs(got - ref) / abs(ref)
+    print("relative error", rel, flush=True)
+    assert rel <= 1e-6, (got, ref, rel)
+    loop = c.loglikelihood_loop(q, p, 0, I, 0, J)
+    assert abs(loop - ref) <= 1e-10 * abs(ref)
+    c.set_precision("f64")
+    assert abs(c.loglikelihood(0) - ref) <= 1e-10 * abs(ref)
+    c.close()
