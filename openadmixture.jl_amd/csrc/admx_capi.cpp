@@ -415,7 +415,7 @@ int ll_mixed(admx_ctx* c, const double* q, const double* p, int slot) {
     ProfScope ps(c, "qside_ll_mfma");
     launch_split_tables((int)K, q, I, rows_q, 0, c->lqt.p, nullptr, c->s);
     launch_split_tables((int)K, p, J, rows_p, 1, c->lpt.p, c->lmt.p, c->s);
-    launch_ll_mfma((int)K, c->bed, c->pitch, I, J, jchunk, (int)nchunk, c->lqt.p, c->lpt.p, c->lmt.p,
+    launch_ll_mfma((int)K, c->bed, c->pitch, I, J, jchunk, (int)nchunk, q, c->lqt.p, c->lpt.p, c->lmt.p,
                    static_cast<double*>(c->llpart.p), c->s);
   }
   CHK(check_launch());

@@ -81,15 +81,15 @@ void launch_qside_bf16(int K, const PassArgs& a, int nchunk, const float* fq, co
                        const void* MMt, int64_t ld, hipStream_t s);
 void launch_pside_bf16(int K, const PassArgs& a, int nchunk, const float* fq, const float* fp, const void* QQt,
                        int64_t ld, hipStream_t s);
-// Log-likelihood with s = q.p and t = q.(1-p) on the matrix cores (three-term
-// BF16 splits, FP32 accumulation), products and reduction in FP64: tables of
+// Log-likelihood with s = q.p and t = q.(1-p) + (1 - sum q) on the matrix cores
+// (three-term BF16 splits, FP32 accumulation), products and reduction in FP64: tables of
 // split_table_cols(K) BF16 per row (individuals: role 0; SNPs: role 1, T2 from
 // 1 - p), then one partial per (256-individual block, SNP chunk).
 int split_table_cols(int K);
 void launch_split_tables(int K, const double* x, int64_t n, int64_t rows, int role, void* T1, void* T2,
                          hipStream_t s);
 void launch_ll_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
-                    const void* QT, const void* PT, const void* MT, double* part, hipStream_t s);
+                    const double* q, const void* QT, const void* PT, const void* MT, double* part, hipStream_t s);
 
 // Batched QP on host-provided tableau inputs (XtX, negated Xtz, x): delta out.
 template <typename T> void launch_qp_batch(int K, bool simplex, int64_t n, const void* XtX, const void* Xtz_neg,

@@ -387,8 +387,9 @@ static void pair_tables_k(const double* x, int64_t n, int64_t ld, __bf16* PP, __
 // [p1 p2 p1 p3 p2 p1] (K values each), row i of the individual table
 // [q1 q1 q2 q1 q2 q3], zero-padded to KD = 16 ceil(6K / 16); a second SNP table
 // holds the same for m = 1 - p, so t is accurate relative to itself even where
-// s is close to 1.  The dropped products and the FP32 accumulation leave s and
-// t with relative errors of a few 1e-7.
+// s is close to 1; t then takes 1 - sum_k q_ik on top, so it is 1 - s also when
+// q's column does not sum to 1.  The dropped products and the FP32
+// accumulation leave s and t with relative errors of a few 1e-7.
 template <int K> struct SplitDim {
   static constexpr int KD = ((6 * K + 15) / 16) * 16;
   static constexpr int NM = KD / 16;
@@ -400,6 +401,17 @@ __device__ __forceinline__ void split3(double v, __bf16& h1, __bf16& h2, __bf16&
   h2 = (__bf16)(float)r1;
   const double r2 = r1 - (double)(float)h2;
   h3 = (__bf16)(float)r2;
+}
+
+// 1 - sum_k q_ik in FP64, rounded to FP32: q.(1-p) + (1 - sum q) = 1 - q.p is the
+// reference's t also where the columns of q do not sum to 1 (after EM steps,
+// whose q sums to the observed fraction, src/algorithms_inner.jl:76-89).
+template <int K>
+__device__ __forceinline__ float sum_complement(const double* __restrict__ q, int64_t i) {
+  double sq = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) sq += q[(int64_t)K * i + k];
+  return (float)(1.0 - sq);
 }
 
 // Individual table (role 0) or SNP tables (role 1: T1 from p, T2 from 1 - p),
@@ -465,9 +477,9 @@ __global__ __launch_bounds__(256) void k_split_tables(const double* __restrict__
 // product renormalised every 8 genotypes: one log per lane at the end.
 template <int K>
 __global__ __launch_bounds__(256) void k_ll_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int64_t J,
-                                                 int64_t jchunk, const __bf16* __restrict__ QT,
-                                                 const __bf16* __restrict__ PT, const __bf16* __restrict__ MT,
-                                                 double* __restrict__ part) {
+                                                 int64_t jchunk, const double* __restrict__ q64,
+                                                 const __bf16* __restrict__ QT, const __bf16* __restrict__ PT,
+                                                 const __bf16* __restrict__ MT, double* __restrict__ part) {
   constexpr int KD = SplitDim<K>::KD, NM = SplitDim<K>::NM;
   __shared__ __align__(16) uint8_t tbuf[4][512];
   __shared__ double sh_red[256];
@@ -485,6 +497,8 @@ __global__ __launch_bounds__(256) void k_ll_mfma(const uint8_t* __restrict__ bed
   const bool ok0 = i0 + cn < I, ok1 = i0 + 32 + cn < I;
   const uint32_t shift = 2u * (uint32_t)(cn & 3);
   const int bi0 = cn >> 2, bi1 = 8 + (cn >> 2);
+  const float corr0 = ok0 ? sum_complement<K>(q64, i0 + cn) : 0.f;
+  const float corr1 = ok1 ? sum_complement<K>(q64, i0 + 32 + cn) : 0.f;
 
   const uint8_t* __restrict__ gbase = bed + (i0 >> 2) + 8 * (lane & 1);
   auto gload = [&](int64_t j0) {
@@ -533,7 +547,7 @@ __global__ __launch_bounds__(256) void k_ll_mfma(const uint8_t* __restrict__ bed
 #pragma unroll
         for (int b = 0; b < 4; ++b) {
           const uint32_t code = (rd[T][a] >> (8 * b + shift)) & 3u;
-          const float sv = sa[4 * a + b], tv = ta[4 * a + b];
+          const float sv = sa[4 * a + b], tv = ta[4 * a + b] + (T ? corr1 : corr0);
           const float x = (code >= 2u) ? sv : tv;
           const float y = (code == 3u) ? sv : tv;
           float f = x * y;
@@ -566,13 +580,14 @@ int split_table_cols(int K) { return ((6 * K + 15) / 16) * 16; }
 
 template <int K>
 static void ll_mfma_k(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
-                      const __bf16* QT, const __bf16* PT, const __bf16* MT, double* part, hipStream_t s) {
+                      const double* q, const __bf16* QT, const __bf16* PT, const __bf16* MT, double* part,
+                      hipStream_t s) {
   const dim3 grid((unsigned)((I + 255) / 256), (unsigned)nchunk);
-  hipLaunchKernelGGL((k_ll_mfma<K>), grid, dim3(256), 0, s, bed, pitch, I, J, jchunk, QT, PT, MT, part);
+  hipLaunchKernelGGL((k_ll_mfma<K>), grid, dim3(256), 0, s, bed, pitch, I, J, jchunk, q, QT, PT, MT, part);
 }
 void launch_ll_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
-                    const void* QT, const void* PT, const void* MT, double* part, hipStream_t s) {
-  ADMX_DISPATCH_K(K, (ll_mfma_k<KK>(bed, pitch, I, J, jchunk, nchunk, (const __bf16*)QT, (const __bf16*)PT,
+                    const double* q, const void* QT, const void* PT, const void* MT, double* part, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (ll_mfma_k<KK>(bed, pitch, I, J, jchunk, nchunk, q, (const __bf16*)QT, (const __bf16*)PT,
                                     (const __bf16*)MT, part, s)));
 }
 

@@ -76,18 +76,20 @@ def test_mixed_converges_to_f64_optimum(oa, O, form):
     assert abs(ll - llf) <= 1e-6 * abs(llf)
 
 
+@pytest.mark.parametrize("qsum", [1.0, 0.97])
 @pytest.mark.parametrize("I,J,K", [(1, 1, 1), (37, 45, 3), (1500, 4100, 8), (700, 333, 12), (300, 260, 16)])
-def test_matrix_core_loglikelihood(oa, O, I, J, K):
+def test_matrix_core_loglikelihood(oa, O, I, J, K, qsum):
     """BF16 form: the device state's LL with s and 1 - s from the matrix cores
     (three-term BF16 splits) against the FP64 oracle.  Each term's s and t
     carry a relative error of a few 1e-7 (FP32-level), so the bar is 1e-6
     relative to the LL; over many genotypes the errors average out (a
-    6-million-genotype cohort lands near 1e-10).  The tile-loop entry point
-    stays on the FP64 loop (1e-10)."""
-    print("mfma ll", I, J, K, flush=True)
+    6-million-genotype cohort lands near 1e-10).  Columns of q summing to
+    0.97 (as after EM steps with missing entries) keep t = 1 - s.  The
+    tile-loop entry point stays on the FP64 loop (1e-10)."""
+    print("mfma ll", I, J, K, qsum, flush=True)
     Qt, Pt, bed, pitch = make_cohort(O, I, J, K, alpha=0.3, miss=0.03, seed=I + K, p_lo=0.01, p_hi=0.99, beta=True)
     rng = np.random.default_rng(K)
-    q = rng.dirichlet(0.5 * np.ones(K), size=I).T
+    q = rng.dirichlet(0.5 * np.ones(K), size=I).T * qsum
     p = rng.uniform(1e-5, 1 - 1e-5, size=(K, J))
     p[:, : J // 4] = np.clip(1 - 10 ** rng.uniform(-5, -2, size=(K, J // 4)), 1e-5, 1 - 1e-5)  # s close to 1
     q, p = np.asfortranarray(q), np.asfortranarray(p)
