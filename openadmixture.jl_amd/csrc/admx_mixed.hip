@@ -214,7 +214,7 @@ __global__ __launch_bounds__(256) void k_qside_bf16(PassArgs a, const float* __r
         for (int k = 1; k < K; ++k) s = __builtin_fmaf(qi[k], pv[k], s);
         const float t1 = 1.f - s;
         const float gm = byte_f(gb[(u + h) >> 2], (u + h) & 3), hm = byte_f(hb[(u + h) >> 2], (u + h) & 3);
-        const float r = rcp(s * t1);
+        const float r = __builtin_amdgcn_rcpf(s * t1);  // 1 ulp: the weights are FP32-grade
         const float is = t1 * r, it = s * r;
         const float cc = gm * is, dd = hm * it;
         av[h] = cc * is;
@@ -333,7 +333,7 @@ __global__ __launch_bounds__(256) void k_pside_bf16(PassArgs a, const float* __r
         for (int k = 1; k < K; ++k) s = __builtin_fmaf(qq[k], pj[k], s);
         const float t1 = 1.f - s;
         const float gm = byte_f(gb[v & 3], v >> 2), hm = byte_f(hb[v & 3], v >> 2);
-        const float r = rcp(s * t1);
+        const float r = __builtin_amdgcn_rcpf(s * t1);  // 1 ulp: the weights are FP32-grade
         const float is = t1 * r, it = s * r;
         const float cc = gm * is, dd = hm * it;
         wt[h] = cc * is + dd * it;
