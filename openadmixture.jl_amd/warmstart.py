@@ -208,7 +208,9 @@ def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_i
     x = ctx.get_x(N.ADMX_X)
     x[:, I:] = f[None, :]
     ctx.set_x(N.ADMX_X, x)
-    ctx.set_uv(d.U, d.V)
+    # admixture_qn! iterations from iter = 1 write every quasi-Newton history
+    # column before reading it (src/algorithms_outer.jl:125-133), so the
+    # device's U and V need no upload for these fresh runs
     ctx.set_ll(float("nan"), ctx.loglikelihood(N.ADMX_X))
     for it in range(1, p_steps + 1):
         ctx.qn_iteration(it, fix_q=True)
@@ -225,7 +227,6 @@ def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_i
         else:
             ctx.set_x(N.ADMX_X, x_warm)
     ll0 = ll_warm if chosen == "warm" else d.ll_new
-    ctx.set_uv(d.U, d.V)
     ctx.set_ll(float("nan"), ll0)
     hist = [(time.perf_counter() - t0, ll0)]
     its = 0
