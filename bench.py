@@ -402,6 +402,10 @@ def main():
     ap.add_argument("--graphs", action="store_true", help="replay steady-state iterations from hipGraphs")
     ap.add_argument("--stream-snps", type=int, default=0,
                     help="keep the BED in pinned host memory and stream it in chunks of this many SNPs")
+    ap.add_argument("--stream-from-file", action="store_true",
+                    help="with --stream-snps: write the cohort as a PLINK .bed and stream from the file")
+    ap.add_argument("--resident-snps", type=int, default=0,
+                    help="with --stream-snps: keep the first this many SNPs in HBM")
     ap.add_argument("--ttc-forms", default="bf16,warm_f64,warm_bf16",
                     help="forms timed to convergence beside plain FP64 (bf16, f32, warm_f64, warm_bf16)")
     ap.add_argument("--ttc-max-iter", type=int, default=200)
@@ -442,19 +446,38 @@ def main():
     I, J, K = wl["I"], wl["J"], wl["K"]
     T = np.float64 if args.dtype == "f64" else np.float32
 
+    if (args.stream_from_file or args.resident_snps) and not args.stream_snps:
+        sys.exit("--stream-from-file and --resident-snps need --stream-snps")
+    if args.form != "plain" and args.dtype != "f64":
+        sys.exit("--form f32/bf16 runs on a FP64 context (FP64 log-likelihood, QP and convergence test)")
     ctx = oa.HIPAdmixData(I, J, K, 3, T, stream_snps=args.stream_snps)
+    t0 = time.perf_counter()
+    Qt_true, Pt_true = sim.cohort_on_device(ctx, wl["alpha"], wl["p_dist"], wl["p_lo"], wl["p_hi"], wl["miss_rate"],
+                                            seed=args.seed)
+    t_gen = time.perf_counter() - t0
+    stream_note = ""
+    if args.stream_snps:
+        stream_note = f", streamed in {args.stream_snps}-SNP chunks from pinned host memory"
+    if args.stream_from_file:
+        import atexit
+        import shutil
+        import tempfile
+
+        tmpd = tempfile.mkdtemp(prefix="admx_bench_")
+        atexit.register(shutil.rmtree, tmpd, True)
+        path = oa.write_plink(os.path.join(tmpd, "cohort"), ctx.download_bed(), I, J)
+        ctx.close()
+        ctx = oa.HIPAdmixData(I, J, K, 3, T, stream_snps=args.stream_snps, bed_file=path)
+        stream_note = f", streamed in {args.stream_snps}-SNP chunks from a .bed file (page cache)"
+    if args.resident_snps:
+        ctx.set_resident_snps(args.resident_snps)
+        stream_note += f", SNPs 0..{min(args.resident_snps, J) - 1} resident in HBM"
     if args.target_waves:
         ctx.set_tuning(oa._native.ADMX_TUNE_TARGET_WAVES, args.target_waves)
     if args.graphs:
         ctx.set_graphs(True)
     if args.form != "plain":
-        if args.dtype != "f64":
-            sys.exit("--form f32/bf16 runs on a FP64 context (FP64 log-likelihood, QP and convergence test)")
         ctx.set_precision(args.form)
-    t0 = time.perf_counter()
-    Qt_true, Pt_true = sim.cohort_on_device(ctx, wl["alpha"], wl["p_dist"], wl["p_lo"], wl["p_hi"], wl["miss_rate"],
-                                            seed=args.seed)
-    t_gen = time.perf_counter() - t0
     d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(args.seed + 1), T=T)
     oa.init_em(d, None, 5, d_cu=ctx, log=None)
     ctx.set_uv(d.U, d.V)
@@ -500,7 +523,7 @@ def main():
         "vs_baseline": None,
         "dtype": args.dtype if args.form == "plain" else f"{args.form} passes, f64 LL/QP",
         "data": "synthetic cohort generated on device (Dirichlet Q, Beta/Uniform P, Binomial(2, q.p) genotypes)",
-        "config": {"workload": f"{args.workload}: I={I}, J={J}, K={K}, Q=3, ZAL, {args.form} {args.dtype}",
+        "config": {"workload": f"{args.workload}: I={I}, J={J}, K={K}, Q=3, ZAL, {args.form} {args.dtype}{stream_note}",
                    "I": I, "J": J, "K": K, "genotypes": I * J, "bed_bytes": ((I + 3) // 4) * J},
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,

@@ -32,6 +32,7 @@ extern "C" {
 #define ADMX_EINVAL 1 /* bad argument (shape, range, dtype, K > 16, Q > 8) */
 #define ADMX_EHIP 2   /* HIP runtime error */
 #define ADMX_ESTATE 3 /* missing state (e.g. no genotypes uploaded) */
+#define ADMX_EIO 4    /* reading a .bed file failed */
 
 #define ADMX_F64 0
 #define ADMX_F32 1
@@ -56,6 +57,9 @@ extern "C" {
                                    * (src/loops.jl:493-514, 589-608, 679-698, 840-865, 896-921), where a
                                    * missing genotype counts as the SNP's mean over its observed entries;
                                    * every pass and entry point follows it (FP64/FP32 passes only) */
+#define ADMX_TUNE_RESIDENT_SNPS 6 /* streamed contexts: keep SNPs [0, value) in HBM (copied once, and
+                                   * again after admx_upload_bed / admx_generate_bed); passes stream
+                                   * only the rest.  0 (default): stream everything */
 
 /* values of ADMX_TUNE_QN_MODE: admixture_qn!'s `mode` (src/algorithms_outer.jl:127-136) */
 #define ADMX_QN_ZAL 0
@@ -91,6 +95,18 @@ int admx_create(admx_ctx **out, int64_t I, int64_t J, int64_t K, int64_t Q, int 
  * unfinished). */
 int admx_create_streamed(admx_ctx **out, int64_t I, int64_t J, int64_t K, int64_t Q, int dtype,
                          int64_t snps_per_chunk);
+/* J-chunk streaming straight from a PLINK .bed file (SNP-major, the 3-byte
+ * magic 6C 1B 01 followed by ceil(I/4) * J bytes), for cohorts larger than
+ * host memory as well: no host copy of the BED is kept.  Each chunk is read
+ * (pread, up to 8 host threads, from the page cache or the disk) into one of
+ * two pinned staging buffers, copied to HBM and re-pitched there while the
+ * previous chunk's kernels run.  The file stays open, read-only, until
+ * admx_destroy; admx_upload_bed and admx_generate_bed are refused
+ * (ADMX_ESTATE), admx_download_bed reads the file.  Same pass coverage and
+ * results as admx_create_streamed.  No reference counterpart (the reference
+ * reads the whole file through SnpArrays, src/driver.jl:144-145). */
+int admx_create_streamed_file(admx_ctx **out, const char *bed_path, int64_t I, int64_t J, int64_t K, int64_t Q,
+                              int dtype, int64_t snps_per_chunk);
 int admx_destroy(admx_ctx *ctx);
 /* Upload the raw SNP-major BED payload: J columns of bytes_per_snp bytes
  * (bytes_per_snp >= ceil(I/4); the 3-byte PLINK magic already stripped),

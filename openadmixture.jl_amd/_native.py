@@ -13,13 +13,14 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libadmixhip.so")
 
-ADMX_OK, ADMX_EINVAL, ADMX_EHIP, ADMX_ESTATE = 0, 1, 2, 3
+ADMX_OK, ADMX_EINVAL, ADMX_EHIP, ADMX_ESTATE, ADMX_EIO = 0, 1, 2, 3, 4
 ADMX_F64, ADMX_F32 = 0, 1
 ADMX_X, ADMX_X_NEXT, ADMX_X_NEXT2, ADMX_X_TMP = 0, 1, 2, 3
 ADMX_SIDE_Q, ADMX_SIDE_P = 0, 1
 ADMX_TUNE_TARGET_WAVES, ADMX_TUNE_KEEP_HESSIANS, ADMX_TUNE_PRECISION, ADMX_TUNE_GRAPHS = 0, 1, 2, 3
 ADMX_TUNE_QN_MODE = 4
 ADMX_TUNE_SKIPMISSING = 5
+ADMX_TUNE_RESIDENT_SNPS = 6
 ADMX_QN_ZAL, ADMX_QN_LBQN = 0, 1
 ADMX_PREC_F64, ADMX_PREC_F32, ADMX_PREC_BF16 = 0, 1, 2
 ADMX_SKFR_GLOBAL, ADMX_SKFR_LOCAL = 0, 1
@@ -32,6 +33,7 @@ SIGNATURES = {
     "admx_device_count": ([_vp], _int),
     "admx_create": ([_vp, _i64, _i64, _i64, _i64, _int], _int),
     "admx_create_streamed": ([_vp, _i64, _i64, _i64, _i64, _int, _i64], _int),
+    "admx_create_streamed_file": ([_vp, C.c_char_p, _i64, _i64, _i64, _i64, _int, _i64], _int),
     "admx_destroy": ([_vp], _int),
     "admx_upload_bed": ([_vp, _vp, _i64], _int),
     "admx_download_bed": ([_vp, _vp], _int),

@@ -3,15 +3,20 @@
 // device-resident orchestration of update_q!/update_p!/init_em!/admixture_qn!
 // on one HIP stream.  Only plumbing lives here; all arithmetic on genotypes,
 // QPs, projections and quasi-Newton steps runs in the HIP kernels.
+#include <fcntl.h>
 #include <hip/hip_runtime.h>
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include <algorithm>
+#include <cerrno>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <map>
 #include <mutex>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "admixhip.h"
@@ -97,13 +102,23 @@ struct admx_ctx {
   DevBuf fq, fp, ppt, mmt, qqt;  // mixed-precision pass inputs
   DevBuf lqt, lpt, lmt;           // BF16 split tables of the matrix-core log-likelihood
   DevBuf feat, featpart;          // warm-start features (I x 64) and their chunk partials
-  // J-chunk streaming: the BED lives in pinned host memory and every pass
-  // streams it through two device chunk buffers on a copy stream.
+  // J-chunk streaming: the BED lives in pinned host memory, or in a PLINK .bed
+  // file read chunk by chunk, and every pass streams it through two device
+  // chunk buffers on a copy stream.  The first res_snps SNPs can stay in HBM.
   int64_t stream_snps = 0;        // SNPs per chunk (0: the BED is resident in HBM)
   uint8_t* hbed = nullptr;        // pinned host BED, pitch * (J + 16)
   uint8_t* dchunk[2] = {nullptr, nullptr};
   hipStream_t cs = nullptr;       // copy stream (default priority)
   hipEvent_t ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
+  bool ev_used[2] = {false, false};  // ev_ready[b] has been recorded
+  int64_t held[2][2] = {{0, 0}, {0, 0}};  // rows [held[b][0], held[b][1]) of the BED are in dchunk[b]
+  int last_buf = 1;               // chunk buffer of the latest streamed chunk
+  bool stream_rev = true;         // the latest pass ran its streamed chunks backwards
+  int bed_fd = -1;                // file-backed: the .bed file (rows of ibytes at offset 3)
+  uint8_t* hstage[2] = {nullptr, nullptr};  // file-backed: pinned raw rows, ibytes * (S + 16)
+  uint8_t* draw[2] = {nullptr, nullptr};    // file-backed: the same rows in HBM, before re-pitching
+  int64_t res_snps = 0;           // SNPs [0, res_snps) resident in `dres`
+  uint8_t* dres = nullptr;        // pitch * (res_snps + 16)
   int impute = 0;                 // 1: mean-imputation loops (skipmissing = false)
   bool mu_valid = false;          // mu holds the per-SNP means of the current genotypes
   DevBuf snp_mu, cnt;             // per-SNP means (T, J); count scratch
@@ -266,40 +281,142 @@ const char* mode_name(bool qside, int mode) {
   return mode == MODE_UPDATE ? "pside_update" : "pside_em";
 }
 
-// Genotype source of a pass over SNPs [j0, j1): fn(base, cj0, cj1) launches on
-// c->s with bed rows reachable as base + pitch * j for j in [cj0, cj1 + 16).
-// Resident context: one call with the HBM BED.  Streamed context: chunks of
-// stream_snps SNPs alternate between two device buffers; the copy of chunk
-// k+1 (copy stream) overlaps the kernels of chunk k, and a buffer is refilled
-// only after the kernels that read it have finished (events).  The base pointer
-// of a chunk is the buffer address minus pitch * cj0.
+// Rows [r0, r1) (r1 <= J) of a file-backed BED into `dst`, ibytes per row, as
+// they lie in the file.  pread from the page cache or the disk, split over up
+// to 8 host threads of at least 4 MiB each.
+int read_file_rows(admx_ctx* c, int64_t r0, int64_t r1, uint8_t* dst) {
+  const int64_t n = (r1 - r0) * c->ibytes;
+  if (n <= 0) return ADMX_OK;
+  const int64_t base = 3 + r0 * c->ibytes;
+  const int nt = (int)std::max<int64_t>(1, std::min<int64_t>(8, n >> 22));
+  std::vector<int> err((size_t)nt, 0);
+  auto work = [&](int t) {
+    int64_t a = n * t / nt;
+    const int64_t e = n * (t + 1) / nt;
+    while (a < e) {
+      const ssize_t got = pread(c->bed_fd, dst + a, (size_t)(e - a), (off_t)(base + a));
+      if (got < 0 && errno == EINTR) continue;
+      if (got <= 0) {
+        err[(size_t)t] = got < 0 ? errno : EIO;
+        return;
+      }
+      a += got;
+    }
+  };
+  if (nt == 1) {
+    work(0);
+  } else {
+    std::vector<std::thread> th;
+    for (int t = 0; t < nt; ++t) th.emplace_back(work, t);
+    for (auto& t : th) t.join();
+  }
+  for (int e : err)
+    if (e) return fail(ADMX_EIO, std::string("reading the .bed file: ") + std::strerror(e));
+  return ADMX_OK;
+}
+
+// Enqueue on the copy stream: rows [r0, r1) (r1 <= J + 16) of a streamed BED
+// into `dst` (pitch per row), using chunk slot b's buffers.  Pinned host BED:
+// one copy.  File-backed: the host reads the rows into slot b's pinned staging
+// buffer (after its previous copy has finished), copies them to HBM and
+// re-pitches them there (rows past J become missing codes).
+int stage_rows(admx_ctx* c, int b, int64_t r0, int64_t r1, uint8_t* dst) {
+  if (c->hbed) {
+    HIPCHK(hipMemcpyAsync(dst, c->hbed + c->pitch * r0, (size_t)(c->pitch * (r1 - r0)), hipMemcpyHostToDevice, c->cs));
+    return ADMX_OK;
+  }
+  if (c->ev_used[b]) HIPCHK(hipEventSynchronize(c->ev_ready[b]));
+  const int64_t real = std::max<int64_t>(0, std::min(r1, c->J) - r0);
+  CHK(read_file_rows(c, r0, r0 + real, c->hstage[b]));
+  if (real) HIPCHK(hipMemcpyAsync(c->draw[b], c->hstage[b], (size_t)(c->ibytes * real), hipMemcpyHostToDevice, c->cs));
+  launch_repitch(c->draw[b], c->ibytes, real, r1 - r0, dst, c->pitch, c->cs);
+  return check_launch();
+}
+
+// SNP chunks of a pass over [j0, j1): one for a resident context; for a
+// streamed one, the part below res_snps, then chunks of stream_snps SNPs.
+// for_chunks calls its function once per chunk, in this order.
+std::vector<std::pair<int64_t, int64_t>> snp_chunks(const admx_ctx* c, int64_t j0, int64_t j1) {
+  std::vector<std::pair<int64_t, int64_t>> r;
+  if (!c->stream_snps) {
+    if (j0 < j1) r.push_back({j0, j1});
+    return r;
+  }
+  if (j0 < std::min(j1, c->res_snps)) {
+    r.push_back({j0, std::min(j1, c->res_snps)});
+    j0 = std::min(j1, c->res_snps);
+  }
+  for (int64_t a = j0; a < j1; a += c->stream_snps) r.push_back({a, std::min(j1, a + c->stream_snps)});
+  return r;
+}
+
+// Genotype source of a pass over SNPs [j0, j1): fn(base, cj0, cj1, k) launches
+// on c->s for chunk k of snp_chunks, with bed rows reachable as base + pitch * j
+// for j in [cj0, cj1 + 16).  Resident context: one call with the HBM BED.
+// Streamed context: the HBM-resident chunk first, then the streamed chunks,
+// alternately forwards and backwards from one pass to the next, through two
+// device buffers: the copy of the next chunk (copy stream) overlaps the kernels
+// of the current one, and a buffer is refilled only after the kernels that
+// read it have finished (events).  A chunk whose rows a buffer already holds
+// (the previous pass's last chunk, by the alternation) is not copied again.
+// The base pointer of a chunk is the buffer address minus pitch * (its first row).
 template <typename F>
 int for_chunks(admx_ctx* c, int64_t j0, int64_t j1, F&& fn) {
-  if (!c->stream_snps) return fn(static_cast<const uint8_t*>(c->bed), j0, j1);
-  const int64_t S = c->stream_snps;
-  int k = 0;
-  for (int64_t cj0 = j0; cj0 < j1; cj0 += S, ++k) {
-    const int64_t cj1 = std::min(j1, cj0 + S);
-    const int b = k & 1;
+  if (!c->stream_snps) return fn(static_cast<const uint8_t*>(c->bed), j0, j1, (size_t)0);
+  const auto chunks = snp_chunks(c, j0, j1);
+  std::vector<size_t> order;
+  size_t first = 0;
+  if (!chunks.empty() && chunks[0].second <= c->res_snps) order.push_back(first++);
+  c->stream_rev = !c->stream_rev;
+  for (size_t n = first; n < chunks.size(); ++n) order.push_back(c->stream_rev ? chunks.size() - 1 - (n - first) : n);
+  for (size_t k : order) {
+    const int64_t cj0 = chunks[k].first, cj1 = chunks[k].second;
+    if (cj1 <= c->res_snps) {  // HBM-resident rows
+      CHK(fn(static_cast<const uint8_t*>(c->dres), cj0, cj1, k));
+      continue;
+    }
     const int64_t r1 = std::min(cj1 + 16, c->J + 16);
-    HIPCHK(hipStreamWaitEvent(c->cs, c->ev_free[b], 0));
-    HIPCHK(hipMemcpyAsync(c->dchunk[b], c->hbed + c->pitch * cj0, (size_t)(c->pitch * (r1 - cj0)),
-                          hipMemcpyHostToDevice, c->cs));
-    HIPCHK(hipEventRecord(c->ev_ready[b], c->cs));
+    int b = -1;
+    for (int bb = 0; bb < 2; ++bb)
+      if (c->held[bb][0] <= cj0 && r1 <= c->held[bb][1]) b = bb;
+    if (b < 0) {
+      b = c->last_buf ^ 1;
+      c->held[b][0] = c->held[b][1] = 0;
+      HIPCHK(hipStreamWaitEvent(c->cs, c->ev_free[b], 0));
+      CHK(stage_rows(c, b, cj0, r1, c->dchunk[b]));
+      HIPCHK(hipEventRecord(c->ev_ready[b], c->cs));
+      c->ev_used[b] = true;
+      c->held[b][0] = cj0;
+      c->held[b][1] = r1;
+    }
     HIPCHK(hipStreamWaitEvent(c->s, c->ev_ready[b], 0));
-    CHK(fn(static_cast<const uint8_t*>(c->dchunk[b]) - c->pitch * cj0, cj0, cj1));
+    CHK(fn(static_cast<const uint8_t*>(c->dchunk[b]) - c->pitch * c->held[b][0], cj0, cj1, k));
     HIPCHK(hipEventRecord(c->ev_free[b], c->s));
+    c->last_buf = b;
   }
   return ADMX_OK;
 }
 
-// SNP chunks of a pass over [j0, j1) (one for a resident context).
-std::vector<std::pair<int64_t, int64_t>> snp_chunks(const admx_ctx* c, int64_t j0, int64_t j1) {
-  std::vector<std::pair<int64_t, int64_t>> r;
-  const int64_t S = c->stream_snps ? c->stream_snps : std::max<int64_t>(j1 - j0, 1);
-  for (int64_t a = j0; a < j1; a += S) r.push_back({a, std::min(j1, a + S)});
-  return r;
+// The chunk buffers no longer hold current genotype rows.
+void forget_chunks(admx_ctx* c) {
+  for (auto& h : c->held) h[0] = h[1] = 0;
 }
+
+// Copy SNPs [0, res_snps + 16) of a streamed BED into the HBM-resident rows
+// (after the genotypes change or the resident count is set); synchronous.
+int refresh_resident(admx_ctx* c) {
+  if (!c->stream_snps || !c->res_snps) return ADMX_OK;
+  const int64_t R = std::min(c->res_snps + 16, c->J + 16), S = c->stream_snps;
+  HIPCHK(hipStreamSynchronize(c->s));
+  for (int64_t r0 = 0; r0 < R; r0 += S) {
+    CHK(stage_rows(c, 0, r0, std::min(R, r0 + S), c->dres + c->pitch * r0));
+    HIPCHK(hipEventRecord(c->ev_ready[0], c->cs));
+    c->ev_used[0] = true;
+  }
+  HIPCHK(hipStreamSynchronize(c->cs));
+  return ADMX_OK;
+}
+
 
 int need_resident(admx_ctx* c, const char* what) {
   if (c->stream_snps) return fail(ADMX_ESTATE, std::string(what) + " needs a resident BED (not a streamed context)");
@@ -326,9 +443,11 @@ int ensure_mu(admx_ctx* c) {
 }
 
 // Run a Q-side pass; returns the chunk count (partials in c->part or c->llpart).
+// MODE_LL with a second state (q2, p2): both log-likelihoods from one pass over
+// the genotypes (the second state's partials follow the first's *nparts).
 template <typename T>
 int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0, int64_t j1,
-           int* nchunk, int64_t* nparts) {
+           int* nchunk, int64_t* nparts, const void* q2 = nullptr, const void* p2 = nullptr) {
   const int64_t Ilen = i1 - i0, Jlen = j1 - j0;
   *nchunk = 0;
   *nparts = 0;
@@ -347,23 +466,29 @@ int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
   }
   const int64_t nbx = (Ilen + 255) / 256;
   const int na = pass_na(mode, true, (int)c->K, sizeof(T) == 8);
-  if (mode == MODE_LL) CHK(c->llpart.grow((size_t)(nbx * total) * sizeof(double)));
+  if (mode == MODE_LL) CHK(c->llpart.grow((size_t)(nbx * total) * (q2 ? 2 : 1) * sizeof(double)));
   else CHK(c->part.grow((size_t)total * na * Ilen * sizeof(T)));
+  // chunk k's partial slots start after those of chunks 0..k-1, whatever order
+  // the chunks run in
+  std::vector<int64_t> first(chunks.size(), 0);
+  for (size_t k = 1; k < chunks.size(); ++k) first[k] = first[k - 1] + ns[k - 1];
   {
     ProfScope ps(c, mode_name(true, mode));
-    size_t k = 0;
-    int64_t off = 0;
-    CHK(for_chunks(c, j0, j1, [&](const uint8_t* base, int64_t cj0, int64_t cj1) {
+    CHK(for_chunks(c, j0, j1, [&](const uint8_t* base, int64_t cj0, int64_t cj1, size_t k) {
       PassArgs a{base, c->pitch, c->I, q, p, i0, i1, cj0, cj1, chs[k], nullptr, c->impute ? c->snp_mu.p : nullptr};
       if (mode == MODE_LL) {
-        a.part = static_cast<double*>(c->llpart.p) + off;
-        off += nbx * ns[k];
+        a.part = static_cast<double*>(c->llpart.p) + nbx * first[k];
+        if (q2) {
+          PassArgs a2 = a;
+          a2.q = q2;
+          a2.p = p2;
+          a2.part = static_cast<double*>(c->llpart.p) + nbx * (total + first[k]);
+          launch_qside<T>(mode, (int)c->K, a2, ns[k], c->s);
+        }
       } else {
-        a.part = static_cast<T*>(c->part.p) + off;
-        off += (int64_t)ns[k] * na * Ilen;
+        a.part = static_cast<T*>(c->part.p) + first[k] * na * Ilen;
       }
       launch_qside<T>(mode, (int)c->K, a, ns[k], c->s);
-      ++k;
       return check_launch();
     }));
   }
@@ -385,7 +510,7 @@ int pass_p(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
   CHK(ensure_mu<T>(c));
   {
     ProfScope ps(c, mode_name(false, mode));
-    CHK(for_chunks(c, j0, j1, [&](const uint8_t* base, int64_t cj0, int64_t cj1) {
+    CHK(for_chunks(c, j0, j1, [&](const uint8_t* base, int64_t cj0, int64_t cj1, size_t) {
       PassArgs a{base, c->pitch, c->I, q, p, i0, i1, cj0, cj1, chunk, c->part.p, c->impute ? c->snp_mu.p : nullptr,
                  Jlen, cj0 - j0};
       launch_pside<T>(mode, (int)c->K, a, n, c->s);
@@ -438,6 +563,20 @@ int ll_to_slot(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1
   CHK(pass_q<T>(c, MODE_LL, q, p, i0, i1, j0, j1, &nchunk, &np));
   ProfScope ps(c, "ll_reduce");
   launch_reduce_sum(static_cast<const double*>(c->llpart.p), np, c->lls + slot, 0, c->s);
+  return check_launch();
+}
+
+// Full-cohort LLs of two states into two slots from one pass over the
+// genotypes (FP64/FP32 loop): for a streamed context, one BED transfer instead
+// of two; each LL is summed exactly as ll_to_slot sums it.
+template <typename T>
+int ll2_to_slots(admx_ctx* c, void* x1, int slot1, void* x2, int slot2) {
+  int nchunk = 0;
+  int64_t np = 0;
+  CHK(pass_q<T>(c, MODE_LL, x1, c->pp(x1), 0, c->I, 0, c->J, &nchunk, &np, x2, c->pp(x2)));
+  ProfScope ps(c, "ll_reduce");
+  launch_reduce_sum(static_cast<const double*>(c->llpart.p), np, c->lls + slot1, 0, c->s);
+  launch_reduce_sum(static_cast<const double*>(c->llpart.p) + np, np, c->lls + slot2, 0, c->s);
   return check_launch();
 }
 
@@ -665,10 +804,13 @@ int enqueue_qn_iteration(admx_ctx* c, int64_t iter, int fix_q, int fix_p) {
   else HIPCHK(hipMemcpyAsync(XN2, X, qbytes, hipMemcpyDeviceToDevice, c->s));
   if (!fix_p) CHK(dev_update_p<T>(c, true, nullptr));
   else HIPCHK(hipMemcpyAsync(c->pp(XN2), c->pp(X), pbytes, hipMemcpyDeviceToDevice, c->s));
-  CHK(ll_to_slot<T>(c, XN2, c->pp(XN2), 0, c->I, 0, c->J, LL_BASIC));
+  // A streamed context evaluates ll_basic after the quasi-Newton step (which
+  // does not read it), in the same pass over the genotypes as ll_qn.
+  if (!c->stream_snps) CHK(ll_to_slot<T>(c, XN2, c->pp(XN2), 0, c->I, 0, c->J, LL_BASIC));
   if (c->qn_mode == ADMX_QN_LBQN) CHK(dev_lbqn<T>(c, iter, X, XN, XN2, c->U, c->V, XT));
   else CHK(dev_zal<T>(c, iter, X, XN, XN2, c->U, c->V, XT));
-  CHK(ll_to_slot<T>(c, XT, c->pp(XT), 0, c->I, 0, c->J, LL_QN));
+  if (c->stream_snps) CHK(ll2_to_slots<T>(c, XN2, LL_BASIC, XT, LL_QN));
+  else CHK(ll_to_slot<T>(c, XT, c->pp(XT), 0, c->I, 0, c->J, LL_QN));
   {
     ProfScope ps(c, "accept");
     launch_accept<T>(c->N(), X, XT, XN2, c->lls, c->s);
@@ -1078,7 +1220,8 @@ int admx_device_count(int* n) {
 }  // extern "C"
 
 namespace {
-int create_impl(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int dtype, int64_t stream_snps) {
+int create_impl(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int dtype, int64_t stream_snps,
+                const char* bed_path = nullptr) {
   if (!out) return bad("null output pointer");
   *out = nullptr;
   if (I < 1 || J < 1) return bad("I and J must be >= 1");
@@ -1099,19 +1242,44 @@ int create_impl(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int 
     admx_destroy(c);
     return rc;
   };
+  if (bed_path) {
+    // .bed file: PLINK-1 magic, SNP-major mode, ceil(I/4) * J payload bytes
+    c->bed_fd = open(bed_path, O_RDONLY | O_CLOEXEC);
+    if (c->bed_fd < 0) return cleanup(fail(ADMX_EIO, std::string("cannot open ") + bed_path + ": " + std::strerror(errno)));
+    struct stat st;
+    uint8_t magic[3] = {0, 0, 0};
+    if (fstat(c->bed_fd, &st) != 0 || pread(c->bed_fd, magic, 3, 0) != 3)
+      return cleanup(fail(ADMX_EIO, std::string("cannot read ") + bed_path));
+    if (magic[0] != 0x6C || magic[1] != 0x1B || magic[2] != 0x01)
+      return cleanup(fail(ADMX_EINVAL, std::string(bed_path) + " is not a SNP-major PLINK .bed file"));
+    if ((int64_t)st.st_size != 3 + c->ibytes * J)
+      return cleanup(fail(ADMX_EINVAL, std::string(bed_path) + " does not hold 3 + ceil(I/4) * J bytes"));
+  }
   hipError_t e = hipGetDevice(&c->device);
   if (e != hipSuccess) return cleanup(fail(ADMX_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e)));
   if ((e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking)) != hipSuccess)
     return cleanup(fail(ADMX_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
   const size_t xb = (size_t)c->N() * c->esz;
   auto alloc = [&](void** p, size_t bytes) { return hipMalloc(p, bytes < 256 ? 256 : bytes); };
-  if (stream_snps) {
-    // BED in pinned host memory (zero, padding rows = missing code), two chunk buffers
-    const size_t hb = (size_t)c->pitch * (J + 16), cb = (size_t)c->pitch * (stream_snps + 16);
+  if (stream_snps && bed_path) {
+    const size_t rb = (size_t)c->ibytes * (stream_snps + 16);
+    for (int b = 0; b < 2; ++b) {
+      if (hipHostMalloc((void**)&c->hstage[b], rb, hipHostMallocDefault) != hipSuccess)
+        return cleanup(fail(ADMX_EHIP, "hipHostMalloc failed for the .bed staging buffers"));
+      if (alloc((void**)&c->draw[b], rb) != hipSuccess)
+        return cleanup(fail(ADMX_EHIP, "hipMalloc failed for the .bed staging buffers"));
+    }
+    c->have_bed = true;
+  } else if (stream_snps) {
+    // BED in pinned host memory (zero, padding rows = missing code)
+    const size_t hb = (size_t)c->pitch * (J + 16);
     if (hipHostMalloc((void**)&c->hbed, hb, hipHostMallocDefault) != hipSuccess)
       return cleanup(fail(ADMX_EHIP, "hipHostMalloc failed for the streamed BED"));
     std::memset(c->hbed, 0, (size_t)c->pitch * J);
     std::memset(c->hbed + (size_t)c->pitch * J, 0x55, (size_t)c->pitch * 16);
+  }
+  if (stream_snps) {  // two chunk buffers, a copy stream and its events
+    const size_t cb = (size_t)c->pitch * (stream_snps + 16);
     if (alloc((void**)&c->dchunk[0], cb) != hipSuccess || alloc((void**)&c->dchunk[1], cb) != hipSuccess)
       return cleanup(fail(ADMX_EHIP, "hipMalloc failed for the BED chunk buffers"));
     if (hipStreamCreateWithFlags(&c->cs, hipStreamNonBlocking) != hipSuccess)
@@ -1155,6 +1323,14 @@ int admx_create_streamed(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_
   return create_impl(out, I, J, K, Q, dtype, std::min<int64_t>(snps_per_chunk, J));
 }
 
+int admx_create_streamed_file(admx_ctx** out, const char* bed_path, int64_t I, int64_t J, int64_t K, int64_t Q,
+                              int dtype, int64_t snps_per_chunk) {
+  if (out) *out = nullptr;
+  if (!bed_path) return bad("null .bed path");
+  if (snps_per_chunk < 16) return bad("a streamed context needs at least 16 SNPs per chunk");
+  return create_impl(out, I, J, K, Q, dtype, std::min<int64_t>(snps_per_chunk, J), bed_path);
+}
+
 int admx_destroy(admx_ctx* c) {
   if (!c) return ADMX_OK;
   (void)hipSetDevice(c->device);
@@ -1173,7 +1349,11 @@ int admx_destroy(admx_ctx* c) {
   if (c->h_lls) (void)hipHostFree(c->h_lls);
   if (c->cs) (void)hipStreamSynchronize(c->cs);
   if (c->hbed) (void)hipHostFree(c->hbed);
+  if (c->dres) (void)hipFree(c->dres);
+  if (c->bed_fd >= 0) (void)close(c->bed_fd);
   for (int b = 0; b < 2; ++b) {
+    if (c->hstage[b]) (void)hipHostFree(c->hstage[b]);
+    if (c->draw[b]) (void)hipFree(c->draw[b]);
     if (c->dchunk[b]) (void)hipFree(c->dchunk[b]);
     if (c->ev_ready[b]) (void)hipEventDestroy(c->ev_ready[b]);
     if (c->ev_free[b]) (void)hipEventDestroy(c->ev_free[b]);
@@ -1209,7 +1389,10 @@ int admx_upload_bed(admx_ctx* c, const uint8_t* bed, int64_t bytes_per_snp) {
   Guard g(c);
   if (!bed) return bad("null BED pointer");
   if (bytes_per_snp < c->ibytes) return bad("bytes_per_snp < ceil(I/4)");
+  if (c->bed_fd >= 0) return fail(ADMX_ESTATE, "a file-backed context reads its genotypes from the .bed file");
   if (c->stream_snps) {  // host copy into the pinned BED; the padding rows stay missing
+    HIPCHK(hipStreamSynchronize(c->s));  // no copy out of the pinned BED is still in flight
+    HIPCHK(hipStreamSynchronize(c->cs));
     for (int64_t j = 0; j < c->J; ++j) {
       uint8_t* row = c->hbed + (size_t)c->pitch * j;
       std::memcpy(row, bed + (size_t)bytes_per_snp * j, (size_t)c->ibytes);
@@ -1217,7 +1400,8 @@ int admx_upload_bed(admx_ctx* c, const uint8_t* bed, int64_t bytes_per_snp) {
     }
     c->have_bed = true;
     c->mu_valid = false;
-    return ADMX_OK;
+    forget_chunks(c);
+    return refresh_resident(c);
   }
   HIPCHK(hipMemsetAsync(c->bed, 0, (size_t)c->pitch * c->J, c->s));
   CHK(pad_bed(c));
@@ -1233,6 +1417,7 @@ int admx_download_bed(admx_ctx* c, uint8_t* bed) {
   Guard g(c);
   CHK(need_bed(c));
   if (!bed) return bad("null BED pointer");
+  if (c->bed_fd >= 0) return read_file_rows(c, 0, c->J, bed);
   if (c->stream_snps) {
     for (int64_t j = 0; j < c->J; ++j)
       std::memcpy(bed + (size_t)c->ibytes * j, c->hbed + (size_t)c->pitch * j, (size_t)c->ibytes);
@@ -1247,6 +1432,7 @@ int admx_generate_bed(admx_ctx* c, const double* Qt, const double* Pt, uint64_t 
   CHK(check_ctx(c));
   Guard g(c);
   if (!Qt || !Pt) return bad("null Q or P");
+  if (c->bed_fd >= 0) return fail(ADMX_ESTATE, "a file-backed context reads its genotypes from the .bed file");
   const size_t qb = (size_t)(c->K * c->I) * sizeof(double), pb = (size_t)(c->K * c->J) * sizeof(double);
   CHK(stage_in(c, 0, Qt, qb, qb));
   CHK(stage_in(c, 1, Pt, pb, pb));
@@ -1263,7 +1449,8 @@ int admx_generate_bed(admx_ctx* c, const double* Qt, const double* Pt, uint64_t 
     HIPCHK(hipStreamSynchronize(c->s));
     c->have_bed = true;
     c->mu_valid = false;
-    return ADMX_OK;
+    forget_chunks(c);
+    return refresh_resident(c);
   }
   {
     ProfScope ps(c, "gen_bed");
@@ -1925,6 +2112,28 @@ int admx_set_tuning(admx_ctx* c, int key, int64_t value) {
   } else if (key == ADMX_TUNE_GRAPHS) {
     c->use_graphs = value != 0;
     if (!c->use_graphs) drop_graphs(c);
+  } else if (key == ADMX_TUNE_RESIDENT_SNPS) {
+    if (!c->stream_snps) return bad("resident SNPs apply to a streamed context");
+    if (value < 0) return bad("resident SNPs must be >= 0");
+    const int64_t R = std::min(value, c->J);
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (c->dres) HIPCHK(hipFree(c->dres));
+    c->dres = nullptr;
+    c->res_snps = 0;
+    if (R > 0) {
+      if (hipMalloc((void**)&c->dres, (size_t)c->pitch * (R + 16)) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(ADMX_EHIP, "hipMalloc failed for the resident SNPs");
+      }
+      c->res_snps = R;
+      const int rc = refresh_resident(c);
+      if (rc != ADMX_OK) {
+        (void)hipFree(c->dres);
+        c->dres = nullptr;
+        c->res_snps = 0;
+        return rc;
+      }
+    }
   } else {
     return bad("unknown tuning key");
   }

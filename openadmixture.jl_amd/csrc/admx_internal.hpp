@@ -134,6 +134,10 @@ void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, in
 // Rows [j0, j1) only, into bed + pitch * j (a streamed context's chunk buffer).
 void launch_gen_bed_rows(const double* Qt, const double* Pt, int64_t I, int64_t j0, int64_t j1, int K,
                          uint64_t seed, double miss_rate, uint8_t* bed, int64_t pitch, hipStream_t s);
+// `rows` rows of a .bed file's payload (ibytes each, back to back; `real` of
+// them present, the rest become missing codes) into the pitched device layout.
+void launch_repitch(const uint8_t* raw, int64_t ibytes, int64_t real, int64_t rows, uint8_t* dst, int64_t pitch,
+                    hipStream_t s);
 
 // Slots of the device log-likelihood vector.
 enum LLSlot : int { LL_PREV = 0, LL_NEW = 1, LL_BASIC = 2, LL_QN = 3, LL_TMP = 4, LL_NSLOTS = 8 };

@@ -524,6 +524,34 @@ void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, in
   launch_gen_bed_rows(Qt, Pt, I, 0, J, K, seed, miss_rate, bed, pitch, s);
 }
 
+// BED rows as a .bed file holds them (ibytes per row, back to back) into the
+// device layout (pitch per row, bytes past ibytes zero); rows from `real` on
+// are the missing code.  One lane per 32-bit word of the destination.
+__global__ __launch_bounds__(256) void k_repitch(const uint8_t* __restrict__ raw, int64_t ibytes, int64_t real,
+                                                 int64_t rows, uint8_t* __restrict__ dst, int64_t pitch) {
+  const int64_t wpr = pitch >> 2, n = rows * wpr;
+  for (int64_t w = (int64_t)blockIdx.x * 256 + threadIdx.x; w < n; w += (int64_t)gridDim.x * 256) {
+    const int64_t r = w / wpr, b0 = (w - r * wpr) * 4;
+    uint32_t v = 0x55555555u;
+    if (r < real) {
+      const uint8_t* __restrict__ src = raw + r * ibytes;
+      v = 0;
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        if (b0 + e < ibytes) v |= (uint32_t)src[b0 + e] << (8 * e);
+    }
+    reinterpret_cast<uint32_t*>(dst + r * pitch)[b0 >> 2] = v;
+  }
+}
+
+void launch_repitch(const uint8_t* raw, int64_t ibytes, int64_t real, int64_t rows, uint8_t* dst, int64_t pitch,
+                    hipStream_t s) {
+  const int64_t n = rows * (pitch >> 2);
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 65536);
+  hipLaunchKernelGGL(k_repitch, dim3((unsigned)blocks), dim3(256), 0, s, raw, ibytes, real, rows, dst, pitch);
+}
+
 // ------------------------------------------------------------------ cross-validation folds
 // Fold of genotype entry (i, j) for an nfolds-fold holdout: the entry counter
 // j*I + i hashed with the fold seed, mapped to [0, nfolds) by a multiply-shift

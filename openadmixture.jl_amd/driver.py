@@ -10,21 +10,33 @@ from .skfr import ImputedSnpMatrix, sparsekmeans1, sparsekmeans2, sparsekmeans_p
 from .structs import AdmixData
 
 
-def hip_admixture_base(d: AdmixData, g_la: SnpLinAlg, I: int, J: int, stream_snps: int = 0):
+def hip_admixture_base(d: AdmixData, g_la: SnpLinAlg, I: int, J: int, stream_snps: int = 0,
+                       stream_from_file: bool = False, resident_snps: int = 0):
     """Counterpart of _cu_admixture_base (src/cuda/structs.jl:33-38): allocate
-    the device mirror and copy the BED payload to HBM once (stream_snps > 0:
-    keep it in pinned host memory and stream it through HBM in chunks of that
-    many SNPs, for cohorts larger than HBM).  Returns (d_cu, g_cu); both are
-    the same device context."""
-    ctx = HIPAdmixData(I, J, d.K, d.Q, d.T, stream_snps=stream_snps)
-    ctx.upload_bed(g_la.s.data)
+    the device mirror and copy the BED payload to HBM once.  stream_snps > 0
+    streams it through HBM in chunks of that many SNPs instead, for cohorts
+    larger than HBM: from pinned host memory, or with stream_from_file from
+    the .bed file itself (cohorts larger than host memory); resident_snps
+    leading SNPs then stay in HBM.  Returns (d_cu, g_cu); both are the same
+    device context."""
+    if stream_from_file:
+        if not stream_snps:
+            raise ValueError("stream_from_file needs stream_snps > 0")
+        ctx = HIPAdmixData(I, J, d.K, d.Q, d.T, stream_snps=stream_snps, bed_file=g_la.s.path,
+                           resident_snps=resident_snps)
+    else:
+        ctx = HIPAdmixData(I, J, d.K, d.Q, d.T, stream_snps=stream_snps)
+        ctx.upload_bed(g_la.s.data)
+        if resident_snps:
+            ctx.set_resident_snps(resident_snps)
     d.hip = ctx
     return ctx, ctx
 
 
 def _admixture_base(filename, K, n_iter=1000, rtol=1e-7, rng=None, em_iters=5, T=np.float64, Q=3,
                     use_gpu=True, verbose=False, progress_bar=False, fix_q=False, fix_p=False,
-                    init_q=None, init_p=None, log=print, mode="ZAL", stream_snps=0):
+                    init_q=None, init_p=None, log=print, mode="ZAL", stream_snps=0, stream_from_file=False,
+                    resident_snps=0):
     if not use_gpu:
         raise ValueError("this package computes on the HIP device only (use_gpu=true)")
     if log:
@@ -35,7 +47,8 @@ def _admixture_base(filename, K, n_iter=1000, rtol=1e-7, rng=None, em_iters=5, T
     if log:
         log(f"Loaded {I} samples and {J} SNPs")
     d = AdmixData(I, J, K, Q, skipmissing=True, rng=rng, T=T)
-    d_cu, g_cu = hip_admixture_base(d, g_la, I, J, stream_snps=stream_snps)
+    d_cu, g_cu = hip_admixture_base(d, g_la, I, J, stream_snps=stream_snps, stream_from_file=stream_from_file,
+                                    resident_snps=resident_snps)
     if init_q is None or init_p is None:
         init_em(d, g_la, em_iters, d_cu=d_cu, g_cu=g_cu, log=log)
     if init_q is not None:
@@ -85,12 +98,13 @@ def _filter_SKFR(filename, K, sparsity, rng=None, prefix=None, tries=10, max_inn
 def run_admixture(filename, K, rng=None, sparsity=None, prefix=None, skfr_tries=1, skfr_max_inner_iter=50,
                   skfr_mode="global", admix_n_iter=1000, admix_rtol=1e-7, admix_n_em_iters=5, T=np.float64, Q=3,
                   use_gpu=True, verbose=False, progress_bar=False, fix_q=False, fix_p=False, init_q=None,
-                  init_p=None, log=print, mode="ZAL", stream_snps=0):
+                  init_p=None, log=print, mode="ZAL", stream_snps=0, stream_from_file=False, resident_snps=0):
     """run_admixture(filename, K; ...)  (src/driver.jl:35-75).  Returns
     (d, clusters, aims); with sparsity given, SKFR first selects that many
     AIMs (skfr_mode "global": sparsekmeans1, "local": sparsekmeans2) and the
     fit runs on the filtered PLINK file prefix_K_{sparsity}aims.bed.  Cluster
-    labels and AIM indices are 0-based."""
+    labels and AIM indices are 0-based.  stream_snps / stream_from_file /
+    resident_snps select J-chunk streaming (see hip_admixture_base)."""
     if not filename.endswith(".bed"):
         raise ValueError("filename should end with .bed")
     if log:
@@ -110,5 +124,6 @@ def run_admixture(filename, K, rng=None, sparsity=None, prefix=None, skfr_tries=
         admix_input, clusters, aims = filename, None, None
     d = _admixture_base(admix_input, K, n_iter=admix_n_iter, rtol=admix_rtol, rng=rng, em_iters=admix_n_em_iters,
                         T=T, Q=Q, use_gpu=use_gpu, verbose=verbose, progress_bar=progress_bar, fix_q=fix_q,
-                        fix_p=fix_p, init_q=init_q, init_p=init_p, log=log, mode=mode, stream_snps=stream_snps)
+                        fix_p=fix_p, init_q=init_q, init_p=init_p, log=log, mode=mode, stream_snps=stream_snps,
+                        stream_from_file=stream_from_file, resident_snps=resident_snps)
     return d, clusters, aims

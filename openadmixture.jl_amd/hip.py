@@ -28,19 +28,31 @@ def _f(a, dtype, shape=None):
 class HIPAdmixData:
     """Device mirror of AdmixData{T} plus the device-resident BED."""
 
-    def __init__(self, I: int, J: int, K: int, Q: int = 3, T=np.float64, stream_snps: int = 0):
+    def __init__(self, I: int, J: int, K: int, Q: int = 3, T=np.float64, stream_snps: int = 0,
+                 bed_file: str | None = None, resident_snps: int = 0):
         """stream_snps > 0: the genotypes stay in pinned host memory and every
         pass streams them through HBM in chunks of that many SNPs (cohorts
-        larger than HBM)."""
+        larger than HBM).  With bed_file as well, they are read from that
+        PLINK .bed file chunk by chunk instead (no host copy; cohorts larger
+        than host memory), and the context needs no upload.  resident_snps:
+        the first that many SNPs of a streamed context stay in HBM."""
         self.I, self.J, self.K, self.Q = int(I), int(J), int(K), int(Q)
         self.T = np.dtype(T)
         self.stream_snps = int(stream_snps)
+        self.bed_file = bed_file
         self._h = C.c_void_p()
-        if self.stream_snps:
+        if bed_file is not None:
+            if not self.stream_snps:
+                raise ValueError("a file-backed context streams: give stream_snps > 0")
+            N.check(N.lib().admx_create_streamed_file(C.byref(self._h), str(bed_file).encode(), self.I, self.J,
+                                                      self.K, self.Q, N.dtype_code(self.T), self.stream_snps))
+        elif self.stream_snps:
             N.check(N.lib().admx_create_streamed(C.byref(self._h), self.I, self.J, self.K, self.Q,
                                                  N.dtype_code(self.T), self.stream_snps))
         else:
             N.check(N.lib().admx_create(C.byref(self._h), self.I, self.J, self.K, self.Q, N.dtype_code(self.T)))
+        if resident_snps:
+            self.set_resident_snps(resident_snps)
 
     # ------------------------------------------------------------ lifetime
     def close(self):
@@ -340,6 +352,10 @@ class HIPAdmixData:
         codes = {"f64": N.ADMX_PREC_F64, "f32": N.ADMX_PREC_F32, "bf16": N.ADMX_PREC_BF16}
         self.set_tuning(N.ADMX_TUNE_PRECISION, codes[mode] if isinstance(mode, str) else int(mode))
         self._precision = mode
+
+    def set_resident_snps(self, n: int):
+        """Streamed context: keep SNPs [0, n) in HBM; passes stream only the rest."""
+        self.set_tuning(N.ADMX_TUNE_RESIDENT_SNPS, int(n))
 
     def set_graphs(self, on: bool = True):
         """Replay steady-state qn_iteration calls from captured hipGraphs."""

@@ -32,6 +32,7 @@ class SnpArray:
         if J is None:
             J = _count_lines(stem + ".bim")
         self.I, self.J = int(I), int(J)
+        self.path = filename
         nb = (self.I + 3) // 4
         size = os.path.getsize(filename)
         if size != 3 + nb * self.J:

@@ -120,6 +120,28 @@ def test_plink_round_trip(oa, tmp_path):
         oa.SnpArray(str(tmp_path / "toy.txt"))
 
 
+def test_streamed_file_context_validates_the_bed(oa, tmp_path):
+    """admx_create_streamed_file checks the file before touching the device:
+    missing file -> ADMX_EIO; wrong magic or a size other than 3 + ceil(I/4)*J
+    -> ADMX_EINVAL; stream_snps is required."""
+    I, J = 13, 9
+    data = np.random.default_rng(2).integers(0, 256, size=((I + 3) // 4, J), dtype=np.uint8)
+    bed = oa.write_plink(str(tmp_path / "toy"), data, I, J)
+    N = oa._native
+    with pytest.raises(oa.AdmixHIPError, match=f"error {N.ADMX_EIO}: cannot open"):
+        oa.HIPAdmixData(I, J, 3, 3, stream_snps=16, bed_file=str(tmp_path / "none.bed"))
+    with pytest.raises(oa.AdmixHIPError, match=f"error {N.ADMX_EINVAL}: .*3 \\+ ceil"):
+        oa.HIPAdmixData(I + 4, J, 3, 3, stream_snps=16, bed_file=bed)
+    with pytest.raises(oa.AdmixHIPError, match=f"error {N.ADMX_EINVAL}: .*at least 16"):
+        oa.HIPAdmixData(I, J, 3, 3, stream_snps=8, bed_file=bed)
+    with pytest.raises(ValueError, match="stream_snps"):
+        oa.HIPAdmixData(I, J, 3, 3, bed_file=bed)
+    with open(bed, "r+b") as f:
+        f.write(b"\x6c\x1b\x00")  # individual-major mode
+    with pytest.raises(oa.AdmixHIPError, match=f"error {N.ADMX_EINVAL}: .*SNP-major"):
+        oa.HIPAdmixData(I, J, 3, 3, stream_snps=16, bed_file=bed)
+
+
 def test_run_admixture_argument_checks(oa, tmp_path):
     with pytest.raises(ValueError):
         oa.run_admixture(str(tmp_path / "x.txt"), 3, log=None)
