@@ -33,6 +33,7 @@ extern "C" {
 #define ADMX_EHIP 2   /* HIP runtime error */
 #define ADMX_ESTATE 3 /* missing state (e.g. no genotypes uploaded) */
 #define ADMX_EIO 4    /* reading a .bed file failed */
+#define ADMX_EHOST 5  /* host-side failure (out of host memory, no thread) */
 
 #define ADMX_F64 0
 #define ADMX_F32 1

@@ -13,7 +13,7 @@ import numpy as np
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libadmixhip.so")
 
-ADMX_OK, ADMX_EINVAL, ADMX_EHIP, ADMX_ESTATE, ADMX_EIO = 0, 1, 2, 3, 4
+ADMX_OK, ADMX_EINVAL, ADMX_EHIP, ADMX_ESTATE, ADMX_EIO, ADMX_EHOST = 0, 1, 2, 3, 4, 5
 ADMX_F64, ADMX_F32 = 0, 1
 ADMX_X, ADMX_X_NEXT, ADMX_X_NEXT2, ADMX_X_TMP = 0, 1, 2, 3
 ADMX_SIDE_Q, ADMX_SIDE_P = 0, 1

@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <exception>
 #include <map>
 #include <mutex>
 #include <string>
@@ -31,6 +32,29 @@ thread_local std::string g_err;
 int fail(int code, const std::string& msg) {
   g_err = msg;
   return code;
+}
+
+// Host-side exceptions (allocation failure, thread creation) never cross the C
+// ABI: every entry point runs inside guarded(), which turns them into
+// ADMX_EHOST with the message in admx_last_error().
+int fail_nothrow(const char* msg) noexcept {
+  try {
+    g_err = msg;
+  } catch (...) {
+    g_err.clear();
+  }
+  return ADMX_EHOST;
+}
+
+template <typename F>
+int guarded(F&& f) noexcept {
+  try {
+    return f();
+  } catch (const std::exception& e) {
+    return fail_nothrow(e.what());
+  } catch (...) {
+    return fail_nothrow("unknown host-side exception");
+  }
 }
 
 #define HIPCHK(expr)                                                                   \
@@ -303,13 +327,15 @@ int read_file_rows(admx_ctx* c, int64_t r0, int64_t r1, uint8_t* dst) {
       a += got;
     }
   };
-  if (nt == 1) {
-    work(0);
-  } else {
-    std::vector<std::thread> th;
-    for (int t = 0; t < nt; ++t) th.emplace_back(work, t);
-    for (auto& t : th) t.join();
+  std::vector<std::thread> th;
+  int started = 1;  // slice 0 runs on the calling thread
+  try {
+    for (; started < nt; ++started) th.emplace_back(work, started);
+  } catch (...) {  // no more threads: the calling thread reads the rest
   }
+  work(0);
+  for (int t = started; t < nt; ++t) work(t);
+  for (auto& t : th) t.join();
   for (int e : err)
     if (e) return fail(ADMX_EIO, std::string("reading the .bed file: ") + std::strerror(e));
   return ADMX_OK;
@@ -1210,11 +1236,13 @@ const char* admx_version(void) { return "libadmixhip 0.1.0 (gfx950)"; }
 const char* admx_last_error(void) { return g_err.c_str(); }
 
 int admx_device_count(int* n) {
-  if (!n) return bad("null pointer");
-  int c = 0;
-  HIPCHK(hipGetDeviceCount(&c));
-  *n = c;
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    if (!n) return bad("null pointer");
+    int c = 0;
+    HIPCHK(hipGetDeviceCount(&c));
+    *n = c;
+    return ADMX_OK;
+  });
 }
 
 }  // extern "C"
@@ -1314,216 +1342,234 @@ int create_impl(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int 
 extern "C" {
 
 int admx_create(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int dtype) {
-  return create_impl(out, I, J, K, Q, dtype, 0);
+  return guarded([&]() -> int {
+    return create_impl(out, I, J, K, Q, dtype, 0);
+  });
 }
 
 int admx_create_streamed(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int dtype,
                          int64_t snps_per_chunk) {
-  if (snps_per_chunk < 16) return bad("a streamed context needs at least 16 SNPs per chunk");
-  return create_impl(out, I, J, K, Q, dtype, std::min<int64_t>(snps_per_chunk, J));
+  return guarded([&]() -> int {
+    if (snps_per_chunk < 16) return bad("a streamed context needs at least 16 SNPs per chunk");
+    return create_impl(out, I, J, K, Q, dtype, std::min<int64_t>(snps_per_chunk, J));
+  });
 }
 
 int admx_create_streamed_file(admx_ctx** out, const char* bed_path, int64_t I, int64_t J, int64_t K, int64_t Q,
                               int dtype, int64_t snps_per_chunk) {
-  if (out) *out = nullptr;
-  if (!bed_path) return bad("null .bed path");
-  if (snps_per_chunk < 16) return bad("a streamed context needs at least 16 SNPs per chunk");
-  return create_impl(out, I, J, K, Q, dtype, std::min<int64_t>(snps_per_chunk, J), bed_path);
+  return guarded([&]() -> int {
+    if (out) *out = nullptr;
+    if (!bed_path) return bad("null .bed path");
+    if (snps_per_chunk < 16) return bad("a streamed context needs at least 16 SNPs per chunk");
+    return create_impl(out, I, J, K, Q, dtype, std::min<int64_t>(snps_per_chunk, J), bed_path);
+  });
 }
 
 int admx_destroy(admx_ctx* c) {
-  if (!c) return ADMX_OK;
-  (void)hipSetDevice(c->device);
-  if (c->s) (void)hipStreamSynchronize(c->s);
-  for (auto& e : c->prof.pend) {
-    (void)hipEventDestroy(e.a);
-    (void)hipEventDestroy(e.b);
-  }
-  for (auto e : c->prof.pool) (void)hipEventDestroy(e);
-  for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
-  c->graphs.clear();
-  void* bufs[] = {c->bed, c->x[0], c->x[1], c->x[2], c->x[3], c->XtX_q, c->Xtz_q, c->XtX_p,
-                  c->Xtz_p, c->U,   c->V,    c->lls,  c->status};
-  for (void* b : bufs)
-    if (b) (void)hipFree(b);
-  if (c->h_lls) (void)hipHostFree(c->h_lls);
-  if (c->cs) (void)hipStreamSynchronize(c->cs);
-  if (c->hbed) (void)hipHostFree(c->hbed);
-  if (c->dres) (void)hipFree(c->dres);
-  if (c->bed_fd >= 0) (void)close(c->bed_fd);
-  for (int b = 0; b < 2; ++b) {
-    if (c->hstage[b]) (void)hipHostFree(c->hstage[b]);
-    if (c->draw[b]) (void)hipFree(c->draw[b]);
-    if (c->dchunk[b]) (void)hipFree(c->dchunk[b]);
-    if (c->ev_ready[b]) (void)hipEventDestroy(c->ev_ready[b]);
-    if (c->ev_free[b]) (void)hipEventDestroy(c->ev_free[b]);
-  }
-  if (c->cs) (void)hipStreamDestroy(c->cs);
-  c->part.release();
-  c->llpart.release();
-  c->zal.release();
-  for (auto& b : c->st) b.release();
-  c->fq.release();
-  c->fp.release();
-  c->ppt.release();
-  c->mmt.release();
-  c->qqt.release();
-  c->lqt.release();
-  c->lpt.release();
-  c->lmt.release();
-  c->feat.release();
-  c->featpart.release();
-  c->snp_mu.release();
-  c->cnt.release();
-  for (DevBuf* b : {&c->sk_lab, &c->sk_cm, &c->sk_small, &c->sk_cnt, &c->sk_part, &c->sk_stat, &c->sk_E, &c->sk_key,
-                    &c->sk_keys, &c->sk_idx, &c->sk_sidx, &c->sk_flag, &c->sk_sel, &c->sk_T, &c->sk_P, &c->sk_vec,
-                    &c->sk_Dc, &c->sk_tmp})
-    b->release();
-  if (c->s) (void)hipStreamDestroy(c->s);
-  delete c;
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    if (!c) return ADMX_OK;
+    (void)hipSetDevice(c->device);
+    if (c->s) (void)hipStreamSynchronize(c->s);
+    for (auto& e : c->prof.pend) {
+      (void)hipEventDestroy(e.a);
+      (void)hipEventDestroy(e.b);
+    }
+    for (auto e : c->prof.pool) (void)hipEventDestroy(e);
+    for (auto& g : c->graphs) (void)hipGraphExecDestroy(g.exec);
+    c->graphs.clear();
+    void* bufs[] = {c->bed, c->x[0], c->x[1], c->x[2], c->x[3], c->XtX_q, c->Xtz_q, c->XtX_p,
+                    c->Xtz_p, c->U,   c->V,    c->lls,  c->status};
+    for (void* b : bufs)
+      if (b) (void)hipFree(b);
+    if (c->h_lls) (void)hipHostFree(c->h_lls);
+    if (c->cs) (void)hipStreamSynchronize(c->cs);
+    if (c->hbed) (void)hipHostFree(c->hbed);
+    if (c->dres) (void)hipFree(c->dres);
+    if (c->bed_fd >= 0) (void)close(c->bed_fd);
+    for (int b = 0; b < 2; ++b) {
+      if (c->hstage[b]) (void)hipHostFree(c->hstage[b]);
+      if (c->draw[b]) (void)hipFree(c->draw[b]);
+      if (c->dchunk[b]) (void)hipFree(c->dchunk[b]);
+      if (c->ev_ready[b]) (void)hipEventDestroy(c->ev_ready[b]);
+      if (c->ev_free[b]) (void)hipEventDestroy(c->ev_free[b]);
+    }
+    if (c->cs) (void)hipStreamDestroy(c->cs);
+    c->part.release();
+    c->llpart.release();
+    c->zal.release();
+    for (auto& b : c->st) b.release();
+    c->fq.release();
+    c->fp.release();
+    c->ppt.release();
+    c->mmt.release();
+    c->qqt.release();
+    c->lqt.release();
+    c->lpt.release();
+    c->lmt.release();
+    c->feat.release();
+    c->featpart.release();
+    c->snp_mu.release();
+    c->cnt.release();
+    for (DevBuf* b : {&c->sk_lab, &c->sk_cm, &c->sk_small, &c->sk_cnt, &c->sk_part, &c->sk_stat, &c->sk_E, &c->sk_key,
+                      &c->sk_keys, &c->sk_idx, &c->sk_sidx, &c->sk_flag, &c->sk_sel, &c->sk_T, &c->sk_P, &c->sk_vec,
+                      &c->sk_Dc, &c->sk_tmp})
+      b->release();
+    if (c->s) (void)hipStreamDestroy(c->s);
+    delete c;
+    return ADMX_OK;
+  });
 }
 
 int admx_upload_bed(admx_ctx* c, const uint8_t* bed, int64_t bytes_per_snp) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  if (!bed) return bad("null BED pointer");
-  if (bytes_per_snp < c->ibytes) return bad("bytes_per_snp < ceil(I/4)");
-  if (c->bed_fd >= 0) return fail(ADMX_ESTATE, "a file-backed context reads its genotypes from the .bed file");
-  if (c->stream_snps) {  // host copy into the pinned BED; the padding rows stay missing
-    HIPCHK(hipStreamSynchronize(c->s));  // no copy out of the pinned BED is still in flight
-    HIPCHK(hipStreamSynchronize(c->cs));
-    for (int64_t j = 0; j < c->J; ++j) {
-      uint8_t* row = c->hbed + (size_t)c->pitch * j;
-      std::memcpy(row, bed + (size_t)bytes_per_snp * j, (size_t)c->ibytes);
-      std::memset(row + c->ibytes, 0, (size_t)(c->pitch - c->ibytes));
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (!bed) return bad("null BED pointer");
+    if (bytes_per_snp < c->ibytes) return bad("bytes_per_snp < ceil(I/4)");
+    if (c->bed_fd >= 0) return fail(ADMX_ESTATE, "a file-backed context reads its genotypes from the .bed file");
+    if (c->stream_snps) {  // host copy into the pinned BED; the padding rows stay missing
+      HIPCHK(hipStreamSynchronize(c->s));  // no copy out of the pinned BED is still in flight
+      HIPCHK(hipStreamSynchronize(c->cs));
+      for (int64_t j = 0; j < c->J; ++j) {
+        uint8_t* row = c->hbed + (size_t)c->pitch * j;
+        std::memcpy(row, bed + (size_t)bytes_per_snp * j, (size_t)c->ibytes);
+        std::memset(row + c->ibytes, 0, (size_t)(c->pitch - c->ibytes));
+      }
+      c->have_bed = true;
+      c->mu_valid = false;
+      forget_chunks(c);
+      return refresh_resident(c);
     }
-    c->have_bed = true;
-    c->mu_valid = false;
-    forget_chunks(c);
-    return refresh_resident(c);
-  }
-  HIPCHK(hipMemsetAsync(c->bed, 0, (size_t)c->pitch * c->J, c->s));
-  CHK(pad_bed(c));
-  HIPCHK(hipMemcpy2DAsync(c->bed, c->pitch, bed, bytes_per_snp, c->ibytes, c->J, hipMemcpyHostToDevice, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  c->have_bed = true;
-  c->mu_valid = false;
-  return ADMX_OK;
-}
-
-int admx_download_bed(admx_ctx* c, uint8_t* bed) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  if (!bed) return bad("null BED pointer");
-  if (c->bed_fd >= 0) return read_file_rows(c, 0, c->J, bed);
-  if (c->stream_snps) {
-    for (int64_t j = 0; j < c->J; ++j)
-      std::memcpy(bed + (size_t)c->ibytes * j, c->hbed + (size_t)c->pitch * j, (size_t)c->ibytes);
-    return ADMX_OK;
-  }
-  HIPCHK(hipMemcpy2DAsync(bed, c->ibytes, c->bed, c->pitch, c->ibytes, c->J, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
-}
-
-int admx_generate_bed(admx_ctx* c, const double* Qt, const double* Pt, uint64_t seed, double miss_rate) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  if (!Qt || !Pt) return bad("null Q or P");
-  if (c->bed_fd >= 0) return fail(ADMX_ESTATE, "a file-backed context reads its genotypes from the .bed file");
-  const size_t qb = (size_t)(c->K * c->I) * sizeof(double), pb = (size_t)(c->K * c->J) * sizeof(double);
-  CHK(stage_in(c, 0, Qt, qb, qb));
-  CHK(stage_in(c, 1, Pt, pb, pb));
-  if (c->stream_snps) {  // chunk by chunk through a device buffer into the pinned BED
-    const int64_t S = c->stream_snps;
-    for (int64_t j0 = 0; j0 < c->J; j0 += S) {
-      const int64_t j1 = std::min(c->J, j0 + S);
-      launch_gen_bed_rows(static_cast<const double*>(c->st[0].p), static_cast<const double*>(c->st[1].p), c->I, j0,
-                          j1, (int)c->K, seed, miss_rate, c->dchunk[0] - c->pitch * j0, c->pitch, c->s);
-      CHK(check_launch());
-      HIPCHK(hipMemcpyAsync(c->hbed + (size_t)c->pitch * j0, c->dchunk[0], (size_t)(c->pitch * (j1 - j0)),
-                            hipMemcpyDeviceToHost, c->s));
-    }
+    HIPCHK(hipMemsetAsync(c->bed, 0, (size_t)c->pitch * c->J, c->s));
+    CHK(pad_bed(c));
+    HIPCHK(hipMemcpy2DAsync(c->bed, c->pitch, bed, bytes_per_snp, c->ibytes, c->J, hipMemcpyHostToDevice, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     c->have_bed = true;
     c->mu_valid = false;
-    forget_chunks(c);
-    return refresh_resident(c);
-  }
-  {
-    ProfScope ps(c, "gen_bed");
-    launch_gen_bed(static_cast<const double*>(c->st[0].p), static_cast<const double*>(c->st[1].p), c->I, c->J,
-                   (int)c->K, seed, miss_rate, c->bed, c->pitch, c->s);
-  }
-  CHK(pad_bed(c));
-  CHK(check_launch());
-  HIPCHK(hipStreamSynchronize(c->s));
-  c->have_bed = true;
-  c->mu_valid = false;
-  return ADMX_OK;
+    return ADMX_OK;
+  });
+}
+
+int admx_download_bed(admx_ctx* c, uint8_t* bed) {
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    if (!bed) return bad("null BED pointer");
+    if (c->bed_fd >= 0) return read_file_rows(c, 0, c->J, bed);
+    if (c->stream_snps) {
+      for (int64_t j = 0; j < c->J; ++j)
+        std::memcpy(bed + (size_t)c->ibytes * j, c->hbed + (size_t)c->pitch * j, (size_t)c->ibytes);
+      return ADMX_OK;
+    }
+    HIPCHK(hipMemcpy2DAsync(bed, c->ibytes, c->bed, c->pitch, c->ibytes, c->J, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
+}
+
+int admx_generate_bed(admx_ctx* c, const double* Qt, const double* Pt, uint64_t seed, double miss_rate) {
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (!Qt || !Pt) return bad("null Q or P");
+    if (c->bed_fd >= 0) return fail(ADMX_ESTATE, "a file-backed context reads its genotypes from the .bed file");
+    const size_t qb = (size_t)(c->K * c->I) * sizeof(double), pb = (size_t)(c->K * c->J) * sizeof(double);
+    CHK(stage_in(c, 0, Qt, qb, qb));
+    CHK(stage_in(c, 1, Pt, pb, pb));
+    if (c->stream_snps) {  // chunk by chunk through a device buffer into the pinned BED
+      const int64_t S = c->stream_snps;
+      for (int64_t j0 = 0; j0 < c->J; j0 += S) {
+        const int64_t j1 = std::min(c->J, j0 + S);
+        launch_gen_bed_rows(static_cast<const double*>(c->st[0].p), static_cast<const double*>(c->st[1].p), c->I, j0,
+                            j1, (int)c->K, seed, miss_rate, c->dchunk[0] - c->pitch * j0, c->pitch, c->s);
+        CHK(check_launch());
+        HIPCHK(hipMemcpyAsync(c->hbed + (size_t)c->pitch * j0, c->dchunk[0], (size_t)(c->pitch * (j1 - j0)),
+                              hipMemcpyDeviceToHost, c->s));
+      }
+      HIPCHK(hipStreamSynchronize(c->s));
+      c->have_bed = true;
+      c->mu_valid = false;
+      forget_chunks(c);
+      return refresh_resident(c);
+    }
+    {
+      ProfScope ps(c, "gen_bed");
+      launch_gen_bed(static_cast<const double*>(c->st[0].p), static_cast<const double*>(c->st[1].p), c->I, c->J,
+                     (int)c->K, seed, miss_rate, c->bed, c->pitch, c->s);
+    }
+    CHK(pad_bed(c));
+    CHK(check_launch());
+    HIPCHK(hipStreamSynchronize(c->s));
+    c->have_bed = true;
+    c->mu_valid = false;
+    return ADMX_OK;
+  });
 }
 
 int admx_bed_from(admx_ctx* dst, admx_ctx* src, int32_t fold, int32_t nfolds, uint64_t seed) {
-  CHK(check_ctx(dst));
-  CHK(check_ctx(src));
-  if (dst == src) return bad("source and destination must be different contexts");
-  Guard2 g(dst, src);
-  CHK(need_bed(src));
-  CHK(need_resident(src, "fold masking"));
-  CHK(need_resident(dst, "fold masking"));
-  if (dst->I != src->I || dst->J != src->J) return bad("contexts differ in I or J");
-  if (fold >= 0 && (nfolds < 2 || fold >= nfolds)) return bad("need 0 <= fold < nfolds and nfolds >= 2");
-  if (dst->device != src->device) return bad("contexts must live on the same device");
-  HIPCHK(hipStreamSynchronize(src->s));
-  HIPCHK(hipSetDevice(dst->device));
-  {
-    ProfScope ps(dst, "mask_fold");
-    launch_mask_fold(src->bed, dst->bed, dst->pitch, dst->I, dst->J, fold, fold >= 0 ? nfolds : 1, seed, dst->s);
-  }
-  CHK(check_launch());
-  CHK(pad_bed(dst));
-  HIPCHK(hipStreamSynchronize(dst->s));
-  dst->have_bed = true;
-  dst->mu_valid = false;
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(dst));
+    CHK(check_ctx(src));
+    if (dst == src) return bad("source and destination must be different contexts");
+    Guard2 g(dst, src);
+    CHK(need_bed(src));
+    CHK(need_resident(src, "fold masking"));
+    CHK(need_resident(dst, "fold masking"));
+    if (dst->I != src->I || dst->J != src->J) return bad("contexts differ in I or J");
+    if (fold >= 0 && (nfolds < 2 || fold >= nfolds)) return bad("need 0 <= fold < nfolds and nfolds >= 2");
+    if (dst->device != src->device) return bad("contexts must live on the same device");
+    HIPCHK(hipStreamSynchronize(src->s));
+    HIPCHK(hipSetDevice(dst->device));
+    {
+      ProfScope ps(dst, "mask_fold");
+      launch_mask_fold(src->bed, dst->bed, dst->pitch, dst->I, dst->J, fold, fold >= 0 ? nfolds : 1, seed, dst->s);
+    }
+    CHK(check_launch());
+    CHK(pad_bed(dst));
+    HIPCHK(hipStreamSynchronize(dst->s));
+    dst->have_bed = true;
+    dst->mu_valid = false;
+    return ADMX_OK;
+  });
 }
 
 int admx_heldout_deviance(admx_ctx* fit, admx_ctx* src, int32_t fold, int32_t nfolds, uint64_t seed,
                           double* deviance_sum, int64_t* count) {
-  CHK(check_ctx(fit));
-  CHK(check_ctx(src));
-  Guard2 g(fit, src);
-  CHK(need_bed(src));
-  CHK(need_resident(src, "the held-out deviance"));
-  if (!deviance_sum || !count) return bad("null output");
-  if (fit->I != src->I || fit->J != src->J) return bad("contexts differ in I or J");
-  if (fit->dtype != ADMX_F64) return bad("held-out deviance needs a FP64 fit context");
-  if (nfolds < 2 || fold < 0 || fold >= nfolds) return bad("need 0 <= fold < nfolds and nfolds >= 2");
-  if (fit->device != src->device) return bad("contexts must live on the same device");
-  if (src != fit) HIPCHK(hipStreamSynchronize(src->s));
-  HIPCHK(hipSetDevice(fit->device));
-  int64_t jchunk = 0;
-  const int64_t nb = heldout_dev_blocks(fit->I, fit->J, &jchunk);
-  CHK(fit->st[0].grow(sizeof(double) * 2 * nb + 2 * sizeof(double)));
-  double* part = static_cast<double*>(fit->st[0].p);
-  double* tot = part + 2 * nb;
-  const double* q = static_cast<const double*>(fit->x[ADMX_X]);
-  {
-    ProfScope ps(fit, "heldout_dev");
-    launch_heldout_dev(src->bed, src->pitch, fit->I, fit->J, (int)fit->K, q, q + fit->K * fit->I, fold, nfolds, seed,
-                       part, fit->s);
-    launch_reduce_sum(part, nb, tot, 0, fit->s);
-    launch_reduce_sum(part + nb, nb, tot + 1, 0, fit->s);
-  }
-  CHK(check_launch());
-  double h[2];
-  HIPCHK(hipMemcpyAsync(h, tot, sizeof(h), hipMemcpyDeviceToHost, fit->s));
-  HIPCHK(hipStreamSynchronize(fit->s));
-  *deviance_sum = h[0];
-  *count = (int64_t)h[1];
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(fit));
+    CHK(check_ctx(src));
+    Guard2 g(fit, src);
+    CHK(need_bed(src));
+    CHK(need_resident(src, "the held-out deviance"));
+    if (!deviance_sum || !count) return bad("null output");
+    if (fit->I != src->I || fit->J != src->J) return bad("contexts differ in I or J");
+    if (fit->dtype != ADMX_F64) return bad("held-out deviance needs a FP64 fit context");
+    if (nfolds < 2 || fold < 0 || fold >= nfolds) return bad("need 0 <= fold < nfolds and nfolds >= 2");
+    if (fit->device != src->device) return bad("contexts must live on the same device");
+    if (src != fit) HIPCHK(hipStreamSynchronize(src->s));
+    HIPCHK(hipSetDevice(fit->device));
+    int64_t jchunk = 0;
+    const int64_t nb = heldout_dev_blocks(fit->I, fit->J, &jchunk);
+    CHK(fit->st[0].grow(sizeof(double) * 2 * nb + 2 * sizeof(double)));
+    double* part = static_cast<double*>(fit->st[0].p);
+    double* tot = part + 2 * nb;
+    const double* q = static_cast<const double*>(fit->x[ADMX_X]);
+    {
+      ProfScope ps(fit, "heldout_dev");
+      launch_heldout_dev(src->bed, src->pitch, fit->I, fit->J, (int)fit->K, q, q + fit->K * fit->I, fold, nfolds, seed,
+                         part, fit->s);
+      launch_reduce_sum(part, nb, tot, 0, fit->s);
+      launch_reduce_sum(part + nb, nb, tot + 1, 0, fit->s);
+    }
+    CHK(check_launch());
+    double h[2];
+    HIPCHK(hipMemcpyAsync(h, tot, sizeof(h), hipMemcpyDeviceToHost, fit->s));
+    HIPCHK(hipStreamSynchronize(fit->s));
+    *deviance_sum = h[0];
+    *count = (int64_t)h[1];
+    return ADMX_OK;
+  });
 }
 
 }  // extern "C"
@@ -1630,754 +1676,840 @@ double skfr_u(uint64_t seed, int cc) {
 extern "C" {
 
 int admx_skfr_init(admx_ctx* c, int32_t k, uint64_t seed, int32_t nsplit, int32_t* clusters) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(need_resident(c, "SKFR"));
-  if (!clusters) return bad("null output");
-  if (k < 1 || k > kMaxK) return bad("need 1 <= k <= 16");
-  if (k > c->I) return bad("more clusters than individuals");
-  if (nsplit < 0 || nsplit > 64) return bad("need 0 <= nsplit <= 64");
-  if (c->I >= (1LL << 31) || c->J >= (1LL << 31)) return bad("SKFR needs I, J < 2^31");
-  SkfrView v;
-  const int ns = skfr_split(c, nsplit, c->J, 1);
-  CHK(skfr_prepare(c, k, 0, (int64_t)ns * c->I, true, &v));
-  {
-    ProfScope ps(c, "skfr_kmeanspp");
-    for (int cc = 0; cc < k; ++cc) {
-      launch_skfr_pick(v.D2, c->I, skfr_u(seed, cc), cc == 0, v.center, c->s);
-      launch_skfr_pp_table(c->bed, c->pitch, v.center, v.mu, v.var, c->J, v.T, c->s);
-      launch_skfr_tabsum(1, c->bed, c->pitch, c->I, nullptr, c->J, ns, v.T, v.P, c->s);
-      launch_skfr_combine(v.P, ns, c->I, v.Dc + (int64_t)cc * c->I, v.D2, cc == 0, c->s);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(need_resident(c, "SKFR"));
+    if (!clusters) return bad("null output");
+    if (k < 1 || k > kMaxK) return bad("need 1 <= k <= 16");
+    if (k > c->I) return bad("more clusters than individuals");
+    if (nsplit < 0 || nsplit > 64) return bad("need 0 <= nsplit <= 64");
+    if (c->I >= (1LL << 31) || c->J >= (1LL << 31)) return bad("SKFR needs I, J < 2^31");
+    SkfrView v;
+    const int ns = skfr_split(c, nsplit, c->J, 1);
+    CHK(skfr_prepare(c, k, 0, (int64_t)ns * c->I, true, &v));
+    {
+      ProfScope ps(c, "skfr_kmeanspp");
+      for (int cc = 0; cc < k; ++cc) {
+        launch_skfr_pick(v.D2, c->I, skfr_u(seed, cc), cc == 0, v.center, c->s);
+        launch_skfr_pp_table(c->bed, c->pitch, v.center, v.mu, v.var, c->J, v.T, c->s);
+        launch_skfr_tabsum(1, c->bed, c->pitch, c->I, nullptr, c->J, ns, v.T, v.P, c->s);
+        launch_skfr_combine(v.P, ns, c->I, v.Dc + (int64_t)cc * c->I, v.D2, cc == 0, c->s);
+      }
+      launch_skfr_argmin(v.Dc, k, c->I, v.lab, c->s);
     }
-    launch_skfr_argmin(v.Dc, k, c->I, v.lab, c->s);
-  }
-  CHK(check_launch());
-  HIPCHK(hipMemcpyAsync(clusters, v.lab, sizeof(int32_t) * c->I, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+    CHK(check_launch());
+    HIPCHK(hipMemcpyAsync(clusters, v.lab, sizeof(int32_t) * c->I, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_skfr_run(admx_ctx* c, int32_t k, int32_t mode, int64_t sparsity, int32_t max_iter, int32_t nsplit,
                   int32_t* clusters, int64_t* aims, int64_t* n_aims, double* centers, double* wss, double* obj,
                   int32_t* iters) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(need_resident(c, "SKFR"));
-  if (!clusters || !aims || !n_aims) return bad("null clusters or aims");
-  if (k < 1 || k > kMaxK) return bad("need 1 <= k <= 16");
-  if (mode != ADMX_SKFR_GLOBAL && mode != ADMX_SKFR_LOCAL) return bad("mode must be ADMX_SKFR_GLOBAL or _LOCAL");
-  if (sparsity < 1 || sparsity > c->J) return bad("need 1 <= sparsity <= J");
-  if (max_iter < 1) return bad("need max_iter >= 1");
-  if (nsplit < 0 || nsplit > 64) return bad("need 0 <= nsplit <= 64");
-  if (c->I >= (1LL << 31) || c->J >= (1LL << 31)) return bad("SKFR needs I, J < 2^31");
-  for (int64_t i = 0; i < c->I; ++i)
-    if (clusters[i] < 0 || clusters[i] >= k) return bad("cluster labels must lie in [0, k)");
-  const bool local = mode == ADMX_SKFR_LOCAL;
-  const int64_t I = c->I, J = c->J;
-  const int64_t ncols_max = local ? std::min<int64_t>(J, (int64_t)k * sparsity) : sparsity;
-  // slices are nondecreasing in the column count
-  const int64_t p_elems = std::max<int64_t>((int64_t)skfr_split(c, nsplit, J, 1) * I,
-                                            (int64_t)skfr_split(c, nsplit, ncols_max, k) * k * I);
-  SkfrView v;
-  CHK(skfr_prepare(c, k, ncols_max, p_elems, false, &v));
-  HIPCHK(hipMemcpyAsync(v.lab, clusters, sizeof(int32_t) * I, hipMemcpyHostToDevice, c->s));
-  {
-    // sum_j x_ij^2 of every individual (the WSS term common to all clusters)
-    ProfScope ps(c, "skfr_x2");
-    const int ns = skfr_split(c, nsplit, J, 1);
-    launch_skfr_pp_table(c->bed, c->pitch, nullptr, v.mu, v.var, J, v.T, c->s);
-    launch_skfr_tabsum(1, c->bed, c->pitch, I, nullptr, J, ns, v.T, v.P, c->s);
-    launch_skfr_combine(v.P, ns, I, v.x2, nullptr, 0, c->s);
-  }
-  CHK(check_launch());
-  int it = 0;
-  int64_t ncols = 0;
-  for (it = 1; it <= max_iter; ++it) {
-    HIPCHK(hipMemsetAsync(v.nk, 0, sizeof(unsigned) * kMaxK, c->s));
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(need_resident(c, "SKFR"));
+    if (!clusters || !aims || !n_aims) return bad("null clusters or aims");
+    if (k < 1 || k > kMaxK) return bad("need 1 <= k <= 16");
+    if (mode != ADMX_SKFR_GLOBAL && mode != ADMX_SKFR_LOCAL) return bad("mode must be ADMX_SKFR_GLOBAL or _LOCAL");
+    if (sparsity < 1 || sparsity > c->J) return bad("need 1 <= sparsity <= J");
+    if (max_iter < 1) return bad("need max_iter >= 1");
+    if (nsplit < 0 || nsplit > 64) return bad("need 0 <= nsplit <= 64");
+    if (c->I >= (1LL << 31) || c->J >= (1LL << 31)) return bad("SKFR needs I, J < 2^31");
+    for (int64_t i = 0; i < c->I; ++i)
+      if (clusters[i] < 0 || clusters[i] >= k) return bad("cluster labels must lie in [0, k)");
+    const bool local = mode == ADMX_SKFR_LOCAL;
+    const int64_t I = c->I, J = c->J;
+    const int64_t ncols_max = local ? std::min<int64_t>(J, (int64_t)k * sparsity) : sparsity;
+    // slices are nondecreasing in the column count
+    const int64_t p_elems = std::max<int64_t>((int64_t)skfr_split(c, nsplit, J, 1) * I,
+                                              (int64_t)skfr_split(c, nsplit, ncols_max, k) * k * I);
+    SkfrView v;
+    CHK(skfr_prepare(c, k, ncols_max, p_elems, false, &v));
+    HIPCHK(hipMemcpyAsync(v.lab, clusters, sizeof(int32_t) * I, hipMemcpyHostToDevice, c->s));
     {
-      ProfScope ps(c, "skfr_masks");
-      launch_skfr_masks(v.lab, I, k, v.nwords, v.cm, v.nk, c->s);
-    }
-    {
-      ProfScope ps(c, "skfr_centers");
-      launch_skfr_centers(k, c->bed, c->pitch, J, v.ntiles, v.tper, v.nslab, v.cm, v.part, c->s);
-    }
-    {
-      ProfScope ps(c, "skfr_score");
-      launch_skfr_score(k, local, v.part, v.nslab, v.nk, v.nm, v.alt, I, v.mu, v.var, J, v.E, v.key, v.idx, c->s);
-    }
-    {
-      ProfScope ps(c, "skfr_select");
-      if (!local) {
-        launch_skfr_sort(v.tmp, v.tmp_bytes, v.key, v.keys, v.idx, v.sidx, J, c->s);
-        ncols = sparsity;
-      } else {
-        for (int cc = 0; cc < k; ++cc)
-          launch_skfr_sort(v.tmp, v.tmp_bytes, v.key + (int64_t)cc * J, v.keys + (int64_t)cc * J, v.idx,
-                           v.sidx + (int64_t)cc * J, J, c->s);
-        HIPCHK(hipMemsetAsync(v.flag, 0, (size_t)k * J, c->s));
-        HIPCHK(hipMemsetAsync(v.count, 0, sizeof(unsigned long long), c->s));
-        launch_skfr_flag(v.sidx, J, sparsity, k, v.flag, c->s);
-        launch_skfr_union(v.flag, v.key, J, k, v.ukey, v.idx, v.count, c->s);
-        launch_skfr_sort(v.tmp, v.tmp_bytes, v.ukey, v.keys, v.idx, v.sidx, J, c->s);
-      }
+      // sum_j x_ij^2 of every individual (the WSS term common to all clusters)
+      ProfScope ps(c, "skfr_x2");
+      const int ns = skfr_split(c, nsplit, J, 1);
+      launch_skfr_pp_table(c->bed, c->pitch, nullptr, v.mu, v.var, J, v.T, c->s);
+      launch_skfr_tabsum(1, c->bed, c->pitch, I, nullptr, J, ns, v.T, v.P, c->s);
+      launch_skfr_combine(v.P, ns, I, v.x2, nullptr, 0, c->s);
     }
     CHK(check_launch());
-    if (local) {
+    int it = 0;
+    int64_t ncols = 0;
+    for (it = 1; it <= max_iter; ++it) {
+      HIPCHK(hipMemsetAsync(v.nk, 0, sizeof(unsigned) * kMaxK, c->s));
+      {
+        ProfScope ps(c, "skfr_masks");
+        launch_skfr_masks(v.lab, I, k, v.nwords, v.cm, v.nk, c->s);
+      }
+      {
+        ProfScope ps(c, "skfr_centers");
+        launch_skfr_centers(k, c->bed, c->pitch, J, v.ntiles, v.tper, v.nslab, v.cm, v.part, c->s);
+      }
+      {
+        ProfScope ps(c, "skfr_score");
+        launch_skfr_score(k, local, v.part, v.nslab, v.nk, v.nm, v.alt, I, v.mu, v.var, J, v.E, v.key, v.idx, c->s);
+      }
+      {
+        ProfScope ps(c, "skfr_select");
+        if (!local) {
+          launch_skfr_sort(v.tmp, v.tmp_bytes, v.key, v.keys, v.idx, v.sidx, J, c->s);
+          ncols = sparsity;
+        } else {
+          for (int cc = 0; cc < k; ++cc)
+            launch_skfr_sort(v.tmp, v.tmp_bytes, v.key + (int64_t)cc * J, v.keys + (int64_t)cc * J, v.idx,
+                             v.sidx + (int64_t)cc * J, J, c->s);
+          HIPCHK(hipMemsetAsync(v.flag, 0, (size_t)k * J, c->s));
+          HIPCHK(hipMemsetAsync(v.count, 0, sizeof(unsigned long long), c->s));
+          launch_skfr_flag(v.sidx, J, sparsity, k, v.flag, c->s);
+          launch_skfr_union(v.flag, v.key, J, k, v.ukey, v.idx, v.count, c->s);
+          launch_skfr_sort(v.tmp, v.tmp_bytes, v.ukey, v.keys, v.idx, v.sidx, J, c->s);
+        }
+      }
+      CHK(check_launch());
+      if (local) {
+        unsigned long long h = 0;
+        HIPCHK(hipMemcpyAsync(&h, v.count, sizeof(h), hipMemcpyDeviceToHost, c->s));
+        HIPCHK(hipStreamSynchronize(c->s));
+        ncols = (int64_t)h;
+        if (ncols < 1 || ncols > ncols_max) return fail(ADMX_EHIP, "SKFR: inconsistent union of local selections");
+      }
+      const int ns = skfr_split(c, nsplit, ncols, k);
+      HIPCHK(hipMemsetAsync(v.changes, 0, sizeof(unsigned long long), c->s));
+      {
+        ProfScope ps(c, "skfr_tables");
+        launch_skfr_tables(v.sidx, ncols, k, local ? v.flag : nullptr, v.nk, v.mu, v.var, v.E, J, v.T, c->s);
+      }
+      {
+        ProfScope ps(c, "skfr_tabsum");
+        launch_skfr_tabsum(k, c->bed, c->pitch, I, v.sidx, ncols, ns, v.T, v.P, c->s);
+      }
+      {
+        ProfScope ps(c, "skfr_assign");
+        launch_skfr_assign(v.P, ns, k, I, v.nk, v.lab, v.dmin, v.changes, c->s);
+      }
+      CHK(check_launch());
       unsigned long long h = 0;
-      HIPCHK(hipMemcpyAsync(&h, v.count, sizeof(h), hipMemcpyDeviceToHost, c->s));
+      HIPCHK(hipMemcpyAsync(&h, v.changes, sizeof(h), hipMemcpyDeviceToHost, c->s));
       HIPCHK(hipStreamSynchronize(c->s));
-      ncols = (int64_t)h;
-      if (ncols < 1 || ncols > ncols_max) return fail(ADMX_EHIP, "SKFR: inconsistent union of local selections");
+      if (h == 0) break;
     }
-    const int ns = skfr_split(c, nsplit, ncols, k);
-    HIPCHK(hipMemsetAsync(v.changes, 0, sizeof(unsigned long long), c->s));
+    if (it > max_iter) it = max_iter;
     {
-      ProfScope ps(c, "skfr_tables");
-      launch_skfr_tables(v.sidx, ncols, k, local ? v.flag : nullptr, v.nk, v.mu, v.var, v.E, J, v.T, c->s);
-    }
-    {
-      ProfScope ps(c, "skfr_tabsum");
-      launch_skfr_tabsum(k, c->bed, c->pitch, I, v.sidx, ncols, ns, v.T, v.P, c->s);
-    }
-    {
-      ProfScope ps(c, "skfr_assign");
-      launch_skfr_assign(v.P, ns, k, I, v.nk, v.lab, v.dmin, v.changes, c->s);
+      ProfScope ps(c, "skfr_wss");
+      launch_skfr_wss(v.lab, v.x2, v.dmin, I, k, v.out, c->s);
+      if (centers) {
+        if (!local) {
+          HIPCHK(hipMemsetAsync(v.sel, 0, (size_t)J, c->s));
+          launch_skfr_mark(v.sidx, ncols, v.sel, c->s);
+        }
+        // the distance tables are no longer needed: reuse them for the centers
+        launch_skfr_center_out(v.sel, local ? v.flag : nullptr, v.nk, v.var, v.E, J, k, v.T, c->s);
+      }
     }
     CHK(check_launch());
-    unsigned long long h = 0;
-    HIPCHK(hipMemcpyAsync(&h, v.changes, sizeof(h), hipMemcpyDeviceToHost, c->s));
+    std::vector<int32_t> h_aims((size_t)ncols);
+    std::vector<double> h_out((size_t)k + 1);
+    HIPCHK(hipMemcpyAsync(clusters, v.lab, sizeof(int32_t) * I, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(h_aims.data(), v.sidx, sizeof(int32_t) * ncols, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(h_out.data(), v.out, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, c->s));
+    if (centers) HIPCHK(hipMemcpyAsync(centers, v.T, sizeof(double) * k * J, hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
-    if (h == 0) break;
-  }
-  if (it > max_iter) it = max_iter;
-  {
-    ProfScope ps(c, "skfr_wss");
-    launch_skfr_wss(v.lab, v.x2, v.dmin, I, k, v.out, c->s);
-    if (centers) {
-      if (!local) {
-        HIPCHK(hipMemsetAsync(v.sel, 0, (size_t)J, c->s));
-        launch_skfr_mark(v.sidx, ncols, v.sel, c->s);
-      }
-      // the distance tables are no longer needed: reuse them for the centers
-      launch_skfr_center_out(v.sel, local ? v.flag : nullptr, v.nk, v.var, v.E, J, k, v.T, c->s);
-    }
-  }
-  CHK(check_launch());
-  std::vector<int32_t> h_aims((size_t)ncols);
-  std::vector<double> h_out((size_t)k + 1);
-  HIPCHK(hipMemcpyAsync(clusters, v.lab, sizeof(int32_t) * I, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipMemcpyAsync(h_aims.data(), v.sidx, sizeof(int32_t) * ncols, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipMemcpyAsync(h_out.data(), v.out, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, c->s));
-  if (centers) HIPCHK(hipMemcpyAsync(centers, v.T, sizeof(double) * k * J, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  for (int64_t t = 0; t < ncols; ++t) aims[t] = h_aims[(size_t)t];
-  *n_aims = ncols;
-  if (wss)
-    for (int cc = 0; cc < k; ++cc) wss[cc] = h_out[(size_t)cc];
-  if (obj) *obj = h_out[(size_t)k];
-  if (iters) *iters = it;
-  return ADMX_OK;
+    for (int64_t t = 0; t < ncols; ++t) aims[t] = h_aims[(size_t)t];
+    *n_aims = ncols;
+    if (wss)
+      for (int cc = 0; cc < k; ++cc) wss[cc] = h_out[(size_t)cc];
+    if (obj) *obj = h_out[(size_t)k];
+    if (iters) *iters = it;
+    return ADMX_OK;
+  });
 }
 
 int admx_decode(admx_ctx* c, int8_t* gout) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(need_resident(c, "decode"));
-  if (!gout) return bad("null output");
-  const size_t bytes = (size_t)(c->I * c->J);
-  CHK(c->st[0].grow(bytes));
-  {
-    ProfScope ps(c, "decode");
-    launch_decode(c->bed, c->pitch, c->I, c->J, static_cast<int8_t*>(c->st[0].p), c->s);
-  }
-  CHK(check_launch());
-  HIPCHK(hipMemcpyAsync(gout, c->st[0].p, bytes, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(need_resident(c, "decode"));
+    if (!gout) return bad("null output");
+    const size_t bytes = (size_t)(c->I * c->J);
+    CHK(c->st[0].grow(bytes));
+    {
+      ProfScope ps(c, "decode");
+      launch_decode(c->bed, c->pitch, c->I, c->J, static_cast<int8_t*>(c->st[0].p), c->s);
+    }
+    CHK(check_launch());
+    HIPCHK(hipMemcpyAsync(gout, c->st[0].p, bytes, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_counts(admx_ctx* c, int64_t* snp_nm, int64_t* snp_alt, int64_t* ind_nm) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(need_resident(c, "counts"));
-  if (!snp_nm || !snp_alt || !ind_nm) return bad("null output");
-  const size_t jb = sizeof(unsigned long long) * c->J, ib = sizeof(unsigned long long) * c->I;
-  CHK(c->st[0].grow(jb));
-  CHK(c->st[1].grow(jb));
-  CHK(c->st[2].grow(ib));
-  HIPCHK(hipMemsetAsync(c->st[2].p, 0, ib, c->s));
-  {
-    ProfScope ps(c, "counts");
-    launch_counts(c->bed, c->pitch, c->I, c->J, static_cast<unsigned long long*>(c->st[0].p),
-                  static_cast<unsigned long long*>(c->st[1].p), static_cast<unsigned long long*>(c->st[2].p), c->s);
-  }
-  CHK(check_launch());
-  HIPCHK(hipMemcpyAsync(snp_nm, c->st[0].p, jb, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipMemcpyAsync(snp_alt, c->st[1].p, jb, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipMemcpyAsync(ind_nm, c->st[2].p, ib, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(need_resident(c, "counts"));
+    if (!snp_nm || !snp_alt || !ind_nm) return bad("null output");
+    const size_t jb = sizeof(unsigned long long) * c->J, ib = sizeof(unsigned long long) * c->I;
+    CHK(c->st[0].grow(jb));
+    CHK(c->st[1].grow(jb));
+    CHK(c->st[2].grow(ib));
+    HIPCHK(hipMemsetAsync(c->st[2].p, 0, ib, c->s));
+    {
+      ProfScope ps(c, "counts");
+      launch_counts(c->bed, c->pitch, c->I, c->J, static_cast<unsigned long long*>(c->st[0].p),
+                    static_cast<unsigned long long*>(c->st[1].p), static_cast<unsigned long long*>(c->st[2].p), c->s);
+    }
+    CHK(check_launch());
+    HIPCHK(hipMemcpyAsync(snp_nm, c->st[0].p, jb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(snp_alt, c->st[1].p, jb, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(ind_nm, c->st[2].p, ib, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_loglikelihood_loop(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0,
                             int64_t j1, double* out) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(check_ranges(c, i0, i1, j0, j1));
-  if (!q || !p || !out) return bad("null pointer");
-  return BY_DTYPE(c, loop_ll, c, q, p, i0, i1, j0, j1, out);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(check_ranges(c, i0, i1, j0, j1));
+    if (!q || !p || !out) return bad("null pointer");
+    return BY_DTYPE(c, loop_ll, c, q, p, i0, i1, j0, j1, out);
+  });
 }
 
 int admx_em_q_loop(admx_ctx* c, void* q_next, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0,
                    int64_t j1) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(check_ranges(c, i0, i1, j0, j1));
-  if (!q_next || !q || !p) return bad("null pointer");
-  return BY_DTYPE(c, loop_em_q, c, q_next, q, p, i0, i1, j0, j1);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(check_ranges(c, i0, i1, j0, j1));
+    if (!q_next || !q || !p) return bad("null pointer");
+    return BY_DTYPE(c, loop_em_q, c, q_next, q, p, i0, i1, j0, j1);
+  });
 }
 
 int admx_em_p_loop(admx_ctx* c, void* p_next, void* p_tmp, const void* q, const void* p, int64_t i0, int64_t i1,
                    int64_t j0, int64_t j1) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(check_ranges(c, i0, i1, j0, j1));
-  if (!p_next || !p_tmp || !q || !p) return bad("null pointer");
-  return BY_DTYPE(c, loop_em_p, c, p_next, p_tmp, q, p, i0, i1, j0, j1);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(check_ranges(c, i0, i1, j0, j1));
+    if (!p_next || !p_tmp || !q || !p) return bad("null pointer");
+    return BY_DTYPE(c, loop_em_p, c, p_next, p_tmp, q, p, i0, i1, j0, j1);
+  });
 }
 
 int admx_update_q_loop(admx_ctx* c, void* XtX, void* Xtz, const void* q, const void* p, int64_t i0, int64_t i1,
                        int64_t j0, int64_t j1) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(check_ranges(c, i0, i1, j0, j1));
-  if (!XtX || !Xtz || !q || !p) return bad("null pointer");
-  return BY_DTYPE(c, loop_update, c, true, XtX, Xtz, q, p, i0, i1, j0, j1);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(check_ranges(c, i0, i1, j0, j1));
+    if (!XtX || !Xtz || !q || !p) return bad("null pointer");
+    return BY_DTYPE(c, loop_update, c, true, XtX, Xtz, q, p, i0, i1, j0, j1);
+  });
 }
 
 int admx_update_p_loop(admx_ctx* c, void* XtX, void* Xtz, const void* q, const void* p, int64_t i0, int64_t i1,
                        int64_t j0, int64_t j1) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(check_ranges(c, i0, i1, j0, j1));
-  if (!XtX || !Xtz || !q || !p) return bad("null pointer");
-  return BY_DTYPE(c, loop_update, c, false, XtX, Xtz, q, p, i0, i1, j0, j1);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(check_ranges(c, i0, i1, j0, j1));
+    if (!XtX || !Xtz || !q || !p) return bad("null pointer");
+    return BY_DTYPE(c, loop_update, c, false, XtX, Xtz, q, p, i0, i1, j0, j1);
+  });
 }
 
 int admx_qp_batch(admx_ctx* c, int simplex, int64_t n, const void* XtX, const void* Xtz_neg, const void* x,
                   void* delta, int32_t* status) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  if (n < 0) return bad("n < 0");
-  if (n == 0) return ADMX_OK;
-  if (!XtX || !Xtz_neg || !x || !delta) return bad("null pointer");
-  return BY_DTYPE(c, host_qp_batch, c, simplex, n, XtX, Xtz_neg, x, delta, status);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (n < 0) return bad("n < 0");
+    if (n == 0) return ADMX_OK;
+    if (!XtX || !Xtz_neg || !x || !delta) return bad("null pointer");
+    return BY_DTYPE(c, host_qp_batch, c, simplex, n, XtX, Xtz_neg, x, delta, status);
+  });
 }
 
 int admx_project_q(admx_ctx* c, void* b, int64_t n) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  if (n < 0 || (n > 0 && !b)) return bad("bad buffer");
-  if (n == 0) return ADMX_OK;
-  return BY_DTYPE(c, host_project, c, true, b, n);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (n < 0 || (n > 0 && !b)) return bad("bad buffer");
+    if (n == 0) return ADMX_OK;
+    return BY_DTYPE(c, host_project, c, true, b, n);
+  });
 }
 
 int admx_project_p(admx_ctx* c, void* b, int64_t n) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  if (n < 0 || (n > 0 && !b)) return bad("bad buffer");
-  if (n == 0) return ADMX_OK;
-  return BY_DTYPE(c, host_project, c, false, b, n);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (n < 0 || (n > 0 && !b)) return bad("bad buffer");
+    if (n == 0) return ADMX_OK;
+    return BY_DTYPE(c, host_project, c, false, b, n);
+  });
 }
 
 int admx_zal_step(admx_ctx* c, int64_t iter, const void* x, const void* xn, const void* xn2, void* U, void* V,
                   void* x_tmp) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  if (iter < 1) return bad("iter must be >= 1");
-  if (!x || !xn || !xn2 || !U || !V || !x_tmp) return bad("null pointer");
-  return BY_DTYPE(c, host_zal, c, iter, x, xn, xn2, U, V, x_tmp);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (iter < 1) return bad("iter must be >= 1");
+    if (!x || !xn || !xn2 || !U || !V || !x_tmp) return bad("null pointer");
+    return BY_DTYPE(c, host_zal, c, iter, x, xn, xn2, U, V, x_tmp);
+  });
 }
 
 int admx_lbqn_step(admx_ctx* c, int64_t iter, const void* x, const void* xn, const void* xn2, void* U, void* V,
                    void* x_tmp) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  if (iter < 1) return bad("iter must be >= 1");
-  if (!x || !xn || !xn2 || !U || !V || !x_tmp) return bad("null pointer");
-  return BY_DTYPE(c, host_lbqn, c, iter, x, xn, xn2, U, V, x_tmp);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (iter < 1) return bad("iter must be >= 1");
+    if (!x || !xn || !xn2 || !U || !V || !x_tmp) return bad("null pointer");
+    return BY_DTYPE(c, host_lbqn, c, iter, x, xn, xn2, U, V, x_tmp);
+  });
 }
 
 int admx_set_x(admx_ctx* c, int which, const void* host) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  if (which < 0 || which > 3 || !host) return bad("bad state vector");
-  HIPCHK(hipMemcpyAsync(c->x[which], host, (size_t)c->N() * c->esz, hipMemcpyHostToDevice, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (which < 0 || which > 3 || !host) return bad("bad state vector");
+    HIPCHK(hipMemcpyAsync(c->x[which], host, (size_t)c->N() * c->esz, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_get_x(admx_ctx* c, int which, void* host) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  if (which < 0 || which > 3 || !host) return bad("bad state vector");
-  HIPCHK(hipMemcpyAsync(host, c->x[which], (size_t)c->N() * c->esz, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (which < 0 || which > 3 || !host) return bad("bad state vector");
+    HIPCHK(hipMemcpyAsync(host, c->x[which], (size_t)c->N() * c->esz, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_set_uv(admx_ctx* c, const void* U, const void* V) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  if (!U || !V) return bad("null pointer");
-  const size_t b = (size_t)c->N() * c->Q * c->esz;
-  HIPCHK(hipMemcpyAsync(c->U, U, b, hipMemcpyHostToDevice, c->s));
-  HIPCHK(hipMemcpyAsync(c->V, V, b, hipMemcpyHostToDevice, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (!U || !V) return bad("null pointer");
+    const size_t b = (size_t)c->N() * c->Q * c->esz;
+    HIPCHK(hipMemcpyAsync(c->U, U, b, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipMemcpyAsync(c->V, V, b, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_get_uv(admx_ctx* c, void* U, void* V) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  if (!U || !V) return bad("null pointer");
-  const size_t b = (size_t)c->N() * c->Q * c->esz;
-  HIPCHK(hipMemcpyAsync(U, c->U, b, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipMemcpyAsync(V, c->V, b, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (!U || !V) return bad("null pointer");
+    const size_t b = (size_t)c->N() * c->Q * c->esz;
+    HIPCHK(hipMemcpyAsync(U, c->U, b, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(V, c->V, b, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_set_ll(admx_ctx* c, double ll_prev, double ll_new) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  double v[2] = {ll_prev, ll_new};
-  HIPCHK(hipMemcpyAsync(c->lls + LL_PREV, v, sizeof(v), hipMemcpyHostToDevice, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    double v[2] = {ll_prev, ll_new};
+    HIPCHK(hipMemcpyAsync(c->lls + LL_PREV, v, sizeof(v), hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_get_ll(admx_ctx* c, double* ll_prev, double* ll_new) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  HIPCHK(hipMemcpyAsync(c->h_lls, c->lls, sizeof(double) * LL_NSLOTS, hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  if (ll_prev) *ll_prev = c->h_lls[LL_PREV];
-  if (ll_new) *ll_new = c->h_lls[LL_NEW];
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    HIPCHK(hipMemcpyAsync(c->h_lls, c->lls, sizeof(double) * LL_NSLOTS, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    if (ll_prev) *ll_prev = c->h_lls[LL_PREV];
+    if (ll_new) *ll_new = c->h_lls[LL_NEW];
+    return ADMX_OK;
+  });
 }
 
 int admx_get_hessians(admx_ctx* c, int side, void* XtX, void* Xtz) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  if (!XtX || !Xtz) return bad("null pointer");
-  const int64_t n = side == ADMX_SIDE_Q ? c->I : c->J;
-  HIPCHK(hipMemcpyAsync(XtX, side == ADMX_SIDE_Q ? c->XtX_q : c->XtX_p, (size_t)(c->K * c->K * n) * c->esz,
-                        hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipMemcpyAsync(Xtz, side == ADMX_SIDE_Q ? c->Xtz_q : c->Xtz_p, (size_t)(c->K * n) * c->esz,
-                        hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (!XtX || !Xtz) return bad("null pointer");
+    const int64_t n = side == ADMX_SIDE_Q ? c->I : c->J;
+    HIPCHK(hipMemcpyAsync(XtX, side == ADMX_SIDE_Q ? c->XtX_q : c->XtX_p, (size_t)(c->K * c->K * n) * c->esz,
+                          hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipMemcpyAsync(Xtz, side == ADMX_SIDE_Q ? c->Xtz_q : c->Xtz_p, (size_t)(c->K * n) * c->esz,
+                          hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_loglikelihood(admx_ctx* c, int which, double* out) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  if (which < 0 || which > 3 || !out) return bad("bad state vector");
-  void* X = c->x[which];
-  CHK(BY_DTYPE(c, ll_to_slot, c, X, c->pp(X), 0, c->I, 0, c->J, LL_TMP));
-  HIPCHK(hipMemcpyAsync(out, c->lls + LL_TMP, sizeof(double), hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    if (which < 0 || which > 3 || !out) return bad("bad state vector");
+    void* X = c->x[which];
+    CHK(BY_DTYPE(c, ll_to_slot, c, X, c->pp(X), 0, c->I, 0, c->J, LL_TMP));
+    HIPCHK(hipMemcpyAsync(out, c->lls + LL_TMP, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_em_q(admx_ctx* c) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(BY_DTYPE(c, dev_em_q, c));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(BY_DTYPE(c, dev_em_q, c));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_em_p(admx_ctx* c) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(BY_DTYPE(c, dev_em_p, c));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(BY_DTYPE(c, dev_em_p, c));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_init_em(admx_ctx* c, int64_t iters, double* lls) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  if (iters < 0) return bad("iters < 0");
-  return BY_DTYPE(c, dev_init_em, c, iters, lls);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    if (iters < 0) return bad("iters < 0");
+    return BY_DTYPE(c, dev_init_em, c, iters, lls);
+  });
 }
 
 int admx_update_q(admx_ctx* c, int update2, int32_t* status) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(BY_DTYPE(c, dev_update_q, c, update2 != 0, status));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(BY_DTYPE(c, dev_update_q, c, update2 != 0, status));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_update_p(admx_ctx* c, int update2, int32_t* status) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(BY_DTYPE(c, dev_update_p, c, update2 != 0, status));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(BY_DTYPE(c, dev_update_p, c, update2 != 0, status));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_qn_iteration(admx_ctx* c, int64_t iter, int fix_q, int fix_p, double* out) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  if (iter < 1 || !out) return bad("iter must be >= 1 and out non-null");
-  return BY_DTYPE(c, dev_qn_iteration, c, iter, fix_q, fix_p, out);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    if (iter < 1 || !out) return bad("iter must be >= 1 and out non-null");
+    return BY_DTYPE(c, dev_qn_iteration, c, iter, fix_q, fix_p, out);
+  });
 }
 
 int admx_synchronize(admx_ctx* c) {
-  CHK(check_ctx(c));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_profile_enable(admx_ctx* c, int on) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(prof_collect(c));
-  c->prof.on = on != 0;
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(prof_collect(c));
+    c->prof.on = on != 0;
+    return ADMX_OK;
+  });
 }
 
 int admx_profile_reset(admx_ctx* c) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(prof_collect(c));
-  c->prof.acc.clear();
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(prof_collect(c));
+    c->prof.acc.clear();
+    return ADMX_OK;
+  });
 }
 
 int admx_profile_report(admx_ctx* c, char* buf, size_t len) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(prof_collect(c));
-  std::string s;
-  char line[256];
-  for (auto& kv : c->prof.acc) {
-    std::snprintf(line, sizeof(line), "%s %ld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
-    s += line;
-  }
-  if (!buf || len == 0) return bad("null buffer");
-  if (s.size() + 1 > len) return bad("buffer too small");
-  std::memcpy(buf, s.c_str(), s.size() + 1);
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(prof_collect(c));
+    std::string s;
+    char line[256];
+    for (auto& kv : c->prof.acc) {
+      std::snprintf(line, sizeof(line), "%s %ld %.6f\n", kv.first.c_str(), kv.second.first, kv.second.second);
+      s += line;
+    }
+    if (!buf || len == 0) return bad("null buffer");
+    if (s.size() + 1 > len) return bad("buffer too small");
+    std::memcpy(buf, s.c_str(), s.size() + 1);
+    return ADMX_OK;
+  });
 }
 
 int admx_set_tuning(admx_ctx* c, int key, int64_t value) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  if (key == ADMX_TUNE_TARGET_WAVES) {
-    if (value < 1) return bad("target waves must be >= 1");
-    c->target_waves = value;
-  } else if (key == ADMX_TUNE_KEEP_HESSIANS) {
-    c->keep_hess = value != 0;
-  } else if (key == ADMX_TUNE_PRECISION) {
-    if (value < ADMX_PREC_F64 || value > ADMX_PREC_BF16) return bad("precision must be ADMX_PREC_F64/F32/BF16");
-    if (value != ADMX_PREC_F64 && c->dtype != ADMX_F64) return bad("mixed precision needs a FP64 context");
-    c->precision = (int)value;
-  } else if (key == ADMX_TUNE_SKIPMISSING) {
-    c->impute = value == 0;
-  } else if (key == ADMX_TUNE_QN_MODE) {
-    if (value != ADMX_QN_ZAL && value != ADMX_QN_LBQN) return bad("quasi-Newton mode must be ADMX_QN_ZAL or ADMX_QN_LBQN");
-    c->qn_mode = (int)value;
-  } else if (key == ADMX_TUNE_GRAPHS) {
-    c->use_graphs = value != 0;
-    if (!c->use_graphs) drop_graphs(c);
-  } else if (key == ADMX_TUNE_RESIDENT_SNPS) {
-    if (!c->stream_snps) return bad("resident SNPs apply to a streamed context");
-    if (value < 0) return bad("resident SNPs must be >= 0");
-    const int64_t R = std::min(value, c->J);
-    HIPCHK(hipStreamSynchronize(c->s));
-    if (c->dres) HIPCHK(hipFree(c->dres));
-    c->dres = nullptr;
-    c->res_snps = 0;
-    if (R > 0) {
-      if (hipMalloc((void**)&c->dres, (size_t)c->pitch * (R + 16)) != hipSuccess) {
-        (void)hipGetLastError();
-        return fail(ADMX_EHIP, "hipMalloc failed for the resident SNPs");
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (key == ADMX_TUNE_TARGET_WAVES) {
+      if (value < 1) return bad("target waves must be >= 1");
+      c->target_waves = value;
+    } else if (key == ADMX_TUNE_KEEP_HESSIANS) {
+      c->keep_hess = value != 0;
+    } else if (key == ADMX_TUNE_PRECISION) {
+      if (value < ADMX_PREC_F64 || value > ADMX_PREC_BF16) return bad("precision must be ADMX_PREC_F64/F32/BF16");
+      if (value != ADMX_PREC_F64 && c->dtype != ADMX_F64) return bad("mixed precision needs a FP64 context");
+      c->precision = (int)value;
+    } else if (key == ADMX_TUNE_SKIPMISSING) {
+      c->impute = value == 0;
+    } else if (key == ADMX_TUNE_QN_MODE) {
+      if (value != ADMX_QN_ZAL && value != ADMX_QN_LBQN) return bad("quasi-Newton mode must be ADMX_QN_ZAL or ADMX_QN_LBQN");
+      c->qn_mode = (int)value;
+    } else if (key == ADMX_TUNE_GRAPHS) {
+      c->use_graphs = value != 0;
+      if (!c->use_graphs) drop_graphs(c);
+    } else if (key == ADMX_TUNE_RESIDENT_SNPS) {
+      if (!c->stream_snps) return bad("resident SNPs apply to a streamed context");
+      if (value < 0) return bad("resident SNPs must be >= 0");
+      const int64_t R = std::min(value, c->J);
+      HIPCHK(hipStreamSynchronize(c->s));
+      if (c->dres) HIPCHK(hipFree(c->dres));
+      c->dres = nullptr;
+      c->res_snps = 0;
+      if (R > 0) {
+        if (hipMalloc((void**)&c->dres, (size_t)c->pitch * (R + 16)) != hipSuccess) {
+          (void)hipGetLastError();
+          return fail(ADMX_EHIP, "hipMalloc failed for the resident SNPs");
+        }
+        c->res_snps = R;
+        const int rc = refresh_resident(c);
+        if (rc != ADMX_OK) {
+          (void)hipFree(c->dres);
+          c->dres = nullptr;
+          c->res_snps = 0;
+          return rc;
+        }
       }
-      c->res_snps = R;
-      const int rc = refresh_resident(c);
-      if (rc != ADMX_OK) {
-        (void)hipFree(c->dres);
-        c->dres = nullptr;
-        c->res_snps = 0;
-        return rc;
-      }
+    } else {
+      return bad("unknown tuning key");
     }
-  } else {
-    return bad("unknown tuning key");
-  }
-  return ADMX_OK;
+    return ADMX_OK;
+  });
 }
 
 // ---- learned warm start
 int admx_mlp_create(admx_mlp** out, int32_t in_dim, int32_t hidden, int32_t out_dim, uint64_t seed) {
-  if (!out) return bad("null output handle");
-  *out = nullptr;
-  if (in_dim < 1 || hidden < 1 || out_dim < 2 || out_dim > 16) return bad("need in >= 1, hidden >= 1, 2 <= out <= 16");
-  admx_mlp* m = new admx_mlp();
-  m->in = in_dim;
-  m->hid = hidden;
-  m->out = out_dim;
-  m->P = m->ob3() + out_dim;
-  auto cleanup = [&](int rc) {
-    admx_mlp_destroy(m);
-    return rc;
-  };
-  hipError_t e = hipGetDevice(&m->device);
-  if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking);
-  if (e != hipSuccess) return cleanup(fail(ADMX_EHIP, std::string("mlp stream: ") + hipGetErrorString(e)));
-  const size_t pb = sizeof(float) * (size_t)m->P;
-  int rc = m->w.grow(pb);
-  if (rc == ADMX_OK) rc = m->g.grow(pb);
-  if (rc == ADMX_OK) rc = m->m.grow(pb);
-  if (rc == ADMX_OK) rc = m->v.grow(pb);
-  if (rc != ADMX_OK) return cleanup(rc);
-  // He-uniform weights U(-sqrt(6/fan_in), sqrt(6/fan_in)), zero biases
-  std::vector<float> w((size_t)m->P, 0.f);
-  uint64_t ctr = seed * 0x9E3779B97F4A7C15ULL;
-  auto fill = [&](int64_t off, int64_t n, int fan_in) {
-    const double a = std::sqrt(6.0 / fan_in);
-    for (int64_t e2 = 0; e2 < n; ++e2) {
-      const double u = (double)(host_smix(ctr++) >> 11) * 0x1.0p-53;
-      w[(size_t)(off + e2)] = (float)((2.0 * u - 1.0) * a);
-    }
-  };
-  fill(m->oW1(), (int64_t)hidden * in_dim, in_dim);
-  fill(m->oW2(), (int64_t)hidden * hidden, hidden);
-  fill(m->oW3(), (int64_t)out_dim * hidden, hidden);
-  e = hipMemcpy(m->w.p, w.data(), pb, hipMemcpyHostToDevice);
-  if (e == hipSuccess) e = hipMemset(m->m.p, 0, pb);
-  if (e == hipSuccess) e = hipMemset(m->v.p, 0, pb);
-  if (e != hipSuccess) return cleanup(fail(ADMX_EHIP, std::string("mlp init: ") + hipGetErrorString(e)));
-  *out = m;
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    if (!out) return bad("null output handle");
+    *out = nullptr;
+    if (in_dim < 1 || hidden < 1 || out_dim < 2 || out_dim > 16) return bad("need in >= 1, hidden >= 1, 2 <= out <= 16");
+    admx_mlp* m = new admx_mlp();
+    m->in = in_dim;
+    m->hid = hidden;
+    m->out = out_dim;
+    m->P = m->ob3() + out_dim;
+    auto cleanup = [&](int rc) {
+      admx_mlp_destroy(m);
+      return rc;
+    };
+    hipError_t e = hipGetDevice(&m->device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking);
+    if (e != hipSuccess) return cleanup(fail(ADMX_EHIP, std::string("mlp stream: ") + hipGetErrorString(e)));
+    const size_t pb = sizeof(float) * (size_t)m->P;
+    int rc = m->w.grow(pb);
+    if (rc == ADMX_OK) rc = m->g.grow(pb);
+    if (rc == ADMX_OK) rc = m->m.grow(pb);
+    if (rc == ADMX_OK) rc = m->v.grow(pb);
+    if (rc != ADMX_OK) return cleanup(rc);
+    // He-uniform weights U(-sqrt(6/fan_in), sqrt(6/fan_in)), zero biases
+    std::vector<float> w((size_t)m->P, 0.f);
+    uint64_t ctr = seed * 0x9E3779B97F4A7C15ULL;
+    auto fill = [&](int64_t off, int64_t n, int fan_in) {
+      const double a = std::sqrt(6.0 / fan_in);
+      for (int64_t e2 = 0; e2 < n; ++e2) {
+        const double u = (double)(host_smix(ctr++) >> 11) * 0x1.0p-53;
+        w[(size_t)(off + e2)] = (float)((2.0 * u - 1.0) * a);
+      }
+    };
+    fill(m->oW1(), (int64_t)hidden * in_dim, in_dim);
+    fill(m->oW2(), (int64_t)hidden * hidden, hidden);
+    fill(m->oW3(), (int64_t)out_dim * hidden, hidden);
+    e = hipMemcpy(m->w.p, w.data(), pb, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(m->m.p, 0, pb);
+    if (e == hipSuccess) e = hipMemset(m->v.p, 0, pb);
+    if (e != hipSuccess) return cleanup(fail(ADMX_EHIP, std::string("mlp init: ") + hipGetErrorString(e)));
+    *out = m;
+    return ADMX_OK;
+  });
 }
 
 int admx_mlp_destroy(admx_mlp* m) {
-  if (!m) return ADMX_OK;
-  (void)hipSetDevice(m->device);
-  if (m->s) (void)hipStreamSynchronize(m->s);
-  DevBuf* bufs[] = {&m->w, &m->g, &m->m, &m->v, &m->X, &m->T, &m->idx, &m->act};
-  for (DevBuf* b : bufs) b->release();
-  if (m->s) (void)hipStreamDestroy(m->s);
-  delete m;
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    if (!m) return ADMX_OK;
+    (void)hipSetDevice(m->device);
+    if (m->s) (void)hipStreamSynchronize(m->s);
+    DevBuf* bufs[] = {&m->w, &m->g, &m->m, &m->v, &m->X, &m->T, &m->idx, &m->act};
+    for (DevBuf* b : bufs) b->release();
+    if (m->s) (void)hipStreamDestroy(m->s);
+    delete m;
+    return ADMX_OK;
+  });
 }
 
 int admx_mlp_num_params(admx_mlp* m, int64_t* n) {
-  if (!m || !n) return bad("null argument");
-  *n = m->P;
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    if (!m || !n) return bad("null argument");
+    *n = m->P;
+    return ADMX_OK;
+  });
 }
 
 int admx_mlp_get_params(admx_mlp* m, float* w) {
-  if (!m || !w) return bad("null argument");
-  MGuard g(m);
-  HIPCHK(hipSetDevice(m->device));
-  HIPCHK(hipMemcpyAsync(w, m->w.p, sizeof(float) * (size_t)m->P, hipMemcpyDeviceToHost, m->s));
-  HIPCHK(hipStreamSynchronize(m->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    if (!m || !w) return bad("null argument");
+    MGuard g(m);
+    HIPCHK(hipSetDevice(m->device));
+    HIPCHK(hipMemcpyAsync(w, m->w.p, sizeof(float) * (size_t)m->P, hipMemcpyDeviceToHost, m->s));
+    HIPCHK(hipStreamSynchronize(m->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_mlp_set_params(admx_mlp* m, const float* w) {
-  if (!m || !w) return bad("null argument");
-  MGuard g(m);
-  HIPCHK(hipSetDevice(m->device));
-  HIPCHK(hipMemcpyAsync(m->w.p, w, sizeof(float) * (size_t)m->P, hipMemcpyHostToDevice, m->s));
-  HIPCHK(hipMemsetAsync(m->m.p, 0, sizeof(float) * (size_t)m->P, m->s));
-  HIPCHK(hipMemsetAsync(m->v.p, 0, sizeof(float) * (size_t)m->P, m->s));
-  HIPCHK(hipStreamSynchronize(m->s));
-  m->step = 0;
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    if (!m || !w) return bad("null argument");
+    MGuard g(m);
+    HIPCHK(hipSetDevice(m->device));
+    HIPCHK(hipMemcpyAsync(m->w.p, w, sizeof(float) * (size_t)m->P, hipMemcpyHostToDevice, m->s));
+    HIPCHK(hipMemsetAsync(m->m.p, 0, sizeof(float) * (size_t)m->P, m->s));
+    HIPCHK(hipMemsetAsync(m->v.p, 0, sizeof(float) * (size_t)m->P, m->s));
+    HIPCHK(hipStreamSynchronize(m->s));
+    m->step = 0;
+    return ADMX_OK;
+  });
 }
 
 int admx_mlp_loss_grad(admx_mlp* m, const float* X, const float* T, int64_t B, float* loss, float* grad) {
-  if (!m || !X || !T || !loss || !grad) return bad("null argument");
-  if (B < 1 || B > (1 << 30)) return bad("batch rows out of range");
-  MGuard g(m);
-  HIPCHK(hipSetDevice(m->device));
-  MlpActs a;
-  CHK(mlp_acts(m, B, &a));
-  HIPCHK(hipMemcpyAsync(a.Xb, X, sizeof(float) * (size_t)(B * m->in), hipMemcpyHostToDevice, m->s));
-  HIPCHK(hipMemcpyAsync(a.Tb, T, sizeof(float) * (size_t)(B * m->out), hipMemcpyHostToDevice, m->s));
-  mlp_forward(m, a.Xb, a.Tb, (int)B, a, a.Y, m->s);
-  mlp_backward(m, a.Xb, (int)B, a, a.loss, m->s);
-  CHK(check_launch());
-  float ls = 0.f;
-  HIPCHK(hipMemcpyAsync(&ls, a.loss, sizeof(float), hipMemcpyDeviceToHost, m->s));
-  HIPCHK(hipMemcpyAsync(grad, m->g.p, sizeof(float) * (size_t)m->P, hipMemcpyDeviceToHost, m->s));
-  HIPCHK(hipStreamSynchronize(m->s));
-  *loss = ls / (float)B;
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    if (!m || !X || !T || !loss || !grad) return bad("null argument");
+    if (B < 1 || B > (1 << 30)) return bad("batch rows out of range");
+    MGuard g(m);
+    HIPCHK(hipSetDevice(m->device));
+    MlpActs a;
+    CHK(mlp_acts(m, B, &a));
+    HIPCHK(hipMemcpyAsync(a.Xb, X, sizeof(float) * (size_t)(B * m->in), hipMemcpyHostToDevice, m->s));
+    HIPCHK(hipMemcpyAsync(a.Tb, T, sizeof(float) * (size_t)(B * m->out), hipMemcpyHostToDevice, m->s));
+    mlp_forward(m, a.Xb, a.Tb, (int)B, a, a.Y, m->s);
+    mlp_backward(m, a.Xb, (int)B, a, a.loss, m->s);
+    CHK(check_launch());
+    float ls = 0.f;
+    HIPCHK(hipMemcpyAsync(&ls, a.loss, sizeof(float), hipMemcpyDeviceToHost, m->s));
+    HIPCHK(hipMemcpyAsync(grad, m->g.p, sizeof(float) * (size_t)m->P, hipMemcpyDeviceToHost, m->s));
+    HIPCHK(hipStreamSynchronize(m->s));
+    *loss = ls / (float)B;
+    return ADMX_OK;
+  });
 }
 
 int admx_mlp_predict(admx_mlp* m, const float* X, int64_t n, float* Y) {
-  if (!m || !X || !Y) return bad("null argument");
-  if (n < 0) return bad("negative row count");
-  MGuard g(m);
-  HIPCHK(hipSetDevice(m->device));
-  const int64_t Bmax = 65536;
-  MlpActs a;
-  CHK(mlp_acts(m, std::min<int64_t>(std::max<int64_t>(n, 1), Bmax), &a));
-  for (int64_t r0 = 0; r0 < n; r0 += Bmax) {
-    const int64_t B = std::min(Bmax, n - r0);
-    HIPCHK(hipMemcpyAsync(a.Xb, X + r0 * m->in, sizeof(float) * (size_t)(B * m->in), hipMemcpyHostToDevice, m->s));
-    mlp_forward(m, a.Xb, nullptr, (int)B, a, a.Y, m->s);
-    CHK(check_launch());
-    HIPCHK(hipMemcpyAsync(Y + r0 * m->out, a.Y, sizeof(float) * (size_t)(B * m->out), hipMemcpyDeviceToHost, m->s));
-  }
-  HIPCHK(hipStreamSynchronize(m->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    if (!m || !X || !Y) return bad("null argument");
+    if (n < 0) return bad("negative row count");
+    MGuard g(m);
+    HIPCHK(hipSetDevice(m->device));
+    const int64_t Bmax = 65536;
+    MlpActs a;
+    CHK(mlp_acts(m, std::min<int64_t>(std::max<int64_t>(n, 1), Bmax), &a));
+    for (int64_t r0 = 0; r0 < n; r0 += Bmax) {
+      const int64_t B = std::min(Bmax, n - r0);
+      HIPCHK(hipMemcpyAsync(a.Xb, X + r0 * m->in, sizeof(float) * (size_t)(B * m->in), hipMemcpyHostToDevice, m->s));
+      mlp_forward(m, a.Xb, nullptr, (int)B, a, a.Y, m->s);
+      CHK(check_launch());
+      HIPCHK(hipMemcpyAsync(Y + r0 * m->out, a.Y, sizeof(float) * (size_t)(B * m->out), hipMemcpyDeviceToHost, m->s));
+    }
+    HIPCHK(hipStreamSynchronize(m->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_mlp_train(admx_mlp* m, const float* X, const float* T, int64_t n, int32_t epochs, int32_t batch, float lr,
                    uint64_t seed, float* epoch_loss) {
-  if (!m || !X || !T) return bad("null argument");
-  if (n < 1 || n > (1 << 30) || epochs < 0 || batch < 1) return bad("need n >= 1, epochs >= 0, batch >= 1");
-  MGuard g(m);
-  HIPCHK(hipSetDevice(m->device));
-  CHK(m->X.grow(sizeof(float) * (size_t)(n * m->in)));
-  CHK(m->T.grow(sizeof(float) * (size_t)(n * m->out)));
-  CHK(m->idx.grow(sizeof(int32_t) * (size_t)n));
-  HIPCHK(hipMemcpyAsync(m->X.p, X, sizeof(float) * (size_t)(n * m->in), hipMemcpyHostToDevice, m->s));
-  HIPCHK(hipMemcpyAsync(m->T.p, T, sizeof(float) * (size_t)(n * m->out), hipMemcpyHostToDevice, m->s));
-  m->n = n;
-  const int64_t B = std::min<int64_t>(batch, n);
-  const int64_t nb = (n + B - 1) / B;
-  MlpActs a;
-  CHK(mlp_acts(m, B, &a));
-  DevBuf lossbuf;
-  CHK(lossbuf.grow(sizeof(float) * (size_t)nb));
-  std::vector<int32_t> perm((size_t)n);
-  std::vector<float> lh((size_t)nb);
-  const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-  int rc = ADMX_OK;
-  for (int ep = 0; ep < epochs && rc == ADMX_OK; ++ep) {
-    // Fisher-Yates shuffle of the rows from the seeded counter hash
-    for (int64_t r = 0; r < n; ++r) perm[(size_t)r] = (int32_t)r;
-    for (int64_t r = n - 1; r > 0; --r) {
-      const uint64_t h = host_smix(seed ^ host_smix((uint64_t)ep * 0x100000001B3ULL + (uint64_t)r));
-      const int64_t k = (int64_t)(h % (uint64_t)(r + 1));
-      std::swap(perm[(size_t)r], perm[(size_t)k]);
+  return guarded([&]() -> int {
+    if (!m || !X || !T) return bad("null argument");
+    if (n < 1 || n > (1 << 30) || epochs < 0 || batch < 1) return bad("need n >= 1, epochs >= 0, batch >= 1");
+    MGuard g(m);
+    HIPCHK(hipSetDevice(m->device));
+    CHK(m->X.grow(sizeof(float) * (size_t)(n * m->in)));
+    CHK(m->T.grow(sizeof(float) * (size_t)(n * m->out)));
+    CHK(m->idx.grow(sizeof(int32_t) * (size_t)n));
+    HIPCHK(hipMemcpyAsync(m->X.p, X, sizeof(float) * (size_t)(n * m->in), hipMemcpyHostToDevice, m->s));
+    HIPCHK(hipMemcpyAsync(m->T.p, T, sizeof(float) * (size_t)(n * m->out), hipMemcpyHostToDevice, m->s));
+    m->n = n;
+    const int64_t B = std::min<int64_t>(batch, n);
+    const int64_t nb = (n + B - 1) / B;
+    MlpActs a;
+    CHK(mlp_acts(m, B, &a));
+    DevBuf lossbuf;
+    CHK(lossbuf.grow(sizeof(float) * (size_t)nb));
+    std::vector<int32_t> perm((size_t)n);
+    std::vector<float> lh((size_t)nb);
+    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
+    int rc = ADMX_OK;
+    for (int ep = 0; ep < epochs && rc == ADMX_OK; ++ep) {
+      // Fisher-Yates shuffle of the rows from the seeded counter hash
+      for (int64_t r = 0; r < n; ++r) perm[(size_t)r] = (int32_t)r;
+      for (int64_t r = n - 1; r > 0; --r) {
+        const uint64_t h = host_smix(seed ^ host_smix((uint64_t)ep * 0x100000001B3ULL + (uint64_t)r));
+        const int64_t k = (int64_t)(h % (uint64_t)(r + 1));
+        std::swap(perm[(size_t)r], perm[(size_t)k]);
+      }
+      HIPCHK(hipMemcpyAsync(m->idx.p, perm.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, m->s));
+      for (int64_t b = 0; b < nb; ++b) {
+        const int64_t r0 = b * B;
+        const int Bb = (int)std::min(B, n - r0);
+        const int32_t* ix = static_cast<const int32_t*>(m->idx.p) + r0;
+        launch_gather_rows(static_cast<const float*>(m->X.p), m->in, ix, Bb, a.Xb, m->s);
+        launch_gather_rows(static_cast<const float*>(m->T.p), m->out, ix, Bb, a.Tb, m->s);
+        mlp_forward(m, a.Xb, a.Tb, Bb, a, a.Y, m->s);
+        mlp_backward(m, a.Xb, Bb, a, static_cast<float*>(lossbuf.p) + b, m->s);
+        ++m->step;
+        const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
+        launch_adam(m->P, static_cast<float*>(m->w.p), static_cast<const float*>(m->g.p), static_cast<float*>(m->m.p),
+                    static_cast<float*>(m->v.p), lr, b1, b2, c1, c2, eps, m->s);
+      }
+      rc = check_launch();
+      if (rc != ADMX_OK) break;
+      HIPCHK(hipMemcpyAsync(lh.data(), lossbuf.p, sizeof(float) * (size_t)nb, hipMemcpyDeviceToHost, m->s));
+      HIPCHK(hipStreamSynchronize(m->s));
+      double tot = 0.0;
+      for (float v : lh) tot += v;
+      if (epoch_loss) epoch_loss[ep] = (float)(tot / (double)n);
     }
-    HIPCHK(hipMemcpyAsync(m->idx.p, perm.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, m->s));
-    for (int64_t b = 0; b < nb; ++b) {
-      const int64_t r0 = b * B;
-      const int Bb = (int)std::min(B, n - r0);
-      const int32_t* ix = static_cast<const int32_t*>(m->idx.p) + r0;
-      launch_gather_rows(static_cast<const float*>(m->X.p), m->in, ix, Bb, a.Xb, m->s);
-      launch_gather_rows(static_cast<const float*>(m->T.p), m->out, ix, Bb, a.Tb, m->s);
-      mlp_forward(m, a.Xb, a.Tb, Bb, a, a.Y, m->s);
-      mlp_backward(m, a.Xb, Bb, a, static_cast<float*>(lossbuf.p) + b, m->s);
-      ++m->step;
-      const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
-      launch_adam(m->P, static_cast<float*>(m->w.p), static_cast<const float*>(m->g.p), static_cast<float*>(m->m.p),
-                  static_cast<float*>(m->v.p), lr, b1, b2, c1, c2, eps, m->s);
-    }
-    rc = check_launch();
-    if (rc != ADMX_OK) break;
-    HIPCHK(hipMemcpyAsync(lh.data(), lossbuf.p, sizeof(float) * (size_t)nb, hipMemcpyDeviceToHost, m->s));
     HIPCHK(hipStreamSynchronize(m->s));
-    double tot = 0.0;
-    for (float v : lh) tot += v;
-    if (epoch_loss) epoch_loss[ep] = (float)(tot / (double)n);
-  }
-  HIPCHK(hipStreamSynchronize(m->s));
-  lossbuf.release();
-  return rc;
+    lossbuf.release();
+    return rc;
+  });
 }
 
 int admx_features(admx_ctx* c, const float* R, const float* mu, const float* isd, int64_t Jp, float* F) {
-  CHK(check_ctx(c));
-  Guard g(c);
-  CHK(need_bed(c));
-  CHK(need_resident(c, "the warm-start features"));
-  if (!R || !mu || !isd) return bad("null panel array");
-  if (Jp < 1 || Jp > c->J) return bad("panel SNP count must be in [1, J]");
-  const size_t rb = sizeof(float) * (size_t)(Jp * kFeat), vb = sizeof(float) * (size_t)Jp;
-  CHK(stage_in(c, 0, R, rb, rb));
-  CHK(stage_in(c, 1, mu, vb, vb));
-  CHK(stage_in(c, 2, isd, vb, vb));
-  const int64_t waves = (c->I + 63) / 64;
-  int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(64, c->target_waves / std::max<int64_t>(waves, 1) / 8));
-  nchunk = std::min(nchunk, Jp);
-  const int64_t jchunk = (Jp + nchunk - 1) / nchunk;
-  nchunk = (Jp + jchunk - 1) / jchunk;
-  CHK(c->feat.grow(sizeof(float) * (size_t)(c->I * kFeat)));
-  CHK(c->featpart.grow(sizeof(float) * (size_t)(nchunk * c->I * kFeat)));
-  {
-    ProfScope ps(c, "features");
-    launch_features(c->bed, c->pitch, c->I, Jp, static_cast<const float*>(c->st[0].p),
-                    static_cast<const float*>(c->st[1].p), static_cast<const float*>(c->st[2].p), (int)nchunk, jchunk,
-                    static_cast<float*>(c->featpart.p), static_cast<float*>(c->feat.p), c->s);
-  }
-  CHK(check_launch());
-  c->feat_rows = c->I;
-  if (F) HIPCHK(hipMemcpyAsync(F, c->feat.p, sizeof(float) * (size_t)(c->I * kFeat), hipMemcpyDeviceToHost, c->s));
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    CHK(need_resident(c, "the warm-start features"));
+    if (!R || !mu || !isd) return bad("null panel array");
+    if (Jp < 1 || Jp > c->J) return bad("panel SNP count must be in [1, J]");
+    const size_t rb = sizeof(float) * (size_t)(Jp * kFeat), vb = sizeof(float) * (size_t)Jp;
+    CHK(stage_in(c, 0, R, rb, rb));
+    CHK(stage_in(c, 1, mu, vb, vb));
+    CHK(stage_in(c, 2, isd, vb, vb));
+    const int64_t waves = (c->I + 63) / 64;
+    int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(64, c->target_waves / std::max<int64_t>(waves, 1) / 8));
+    nchunk = std::min(nchunk, Jp);
+    const int64_t jchunk = (Jp + nchunk - 1) / nchunk;
+    nchunk = (Jp + jchunk - 1) / jchunk;
+    CHK(c->feat.grow(sizeof(float) * (size_t)(c->I * kFeat)));
+    CHK(c->featpart.grow(sizeof(float) * (size_t)(nchunk * c->I * kFeat)));
+    {
+      ProfScope ps(c, "features");
+      launch_features(c->bed, c->pitch, c->I, Jp, static_cast<const float*>(c->st[0].p),
+                      static_cast<const float*>(c->st[1].p), static_cast<const float*>(c->st[2].p), (int)nchunk, jchunk,
+                      static_cast<float*>(c->featpart.p), static_cast<float*>(c->feat.p), c->s);
+    }
+    CHK(check_launch());
+    c->feat_rows = c->I;
+    if (F) HIPCHK(hipMemcpyAsync(F, c->feat.p, sizeof(float) * (size_t)(c->I * kFeat), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 int admx_warm_q(admx_ctx* c, admx_mlp* m) {
-  CHK(check_ctx(c));
-  if (!m) return bad("null network");
-  Guard g(c);
-  MGuard gm(m);
-  if (c->dtype != ADMX_F64) return bad("the warm start writes a FP64 context");
-  if (m->device != c->device) return bad("network and context must live on the same device");
-  if (m->in != kFeat || m->out != c->K) return bad("network input must be 64 features and its output K");
-  if (c->feat_rows != c->I) return fail(ADMX_ESTATE, "no features: call admx_features first");
-  HIPCHK(hipStreamSynchronize(m->s));
-  const int64_t Bmax = 65536;
-  MlpActs a;
-  CHK(mlp_acts(m, std::min<int64_t>(c->I, Bmax), &a));
-  CHK(c->st[3].grow(sizeof(float) * (size_t)(c->I * c->K)));
-  float* Y = static_cast<float*>(c->st[3].p);
-  const float* Fd = static_cast<const float*>(c->feat.p);
-  {
-    ProfScope ps(c, "warm_q");
-    for (int64_t r0 = 0; r0 < c->I; r0 += Bmax) {
-      const int64_t B = std::min(Bmax, c->I - r0);
-      mlp_forward(m, Fd + r0 * kFeat, nullptr, (int)B, a, Y + r0 * c->K, c->s);
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    if (!m) return bad("null network");
+    Guard g(c);
+    MGuard gm(m);
+    if (c->dtype != ADMX_F64) return bad("the warm start writes a FP64 context");
+    if (m->device != c->device) return bad("network and context must live on the same device");
+    if (m->in != kFeat || m->out != c->K) return bad("network input must be 64 features and its output K");
+    if (c->feat_rows != c->I) return fail(ADMX_ESTATE, "no features: call admx_features first");
+    HIPCHK(hipStreamSynchronize(m->s));
+    const int64_t Bmax = 65536;
+    MlpActs a;
+    CHK(mlp_acts(m, std::min<int64_t>(c->I, Bmax), &a));
+    CHK(c->st[3].grow(sizeof(float) * (size_t)(c->I * c->K)));
+    float* Y = static_cast<float*>(c->st[3].p);
+    const float* Fd = static_cast<const float*>(c->feat.p);
+    {
+      ProfScope ps(c, "warm_q");
+      for (int64_t r0 = 0; r0 < c->I; r0 += Bmax) {
+        const int64_t B = std::min(Bmax, c->I - r0);
+        mlp_forward(m, Fd + r0 * kFeat, nullptr, (int)B, a, Y + r0 * c->K, c->s);
+      }
+      double* q = static_cast<double*>(c->x[ADMX_X]);
+      launch_probs_to_q(Y, c->K * c->I, q, c->s);
+      launch_project_q<double>((int)c->K, c->I, q, c->s);
     }
-    double* q = static_cast<double*>(c->x[ADMX_X]);
-    launch_probs_to_q(Y, c->K * c->I, q, c->s);
-    launch_project_q<double>((int)c->K, c->I, q, c->s);
-  }
-  CHK(check_launch());
-  HIPCHK(hipStreamSynchronize(c->s));
-  return ADMX_OK;
+    CHK(check_launch());
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
 }
 
 }  // extern "C"
