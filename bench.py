@@ -527,7 +527,8 @@ def main():
                    "I": I, "J": J, "K": K, "genotypes": I * J, "bed_bytes": ((I + 3) // 4) * J},
         "roofline": {"bound": "mfma", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": achieved / peak,
-                     "traffic": measured_traffic(dom, args.workload, args.dtype if args.form == "plain" else args.form),
+                     "traffic": None if args.stream_snps else
+                     measured_traffic(dom, args.workload, args.dtype if args.form == "plain" else args.form),
                      "note": f"{FLOPS[base(dom)](K)} algorithmic flops/genotype x {I * J} genotypes per launch; "
                              f"{args.dtype} vector ALU (MI355X dense FP64 vector = matrix = 78.6 TF)"},
         "kernels": kernels,
