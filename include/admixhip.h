@@ -7,7 +7,9 @@
  *     k + K*j (K x J), x = [q p] (K x (I+J)), XtX[k2,k,n] at k2 + K*k + K*K*n,
  *     U/V at n + K*(I+J)*c (K(I+J) x Q);
  *   - element type is the context's dtype: double (ADMX_F64) or float (ADMX_F32);
- *     log-likelihoods are always double;
+ *     log-likelihoods are always double.  The genotype passes of an ADMX_F32
+ *     context run in FP64 on widened copies of q and p (same speed on gfx950,
+ *     fewer instructions); its state, Hessians and QPs stay float;
  *   - index ranges are 0-based half-open [i0,i1) x [j0,j1); the reference's
  *     1-based UnitRange a:b is [a-1, b);
  *   - host pointers are read or written only during the call (the caller keeps

@@ -145,7 +145,8 @@ struct admx_ctx {
   uint8_t* dres = nullptr;        // pitch * (res_snps + 16)
   int impute = 0;                 // 1: mean-imputation loops (skipmissing = false)
   bool mu_valid = false;          // mu holds the per-SNP means of the current genotypes
-  DevBuf snp_mu, cnt;             // per-SNP means (T, J); count scratch
+  DevBuf snp_mu, cnt;             // per-SNP means (FP64, J); count scratch
+  DevBuf wide[4];                 // FP32 context: FP64 copies of a pass's q and p (and a second state's)
   int64_t feat_rows = 0;          // individuals with features in `feat` (0: none)
   // sparse k-means feature ranking scratch: labels, cluster masks, small
   // counters, SNP counts, centers-pass partials, SNP stats (mu, v, union key),
@@ -455,17 +456,39 @@ template <typename T>
 int ensure_mu(admx_ctx* c) {
   if (!c->impute || c->mu_valid) return ADMX_OK;
   CHK(need_resident(c, "the mean-imputation loops"));
-  CHK(c->snp_mu.grow(sizeof(T) * (size_t)c->J));
+  CHK(c->snp_mu.grow(sizeof(double) * (size_t)c->J));
   CHK(c->cnt.grow(sizeof(unsigned long long) * (size_t)(2 * c->J + c->I)));
   auto* nm = static_cast<unsigned long long*>(c->cnt.p);
   {
     ProfScope ps(c, "snp_mean");
     launch_counts(c->bed, c->pitch, c->I, c->J, nm, nm + c->J, nm + 2 * c->J, c->s);
-    launch_snp_mean<T>(nm, nm + c->J, c->J, static_cast<T*>(c->snp_mu.p), c->s);
+    launch_snp_mean<double>(nm, nm + c->J, c->J, static_cast<double*>(c->snp_mu.p), c->s);
   }
   CHK(check_launch());
   c->mu_valid = true;
   return ADMX_OK;
+}
+
+// The genotype passes run in FP64 in every context: on gfx950 FP32 and FP64
+// FMAs issue at the same rate, and the FP64 rank-1 Hessian form needs fewer
+// instructions than the two-term form FP32 would need.  An FP32 context hands
+// the pass FP64 copies of its q and p (scratch slot w, n values); an FP64
+// context's arrays pass through.
+template <typename T>
+int widen(admx_ctx* c, int w, const void* x, int64_t n, const void** out) {
+  if constexpr (sizeof(T) == 8) {
+    *out = x;
+    return ADMX_OK;
+  } else {
+    if (!x) {
+      *out = nullptr;
+      return ADMX_OK;
+    }
+    CHK(c->wide[w].grow(sizeof(double) * (size_t)n));
+    launch_widen(static_cast<const float*>(x), n, static_cast<double*>(c->wide[w].p), c->s);
+    *out = c->wide[w].p;
+    return check_launch();
+  }
 }
 
 // Run a Q-side pass; returns the chunk count (partials in c->part or c->llpart).
@@ -479,6 +502,10 @@ int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
   *nparts = 0;
   if (Ilen <= 0 || Jlen <= 0) return ADMX_OK;
   CHK(ensure_mu<T>(c));
+  CHK(widen<T>(c, 0, q, c->K * c->I, &q));
+  CHK(widen<T>(c, 1, p, c->K * c->J, &p));
+  CHK(widen<T>(c, 2, q2, c->K * c->I, &q2));
+  CHK(widen<T>(c, 3, p2, c->K * c->J, &p2));
   // one launch per SNP chunk (one for a resident BED); their partial slots sit side by side
   const auto chunks = snp_chunks(c, j0, j1);
   std::vector<int> ns;
@@ -491,9 +518,9 @@ int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
     total += ns.back();
   }
   const int64_t nbx = (Ilen + 255) / 256;
-  const int na = pass_na(mode, true, (int)c->K, sizeof(T) == 8);
+  const int na = pass_na(mode, true, (int)c->K, true);
   if (mode == MODE_LL) CHK(c->llpart.grow((size_t)(nbx * total) * (q2 ? 2 : 1) * sizeof(double)));
-  else CHK(c->part.grow((size_t)total * na * Ilen * sizeof(T)));
+  else CHK(c->part.grow((size_t)total * na * Ilen * sizeof(double)));
   // chunk k's partial slots start after those of chunks 0..k-1, whatever order
   // the chunks run in
   std::vector<int64_t> first(chunks.size(), 0);
@@ -509,12 +536,12 @@ int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
           a2.q = q2;
           a2.p = p2;
           a2.part = static_cast<double*>(c->llpart.p) + nbx * (total + first[k]);
-          launch_qside<T>(mode, (int)c->K, a2, ns[k], c->s);
+          launch_qside<double>(mode, (int)c->K, a2, ns[k], c->s);
         }
       } else {
-        a.part = static_cast<T*>(c->part.p) + first[k] * na * Ilen;
+        a.part = static_cast<double*>(c->part.p) + first[k] * na * Ilen;
       }
-      launch_qside<T>(mode, (int)c->K, a, ns[k], c->s);
+      launch_qside<double>(mode, (int)c->K, a, ns[k], c->s);
       return check_launch();
     }));
   }
@@ -531,15 +558,17 @@ int pass_p(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
   if (Ilen <= 0 || Jlen <= 0) return ADMX_OK;
   int64_t chunk = 0;
   const int n = pside_chunks(c, i0, i1, Jlen, &chunk);
-  const int na = pass_na(mode, false, (int)c->K, sizeof(T) == 8);
-  CHK(c->part.grow((size_t)n * na * Jlen * sizeof(T)));
+  const int na = pass_na(mode, false, (int)c->K, true);
+  CHK(c->part.grow((size_t)n * na * Jlen * sizeof(double)));
   CHK(ensure_mu<T>(c));
+  CHK(widen<T>(c, 0, q, c->K * c->I, &q));
+  CHK(widen<T>(c, 1, p, c->K * c->J, &p));
   {
     ProfScope ps(c, mode_name(false, mode));
     CHK(for_chunks(c, j0, j1, [&](const uint8_t* base, int64_t cj0, int64_t cj1, size_t) {
       PassArgs a{base, c->pitch, c->I, q, p, i0, i1, cj0, cj1, chunk, c->part.p, c->impute ? c->snp_mu.p : nullptr,
                  Jlen, cj0 - j0};
-      launch_pside<T>(mode, (int)c->K, a, n, c->s);
+      launch_pside<double>(mode, (int)c->K, a, n, c->s);
       return check_launch();
     }));
   }
@@ -852,7 +881,8 @@ uint64_t layout_sig(const admx_ctx* c) {
     h ^= v;
     h *= 1099511628211ULL;
   };
-  const DevBuf* bufs[] = {&c->part, &c->llpart, &c->zal, &c->fq, &c->fp, &c->ppt, &c->mmt, &c->qqt};
+  const DevBuf* bufs[] = {&c->part,  &c->llpart, &c->zal,     &c->fq,      &c->fp,     &c->ppt,
+                          &c->mmt,   &c->qqt,    &c->wide[0], &c->wide[1], &c->wide[2], &c->wide[3]};
   for (const DevBuf* b : bufs) {
     mix((uint64_t)(uintptr_t)b->p);
     mix((uint64_t)b->n);
@@ -1410,6 +1440,7 @@ int admx_destroy(admx_ctx* c) {
     c->featpart.release();
     c->snp_mu.release();
     c->cnt.release();
+    for (auto& b : c->wide) b.release();
     for (DevBuf* b : {&c->sk_lab, &c->sk_cm, &c->sk_small, &c->sk_cnt, &c->sk_part, &c->sk_stat, &c->sk_E, &c->sk_key,
                       &c->sk_keys, &c->sk_idx, &c->sk_sidx, &c->sk_flag, &c->sk_sel, &c->sk_T, &c->sk_P, &c->sk_vec,
                       &c->sk_Dc, &c->sk_tmp})

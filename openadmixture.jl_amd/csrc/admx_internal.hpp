@@ -22,7 +22,7 @@ struct PassArgs {
   int64_t i0, i1, j0, j1;
   int64_t chunk;   // SNPs per chunk (Q side) / individuals per chunk, multiple of 16 (P side)
   void* part;      // partial sums [nchunk][NA][n] (UPDATE/EM) or per-block doubles (LL)
-  const void* mu = nullptr;  // per-SNP means (T, J): mean-imputation loops; nullptr: skipmissing
+  const void* mu = nullptr;  // per-SNP means (FP64, J): mean-imputation loops; nullptr: skipmissing
   int64_t ldp = 0;           // P side: column stride of the partials (0: j1 - j0)
   int64_t col0 = 0;          // P side: column of SNP j0 in the partials
 };
@@ -134,6 +134,8 @@ void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, in
 // Rows [j0, j1) only, into bed + pitch * j (a streamed context's chunk buffer).
 void launch_gen_bed_rows(const double* Qt, const double* Pt, int64_t I, int64_t j0, int64_t j1, int K,
                          uint64_t seed, double miss_rate, uint8_t* bed, int64_t pitch, hipStream_t s);
+// y = (double)x, n values.
+void launch_widen(const float* x, int64_t n, double* y, hipStream_t s);
 // `rows` rows of a .bed file's payload (ibytes each, back to back; `real` of
 // them present, the rest become missing codes) into the pitched device layout.
 void launch_repitch(const uint8_t* raw, int64_t ibytes, int64_t real, int64_t rows, uint8_t* dst, int64_t pitch,

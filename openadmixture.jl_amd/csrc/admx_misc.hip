@@ -524,6 +524,17 @@ void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, in
   launch_gen_bed_rows(Qt, Pt, I, 0, J, K, seed, miss_rate, bed, pitch, s);
 }
 
+// FP64 copy of an FP32 array (an FP32 context's q or p for the FP64 passes).
+__global__ __launch_bounds__(256) void k_widen(const float* __restrict__ x, int64_t n, double* __restrict__ y) {
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) y[e] = (double)x[e];
+}
+
+void launch_widen(const float* x, int64_t n, double* y, hipStream_t s) {
+  if (n <= 0) return;
+  const int64_t blocks = std::min<int64_t>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(k_widen, dim3((unsigned)blocks), dim3(256), 0, s, x, n, y);
+}
+
 // BED rows as a .bed file holds them (ibytes per row, back to back) into the
 // device layout (pitch per row, bytes past ibytes zero); rows from `real` on
 // are the missing code.  One lane per 32-bit word of the destination.

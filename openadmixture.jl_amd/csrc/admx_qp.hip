@@ -13,6 +13,8 @@
 // touches consecutive addresses.  The active-set loop is the reference's
 // Jennrich-Sampson method step for step; arithmetic is evaluated without FMA
 // contraction, as Julia evaluates it outside @turbo.
+#include <type_traits>
+
 #include "admx_common.hpp"
 #include "admx_internal.hpp"
 #include "admx_project.hpp"
@@ -181,8 +183,11 @@ __device__ __forceinline__ int qp_from_hessian(T* __restrict__ L, int t, const T
 }
 
 // ------------------------------------------------------------------ Q side finalize
+// Partials of type PT are summed, and the rank-1 form recombined, in the wider
+// of T and PT (an FP32 context's passes run in FP64); the QP runs in T.
 template <typename T, int K, bool TRICK, bool DO_QP, typename PT = T>
 __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_q_update(FinArgs f) {
+  using AT = typename std::conditional<(sizeof(PT) > sizeof(T)), PT, T>::type;
   constexpr int NT = QPLanes<K>::N;
   constexpr int NTRI = Tri<K>::N;
   constexpr int NA = TRICK ? NTRI + 2 * K + 2 : NTRI + K;
@@ -192,13 +197,13 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_q_update(FinArgs f) {
   const int64_t col = (int64_t)blockIdx.x * NT + t;
   if (col >= f.n) return;
   const PT* __restrict__ part = static_cast<const PT*>(f.part);
-  T acc[NA];
+  AT acc[NA];
 #pragma unroll
   for (int a = 0; a < NA; ++a) acc[a] = 0;
   for (int c = 0; c < f.nchunk; ++c) {
     const PT* src = part + (int64_t)c * NA * f.n + col;
 #pragma unroll
-    for (int a = 0; a < NA; ++a) acc[a] += (T)src[(int64_t)a * f.n];
+    for (int a = 0; a < NA; ++a) acc[a] += (AT)src[(int64_t)a * f.n];
   }
   T Hf[K * K], xtz[K];
   {
@@ -207,13 +212,13 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_q_update(FinArgs f) {
     for (int k = 0; k < K; ++k)
 #pragma unroll
       for (int k2 = k; k2 < K; ++k2, ++n) {
-        T h = acc[n];
+        AT h = acc[n];
         if constexpr (TRICK) h = h + acc[NTRI + 2 * K] - acc[NTRI + K + k] - acc[NTRI + K + k2];
-        Hf[k2 + K * k] = h;
-        Hf[k + K * k2] = h;
+        Hf[k2 + K * k] = (T)h;
+        Hf[k + K * k2] = (T)h;
       }
 #pragma unroll
-    for (int k = 0; k < K; ++k) xtz[k] = TRICK ? acc[NTRI + k] + acc[NTRI + 2 * K + 1] : acc[NTRI + k];
+    for (int k = 0; k < K; ++k) xtz[k] = (T)(TRICK ? acc[NTRI + k] + acc[NTRI + 2 * K + 1] : acc[NTRI + k]);
   }
   const int64_t g = f.off + col;
   if (f.XtX) {
@@ -248,34 +253,33 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_q_update(FinArgs f) {
 }
 
 // em_q! finalize (src/algorithms_inner.jl:76-89): q_next = q .* E / 2J, or the
-// tile loop's q_next += q .* E.
+// tile loop's q_next += q .* E.  The FP64 partials (every context's passes run
+// in FP64) carry sum (c-d) p_k and sum d, K+1 per chunk; the sums are formed in
+// FP64 and the result is stored in T.
 template <typename T, int K>
 __global__ __launch_bounds__(256) void k_fin_q_em(FinArgs f) {
   const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (col >= f.n) return;
-  const T* __restrict__ part = static_cast<const T*>(f.part);
-  // FP64 partials carry sum (c-d) p_k and sum d (K+1 per chunk); FP32 the direct sums
-  constexpr int NA = sizeof(T) == 8 ? K + 1 : K;
-  T acc[K], dsum = 0;
+  const double* __restrict__ part = static_cast<const double*>(f.part);
+  constexpr int NA = K + 1;
+  double acc[K], dsum = 0;
 #pragma unroll
   for (int k = 0; k < K; ++k) acc[k] = 0;
   for (int c = 0; c < f.nchunk; ++c) {
-    const T* src = part + (int64_t)c * NA * f.n + col;
+    const double* src = part + (int64_t)c * NA * f.n + col;
 #pragma unroll
     for (int k = 0; k < K; ++k) acc[k] += src[(int64_t)k * f.n];
-    if constexpr (sizeof(T) == 8) dsum += src[(int64_t)K * f.n];
+    dsum += src[(int64_t)K * f.n];
   }
-  if constexpr (sizeof(T) == 8) {
 #pragma unroll
-    for (int k = 0; k < K; ++k) acc[k] += dsum;
-  }
+  for (int k = 0; k < K; ++k) acc[k] += dsum;
   const int64_t g = f.off + col;
   const T* q = static_cast<const T*>(f.x_in) + (int64_t)K * g;
   T* o = static_cast<T*>(f.x_out) + (int64_t)K * g;
   const T div = (T)(2 * f.J);
 #pragma unroll
   for (int k = 0; k < K; ++k) {
-    const T v = q[k] * acc[k];
+    const T v = q[k] * (T)acc[k];
     o[k] = f.accumulate ? o[k] + v : v / div;
   }
 }
@@ -283,6 +287,7 @@ __global__ __launch_bounds__(256) void k_fin_q_em(FinArgs f) {
 // ------------------------------------------------------------------ P side finalize
 template <typename T, int K, bool DO_QP, typename PT = T>
 __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_p_update(FinArgs f) {
+  using AT = typename std::conditional<(sizeof(PT) > sizeof(T)), PT, T>::type;
   constexpr int NT = QPLanes<K>::N;
   constexpr int NTRI = Tri<K>::N;
   constexpr int NA = NTRI + K;
@@ -292,13 +297,13 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_p_update(FinArgs f) {
   const int64_t col = (int64_t)blockIdx.x * NT + t;
   if (col >= f.n) return;
   const PT* __restrict__ part = static_cast<const PT*>(f.part);
-  T acc[NA];
+  AT acc[NA];
 #pragma unroll
   for (int a = 0; a < NA; ++a) acc[a] = 0;
   for (int c = 0; c < f.nchunk; ++c) {
     const PT* src = part + (int64_t)c * NA * f.n + col;
 #pragma unroll
-    for (int a = 0; a < NA; ++a) acc[a] += (T)src[(int64_t)a * f.n];
+    for (int a = 0; a < NA; ++a) acc[a] += (AT)src[(int64_t)a * f.n];
   }
   T Hf[K * K], xtz[K];
   {
@@ -307,11 +312,11 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_p_update(FinArgs f) {
     for (int k = 0; k < K; ++k)
 #pragma unroll
       for (int k2 = k; k2 < K; ++k2, ++n) {
-        Hf[k2 + K * k] = acc[n];
-        Hf[k + K * k2] = acc[n];
+        Hf[k2 + K * k] = (T)acc[n];
+        Hf[k + K * k2] = (T)acc[n];
       }
 #pragma unroll
-    for (int k = 0; k < K; ++k) xtz[k] = acc[NTRI + k];
+    for (int k = 0; k < K; ++k) xtz[k] = (T)acc[NTRI + k];
   }
   const int64_t g = f.off + col;
   if (f.XtX) {
@@ -347,12 +352,12 @@ template <typename T, int K>
 __global__ __launch_bounds__(256) void k_fin_p_em(FinArgs f) {
   const int64_t col = (int64_t)blockIdx.x * 256 + threadIdx.x;
   if (col >= f.n) return;
-  const T* __restrict__ part = static_cast<const T*>(f.part);
-  T A[K], B[K];
+  const double* __restrict__ part = static_cast<const double*>(f.part);  // FP64 partials (FP64 passes)
+  double A[K], B[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) A[k] = B[k] = 0;
   for (int c = 0; c < f.nchunk; ++c) {
-    const T* src = part + (int64_t)c * 2 * K * f.n + col;
+    const double* src = part + (int64_t)c * 2 * K * f.n + col;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
       A[k] += src[(int64_t)k * f.n];
@@ -366,13 +371,13 @@ __global__ __launch_bounds__(256) void k_fin_p_em(FinArgs f) {
     T* pt = static_cast<T*>(f.aux) + (int64_t)K * g;
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      pt[k] = pt[k] + p[k] * A[k];
-      o[k] = o[k] + ((T)1 - p[k]) * B[k];
+      pt[k] = pt[k] + p[k] * (T)A[k];
+      o[k] = o[k] + ((T)1 - p[k]) * (T)B[k];
     }
   } else {
 #pragma unroll
     for (int k = 0; k < K; ++k) {
-      const T pt = p[k] * A[k], pn = ((T)1 - p[k]) * B[k];
+      const T pt = p[k] * (T)A[k], pn = ((T)1 - p[k]) * (T)B[k];
       o[k] = pt / (pt + pn);
     }
   }
@@ -496,20 +501,21 @@ __global__ __launch_bounds__(256) void k_project_p(int64_t n, T* __restrict__ b)
 // ------------------------------------------------------------------ launchers
 static inline unsigned nblk(int64_t n, int nt) { return (unsigned)((n + nt - 1) / nt); }
 
+// The genotype passes of every context run in FP64 and write FP64 partials in
+// the rank-1 Q-side layout; T is the context's state and QP type.
 template <typename T, int K>
 static void fin_q_update_k(const FinArgs& f, bool do_qp, hipStream_t s) {
-  constexpr bool trick = sizeof(T) == 8;
   constexpr int NT = QPLanes<K>::N;
-  if (do_qp && fin_split<T, K, 1>(f, s, k_fin_q_update<T, K, trick, false>)) return;
-  if (do_qp) hipLaunchKernelGGL((k_fin_q_update<T, K, trick, true>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
-  else hipLaunchKernelGGL((k_fin_q_update<T, K, trick, false>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
+  if (do_qp && fin_split<T, K, 1>(f, s, k_fin_q_update<T, K, true, false, double>)) return;
+  if (do_qp) hipLaunchKernelGGL((k_fin_q_update<T, K, true, true, double>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
+  else hipLaunchKernelGGL((k_fin_q_update<T, K, true, false, double>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
 }
 template <typename T, int K>
 static void fin_p_update_k(const FinArgs& f, bool do_qp, hipStream_t s) {
   constexpr int NT = QPLanes<K>::N;
-  if (do_qp && fin_split<T, K, 0>(f, s, k_fin_p_update<T, K, false>)) return;
-  if (do_qp) hipLaunchKernelGGL((k_fin_p_update<T, K, true>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
-  else hipLaunchKernelGGL((k_fin_p_update<T, K, false>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
+  if (do_qp && fin_split<T, K, 0>(f, s, k_fin_p_update<T, K, false, double>)) return;
+  if (do_qp) hipLaunchKernelGGL((k_fin_p_update<T, K, true, double>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
+  else hipLaunchKernelGGL((k_fin_p_update<T, K, false, double>), dim3(nblk(f.n, NT)), dim3(NT), 0, s, f);
 }
 template <typename T, int K>
 static void fin_q_em_k(const FinArgs& f, hipStream_t s) {
