@@ -62,7 +62,8 @@ extern "C" {
                                    * every pass and entry point follows it (FP64/FP32 passes only) */
 #define ADMX_TUNE_RESIDENT_SNPS 6 /* streamed contexts: keep SNPs [0, value) in HBM (copied once, and
                                    * again after admx_upload_bed / admx_generate_bed); passes stream
-                                   * only the rest.  0 (default): stream everything */
+                                   * only the rest.  0 (default): stream everything; -1: as many as
+                                   * the free HBM holds, less max(4 GiB, 10%) kept for pass scratch */
 
 /* values of ADMX_TUNE_QN_MODE: admixture_qn!'s `mode` (src/algorithms_outer.jl:127-136) */
 #define ADMX_QN_ZAL 0

@@ -2256,12 +2256,19 @@ int admx_set_tuning(admx_ctx* c, int key, int64_t value) {
       if (!c->use_graphs) drop_graphs(c);
     } else if (key == ADMX_TUNE_RESIDENT_SNPS) {
       if (!c->stream_snps) return bad("resident SNPs apply to a streamed context");
-      if (value < 0) return bad("resident SNPs must be >= 0");
-      const int64_t R = std::min(value, c->J);
+      if (value < -1) return bad("resident SNPs must be >= 0, or -1 for as many as fit");
       HIPCHK(hipStreamSynchronize(c->s));
       if (c->dres) HIPCHK(hipFree(c->dres));
       c->dres = nullptr;
       c->res_snps = 0;
+      int64_t want = value;
+      if (want == -1) {  // as many rows as the free HBM holds beyond a reserve for pass scratch
+        size_t fr = 0, tot = 0;
+        HIPCHK(hipMemGetInfo(&fr, &tot));
+        const size_t reserve = std::max<size_t>((size_t)4 << 30, tot / 10);
+        want = fr > reserve ? std::max<int64_t>((int64_t)((fr - reserve) / (size_t)c->pitch) - 16, 0) : 0;
+      }
+      const int64_t R = std::min(want, c->J);
       if (R > 0) {
         if (hipMalloc((void**)&c->dres, (size_t)c->pitch * (R + 16)) != hipSuccess) {
           (void)hipGetLastError();

@@ -354,7 +354,9 @@ class HIPAdmixData:
         self._precision = mode
 
     def set_resident_snps(self, n: int):
-        """Streamed context: keep SNPs [0, n) in HBM; passes stream only the rest."""
+        """Streamed context: keep SNPs [0, n) in HBM; passes stream only the
+        rest.  n = -1: as many as the free HBM holds (a reserve stays free for
+        the passes' scratch)."""
         self.set_tuning(N.ADMX_TUNE_RESIDENT_SNPS, int(n))
 
     def set_graphs(self, on: bool = True):

@@ -89,8 +89,9 @@ def _plink(oa, tmp_path, bed, pitch, I, J):
     return oa.write_plink(str(tmp_path / "cohort"), data, I, J), data
 
 
-# (chunk SNPs, SNPs kept in HBM): none, part, all (R = J), more than J (clamped)
-@pytest.mark.parametrize("S,R", [(16, 0), (700, 0), (700, 1000), (333, 2100), (1000, 5000), (100, 37)])
+# (chunk SNPs, SNPs kept in HBM): none, part, all (R = J), more than J (clamped),
+# as many as fit (-1: all of this small cohort)
+@pytest.mark.parametrize("S,R", [(16, 0), (700, 0), (700, 1000), (333, 2100), (1000, 5000), (100, 37), (100, -1)])
 def test_file_streamed_loops_match_resident(oa, O, tmp_path, S, R):
     """A context streaming straight from the .bed file (pread into pinned
     staging, re-pitched in HBM), with SNPs [0, R) resident: every loop matches
@@ -178,7 +179,7 @@ def test_resident_snps_follow_the_genotypes(oa, O):
     with pytest.raises(oa.AdmixHIPError):
         a.set_resident_snps(10)  # resident context
     with pytest.raises(oa.AdmixHIPError):
-        b.set_resident_snps(-1)
+        b.set_resident_snps(-2)
     a.close()
     b.close()
 
