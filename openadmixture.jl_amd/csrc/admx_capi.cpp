@@ -695,10 +695,10 @@ int mixed_update(admx_ctx* c, bool qside, const double* q, const double* p, cons
                           : (bf16 ? "pside_update_bf16" : "pside_update_f32"));
     if (qside) {
       if (bf16) launch_qside_bf16((int)K, a, n, fq, fp, c->ppt.p, c->mmt.p, ldj, c->s);
-      else launch_qside<float>(MODE_UPDATE, (int)K, a, n, c->s);
+      else launch_qside_f32_update((int)K, a, n, c->s);
     } else {
       if (bf16) launch_pside_bf16((int)K, a, n, fq, fp, c->qqt.p, ldi, c->s);
-      else launch_pside<float>(MODE_UPDATE, (int)K, a, n, c->s);
+      else launch_pside_f32_update((int)K, a, n, c->s);
     }
   }
   CHK(check_launch());

@@ -59,8 +59,12 @@ inline int pass_na(int mode, bool qside, int K, bool f64) {
   return 1;
 }
 
+// Genotype passes (FP64 kernels; instantiated for T = double only).
 template <typename T> void launch_qside(int mode, int K, const PassArgs& a, int nchunk, hipStream_t s);
 template <typename T> void launch_pside(int mode, int K, const PassArgs& a, int nchunk, hipStream_t s);
+// FP32 update passes of the FP32 form (FP32 direct-form partials).
+void launch_qside_f32_update(int K, const PassArgs& a, int nchunk, hipStream_t s);
+void launch_pside_f32_update(int K, const PassArgs& a, int nchunk, hipStream_t s);
 
 // Q side: update finalize (+QP simplex +project_q); EM-q finalize.
 template <typename T> void launch_fin_q_update(int K, const FinArgs& f, bool do_qp, hipStream_t s);

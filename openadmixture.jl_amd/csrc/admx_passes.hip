@@ -455,6 +455,17 @@ static void pside_k(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
   else pside_ki<T, K, false>(mode, a, nchunk, s);
 }
 
+#ifdef ADMX_PASSES_F32_FORM
+// The FP32 build serves only the update passes of the FP32 form (ADMX_PREC_F32
+// on an FP64 context, skipmissing loops): every context's other passes, FP32
+// contexts included, run the FP64 kernels.
+void launch_qside_f32_update(int K, const PassArgs& a, int nchunk, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (qside_ki<float, KK, false>(MODE_UPDATE, a, nchunk, s)));
+}
+void launch_pside_f32_update(int K, const PassArgs& a, int nchunk, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (pside_ki<float, KK, false>(MODE_UPDATE, a, nchunk, s)));
+}
+#else
 template <typename T> void launch_qside(int mode, int K, const PassArgs& a, int nchunk, hipStream_t s) {
   ADMX_DISPATCH_K(K, (qside_k<T, KK>(mode, a, nchunk, s)));
 }
@@ -464,5 +475,6 @@ template <typename T> void launch_pside(int mode, int K, const PassArgs& a, int 
 
 template void launch_qside<ADMX_T>(int, int, const PassArgs&, int, hipStream_t);
 template void launch_pside<ADMX_T>(int, int, const PassArgs&, int, hipStream_t);
+#endif
 
 }  // namespace admx
