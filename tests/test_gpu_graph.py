@@ -12,13 +12,15 @@ pytestmark = pytest.mark.gpu
 
 def _pair(oa, O, I, J, K, seed, form):
     Qt, Pt, bed, pitch = make_cohort(O, I, J, K, alpha=0.4, miss=0.02, seed=seed)
+    T = np.float32 if form == "ctx_f32" else np.float64  # ctx_f32: FP32 context (FP64 passes on widened q, p)
     out = []
     for graphs in (False, True):
-        c = oa.HIPAdmixData(I, J, K, 3)
+        c = oa.HIPAdmixData(I, J, K, 3, T)
         c.upload_bed(bed.reshape((pitch, J), order="F"))
-        c.set_precision(form)
+        if T == np.float64:
+            c.set_precision(form)
         c.set_graphs(graphs)
-        d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(seed + 1))
+        d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(seed + 1), T=T)
         oa.init_em(d, None, 3, d_cu=c, log=None)
         c.set_uv(d.U, d.V)
         c.set_ll(d.ll_prev, d.ll_new)
@@ -26,7 +28,7 @@ def _pair(oa, O, I, J, K, seed, form):
     return out
 
 
-@pytest.mark.parametrize("form", ["f64", "bf16"])
+@pytest.mark.parametrize("form", ["f64", "bf16", "ctx_f32"])
 def test_graph_replay_is_bit_identical(oa, O, form):
     print("graph replay", form, flush=True)
     I, J, K = 900, 2500, 4
