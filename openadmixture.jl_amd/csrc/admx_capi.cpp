@@ -147,6 +147,7 @@ struct admx_ctx {
   bool mu_valid = false;          // mu holds the per-SNP means of the current genotypes
   DevBuf snp_mu, cnt;             // per-SNP means (FP64, J); count scratch
   DevBuf wide[4];                 // FP32 context: FP64 copies of a pass's q and p (and a second state's)
+  DevBuf llflag;                  // two ints: log-likelihood state checks of a pass's two states
   int64_t feat_rows = 0;          // individuals with features in `feat` (0: none)
   // sparse k-means feature ranking scratch: labels, cluster masks, small
   // counters, SNP counts, centers-pass partials, SNP stats (mu, v, union key),
@@ -519,6 +520,18 @@ int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
   }
   const int64_t nbx = (Ilen + 255) / 256;
   const int na = pass_na(mode, true, (int)c->K, true);
+  // log-likelihood: the term-by-term form for states where some s may be 0 or 1
+  int* flags = nullptr;
+  if (mode == MODE_LL && !c->impute) {
+    CHK(c->llflag.grow(2 * sizeof(int)));
+    flags = static_cast<int*>(c->llflag.p);
+    launch_ll_state_check((int)c->K, static_cast<const double*>(q), c->I, static_cast<const double*>(p), c->J,
+                          flags, c->s);
+    if (q2)
+      launch_ll_state_check((int)c->K, static_cast<const double*>(q2), c->I, static_cast<const double*>(p2), c->J,
+                            flags + 1, c->s);
+    CHK(check_launch());
+  }
   if (mode == MODE_LL) CHK(c->llpart.grow((size_t)(nbx * total) * (q2 ? 2 : 1) * sizeof(double)));
   else CHK(c->part.grow((size_t)total * na * Ilen * sizeof(double)));
   // chunk k's partial slots start after those of chunks 0..k-1, whatever order
@@ -531,10 +544,12 @@ int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
       PassArgs a{base, c->pitch, c->I, q, p, i0, i1, cj0, cj1, chs[k], nullptr, c->impute ? c->snp_mu.p : nullptr};
       if (mode == MODE_LL) {
         a.part = static_cast<double*>(c->llpart.p) + nbx * first[k];
+        a.strict = flags;
         if (q2) {
           PassArgs a2 = a;
           a2.q = q2;
           a2.p = p2;
+          a2.strict = flags ? flags + 1 : nullptr;
           a2.part = static_cast<double*>(c->llpart.p) + nbx * (total + first[k]);
           launch_qside<double>(mode, (int)c->K, a2, ns[k], c->s);
         }
@@ -882,7 +897,8 @@ uint64_t layout_sig(const admx_ctx* c) {
     h *= 1099511628211ULL;
   };
   const DevBuf* bufs[] = {&c->part,  &c->llpart, &c->zal,     &c->fq,      &c->fp,     &c->ppt,
-                          &c->mmt,   &c->qqt,    &c->wide[0], &c->wide[1], &c->wide[2], &c->wide[3]};
+                          &c->mmt,   &c->qqt,    &c->wide[0], &c->wide[1], &c->wide[2], &c->wide[3],
+                          &c->llflag};
   for (const DevBuf* b : bufs) {
     mix((uint64_t)(uintptr_t)b->p);
     mix((uint64_t)b->n);
@@ -1441,6 +1457,7 @@ int admx_destroy(admx_ctx* c) {
     c->snp_mu.release();
     c->cnt.release();
     for (auto& b : c->wide) b.release();
+    c->llflag.release();
     for (DevBuf* b : {&c->sk_lab, &c->sk_cm, &c->sk_small, &c->sk_cnt, &c->sk_part, &c->sk_stat, &c->sk_E, &c->sk_key,
                       &c->sk_keys, &c->sk_idx, &c->sk_sidx, &c->sk_flag, &c->sk_sel, &c->sk_T, &c->sk_P, &c->sk_vec,
                       &c->sk_Dc, &c->sk_tmp})

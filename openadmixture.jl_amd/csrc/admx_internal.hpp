@@ -25,6 +25,10 @@ struct PassArgs {
   const void* mu = nullptr;  // per-SNP means (FP64, J): mean-imputation loops; nullptr: skipmissing
   int64_t ldp = 0;           // P side: column stride of the partials (0: j1 - j0)
   int64_t col0 = 0;          // P side: column of SNP j0 in the partials
+  // FP64 log-likelihood: device flag set when some s = q.p may be 0 or 1 (see
+  // launch_ll_state_check); the pass then sums g log s + (2-g) log(1-s) term by
+  // term instead of the product form.  nullptr: product form.
+  const int* strict = nullptr;
 };
 
 // Per-column reduction of pass partials, optional QP + projection.
@@ -138,6 +142,11 @@ void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, in
 // Rows [j0, j1) only, into bed + pitch * j (a streamed context's chunk buffer).
 void launch_gen_bed_rows(const double* Qt, const double* Pt, int64_t I, int64_t j0, int64_t j1, int K,
                          uint64_t seed, double miss_rate, uint8_t* bed, int64_t pitch, hipStream_t s);
+// *flag = 1 if the FP64 state (q: K x I, p: K x J) may give some s = q.p of
+// exactly 0 or 1, or holds a non-finite value; else 0.  Safe states: every p
+// in [1e-12, 1 - 1e-12], every q finite and >= 0, every column sum of q in
+// [1e-200, 1 + 1e-13]; then every s and 1 - s lies strictly inside (0, 1).
+void launch_ll_state_check(int K, const double* q, int64_t I, const double* p, int64_t J, int* flag, hipStream_t s);
 // y = (double)x, n values.
 void launch_widen(const float* x, int64_t n, double* y, hipStream_t s);
 // `rows` rows of a .bed file's payload (ibytes each, back to back; `real` of

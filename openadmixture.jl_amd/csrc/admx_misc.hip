@@ -524,6 +524,42 @@ void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, in
   launch_gen_bed_rows(Qt, Pt, I, 0, J, K, seed, miss_rate, bed, pitch, s);
 }
 
+// ------------------------------------------------------------------ log-likelihood state check
+// The FP64 log-likelihood pass multiplies s^g (1-s)^(2-g) into a product; the
+// reference sums m [g log s + (2-g) log(1-s)] (src/loops.jl:517-539), whose
+// terms 0 log 0 are NaN when s reaches exactly 0 or 1 (FRAPPE EM drives p to 0
+// or 1 on a SNP whose observed genotypes are all 0 or all 2).  The pass takes
+// the term-by-term form whenever this check cannot rule that out.
+__global__ void k_flag_clear(int* flag) { *flag = 0; }
+
+__global__ __launch_bounds__(256) void k_ll_state_check(int K, const double* __restrict__ q, int64_t I,
+                                                        const double* __restrict__ p, int64_t J, int* flag) {
+  bool bad = false;
+  const int64_t stride = (int64_t)gridDim.x * 256;
+  for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < I; i += stride) {
+    double sum = 0.0;
+    for (int k = 0; k < K; ++k) {
+      const double v = q[(int64_t)K * i + k];
+      bad |= !(v >= 0.0 && v <= 1e300);  // also NaN
+      sum += v;
+    }
+    bad |= !(sum >= 1e-200 && sum <= 1.0 + 1e-13);
+  }
+  const int64_t np = (int64_t)K * J;
+  for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < np; e += stride) {
+    const double v = p[e];
+    bad |= !(v >= 1e-12 && v <= 1.0 - 1e-12);
+  }
+  if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(flag, 1);
+}
+
+void launch_ll_state_check(int K, const double* q, int64_t I, const double* p, int64_t J, int* flag, hipStream_t s) {
+  hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(1), 0, s, flag);
+  const int64_t n = std::max<int64_t>(I, (int64_t)K * J);
+  const unsigned blocks = (unsigned)std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 1024);
+  hipLaunchKernelGGL(k_ll_state_check, dim3(blocks), dim3(256), 0, s, K, q, I, p, J, flag);
+}
+
 // FP64 copy of an FP32 array (an FP32 context's q or p for the FP64 passes).
 __global__ __launch_bounds__(256) void k_widen(const float* __restrict__ x, int64_t n, double* __restrict__ y) {
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < n; e += (int64_t)gridDim.x * 256) y[e] = (double)x[e];

@@ -39,8 +39,15 @@ namespace admx {
 // PART 1 with the first half of the Hessian triangle plus the vector sums and
 // PART 2 with the second half, so that each launch holds half the accumulators
 // (two waves per SIMD instead of one at K = 12).
-template <typename T, int K, int MODE, bool TRICK, bool IMP, int PART = 0>
+// STRICT (FP64 log-likelihood): the term-by-term form.  With a state-check
+// flag (a.strict) the product-form and term-by-term instances are launched one
+// after the other and the one that does not match the flag returns at once, so
+// the product form keeps its registers and occupancy.
+template <typename T, int K, int MODE, bool TRICK, bool IMP, int PART = 0, bool STRICT = false>
 __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
+  if constexpr (MODE == MODE_LL) {
+    if (a.strict && ((*a.strict != 0) != STRICT)) return;  // uniform: the other instance runs
+  }
   constexpr int NT = Tri<K>::N;
   constexpr int N0 = (PART == 2) ? NT / 2 : 0, N1 = (PART == 1) ? NT / 2 : NT;  // owned triangle entries
   constexpr bool VEC = PART != 2;                                               // owns z, hb, sb, sd
@@ -91,13 +98,14 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
     for (int k = 1; k < K; ++k) s = __builtin_fma(qi[k], pv[k], s);
     const T t = (T)1 - s;
     if constexpr (MODE == MODE_LL) {
-      if constexpr (sizeof(T) == 8 && !IMP) {
+      if constexpr (sizeof(T) == 8 && !IMP && !STRICT) {
         // prod_j s^g (1-s)^(2-g), renormalised every 8 SNPs; one log per lane.
         const double x = (c >= 2u) ? (double)s : (double)t;
         const double y = (c == 3u) ? (double)s : (double)t;
         prod *= (c == 1u) ? 1.0 : x * y;
       } else {
-        // Terms in T, accumulated in Float64 (src/loops.jl:523, 535).
+        // Terms in T, accumulated in Float64 (src/loops.jl:523, 535); a missing
+        // entry adds 0 log s + 0 log(1-s), NaN where s is 0 or 1, as there.
         T gm, hm;
         weights(c, j, gm, hm);
         const T term = gm * tlog<T>(s) + hm * tlog<T>(t);
@@ -154,7 +162,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
     }
   };
   auto renorm = [&]() {
-    if constexpr (MODE == MODE_LL && sizeof(T) == 8 && !IMP) {
+    if constexpr (MODE == MODE_LL && sizeof(T) == 8 && !IMP && !STRICT) {
       int e;
       prod = __builtin_frexp(prod, &e);
       esum += e;
@@ -232,7 +240,8 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
 
   if constexpr (MODE == MODE_LL) {
     __shared__ double sh_red[256];
-    if constexpr (sizeof(T) == 8 && !IMP) acc = log(prod) + (double)esum * 0.69314718055994530942;
+    // product form: acc is 0 and prod holds the terms; term by term: prod = 1, esum = 0
+    if constexpr (sizeof(T) == 8 && !IMP) acc += log(prod) + (double)esum * 0.69314718055994530942;
     const double tot = block_sum<256>(valid ? acc : 0.0, sh_red);
     if (threadIdx.x == 0) static_cast<double*>(a.part)[tile.y * nbx + tile.x] = tot;
     return;
@@ -425,7 +434,11 @@ static void qside_ki(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
     }
   }
   else if (mode == MODE_EM) hipLaunchKernelGGL((k_qside<T, K, MODE_EM, false, IMP>), grid, dim3(256), 0, s, a);
-  else hipLaunchKernelGGL((k_qside<T, K, MODE_LL, false, IMP>), grid, dim3(256), 0, s, a);
+  else {
+    hipLaunchKernelGGL((k_qside<T, K, MODE_LL, false, IMP>), grid, dim3(256), 0, s, a);
+    if constexpr (sizeof(T) == 8 && !IMP)
+      if (a.strict) hipLaunchKernelGGL((k_qside<T, K, MODE_LL, false, IMP, 0, true>), grid, dim3(256), 0, s, a);
+  }
 }
 template <typename T, int K>
 static void qside_k(int mode, const PassArgs& a, int nchunk, hipStream_t s) {

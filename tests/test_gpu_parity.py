@@ -253,8 +253,8 @@ def test_update_steps_without_kept_hessians(oa, O, K):
 
 
 # ------------------------------------------------------------------ algorithms on device state
-def _pair(oa, O, I, J, K, Q=3, T=F64, seed=0, miss=0.01, beta=False):
-    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, seed=seed, miss=miss, beta=beta)
+def _pair(oa, O, I, J, K, Q=3, T=F64, seed=0, miss=0.01, beta=False, p_lo=0.05, p_hi=0.95):
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, seed=seed, miss=miss, beta=beta, p_lo=p_lo, p_hi=p_hi)
     d = oa.AdmixData(I, J, K, Q, rng=np.random.default_rng(seed + 100), T=T)
     c = ctx_for(oa, bed, pitch, I, J, K, Q, T)
     ref = O.OracleAdmix(bed, pitch, I, J, K, Q, d.x, d.U, d.V, dtype=T)
@@ -287,10 +287,18 @@ def test_update_q_p_device(oa, O, K):
     c.close()
 
 
-@pytest.mark.parametrize("I,J,K", [(500, 3000, 3), (1000, 2000, 5)])
+# the last two: a cohort smaller than one wave and one SNP step, and one of
+# exactly one wave of individuals and one 16-SNP step (allele frequencies in
+# [0.4, 0.6], so that no SNP of these few individuals is monomorphic: see
+# test_loglikelihood_degenerate_states for that case)
+@pytest.mark.parametrize("I,J,K", [(500, 3000, 3), (1000, 2000, 5), (9, 17, 2), (64, 16, 3)])
 def test_em_and_qn_iterations_f64(oa, O, I, J, K):
     note("qn iterations f64", I, J, K)
-    d, c, ref = _pair(oa, O, I, J, K, seed=I)
+    tiny = I < 100
+    d, c, ref = _pair(oa, O, I, J, K, seed=I, p_lo=0.4 if tiny else 0.05, p_hi=0.6 if tiny else 0.95)
+    if tiny:
+        g = c.decode()
+        assert all(len(set(g[:, j][g[:, j] >= 0].tolist())) > 1 for j in range(J)), "monomorphic SNP"
     lls = oa.init_em(d, None, 5, d_cu=c, log=None)
     rlls = ref.init_em(5)
     np.testing.assert_allclose(lls, rlls, rtol=1e-10, atol=0)
@@ -355,3 +363,68 @@ def test_reproducible_bitwise(oa, O):
     assert outs[0][0] == outs[1][0]
     assert np.array_equal(outs[0][1], outs[1][1])
 
+
+
+def _ll_class(v):
+    return "nan" if np.isnan(v) else ("-inf" if v == -np.inf else "finite")
+
+
+def test_loglikelihood_degenerate_states(oa, O):
+    """States with s = q.p of exactly 0 or 1, as FRAPPE EM produces on a SNP
+    whose observed genotypes are all 0 (p -> 0): the reference's terms
+    m [g log s + (2-g) log(1-s)] (src/loops.jl:517-539) give NaN for 0 log 0
+    (g = 0 or a missing entry at s = 0) and -Inf for g log 0.  The tile loop
+    and the device-resident log-likelihood reproduce NaN / -Inf / finite as the
+    oracle does, finite values to 1e-10."""
+    note("degenerate LL")
+    I, J, K = 300, 64, 3
+    _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.02, seed=61)
+    B = bed.reshape((pitch, J), order="F").copy()
+    B[: I // 4, 20] = 0xAA  # SNP 20: every individual heterozygous (code 10), none missing
+    B[: I // 4, 21] = 0x00  # SNP 21: monomorphic, every genotype 0
+    bed2 = np.asfortranarray(B).ravel(order="F")
+    c = ctx_for(oa, bed2, pitch, I, J, K)
+    q, p = random_state(I, J, K, 7)
+    cases = {"plain": p}
+    for name, (cols, val) in {"zero_p_mixed_snp": ([5], 0.0), "zero_p_het_snp": ([20], 0.0),
+                              "zero_p_mono_snp": ([21], 0.0), "one_p": ([7], 1.0), "nan_p": ([9], np.nan),
+                              "tiny_p": ([11], 1e-13)}.items():
+        pp = p.copy()
+        pp[:, cols] = val
+        cases[name] = np.asfortranarray(pp)
+    seen = set()
+    for name, pp in cases.items():
+        ref = O.loglikelihood(bed2, pitch, q, pp)
+        loop = c.loglikelihood_loop(q, pp, 0, I, 0, J)
+        c.set_x(0, np.asfortranarray(np.hstack([q, pp])))
+        dev = c.loglikelihood(0)
+        print(name, ref, loop, dev, flush=True)
+        for got in (loop, dev):
+            assert _ll_class(got) == _ll_class(ref), (name, got, ref)
+            if np.isfinite(ref):
+                assert abs(got - ref) <= 1e-10 * abs(ref), (name, got, ref)
+        seen.add(_ll_class(ref))
+    assert seen == {"nan", "-inf", "finite"}
+    c.close()
+
+
+def test_em_on_a_monomorphic_snp_matches_reference(oa, O):
+    """FRAPPE EM from a random start on a cohort with a SNP whose genotypes
+    are all 0: p of that SNP becomes exactly 0 after the first step, so the
+    reference's log-likelihood is NaN from there on; the device's EM
+    log-likelihoods match the oracle's, NaN for NaN."""
+    note("EM monomorphic")
+    I, J, K = 200, 300, 3
+    _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.0, seed=62)
+    B = bed.reshape((pitch, J), order="F").copy()
+    B[: I // 4, 40] = 0x00
+    bed2 = np.asfortranarray(B).ravel(order="F")
+    d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(5))
+    c = ctx_for(oa, bed2, pitch, I, J, K)
+    ref = O.OracleAdmix(bed2, pitch, I, J, K, 3, d.x, d.U, d.V)
+    lls = oa.init_em(d, None, 3, d_cu=c, log=None)
+    rlls = ref.init_em(3)
+    print(lls, rlls, flush=True)
+    assert [_ll_class(v) for v in lls] == [_ll_class(v) for v in rlls]
+    assert _ll_class(rlls[0]) == "nan"
+    c.close()
