@@ -21,264 +21,15 @@
 #include <vector>
 
 #include "admixhip.h"
+#include "admx_host.hpp"
 #include "admx_internal.hpp"
 
 using namespace admx;
+using namespace admx_host;
+
+thread_local std::string admx_host::g_err;
 
 namespace {
-
-thread_local std::string g_err;
-
-int fail(int code, const std::string& msg) {
-  g_err = msg;
-  return code;
-}
-
-// Host-side exceptions (allocation failure, thread creation) never cross the C
-// ABI: every entry point runs inside guarded(), which turns them into
-// ADMX_EHOST with the message in admx_last_error().
-int fail_nothrow(const char* msg) noexcept {
-  try {
-    g_err = msg;
-  } catch (...) {
-    g_err.clear();
-  }
-  return ADMX_EHOST;
-}
-
-template <typename F>
-int guarded(F&& f) noexcept {
-  try {
-    return f();
-  } catch (const std::exception& e) {
-    return fail_nothrow(e.what());
-  } catch (...) {
-    return fail_nothrow("unknown host-side exception");
-  }
-}
-
-#define HIPCHK(expr)                                                                   \
-  do {                                                                                 \
-    hipError_t e_ = (expr);                                                            \
-    if (e_ != hipSuccess) return fail(ADMX_EHIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
-
-#define CHK(expr)            \
-  do {                       \
-    int r_ = (expr);         \
-    if (r_ != ADMX_OK) return r_; \
-  } while (0)
-
-struct DevBuf {
-  void* p = nullptr;
-  size_t n = 0;
-  int grow(size_t bytes) {
-    if (bytes <= n) return ADMX_OK;
-    if (p) HIPCHK(hipFree(p));
-    p = nullptr;
-    n = 0;
-    const size_t want = bytes < 256 ? 256 : bytes;
-    HIPCHK(hipMalloc(&p, want));
-    n = want;
-    return ADMX_OK;
-  }
-  void release() {
-    if (p) (void)hipFree(p);
-    p = nullptr;
-    n = 0;
-  }
-};
-
-struct ProfEv {
-  const char* name;
-  hipEvent_t a, b;
-};
-
-struct Prof {
-  bool on = false;
-  std::vector<ProfEv> pend;
-  std::vector<hipEvent_t> pool;
-  std::map<std::string, std::pair<long, double>> acc;
-};
-
-}  // namespace
-
-struct admx_ctx {
-  int64_t I = 0, J = 0, K = 0, Q = 0;
-  int dtype = ADMX_F64;
-  size_t esz = 8;
-  int64_t ibytes = 0, pitch = 0;
-  int device = 0;
-  hipStream_t s = nullptr;
-  uint8_t* bed = nullptr;
-  bool have_bed = false;
-  void* x[4] = {nullptr, nullptr, nullptr, nullptr};
-  void *XtX_q = nullptr, *Xtz_q = nullptr, *XtX_p = nullptr, *Xtz_p = nullptr;
-  void *U = nullptr, *V = nullptr;
-  double* lls = nullptr;
-  double* h_lls = nullptr;
-  int32_t* status = nullptr;
-  DevBuf part, llpart, zal;
-  DevBuf st[8];  // staging for host-array entry points
-  int64_t target_waves = 65536;
-  int keep_hess = 1;
-  int precision = ADMX_PREC_F64;
-  DevBuf fq, fp, ppt, mmt, qqt;  // mixed-precision pass inputs
-  DevBuf lqt, lpt, lmt;           // BF16 split tables of the matrix-core log-likelihood
-  DevBuf feat, featpart;          // warm-start features (I x 64) and their chunk partials
-  // J-chunk streaming: the BED lives in pinned host memory, or in a PLINK .bed
-  // file read chunk by chunk, and every pass streams it through two device
-  // chunk buffers on a copy stream.  The first res_snps SNPs can stay in HBM.
-  int64_t stream_snps = 0;        // SNPs per chunk (0: the BED is resident in HBM)
-  uint8_t* hbed = nullptr;        // pinned host BED, pitch * (J + 16)
-  uint8_t* dchunk[2] = {nullptr, nullptr};
-  hipStream_t cs = nullptr;       // copy stream (default priority)
-  hipEvent_t ev_ready[2] = {nullptr, nullptr}, ev_free[2] = {nullptr, nullptr};
-  bool ev_used[2] = {false, false};  // ev_ready[b] has been recorded
-  int64_t held[2][2] = {{0, 0}, {0, 0}};  // rows [held[b][0], held[b][1]) of the BED are in dchunk[b]
-  int last_buf = 1;               // chunk buffer of the latest streamed chunk
-  bool stream_rev = true;         // the latest pass ran its streamed chunks backwards
-  int bed_fd = -1;                // file-backed: the .bed file (rows of ibytes at offset 3)
-  uint8_t* hstage[2] = {nullptr, nullptr};  // file-backed: pinned raw rows, ibytes * (S + 16)
-  uint8_t* draw[2] = {nullptr, nullptr};    // file-backed: the same rows in HBM, before re-pitching
-  int64_t res_snps = 0;           // SNPs [0, res_snps) resident in `dres`
-  uint8_t* dres = nullptr;        // pitch * (res_snps + 16)
-  int impute = 0;                 // 1: mean-imputation loops (skipmissing = false)
-  bool mu_valid = false;          // mu holds the per-SNP means of the current genotypes
-  DevBuf snp_mu, cnt;             // per-SNP means (FP64, J); count scratch
-  DevBuf wide[4];                 // FP32 context: FP64 copies of a pass's q and p (and a second state's)
-  DevBuf llflag;                  // two ints: log-likelihood state checks of a pass's two states
-  int64_t feat_rows = 0;          // individuals with features in `feat` (0: none)
-  // sparse k-means feature ranking scratch: labels, cluster masks, small
-  // counters, SNP counts, centers-pass partials, SNP stats (mu, v, union key),
-  // E, keys and sorted keys, index and sorted index, flags, distance tables,
-  // table-sum partials, per-individual vectors (x2, dmin, D2, WSS), k-means++
-  // distances, sort scratch
-  DevBuf sk_lab, sk_cm, sk_small, sk_cnt, sk_part, sk_stat, sk_E, sk_key, sk_keys, sk_idx, sk_sidx, sk_flag, sk_sel,
-      sk_T, sk_P, sk_vec, sk_Dc, sk_tmp;
-  std::mutex mu;
-  Prof prof;
-  // hipGraph replay of steady-state quasi-Newton iterations (iter > Q): one
-  // executable graph per (history column, fix_q, fix_p), valid while the
-  // buffer layout signature graph_sig holds; graph_warm has bit (fix_q | 2 fix_p)
-  // set once an eager iteration of that kind ran under the signature.
-  int use_graphs = 0;
-  int qn_mode = ADMX_QN_ZAL;
-  struct GraphEntry {
-    int64_t key;
-    hipGraphExec_t exec;
-  };
-  std::vector<GraphEntry> graphs;
-  uint64_t graph_sig = 0;
-  unsigned graph_warm = 0;
-
-  int64_t N() const { return K * (I + J); }
-  void* qp(void* v) const { return v; }
-  void* pp(void* v) const { return static_cast<char*>(v) + (size_t)(K * I) * esz; }
-};
-
-// The warm-start network: FP32 parameters, gradients and Adam moments as flat
-// vectors [W1 (hid x in), b1, W2 (hid x hid), b2, W3 (out x hid), b3], plus
-// training data and per-batch activations.
-struct admx_mlp {
-  int in = 0, hid = 0, out = 0;
-  int device = 0;
-  hipStream_t s = nullptr;
-  int64_t P = 0;
-  int64_t step = 0;
-  DevBuf w, g, m, v;
-  DevBuf X, T, idx;  // training set (n x in, n x out) and the epoch's row order
-  int64_t n = 0;
-  DevBuf act;        // Xb, Tb, H1, H2, Z, Y, dZ, dH2, dH1, rowloss, loss slots for a batch
-  int64_t act_rows = 0;
-  std::mutex mu;
-  int64_t oW1() const { return 0; }
-  int64_t ob1() const { return (int64_t)hid * in; }
-  int64_t oW2() const { return ob1() + hid; }
-  int64_t ob2() const { return oW2() + (int64_t)hid * hid; }
-  int64_t oW3() const { return ob2() + hid; }
-  int64_t ob3() const { return oW3() + (int64_t)out * hid; }
-};
-
-namespace {
-
-hipEvent_t ev_get(admx_ctx* c) {
-  if (!c->prof.pool.empty()) {
-    hipEvent_t e = c->prof.pool.back();
-    c->prof.pool.pop_back();
-    return e;
-  }
-  hipEvent_t e = nullptr;
-  (void)hipEventCreate(&e);
-  return e;
-}
-
-struct ProfScope {
-  admx_ctx* c;
-  const char* name;
-  hipEvent_t a = nullptr, b = nullptr;
-  ProfScope(admx_ctx* c_, const char* n) : c(c_), name(n) {
-    if (c->prof.on) {
-      a = ev_get(c);
-      b = ev_get(c);
-      (void)hipEventRecord(a, c->s);
-    }
-  }
-  ~ProfScope() {
-    if (c->prof.on) {
-      (void)hipEventRecord(b, c->s);
-      c->prof.pend.push_back({name, a, b});
-    }
-  }
-};
-
-int prof_collect(admx_ctx* c) {
-  if (c->prof.pend.empty()) return ADMX_OK;
-  HIPCHK(hipStreamSynchronize(c->s));
-  for (auto& e : c->prof.pend) {
-    float ms = 0.f;
-    HIPCHK(hipEventElapsedTime(&ms, e.a, e.b));
-    auto& slot = c->prof.acc[e.name];
-    slot.first += 1;
-    slot.second += ms;
-    c->prof.pool.push_back(e.a);
-    c->prof.pool.push_back(e.b);
-  }
-  c->prof.pend.clear();
-  return ADMX_OK;
-}
-
-// The 16 SNP rows after the last one hold the missing code (01 in every field):
-// passes that step 16 SNPs at a time read them in the cohort's last step.
-int pad_bed(admx_ctx* c) {
-  HIPCHK(hipMemsetAsync(c->bed + (size_t)c->pitch * c->J, 0x55, (size_t)c->pitch * 16, c->s));
-  return ADMX_OK;
-}
-
-int check_launch() {
-  hipError_t e = hipGetLastError();
-  if (e != hipSuccess) return fail(ADMX_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
-  return ADMX_OK;
-}
-
-int bad(const char* msg) { return fail(ADMX_EINVAL, msg); }
-
-int check_ctx(admx_ctx* c) {
-  if (!c) return bad("null context");
-  HIPCHK(hipSetDevice(c->device));
-  return ADMX_OK;
-}
-
-int check_ranges(admx_ctx* c, int64_t i0, int64_t i1, int64_t j0, int64_t j1) {
-  if (i0 < 0 || i1 < i0 || i1 > c->I || j0 < 0 || j1 < j0 || j1 > c->J)
-    return bad("range outside [0,I) x [0,J) or reversed");
-  return ADMX_OK;
-}
-
-int need_bed(admx_ctx* c) {
-  if (!c->have_bed) return fail(ADMX_ESTATE, "no genotypes: call admx_upload_bed or admx_generate_bed first");
-  return ADMX_OK;
-}
 
 // ---- chunking of the genotype passes
 int qside_chunks(admx_ctx* c, int64_t Ilen, int64_t Jlen, int64_t* chunk, int rows_per_wave = 64) {
@@ -442,12 +193,6 @@ int refresh_resident(admx_ctx* c) {
     c->ev_used[0] = true;
   }
   HIPCHK(hipStreamSynchronize(c->cs));
-  return ADMX_OK;
-}
-
-
-int need_resident(admx_ctx* c, const char* what) {
-  if (c->stream_snps) return fail(ADMX_ESTATE, std::string(what) + " needs a resident BED (not a streamed context)");
   return ADMX_OK;
 }
 
@@ -988,11 +733,6 @@ int dev_init_em(admx_ctx* c, int64_t iters, double* lls) {
 }
 
 // ---- host-array tile loops (stage, run, accumulate on device, copy back)
-int stage_in(admx_ctx* c, int slot, const void* host, size_t bytes, size_t cap) {
-  CHK(c->st[slot].grow(cap));
-  if (bytes) HIPCHK(hipMemcpyAsync(c->st[slot].p, host, bytes, hipMemcpyHostToDevice, c->s));
-  return ADMX_OK;
-}
 
 template <typename T>
 int loop_ll(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1, int64_t j0, int64_t j1, double* out) {
@@ -1181,96 +921,6 @@ int host_lbqn(admx_ctx* c, int64_t iter, const void* x, const void* xn, const vo
   HIPCHK(hipStreamSynchronize(c->s));
   return ADMX_OK;
 }
-
-#define BY_DTYPE(c, fn, ...) ((c)->dtype == ADMX_F64 ? fn<double>(__VA_ARGS__) : fn<float>(__VA_ARGS__))
-
-struct Guard {
-  std::unique_lock<std::mutex> lk;
-  explicit Guard(admx_ctx* c) : lk(c->mu) {}
-};
-
-// Both contexts' locks, taken in address order (or one lock when a == b).
-struct Guard2 {
-  std::unique_lock<std::mutex> l1, l2;
-  Guard2(admx_ctx* a, admx_ctx* b) {
-    admx_ctx* lo = a < b ? a : b;
-    admx_ctx* hi = a < b ? b : a;
-    l1 = std::unique_lock<std::mutex>(lo->mu);
-    if (hi != lo) l2 = std::unique_lock<std::mutex>(hi->mu);
-  }
-};
-
-// ---- warm-start network helpers
-struct MlpActs {
-  float *Xb, *Tb, *H1, *H2, *Z, *Y, *dZ, *dH2, *dH1, *rowloss, *loss;
-};
-
-int mlp_acts(admx_mlp* m, int64_t B, MlpActs* a) {
-  const int64_t per = m->in + 4 * (int64_t)m->out + 4 * (int64_t)m->hid + 1;
-  if (B > m->act_rows) {
-    CHK(m->act.grow(sizeof(float) * (size_t)(per * B + 64)));
-    m->act_rows = B;
-  }
-  float* p = static_cast<float*>(m->act.p);
-  const int64_t R = m->act_rows;
-  a->Xb = p;
-  a->Tb = a->Xb + R * m->in;
-  a->H1 = a->Tb + R * m->out;
-  a->H2 = a->H1 + R * m->hid;
-  a->Z = a->H2 + R * m->hid;
-  a->Y = a->Z + R * m->out;
-  a->dZ = a->Y + R * m->out;
-  a->dH2 = a->dZ + R * m->out;
-  a->dH1 = a->dH2 + R * m->hid;
-  a->rowloss = a->dH1 + R * m->hid;
-  a->loss = a->rowloss + R;
-  return ADMX_OK;
-}
-
-// Forward over B rows of X into a->H1, H2, Z and Y (and dZ, rowloss when T).
-void mlp_forward(admx_mlp* m, const float* X, const float* T, int B, const MlpActs& a, float* Y, hipStream_t s) {
-  const float* w = static_cast<const float*>(m->w.p);
-  launch_sgemm(false, true, B, m->hid, m->in, X, m->in, w + m->oW1(), m->in, a.H1, m->hid, w + m->ob1(), true,
-               nullptr, false, s);
-  launch_sgemm(false, true, B, m->hid, m->hid, a.H1, m->hid, w + m->oW2(), m->hid, a.H2, m->hid, w + m->ob2(), true,
-               nullptr, false, s);
-  launch_sgemm(false, true, B, m->out, m->hid, a.H2, m->hid, w + m->oW3(), m->hid, a.Z, m->out, w + m->ob3(), false,
-               nullptr, false, s);
-  launch_softmax_ce(a.Z, B, m->out, T, Y, T ? a.dZ : nullptr, T ? a.rowloss : nullptr, s);
-}
-
-// Gradients of the mean soft-target cross entropy into m->g; the loss sum of
-// the batch into *loss_dev.
-void mlp_backward(admx_mlp* m, const float* X, int B, const MlpActs& a, float* loss_dev, hipStream_t s) {
-  const float* w = static_cast<const float*>(m->w.p);
-  float* g = static_cast<float*>(m->g.p);
-  launch_sgemm(true, false, m->out, m->hid, B, a.dZ, m->out, a.H2, m->hid, g + m->oW3(), m->hid, nullptr, false,
-               nullptr, false, s);
-  launch_colsum(a.dZ, B, m->out, g + m->ob3(), s);
-  launch_sgemm(false, false, B, m->hid, m->out, a.dZ, m->out, w + m->oW3(), m->hid, a.dH2, m->hid, nullptr, false,
-               a.H2, false, s);
-  launch_sgemm(true, false, m->hid, m->hid, B, a.dH2, m->hid, a.H1, m->hid, g + m->oW2(), m->hid, nullptr, false,
-               nullptr, false, s);
-  launch_colsum(a.dH2, B, m->hid, g + m->ob2(), s);
-  launch_sgemm(false, false, B, m->hid, m->hid, a.dH2, m->hid, w + m->oW2(), m->hid, a.dH1, m->hid, nullptr, false,
-               a.H1, false, s);
-  launch_sgemm(true, false, m->hid, m->in, B, a.dH1, m->hid, X, m->in, g + m->oW1(), m->in, nullptr, false, nullptr,
-               false, s);
-  launch_colsum(a.dH1, B, m->hid, g + m->ob1(), s);
-  launch_colsum(a.rowloss, B, 1, loss_dev, s);
-}
-
-uint64_t host_smix(uint64_t z) {
-  z += 0x9E3779B97F4A7C15ULL;
-  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ULL;
-  z = (z ^ (z >> 27)) * 0x94D049BB133111EBULL;
-  return z ^ (z >> 31);
-}
-
-struct MGuard {
-  std::unique_lock<std::mutex> lk;
-  explicit MGuard(admx_mlp* m) : lk(m->mu) {}
-};
 
 }  // namespace
 
@@ -1620,265 +1270,6 @@ int admx_heldout_deviance(admx_ctx* fit, admx_ctx* src, int32_t fold, int32_t nf
   });
 }
 
-}  // extern "C"
-
-namespace {
-
-// ---- sparse k-means feature ranking (SKFR)
-struct SkfrView {
-  int32_t* lab;
-  uint32_t* cm;
-  unsigned* nk;
-  unsigned long long *changes, *count;
-  int32_t* center;
-  unsigned long long *nm, *alt;
-  uint32_t* part;
-  double *mu, *var, *ukey, *E, *key, *keys;
-  int32_t *idx, *sidx;
-  uint8_t *flag, *sel;
-  double *T, *P, *x2, *dmin, *D2, *out, *Dc;
-  void* tmp;
-  size_t tmp_bytes;
-  int64_t ntiles, nwords, tper, nslab;
-};
-
-// Sizes every SKFR buffer for k clusters, tables of up to ncols_max selected
-// SNPs and p_elems table-sum partials, and computes the SNP counts and
-// (mu_j, v_j) of the context's genotypes.
-int skfr_prepare(admx_ctx* c, int k, int64_t ncols_max, int64_t p_elems, bool need_dc, SkfrView* v) {
-  const int64_t I = c->I, J = c->J;
-  const int64_t nb = (I + 3) / 4;
-  v->ntiles = (nb + 63) / 64;  // 64-byte tiles; the row pitch is a multiple of 64
-  v->nwords = 16 * v->ntiles;
-  v->nslab = skfr_slabs(J, v->ntiles, &v->tper);
-  CHK(c->sk_lab.grow(sizeof(int32_t) * I));
-  CHK(c->sk_cm.grow(sizeof(uint32_t) * v->nwords * std::max(k - 1, 1)));
-  CHK(c->sk_small.grow(256));
-  CHK(c->sk_cnt.grow(sizeof(unsigned long long) * 2 * J));
-  CHK(c->sk_part.grow(sizeof(uint32_t) * v->nslab * 2 * std::max(k - 1, 1) * J));
-  CHK(c->sk_stat.grow(sizeof(double) * 3 * J));
-  CHK(c->sk_E.grow(sizeof(double) * k * J));
-  CHK(c->sk_key.grow(sizeof(double) * k * J));
-  CHK(c->sk_keys.grow(sizeof(double) * k * J));
-  CHK(c->sk_idx.grow(sizeof(int32_t) * J));
-  CHK(c->sk_sidx.grow(sizeof(int32_t) * k * J));
-  CHK(c->sk_flag.grow((size_t)k * J));
-  CHK(c->sk_sel.grow((size_t)J));
-  const int64_t trows = std::max<int64_t>(std::max<int64_t>(4 * J, (int64_t)k * J), 4 * k * ncols_max);
-  CHK(c->sk_T.grow(sizeof(double) * trows));
-  CHK(c->sk_P.grow(sizeof(double) * (size_t)p_elems));
-  CHK(c->sk_vec.grow(sizeof(double) * (3 * I + kMaxK + 1)));
-  if (need_dc) CHK(c->sk_Dc.grow(sizeof(double) * k * I));
-  const size_t tb = skfr_sort_bytes(J);
-  CHK(c->sk_tmp.grow(tb));
-  char* sm = static_cast<char*>(c->sk_small.p);
-  v->lab = static_cast<int32_t*>(c->sk_lab.p);
-  v->cm = static_cast<uint32_t*>(c->sk_cm.p);
-  v->nk = reinterpret_cast<unsigned*>(sm);
-  v->changes = reinterpret_cast<unsigned long long*>(sm + 64);
-  v->count = reinterpret_cast<unsigned long long*>(sm + 72);
-  v->center = reinterpret_cast<int32_t*>(sm + 80);
-  v->nm = static_cast<unsigned long long*>(c->sk_cnt.p);
-  v->alt = v->nm + J;
-  v->part = static_cast<uint32_t*>(c->sk_part.p);
-  v->mu = static_cast<double*>(c->sk_stat.p);
-  v->var = v->mu + J;
-  v->ukey = v->var + J;
-  v->E = static_cast<double*>(c->sk_E.p);
-  v->key = static_cast<double*>(c->sk_key.p);
-  v->keys = static_cast<double*>(c->sk_keys.p);
-  v->idx = static_cast<int32_t*>(c->sk_idx.p);
-  v->sidx = static_cast<int32_t*>(c->sk_sidx.p);
-  v->flag = static_cast<uint8_t*>(c->sk_flag.p);
-  v->sel = static_cast<uint8_t*>(c->sk_sel.p);
-  v->T = static_cast<double*>(c->sk_T.p);
-  v->P = static_cast<double*>(c->sk_P.p);
-  v->x2 = static_cast<double*>(c->sk_vec.p);
-  v->dmin = v->x2 + I;
-  v->D2 = v->dmin + I;
-  v->out = v->D2 + I;
-  v->Dc = static_cast<double*>(c->sk_Dc.p);
-  v->tmp = c->sk_tmp.p;
-  v->tmp_bytes = c->sk_tmp.n;
-  HIPCHK(hipSetDevice(c->device));
-  {
-    ProfScope ps(c, "skfr_stats");
-    launch_counts_snp(c->bed, c->pitch, I, J, v->nm, v->alt, c->s);
-    launch_skfr_stats(v->nm, v->alt, J, v->mu, v->var, c->s);
-  }
-  return check_launch();
-}
-
-int skfr_split(admx_ctx* c, int nsplit, int64_t ncols, int R) {
-  if (nsplit <= 0) return skfr_nsplit(c->I, ncols, R);
-  return (int)std::max<int64_t>(1, std::min<int64_t>(nsplit, ncols));
-}
-
-// k-means++ draw u of center cc in [0, 1)
-double skfr_u(uint64_t seed, int cc) {
-  return (double)(host_smix(seed ^ (0xD1B54A32D192ED03ULL * (uint64_t)(cc + 1))) >> 11) * 0x1.0p-53;
-}
-
-}  // namespace
-
-extern "C" {
-
-int admx_skfr_init(admx_ctx* c, int32_t k, uint64_t seed, int32_t nsplit, int32_t* clusters) {
-  return guarded([&]() -> int {
-    CHK(check_ctx(c));
-    Guard g(c);
-    CHK(need_bed(c));
-    CHK(need_resident(c, "SKFR"));
-    if (!clusters) return bad("null output");
-    if (k < 1 || k > kMaxK) return bad("need 1 <= k <= 16");
-    if (k > c->I) return bad("more clusters than individuals");
-    if (nsplit < 0 || nsplit > 64) return bad("need 0 <= nsplit <= 64");
-    if (c->I >= (1LL << 31) || c->J >= (1LL << 31)) return bad("SKFR needs I, J < 2^31");
-    SkfrView v;
-    const int ns = skfr_split(c, nsplit, c->J, 1);
-    CHK(skfr_prepare(c, k, 0, (int64_t)ns * c->I, true, &v));
-    {
-      ProfScope ps(c, "skfr_kmeanspp");
-      for (int cc = 0; cc < k; ++cc) {
-        launch_skfr_pick(v.D2, c->I, skfr_u(seed, cc), cc == 0, v.center, c->s);
-        launch_skfr_pp_table(c->bed, c->pitch, v.center, v.mu, v.var, c->J, v.T, c->s);
-        launch_skfr_tabsum(1, c->bed, c->pitch, c->I, nullptr, c->J, ns, v.T, v.P, c->s);
-        launch_skfr_combine(v.P, ns, c->I, v.Dc + (int64_t)cc * c->I, v.D2, cc == 0, c->s);
-      }
-      launch_skfr_argmin(v.Dc, k, c->I, v.lab, c->s);
-    }
-    CHK(check_launch());
-    HIPCHK(hipMemcpyAsync(clusters, v.lab, sizeof(int32_t) * c->I, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    return ADMX_OK;
-  });
-}
-
-int admx_skfr_run(admx_ctx* c, int32_t k, int32_t mode, int64_t sparsity, int32_t max_iter, int32_t nsplit,
-                  int32_t* clusters, int64_t* aims, int64_t* n_aims, double* centers, double* wss, double* obj,
-                  int32_t* iters) {
-  return guarded([&]() -> int {
-    CHK(check_ctx(c));
-    Guard g(c);
-    CHK(need_bed(c));
-    CHK(need_resident(c, "SKFR"));
-    if (!clusters || !aims || !n_aims) return bad("null clusters or aims");
-    if (k < 1 || k > kMaxK) return bad("need 1 <= k <= 16");
-    if (mode != ADMX_SKFR_GLOBAL && mode != ADMX_SKFR_LOCAL) return bad("mode must be ADMX_SKFR_GLOBAL or _LOCAL");
-    if (sparsity < 1 || sparsity > c->J) return bad("need 1 <= sparsity <= J");
-    if (max_iter < 1) return bad("need max_iter >= 1");
-    if (nsplit < 0 || nsplit > 64) return bad("need 0 <= nsplit <= 64");
-    if (c->I >= (1LL << 31) || c->J >= (1LL << 31)) return bad("SKFR needs I, J < 2^31");
-    for (int64_t i = 0; i < c->I; ++i)
-      if (clusters[i] < 0 || clusters[i] >= k) return bad("cluster labels must lie in [0, k)");
-    const bool local = mode == ADMX_SKFR_LOCAL;
-    const int64_t I = c->I, J = c->J;
-    const int64_t ncols_max = local ? std::min<int64_t>(J, (int64_t)k * sparsity) : sparsity;
-    // slices are nondecreasing in the column count
-    const int64_t p_elems = std::max<int64_t>((int64_t)skfr_split(c, nsplit, J, 1) * I,
-                                              (int64_t)skfr_split(c, nsplit, ncols_max, k) * k * I);
-    SkfrView v;
-    CHK(skfr_prepare(c, k, ncols_max, p_elems, false, &v));
-    HIPCHK(hipMemcpyAsync(v.lab, clusters, sizeof(int32_t) * I, hipMemcpyHostToDevice, c->s));
-    {
-      // sum_j x_ij^2 of every individual (the WSS term common to all clusters)
-      ProfScope ps(c, "skfr_x2");
-      const int ns = skfr_split(c, nsplit, J, 1);
-      launch_skfr_pp_table(c->bed, c->pitch, nullptr, v.mu, v.var, J, v.T, c->s);
-      launch_skfr_tabsum(1, c->bed, c->pitch, I, nullptr, J, ns, v.T, v.P, c->s);
-      launch_skfr_combine(v.P, ns, I, v.x2, nullptr, 0, c->s);
-    }
-    CHK(check_launch());
-    int it = 0;
-    int64_t ncols = 0;
-    for (it = 1; it <= max_iter; ++it) {
-      HIPCHK(hipMemsetAsync(v.nk, 0, sizeof(unsigned) * kMaxK, c->s));
-      {
-        ProfScope ps(c, "skfr_masks");
-        launch_skfr_masks(v.lab, I, k, v.nwords, v.cm, v.nk, c->s);
-      }
-      {
-        ProfScope ps(c, "skfr_centers");
-        launch_skfr_centers(k, c->bed, c->pitch, J, v.ntiles, v.tper, v.nslab, v.cm, v.part, c->s);
-      }
-      {
-        ProfScope ps(c, "skfr_score");
-        launch_skfr_score(k, local, v.part, v.nslab, v.nk, v.nm, v.alt, I, v.mu, v.var, J, v.E, v.key, v.idx, c->s);
-      }
-      {
-        ProfScope ps(c, "skfr_select");
-        if (!local) {
-          launch_skfr_sort(v.tmp, v.tmp_bytes, v.key, v.keys, v.idx, v.sidx, J, c->s);
-          ncols = sparsity;
-        } else {
-          for (int cc = 0; cc < k; ++cc)
-            launch_skfr_sort(v.tmp, v.tmp_bytes, v.key + (int64_t)cc * J, v.keys + (int64_t)cc * J, v.idx,
-                             v.sidx + (int64_t)cc * J, J, c->s);
-          HIPCHK(hipMemsetAsync(v.flag, 0, (size_t)k * J, c->s));
-          HIPCHK(hipMemsetAsync(v.count, 0, sizeof(unsigned long long), c->s));
-          launch_skfr_flag(v.sidx, J, sparsity, k, v.flag, c->s);
-          launch_skfr_union(v.flag, v.key, J, k, v.ukey, v.idx, v.count, c->s);
-          launch_skfr_sort(v.tmp, v.tmp_bytes, v.ukey, v.keys, v.idx, v.sidx, J, c->s);
-        }
-      }
-      CHK(check_launch());
-      if (local) {
-        unsigned long long h = 0;
-        HIPCHK(hipMemcpyAsync(&h, v.count, sizeof(h), hipMemcpyDeviceToHost, c->s));
-        HIPCHK(hipStreamSynchronize(c->s));
-        ncols = (int64_t)h;
-        if (ncols < 1 || ncols > ncols_max) return fail(ADMX_EHIP, "SKFR: inconsistent union of local selections");
-      }
-      const int ns = skfr_split(c, nsplit, ncols, k);
-      HIPCHK(hipMemsetAsync(v.changes, 0, sizeof(unsigned long long), c->s));
-      {
-        ProfScope ps(c, "skfr_tables");
-        launch_skfr_tables(v.sidx, ncols, k, local ? v.flag : nullptr, v.nk, v.mu, v.var, v.E, J, v.T, c->s);
-      }
-      {
-        ProfScope ps(c, "skfr_tabsum");
-        launch_skfr_tabsum(k, c->bed, c->pitch, I, v.sidx, ncols, ns, v.T, v.P, c->s);
-      }
-      {
-        ProfScope ps(c, "skfr_assign");
-        launch_skfr_assign(v.P, ns, k, I, v.nk, v.lab, v.dmin, v.changes, c->s);
-      }
-      CHK(check_launch());
-      unsigned long long h = 0;
-      HIPCHK(hipMemcpyAsync(&h, v.changes, sizeof(h), hipMemcpyDeviceToHost, c->s));
-      HIPCHK(hipStreamSynchronize(c->s));
-      if (h == 0) break;
-    }
-    if (it > max_iter) it = max_iter;
-    {
-      ProfScope ps(c, "skfr_wss");
-      launch_skfr_wss(v.lab, v.x2, v.dmin, I, k, v.out, c->s);
-      if (centers) {
-        if (!local) {
-          HIPCHK(hipMemsetAsync(v.sel, 0, (size_t)J, c->s));
-          launch_skfr_mark(v.sidx, ncols, v.sel, c->s);
-        }
-        // the distance tables are no longer needed: reuse them for the centers
-        launch_skfr_center_out(v.sel, local ? v.flag : nullptr, v.nk, v.var, v.E, J, k, v.T, c->s);
-      }
-    }
-    CHK(check_launch());
-    std::vector<int32_t> h_aims((size_t)ncols);
-    std::vector<double> h_out((size_t)k + 1);
-    HIPCHK(hipMemcpyAsync(clusters, v.lab, sizeof(int32_t) * I, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(h_aims.data(), v.sidx, sizeof(int32_t) * ncols, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(h_out.data(), v.out, sizeof(double) * (k + 1), hipMemcpyDeviceToHost, c->s));
-    if (centers) HIPCHK(hipMemcpyAsync(centers, v.T, sizeof(double) * k * J, hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    for (int64_t t = 0; t < ncols; ++t) aims[t] = h_aims[(size_t)t];
-    *n_aims = ncols;
-    if (wss)
-      for (int cc = 0; cc < k; ++cc) wss[cc] = h_out[(size_t)cc];
-    if (obj) *obj = h_out[(size_t)k];
-    if (iters) *iters = it;
-    return ADMX_OK;
-  });
-}
 
 int admx_decode(admx_ctx* c, int8_t* gout) {
   return guarded([&]() -> int {
@@ -2303,266 +1694,6 @@ int admx_set_tuning(admx_ctx* c, int key, int64_t value) {
     } else {
       return bad("unknown tuning key");
     }
-    return ADMX_OK;
-  });
-}
-
-// ---- learned warm start
-int admx_mlp_create(admx_mlp** out, int32_t in_dim, int32_t hidden, int32_t out_dim, uint64_t seed) {
-  return guarded([&]() -> int {
-    if (!out) return bad("null output handle");
-    *out = nullptr;
-    if (in_dim < 1 || hidden < 1 || out_dim < 2 || out_dim > 16) return bad("need in >= 1, hidden >= 1, 2 <= out <= 16");
-    admx_mlp* m = new admx_mlp();
-    m->in = in_dim;
-    m->hid = hidden;
-    m->out = out_dim;
-    m->P = m->ob3() + out_dim;
-    auto cleanup = [&](int rc) {
-      admx_mlp_destroy(m);
-      return rc;
-    };
-    hipError_t e = hipGetDevice(&m->device);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&m->s, hipStreamNonBlocking);
-    if (e != hipSuccess) return cleanup(fail(ADMX_EHIP, std::string("mlp stream: ") + hipGetErrorString(e)));
-    const size_t pb = sizeof(float) * (size_t)m->P;
-    int rc = m->w.grow(pb);
-    if (rc == ADMX_OK) rc = m->g.grow(pb);
-    if (rc == ADMX_OK) rc = m->m.grow(pb);
-    if (rc == ADMX_OK) rc = m->v.grow(pb);
-    if (rc != ADMX_OK) return cleanup(rc);
-    // He-uniform weights U(-sqrt(6/fan_in), sqrt(6/fan_in)), zero biases
-    std::vector<float> w((size_t)m->P, 0.f);
-    uint64_t ctr = seed * 0x9E3779B97F4A7C15ULL;
-    auto fill = [&](int64_t off, int64_t n, int fan_in) {
-      const double a = std::sqrt(6.0 / fan_in);
-      for (int64_t e2 = 0; e2 < n; ++e2) {
-        const double u = (double)(host_smix(ctr++) >> 11) * 0x1.0p-53;
-        w[(size_t)(off + e2)] = (float)((2.0 * u - 1.0) * a);
-      }
-    };
-    fill(m->oW1(), (int64_t)hidden * in_dim, in_dim);
-    fill(m->oW2(), (int64_t)hidden * hidden, hidden);
-    fill(m->oW3(), (int64_t)out_dim * hidden, hidden);
-    e = hipMemcpy(m->w.p, w.data(), pb, hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemset(m->m.p, 0, pb);
-    if (e == hipSuccess) e = hipMemset(m->v.p, 0, pb);
-    if (e != hipSuccess) return cleanup(fail(ADMX_EHIP, std::string("mlp init: ") + hipGetErrorString(e)));
-    *out = m;
-    return ADMX_OK;
-  });
-}
-
-int admx_mlp_destroy(admx_mlp* m) {
-  return guarded([&]() -> int {
-    if (!m) return ADMX_OK;
-    (void)hipSetDevice(m->device);
-    if (m->s) (void)hipStreamSynchronize(m->s);
-    DevBuf* bufs[] = {&m->w, &m->g, &m->m, &m->v, &m->X, &m->T, &m->idx, &m->act};
-    for (DevBuf* b : bufs) b->release();
-    if (m->s) (void)hipStreamDestroy(m->s);
-    delete m;
-    return ADMX_OK;
-  });
-}
-
-int admx_mlp_num_params(admx_mlp* m, int64_t* n) {
-  return guarded([&]() -> int {
-    if (!m || !n) return bad("null argument");
-    *n = m->P;
-    return ADMX_OK;
-  });
-}
-
-int admx_mlp_get_params(admx_mlp* m, float* w) {
-  return guarded([&]() -> int {
-    if (!m || !w) return bad("null argument");
-    MGuard g(m);
-    HIPCHK(hipSetDevice(m->device));
-    HIPCHK(hipMemcpyAsync(w, m->w.p, sizeof(float) * (size_t)m->P, hipMemcpyDeviceToHost, m->s));
-    HIPCHK(hipStreamSynchronize(m->s));
-    return ADMX_OK;
-  });
-}
-
-int admx_mlp_set_params(admx_mlp* m, const float* w) {
-  return guarded([&]() -> int {
-    if (!m || !w) return bad("null argument");
-    MGuard g(m);
-    HIPCHK(hipSetDevice(m->device));
-    HIPCHK(hipMemcpyAsync(m->w.p, w, sizeof(float) * (size_t)m->P, hipMemcpyHostToDevice, m->s));
-    HIPCHK(hipMemsetAsync(m->m.p, 0, sizeof(float) * (size_t)m->P, m->s));
-    HIPCHK(hipMemsetAsync(m->v.p, 0, sizeof(float) * (size_t)m->P, m->s));
-    HIPCHK(hipStreamSynchronize(m->s));
-    m->step = 0;
-    return ADMX_OK;
-  });
-}
-
-int admx_mlp_loss_grad(admx_mlp* m, const float* X, const float* T, int64_t B, float* loss, float* grad) {
-  return guarded([&]() -> int {
-    if (!m || !X || !T || !loss || !grad) return bad("null argument");
-    if (B < 1 || B > (1 << 30)) return bad("batch rows out of range");
-    MGuard g(m);
-    HIPCHK(hipSetDevice(m->device));
-    MlpActs a;
-    CHK(mlp_acts(m, B, &a));
-    HIPCHK(hipMemcpyAsync(a.Xb, X, sizeof(float) * (size_t)(B * m->in), hipMemcpyHostToDevice, m->s));
-    HIPCHK(hipMemcpyAsync(a.Tb, T, sizeof(float) * (size_t)(B * m->out), hipMemcpyHostToDevice, m->s));
-    mlp_forward(m, a.Xb, a.Tb, (int)B, a, a.Y, m->s);
-    mlp_backward(m, a.Xb, (int)B, a, a.loss, m->s);
-    CHK(check_launch());
-    float ls = 0.f;
-    HIPCHK(hipMemcpyAsync(&ls, a.loss, sizeof(float), hipMemcpyDeviceToHost, m->s));
-    HIPCHK(hipMemcpyAsync(grad, m->g.p, sizeof(float) * (size_t)m->P, hipMemcpyDeviceToHost, m->s));
-    HIPCHK(hipStreamSynchronize(m->s));
-    *loss = ls / (float)B;
-    return ADMX_OK;
-  });
-}
-
-int admx_mlp_predict(admx_mlp* m, const float* X, int64_t n, float* Y) {
-  return guarded([&]() -> int {
-    if (!m || !X || !Y) return bad("null argument");
-    if (n < 0) return bad("negative row count");
-    MGuard g(m);
-    HIPCHK(hipSetDevice(m->device));
-    const int64_t Bmax = 65536;
-    MlpActs a;
-    CHK(mlp_acts(m, std::min<int64_t>(std::max<int64_t>(n, 1), Bmax), &a));
-    for (int64_t r0 = 0; r0 < n; r0 += Bmax) {
-      const int64_t B = std::min(Bmax, n - r0);
-      HIPCHK(hipMemcpyAsync(a.Xb, X + r0 * m->in, sizeof(float) * (size_t)(B * m->in), hipMemcpyHostToDevice, m->s));
-      mlp_forward(m, a.Xb, nullptr, (int)B, a, a.Y, m->s);
-      CHK(check_launch());
-      HIPCHK(hipMemcpyAsync(Y + r0 * m->out, a.Y, sizeof(float) * (size_t)(B * m->out), hipMemcpyDeviceToHost, m->s));
-    }
-    HIPCHK(hipStreamSynchronize(m->s));
-    return ADMX_OK;
-  });
-}
-
-int admx_mlp_train(admx_mlp* m, const float* X, const float* T, int64_t n, int32_t epochs, int32_t batch, float lr,
-                   uint64_t seed, float* epoch_loss) {
-  return guarded([&]() -> int {
-    if (!m || !X || !T) return bad("null argument");
-    if (n < 1 || n > (1 << 30) || epochs < 0 || batch < 1) return bad("need n >= 1, epochs >= 0, batch >= 1");
-    MGuard g(m);
-    HIPCHK(hipSetDevice(m->device));
-    CHK(m->X.grow(sizeof(float) * (size_t)(n * m->in)));
-    CHK(m->T.grow(sizeof(float) * (size_t)(n * m->out)));
-    CHK(m->idx.grow(sizeof(int32_t) * (size_t)n));
-    HIPCHK(hipMemcpyAsync(m->X.p, X, sizeof(float) * (size_t)(n * m->in), hipMemcpyHostToDevice, m->s));
-    HIPCHK(hipMemcpyAsync(m->T.p, T, sizeof(float) * (size_t)(n * m->out), hipMemcpyHostToDevice, m->s));
-    m->n = n;
-    const int64_t B = std::min<int64_t>(batch, n);
-    const int64_t nb = (n + B - 1) / B;
-    MlpActs a;
-    CHK(mlp_acts(m, B, &a));
-    DevBuf lossbuf;
-    CHK(lossbuf.grow(sizeof(float) * (size_t)nb));
-    std::vector<int32_t> perm((size_t)n);
-    std::vector<float> lh((size_t)nb);
-    const float b1 = 0.9f, b2 = 0.999f, eps = 1e-8f;
-    int rc = ADMX_OK;
-    for (int ep = 0; ep < epochs && rc == ADMX_OK; ++ep) {
-      // Fisher-Yates shuffle of the rows from the seeded counter hash
-      for (int64_t r = 0; r < n; ++r) perm[(size_t)r] = (int32_t)r;
-      for (int64_t r = n - 1; r > 0; --r) {
-        const uint64_t h = host_smix(seed ^ host_smix((uint64_t)ep * 0x100000001B3ULL + (uint64_t)r));
-        const int64_t k = (int64_t)(h % (uint64_t)(r + 1));
-        std::swap(perm[(size_t)r], perm[(size_t)k]);
-      }
-      HIPCHK(hipMemcpyAsync(m->idx.p, perm.data(), sizeof(int32_t) * (size_t)n, hipMemcpyHostToDevice, m->s));
-      for (int64_t b = 0; b < nb; ++b) {
-        const int64_t r0 = b * B;
-        const int Bb = (int)std::min(B, n - r0);
-        const int32_t* ix = static_cast<const int32_t*>(m->idx.p) + r0;
-        launch_gather_rows(static_cast<const float*>(m->X.p), m->in, ix, Bb, a.Xb, m->s);
-        launch_gather_rows(static_cast<const float*>(m->T.p), m->out, ix, Bb, a.Tb, m->s);
-        mlp_forward(m, a.Xb, a.Tb, Bb, a, a.Y, m->s);
-        mlp_backward(m, a.Xb, Bb, a, static_cast<float*>(lossbuf.p) + b, m->s);
-        ++m->step;
-        const float c1 = 1.f - std::pow(b1, (float)m->step), c2 = 1.f - std::pow(b2, (float)m->step);
-        launch_adam(m->P, static_cast<float*>(m->w.p), static_cast<const float*>(m->g.p), static_cast<float*>(m->m.p),
-                    static_cast<float*>(m->v.p), lr, b1, b2, c1, c2, eps, m->s);
-      }
-      rc = check_launch();
-      if (rc != ADMX_OK) break;
-      HIPCHK(hipMemcpyAsync(lh.data(), lossbuf.p, sizeof(float) * (size_t)nb, hipMemcpyDeviceToHost, m->s));
-      HIPCHK(hipStreamSynchronize(m->s));
-      double tot = 0.0;
-      for (float v : lh) tot += v;
-      if (epoch_loss) epoch_loss[ep] = (float)(tot / (double)n);
-    }
-    HIPCHK(hipStreamSynchronize(m->s));
-    lossbuf.release();
-    return rc;
-  });
-}
-
-int admx_features(admx_ctx* c, const float* R, const float* mu, const float* isd, int64_t Jp, float* F) {
-  return guarded([&]() -> int {
-    CHK(check_ctx(c));
-    Guard g(c);
-    CHK(need_bed(c));
-    CHK(need_resident(c, "the warm-start features"));
-    if (!R || !mu || !isd) return bad("null panel array");
-    if (Jp < 1 || Jp > c->J) return bad("panel SNP count must be in [1, J]");
-    const size_t rb = sizeof(float) * (size_t)(Jp * kFeat), vb = sizeof(float) * (size_t)Jp;
-    CHK(stage_in(c, 0, R, rb, rb));
-    CHK(stage_in(c, 1, mu, vb, vb));
-    CHK(stage_in(c, 2, isd, vb, vb));
-    const int64_t waves = (c->I + 63) / 64;
-    int64_t nchunk = std::max<int64_t>(1, std::min<int64_t>(64, c->target_waves / std::max<int64_t>(waves, 1) / 8));
-    nchunk = std::min(nchunk, Jp);
-    const int64_t jchunk = (Jp + nchunk - 1) / nchunk;
-    nchunk = (Jp + jchunk - 1) / jchunk;
-    CHK(c->feat.grow(sizeof(float) * (size_t)(c->I * kFeat)));
-    CHK(c->featpart.grow(sizeof(float) * (size_t)(nchunk * c->I * kFeat)));
-    {
-      ProfScope ps(c, "features");
-      launch_features(c->bed, c->pitch, c->I, Jp, static_cast<const float*>(c->st[0].p),
-                      static_cast<const float*>(c->st[1].p), static_cast<const float*>(c->st[2].p), (int)nchunk, jchunk,
-                      static_cast<float*>(c->featpart.p), static_cast<float*>(c->feat.p), c->s);
-    }
-    CHK(check_launch());
-    c->feat_rows = c->I;
-    if (F) HIPCHK(hipMemcpyAsync(F, c->feat.p, sizeof(float) * (size_t)(c->I * kFeat), hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipStreamSynchronize(c->s));
-    return ADMX_OK;
-  });
-}
-
-int admx_warm_q(admx_ctx* c, admx_mlp* m) {
-  return guarded([&]() -> int {
-    CHK(check_ctx(c));
-    if (!m) return bad("null network");
-    Guard g(c);
-    MGuard gm(m);
-    if (c->dtype != ADMX_F64) return bad("the warm start writes a FP64 context");
-    if (m->device != c->device) return bad("network and context must live on the same device");
-    if (m->in != kFeat || m->out != c->K) return bad("network input must be 64 features and its output K");
-    if (c->feat_rows != c->I) return fail(ADMX_ESTATE, "no features: call admx_features first");
-    HIPCHK(hipStreamSynchronize(m->s));
-    const int64_t Bmax = 65536;
-    MlpActs a;
-    CHK(mlp_acts(m, std::min<int64_t>(c->I, Bmax), &a));
-    CHK(c->st[3].grow(sizeof(float) * (size_t)(c->I * c->K)));
-    float* Y = static_cast<float*>(c->st[3].p);
-    const float* Fd = static_cast<const float*>(c->feat.p);
-    {
-      ProfScope ps(c, "warm_q");
-      for (int64_t r0 = 0; r0 < c->I; r0 += Bmax) {
-        const int64_t B = std::min(Bmax, c->I - r0);
-        mlp_forward(m, Fd + r0 * kFeat, nullptr, (int)B, a, Y + r0 * c->K, c->s);
-      }
-      double* q = static_cast<double*>(c->x[ADMX_X]);
-      launch_probs_to_q(Y, c->K * c->I, q, c->s);
-      launch_project_q<double>((int)c->K, c->I, q, c->s);
-    }
-    CHK(check_launch());
-    HIPCHK(hipStreamSynchronize(c->s));
     return ADMX_OK;
   });
 }
