@@ -173,6 +173,10 @@ def converge_run(oa, ctx, wl, T, seed, form, max_iter, rtol=1e-7, stop_at=None):
     return hist, its
 
 
+def _warm_kw(args):
+    return {} if args.warm_p_steps is None else {"p_steps": args.warm_p_steps}
+
+
 def first_within(hist, ll_star, tol):
     """(index, seconds, LL) of the first entry of hist within tol of ll_star."""
     for n, (t, ll) in enumerate(hist):
@@ -265,7 +269,7 @@ def run_warmstart(args):
 
     def warm(form, stop_at):
         ctx.set_precision(form)
-        r = oa.warm_started_fit(ctx, net, panel, seed=args.seed + 1, max_iter=args.ttc_max_iter,
+        r = oa.warm_started_fit(ctx, net, panel, seed=args.seed + 1, max_iter=args.ttc_max_iter, **_warm_kw(args),
                                 stop_at=(stop_at, tol) if stop_at else None)
         return r["history"], r["chosen"]
 
@@ -415,6 +419,8 @@ def main():
     ap.add_argument("--fold-start", default="full", choices=["full", "random"],
                     help="CV fold fits start from the full fit's estimates or from their own random start")
     ap.add_argument("--train-epochs", type=int, default=20, help="network training epochs (workload c5)")
+    ap.add_argument("--warm-p-steps", type=int, default=None,
+                    help="fixed-Q P refinement iterations of the warm start (default: the library's)")
     ap.add_argument("--K", type=int, default=0, help="override the workload's K (throughput runs)")
     ap.add_argument("--skfr-iters", type=int, default=10, help="SKFR iterations of the AIM pre-selection block")
     ap.add_argument("--no-skfr", action="store_true", help="skip the SKFR AIM pre-selection block")
@@ -573,7 +579,7 @@ def main():
             if form.startswith("warm"):
                 prec = form.split("_", 1)[1] if "_" in form else "f64"
                 ctx.set_precision(prec)
-                r = oa.warm_started_fit(ctx, net, panel, seed=args.seed + 1, max_iter=args.ttc_max_iter,
+                r = oa.warm_started_fit(ctx, net, panel, seed=args.seed + 1, max_iter=args.ttc_max_iter, **_warm_kw(args),
                                         stop_at=(ll_star, tol))
                 h2, its2 = r["history"], r["iterations"]
             else:

@@ -183,9 +183,14 @@ def training_set(P_world, panel: Panel, n_cohorts: int, I: int = 5000, J: int = 
 
 
 def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_iter: int = 1000,
-                     rtol: float = 1e-7, p_steps: int = 3, judge: bool = True, em_iters: int = 5, Q: int = 3,
+                     rtol: float = 1e-7, p_steps: int = 1, judge: bool = True, em_iters: int = 5, Q: int = 3,
                      stop_at=None):
     """admixture_qn! from the network's proposal (with the EM judge/fallback).
+
+    p starts at the observed allele frequencies and gets p_steps fixed-Q
+    iterations (one: on the c3 and c5 workloads a second or third step cost
+    an iteration's time each and saved no quasi-Newton iteration; none
+    doubled the iterations on c5).
 
     Returns dict(history=[(seconds, LL)], chosen='warm'|'em', ll_warm, ll_em,
     iterations).  history[0] is the starting point; seconds count from after
