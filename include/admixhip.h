@@ -191,6 +191,11 @@ int admx_features(admx_ctx *ctx, const float *R, const float *mu, const float *i
 /* q of the state x <- simplex projection (pseudocount 1e-5) of the network's
  * output on ctx's features; FP64 context, network out == K, in == 64 */
 int admx_warm_q(admx_ctx *ctx, admx_mlp *m);
+/* p of the state x <- every SNP's observed allele frequency (alt / 2 nonmissing,
+ * clipped to [1e-5, 1 - 1e-5]) for all K populations, computed on the device
+ * from the resident BED; FP64 context.  No reference counterpart (the warm
+ * start's first p). */
+int admx_warm_p(admx_ctx *ctx);
 
 /* ---- BED decode and counts (decode tables src/loops.jl:2-5).  No count
  * function exists in the reference; these are exact integer counts. */
@@ -238,6 +243,9 @@ int admx_lbqn_step(admx_ctx *ctx, int64_t iter, const void *x, const void *x_nex
 /* ---- device-resident state (AdmixData mirrored in HBM) */
 int admx_set_x(admx_ctx *ctx, int which, const void *host);
 int admx_get_x(admx_ctx *ctx, int which, void *host);
+/* state vector dst <- state vector src, on the device (copyto! between the
+ * device mirror's x, x_next, x_next2, x_tmp) */
+int admx_copy_x(admx_ctx *ctx, int dst, int src);
 int admx_set_uv(admx_ctx *ctx, const void *U, const void *V);
 int admx_get_uv(admx_ctx *ctx, void *U, void *V);
 int admx_set_ll(admx_ctx *ctx, double ll_prev, double ll_new);

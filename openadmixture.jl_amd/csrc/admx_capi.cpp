@@ -1442,6 +1442,18 @@ int admx_set_x(admx_ctx* c, int which, const void* host) {
   });
 }
 
+int admx_copy_x(admx_ctx* c, int dst, int src) {
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (dst < 0 || dst > 3 || src < 0 || src > 3) return bad("bad state vector");
+    if (dst != src)
+      HIPCHK(hipMemcpyAsync(c->x[dst], c->x[src], (size_t)c->N() * c->esz, hipMemcpyDeviceToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
+}
+
 int admx_get_x(admx_ctx* c, int which, void* host) {
   return guarded([&]() -> int {
     CHK(check_ctx(c));

@@ -192,8 +192,26 @@ __global__ __launch_bounds__(256) void k_probs_to_q(const float* __restrict__ Y,
   if (e < n) q[e] = (double)Y[e];
 }
 
+// p[k,j] = observed allele frequency of SNP j, alt / max(2 nonmissing, 1),
+// clipped to [1e-5, 1 - 1e-5], for every k (the warm start's first p).
+__global__ __launch_bounds__(256) void k_freq_to_p(const unsigned long long* __restrict__ nm,
+                                                   const unsigned long long* __restrict__ alt, int64_t J, int K,
+                                                   double* __restrict__ p) {
+  const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  if (j >= J) return;
+  const unsigned long long d = 2ull * nm[j];
+  double f = (double)alt[j] / (double)(d > 0 ? d : 1ull);
+  f = fmin(fmax(f, 1e-5), 1.0 - 1e-5);
+  for (int k = 0; k < K; ++k) p[(int64_t)K * j + k] = f;
+}
+
 // ------------------------------------------------------------------ launchers
 static unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
+
+void launch_freq_to_p(const unsigned long long* nm, const unsigned long long* alt, int64_t J, int K, double* p,
+                      hipStream_t s) {
+  hipLaunchKernelGGL(k_freq_to_p, dim3(blocks_for(J)), dim3(256), 0, s, nm, alt, J, K, p);
+}
 
 void launch_features(const uint8_t* bed, int64_t pitch, int64_t I, int64_t Jp, const float* R, const float* mu,
                      const float* isd, int nchunk, int64_t jchunk, float* part, float* F, hipStream_t s) {

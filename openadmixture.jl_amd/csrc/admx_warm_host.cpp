@@ -343,4 +343,27 @@ int admx_warm_q(admx_ctx* c, admx_mlp* m) {
   });
 }
 
+int admx_warm_p(admx_ctx* c) {
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (c->dtype != ADMX_F64) return bad("the warm start writes a FP64 context");
+    CHK(need_bed(c));
+    CHK(need_resident(c, "the warm start's allele frequencies"));
+    const size_t jb = sizeof(unsigned long long) * (size_t)c->J;
+    CHK(c->st[0].grow(jb));
+    CHK(c->st[1].grow(jb));
+    auto* nm = static_cast<unsigned long long*>(c->st[0].p);
+    auto* alt = static_cast<unsigned long long*>(c->st[1].p);
+    {
+      ProfScope ps(c, "warm_p");
+      launch_counts_snp(c->bed, c->pitch, c->I, c->J, nm, alt, c->s);
+      launch_freq_to_p(nm, alt, c->J, (int)c->K, static_cast<double*>(c->pp(c->x[ADMX_X])), c->s);
+    }
+    CHK(check_launch());
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
+}
+
 }  // extern "C"

@@ -171,6 +171,14 @@ class HIPAdmixData:
         """q of the state x <- projected network output on the stored features."""
         N.check(N.lib().admx_warm_q(self._h, net.handle))
 
+    def warm_p(self):
+        """p of the state x <- observed allele frequencies (clipped), on the device."""
+        N.check(N.lib().admx_warm_p(self._h))
+
+    def copy_x(self, dst: int, src: int):
+        """State vector dst <- state vector src, on the device."""
+        N.check(N.lib().admx_copy_x(self._h, int(dst), int(src)))
+
     # ------------------------------------------------------------ state
     def set_x(self, which: int, x):
         x = _f(x, self.T, (self.K, self.I + self.J))

@@ -31,7 +31,6 @@ import time
 import numpy as np
 
 from . import _native as N
-from .algorithms import init_em
 from .hip import HIPAdmixData
 from .structs import AdmixData
 
@@ -208,11 +207,7 @@ def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_i
     t0 = time.perf_counter()
     panel.features(ctx, to_host=False)
     ctx.warm_q(net)
-    snm, salt, _ = ctx.counts()
-    f = np.clip(salt / np.maximum(2 * snm, 1), 1e-5, 1 - 1e-5)
-    x = ctx.get_x(N.ADMX_X)
-    x[:, I:] = f[None, :]
-    ctx.set_x(N.ADMX_X, x)
+    ctx.warm_p()  # observed allele frequencies, on the device
     # admixture_qn! iterations from iter = 1 write every quasi-Newton history
     # column before reading it (src/algorithms_outer.jl:125-133), so the
     # device's U and V need no upload for these fresh runs
@@ -222,16 +217,19 @@ def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_i
     ll_warm = ctx.get_ll()[1]
     chosen, ll_em = "warm", None
     if judge:
-        x_warm = ctx.get_x(N.ADMX_X)
-        init_em(d, None, 1, d_cu=ctx, log=None)
-        ll_em = d.ll_new
+        # the warm start waits in x_tmp while one FRAPPE EM step runs from the
+        # random start (init_em!, src/algorithms_outer.jl:12-45)
+        ctx.copy_x(N.ADMX_X_TMP, N.ADMX_X)
+        ctx.set_x(N.ADMX_X, d.x)
+        ctx.init_em(1)
+        ll_em = ctx.get_ll()[1]
         if not (ll_warm >= ll_em):
             chosen = "em"
             if em_iters > 1:
-                init_em(d, None, em_iters - 1, d_cu=ctx, log=None)
+                ctx.init_em(em_iters - 1)
         else:
-            ctx.set_x(N.ADMX_X, x_warm)
-    ll0 = ll_warm if chosen == "warm" else d.ll_new
+            ctx.copy_x(N.ADMX_X, N.ADMX_X_TMP)
+    ll0 = ll_warm if chosen == "warm" else ctx.get_ll()[1]
     ctx.set_ll(float("nan"), ll0)
     hist = [(time.perf_counter() - t0, ll0)]
     its = 0

@@ -127,3 +127,31 @@ def test_warm_start_gains_over_random_init(oa, O):
     assert n_warm < n_plain
     assert abs(res["warm"][-1][1] - ll_star) <= 1e-6 * abs(ll_star)
     net.close()
+
+
+def test_warm_p_and_state_copy(oa, O):
+    """admx_warm_p: every p[k, j] is SNP j's observed allele frequency
+    alt / max(2 nonmissing, 1), clipped to [1e-5, 1 - 1e-5] (checked against the
+    oracle's counts, bit for bit, including an all-missing and a monomorphic
+    SNP); q is untouched.  admx_copy_x copies a state vector exactly."""
+    print("warm p", flush=True)
+    I, J, K = 333, 700, 4
+    _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.05, seed=71)
+    B = bed.reshape((pitch, J), order="F").copy()
+    B[: (I + 3) // 4, 10] = 0x55  # all missing
+    B[: (I + 3) // 4, 11] = 0xFF  # every genotype 2
+    B = np.asfortranarray(B)
+    c = oa.HIPAdmixData(I, J, K, 3)
+    c.upload_bed(B)
+    nm, alt, _ = O.counts(np.asfortranarray(B).ravel(order="F"), pitch, I, J)
+    x0 = np.asfortranarray(np.random.default_rng(2).random((K, I + J)))
+    c.set_x(0, x0)
+    c.warm_p()
+    x = c.get_x(0)
+    f = np.clip(alt / np.maximum(2 * nm, 1), 1e-5, 1 - 1e-5)
+    assert np.array_equal(x[:, I:], np.tile(f, (K, 1)))
+    assert np.array_equal(x[:, :I], x0[:, :I])
+    assert f[10] == 1e-5 and f[11] == 1 - 1e-5
+    c.copy_x(3, 0)
+    assert np.array_equal(c.get_x(3), x)
+    c.close()

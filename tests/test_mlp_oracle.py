@@ -84,9 +84,13 @@ def _scripted(oa, ll_warm, ll_em):
         def __init__(self):
             self.I, self.J, self.K, self.Q, self.T = 4, 6, 2, 3, np.dtype(np.float64)
             self._h = None
-            self.x = np.full((2, 10), 0.5, order="F")
+            self.xs = {w: np.full((2, 10), 0.5, order="F") for w in range(4)}  # x, x_next, x_next2, x_tmp
             self.ll = (float("nan"), float("nan"))
             self.calls = []
+
+        @property
+        def x(self):
+            return self.xs[0]
 
         def close(self):
             pass
@@ -103,16 +107,21 @@ def _scripted(oa, ll_warm, ll_em):
 
         def warm_q(self, net):
             self.calls.append("warm_q")
-            self.x[:, :4] = 0.9  # marks the proposal
+            self.xs[0][:, :4] = 0.9  # marks the proposal
 
-        def counts(self):
-            return np.full(6, 4), np.full(6, 3), np.full(4, 6)
+        def warm_p(self):
+            self.calls.append("warm_p")
+            self.xs[0][:, 4:] = 0.375
+
+        def copy_x(self, dst, src):
+            self.xs[dst] = self.xs[src].copy(order="F")
 
         def get_x(self, which, out=None):
-            return self.x.copy(order="F") if out is None else np.copyto(out, self.x) or out
+            x = self.xs[which]
+            return x.copy(order="F") if out is None else np.copyto(out, x) or out
 
         def set_x(self, which, x):
-            self.x = np.array(x, order="F")
+            self.xs[which] = np.array(x, order="F")
 
         def set_uv(self, U, V):
             pass
@@ -136,7 +145,7 @@ def _scripted(oa, ll_warm, ll_em):
         def init_em(self, iters):
             self.calls.append(f"em{iters}")
             self.ll = (self.ll[1], ll_em)
-            self.x[:, :4] = 0.1  # marks the EM path
+            self.xs[0][:, :4] = 0.1  # marks the EM path
             return np.full(iters, ll_em)
 
     return Fake()
@@ -155,9 +164,11 @@ def test_judge_keeps_the_better_start(oa, ll_warm, ll_em, chosen):
     r = warm_started_fit(ctx, _Net(), panel, seed=1, max_iter=3, p_steps=2)
     assert r["chosen"] == chosen
     if chosen == "warm":
-        assert ctx.calls == ["features", "warm_q", "em1", "qn"]
+        assert ctx.calls == ["features", "warm_q", "warm_p", "em1", "qn"]
         assert np.all(ctx.x[:, :4] == 0.9)  # the proposal is restored after the judge's EM step
+        assert np.all(ctx.x[:, 4:] == 0.375)
         assert r["history"][0][1] == ll_warm
     else:
-        assert ctx.calls == ["features", "warm_q", "em1", "em4", "qn"]  # fallback: EM(5) from the random start
+        assert ctx.calls == ["features", "warm_q", "warm_p", "em1", "em4", "qn"]  # fallback: EM(5) from the random start
+        assert np.all(ctx.x[:, :4] == 0.1)
         assert r["history"][0][1] == ll_em
