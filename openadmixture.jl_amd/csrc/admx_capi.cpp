@@ -727,7 +727,12 @@ int dev_init_em(admx_ctx* c, int64_t iters, double* lls) {
       HIPCHK(hipMemcpyAsync(lls + it, c->lls + LL_TMP, sizeof(double), hipMemcpyDeviceToHost, c->s));
     }
   }
-  CHK(ll_to_slot<T>(c, X, c->pp(X), 0, c->I, 0, c->J, LL_NEW));
+  // d.ll_new is the log-likelihood of the state the last EM step left, which
+  // that step's pass has just summed (same kernels and order: the same bits).
+  if (lls && iters > 0)
+    HIPCHK(hipMemcpyAsync(c->lls + LL_NEW, c->lls + LL_TMP, sizeof(double), hipMemcpyDeviceToDevice, c->s));
+  else
+    CHK(ll_to_slot<T>(c, X, c->pp(X), 0, c->I, 0, c->J, LL_NEW));
   HIPCHK(hipStreamSynchronize(c->s));
   return ADMX_OK;
 }
