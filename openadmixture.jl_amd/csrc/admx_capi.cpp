@@ -364,6 +364,54 @@ int ll_mixed(admx_ctx* c, const double* q, const double* p, int slot) {
   return check_launch();
 }
 
+// BF16 form: the FRAPPE EM sums of the state x on the matrix cores, into the
+// FP64 partial layout of the FP64 EM passes (chunk count in *nchunk).
+bool em_on_matrix_cores(const admx_ctx* c) {
+  return c->dtype == ADMX_F64 && c->precision == ADMX_PREC_BF16 && !c->impute && !c->stream_snps;
+}
+
+int em_mixed(admx_ctx* c, bool qside, const double* q, const double* p, int* nchunk_out) {
+  const int64_t I = c->I, J = c->J, K = c->K;
+  const int KD = split_table_cols((int)K);
+  const int64_t rows_q = (I + 255) / 256 * 256, rows_p = J + 32;
+  CHK(c->lqt.grow(sizeof(uint16_t) * KD * rows_q));
+  CHK(c->lpt.grow(sizeof(uint16_t) * KD * rows_p));
+  CHK(c->lmt.grow(sizeof(uint16_t) * KD * rows_p));
+  const int64_t ld = (qside ? (J + 31) / 32 * 32 : rows_q) + 32;  // split rows of p (em_q) or q (em_p)
+  CHK(c->lps.grow(sizeof(uint16_t) * (qside ? 64 : 32) * ld));
+  if (!qside) CHK(c->lqc.grow(sizeof(float) * I));
+  int64_t nchunk = 0;
+  {
+    ProfScope ps(c, qside ? "qside_em_mfma" : "pside_em_mfma");
+    launch_split_tables((int)K, q, I, rows_q, 0, c->lqt.p, nullptr, c->s);
+    launch_split_tables((int)K, p, J, rows_p, 1, c->lpt.p, c->lmt.p, c->s);
+    if (qside) {
+      uint16_t* ps = static_cast<uint16_t*>(c->lps.p);  // BF16 rows: p, then 1 - p
+      launch_em_split((int)K, p, J, ld, ps, ps + 32 * ld, nullptr, c->s);
+      const int64_t nbx = (I + 255) / 256;
+      nchunk = std::max<int64_t>(1, (c->target_waves / 4 + nbx - 1) / nbx);
+      nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, J / 256));
+      const int64_t jchunk = ((J + nchunk - 1) / nchunk + 31) / 32 * 32;
+      nchunk = (J + jchunk - 1) / jchunk;
+      CHK(c->part.grow(sizeof(double) * (size_t)(nchunk * (K + 1) * I)));
+      launch_emq_mfma((int)K, c->bed, c->pitch, I, J, jchunk, (int)nchunk, q, c->lqt.p, c->lpt.p, c->lmt.p, ps,
+                      ps + 32 * ld, ld, static_cast<double*>(c->part.p), c->s);
+    } else {
+      launch_em_split((int)K, q, I, ld, c->lps.p, nullptr, static_cast<float*>(c->lqc.p), c->s);
+      const int64_t nbj = (J + 255) / 256;
+      nchunk = std::max<int64_t>(1, (c->target_waves / 4 + nbj - 1) / nbj);
+      nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, I / 256));
+      const int64_t ichunk = ((I + nchunk - 1) / nchunk + 63) / 64 * 64;
+      nchunk = (I + ichunk - 1) / ichunk;
+      CHK(c->part.grow(sizeof(double) * (size_t)(nchunk * 2 * K * J)));
+      launch_emp_mfma((int)K, c->bed, c->pitch, I, J, ichunk, (int)nchunk, static_cast<const float*>(c->lqc.p),
+                      c->lqt.p, c->lpt.p, c->lmt.p, c->lps.p, ld, static_cast<double*>(c->part.p), c->s);
+    }
+  }
+  *nchunk_out = (int)nchunk;
+  return check_launch();
+}
+
 // LL of (q, p) over a range into device slot lls[slot] (accumulate optional).
 // The BF16 form takes the matrix-core path for the device state's full-cohort
 // LL (allow_mixed); the tile-loop entry point always runs the FP64/FP32 loop.
@@ -547,7 +595,12 @@ int dev_em_q(admx_ctx* c) {
   void* X = c->x[ADMX_X];
   int nchunk = 0;
   int64_t np = 0;
-  CHK(pass_q<T>(c, MODE_EM, X, c->pp(X), 0, c->I, 0, c->J, &nchunk, &np));
+  if (em_on_matrix_cores(c)) {
+    CHK(need_resident(c, "the matrix-core EM passes"));
+    CHK(em_mixed(c, true, static_cast<const double*>(X), static_cast<const double*>(c->pp(X)), &nchunk));
+  } else {
+    CHK(pass_q<T>(c, MODE_EM, X, c->pp(X), 0, c->I, 0, c->J, &nchunk, &np));
+  }
   FinArgs f{c->part.p, c->I, nchunk, 0, c->J, X, c->x[ADMX_X_NEXT], nullptr, nullptr, nullptr, 0, 0, nullptr};
   ProfScope ps(c, "fin_q_em");
   launch_fin_q_em<T>((int)c->K, f, c->s);
@@ -558,7 +611,12 @@ template <typename T>
 int dev_em_p(admx_ctx* c) {
   void* X = c->x[ADMX_X];
   int nchunk = 0;
-  CHK(pass_p<T>(c, MODE_EM, X, c->pp(X), 0, c->I, 0, c->J, &nchunk));
+  if (em_on_matrix_cores(c)) {
+    CHK(need_resident(c, "the matrix-core EM passes"));
+    CHK(em_mixed(c, false, static_cast<const double*>(X), static_cast<const double*>(c->pp(X)), &nchunk));
+  } else {
+    CHK(pass_p<T>(c, MODE_EM, X, c->pp(X), 0, c->I, 0, c->J, &nchunk));
+  }
   FinArgs f{c->part.p, c->J, nchunk, 0, c->J, c->pp(X), c->pp(c->x[ADMX_X_NEXT]), nullptr, nullptr, nullptr, 0, 0,
             nullptr};
   ProfScope ps(c, "fin_p_em");

@@ -117,6 +117,7 @@ struct admx_ctx {
   int precision = ADMX_PREC_F64;
   admx_host::DevBuf fq, fp, ppt, mmt, qqt;  // mixed-precision pass inputs
   admx_host::DevBuf lqt, lpt, lmt;           // BF16 split tables of the matrix-core log-likelihood
+  admx_host::DevBuf lps, lqc;                // BF16 form's EM: two-term split rows of q or p, 1 - sum_k q_ik
   admx_host::DevBuf feat, featpart;          // warm-start features (I x 64) and their chunk partials
   // J-chunk streaming: the BED lives in pinned host memory, or in a PLINK .bed
   // file read chunk by chunk, and every pass streams it through two device

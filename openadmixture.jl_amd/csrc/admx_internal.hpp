@@ -98,6 +98,19 @@ void launch_split_tables(int K, const double* x, int64_t n, int64_t rows, int ro
                          hipStream_t s);
 void launch_ll_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
                     const double* q, const void* QT, const void* PT, const void* MT, double* part, hipStream_t s);
+// FRAPPE EM sums of the BF16 form on the matrix cores: s and t as for the
+// log-likelihood, the per-genotype weights in FP32, and the sums over SNPs
+// (em_q) or individuals (em_p) as BF16 products against the two-term split
+// rows of p and 1 - p, or of q ([32][ld], rows 0..K-1 the leading BF16 term,
+// rows 16..16+K-1 the residual), accumulated in FP32 and written as the FP64
+// partials of the FP64 passes (K+1 per individual, 2K per SNP, per chunk).
+void launch_em_split(int K, const double* x, int64_t n, int64_t ld, void* S, void* S2, float* qc, hipStream_t s);
+void launch_emq_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
+                     const double* q, const void* QT, const void* PT, const void* MT, const void* PS, const void* MS,
+                     int64_t ldps, double* part, hipStream_t s);
+void launch_emp_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t ichunk, int nchunk,
+                     const float* qc, const void* QT, const void* PT, const void* MT, const void* QS, int64_t ldqs,
+                     double* part, hipStream_t s);
 
 // Batched QP on host-provided tableau inputs (XtX, negated Xtz, x): delta out.
 template <typename T> void launch_qp_batch(int K, bool simplex, int64_t n, const void* XtX, const void* Xtz_neg,

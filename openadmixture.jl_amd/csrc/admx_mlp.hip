@@ -35,12 +35,28 @@ __global__ __launch_bounds__(256) void k_features(const uint8_t* __restrict__ be
   float acc[kFeat];
 #pragma unroll
   for (int e = 0; e < kFeat; ++e) acc[e] = 0.f;
-  for (int64_t j = jb; j < je; ++j) {
-    const uint32_t c = (gp[pitch * j] >> sh) & 3u;
-    const float z = (c == 1u) ? 0.f : ((float)code_g(c) - mu[j]) * isd[j];
-    const float* __restrict__ r = R + (int64_t)kFeat * j;
+  // Genotype bytes of SNPs [j, j+8) are in registers while those of [j+8, j+16)
+  // are loaded: one wait per 8 SNPs instead of one per SNP.  Rows up to
+  // je + 14 <= J + 14 are read; the BED holds J + 16 rows.
+  constexpr int U = 8;
+  uint32_t cur[U], nxt[U];
 #pragma unroll
-    for (int e = 0; e < kFeat; ++e) acc[e] = __builtin_fmaf(z, r[e], acc[e]);
+  for (int u = 0; u < U; ++u) cur[u] = gp[pitch * (jb + u)];
+  for (int64_t j = jb; j < je; j += U) {
+#pragma unroll
+    for (int u = 0; u < U; ++u) nxt[u] = gp[pitch * (j + U + u)];
+#pragma unroll
+    for (int u = 0; u < U; ++u) {
+      if (j + u < je) {
+        const uint32_t c = (cur[u] >> sh) & 3u;
+        const float z = (c == 1u) ? 0.f : ((float)code_g(c) - mu[j + u]) * isd[j + u];
+        const float* __restrict__ r = R + (int64_t)kFeat * (j + u);
+#pragma unroll
+        for (int e = 0; e < kFeat; ++e) acc[e] = __builtin_fmaf(z, r[e], acc[e]);
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; ++u) cur[u] = nxt[u];
   }
   if (i >= I) return;
   float* __restrict__ out = part + ((int64_t)blockIdx.y * I + i) * kFeat;

@@ -107,3 +107,68 @@ def test_matrix_core_loglikelihood(oa, O, I, J, K, qsum):
     c.set_precision("f64")
     assert abs(c.loglikelihood(0) - ref) <= 1e-10 * abs(ref)
     c.close()
+
+
+@pytest.mark.parametrize("qsum", [1.0, 0.97])
+@pytest.mark.parametrize("I,J,K", [(1, 1, 1), (37, 45, 3), (1500, 4100, 8), (700, 333, 12), (300, 260, 16),
+                                   (130, 700, 2)])
+def test_matrix_core_em_step(oa, O, I, J, K, qsum):
+    """BF16 form: one FRAPPE EM step (em_q!, em_p!) with s, t and the sums over
+    SNPs / individuals on the matrix cores against the FP64 oracle on the same
+    state.  The weights are FP32 and every product pairs two-term BF16 splits
+    (about 2^-17 relative per term), so the bar is 2e-5 relative to each
+    column's maximum for q_next and 2e-5 absolute for p_next (the ratio
+    p_tmp / (p_tmp + p_next) lies in [0, 1]).  Columns of q summing to 0.97 (as
+    after EM steps with missing entries) exercise the t = 1 - s correction."""
+    print("mfma em", I, J, K, qsum, flush=True)
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, alpha=0.3, miss=0.03, seed=I + K, p_lo=0.01, p_hi=0.99, beta=True)
+    rng = np.random.default_rng(K + 5)
+    q = rng.dirichlet(0.5 * np.ones(K), size=I).T * qsum
+    p = rng.uniform(1e-3, 1 - 1e-3, size=(K, J))
+    p[:, : J // 4] = np.clip(1 - 10 ** rng.uniform(-3, -1, size=(K, J // 4)), 1e-3, 1 - 1e-3)  # s close to 1
+    x = np.asfortranarray(np.hstack([q, p]))
+    c = oa.HIPAdmixData(I, J, K, 3)
+    c.upload_bed(bed.reshape((pitch, J), order="F"))
+    c.set_precision("bf16")
+    c.set_x(0, x)
+    c.em_q()
+    c.em_p()
+    got = c.get_x(1)
+    st = O.OracleAdmix(bed, pitch, I, J, K, 3, x)
+    st.em_q()
+    st.em_p()
+    ref = st.x_next
+    eq = (np.abs(got[:, :I] - ref[:, :I]).max(0) / np.abs(ref[:, :I]).max(0)).max()
+    ep = np.abs(got[:, I:] - ref[:, I:]).max()
+    print(f"q_next {eq:.2e} relative, p_next {ep:.2e} absolute", flush=True)
+    assert eq <= 2e-5 and ep <= 2e-5, (eq, ep)
+    # the FP64 form on the same context still matches the oracle to 1e-10
+    c.set_precision("f64")
+    c.em_q()
+    c.em_p()
+    f64 = c.get_x(1)
+    assert np.abs(f64[:, :I] - ref[:, :I]).max() <= 1e-10 * np.abs(ref[:, :I]).max()
+    c.close()
+
+
+def test_matrix_core_em_priming_lls(oa, O):
+    """init_em! in the BF16 form (matrix-core EM sums and log-likelihood) follows
+    the FP64 oracle's per-step log-likelihoods to 1e-6 relative."""
+    print("mfma init_em", flush=True)
+    I, J, K = 1200, 3000, 5
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, alpha=0.3, miss=0.02, seed=11)
+    rng = np.random.default_rng(3)
+    x = rng.random((K, I + J))
+    x[:, :I] /= x[:, :I].sum(0)
+    x = np.asfortranarray(x)
+    c = oa.HIPAdmixData(I, J, K, 3)
+    c.upload_bed(bed.reshape((pitch, J), order="F"))
+    c.set_precision("bf16")
+    c.set_x(0, x)
+    got = c.init_em(5)
+    st = O.OracleAdmix(bed, pitch, I, J, K, 3, x)
+    ref = st.init_em(5)
+    rel = np.abs(got - ref) / np.abs(ref)
+    print("per-step LL relative error", rel, flush=True)
+    assert rel.max() <= 1e-6
+    c.close()

@@ -63,9 +63,9 @@ def test_mlp_adam_steps_match_oracle(oa):
     net.close()
 
 
-def test_features_match_oracle(oa, O):
-    print("features", flush=True)
-    I, J, Jp = 1000, 3000, 2500
+@pytest.mark.parametrize("I,J,Jp", [(1000, 3000, 2500), (1001, 2003, 2003), (37, 13, 13), (300, 500, 1)])
+def test_features_match_oracle(oa, O, I, J, Jp):
+    print("features", I, J, Jp, flush=True)
     Qt, Pt, bed, pitch = make_cohort(O, I, J, 3, miss=0.02, seed=3)
     c = oa.HIPAdmixData(I, J, 3, 3)
     c.upload_bed(bed.reshape((pitch, J), order="F"))
