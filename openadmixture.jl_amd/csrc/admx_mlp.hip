@@ -19,50 +19,94 @@
 namespace admx {
 
 // ------------------------------------------------------------------ features
-// One lane per individual; a block per (256 individuals, SNP chunk); partial
-// feature vectors per chunk [chunk][i][64], reduced in chunk order by
-// k_feat_reduce.  The panel row R[j, :] and (mu_j, isd_j) are wave-uniform.
+// F = Z R over the panel SNPs, Z[i][j] = (g_ij - mu_j) isd_j (0 if missing):
+// a GEMM on the matrix cores in FP32 (v_mfma_f32_32x32x2f32: products exact in
+// FP32, FP32 accumulation).  A wave owns 64 individuals (two 32-row tiles) and
+// all 64 features (two 32-column tiles) and walks its SNP chunk two SNPs per
+// product: A[i][k] (lane i + 32k) is Z of individual i at SNP j + k, B[k][n]
+// (lane n + 32k) is R[j + k][n], D[i][n] in lane n + 32((i/4)%2), register
+// (i%4) + 4(i/8).  The workgroup's four waves share the chunk's R rows through
+// LDS, 16 SNPs (4 KB) per block, double-buffered with one barrier per block;
+// each lane's genotype bytes of the next block are loaded while the current
+// one is multiplied.  Partial feature vectors per chunk [chunk][i][64], reduced
+// in chunk order by k_feat_reduce.
+typedef float f32x16 __attribute__((ext_vector_type(16)));
 __global__ __launch_bounds__(256) void k_features(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I,
                                                   int64_t Jp, const float* __restrict__ R,
                                                   const float* __restrict__ mu, const float* __restrict__ isd,
                                                   int64_t jchunk, float* __restrict__ part) {
-  const int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x;
+  __shared__ __align__(16) float rs[2][16][kFeat];
+  const int t = threadIdx.x, lane = t & 63, h = lane >> 5, cn = lane & 31;
+  const int64_t i0 = ((int64_t)blockIdx.x * 4 + (t >> 6)) * 64;
   const int64_t jb = (int64_t)blockIdx.y * jchunk;
   const int64_t je = (jb + jchunk < Jp) ? jb + jchunk : Jp;
-  const int64_t ii = i < I ? i : I - 1;
-  const uint8_t* __restrict__ gp = bed + (ii >> 2);
-  const uint32_t sh = (uint32_t)(ii & 3) * 2u;
-  float acc[kFeat];
+  // genotype byte addresses of the lane's two individuals (clamped past I: their rows are not written)
+  const uint8_t* __restrict__ g0 = bed + ((i0 + cn < I ? i0 + cn : I - 1) >> 2);
+  const uint8_t* __restrict__ g1 = bed + ((i0 + 32 + cn < I ? i0 + 32 + cn : I - 1) >> 2);
+  const uint32_t sh0 = 2u * (uint32_t)((i0 + cn) & 3), sh1 = 2u * (uint32_t)((i0 + 32 + cn) & 3);
+  // R block of SNPs [j, j+16): thread t moves 16 bytes of row j + t/16 (rows past Jp clamped; their Z is 0)
+  auto rload = [&](int64_t j) {
+    const int64_t r = (j + (t >> 4) < Jp) ? j + (t >> 4) : Jp - 1;
+    return *reinterpret_cast<const float4*>(R + r * kFeat + 4 * (t & 15));
+  };
+  // genotype bytes of SNPs j + 2u + h, u = 0..7 (rows up to je + 14 <= J + 14; the BED holds J + 16 rows)
+  uint32_t b0[8], b1[8];
+  auto gload = [&](int64_t j) {
 #pragma unroll
-  for (int e = 0; e < kFeat; ++e) acc[e] = 0.f;
-  // Genotype bytes of SNPs [j, j+8) are in registers while those of [j+8, j+16)
-  // are loaded: one wait per 8 SNPs instead of one per SNP.  Rows up to
-  // je + 14 <= J + 14 are read; the BED holds J + 16 rows.
-  constexpr int U = 8;
-  uint32_t cur[U], nxt[U];
+    for (int u = 0; u < 8; ++u) {
+      b0[u] = g0[pitch * (j + 2 * u + h)];
+      b1[u] = g1[pitch * (j + 2 * u + h)];
+    }
+  };
+  f32x16 acc[2][2] = {{(f32x16){}, (f32x16){}}, {(f32x16){}, (f32x16){}}};
+  float4 rn = rload(jb);
+  *reinterpret_cast<float4*>(&rs[0][t >> 4][4 * (t & 15)]) = rn;
+  gload(jb);
+  __syncthreads();
+  int buf = 0;
+  for (int64_t j = jb; j < je; j += 16) {
+    uint32_t c0[8], c1[8];
 #pragma unroll
-  for (int u = 0; u < U; ++u) cur[u] = gp[pitch * (jb + u)];
-  for (int64_t j = jb; j < je; j += U) {
-#pragma unroll
-    for (int u = 0; u < U; ++u) nxt[u] = gp[pitch * (j + U + u)];
-#pragma unroll
-    for (int u = 0; u < U; ++u) {
-      if (j + u < je) {
-        const uint32_t c = (cur[u] >> sh) & 3u;
-        const float z = (c == 1u) ? 0.f : ((float)code_g(c) - mu[j + u]) * isd[j + u];
-        const float* __restrict__ r = R + (int64_t)kFeat * (j + u);
-#pragma unroll
-        for (int e = 0; e < kFeat; ++e) acc[e] = __builtin_fmaf(z, r[e], acc[e]);
-      }
+    for (int u = 0; u < 8; ++u) {
+      c0[u] = b0[u];
+      c1[u] = b1[u];
+    }
+    const bool more = j + 16 < je;
+    if (more) {
+      rn = rload(j + 16);
+      gload(j + 16);
     }
 #pragma unroll
-    for (int u = 0; u < U; ++u) cur[u] = nxt[u];
-  }
-  if (i >= I) return;
-  float* __restrict__ out = part + ((int64_t)blockIdx.y * I + i) * kFeat;
+    for (int u = 0; u < 8; ++u) {
+      const int64_t jj = j + 2 * u + h;
+      const bool live = jj < je;
+      const int64_t jc = live ? jj : je - 1;
+      const float m = mu[jc], sd = isd[jc];
+      const uint32_t k0 = (c0[u] >> sh0) & 3u, k1 = (c1[u] >> sh1) & 3u;
+      const float z0 = (live && k0 != 1u) ? ((float)code_g(k0) - m) * sd : 0.f;
+      const float z1 = (live && k1 != 1u) ? ((float)code_g(k1) - m) * sd : 0.f;
 #pragma unroll
-  for (int e = 0; e < kFeat; e += 4)
-    *reinterpret_cast<float4*>(out + e) = make_float4(acc[e], acc[e + 1], acc[e + 2], acc[e + 3]);
+      for (int nt = 0; nt < 2; ++nt) {
+        const float bv = rs[buf][2 * u + h][32 * nt + cn];
+        acc[0][nt] = __builtin_amdgcn_mfma_f32_32x32x2f32(z0, bv, acc[0][nt], 0, 0, 0);
+        acc[1][nt] = __builtin_amdgcn_mfma_f32_32x32x2f32(z1, bv, acc[1][nt], 0, 0, 0);
+      }
+    }
+    if (more) *reinterpret_cast<float4*>(&rs[buf ^ 1][t >> 4][4 * (t & 15)]) = rn;
+    __syncthreads();
+    buf ^= 1;
+  }
+#pragma unroll
+  for (int T = 0; T < 2; ++T)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int64_t i = i0 + 32 * T + 8 * (r >> 2) + 4 * h + (r & 3);
+      if (i < I) {
+        float* __restrict__ out = part + ((int64_t)blockIdx.y * I + i) * kFeat + cn;
+        out[0] = acc[T][0][r];
+        out[32] = acc[T][1][r];
+      }
+    }
 }
 
 __global__ __launch_bounds__(256) void k_feat_reduce(const float* __restrict__ part, int64_t n, int nchunk,
@@ -231,7 +275,7 @@ void launch_freq_to_p(const unsigned long long* nm, const unsigned long long* al
 
 void launch_features(const uint8_t* bed, int64_t pitch, int64_t I, int64_t Jp, const float* R, const float* mu,
                      const float* isd, int nchunk, int64_t jchunk, float* part, float* F, hipStream_t s) {
-  dim3 grid(blocks_for(I), (unsigned)nchunk);
+  dim3 grid((unsigned)((I + 255) / 256), (unsigned)nchunk);
   hipLaunchKernelGGL(k_features, grid, dim3(256), 0, s, bed, pitch, I, Jp, R, mu, isd, jchunk, part);
   const int64_t n = I * kFeat;
   const unsigned nb = blocks_for(n) < 8192u ? blocks_for(n) : 8192u;
