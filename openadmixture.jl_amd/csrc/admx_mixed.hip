@@ -652,11 +652,16 @@ __device__ __forceinline__ bf16x8 em_rows(const __bf16* __restrict__ row, int ri
   return __builtin_bit_cast(bf16x8, v);
 }
 
-// FP32 weights of one genotype from its code and the matrix-core s, t.
-__device__ __forceinline__ void em_weights(uint32_t code, float sv, float tv, float& cc, float& dd) {
-  const float r = __builtin_amdgcn_rcpf(sv * tv);
-  cc = (float)code_g(code) * (tv * r);
-  dd = (float)code_h(code) * (sv * r);
+// FP32 weights c = g/s, d = (2-g)/t of one genotype from gm = m g, hm = m (2-g)
+// (0 for missing entries) and the matrix-core s, t.  s t is clamped at 1e-30,
+// so that padding rows (s = 0, from the zero rows of the split tables) and
+// missing entries give c = d = 0 without a select; real genotypes have
+// s t >= 1e-30 by many orders of magnitude (q and p are at least 1e-5 after
+// the projections).
+__device__ __forceinline__ void em_weights(float gm, float hm, float sv, float tv, float& cc, float& dd) {
+  const float r = __builtin_amdgcn_rcpf(fmaxf(sv * tv, 1e-30f));
+  cc = gm * (tv * r);
+  dd = hm * (sv * r);
 }
 
 // Leading and residual BF16 of two FP32 weights, packed.
@@ -742,23 +747,21 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
         sa = __builtin_amdgcn_mfma_f32_32x32x16_bf16(as[m], bq[T][m], sa, 0, 0, 0);
         ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(at[m], bq[T][m], ta, 0, 0, 0);
       }
-      const bool ok = T ? ok1 : ok0;
       const float corr = T ? corr1 : corr0;
+      // Individuals past I (zero split rows: s = 0; their sums are not written)
+      // and SNP rows past the chunk (missing codes) contribute 0.
       uint32_t ch[8], cl[8], dh[8], dl[8];
 #pragma unroll
       for (int a = 0; a < 4; ++a) {
+        const uint32_t codes = (rd[T][a] >> shift) & 0x03030303u;  // SNP 8a + 4h + b in byte b
+        const uint32_t gb = g_bytes(codes), hb = h_bytes(codes);
 #pragma unroll
         for (int b = 0; b < 4; b += 2) {
           float cv[2], dv[2];
 #pragma unroll
-          for (int x = 0; x < 2; ++x) {
-            const uint32_t code = (rd[T][a] >> (8 * (b + x) + shift)) & 3u;
-            float cc, dd;
-            em_weights(code, sa[4 * a + b + x], ta[4 * a + b + x] + corr, cc, dd);
-            const bool live = ok && code != 1u;
-            cv[x] = live ? cc : 0.f;
-            dv[x] = live ? dd : 0.f;
-          }
+          for (int x = 0; x < 2; ++x)
+            em_weights(byte_f(gb, b + x), byte_f(hb, b + x), sa[4 * a + b + x], ta[4 * a + b + x] + corr, cv[x],
+                       dv[x]);
           split_pack(cv[0], cv[1], ch[(4 * a + b) >> 1], cl[(4 * a + b) >> 1]);
           split_pack(dv[0], dv[1], dh[(4 * a + b) >> 1], dl[(4 * a + b) >> 1]);
         }
@@ -851,23 +854,24 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
           sa = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aq[m], bp[T][m], sa, 0, 0, 0);
           ta = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aq[m], bm[T][m], ta, 0, 0, 0);
         }
+        // Individuals past I have zero split rows (s = 0) and t correction 0, so
+        // they contribute 0 whatever their padding bits hold; chunks other than
+        // the last end on a 64-individual block.
         uint32_t ch[8], cl[8], dh[8], dl[8];
 #pragma unroll
         for (int a = 0; a < 4; ++a) {
           const int xb = 8 * sub + 2 * a + h;  // byte of the 64-individual block
           const uint32_t byte = (gw[T][xb >> 2] >> (8 * (xb & 3))) & 0xffu;
+          // the four 2-bit codes of individuals 8a + 4h + b into byte b
+          const uint32_t codes = (byte | (byte << 6) | (byte << 12) | (byte << 18)) & 0x03030303u;
+          const uint32_t gb = g_bytes(codes), hb = h_bytes(codes);
 #pragma unroll
           for (int b = 0; b < 4; b += 2) {
             float cv[2], dv[2];
 #pragma unroll
             for (int x = 0; x < 2; ++x) {
               const int r = 4 * a + b + x;
-              const uint32_t code = (byte >> (2 * (b + x))) & 3u;
-              float cc, dd;
-              em_weights(code, sa[r], ta[r] + corr[r], cc, dd);
-              const bool live = is + 8 * a + 4 * h + b + x < ib1 && code != 1u;
-              cv[x] = live ? cc : 0.f;
-              dv[x] = live ? dd : 0.f;
+              em_weights(byte_f(gb, b + x), byte_f(hb, b + x), sa[r], ta[r] + corr[r], cv[x], dv[x]);
             }
             split_pack(cv[0], cv[1], ch[(4 * a + b) >> 1], cl[(4 * a + b) >> 1]);
             split_pack(dv[0], dv[1], dh[(4 * a + b) >> 1], dl[(4 * a + b) >> 1]);
