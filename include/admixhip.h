@@ -76,7 +76,9 @@ extern "C" {
 #define ADMX_PREC_BF16 2 /* gradient in FP32, Hessian on BF16 matrix cores with FP32 accumulation; the
                           * device state's log-likelihood takes s = q.p and 1 - s = q.(1-p) from the matrix
                           * cores too (three-term BF16 splits, FP32 accumulation, relative error a few 1e-7),
-                          * with the per-genotype products and the reduction in FP64 */
+                          * with the per-genotype products and the reduction in FP64; admx_em_q / admx_em_p
+                          * and admx_init_em form their sums on the matrix cores (FP32 weights, two-term BF16
+                          * products, FP32 accumulation; q_next, p_next within 2e-5 of FP64) */
 
 typedef struct admx_ctx admx_ctx;
 
