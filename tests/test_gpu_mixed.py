@@ -6,6 +6,8 @@ per test:
   Hessian (XtX)   FP32: 1e-5, BF16: 5e-3   relative to the column maximum
   gradient (Xtz)  FP32: 1e-5, BF16: 1e-5   relative to the column maximum
   converged LL    |LL_form - LL_f64| / |LL_f64| < 1e-6 (the benchmark bar is 1e-4)
+  BF16 EM step    q_next 2e-5 relative to the column maximum, p_next 2e-5 absolute
+  BF16 LL         1e-6 relative (matrix-core s and t)
 """
 import numpy as np
 import pytest
