@@ -134,20 +134,22 @@ __global__ __launch_bounds__(256) void k_sgemm(int M, int N, int Kd, const float
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
   float acc[4][4] = {};
   for (int k0 = 0; k0 < Kd; k0 += 16) {
-    // 64 x 16 tiles of op(A) and op(B): 1024 values, 4 per lane
+    // 64 x 16 tiles of op(A) and op(B): 1024 values, 4 per lane; consecutive
+    // lanes walk the dimension that is contiguous in memory (k for A and for a
+    // transposed B, m for a transposed A, n for B), so every load is coalesced
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int idx = threadIdx.x + 256 * r;
-      const int mm = idx >> 4, kk = idx & 15;  // A tile as [m][k]
-      const int gm = m0 + mm, gk = k0 + kk;
+      const int mm = TA ? (idx & 63) : (idx >> 4), ka = TA ? (idx >> 6) : (idx & 15);  // A tile as [m][k]
+      const int gm = m0 + mm, gka = k0 + ka;
       float av = 0.f;
-      if (gm < M && gk < Kd) av = TA ? A[(int64_t)gk * lda + gm] : A[(int64_t)gm * lda + gk];
-      As[kk][mm] = av;
-      const int nn = idx >> 4;  // B tile as [k][n]
-      const int gn = n0 + nn;
+      if (gm < M && gka < Kd) av = TA ? A[(int64_t)gka * lda + gm] : A[(int64_t)gm * lda + gka];
+      As[ka][mm] = av;
+      const int nn = TB ? (idx >> 4) : (idx & 63), kb = TB ? (idx & 15) : (idx >> 6);  // B tile as [k][n]
+      const int gn = n0 + nn, gkb = k0 + kb;
       float bv = 0.f;
-      if (gn < N && gk < Kd) bv = TB ? B[(int64_t)gn * ldb + gk] : B[(int64_t)gk * ldb + gn];
-      Bs[kk][nn] = bv;
+      if (gn < N && gkb < Kd) bv = TB ? B[(int64_t)gn * ldb + gkb] : B[(int64_t)gkb * ldb + gn];
+      Bs[kb][nn] = bv;
     }
     __syncthreads();
 #pragma unroll
@@ -183,13 +185,25 @@ __global__ __launch_bounds__(256) void k_sgemm(int M, int N, int Kd, const float
   }
 }
 
-// db[n] = sum_b dZ[b, n]  (fixed-order per column)
+// db[n] = sum_b dZ[b, n] in a fixed order: a block owns 32 columns, thread
+// (r, c) sums rows r, r + 8, r + 16, ... of column c (coalesced 128-byte rows,
+// 8 independent load streams per column instead of one), then the 8 slices
+// are added in slice order through LDS.
 __global__ __launch_bounds__(256) void k_colsum(const float* __restrict__ X, int M, int N, float* __restrict__ out) {
-  const int n = blockIdx.x * 256 + threadIdx.x;
-  if (n >= N) return;
+  __shared__ float part[8][32];
+  const int c = threadIdx.x & 31, r = threadIdx.x >> 5;
+  const int n = blockIdx.x * 32 + c;
   float s = 0.f;
-  for (int m = 0; m < M; ++m) s += X[(int64_t)m * N + n];
-  out[n] = s;
+  if (n < N)
+    for (int m = r; m < M; m += 8) s += X[(int64_t)m * N + n];
+  part[r][c] = s;
+  __syncthreads();
+  if (r == 0 && n < N) {
+    float t = part[0][c];
+#pragma unroll
+    for (int k = 1; k < 8; ++k) t += part[k][c];
+    out[n] = t;
+  }
 }
 
 // Softmax rows of Z (M x K, K <= 16) into Y; with targets T, dZ = (Y - T)/M and
@@ -300,7 +314,7 @@ void launch_sgemm(bool ta, bool tb, int M, int N, int Kd, const float* A, int ld
 }
 
 void launch_colsum(const float* X, int M, int N, float* out, hipStream_t s) {
-  hipLaunchKernelGGL(k_colsum, dim3(blocks_for(N)), dim3(256), 0, s, X, M, N, out);
+  hipLaunchKernelGGL(k_colsum, dim3((unsigned)((N + 31) / 32)), dim3(256), 0, s, X, M, N, out);
 }
 
 void launch_softmax_ce(const float* Z, int M, int K, const float* T, float* Y, float* dZ, float* rowloss,
