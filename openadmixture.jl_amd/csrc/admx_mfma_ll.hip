@@ -307,9 +307,13 @@ __device__ __forceinline__ bf16x8 b8(const uint32_t* w) {
 
 // em_q: the wave and tile layout of k_ll_mfma (64 individuals, 32-SNP tiles);
 // partials [chunk][K+1][I]: sum c p_k + d (1 - p_k) (k < K) and 0 (the FP64
-// pass's separate sum d).
+// pass's separate sum d).  Both EM kernels are held to 256 registers (two waves
+// per SIMD): left to the compiler they take 266-340 and run one wave per SIMD,
+// which leaves the MFMA and table-load latencies exposed (13.9 / 12.0 ms
+// instead of 6.6 / 6.4 ms at I = J = 100,000, K = 8).
 template <int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_emq_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int64_t J,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_emq_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int64_t J,
                                                   int64_t jchunk, const double* __restrict__ q64,
                                                   const __bf16* __restrict__ QT, const __bf16* __restrict__ PT,
                                                   const __bf16* __restrict__ MT, const __bf16* __restrict__ PS,
@@ -427,7 +431,8 @@ __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k
 // ahead; the t correction 1 - sum q of individual i comes from lane i % 32.
 // Partials [chunk][2K][J]: sum c q_k, then sum d q_k.
 template <int K>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2))) void k_emp_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int64_t J,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_emp_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int64_t J,
                                                   int64_t ichunk, const float* __restrict__ qc,
                                                   const __bf16* __restrict__ QT, const __bf16* __restrict__ PT,
                                                   const __bf16* __restrict__ MT, const __bf16* __restrict__ QS,
