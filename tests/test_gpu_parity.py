@@ -364,6 +364,26 @@ def test_reproducible_bitwise(oa, O):
     assert np.array_equal(outs[0][1], outs[1][1])
 
 
+@pytest.mark.parametrize("prec", ["f64", "bf16"])
+def test_init_em_ll_new_is_last_step(oa, O, prec):
+    """init_em! (src/algorithms_outer.jl:12-45) sums the log-likelihood after
+    every EM step and then d.ll_new of the same state; the device keeps the last
+    step's sum for ll_new, which must equal a fresh pass over that state bit
+    for bit (FP64 passes and the BF16 form's matrix-core pass alike)."""
+    note("init_em ll_new", prec)
+    I, J, K = 900, 1300, 4
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, seed=23)
+    d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(6))
+    c = ctx_for(oa, bed, pitch, I, J, K)
+    c.set_precision(prec)
+    c.set_x(0, d.x)
+    lls = c.init_em(3)
+    ll_new = c.get_ll()[1]
+    assert ll_new == lls[-1]
+    assert c.loglikelihood(0) == ll_new
+    c.close()
+
+
 
 def _ll_class(v):
     return "nan" if np.isnan(v) else ("-inf" if v == -np.inf else "finite")
