@@ -121,8 +121,11 @@ __global__ __launch_bounds__(256) void k_feat_reduce(const float* __restrict__ p
 // ------------------------------------------------------------------ FP32 GEMM
 // C[M x N] = op(A) op(B) (+ bias[n]) (relu) (* (mask > 0)) (+= C if accumulate),
 // row-major; op(A) is M x Kd (A is Kd x M when TA), op(B) is Kd x N (B is N x Kd
-// when TB).  64 x 64 tile per 256-lane block, 4 x 4 outputs per lane, Kd in
-// steps of 16 through LDS.
+// when TB).  64 x 64 tile per 256-lane block, Kd in steps of 16 through LDS;
+// each wave multiplies one 32 x 32 quarter of the tile on the matrix cores
+// (v_mfma_f32_32x32x2f32: FP32 products, FP32 accumulation), A[i][k] in lane
+// i + 32k, B[k][n] in lane n + 32k, D[i][n] in lane n + 32((i/4)%2), register
+// (i%4) + 4(i/8).
 template <bool TA, bool TB>
 __global__ __launch_bounds__(256) void k_sgemm(int M, int N, int Kd, const float* __restrict__ A, int lda,
                                                const float* __restrict__ B, int ldb, float* __restrict__ C, int ldc,
@@ -130,9 +133,10 @@ __global__ __launch_bounds__(256) void k_sgemm(int M, int N, int Kd, const float
                                                const float* __restrict__ mask, int accumulate) {
   __shared__ float As[16][64 + 4];
   __shared__ float Bs[16][64 + 4];
-  const int tx = threadIdx.x & 15, ty = threadIdx.x >> 4;
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6, h = lane >> 5, cn = lane & 31;
+  const int wm = 32 * (w >> 1), wn = 32 * (w & 1);  // the wave's quarter of the tile
   const int m0 = blockIdx.y * 64, n0 = blockIdx.x * 64;
-  float acc[4][4] = {};
+  f32x16 acc = (f32x16){};
   for (int k0 = 0; k0 < Kd; k0 += 16) {
     // 64 x 16 tiles of op(A) and op(B): 1024 values, 4 per lane; consecutive
     // lanes walk the dimension that is contiguous in memory (k for A and for a
@@ -153,35 +157,20 @@ __global__ __launch_bounds__(256) void k_sgemm(int M, int N, int Kd, const float
     }
     __syncthreads();
 #pragma unroll
-    for (int kk = 0; kk < 16; ++kk) {
-      float a[4], b[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        a[u] = As[kk][ty + 16 * u];
-        b[u] = Bs[kk][tx + 16 * u];
-      }
-#pragma unroll
-      for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] = __builtin_fmaf(a[u], b[v], acc[u][v]);
-    }
+    for (int kk = 0; kk < 16; kk += 2)
+      acc = __builtin_amdgcn_mfma_f32_32x32x2f32(As[kk + h][wm + cn], Bs[kk + h][wn + cn], acc, 0, 0, 0);
     __syncthreads();
   }
 #pragma unroll
-  for (int u = 0; u < 4; ++u) {
-    const int gm = m0 + ty + 16 * u;
-    if (gm >= M) continue;
-#pragma unroll
-    for (int v = 0; v < 4; ++v) {
-      const int gn = n0 + tx + 16 * v;
-      if (gn >= N) continue;
-      float val = acc[u][v];
-      if (bias) val += bias[gn];
-      if (relu) val = val > 0.f ? val : 0.f;
-      if (mask && !(mask[(int64_t)gm * ldc + gn] > 0.f)) val = 0.f;
-      float* cp = C + (int64_t)gm * ldc + gn;
-      *cp = accumulate ? *cp + val : val;
-    }
+  for (int r = 0; r < 16; ++r) {
+    const int gm = m0 + wm + 8 * (r >> 2) + 4 * h + (r & 3), gn = n0 + wn + cn;
+    if (gm >= M || gn >= N) continue;
+    float val = acc[r];
+    if (bias) val += bias[gn];
+    if (relu) val = val > 0.f ? val : 0.f;
+    if (mask && !(mask[(int64_t)gm * ldc + gn] > 0.f)) val = 0.f;
+    float* cp = C + (int64_t)gm * ldc + gn;
+    *cp = accumulate ? *cp + val : val;
   }
 }
 
