@@ -556,6 +556,45 @@ void FN(orc_init_em)(FN(orc_state) * st, int64_t iters, double *lls) {
   st->ll_new = FN(st_ll)(st, st->x);
 }
 
+/* The host half of update_q! (src/algorithms_inner.jl:197-230), which the
+ * reference runs on the CPU after its GPU branch has copied XtX and Xtz back:
+ * Xtz .*= -1, one simplex QP per individual (tableau, quadratic_program!),
+ * q_next = q + qdiff, project_q!.  XtX (K x K x I) and Xtz (K x I, negated in
+ * place) are the pass outputs; qdiff is K x I scratch. */
+void FN(orc_qp_update_q)(T *XtX, T *Xtz, const T *q, T *q_next, T *qdiff, int64_t K, int64_t I, int32_t *status,
+                         int nthreads) {
+  for (int64_t a = 0; a < K * I; ++a) Xtz[a] *= (T)-1;
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t i = 0; i < I; ++i) {
+    T tab[18 * 18], dd[18], tmp[18], pmin[16], pmax[16];
+    uint8_t swept[16];
+    for (int64_t k = 0; k < K; ++k) { pmin[k] = 0; pmax[k] = 1; }
+    FN(orc_create_tableau)(tab, XtX + K * K * i, Xtz + K * i, q + K * i, K, 1, (T)0);
+    const int r = FN(orc_quadratic_program)(qdiff + K * i, tab, q + K * i, pmin, pmax, K, 1, dd, tmp, swept);
+    if (status) status[i] = r;
+  }
+  for (int64_t a = 0; a < K * I; ++a) q_next[a] = q[a] + qdiff[a];
+  FN(orc_project_q)(q_next, K, I);
+}
+
+/* The host half of update_p! (src/algorithms_inner.jl:273-307): Xtz .*= -1,
+ * one box QP per SNP, p_next = p + pdiff, project_p!. */
+void FN(orc_qp_update_p)(T *XtX, T *Xtz, const T *p, T *p_next, T *pdiff, int64_t K, int64_t J, int32_t *status,
+                         int nthreads) {
+  for (int64_t a = 0; a < K * J; ++a) Xtz[a] *= (T)-1;
+#pragma omp parallel for num_threads(nthreads > 0 ? nthreads : 1)
+  for (int64_t j = 0; j < J; ++j) {
+    T tab[18 * 18], dd[18], tmp[18], pmin[16], pmax[16];
+    uint8_t swept[16];
+    for (int64_t k = 0; k < K; ++k) { pmin[k] = 0; pmax[k] = 1; }
+    FN(orc_create_tableau)(tab, XtX + K * K * j, Xtz + K * j, p + K * j, K, 0, (T)0);
+    const int r = FN(orc_quadratic_program)(pdiff + K * j, tab, p + K * j, pmin, pmax, K, 0, dd, tmp, swept);
+    if (status) status[j] = r;
+  }
+  for (int64_t a = 0; a < K * J; ++a) p_next[a] = p[a] + pdiff[a];
+  FN(orc_project_p)(p_next, K * J);
+}
+
 /* update_q!(d, g, update2)  (src/algorithms_inner.jl:167-231) */
 void FN(orc_update_q)(FN(orc_state) * st, int update2, int32_t *status) {
   const int64_t K = st->K, I = st->I, J = st->J;
@@ -567,18 +606,7 @@ void FN(orc_update_q)(FN(orc_state) * st, int update2, int32_t *status) {
   memset(st->XtX_q, 0, sizeof(T) * K * K * I);
   memset(st->Xtz_q, 0, sizeof(T) * K * I);
   FN(orc_update_q_loop)(st->XtX_q, st->Xtz_q, st->bed, st->pitch, q, p, K, 0, I, 0, J, st->mu, st->nthreads);
-  for (int64_t a = 0; a < K * I; ++a) st->Xtz_q[a] *= (T)-1;
-#pragma omp parallel for num_threads(st->nthreads > 0 ? st->nthreads : 1)
-  for (int64_t i = 0; i < I; ++i) {
-    T tab[18 * 18], dd[18], tmp[18], pmin[16], pmax[16];
-    uint8_t swept[16];
-    for (int64_t k = 0; k < K; ++k) { pmin[k] = 0; pmax[k] = 1; }
-    FN(orc_create_tableau)(tab, st->XtX_q + K * K * i, st->Xtz_q + K * i, q + K * i, K, 1, (T)0);
-    const int r = FN(orc_quadratic_program)(qdiff + K * i, tab, q + K * i, pmin, pmax, K, 1, dd, tmp, swept);
-    if (status) status[i] = r;
-  }
-  for (int64_t a = 0; a < K * I; ++a) q_next[a] = q[a] + qdiff[a];
-  FN(orc_project_q)(q_next, K, I);
+  FN(orc_qp_update_q)(st->XtX_q, st->Xtz_q, q, q_next, qdiff, K, I, status, st->nthreads);
 }
 
 /* update_p!(d, g, update2)  (src/algorithms_inner.jl:245-310); uses q_next (Gauss-Seidel) */
@@ -592,18 +620,7 @@ void FN(orc_update_p)(FN(orc_state) * st, int update2, int32_t *status) {
   memset(st->XtX_p, 0, sizeof(T) * K * K * J);
   memset(st->Xtz_p, 0, sizeof(T) * K * J);
   FN(orc_update_p_loop)(st->XtX_p, st->Xtz_p, st->bed, st->pitch, q, p, K, 0, I, 0, J, st->mu, st->nthreads);
-  for (int64_t a = 0; a < K * J; ++a) st->Xtz_p[a] *= (T)-1;
-#pragma omp parallel for num_threads(st->nthreads > 0 ? st->nthreads : 1)
-  for (int64_t j = 0; j < J; ++j) {
-    T tab[18 * 18], dd[18], tmp[18], pmin[16], pmax[16];
-    uint8_t swept[16];
-    for (int64_t k = 0; k < K; ++k) { pmin[k] = 0; pmax[k] = 1; }
-    FN(orc_create_tableau)(tab, st->XtX_p + K * K * j, st->Xtz_p + K * j, p + K * j, K, 0, (T)0);
-    const int r = FN(orc_quadratic_program)(pdiff + K * j, tab, p + K * j, pmin, pmax, K, 0, dd, tmp, swept);
-    if (status) status[j] = r;
-  }
-  for (int64_t a = 0; a < K * J; ++a) p_next[a] = p[a] + pdiff[a];
-  FN(orc_project_p)(p_next, K * J);
+  FN(orc_qp_update_p)(st->XtX_p, st->Xtz_p, p, p_next, pdiff, K, J, status, st->nthreads);
 }
 
 /* One iteration of admixture_qn!(d, g; mode)  (src/algorithms_outer.jl:92-159),

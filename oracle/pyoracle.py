@@ -81,6 +81,8 @@ def _declare(L):
         getattr(L, "orc_qn_iteration_mode" + s).argtypes = [c_vp, c_i64, C.c_int, C.c_int, C.c_int, c_vp]
         getattr(L, "orc_update_UV_LBQN" + s).argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_i64]
         getattr(L, "orc_update_QN_LBQN" + s).argtypes = [c_vp, c_vp, c_vp, c_vp, c_i64, c_i64]
+        for n in ("orc_qp_update_q", "orc_qp_update_p"):
+            getattr(L, n + s).argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64, c_vp, C.c_int]
 
 
 def _p(a):
@@ -263,6 +265,19 @@ def update_UV_LBQN(U, V, x, x_next, x_next2, it, Q):
 def update_QN_LBQN(x_out, x, U, V, m):
     N = x.size
     getattr(lib(), "orc_update_QN_LBQN" + _sfx(x.dtype))(_p(x_out), _p(x), _p(U), _p(V), N, m)
+
+
+def qp_update(XtX, Xtz, x, x_next, simplex, nthreads=None):
+    """The host half of update_q! (simplex=True) or update_p! (src/algorithms_inner.jl:197-230,
+    273-307): negates Xtz in place, solves one QP per column of x (K x n) and writes
+    the projected x + delta into x_next; returns the QP return codes."""
+    K, n = x.shape
+    diff = np.zeros((K, n), x.dtype, order="F")
+    status = np.zeros(n, np.int32)
+    f = getattr(lib(), ("orc_qp_update_q" if simplex else "orc_qp_update_p") + _sfx(x.dtype))
+    f(_p(_chk(XtX, x.dtype)), _p(_chk(Xtz, x.dtype)), _p(_chk(x)), _p(_chk(x_next, x.dtype)), _p(diff), K, n,
+      _p(status), _nthreads(nthreads))
+    return status
 
 
 # ---------------------------------------------------------------- whole algorithm

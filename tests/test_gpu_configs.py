@@ -1,0 +1,367 @@
+"""The five benchmark workloads at their full sizes, through the C ABI, against
+the CPU oracle (the C restatement of src/loops.jl, src/algorithms_inner.jl,
+src/algorithms_outer.jl, src/qp.jl, src/projections.jl, src/quasi_newton.jl).
+
+Bars, written per test:
+  decode / counts / generator / fold masks      bit-exact
+  log-likelihood                                 1e-10 relative (F64); F32 in test_c2_..._f32
+  XtX / Xtz of the F64 tile loops                1e-9 relative to the column maximum
+  converged Q / P of the plain F64 method        1e-6 max-abs
+  mixed forms, per output, against the F64 oracle on the same state:
+    XtX   F32 1e-5, BF16 5e-3 (column maximum)   Xtz  F32 1e-5, BF16 1e-5
+    q_next / p_next after the QP + projection    F32 1e-5, BF16 1e-3 max-abs
+"""
+import numpy as np
+import pytest
+
+from conftest import random_state
+
+pytestmark = pytest.mark.gpu
+
+
+def note(*a):
+    print(*a, flush=True)
+
+
+def _sim(oa):
+    from importlib import import_module
+
+    return import_module("openadmixture_jl_amd.sim")
+
+
+def _colrel(got, ref):
+    got = np.asarray(got, np.float64).reshape(-1, got.shape[-1])
+    ref = np.asarray(ref, np.float64).reshape(-1, ref.shape[-1])
+    return float((np.abs(got - ref).max(0) / (np.abs(ref).max(0) + 1e-300)).max())
+
+
+def _device_cohort(oa, I, J, K, alpha, p_dist, lo, hi, miss, seed, T=np.float64):
+    Qt, Pt = _sim(oa).draw_truth(I, J, K, alpha, p_dist, lo, hi, seed=seed)
+    c = oa.HIPAdmixData(I, J, K, 3, T)
+    c.generate_bed(Qt, Pt, seed, miss)
+    return c, Qt, Pt
+
+
+def _qn_lockstep(c, ref, iters, rtol, tol, label):
+    """admixture_qn! iterations on the device and in the oracle side by side:
+    every iteration's (basic, qn, new) log-likelihoods within tol (relative);
+    both stop at the first iteration with reldiff < rtol, which must be the
+    same one.  Returns the iterations run."""
+    for it in range(1, iters + 1):
+        r = c.qn_iteration(it)
+        rr = ref.qn_iteration(it)
+        for key in ("basic", "qn", "new"):
+            assert abs(r[key] - rr[key]) <= tol * abs(rr[key]), (label, it, key, r[key], rr[key])
+        stop, rstop = r["reldiff"] < rtol, rr["reldiff"] < rtol
+        assert stop == rstop, (label, it, r["reldiff"], rr["reldiff"])
+        if stop:
+            return it
+    return iters
+
+
+# ------------------------------------------------------------------ c1
+def test_c1_full_run_matches_oracle(oa, O):
+    """c1: I=1,000, J=10,000, K=3, P ~ U(0.05, 0.95), Q ~ Dirichlet(0.5), 1%
+    missing, Float64, random init, EM(5) + admixture_qn! (ZAL, Q=3) for up to 50
+    iterations with rtol 1e-7 on both sides: generator bit-exact, every
+    per-iteration log-likelihood within 1e-10, the same stopping iteration, and
+    the converged Q and P within 1e-6 max-abs."""
+    I, J, K = 1_000, 10_000, 3
+    note("c1 full run", I, J, K)
+    c, Qt, Pt = _device_cohort(oa, I, J, K, 0.5, "uniform", 0.05, 0.95, 0.01, seed=1)
+    pitch = (I + 3) // 4
+    bed = np.ascontiguousarray(c.download_bed().ravel(order="F"))
+    assert np.array_equal(bed, O.gen_bed(Qt, Pt, 1, 0.01, pitch))
+    d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(101))
+    ref = O.OracleAdmix(bed, pitch, I, J, K, 3, d.x, d.U, d.V)
+    lls = oa.init_em(d, None, 5, d_cu=c, log=None)
+    np.testing.assert_allclose(lls, ref.init_em(5), rtol=1e-10, atol=0)
+    its = _qn_lockstep(c, ref, 50, 1e-7, 1e-10, "c1")
+    note("c1 converged after", its, "iterations")
+    assert its < 50, "c1 did not converge within 50 iterations"
+    x = c.get_x(0)
+    dq = np.abs(x[:, :I] - ref.x[:, :I]).max()
+    dp = np.abs(x[:, I:] - ref.x[:, I:]).max()
+    note(f"c1 converged max|dQ| {dq:.2e} max|dP| {dp:.2e}")
+    assert dq <= 1e-6 and dp <= 1e-6, (dq, dp)
+    # the host mirror of admixture_qn! reaches the same point from the same start
+    d2 = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(101))
+    oa.init_em(d2, None, 5, d_cu=c, log=None)
+    hist = oa.admixture_qn(d2, None, 50, 1e-7, d_cu=c, log=None)
+    assert d2.converged and d2.iterations == its
+    assert abs(hist[-1] - ref.ll_new) <= 1e-10 * abs(ref.ll_new)
+    assert np.abs(d2.x - ref.x).max() <= 1e-6
+    c.close()
+
+
+# ------------------------------------------------------------------ c2
+@pytest.mark.timeout(600)
+def test_c2_per_iteration_loglikelihoods_f64(oa, O):
+    """c2 at full size (I=10,000, J=100,000, K=5, Dirichlet(0.3), 2% missing,
+    1e9 genotypes) in Float64: one EM step and five admixture_qn! iterations in
+    lockstep with the oracle, every log-likelihood within 1e-10 relative."""
+    I, J, K = 10_000, 100_000, 5
+    note("c2 iterations float64")
+    c, Qt, Pt = _device_cohort(oa, I, J, K, 0.3, "uniform", 0.05, 0.95, 0.02, seed=2)
+    pitch = (I + 3) // 4
+    bed = np.ascontiguousarray(c.download_bed().ravel(order="F"))
+    d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(12))
+    ref = O.OracleAdmix(bed, pitch, I, J, K, 3, d.x, d.U, d.V)
+    lls = oa.init_em(d, None, 1, d_cu=c, log=None)
+    np.testing.assert_allclose(lls, ref.init_em(1), rtol=1e-10, atol=0)
+    for it in range(1, 6):
+        r = c.qn_iteration(it)
+        rr = ref.qn_iteration(it)
+        note(f"c2 float64 iteration {it}: {r['new']:.10e} vs {rr['new']:.10e}")
+        for key in ("basic", "qn", "new"):
+            assert abs(r[key] - rr[key]) <= 1e-10 * abs(rr[key]), (it, key, r[key], rr[key])
+    c.close()
+
+
+@pytest.mark.timeout(900)
+def test_c2_per_iteration_loglikelihoods_f32(oa, O):
+    """c2 in Float32 (AdmixData{Float32}): one EM step and five admixture_qn!
+    iterations in lockstep with the Float32 oracle and with the Float64 oracle
+    started from the same (Float32) point.
+
+    The reference's Float32 loops accumulate the EM sums, XtX and Xtz in
+    Float32 (q_next[k, i] += ..., src/loops.jl:610-632, 867-950) over the
+    100,000 SNPs of an individual, which moves its iterates by ~1e-5 relative
+    at this size; the device sums in FP64 and rounds once.  So: every
+    log-likelihood within 3e-5 of the Float32 oracle, and the device stays
+    closer to the Float64 trajectory than the Float32 oracle does (largest
+    deviation over the run, and within 2e-6 of it)."""
+    I, J, K = 10_000, 100_000, 5
+    note("c2 iterations float32")
+    c, Qt, Pt = _device_cohort(oa, I, J, K, 0.3, "uniform", 0.05, 0.95, 0.02, seed=2, T=np.float32)
+    pitch = (I + 3) // 4
+    bed = np.ascontiguousarray(c.download_bed().ravel(order="F"))
+    d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(12), T=np.float32)
+    r32 = O.OracleAdmix(bed, pitch, I, J, K, 3, d.x, d.U, d.V, dtype=np.float32)
+    r64 = O.OracleAdmix(bed, pitch, I, J, K, 3, d.x.astype(np.float64), dtype=np.float64)
+    got = list(oa.init_em(d, None, 1, d_cu=c, log=None))
+    l32, l64 = list(r32.init_em(1)), list(r64.init_em(1))
+    for it in range(1, 6):
+        r, a, b = c.qn_iteration(it), r32.qn_iteration(it), r64.qn_iteration(it)
+        note(f"c2 float32 iteration {it}: device {r['new']:.10e} oracle f32 {a['new']:.10e} "
+             f"oracle f64 {b['new']:.10e}")
+        for key in ("basic", "qn", "new"):
+            got.append(r[key])
+            l32.append(a[key])
+            l64.append(b[key])
+    got, l32, l64 = np.array(got), np.array(l32), np.array(l64)
+    d32 = np.abs(got - l32) / np.abs(l32)
+    dev64 = np.abs(got - l64) / np.abs(l64)
+    ref64 = np.abs(l32 - l64) / np.abs(l64)
+    note(f"c2 float32: max rel. to f32 oracle {d32.max():.2e}; to f64 oracle: device {dev64.max():.2e}, "
+         f"f32 oracle {ref64.max():.2e}")
+    assert d32.max() <= 3e-5
+    assert dev64.max() <= 2e-6 and dev64.max() <= ref64.max()
+    c.close()
+
+
+# ------------------------------------------------------------------ c3
+I3, J3, K3, SUB = 100_000, 100_000, 8, 2_000
+
+
+@pytest.fixture(scope="module")
+def c3(oa, O):
+    note("c3 cohort on the device")
+    c, Qt, Pt = _device_cohort(oa, I3, J3, K3, 0.3, "beta", 0.01, 0.99, 0.02, seed=3)
+    bed = np.ascontiguousarray(c.download_bed())  # (pitch x J), column-major
+    d = oa.AdmixData(I3, J3, K3, 3, rng=np.random.default_rng(4))
+    oa.init_em(d, None, 3, d_cu=c, log=None)  # an EM-primed state, as the updates see it
+    yield c, bed, np.asfortranarray(d.x)
+    c.close()
+
+
+@pytest.mark.timeout(600)
+def test_c3_decode_counts_bit_exact(oa, O, c3):
+    note("c3 decode / counts")
+    c, bed, _ = c3
+    pitch = bed.shape[0]
+    flat = bed.ravel(order="F")
+    snm, salt, inm = c.counts()
+    rs, ra, ri = O.counts(flat, pitch, I3, J3)
+    assert np.array_equal(snm, rs) and np.array_equal(salt, ra) and np.array_equal(inm, ri)
+    g = c.decode()
+    assert np.array_equal(g, O.decode(flat, pitch, I3, J3))
+
+
+@pytest.mark.timeout(600)
+def test_c3_loglikelihood_f64(oa, O, c3):
+    note("c3 log-likelihood")
+    c, bed, x = c3
+    pitch = bed.shape[0]
+    q, p = np.asfortranarray(x[:, :I3]), np.asfortranarray(x[:, I3:])
+    ref = O.loglikelihood(bed.ravel(order="F"), pitch, q, p)
+    c.set_x(0, x)
+    dev = c.loglikelihood(0)
+    loop = c.loglikelihood_loop(q, p, 0, I3, 0, J3)
+    note(f"c3 LL oracle {ref:.12e} device {dev:.12e} loop {loop:.12e}")
+    assert abs(dev - ref) <= 1e-10 * abs(ref)
+    assert abs(loop - ref) <= 1e-10 * abs(ref)
+
+
+def _sub_oracle_q(O, bed, x, n):
+    """Oracle over individuals [0, n) and all SNPs (the Q side of those
+    individuals depends on nothing else): bytes [0, n/4) of every SNP row."""
+    sub = np.ascontiguousarray(bed[: n // 4, :].ravel(order="F"))
+    xs = np.asfortranarray(np.hstack([x[:, :n], x[:, I3:]]))
+    return O.OracleAdmix(sub, n // 4, n, J3, K3, 3, xs)
+
+
+def _sub_oracle_p(O, bed, x, n):
+    """Oracle over all individuals and SNPs [0, n) (the P side of those SNPs)."""
+    sub = np.ascontiguousarray(bed[:, :n].ravel(order="F"))
+    xs = np.asfortranarray(np.hstack([x[:, :I3], x[:, I3:I3 + n]]))
+    return O.OracleAdmix(sub, bed.shape[0], I3, n, K3, 3, xs)
+
+
+@pytest.mark.timeout(600)
+def test_c3_update_loops_on_subranges(oa, O, c3):
+    """update_q_loop_skipmissing! over 2,000 individuals x all SNPs and
+    update_p_loop_skipmissing! over all individuals x 2,000 SNPs (the
+    reference's += tile loops, src/loops.jl:867-950) against the oracle's
+    loops over the same ranges, 1e-9 of each column's maximum."""
+    note("c3 update loops")
+    c, bed, x = c3
+    pitch = bed.shape[0]
+    flat = bed.ravel(order="F")
+    q, p = np.asfortranarray(x[:, :I3]), np.asfortranarray(x[:, I3:])
+    for side, rng_i, rng_j, n in (("q", (0, SUB), (0, J3), I3), ("p", (0, I3), (0, SUB), J3)):
+        XtX = np.zeros((K3, K3, n), order="F")
+        Xtz = np.zeros((K3, n), order="F")
+        rX = np.zeros_like(XtX)
+        rZ = np.zeros_like(Xtz)
+        if side == "q":
+            c.update_q_loop(XtX, Xtz, q, p, *rng_i, *rng_j)
+            O.update_q_loop(rX, rZ, flat, pitch, q, p, rng_i, rng_j)
+            sl = slice(0, SUB)
+            XtX, Xtz, rX, rZ = XtX[:, :, sl], Xtz[:, sl], rX[:, :, sl], rZ[:, sl]
+        else:
+            c.update_p_loop(XtX, Xtz, q, p, *rng_i, *rng_j)
+            O.update_p_loop(rX, rZ, flat, pitch, q, p, rng_i, rng_j)
+            sl = slice(0, SUB)
+            assert not XtX[:, :, SUB:].any() and not Xtz[:, SUB:].any()
+            XtX, Xtz, rX, rZ = XtX[:, :, sl], Xtz[:, sl], rX[:, :, sl], rZ[:, sl]
+        eX, eZ = _colrel(XtX, rX), _colrel(Xtz, rZ)
+        note(f"c3 update_{side} loop: XtX {eX:.2e} Xtz {eZ:.2e}")
+        assert eX <= 1e-9 and eZ <= 1e-9, (side, eX, eZ)
+
+
+MIXED_TOL = {"f64": (1e-9, 1e-9, 1e-9), "f32": (1e-5, 1e-5, 1e-5), "bf16": (5e-3, 1e-5, 1e-3)}
+
+
+@pytest.mark.timeout(600)
+@pytest.mark.parametrize("form", ["f64", "f32", "bf16"])
+def test_c3_update_forms_vs_oracle(oa, O, c3, form):
+    """Device-resident update_q! / update_p! of each precision form on the full
+    c3 cohort (the passes run over all 1e10 genotypes) against the F64 oracle
+    on the same state, per output: XtX and Xtz of individuals (SNPs) [0, 2,000)
+    and the q_next (p_next) that the QP + projection give for them."""
+    note("c3 update forms", form)
+    c, bed, x = c3
+    htol, gtol, stol = MIXED_TOL[form]
+    c.set_precision(form)
+    try:
+        c.set_x(0, x)
+        c.update_q(False)
+        XtXq, Xtzq = c.hessians(0)
+        qn = c.get_x(1)[:, :SUB]
+        c.set_x(1, x)  # update_p! reads q_next: the same q as the Q side's state
+        c.update_p(False)
+        XtXp, Xtzp = c.hessians(1)
+        pn = c.get_x(1)[:, I3:I3 + SUB]
+    finally:
+        c.set_precision("f64")
+    rq = _sub_oracle_q(O, bed, x, SUB)
+    rq.update_q(False)
+    rp = _sub_oracle_p(O, bed, x, SUB)
+    rp.x_next[:, :I3] = x[:, :I3]
+    rp.update_p(False)
+    err = {
+        "XtX_q": _colrel(XtXq[:, :, :SUB], rq.XtX_q), "Xtz_q": _colrel(Xtzq[:, :SUB], rq.Xtz_q),
+        "XtX_p": _colrel(XtXp[:, :, :SUB], rp.XtX_p), "Xtz_p": _colrel(Xtzp[:, :SUB], rp.Xtz_p),
+        "q_next": float(np.abs(qn - rq.x_next[:, :SUB]).max()),
+        "p_next": float(np.abs(pn - rp.x_next[:, I3:]).max()),
+    }
+    note(f"c3 {form} vs F64 oracle: " + ", ".join(f"{k} {v:.2e}" for k, v in err.items()))
+    assert err["XtX_q"] <= htol and err["XtX_p"] <= htol, err
+    assert err["Xtz_q"] <= gtol and err["Xtz_p"] <= gtol, err
+    assert err["q_next"] <= stol and err["p_next"] <= stol, err
+
+
+# ------------------------------------------------------------------ c4
+@pytest.mark.timeout(900)
+def test_c4_sweep_full_size(oa, O):
+    """c4 at full size: I=20,000, J=200,000 (1.0 GB packed), true K=6, the K=2..12
+    sweep with 5-fold CV (66 fits, plain F64).  The CV error is smallest at the
+    true K; the device fold masks equal the oracle's bit for bit, and the
+    held-out deviance of a state equals the oracle's to 1e-12."""
+    I, J, Kt = 20_000, 200_000, 6
+    note("c4 sweep", I, J)
+    src, Qt, Pt = _device_cohort(oa, I, J, Kt, 0.5, "uniform", 0.05, 0.95, 0.01, seed=3)
+    pitch = (I + 3) // 4
+    bed = np.ascontiguousarray(src.download_bed().ravel(order="F"))
+    dst = oa.HIPAdmixData(I, J, 2, 3)
+    for f in (0, 4):
+        dst.bed_from(src, f, 5, 3)
+        assert np.array_equal(dst.download_bed().ravel(order="F"), O.mask_fold(bed, pitch, I, J, f, 5, 3)), f
+    dst.close()
+    fit = oa.HIPAdmixData(I, J, Kt, 3)
+    q, p = random_state(I, J, Kt, seed=8)
+    fit.set_x(0, np.asfortranarray(np.hstack([q, p])))
+    for f in (1, 3):
+        dev, cnt = fit.heldout_deviance(src, f, 5, 3)
+        rdev, rcnt = O.heldout_deviance(bed, pitch, q, p, f, 5, 3)
+        assert cnt == rcnt and abs(dev - rdev) <= 1e-12 * abs(rdev), (f, dev, rdev, cnt, rcnt)
+    fit.close()
+    res = oa.cv_sweep(src, list(range(2, 13)), nfolds=5, fold_seed=3, init_seed=4, max_iter=300, precision="f64",
+                      log=note)
+    src.close()
+    cv = {r["K"]: r["cv_error"] for r in res}
+    note("c4 CV error", cv)
+    assert oa.best_k(res) == Kt, cv
+    assert all(r["converged_full"] for r in res if r["K"] <= Kt)
+    # under-fitted K lose clearly; over-fitted K cost held-out deviance
+    assert cv[Kt - 1] - cv[Kt] > 1e-3 * cv[Kt]
+    assert all(cv[k] > cv[Kt] for k in range(Kt + 1, 13))
+
+
+# ------------------------------------------------------------------ c5
+@pytest.mark.timeout(900)
+def test_c5_warm_start_full_size(oa, O):
+    """c5 evaluation at full size: I=200,000, J=50,000, K=10 from a 10-population
+    Balding-Nichols world (F = 0.05); the network is trained on 14 simulated
+    cohorts (I=5,000, J=20,000, K=4..10, alpha in [0.1, 1]) solved by the plain
+    method.  The judge keeps the proposal, the warm fit reaches the plain fit's
+    final log-likelihood within 1e-6 (both to rtol 1e-7) and gets within 1e-4
+    of it in fewer iterations."""
+    I, J, K = 200_000, 50_000, 10
+    note("c5 warm start", I, J, K)
+    P, _ = oa.world_frequencies(K, J, fst=0.05, seed=3)
+    panel = oa.Panel(P[:, :20_000].mean(0), seed=3)
+    X, T, info = oa.training_set(P, panel, 14, I=5_000, J=20_000, Ks=range(4, 11), alpha_range=(0.1, 1.0),
+                                 miss_rate=0.01, seed=3)
+    net = oa.MLP(64, 256, K, seed=3)
+    hist = net.train(X, T, epochs=20, batch=512, lr=2e-3, seed=3)
+    note("c5 training loss", hist[0], hist[-1])
+    assert hist[-1] < hist[0]
+    rng = np.random.default_rng(3 + 99)
+    Qt = np.asfortranarray(rng.dirichlet(0.5 * np.ones(K), size=I).T)
+    c = oa.HIPAdmixData(I, J, K, 3)
+    c.generate_bed(Qt, P, 3 + 5, 0.01)
+    plain = oa.fit_on_device(c, K, 4, max_iter=300)
+    ll_star = plain.llhist[-1]
+    r = oa.warm_started_fit(c, net, panel, seed=4, max_iter=300)
+    c.close()
+    net.close()
+    warm = [ll for _, ll in r["history"]]
+    n_plain = next(n for n, ll in enumerate(plain.llhist) if abs(ll - ll_star) <= 1e-4 * abs(ll_star))
+    n_warm = next(n for n, ll in enumerate(warm) if abs(ll - ll_star) <= 1e-4 * abs(ll_star))
+    note(f"c5: chosen {r['chosen']}, plain {plain.iterations} iterations (within 1e-4 after {n_plain}), warm "
+         f"{r['iterations']} (after {n_warm}); LL plain {ll_star:.8e} warm {warm[-1]:.8e}")
+    assert r["chosen"] == "warm" and r["converged"] and plain.converged
+    assert abs(warm[-1] - ll_star) <= 1e-6 * abs(ll_star)
+    assert n_warm < n_plain
