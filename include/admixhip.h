@@ -276,6 +276,25 @@ int admx_update_p(admx_ctx *ctx, int update2, int32_t *status);
 int admx_qn_iteration(admx_ctx *ctx, int64_t iter, int fix_q, int fix_p, double *out);
 int admx_synchronize(admx_ctx *ctx);
 
+/* ---- allocation-free log-likelihood.  No counterpart in the reference: for
+ * callers that cannot allocate device memory (code under HIP graph capture,
+ * tight serving loops, harnesses that forbid hipMalloc).  The log-likelihood
+ * of admx_loglikelihood / admx_loglikelihood_loop over all I x J genotypes of
+ * a caller-held device BED (SNP-major, `pitch` bytes per SNP: pitch >=
+ * ceil(I/4), a multiple of 16, pitch * J + 32 bytes readable) and device
+ * FP64 q (K x I) and p (K x J); the double goes to the device pointer `out`.
+ * Enqueued on `stream` (a hipStream_t; NULL = the null stream) without a
+ * synchronisation.  It calls no hipMalloc / hipFree: the pass's partial sums
+ * and state-check flag live in a fixed device array of
+ * (ADMX_NOALLOC_LL_PARTS + 32) doubles that each GPU holds until the process
+ * ends; cohorts with ceil(I/256) x (SNP chunks) > ADMX_NOALLOC_LL_PARTS are
+ * refused (ADMX_EINVAL; below 16.7 million individuals every J fits).  Calls
+ * that overlap on one GPU share that array and race: serialise them.  Same
+ * kernels and chunking as a fresh context, so the same result bit for bit. */
+#define ADMX_NOALLOC_LL_PARTS 65536
+int admx_loglikelihood_noalloc(const uint8_t *bed, int64_t pitch, int64_t I, int64_t J, int64_t K, const double *q,
+                               const double *p, double *out, void *stream);
+
 /* ---- measurement: HIP-event timing of every kernel launch on the context's
  * stream while enabled.  Report lines: "<kernel> <launches> <total_ms>\n". */
 int admx_profile_enable(admx_ctx *ctx, int on);

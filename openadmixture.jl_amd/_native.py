@@ -24,6 +24,7 @@ ADMX_TUNE_RESIDENT_SNPS = 6
 ADMX_QN_ZAL, ADMX_QN_LBQN = 0, 1
 ADMX_PREC_F64, ADMX_PREC_F32, ADMX_PREC_BF16 = 0, 1, 2
 ADMX_SKFR_GLOBAL, ADMX_SKFR_LOCAL = 0, 1
+ADMX_NOALLOC_LL_PARTS = 65536  # doubles of admx_loglikelihood_noalloc's fixed scratch
 
 # Every exported symbol with its C signature (return type is int unless noted).
 _i64, _int, _dbl, _vp, _u64 = C.c_int64, C.c_int, C.c_double, C.c_void_p, C.c_uint64
@@ -82,6 +83,7 @@ SIGNATURES = {
     "admx_update_p": ([_vp, _int, _vp], _int),
     "admx_qn_iteration": ([_vp, _i64, _int, _int, _vp], _int),
     "admx_synchronize": ([_vp], _int),
+    "admx_loglikelihood_noalloc": ([_vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp], _int),
     "admx_profile_enable": ([_vp, _int], _int),
     "admx_profile_reset": ([_vp], _int),
     "admx_profile_report": ([_vp, C.c_char_p, C.c_size_t], _int),

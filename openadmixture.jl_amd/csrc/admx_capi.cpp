@@ -1044,6 +1044,8 @@ int create_impl(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int 
   }
   hipError_t e = hipGetDevice(&c->device);
   if (e != hipSuccess) return cleanup(fail(ADMX_EHIP, std::string("hipGetDevice: ") + hipGetErrorString(e)));
+  // set-up of the allocation-free entry point's scratch on this device
+  if (int rc = noalloc_setup(c->device, nullptr); rc != ADMX_OK) return cleanup(rc);
   if ((e = hipStreamCreateWithFlags(&c->s, hipStreamNonBlocking)) != hipSuccess)
     return cleanup(fail(ADMX_EHIP, std::string("hipStreamCreate: ") + hipGetErrorString(e)));
   const size_t xb = (size_t)c->N() * c->esz;

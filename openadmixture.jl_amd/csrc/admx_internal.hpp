@@ -167,6 +167,10 @@ void launch_widen(const float* x, int64_t n, double* y, hipStream_t s);
 void launch_repitch(const uint8_t* raw, int64_t ibytes, int64_t real, int64_t rows, uint8_t* dst, int64_t pitch,
                     hipStream_t s);
 
+// Device address of the allocation-free log-likelihood's fixed scratch on
+// `device` (admx_noalloc.hip), resolved once per device; *out may be null.
+int noalloc_setup(int device, double** out);
+
 // Slots of the device log-likelihood vector.
 enum LLSlot : int { LL_PREV = 0, LL_NEW = 1, LL_BASIC = 2, LL_QN = 3, LL_TMP = 4, LL_NSLOTS = 8 };
 
