@@ -252,9 +252,32 @@ int admx_set_uv(admx_ctx *ctx, const void *U, const void *V);
 int admx_get_uv(admx_ctx *ctx, void *U, void *V);
 int admx_set_ll(admx_ctx *ctx, double ll_prev, double ll_new);
 int admx_get_ll(admx_ctx *ctx, double *ll_prev, double *ll_new);
-/* XtX/Xtz of the last update on side ADMX_SIDE_Q / ADMX_SIDE_P (Xtz negated,
- * as the reference leaves it). */
+/* XtX/Xtz of the last update on side ADMX_SIDE_Q / ADMX_SIDE_P: after
+ * admx_update_q / admx_update_p Xtz is negated, as update_q!/update_p! leave it
+ * (src/algorithms_inner.jl:199, 275); after admx_grad_hessian it is the raw
+ * gradient sum, as update_q_cuda!/update_p_cuda! leave it.  Either pointer may
+ * be NULL (that array is not copied). */
 int admx_get_hessians(admx_ctx *ctx, int side, void *XtX, void *Xtz);
+
+/* ---- per-array device mirror of CuAdmixData (src/cuda/structs.jl:1-31), for
+ * a backend that plugs into the reference's own GPU branch: its device arrays
+ * q, p, q_next, p_next, p_tmp are blocks of the state vectors (q of x = d_cu.q,
+ * p of x = d_cu.p, q/p of x_next = d_cu.q_next/p_next, p of x_tmp = d_cu.p_tmp). */
+/* copyto!(d_cu.q or d_cu.p, host) of copyto_sync! (src/cuda/transfer.jl:1-7):
+ * the K x I (side Q) or K x J (side P) block of state vector `which` from a
+ * column-major host array. */
+int admx_set_block(admx_ctx *ctx, int which, int side, const void *host);
+/* copyto!(host, d_cu.q or d_cu.p): the block to a host array. */
+int admx_get_block(admx_ctx *ctx, int which, int side, void *host);
+/* d_cu.p .= d_cu.p_next, d_cu.q .= d_cu.q_next (src/algorithms_outer.jl:28-29):
+ * block `side` of state vector dst <- the same block of src, on the device. */
+int admx_copy_block(admx_ctx *ctx, int dst, int src, int side);
+/* update_q_cuda!(d_cu, g_cu) / update_p_cuda!(d_cu, g_cu) (src/cuda/runners.jl:69-111,
+ * called at src/algorithms_inner.jl:193, 269): XtX and Xtz of state x's (q, p) on
+ * side Q (K x K x I, K x I) or P (K x K x J, K x J) into the context's Hessian
+ * arrays, zeroed then written, Xtz not negated, no QP (FP64 passes); read them
+ * with admx_get_hessians. */
+int admx_grad_hessian(admx_ctx *ctx, int side);
 
 /* ---- algorithms on the device-resident state */
 /* loglikelihood(g, q, p, ...) of state vector `which` (src/algorithms_inner.jl:27-35) */

@@ -83,6 +83,10 @@ SIGNATURES = {
     "admx_update_p": ([_vp, _int, _vp], _int),
     "admx_qn_iteration": ([_vp, _i64, _int, _int, _vp], _int),
     "admx_synchronize": ([_vp], _int),
+    "admx_set_block": ([_vp, _int, _int, _vp], _int),
+    "admx_get_block": ([_vp, _int, _int, _vp], _int),
+    "admx_copy_block": ([_vp, _int, _int, _int], _int),
+    "admx_grad_hessian": ([_vp, _int], _int),
     "admx_loglikelihood_noalloc": ([_vp, _i64, _i64, _i64, _i64, _vp, _vp, _vp, _vp], _int),
     "admx_profile_enable": ([_vp, _int], _int),
     "admx_profile_reset": ([_vp], _int),

@@ -590,6 +590,27 @@ int dev_update_p(admx_ctx* c, bool update2, int32_t* status_host) {
   return ADMX_OK;
 }
 
+// update_q_cuda!(d_cu, g_cu) / update_p_cuda!(d_cu, g_cu) (src/cuda/runners.jl:69-111):
+// XtX and Xtz of the state x's (q, p) into the context's Hessian arrays, zeroed
+// then written (the reference fills them with zeros and accumulates), Xtz not
+// negated; no QP.  The FP64 passes and finalize of the tile loops.
+template <typename T>
+int dev_grad_hessian(admx_ctx* c, bool qside) {
+  void* X = c->x[ADMX_X];
+  int nchunk = 0;
+  int64_t np = 0;
+  if (qside) CHK(pass_q<T>(c, MODE_UPDATE, X, c->pp(X), 0, c->I, 0, c->J, &nchunk, &np));
+  else CHK(pass_p<T>(c, MODE_UPDATE, X, c->pp(X), 0, c->I, 0, c->J, &nchunk));
+  FinArgs f{c->part.p, qside ? c->I : c->J, nchunk, 0, c->J, qside ? X : c->pp(X), nullptr, nullptr,
+            qside ? c->XtX_q : c->XtX_p, qside ? c->Xtz_q : c->Xtz_p, 0, 0, nullptr};
+  {
+    ProfScope ps(c, qside ? "fin_q_update" : "fin_p_update");
+    if (qside) launch_fin_q_update<T>((int)c->K, f, false, c->s);
+    else launch_fin_p_update<T>((int)c->K, f, false, c->s);
+  }
+  return check_launch();
+}
+
 template <typename T>
 int dev_em_q(admx_ctx* c) {
   void* X = c->x[ADMX_X];
@@ -1583,12 +1604,74 @@ int admx_get_hessians(admx_ctx* c, int side, void* XtX, void* Xtz) {
   return guarded([&]() -> int {
     CHK(check_ctx(c));
     Guard g(c);
-    if (!XtX || !Xtz) return bad("null pointer");
+    if (!XtX && !Xtz) return bad("null pointer");
+    if (side != ADMX_SIDE_Q && side != ADMX_SIDE_P) return bad("side must be ADMX_SIDE_Q or ADMX_SIDE_P");
     const int64_t n = side == ADMX_SIDE_Q ? c->I : c->J;
-    HIPCHK(hipMemcpyAsync(XtX, side == ADMX_SIDE_Q ? c->XtX_q : c->XtX_p, (size_t)(c->K * c->K * n) * c->esz,
-                          hipMemcpyDeviceToHost, c->s));
-    HIPCHK(hipMemcpyAsync(Xtz, side == ADMX_SIDE_Q ? c->Xtz_q : c->Xtz_p, (size_t)(c->K * n) * c->esz,
-                          hipMemcpyDeviceToHost, c->s));
+    if (XtX)
+      HIPCHK(hipMemcpyAsync(XtX, side == ADMX_SIDE_Q ? c->XtX_q : c->XtX_p, (size_t)(c->K * c->K * n) * c->esz,
+                            hipMemcpyDeviceToHost, c->s));
+    if (Xtz)
+      HIPCHK(hipMemcpyAsync(Xtz, side == ADMX_SIDE_Q ? c->Xtz_q : c->Xtz_p, (size_t)(c->K * n) * c->esz,
+                            hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
+}
+
+int admx_set_block(admx_ctx* c, int which, int side, const void* host) {
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (which < 0 || which > 3 || !host) return bad("bad state vector");
+    if (side != ADMX_SIDE_Q && side != ADMX_SIDE_P) return bad("side must be ADMX_SIDE_Q or ADMX_SIDE_P");
+    void* X = c->x[which];
+    void* dst = side == ADMX_SIDE_Q ? X : c->pp(X);
+    const size_t b = (size_t)(c->K * (side == ADMX_SIDE_Q ? c->I : c->J)) * c->esz;
+    HIPCHK(hipMemcpyAsync(dst, host, b, hipMemcpyHostToDevice, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
+}
+
+int admx_get_block(admx_ctx* c, int which, int side, void* host) {
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (which < 0 || which > 3 || !host) return bad("bad state vector");
+    if (side != ADMX_SIDE_Q && side != ADMX_SIDE_P) return bad("side must be ADMX_SIDE_Q or ADMX_SIDE_P");
+    void* X = c->x[which];
+    const void* src = side == ADMX_SIDE_Q ? X : c->pp(X);
+    const size_t b = (size_t)(c->K * (side == ADMX_SIDE_Q ? c->I : c->J)) * c->esz;
+    HIPCHK(hipMemcpyAsync(host, src, b, hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
+}
+
+int admx_copy_block(admx_ctx* c, int dst, int src, int side) {
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    if (dst < 0 || dst > 3 || src < 0 || src > 3) return bad("bad state vector");
+    if (side != ADMX_SIDE_Q && side != ADMX_SIDE_P) return bad("side must be ADMX_SIDE_Q or ADMX_SIDE_P");
+    if (dst != src) {
+      char* d = static_cast<char*>(side == ADMX_SIDE_Q ? c->x[dst] : c->pp(c->x[dst]));
+      const char* s = static_cast<const char*>(side == ADMX_SIDE_Q ? c->x[src] : c->pp(c->x[src]));
+      const size_t b = (size_t)(c->K * (side == ADMX_SIDE_Q ? c->I : c->J)) * c->esz;
+      HIPCHK(hipMemcpyAsync(d, s, b, hipMemcpyDeviceToDevice, c->s));
+    }
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
+}
+
+int admx_grad_hessian(admx_ctx* c, int side) {
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    if (side != ADMX_SIDE_Q && side != ADMX_SIDE_P) return bad("side must be ADMX_SIDE_Q or ADMX_SIDE_P");
+    CHK(BY_DTYPE(c, dev_grad_hessian, c, side == ADMX_SIDE_Q));
     HIPCHK(hipStreamSynchronize(c->s));
     return ADMX_OK;
   });

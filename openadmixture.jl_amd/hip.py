@@ -191,6 +191,33 @@ class HIPAdmixData:
         N.check(N.lib().admx_get_x(self._h, which, N.ptr(out)))
         return out
 
+    # ------------------------------------------------------------ per-array mirror of CuAdmixData
+    def _block_shape(self, side):
+        return (self.K, self.I if side == N.ADMX_SIDE_Q else self.J)
+
+    def set_block(self, which: int, side: int, a):
+        """copyto!(d_cu.q / d_cu.p, a): the q (side Q) or p (side P) block of
+        state vector `which` from a host array."""
+        a = _f(a, self.T, self._block_shape(side))
+        N.check(N.lib().admx_set_block(self._h, int(which), int(side), N.ptr(a)))
+
+    def get_block(self, which: int, side: int, out=None):
+        """copyto!(host, d_cu.q / d_cu.p)."""
+        if out is None:
+            out = np.empty(self._block_shape(side), self.T, order="F")
+        _f(out, self.T, self._block_shape(side))
+        N.check(N.lib().admx_get_block(self._h, int(which), int(side), N.ptr(out)))
+        return out
+
+    def copy_block(self, dst: int, src: int, side: int):
+        """d_cu.p .= d_cu.p_next (side P) / d_cu.q .= d_cu.q_next (side Q), on the device."""
+        N.check(N.lib().admx_copy_block(self._h, int(dst), int(src), int(side)))
+
+    def grad_hessian(self, side: int):
+        """update_q_cuda! / update_p_cuda!: XtX, Xtz of state x's (q, p) on `side`
+        into the context (read them with hessians(side); Xtz not negated)."""
+        N.check(N.lib().admx_grad_hessian(self._h, int(side)))
+
     def set_uv(self, U, V):
         shp = (self.K * (self.I + self.J), self.Q)
         N.check(N.lib().admx_set_uv(self._h, N.ptr(_f(U, self.T, shp)), N.ptr(_f(V, self.T, shp))))
