@@ -125,7 +125,18 @@ def host_cores() -> int:
         return os.cpu_count() or 1
 
 
-def cpu_baseline(wl, dtype, seed, budget_s=20.0):
+def cpu_model() -> str:
+    """The host CPU's model name (first 'model name' of /proc/cpuinfo)."""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline(wl, dtype, seed, budget_s=20.0, full_snps=False, ctx=None):
     """The CPU oracle (C/OpenMP restatement of the reference loops) on a bounded
     sample of the same workload: all I individuals, the first Js SNPs (same
     truth, same generator), one EM priming step, then admixture_qn!
@@ -140,11 +151,15 @@ def cpu_baseline(wl, dtype, seed, budget_s=20.0):
 
     sim = import_module("openadmixture_jl_amd.sim")
     I, K = wl["I"], wl["K"]
-    Js = max(64, min(wl["J"], int(2e8 // I)))
+    Js = wl["J"] if full_snps else max(64, min(wl["J"], int(2e8 // I)))
     Qt, Pt = sim.draw_truth(I, wl["J"], K, wl["alpha"], wl["p_dist"], wl["p_lo"], wl["p_hi"], seed=seed)
     Pt = np.asfortranarray(Pt[:, :Js])
     pitch = (I + 3) // 4
-    bed = O.gen_bed(Qt, Pt, seed, wl["miss_rate"], pitch)
+    if Js == wl["J"] and ctx is not None:
+        # the whole cohort: the device generated the same bytes (the generators agree bit for bit)
+        bed = np.ascontiguousarray(ctx.download_bed().ravel(order="F"))
+    else:
+        bed = O.gen_bed(Qt, Pt, seed, wl["miss_rate"], pitch)
     T = np.float64 if dtype == "f64" else np.float32
     rng = np.random.default_rng(seed + 1)
     x0 = rng.random((K, I + Js))
@@ -161,15 +176,18 @@ def cpu_baseline(wl, dtype, seed, budget_s=20.0):
         st.qn_iteration(its + 1)
         its += 1
         el = time.perf_counter() - t0
-        if el > budget_s or its >= 50 or el * (its + 1) / its > 1.5 * budget_s:
+        print(f"cpu baseline: {its} iterations over {Js} SNPs in {el:.1f} s", file=sys.stderr, flush=True)
+        if its >= 2 and (el > budget_s or its >= 50 or el * (its + 1) / its > 1.5 * budget_s):
             break
     return {
         "value": I * Js * its / el,
         "unit": "genotype*iter/s",
         "cores": cores,
         "kind": "port",
-        "sample": f"oracle (C/OpenMP restatement of src/loops.jl etc.), I={I}, first J={Js} SNPs of the same "
-                  f"cohort, K={K}, {dtype}, {its} admixture_qn! iterations after 1 EM step, {el:.1f} s",
+        "sample": f"oracle (C/OpenMP restatement of src/loops.jl etc.) on {cores} threads of {cpu_model()}, "
+                  f"I={I}, {'all' if Js == wl['J'] else 'first'} J={Js} SNPs of the same cohort, K={K}, {dtype}, "
+                  f"{its} full admixture_qn! iterations after 1 EM step, {el:.1f} s",
+        "cpu_model": cpu_model(),
         "em_step_seconds_full": em_s * wl["J"] / Js,
         "iteration_seconds_full": el / its * wl["J"] / Js,
     }
@@ -229,7 +247,8 @@ def run_sweep(args):
     t0 = time.perf_counter()
     res = oa.cv_sweep(src, Ks, nfolds=args.folds, fold_seed=args.seed, init_seed=args.seed + 1,
                       max_iter=args.sweep_max_iter, precision="f64" if args.form == "plain" else args.form,
-                      fold_start=args.fold_start, log=lambda m: print(m, file=sys.stderr, flush=True))
+                      fold_start=args.fold_start, log=lambda m: print(m, file=sys.stderr, flush=True),
+                      workers=args.sweep_workers)
     el = time.perf_counter() - t0
     src.close()
     out = {
@@ -239,7 +258,7 @@ def run_sweep(args):
         "data": "synthetic cohort generated on device (Dirichlet Q, Uniform P, Binomial(2, q.p) genotypes)",
         "config": {"workload": f"{args.workload}: I={I}, J={J}, true K={Kt}, K={Ks[0]}..{Ks[-1]}, {args.folds}-fold CV",
                    "I": I, "J": J, "K_true": Kt, "folds": args.folds, "max_iter": args.sweep_max_iter,
-                   "fold_start": args.fold_start},
+                   "fold_start": args.fold_start, "workers": args.sweep_workers},
         "best_k": oa.best_k(res),
         "per_k": [{k: r[k] for k in ("K", "seconds", "cv_error", "ll_full", "iterations_full", "converged_full",
                                      "iterations_folds", "waves")} for r in res],
@@ -427,6 +446,9 @@ def main():
     ap.add_argument("--seed", type=int, default=3)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--cpu-full-snps", action="store_true",
+                    help="time the CPU oracle on all J SNPs (two full iterations; minutes at c3) instead of a sample")
+    ap.add_argument("--sweep-workers", type=int, default=1, help="K values fitted at once in the c4 sweep")
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (counter scaling)")
     ap.add_argument("--target-waves", type=int, default=0, help="waves per genotype pass (0: library default)")
     ap.add_argument("--no-ttc", action="store_true", help="skip the time-to-converge runs")
@@ -632,8 +654,9 @@ def main():
     if not args.no_skfr and args.stream_snps == 0:
         out["skfr"] = skfr_block(oa, ctx, wl, args)
     if not args.no_cpu_baseline:
-        cb = cpu_baseline(wl, args.dtype, args.seed, args.cpu_budget)
-        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample")}
+        cb = cpu_baseline(wl, args.dtype, args.seed, args.cpu_budget, full_snps=args.cpu_full_snps,
+                          ctx=None if args.stream_snps else ctx)
+        out["cpu_baseline"] = {k: cb[k] for k in ("value", "unit", "cores", "kind", "sample", "cpu_model")}
         out["gpu_over_cpu"] = value / cb["value"]
         if "time_to_converge" in out:
             ttc = out["time_to_converge"]

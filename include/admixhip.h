@@ -64,6 +64,10 @@ extern "C" {
                                    * again after admx_upload_bed / admx_generate_bed); passes stream
                                    * only the rest.  0 (default): stream everything; -1: as many as
                                    * the free HBM holds, less max(4 GiB, 10%) kept for pass scratch */
+#define ADMX_TUNE_SPLIT_Q 7       /* FP64 update_q pass, K >= 9: 1 runs the Hessian triangle in two launches
+                                   * of half the accumulators each (higher occupancy, the genotypes read
+                                   * twice), 0 in one; -1 (default): two from K = 11.  Same results. */
+#define ADMX_TUNE_SPLIT_P 8       /* the same for the FP64 update_p pass; -1: two from K = 13 */
 
 /* values of ADMX_TUNE_QN_MODE: admixture_qn!'s `mode` (src/algorithms_outer.jl:127-136) */
 #define ADMX_QN_ZAL 0
@@ -286,6 +290,16 @@ int admx_loglikelihood(admx_ctx *ctx, int which, double *out);
 int admx_em_q(admx_ctx *ctx);
 /* em_p!(d, g): x_next.p from x  (src/algorithms_inner.jl:108-141) */
 int admx_em_p(admx_ctx *ctx);
+/* em_q!(d, g; mode) / em_p!(d, g; mode) with the reference's modes
+ * (src/algorithms_inner.jl:54-141): ADMX_EM_BASE as admx_em_q / admx_em_p;
+ * ADMX_EM_FALLBACK: em_q! as base, em_p! writes p of x_next from q of x_next
+ * and p of x; ADMX_EM_FALLBACK2: em_q! writes q of x_next2 from q, p of x_next,
+ * em_p! writes p of x_next2 from q of x_next2 and p of x_next. */
+#define ADMX_EM_BASE 0
+#define ADMX_EM_FALLBACK 1
+#define ADMX_EM_FALLBACK2 2
+int admx_em_q_mode(admx_ctx *ctx, int mode);
+int admx_em_p_mode(admx_ctx *ctx, int mode);
 /* init_em!(d, g, iters): lls[it] = LL after EM step it; sets ll_new
  * (src/algorithms_outer.jl:12-45) */
 int admx_init_em(admx_ctx *ctx, int64_t iters, double *lls);

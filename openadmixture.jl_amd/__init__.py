@@ -32,8 +32,8 @@ from .loops import (UnitRange, em_p_loop, em_p_loop_skipmissing, em_q_loop, em_q
 from .plink import SnpArray, SnpLinAlg, filter_plink, write_plink
 from .skfr import ImputedSnpMatrix, sparsekmeans1, sparsekmeans2, sparsekmeans_path, sparsekmeans_repeat
 from .structs import AdmixData
-from .tune import autotune_waves
-from .warmstart import MLP, Panel, training_set, warm_started_fit, world_frequencies
+from .tune import autotune, autotune_waves
+from .warmstart import MLP, Panel, drifted_populations, training_set, warm_started_fit, world_frequencies
 
 # Fail loudly at import when the native library is absent.
 _native.lib()
@@ -45,8 +45,8 @@ __all__ = [
     "update_q_loop_skipmissing", "update_p_loop_skipmissing", "UnitRange",
     "loglikelihood_loop", "em_q_loop", "em_p_loop", "update_q_loop", "update_p_loop",
     "SnpArray", "SnpLinAlg", "write_plink", "AdmixHIPError", "device_count",
-    "cv_sweep", "best_k", "fit_on_device", "autotune_waves",
-    "MLP", "Panel", "training_set", "warm_started_fit", "world_frequencies",
+    "cv_sweep", "best_k", "fit_on_device", "autotune", "autotune_waves",
+    "MLP", "Panel", "training_set", "warm_started_fit", "world_frequencies", "drifted_populations",
     "_filter_SKFR", "filter_plink", "ImputedSnpMatrix", "sparsekmeans1", "sparsekmeans2", "sparsekmeans_repeat",
     "sparsekmeans_path",
 ]

@@ -21,9 +21,11 @@ ADMX_TUNE_TARGET_WAVES, ADMX_TUNE_KEEP_HESSIANS, ADMX_TUNE_PRECISION, ADMX_TUNE_
 ADMX_TUNE_QN_MODE = 4
 ADMX_TUNE_SKIPMISSING = 5
 ADMX_TUNE_RESIDENT_SNPS = 6
+ADMX_TUNE_SPLIT_Q, ADMX_TUNE_SPLIT_P = 7, 8
 ADMX_QN_ZAL, ADMX_QN_LBQN = 0, 1
 ADMX_PREC_F64, ADMX_PREC_F32, ADMX_PREC_BF16 = 0, 1, 2
 ADMX_SKFR_GLOBAL, ADMX_SKFR_LOCAL = 0, 1
+ADMX_EM_BASE, ADMX_EM_FALLBACK, ADMX_EM_FALLBACK2 = 0, 1, 2
 ADMX_NOALLOC_LL_PARTS = 65536  # doubles of admx_loglikelihood_noalloc's fixed scratch
 
 # Every exported symbol with its C signature (return type is int unless noted).
@@ -83,6 +85,8 @@ SIGNATURES = {
     "admx_update_p": ([_vp, _int, _vp], _int),
     "admx_qn_iteration": ([_vp, _i64, _int, _int, _vp], _int),
     "admx_synchronize": ([_vp], _int),
+    "admx_em_q_mode": ([_vp, _int], _int),
+    "admx_em_p_mode": ([_vp, _int], _int),
     "admx_set_block": ([_vp, _int, _int, _vp], _int),
     "admx_get_block": ([_vp, _int, _int, _vp], _int),
     "admx_copy_block": ([_vp, _int, _int, _int], _int),

@@ -49,25 +49,31 @@ def loglikelihood(g, q, p, qp_small=None, K=None, skipmissing=True) -> float:
 
 
 def em_q(d: AdmixData, g=None, mode="base", d_cu=None, verbose=False) -> None:
-    """em_q!(d, g): q_next from (q, p), divided by 2J  (src/algorithms_inner.jl:68-92)."""
-    if mode not in ("base", "fallback"):
-        raise NotImplementedError(f"em_q! mode {mode!r}")
+    """em_q!(d, g; mode)  (src/algorithms_inner.jl:54-92): divided by 2J.
+    mode "base" / "fallback": q_next from (q, p); "fallback2": q_next2 from
+    (q_next, p_next)."""
+    mode = str(mode).lstrip(":")
     c = _dev(d, d_cu)
-    c.set_x(N.ADMX_X, d.x)
-    c.em_q()
-    xn = c.get_x(N.ADMX_X_NEXT)
-    d.x_next[:, : d.I] = xn[:, : d.I]
+    _sync_in(c, d)
+    c.em_q(mode)
+    if mode == "fallback2":
+        d.x_next2[:, : d.I] = c.get_block(N.ADMX_X_NEXT2, N.ADMX_SIDE_Q)
+    else:
+        d.x_next[:, : d.I] = c.get_block(N.ADMX_X_NEXT, N.ADMX_SIDE_Q)
 
 
 def em_p(d: AdmixData, g=None, mode="base", d_cu=None, verbose=False) -> None:
-    """em_p!(d, g): p_next = p_tmp / (p_tmp + p_next)  (src/algorithms_inner.jl:108-141)."""
-    if mode != "base":
-        raise NotImplementedError(f"em_p! mode {mode!r}")
+    """em_p!(d, g; mode)  (src/algorithms_inner.jl:94-141): p_next = p_tmp / (p_tmp + p_next).
+    mode "base": p_next from (q, p); "fallback": p_next from (q_next, p);
+    "fallback2": p_next2 from (q_next2, p_next)."""
+    mode = str(mode).lstrip(":")
     c = _dev(d, d_cu)
-    c.set_x(N.ADMX_X, d.x)
-    c.em_p()
-    xn = c.get_x(N.ADMX_X_NEXT)
-    d.x_next[:, d.I:] = xn[:, d.I:]
+    _sync_in(c, d)
+    c.em_p(mode)
+    if mode == "fallback2":
+        d.x_next2[:, d.I:] = c.get_block(N.ADMX_X_NEXT2, N.ADMX_SIDE_P)
+    else:
+        d.x_next[:, d.I:] = c.get_block(N.ADMX_X_NEXT, N.ADMX_SIDE_P)
 
 
 def _sync_in(c: HIPAdmixData, d: AdmixData):

@@ -287,6 +287,7 @@ int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
     ProfScope ps(c, mode_name(true, mode));
     CHK(for_chunks(c, j0, j1, [&](const uint8_t* base, int64_t cj0, int64_t cj1, size_t k) {
       PassArgs a{base, c->pitch, c->I, q, p, i0, i1, cj0, cj1, chs[k], nullptr, c->impute ? c->snp_mu.p : nullptr};
+      a.split = c->split_q;
       if (mode == MODE_LL) {
         a.part = static_cast<double*>(c->llpart.p) + nbx * first[k];
         a.strict = flags;
@@ -328,6 +329,7 @@ int pass_p(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
     CHK(for_chunks(c, j0, j1, [&](const uint8_t* base, int64_t cj0, int64_t cj1, size_t) {
       PassArgs a{base, c->pitch, c->I, q, p, i0, i1, cj0, cj1, chunk, c->part.p, c->impute ? c->snp_mu.p : nullptr,
                  Jlen, cj0 - j0};
+      a.split = c->split_p;
       launch_pside<double>(mode, (int)c->K, a, n, c->s);
       return check_launch();
     }));
@@ -350,17 +352,32 @@ int ll_mixed(admx_ctx* c, const double* q, const double* p, int slot) {
   nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, J / 256));
   const int64_t jchunk = ((J + nchunk - 1) / nchunk + 31) / 32 * 32;
   nchunk = (J + jchunk - 1) / jchunk;
-  CHK(c->llpart.grow(sizeof(double) * nbx * nchunk));
+  // A state that may give s = q.p of exactly 0 or 1 (FRAPPE EM on a monomorphic
+  // SNP) gets the reference's NaN / -Inf terms from the FP64 term-by-term pass
+  // instead: both passes are launched, the device flag picks the one that works
+  // (no host round trip, so captured graphs stay valid).
+  int64_t sch = 0;
+  const int sn = qside_chunks(c, I, J, &sch);
+  const int64_t nmf = nbx * nchunk, nst = nbx * sn;
+  CHK(c->llpart.grow(sizeof(double) * (nmf + nst)));
+  CHK(c->llflag.grow(2 * sizeof(int)));
+  int* flag = static_cast<int*>(c->llflag.p);
+  double* parts = static_cast<double*>(c->llpart.p);
   {
     ProfScope ps(c, "qside_ll_mfma");
+    launch_ll_state_check((int)K, q, I, p, J, flag, c->s);
     launch_split_tables((int)K, q, I, rows_q, 0, c->lqt.p, nullptr, c->s);
     launch_split_tables((int)K, p, J, rows_p, 1, c->lpt.p, c->lmt.p, c->s);
-    launch_ll_mfma((int)K, c->bed, c->pitch, I, J, jchunk, (int)nchunk, q, c->lqt.p, c->lpt.p, c->lmt.p,
-                   static_cast<double*>(c->llpart.p), c->s);
+    launch_ll_mfma((int)K, c->bed, c->pitch, I, J, jchunk, (int)nchunk, q, c->lqt.p, c->lpt.p, c->lmt.p, parts,
+                   flag, c->s);
+    PassArgs a{c->bed, c->pitch, I, q, p, 0, I, 0, J, sch, parts + nmf};
+    a.strict = flag;
+    launch_qside_ll_strict((int)K, a, sn, c->s);
   }
   CHK(check_launch());
   ProfScope ps(c, "ll_reduce");
-  launch_reduce_sum(static_cast<const double*>(c->llpart.p), nbx * nchunk, c->lls + slot, 0, c->s);
+  launch_reduce_sum_if(parts, nmf, c->lls + slot, flag, 0, c->s);
+  launch_reduce_sum_if(parts + nmf, nst, c->lls + slot, flag, 1, c->s);
   return check_launch();
 }
 
@@ -368,6 +385,23 @@ int ll_mixed(admx_ctx* c, const double* q, const double* p, int slot) {
 // FP64 partial layout of the FP64 EM passes (chunk count in *nchunk).
 bool em_on_matrix_cores(const admx_ctx* c) {
   return c->dtype == ADMX_F64 && c->precision == ADMX_PREC_BF16 && !c->impute && !c->stream_snps;
+}
+
+// The matrix-core EM clamps s (1 - s) at 1e-30, so on a state where some
+// s = q.p is exactly 0 or 1 it adds 0 where the reference's 0/0 gives NaN.
+// Such states (flagged by the log-likelihood state check) take the FP64 EM
+// pass; the check reads its flag back, which is fine outside captured graphs
+// (EM steps are never captured).
+int em_state_is_safe(admx_ctx* c, const double* q, const double* p, bool* safe) {
+  CHK(c->llflag.grow(2 * sizeof(int)));
+  int* flag = static_cast<int*>(c->llflag.p);
+  launch_ll_state_check((int)c->K, q, c->I, p, c->J, flag, c->s);
+  CHK(check_launch());
+  int h = 0;
+  HIPCHK(hipMemcpyAsync(&h, flag, sizeof(int), hipMemcpyDeviceToHost, c->s));
+  HIPCHK(hipStreamSynchronize(c->s));
+  *safe = h == 0;
+  return ADMX_OK;
 }
 
 int em_mixed(admx_ctx* c, bool qside, const double* q, const double* p, int* nchunk_out) {
@@ -401,7 +435,8 @@ int em_mixed(admx_ctx* c, bool qside, const double* q, const double* p, int* nch
       const int64_t nbj = (J + 255) / 256;
       nchunk = std::max<int64_t>(1, (c->target_waves / 4 + nbj - 1) / nbj);
       nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, I / 256));
-      const int64_t ichunk = ((I + nchunk - 1) / nchunk + 63) / 64 * 64;
+      // chunks of whole 128-byte BED lines (512 individuals), so no line is split between two workgroups
+      const int64_t ichunk = ((I + nchunk - 1) / nchunk + 511) / 512 * 512;
       nchunk = (I + ichunk - 1) / ichunk;
       CHK(c->part.grow(sizeof(double) * (size_t)(nchunk * 2 * K * J)));
       launch_emp_mfma((int)K, c->bed, c->pitch, I, J, ichunk, (int)nchunk, static_cast<const float*>(c->lqc.p),
@@ -611,35 +646,49 @@ int dev_grad_hessian(admx_ctx* c, bool qside) {
   return check_launch();
 }
 
+// em_q!(d, g; mode) (src/algorithms_inner.jl:68-92): q of x_next from (q, p) of x
+// (ADMX_EM_BASE, ADMX_EM_FALLBACK), or q of x_next2 from (q, p) of x_next
+// (ADMX_EM_FALLBACK2).
 template <typename T>
-int dev_em_q(admx_ctx* c) {
-  void* X = c->x[ADMX_X];
+int dev_em_q(admx_ctx* c, int mode = ADMX_EM_BASE) {
+  void* src = c->x[mode == ADMX_EM_FALLBACK2 ? ADMX_X_NEXT : ADMX_X];
+  void* dst = c->x[mode == ADMX_EM_FALLBACK2 ? ADMX_X_NEXT2 : ADMX_X_NEXT];
+  const void* q = src;
+  const void* p = c->pp(src);
   int nchunk = 0;
   int64_t np = 0;
-  if (em_on_matrix_cores(c)) {
+  bool mfma = em_on_matrix_cores(c);
+  if (mfma) CHK(em_state_is_safe(c, static_cast<const double*>(q), static_cast<const double*>(p), &mfma));
+  if (mfma) {
     CHK(need_resident(c, "the matrix-core EM passes"));
-    CHK(em_mixed(c, true, static_cast<const double*>(X), static_cast<const double*>(c->pp(X)), &nchunk));
+    CHK(em_mixed(c, true, static_cast<const double*>(q), static_cast<const double*>(p), &nchunk));
   } else {
-    CHK(pass_q<T>(c, MODE_EM, X, c->pp(X), 0, c->I, 0, c->J, &nchunk, &np));
+    CHK(pass_q<T>(c, MODE_EM, q, p, 0, c->I, 0, c->J, &nchunk, &np));
   }
-  FinArgs f{c->part.p, c->I, nchunk, 0, c->J, X, c->x[ADMX_X_NEXT], nullptr, nullptr, nullptr, 0, 0, nullptr};
+  FinArgs f{c->part.p, c->I, nchunk, 0, c->J, q, dst, nullptr, nullptr, nullptr, 0, 0, nullptr};
   ProfScope ps(c, "fin_q_em");
   launch_fin_q_em<T>((int)c->K, f, c->s);
   return check_launch();
 }
 
+// em_p!(d, g; mode) (src/algorithms_inner.jl:108-141): p of x_next from q, p of x
+// (ADMX_EM_BASE), from q of x_next and p of x (ADMX_EM_FALLBACK), or p of
+// x_next2 from q of x_next2 and p of x_next (ADMX_EM_FALLBACK2).
 template <typename T>
-int dev_em_p(admx_ctx* c) {
-  void* X = c->x[ADMX_X];
+int dev_em_p(admx_ctx* c, int mode = ADMX_EM_BASE) {
+  const void* q = c->x[mode == ADMX_EM_BASE ? ADMX_X : (mode == ADMX_EM_FALLBACK ? ADMX_X_NEXT : ADMX_X_NEXT2)];
+  const void* p = c->pp(c->x[mode == ADMX_EM_FALLBACK2 ? ADMX_X_NEXT : ADMX_X]);
+  void* dst = c->pp(c->x[mode == ADMX_EM_FALLBACK2 ? ADMX_X_NEXT2 : ADMX_X_NEXT]);
   int nchunk = 0;
-  if (em_on_matrix_cores(c)) {
+  bool mfma = em_on_matrix_cores(c);
+  if (mfma) CHK(em_state_is_safe(c, static_cast<const double*>(q), static_cast<const double*>(p), &mfma));
+  if (mfma) {
     CHK(need_resident(c, "the matrix-core EM passes"));
-    CHK(em_mixed(c, false, static_cast<const double*>(X), static_cast<const double*>(c->pp(X)), &nchunk));
+    CHK(em_mixed(c, false, static_cast<const double*>(q), static_cast<const double*>(p), &nchunk));
   } else {
-    CHK(pass_p<T>(c, MODE_EM, X, c->pp(X), 0, c->I, 0, c->J, &nchunk));
+    CHK(pass_p<T>(c, MODE_EM, q, p, 0, c->I, 0, c->J, &nchunk));
   }
-  FinArgs f{c->part.p, c->J, nchunk, 0, c->J, c->pp(X), c->pp(c->x[ADMX_X_NEXT]), nullptr, nullptr, nullptr, 0, 0,
-            nullptr};
+  FinArgs f{c->part.p, c->J, nchunk, 0, c->J, p, dst, nullptr, nullptr, nullptr, 0, 0, nullptr};
   ProfScope ps(c, "fin_p_em");
   launch_fin_p_em<T>((int)c->K, f, c->s);
   return check_launch();
@@ -720,9 +769,9 @@ uint64_t layout_sig(const admx_ctx* c) {
     h ^= v;
     h *= 1099511628211ULL;
   };
-  const DevBuf* bufs[] = {&c->part,  &c->llpart, &c->zal,     &c->fq,      &c->fp,     &c->ppt,
-                          &c->mmt,   &c->qqt,    &c->wide[0], &c->wide[1], &c->wide[2], &c->wide[3],
-                          &c->llflag};
+  const DevBuf* bufs[] = {&c->part,    &c->llpart,  &c->zal,     &c->fq,  &c->fp,  &c->ppt,
+                          &c->mmt,     &c->qqt,     &c->wide[0], &c->wide[1], &c->wide[2], &c->wide[3],
+                          &c->llflag,  &c->lqt,     &c->lpt,     &c->lmt, &c->lps, &c->lqc};
   for (const DevBuf* b : bufs) {
     mix((uint64_t)(uintptr_t)b->p);
     mix((uint64_t)b->n);
@@ -732,6 +781,8 @@ uint64_t layout_sig(const admx_ctx* c) {
   mix((uint64_t)c->precision);
   mix((uint64_t)c->qn_mode);
   mix((uint64_t)c->impute);
+  mix((uint64_t)(c->split_q + 2));
+  mix((uint64_t)(c->split_p + 2));
   mix((uint64_t)(uintptr_t)c->snp_mu.p);
   return h;
 }
@@ -1713,6 +1764,30 @@ int admx_em_p(admx_ctx* c) {
   });
 }
 
+int admx_em_q_mode(admx_ctx* c, int mode) {
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    if (mode < ADMX_EM_BASE || mode > ADMX_EM_FALLBACK2) return bad("mode must be ADMX_EM_BASE, _FALLBACK or _FALLBACK2");
+    CHK(BY_DTYPE(c, dev_em_q, c, mode));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
+}
+
+int admx_em_p_mode(admx_ctx* c, int mode) {
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    if (mode < ADMX_EM_BASE || mode > ADMX_EM_FALLBACK2) return bad("mode must be ADMX_EM_BASE, _FALLBACK or _FALLBACK2");
+    CHK(BY_DTYPE(c, dev_em_p, c, mode));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
+}
+
 int admx_init_em(admx_ctx* c, int64_t iters, double* lls) {
   return guarded([&]() -> int {
     CHK(check_ctx(c));
@@ -1814,6 +1889,9 @@ int admx_set_tuning(admx_ctx* c, int key, int64_t value) {
       if (value < ADMX_PREC_F64 || value > ADMX_PREC_BF16) return bad("precision must be ADMX_PREC_F64/F32/BF16");
       if (value != ADMX_PREC_F64 && c->dtype != ADMX_F64) return bad("mixed precision needs a FP64 context");
       c->precision = (int)value;
+    } else if (key == ADMX_TUNE_SPLIT_Q || key == ADMX_TUNE_SPLIT_P) {
+      if (value < -1 || value > 1) return bad("Hessian split must be -1 (default), 0 or 1");
+      (key == ADMX_TUNE_SPLIT_Q ? c->split_q : c->split_p) = (int)value;
     } else if (key == ADMX_TUNE_SKIPMISSING) {
       c->impute = value == 0;
     } else if (key == ADMX_TUNE_QN_MODE) {

@@ -113,6 +113,7 @@ struct admx_ctx {
   admx_host::DevBuf part, llpart, zal;
   admx_host::DevBuf st[8];  // staging for host-array entry points
   int64_t target_waves = 65536;
+  int split_q = -1, split_p = -1;  // ADMX_TUNE_SPLIT_Q / _P (-1: the default K thresholds)
   int keep_hess = 1;
   int precision = ADMX_PREC_F64;
   admx_host::DevBuf fq, fp, ppt, mmt, qqt;  // mixed-precision pass inputs

@@ -29,6 +29,10 @@ struct PassArgs {
   // launch_ll_state_check); the pass then sums g log s + (2-g) log(1-s) term by
   // term instead of the product form.  nullptr: product form.
   const int* strict = nullptr;
+  // FP64 update passes, K >= 9: 1 runs the Hessian triangle in two launches of
+  // half the accumulators each, 0 in one; -1 the default (Q side K >= 11, P side
+  // K >= 13).  Results are identical either way.
+  int split = -1;
 };
 
 // Per-column reduction of pass partials, optional QP + projection.
@@ -96,8 +100,11 @@ void launch_pside_bf16(int K, const PassArgs& a, int nchunk, const float* fq, co
 int split_table_cols(int K);
 void launch_split_tables(int K, const double* x, int64_t n, int64_t rows, int role, void* T1, void* T2,
                          hipStream_t s);
+// skip (device, nullable): when *skip != 0 every workgroup returns at once (the
+// state check flagged s = 0 or 1; launch_qside_ll_strict then sums the terms).
 void launch_ll_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
-                    const double* q, const void* QT, const void* PT, const void* MT, double* part, hipStream_t s);
+                    const double* q, const void* QT, const void* PT, const void* MT, double* part, const int* skip,
+                    hipStream_t s);
 // FRAPPE EM sums of the BF16 form on the matrix cores: s and t as for the
 // log-likelihood, the per-genotype weights in FP32, and the sums over SNPs
 // (em_q) or individuals (em_p) as BF16 products against the two-term split
@@ -120,6 +127,12 @@ template <typename T> void launch_project_p(int64_t n, void* b, hipStream_t s);
 
 // LL partial reduction: sum nparts doubles (fixed order) into *out (device).
 void launch_reduce_sum(const double* parts, int64_t nparts, double* out, int accumulate, hipStream_t s);
+// The same, done only when (*flag != 0) == want (device flag).
+void launch_reduce_sum_if(const double* parts, int64_t nparts, double* out, const int* flag, int want,
+                          hipStream_t s);
+// FP64 log-likelihood pass, term-by-term instance only, run when *a.strict != 0
+// (a.strict non-null): the fallback of a product-form or matrix-core pass.
+void launch_qside_ll_strict(int K, const PassArgs& a, int nchunk, hipStream_t s);
 
 // ZAL quasi-Newton (src/quasi_newton.jl:5-31) on flat x of length N.
 template <typename T> void launch_zal(int K, int64_t I, int64_t J, const void* x, const void* xn, const void* xn2,

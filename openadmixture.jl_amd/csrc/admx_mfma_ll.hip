@@ -109,14 +109,18 @@ template <int K>
 __global__ __launch_bounds__(256) void k_ll_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int64_t J,
                                                  int64_t jchunk, const double* __restrict__ q64,
                                                  const __bf16* __restrict__ QT, const __bf16* __restrict__ PT,
-                                                 const __bf16* __restrict__ MT, double* __restrict__ part) {
+                                                 const __bf16* __restrict__ MT, double* __restrict__ part,
+                                                 const int* __restrict__ skip) {
+  if (skip && *skip) return;  // uniform: the state may give s = 0 or 1, the term-by-term FP64 pass runs instead
   constexpr int KD = SplitDim<K>::KD, NM = SplitDim<K>::NM;
   __shared__ __align__(16) uint8_t tbuf[4][512];
   __shared__ double sh_red[256];
   uint8_t* __restrict__ tb = tbuf[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63, h = lane >> 5, cn = lane & 31;
-  const int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
-  const int64_t jb = (int64_t)blockIdx.y * jchunk;
+  const int64_t nbx = (I + 255) / 256;
+  const Tile2 tile = xcd_tile(nbx);  // neighbouring 256-individual blocks share BED lines: keep them on one XCD
+  const int64_t i0 = (tile.x * 4 + (threadIdx.x >> 6)) * 64;
+  const int64_t jb = tile.y * jchunk;
   const int64_t je = (jb + jchunk < J) ? jb + jchunk : J;
 
   bf16x8 bq[2][NM];
@@ -194,7 +198,7 @@ __global__ __launch_bounds__(256) void k_ll_mfma(const uint8_t* __restrict__ bed
   }
   const double acc = log(prod) + (double)esum * 0.69314718055994530942;
   const double tot = block_sum<256>(acc, sh_red);
-  if (threadIdx.x == 0) part[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = tot;
+  if (threadIdx.x == 0) part[tile.y * nbx + tile.x] = tot;
 }
 
 template <int K>
@@ -211,14 +215,15 @@ int split_table_cols(int K) { return ((6 * K + 15) / 16) * 16; }
 template <int K>
 static void ll_mfma_k(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
                       const double* q, const __bf16* QT, const __bf16* PT, const __bf16* MT, double* part,
-                      hipStream_t s) {
-  const dim3 grid((unsigned)((I + 255) / 256), (unsigned)nchunk);
-  hipLaunchKernelGGL((k_ll_mfma<K>), grid, dim3(256), 0, s, bed, pitch, I, J, jchunk, q, QT, PT, MT, part);
+                      const int* skip, hipStream_t s) {
+  const dim3 grid((unsigned)((I + 255) / 256 * nchunk));  // 1-D, tiles ordered per XCD (xcd_tile)
+  hipLaunchKernelGGL((k_ll_mfma<K>), grid, dim3(256), 0, s, bed, pitch, I, J, jchunk, q, QT, PT, MT, part, skip);
 }
 void launch_ll_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
-                    const double* q, const void* QT, const void* PT, const void* MT, double* part, hipStream_t s) {
+                    const double* q, const void* QT, const void* PT, const void* MT, double* part, const int* skip,
+                    hipStream_t s) {
   ADMX_DISPATCH_K(K, (ll_mfma_k<KK>(bed, pitch, I, J, jchunk, nchunk, q, (const __bf16*)QT, (const __bf16*)PT,
-                                    (const __bf16*)MT, part, s)));
+                                    (const __bf16*)MT, part, skip, s)));
 }
 
 // ------------------------------------------------------------------ FRAPPE EM sums on the matrix cores
@@ -323,8 +328,9 @@ void k_emq_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int64
   __shared__ __align__(16) uint8_t tbuf[4][512];
   uint8_t* __restrict__ tb = tbuf[threadIdx.x >> 6];
   const int lane = threadIdx.x & 63, h = lane >> 5, cn = lane & 31;
-  const int64_t i0 = ((int64_t)blockIdx.x * 4 + (threadIdx.x >> 6)) * 64;
-  const int64_t jb = (int64_t)blockIdx.y * jchunk;
+  const Tile2 tile = xcd_tile((I + 255) / 256);  // as in k_ll_mfma
+  const int64_t i0 = (tile.x * 4 + (threadIdx.x >> 6)) * 64;
+  const int64_t jb = tile.y * jchunk;
   const int64_t je = (jb + jchunk < J) ? jb + jchunk : J;
 
   bf16x8 bq[2][NM];
@@ -413,7 +419,7 @@ void k_emq_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int64
 #pragma unroll
   for (int T = 0; T < 2; ++T) {
     if (!(T ? ok1 : ok0)) continue;
-    double* __restrict__ out = part + (int64_t)blockIdx.y * (K + 1) * I + i0 + 32 * T + cn;
+    double* __restrict__ out = part + tile.y * (K + 1) * I + i0 + 32 * T + cn;
 #pragma unroll
     for (int r = 0; r < 8; ++r) {
       const int row = 8 * (r >> 2) + 4 * h + (r & 3);
@@ -549,7 +555,7 @@ template <int K>
 static void emq_mfma_k(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
                        const double* q, const __bf16* QT, const __bf16* PT, const __bf16* MT, const __bf16* PS,
                        const __bf16* MS, int64_t ldps, double* part, hipStream_t s) {
-  const dim3 grid((unsigned)((I + 255) / 256), (unsigned)nchunk);
+  const dim3 grid((unsigned)((I + 255) / 256 * nchunk));  // 1-D, tiles ordered per XCD (xcd_tile)
   hipLaunchKernelGGL((k_emq_mfma<K>), grid, dim3(256), 0, s, bed, pitch, I, J, jchunk, q, QT, PT, MT, PS, MS, ldps,
                      part);
 }

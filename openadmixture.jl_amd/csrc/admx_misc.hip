@@ -25,6 +25,22 @@ void launch_reduce_sum(const double* parts, int64_t nparts, double* out, int acc
   hipLaunchKernelGGL(k_reduce_sum, dim3(1), dim3(256), 0, s, parts, nparts, out, accumulate);
 }
 
+__global__ __launch_bounds__(256) void k_reduce_sum_if(const double* __restrict__ parts, int64_t n,
+                                                       double* __restrict__ out, const int* __restrict__ flag,
+                                                       int want) {
+  if ((*flag != 0) != (want != 0)) return;  // uniform
+  __shared__ double sh[256];
+  double s = 0.0;
+  for (int64_t e = threadIdx.x; e < n; e += 256) s += parts[e];
+  const double tot = block_sum<256>(s, sh);
+  if (threadIdx.x == 0) *out = tot;
+}
+
+void launch_reduce_sum_if(const double* parts, int64_t nparts, double* out, const int* flag, int want,
+                          hipStream_t s) {
+  hipLaunchKernelGGL(k_reduce_sum_if, dim3(1), dim3(256), 0, s, parts, nparts, out, flag, want);
+}
+
 // ------------------------------------------------------------------ ZAL quasi-Newton
 // update_UV!: U[:,col] = x_next - x, V[:,col] = x_next2 - x_next, and the
 // partial inner products A[a,b] = U[:,a]'(U[:,b] - V[:,b]) and

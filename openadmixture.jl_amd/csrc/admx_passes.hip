@@ -423,15 +423,18 @@ static void qside_ki(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
   const int64_t n = a.i1 - a.i0;
   dim3 grid((unsigned)(((n + 255) / 256) * nchunk));  // 1-D, tiles ordered per XCD (xcd_tile)
   constexpr bool trick = sizeof(T) == 8;
-  // FP64 from K = 11 on: the triangle in two launches (see k_qside)
-  constexpr bool split = sizeof(T) == 8 && K >= 11;
+  // FP64 from K = 9 on, the triangle can run in two launches (see k_qside);
+  // a.split < 0 takes the default threshold K >= 11
   if (mode == MODE_UPDATE) {
-    if constexpr (split) {
-      hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick, IMP, 1>), grid, dim3(256), 0, s, a);
-      hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick, IMP, 2>), grid, dim3(256), 0, s, a);
-    } else {
-      hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick, IMP>), grid, dim3(256), 0, s, a);
+    bool done = false;
+    if constexpr (sizeof(T) == 8 && K >= 9) {
+      if (a.split > 0 || (a.split < 0 && K >= 11)) {
+        hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick, IMP, 1>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick, IMP, 2>), grid, dim3(256), 0, s, a);
+        done = true;
+      }
     }
+    if (!done) hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick, IMP>), grid, dim3(256), 0, s, a);
   }
   else if (mode == MODE_EM) hipLaunchKernelGGL((k_qside<T, K, MODE_EM, false, IMP>), grid, dim3(256), 0, s, a);
   else {
@@ -450,14 +453,18 @@ template <typename T, int K, bool IMP>
 static void pside_ki(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
   const int64_t n = a.j1 - a.j0;
   dim3 grid((unsigned)((n + 255) / 256), (unsigned)nchunk);
-  constexpr bool split = sizeof(T) == 8 && K >= 13;  // FP64: the triangle in two launches (see k_pside)
+  // FP64 from K = 9 on, the triangle can run in two launches (see k_pside);
+  // a.split < 0 takes the default threshold K >= 13
   if (mode == MODE_UPDATE) {
-    if constexpr (split) {
-      hipLaunchKernelGGL((k_pside<T, K, MODE_UPDATE, IMP, 1>), grid, dim3(256), 0, s, a);
-      hipLaunchKernelGGL((k_pside<T, K, MODE_UPDATE, IMP, 2>), grid, dim3(256), 0, s, a);
-    } else {
-      hipLaunchKernelGGL((k_pside<T, K, MODE_UPDATE, IMP>), grid, dim3(256), 0, s, a);
+    bool done = false;
+    if constexpr (sizeof(T) == 8 && K >= 9) {
+      if (a.split > 0 || (a.split < 0 && K >= 13)) {
+        hipLaunchKernelGGL((k_pside<T, K, MODE_UPDATE, IMP, 1>), grid, dim3(256), 0, s, a);
+        hipLaunchKernelGGL((k_pside<T, K, MODE_UPDATE, IMP, 2>), grid, dim3(256), 0, s, a);
+        done = true;
+      }
     }
+    if (!done) hipLaunchKernelGGL((k_pside<T, K, MODE_UPDATE, IMP>), grid, dim3(256), 0, s, a);
   } else {
     hipLaunchKernelGGL((k_pside<T, K, MODE_EM, IMP>), grid, dim3(256), 0, s, a);
   }
@@ -487,6 +494,15 @@ template <typename T> void launch_pside(int mode, int K, const PassArgs& a, int 
 }
 
 template void launch_qside<ADMX_T>(int, int, const PassArgs&, int, hipStream_t);
+
+template <int K>
+static void qside_ll_strict_k(const PassArgs& a, int nchunk, hipStream_t s) {
+  const dim3 grid((unsigned)(((a.i1 - a.i0 + 255) / 256) * nchunk));
+  hipLaunchKernelGGL((k_qside<double, K, MODE_LL, false, false, 0, true>), grid, dim3(256), 0, s, a);
+}
+void launch_qside_ll_strict(int K, const PassArgs& a, int nchunk, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (qside_ll_strict_k<KK>(a, nchunk, s)));
+}
 template void launch_pside<ADMX_T>(int, int, const PassArgs&, int, hipStream_t);
 #endif
 

@@ -250,11 +250,29 @@ class HIPAdmixData:
         N.check(N.lib().admx_loglikelihood(self._h, which, C.byref(out)))
         return out.value
 
-    def em_q(self):
-        N.check(N.lib().admx_em_q(self._h))
+    EM_MODES = {"base": N.ADMX_EM_BASE, "fallback": N.ADMX_EM_FALLBACK, "fallback2": N.ADMX_EM_FALLBACK2}
 
-    def em_p(self):
-        N.check(N.lib().admx_em_p(self._h))
+    def em_q(self, mode: str = "base"):
+        """em_q!(d, g; mode): base/fallback q_next from (q, p); fallback2 q_next2
+        from (q_next, p_next)."""
+        if mode == "base":
+            N.check(N.lib().admx_em_q(self._h))
+        else:
+            N.check(N.lib().admx_em_q_mode(self._h, self._em_mode(mode)))
+
+    def em_p(self, mode: str = "base"):
+        """em_p!(d, g; mode): base p_next from (q, p); fallback p_next from
+        (q_next, p); fallback2 p_next2 from (q_next2, p_next)."""
+        if mode == "base":
+            N.check(N.lib().admx_em_p(self._h))
+        else:
+            N.check(N.lib().admx_em_p_mode(self._h, self._em_mode(mode)))
+
+    def _em_mode(self, mode):
+        m = str(mode).lstrip(":")
+        if m not in self.EM_MODES:
+            raise ValueError(f"em mode must be one of {sorted(self.EM_MODES)}")
+        return self.EM_MODES[m]
 
     def init_em(self, iters: int):
         lls = np.zeros(max(int(iters), 1))

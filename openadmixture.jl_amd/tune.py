@@ -1,12 +1,21 @@
 """Launch-shape autotuner for the genotype passes.
 
-The tunable is the number of waves a genotype pass is split into
-(ADMX_TUNE_TARGET_WAVES): it fixes the SNP chunk per workgroup on the Q side
-and the individual chunk on the P side, trading occupancy and tail balance
-against the size of the per-chunk partial sums that the finalize kernels
-reduce.  The tuner times one update_q! + update_p! pair per candidate with
-HIP events inside the library and keeps the fastest; results are cached per
-(I, J, K, precision) for the life of the process.
+Two tunables, timed with HIP events inside the library:
+
+* waves per pass (ADMX_TUNE_TARGET_WAVES): the SNP chunk per workgroup on the
+  Q side and the individual chunk on the P side, trading occupancy and tail
+  balance against the size of the per-chunk partial sums that the finalize
+  kernels reduce;
+* for K >= 9 (FP64 passes), whether the Hessian triangle runs in one launch
+  or in two of half the accumulators each (ADMX_TUNE_SPLIT_Q / _SPLIT_P):
+  two launches hold fewer registers per lane (more waves per SIMD) but read
+  the genotypes and compute s = q.p twice.
+
+The waves are tuned on update_q! + update_p! pairs with the default split
+rule; then, at the chosen waves and K >= 9, each side's split is tuned on
+that side's update alone.  Results are cached per (I, J, K, dtype, precision)
+for the life of the process.  Every choice gives the same results up to the
+order of the per-chunk partial sums (the split none at all).
 """
 from __future__ import annotations
 
@@ -16,30 +25,55 @@ CANDIDATES = (8192, 16384, 32768, 65536, 131072)
 _cache: dict = {}
 
 
-def autotune_waves(ctx, candidates=CANDIDATES, reps: int = 2) -> int:
-    """Pick and set the fastest waves-per-pass for ctx (its state x must hold a
-    valid point; the passes write x_next only).  Returns the chosen value."""
+def _time(ctx, calls, reps):
+    for f in calls:  # one untimed round: buffers grown, code loaded
+        f()
+    ctx.profile_reset()
+    for _ in range(reps):
+        for f in calls:
+            f()
+    return sum(t for _, t in ctx.profile_report().values())
+
+
+def autotune(ctx, candidates=CANDIDATES, reps: int = 2) -> dict:
+    """Pick and set the fastest waves per pass and, for K >= 9, Hessian split
+    per side for ctx (its state x must hold a valid point; the passes write
+    x_next only).  Returns dict(waves, split_q, split_p) (-1: default rule)."""
     prec = getattr(ctx, "_precision", "f64")
     key = (ctx.I, ctx.J, ctx.K, str(ctx.T), prec)
-    if key in _cache:
-        ctx.set_tuning(N.ADMX_TUNE_TARGET_WAVES, _cache[key])
-        return _cache[key]
-    best, best_ms = None, float("inf")
-    ctx.profile(True)
-    try:
-        for w in candidates:
-            ctx.set_tuning(N.ADMX_TUNE_TARGET_WAVES, w)
-            ctx.update_q(False)
-            ctx.update_p(False)
-            ctx.profile_reset()
-            for _ in range(reps):
-                ctx.update_q(False)
-                ctx.update_p(False)
-            ms = sum(t for _, t in ctx.profile_report().values())
-            if ms < best_ms:
-                best, best_ms = w, ms
-    finally:
-        ctx.profile(False)
-    ctx.set_tuning(N.ADMX_TUNE_TARGET_WAVES, best)
-    _cache[key] = best
-    return best
+    if key not in _cache:
+        choice = {"waves": None, "split_q": -1, "split_p": -1}
+        uq = lambda: ctx.update_q(False)  # noqa: E731
+        up = lambda: ctx.update_p(False)  # noqa: E731
+        ctx.profile(True)
+        try:
+            ctx.set_tuning(N.ADMX_TUNE_SPLIT_Q, -1)
+            ctx.set_tuning(N.ADMX_TUNE_SPLIT_P, -1)
+            best = float("inf")
+            for w in candidates:
+                ctx.set_tuning(N.ADMX_TUNE_TARGET_WAVES, w)
+                ms = _time(ctx, (uq, up), reps)
+                if ms < best:
+                    choice["waves"], best = w, ms
+            ctx.set_tuning(N.ADMX_TUNE_TARGET_WAVES, choice["waves"])
+            if ctx.K >= 9 and prec == "f64":
+                for side, tkey, call in (("split_q", N.ADMX_TUNE_SPLIT_Q, uq), ("split_p", N.ADMX_TUNE_SPLIT_P, up)):
+                    times = {}
+                    for v in (0, 1):
+                        ctx.set_tuning(tkey, v)
+                        times[v] = _time(ctx, (call,), reps)
+                    choice[side] = min(times, key=times.get)
+                    ctx.set_tuning(tkey, choice[side])
+        finally:
+            ctx.profile(False)
+        _cache[key] = choice
+    choice = _cache[key]
+    ctx.set_tuning(N.ADMX_TUNE_TARGET_WAVES, choice["waves"])
+    ctx.set_tuning(N.ADMX_TUNE_SPLIT_Q, choice["split_q"])
+    ctx.set_tuning(N.ADMX_TUNE_SPLIT_P, choice["split_p"])
+    return dict(choice)
+
+
+def autotune_waves(ctx, candidates=CANDIDATES, reps: int = 2) -> int:
+    """autotune(), returning the chosen waves per pass."""
+    return autotune(ctx, candidates, reps)["waves"]

@@ -130,6 +130,18 @@ def world_frequencies(Kw: int, J: int, fst: float = 0.05, seed: int = 0):
     return np.asfortranarray(P), anc
 
 
+def drifted_populations(P, fst: float, seed: int = 0):
+    """Populations drifted from the rows of P by the Balding-Nichols model:
+    P'[k, j] ~ Beta(P[k, j] (1-F)/F, (1 - P[k, j])(1-F)/F), clipped to
+    [0.01, 0.99] -- study populations related to, but not the same as, the
+    panel populations a network was trained on."""
+    rng = np.random.default_rng(seed)
+    P = np.asarray(P, np.float64)
+    a = P * (1 - fst) / fst
+    b = (1 - P) * (1 - fst) / fst
+    return np.asfortranarray(np.clip(rng.beta(a, b), 0.01, 0.99))
+
+
 def match_components(p_fit, p_ref):
     """Permutation perm with fitted component k <-> reference row perm[k],
     maximising the summed correlation of allele frequencies."""
@@ -183,7 +195,7 @@ def training_set(P_world, panel: Panel, n_cohorts: int, I: int = 5000, J: int = 
 
 def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_iter: int = 1000,
                      rtol: float = 1e-7, p_steps: int = 1, judge: bool = True, em_iters: int = 5, Q: int = 3,
-                     stop_at=None):
+                     stop_at=None, min_gain: float = 1.0):
     """admixture_qn! from the network's proposal (with the EM judge/fallback).
 
     p starts at the observed allele frequencies and gets p_steps fixed-Q
@@ -191,8 +203,17 @@ def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_i
     an iteration's time each and saved no quasi-Newton iteration; none
     doubled the iterations on c5).
 
+    The judge keeps the proposal when its log-likelihood (after the P
+    refinement) is at least that of one FRAPPE EM step from the random start
+    and it explains the genotypes beyond Hardy-Weinberg: with p at the observed
+    allele frequencies every q gives the same, structure-free log-likelihood
+    LL_hw (s = q.f = f), and refining p for a q that carries no information
+    gains about (K-1)J/2 by chance (half the chi-square mean of the (K-1)J free
+    allele frequencies), so the gain LL_warm - LL_hw must exceed
+    min_gain * (K-1) * J.  Otherwise the fit falls back to the random start.
+
     Returns dict(history=[(seconds, LL)], chosen='warm'|'em', ll_warm, ll_em,
-    iterations).  history[0] is the starting point; seconds count from after
+    ll_hw, gain (LL_warm - LL_hw in units of (K-1)J), iterations).  history[0] is the starting point; seconds count from after
     the host AdmixData allocation (features, inference, P refinement and the
     judge included), the same origin as the plain run's timing.  With
     stop_at = (LL*, tol) the run ends at the first iteration within tol of
@@ -211,10 +232,12 @@ def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_i
     # admixture_qn! iterations from iter = 1 write every quasi-Newton history
     # column before reading it (src/algorithms_outer.jl:125-133), so the
     # device's U and V need no upload for these fresh runs
-    ctx.set_ll(float("nan"), ctx.loglikelihood(N.ADMX_X))
+    ll_hw = ctx.loglikelihood(N.ADMX_X)  # p_k = f for every k: s = f whatever q is
+    ctx.set_ll(float("nan"), ll_hw)
     for it in range(1, p_steps + 1):
         ctx.qn_iteration(it, fix_q=True)
     ll_warm = ctx.get_ll()[1]
+    gain = (ll_warm - ll_hw) / max((K - 1) * J, 1)
     chosen, ll_em = "warm", None
     if judge:
         # the warm start waits in x_tmp while one FRAPPE EM step runs from the
@@ -223,7 +246,7 @@ def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_i
         ctx.set_x(N.ADMX_X, d.x)
         ctx.init_em(1)
         ll_em = ctx.get_ll()[1]
-        if not (ll_warm >= ll_em):
+        if not (ll_warm >= ll_em and gain > min_gain):
             chosen = "em"
             if em_iters > 1:
                 ctx.init_em(em_iters - 1)
@@ -241,8 +264,9 @@ def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_i
             break
         if stop_at is not None and abs(r["new"] - stop_at[0]) <= stop_at[1] * abs(stop_at[0]):
             break
-    return dict(history=hist, chosen=chosen, ll_warm=ll_warm, ll_em=ll_em, iterations=its,
+    return dict(history=hist, chosen=chosen, ll_warm=ll_warm, ll_em=ll_em, ll_hw=ll_hw, gain=gain, iterations=its,
                 converged=its < max_iter and not math.isnan(hist[-1][1]))
 
 
-__all__ = ["MLP", "Panel", "world_frequencies", "match_components", "training_set", "warm_started_fit"]
+__all__ = ["MLP", "Panel", "world_frequencies", "drifted_populations", "match_components", "training_set",
+           "warm_started_fit"]

@@ -520,27 +520,37 @@ static double FN(st_ll)(FN(orc_state) * st, T *x) {
                                st->K, 0, st->I, 0, st->J, st->mu, st->nthreads);
 }
 
-/* em_q!(d, g)  (src/algorithms_inner.jl:68-92): q_next from (q, p), divided by 2J */
-void FN(orc_em_q)(FN(orc_state) * st) {
+/* em_q!(d, g; mode)  (src/algorithms_inner.jl:68-92), divided by 2J.
+ * mode 0 (:base) and 1 (:fallback): q_next from (q, p); 2 (:fallback2):
+ * q_next2 from (q_next, p_next). */
+void FN(orc_em_q_mode)(FN(orc_state) * st, int mode) {
   const int64_t K = st->K, I = st->I, J = st->J;
-  T *q_next = FN(qpart)(st, st->x_next);
+  T *dst = mode == 2 ? st->x_next2 : st->x_next, *src = mode == 2 ? st->x_next : st->x;
+  T *q_next = FN(qpart)(st, dst);
   memset(q_next, 0, sizeof(T) * K * I);
-  FN(orc_em_q_loop)(q_next, st->bed, st->pitch, FN(qpart)(st, st->x),
-                    FN(ppart)(st, st->x), K, 0, I, 0, J, st->mu, st->nthreads);
+  FN(orc_em_q_loop)(q_next, st->bed, st->pitch, FN(qpart)(st, src), FN(ppart)(st, src), K, 0, I, 0, J, st->mu,
+                    st->nthreads);
   const T div = (T)(2 * J);
   for (int64_t a = 0; a < K * I; ++a) q_next[a] /= div;
 }
 
-/* em_p!(d, g)  (src/algorithms_inner.jl:108-141): p_next = p_tmp / (p_tmp + p_next) */
-void FN(orc_em_p)(FN(orc_state) * st) {
+/* em_p!(d, g; mode)  (src/algorithms_inner.jl:108-141): p_next = p_tmp / (p_tmp + p_next).
+ * mode 0 (:base): p_next from (q, p); 1 (:fallback): p_next from (q_next, p);
+ * 2 (:fallback2): p_next2 from (q_next2, p_next). */
+void FN(orc_em_p_mode)(FN(orc_state) * st, int mode) {
   const int64_t K = st->K, I = st->I, J = st->J;
-  T *p_next = FN(ppart)(st, st->x_next), *p_tmp = FN(ppart)(st, st->x_tmp);
+  T *dst = mode == 2 ? st->x_next2 : st->x_next;
+  const T *q = FN(qpart)(st, mode == 0 ? st->x : (mode == 1 ? st->x_next : st->x_next2));
+  const T *p = FN(ppart)(st, mode == 2 ? st->x_next : st->x);
+  T *p_next = FN(ppart)(st, dst), *p_tmp = FN(ppart)(st, st->x_tmp);
   memset(p_next, 0, sizeof(T) * K * J);
   memset(p_tmp, 0, sizeof(T) * K * J);
-  FN(orc_em_p_loop)(p_next, p_tmp, st->bed, st->pitch, FN(qpart)(st, st->x),
-                    FN(ppart)(st, st->x), K, 0, I, 0, J, st->mu, st->nthreads);
+  FN(orc_em_p_loop)(p_next, p_tmp, st->bed, st->pitch, q, p, K, 0, I, 0, J, st->mu, st->nthreads);
   for (int64_t a = 0; a < K * J; ++a) p_next[a] = p_tmp[a] / (p_tmp[a] + p_next[a]);
 }
+
+void FN(orc_em_q)(FN(orc_state) * st) { FN(orc_em_q_mode)(st, 0); }
+void FN(orc_em_p)(FN(orc_state) * st) { FN(orc_em_p_mode)(st, 0); }
 
 /* init_em!(d, g, iter)  (src/algorithms_outer.jl:12-45); lls[it] = LL after step it */
 void FN(orc_init_em)(FN(orc_state) * st, int64_t iters, double *lls) {

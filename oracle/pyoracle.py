@@ -74,6 +74,8 @@ def _declare(L):
         getattr(L, "orc_update_QN" + s).argtypes = [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, c_i64]
         getattr(L, "orc_em_q" + s).argtypes = [c_vp]
         getattr(L, "orc_em_p" + s).argtypes = [c_vp]
+        getattr(L, "orc_em_q_mode" + s).argtypes = [c_vp, C.c_int]
+        getattr(L, "orc_em_p_mode" + s).argtypes = [c_vp, C.c_int]
         getattr(L, "orc_init_em" + s).argtypes = [c_vp, c_i64, c_vp]
         getattr(L, "orc_update_q" + s).argtypes = [c_vp, C.c_int, c_vp]
         getattr(L, "orc_update_p" + s).argtypes = [c_vp, C.c_int, c_vp]
@@ -340,11 +342,13 @@ class OracleAdmix:
         x = self.x if x is None else x
         return loglikelihood(self.bed, self.pitch, np.asfortranarray(x[:, : self.I]), np.asfortranarray(x[:, self.I:]), nthreads=self.st.nthreads)
 
-    def em_q(self):
-        self._f("orc_em_q")(C.byref(self.st))
+    EM_MODES = {"base": 0, "fallback": 1, "fallback2": 2}
 
-    def em_p(self):
-        self._f("orc_em_p")(C.byref(self.st))
+    def em_q(self, mode="base"):
+        self._f("orc_em_q_mode")(C.byref(self.st), self.EM_MODES[mode])
+
+    def em_p(self, mode="base"):
+        self._f("orc_em_p_mode")(C.byref(self.st), self.EM_MODES[mode])
 
     def init_em(self, iters):
         lls = np.zeros(max(iters, 1))

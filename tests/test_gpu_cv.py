@@ -110,3 +110,22 @@ def test_autotune_picks_a_candidate(oa, O):
     assert w in (4096, 16384, 65536)
     c.close()
     src.close()
+
+
+def test_concurrent_sweep_equals_serial(oa, O):
+    """cv_sweep(workers=4): the K values run at once, each context on its own
+    stream from its own host thread; every K's CV error, fold deviances, full
+    LL and iteration counts are those of the serial sweep, bit for bit (the
+    autotuner runs alone first, so the chunking is the same)."""
+    print("cv sweep concurrent", flush=True)
+    I, J, Ktrue = 700, 5000, 3
+    src, *_ = _src(oa, O, I, J, Ktrue, seed=14, alpha=0.5)
+    kw = dict(nfolds=3, fold_seed=2, max_iter=200)
+    serial = oa.cv_sweep(src, [2, 3, 4, 5], **kw)
+    conc = oa.cv_sweep(src, [2, 3, 4, 5], workers=4, **kw)
+    for a, b in zip(serial, conc):
+        print(a["K"], a["cv_error"], b["cv_error"], a["iterations_full"], b["iterations_full"], flush=True)
+        assert a["K"] == b["K"]
+        for key in ("cv_error", "ll_full", "iterations_full", "iterations_folds", "fold_deviance", "waves"):
+            assert a[key] == b[key], (a["K"], key, a[key], b[key])
+    src.close()

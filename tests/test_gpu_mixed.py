@@ -193,3 +193,73 @@ def test_matrix_core_em_priming_lls(oa, O):
     print("per-step LL relative error", rel, flush=True)
     assert rel.max() <= 1e-6
     c.close()
+
+
+def _ll_class(v):
+    return "nan" if np.isnan(v) else ("-inf" if v == -np.inf else "finite")
+
+
+def test_bf16_loglikelihood_degenerate_states(oa, O):
+    """BF16 form: states where some s = q.p is exactly 0 or 1 (or p is NaN)
+    are flagged by the state check and summed term by term in FP64, so the
+    matrix-core log-likelihood gives the reference's NaN / -Inf
+    (src/loops.jl:517-539) where the oracle does; a safe state stays on the
+    matrix cores (1e-6), a flagged finite one matches to 1e-10."""
+    print("bf16 degenerate LL", flush=True)
+    I, J, K = 300, 64, 3
+    _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.02, seed=61)
+    B = bed.reshape((pitch, J), order="F").copy()
+    B[: I // 4, 20] = 0xAA  # SNP 20: every individual heterozygous (code 10), none missing
+    B[: I // 4, 21] = 0x00  # SNP 21: monomorphic, every genotype 0
+    bed2 = np.asfortranarray(B).ravel(order="F")
+    c = oa.HIPAdmixData(I, J, K, 3)
+    c.upload_bed(bed2.reshape((pitch, J), order="F"))
+    c.set_precision("bf16")
+    rng = np.random.default_rng(7)
+    q = rng.random((K, I)) + 0.05
+    q = np.asfortranarray(q / q.sum(0))
+    p = np.asfortranarray(rng.uniform(0.02, 0.98, size=(K, J)))
+    cases = {"plain": (p, 1e-6)}
+    for name, (col, val) in {"zero_p_mixed_snp": (5, 0.0), "zero_p_het_snp": (20, 0.0),
+                             "zero_p_mono_snp": (21, 0.0), "one_p": (7, 1.0),
+                             "nan_p": (9, np.nan), "tiny_p": (11, 1e-13)}.items():
+        pp = p.copy()
+        pp[:, col] = val
+        cases[name] = (np.asfortranarray(pp), 1e-10)
+    seen = set()
+    for name, (pp, tol) in cases.items():
+        ref = O.loglikelihood(bed2, pitch, q, pp)
+        c.set_x(0, np.asfortranarray(np.hstack([q, pp])))
+        dev = c.loglikelihood(0)
+        print(name, ref, dev, flush=True)
+        assert _ll_class(dev) == _ll_class(ref), (name, dev, ref)
+        if np.isfinite(ref):
+            assert abs(dev - ref) <= tol * abs(ref), (name, dev, ref)
+        seen.add(_ll_class(ref))
+    assert seen == {"nan", "-inf", "finite"}
+    c.close()
+
+
+def test_bf16_em_on_a_monomorphic_snp_matches_reference(oa, O):
+    """BF16 form, FRAPPE EM from a random start on a cohort with a SNP whose
+    genotypes are all 0: after the first step p of that SNP is exactly 0; the
+    state check sends the following EM steps and log-likelihoods to the FP64
+    passes, so the priming log-likelihoods are NaN where the oracle's are."""
+    print("bf16 EM monomorphic", flush=True)
+    I, J, K = 200, 300, 3
+    _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.0, seed=62)
+    B = bed.reshape((pitch, J), order="F").copy()
+    B[: I // 4, 40] = 0x00
+    bed2 = np.asfortranarray(B).ravel(order="F")
+    d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(5))
+    c = oa.HIPAdmixData(I, J, K, 3)
+    c.upload_bed(bed2.reshape((pitch, J), order="F"))
+    c.set_precision("bf16")
+    ref = O.OracleAdmix(bed2, pitch, I, J, K, 3, d.x)
+    lls = oa.init_em(d, None, 3, d_cu=c, log=None)
+    rlls = ref.init_em(3)
+    print(lls, rlls, flush=True)
+    assert [_ll_class(v) for v in lls] == [_ll_class(v) for v in rlls]
+    assert _ll_class(rlls[0]) == "nan"
+    assert np.array_equal(np.isnan(d.x), np.isnan(ref.x))
+    c.close()

@@ -76,7 +76,7 @@ def test_match_components_recovers_permutation(oa):
     assert np.array_equal(perm, perm_true) and corr.min() > 0.5
 
 
-def _scripted(oa, ll_warm, ll_em):
+def _scripted(oa, ll_warm, ll_em, ll_hw):
     """Stand-in device context that records the decision path of warm_started_fit."""
     from openadmixture_jl_amd.hip import HIPAdmixData
 
@@ -133,7 +133,7 @@ def _scripted(oa, ll_warm, ll_em):
             return self.ll
 
         def loglikelihood(self, which=0):
-            return ll_warm - 1.0
+            return ll_hw  # the state right after warm_p: p_k = f for every k
 
         def qn_iteration(self, it, fix_q=False, fix_p=False):
             if fix_q:
@@ -155,14 +155,18 @@ class _Net:
     out_dim = 2
 
 
-@pytest.mark.parametrize("ll_warm,ll_em,chosen", [(-10.0, -20.0, "warm"), (-30.0, -20.0, "em")])
-def test_judge_keeps_the_better_start(oa, ll_warm, ll_em, chosen):
+# (K - 1) J = 6 for the scripted cohort: the proposal must gain more than 6
+# over the Hardy-Weinberg log-likelihood and beat one EM step
+@pytest.mark.parametrize("ll_warm,ll_em,ll_hw,chosen", [(-10.0, -20.0, -100.0, "warm"), (-30.0, -20.0, -100.0, "em"),
+                                                        (-10.0, -20.0, -15.0, "em")])
+def test_judge_keeps_the_better_start(oa, ll_warm, ll_em, ll_hw, chosen):
     from openadmixture_jl_amd.warmstart import Panel, warm_started_fit
 
-    ctx = _scripted(oa, ll_warm, ll_em)
+    ctx = _scripted(oa, ll_warm, ll_em, ll_hw)
     panel = Panel(np.full(6, 0.3), seed=0)
     r = warm_started_fit(ctx, _Net(), panel, seed=1, max_iter=3, p_steps=2)
     assert r["chosen"] == chosen
+    assert r["ll_hw"] == ll_hw and r["gain"] == (ll_warm - ll_hw) / 6
     if chosen == "warm":
         assert ctx.calls == ["features", "warm_q", "warm_p", "em1", "qn"]
         assert np.all(ctx.x[:, :4] == 0.9)  # the proposal is restored after the judge's EM step
