@@ -291,8 +291,10 @@ def run_warmstart(args):
 
     rng = np.random.default_rng(args.seed + 99)
     Qt = np.asfortranarray(rng.dirichlet(wl["alpha"] * np.ones(K), size=I).T)
+    # the evaluation cohort's populations: drifts (F = --warm-drift) of the panel's
+    P_eval = P if args.warm_drift <= 0 else oa.drifted_populations(P, args.warm_drift, args.seed + 17)
     ctx = oa.HIPAdmixData(I, J, K, 3)
-    ctx.generate_bed(Qt, P, args.seed + 5, wl["miss_rate"])
+    ctx.generate_bed(Qt, P_eval, args.seed + 5, wl["miss_rate"])
     tol = 1e-4
     runs = {}
 
@@ -311,29 +313,30 @@ def run_warmstart(args):
                 break
             if stop_at and abs(r["new"] - stop_at) <= tol * abs(stop_at):
                 break
-        return h, None
+        return h, None, None
 
     def warm(form, stop_at):
         ctx.set_precision(form)
         r = oa.warm_started_fit(ctx, net, panel, seed=args.seed + 1, max_iter=args.ttc_max_iter, **_warm_kw(args),
                                 stop_at=(stop_at, tol) if stop_at else None)
-        return r["history"], r["chosen"]
+        return r["history"], r["chosen"], r["gain"]
 
-    h, _ = plain("f64", None)
+    h, _, _ = plain("f64", None)
     ll_star = h[-1][1]
-    runs["plain_f64"] = (h, None)
+    runs["plain_f64"] = (h, None, None)
     runs["bf16"] = plain("bf16", ll_star)
     runs["warm_f64"] = warm("f64", ll_star)
     runs["warm_bf16"] = warm("bf16", ll_star)
     ctx.close()
     net.close()
     res = {}
-    for name, (h, chosen) in runs.items():
+    for name, (h, chosen, _) in runs.items():
         hit = first_within(h, ll_star, tol)
         res[name] = {"seconds": hit[1] if hit else None, "qn_iterations": hit[0] if hit else None,
                      "start_seconds": h[0][0], "start_ll": h[0][1], "ll_last": h[-1][1]}
         if chosen:
             res[name]["chosen_start"] = chosen
+            res[name]["judge_gain"] = runs[name][2]
     base = res["plain_f64"]["seconds"]
     for name in res:
         if res[name]["seconds"]:
@@ -342,7 +345,8 @@ def run_warmstart(args):
         "metric": "time to |LL - LL*| <= 1e-4 |LL*| with the learned warm start (LL* = plain FP64 final LL)",
         "value": res["warm_bf16"]["seconds"], "unit": "s", "higher_is_better": False, "n_gpus": 1,
         "dtype": "bf16 passes, f64 LL/QP; f32 network",
-        "data": "synthetic: 10-population Balding-Nichols world (F=0.05), Dirichlet(alpha) admixture",
+        "data": "synthetic: 10-population Balding-Nichols world (F=0.05), Dirichlet(alpha) admixture; evaluation "
+                f"cohort from drifted copies of the panel populations (F={args.warm_drift})",
         "config": {"workload": f"{args.workload}: eval I={I}, J={J}, K={K}; train {wl['train_cohorts']} cohorts "
                                f"I={wl['train_I']}, J={wl['train_J']}, K=4..10, alpha in [0.1, 1]",
                    "I": I, "J": J, "K": K},
@@ -468,6 +472,9 @@ def main():
     ap.add_argument("--fold-start", default="full", choices=["full", "random"],
                     help="CV fold fits start from the full fit's estimates or from their own random start")
     ap.add_argument("--train-epochs", type=int, default=20, help="network training epochs (workload c5)")
+    ap.add_argument("--warm-drift", type=float, default=0.05,
+                    help="F of the Balding-Nichols drift between the warm start's panel populations and the "
+                         "evaluation cohort's (0: the same populations)")
     ap.add_argument("--warm-p-steps", type=int, default=None,
                     help="fixed-Q P refinement iterations of the warm start (default: the library's)")
     ap.add_argument("--K", type=int, default=0, help="override the workload's K (throughput runs)")
@@ -619,14 +626,21 @@ def main():
             # first 20,000 SNPs), each solved by the plain method
             tw = time.perf_counter()
             Jp = min(20_000, J)
-            panel = oa.Panel(Pt_true[:, :Jp].mean(0), seed=args.seed)
-            X, Tt, _ = oa.training_set(np.asfortranarray(Pt_true), panel, 10, I=5_000, J=Jp, Ks=range(min(4, K), K + 1),
+            # panel populations: Balding-Nichols drifts (F = --warm-drift) of the cohort's
+            # populations, so the network never sees the evaluation cohort's own
+            # allele frequencies (0: the cohort's populations themselves)
+            P_panel = (np.asfortranarray(Pt_true) if args.warm_drift <= 0 else
+                       oa.drifted_populations(Pt_true, args.warm_drift, args.seed + 17))
+            panel = oa.Panel(P_panel[:, :Jp].mean(0), seed=args.seed)
+            X, Tt, _ = oa.training_set(P_panel, panel, 10, I=5_000, J=Jp, Ks=range(min(4, K), K + 1),
                                        alpha_range=(0.1, 1.0), miss_rate=wl["miss_rate"], seed=args.seed)
             t_data = time.perf_counter() - tw
             net = oa.MLP(64, 256, K, seed=args.seed)
             tw = time.perf_counter()
             lh = net.train(X, Tt, epochs=20, batch=512, lr=2e-3, seed=args.seed)
             ttc["warm_start_training"] = {
+                "panel": (f"Balding-Nichols drifts (F = {args.warm_drift}) of the cohort's populations"
+                          if args.warm_drift > 0 else "the cohort's own populations"),
                 "rows": int(X.shape[0]), "cohorts": 10, "simulate_and_solve_seconds": t_data,
                 "train_seconds": time.perf_counter() - tw, "loss_first": float(lh[0]), "loss_last": float(lh[-1]),
                 "note": "one-time cost, not part of any time to converge"}
@@ -647,6 +661,7 @@ def main():
                           "ll_last": h2[-1][1]})
             if form.startswith("warm"):
                 ttc[form]["chosen_start"] = r["chosen"]
+                ttc[form]["judge_gain"] = r["gain"]
         if net is not None:
             net.close()
         ctx.set_precision("f64" if args.form == "plain" else args.form)

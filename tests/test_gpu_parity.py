@@ -448,3 +448,51 @@ def test_em_on_a_monomorphic_snp_matches_reference(oa, O):
     assert [_ll_class(v) for v in lls] == [_ll_class(v) for v in rlls]
     assert _ll_class(rlls[0]) == "nan"
     c.close()
+
+
+@pytest.mark.parametrize("K", [9, 12])
+def test_hessian_split_switch_is_exact(oa, O, K):
+    """ADMX_TUNE_SPLIT_Q / _SPLIT_P: the FP64 update passes with the Hessian
+    triangle in one launch or in two (each launch owning half of the
+    accumulators) give the same XtX, Xtz, QP steps and projected iterates bit
+    for bit, and match the oracle."""
+    from openadmixture_jl_amd import _native as N
+
+    note("hessian split", K)
+    I, J = 500, 1500
+    d, c, ref = _pair(oa, O, I, J, K, seed=K + 40)
+    ref.init_em(2)
+    x = np.asfortranarray(ref.x.copy())
+    out = {}
+    for split in (0, 1):
+        c.set_tuning(N.ADMX_TUNE_SPLIT_Q, split)
+        c.set_tuning(N.ADMX_TUNE_SPLIT_P, split)
+        c.set_x(0, x)
+        c.update_q(False)
+        hq = c.hessians(0)
+        c.set_x(1, x)
+        c.update_p(False)
+        hp = c.hessians(1)
+        out[split] = (hq, hp, c.get_x(1))
+    for a, b in zip(out[0][0] + out[0][1], out[1][0] + out[1][1]):
+        assert np.array_equal(a, b)
+    assert np.array_equal(out[0][2], out[1][2])
+    ref.update_q(False)
+    ok, e = _colmax_close(out[1][0][0], ref.XtX_q, 1e-9)
+    assert ok, e
+    c.close()
+
+
+def test_autotune_sets_waves_and_split(oa, O):
+    """autotune() picks a waves candidate and, at K >= 9, a Hessian split per
+    side, sets them on the context and caches them per shape."""
+    from openadmixture_jl_amd import _native as N
+
+    note("autotune")
+    I, J, K = 700, 1600, 10
+    d, c, ref = _pair(oa, O, I, J, K, seed=3)
+    c.set_x(0, d.x)
+    t = oa.autotune(c, candidates=(8192, 65536), reps=1)
+    assert t["waves"] in (8192, 65536) and t["split_q"] in (0, 1) and t["split_p"] in (0, 1)
+    assert oa.autotune(c, candidates=(8192, 65536), reps=1) == t
+    c.close()

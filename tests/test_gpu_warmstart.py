@@ -155,3 +155,43 @@ def test_warm_p_and_state_copy(oa, O):
     c.copy_x(3, 0)
     assert np.array_equal(c.get_x(3), x)
     c.close()
+
+
+def test_mismatched_network_is_rejected(oa, O):
+    """A real network trained on one world's populations, used on a cohort of
+    unrelated populations (another Balding-Nichols world): its proposal beats
+    one EM step from the random start but explains the genotypes no better than
+    chance beyond Hardy-Weinberg (gain below (K-1)J), so the judge falls back
+    to the random start + EM, and the fit reaches the plain method's LL.  The
+    same network on a cohort of drifted copies of its populations (F = 0.05)
+    is kept and saves iterations."""
+    print("warm start mismatch", flush=True)
+    Kw, Jp, J, Ie = 5, 4000, 8000, 6000
+    P, _ = oa.world_frequencies(Kw, J, fst=0.05, seed=11)
+    panel = oa.Panel(P[:, :Jp].mean(0), seed=3)
+    X, T, _ = oa.training_set(P, panel, n_cohorts=8, I=1500, J=Jp, Ks=[2, 3, 4, 5], seed=5)
+    net = oa.MLP(64, 256, Kw, seed=1)
+    net.train(X, T, epochs=30, batch=256, lr=2e-3, seed=1)
+    rng = np.random.default_rng(21)
+    Qt = np.asfortranarray(rng.dirichlet(0.5 * np.ones(Kw), size=Ie).T)
+    worlds = {"unrelated": oa.world_frequencies(Kw, J, fst=0.05, seed=99)[0],
+              "drifted": oa.drifted_populations(P, 0.05, 8)}
+    for name, Pe in worlds.items():
+        c = oa.HIPAdmixData(Ie, J, Kw, 3)
+        c.generate_bed(Qt, np.asfortranarray(Pe), 77, 0.01)
+        plain = oa.fit_on_device(c, Kw, 5, max_iter=500)
+        ll_star = plain.llhist[-1]
+        r = oa.warm_started_fit(c, net, panel, seed=5, max_iter=500)
+        c.close()
+        lls = [ll for _, ll in r["history"]]
+        n_plain = next(n for n, ll in enumerate(plain.llhist) if abs(ll - ll_star) <= 1e-4 * abs(ll_star))
+        n_warm = next(n for n, ll in enumerate(lls) if abs(ll - ll_star) <= 1e-4 * abs(ll_star))
+        print(f"{name}: chosen {r['chosen']}, gain {r['gain']:.2f} (K-1)J, LL warm start {r['ll_warm']:.6e} vs one "
+              f"EM step {r['ll_em']:.6e}; iterations to 1e-4 {n_warm} vs plain {n_plain}", flush=True)
+        assert abs(lls[-1] - ll_star) <= 1e-6 * abs(ll_star)
+        if name == "unrelated":
+            assert r["chosen"] == "em" and r["ll_warm"] > r["ll_em"] and r["gain"] < 1.0
+            assert n_warm == n_plain
+        else:
+            assert r["chosen"] == "warm" and n_warm < n_plain
+    net.close()
