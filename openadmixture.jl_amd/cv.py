@@ -54,8 +54,11 @@ def fit_from(ctx: HIPAdmixData, x, seed: int, max_iter: int = 1000, rtol: float 
 
 def cv_sweep(src: HIPAdmixData, Ks, nfolds: int = 5, fold_seed: int = 0, init_seed: int = 1, em_iters: int = 5,
              max_iter: int = 1000, rtol: float = 1e-7, precision: str = "f64", Q: int = 3, autotune: bool = True,
-             fold_start: str = "full", log=None, workers: int = 1):
+             fold_start: str = "full", log=None, workers: int = 1, graphs: bool = True):
     """Fit K in Ks on src's genotypes, each with nfolds holdout problems.
+
+    graphs: steady-state quasi-Newton iterations replay captured hipGraphs
+    (ADMX_TUNE_GRAPHS; bit-identical to eager iterations).
 
     workers > 1 runs that many K values at once, each in its own context (own
     HIP stream) driven by its own host thread; the C ABI releases the
@@ -72,7 +75,8 @@ def cv_sweep(src: HIPAdmixData, Ks, nfolds: int = 5, fold_seed: int = 0, init_se
     if fold_start not in ("full", "random"):
         raise ValueError("fold_start must be 'full' or 'random'")
     Ks = [int(k) for k in Ks]
-    args = (src, nfolds, fold_seed, init_seed, em_iters, max_iter, rtol, precision, Q, autotune, fold_start, log)
+    args = (src, nfolds, fold_seed, init_seed, em_iters, max_iter, rtol, precision, Q, autotune, fold_start, log,
+            graphs)
     if workers > 1 and len(Ks) > 1:
         from concurrent.futures import ThreadPoolExecutor
 
@@ -92,12 +96,13 @@ def cv_sweep(src: HIPAdmixData, Ks, nfolds: int = 5, fold_seed: int = 0, init_se
 
 
 def _sweep_one(K, src, nfolds, fold_seed, init_seed, em_iters, max_iter, rtol, precision, Q, tune, fold_start,
-               log):
+               log, graphs=True):
     """The full fit and the nfolds holdout fits of one K (one context)."""
     t0 = time.perf_counter()
     ctx = HIPAdmixData(src.I, src.J, int(K), Q, src.T)
     try:
         ctx.set_precision(precision)
+        ctx.set_graphs(graphs)
         ctx.bed_from(src, -1)
         tuning = None
         if tune:
