@@ -485,6 +485,8 @@ def main():
     ap.add_argument("--target-waves", type=int, default=0, help="waves per genotype pass (0: library default)")
     ap.add_argument("--no-ttc", action="store_true", help="skip the time-to-converge runs")
     ap.add_argument("--graphs", action="store_true", help="replay steady-state iterations from hipGraphs")
+    ap.add_argument("--autotune", action="store_true",
+                    help="pick waves per pass and the Hessian split with the autotuner before timing")
     ap.add_argument("--stream-snps", type=int, default=0,
                     help="keep the BED in pinned host memory and stream it in chunks of this many SNPs")
     ap.add_argument("--stream-from-file", action="store_true",
@@ -571,6 +573,9 @@ def main():
     d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(args.seed + 1), T=T)
     oa.init_em(d, None, 5, d_cu=ctx, log=None)
     ctx.set_uv(d.U, d.V)
+    tuning = None
+    if args.autotune:
+        tuning = oa.autotune(ctx)  # on the primed state; writes x_next only
     it = 0
     for _ in range(args.warmup):
         it += 1
@@ -633,6 +638,7 @@ def main():
         "kernels": kernels,
         "ll_last": lls[-1],
         "gen_seconds": t_gen,
+        "tuning": tuning,
     }
     if not args.no_ttc and args.dtype == "f64":
         tol = 1e-4
