@@ -68,6 +68,11 @@ extern "C" {
                                    * of half the accumulators each (higher occupancy, the genotypes read
                                    * twice), 0 in one; -1 (default): two from K = 11.  Same results. */
 #define ADMX_TUNE_SPLIT_P 8       /* the same for the FP64 update_p pass; -1: two from K = 13 */
+#define ADMX_TUNE_QSIDE_MFMA 9    /* BF16 form's update_q pass: 1 forms s, the Hessian and the gradient on the
+                                   * matrix cores (gradient from two-term BF16 products), 0 the Hessian only
+                                   * (s and the gradient in FP32 on the vector ALU); -1 (default): 1 from
+                                   * K = 7 (faster there, slower below).  Both within the BF16 form's error
+                                   * bars against the FP64 oracle. */
 
 /* values of ADMX_TUNE_QN_MODE: admixture_qn!'s `mode` (src/algorithms_outer.jl:127-136) */
 #define ADMX_QN_ZAL 0

@@ -478,12 +478,59 @@ int ll2_to_slots(admx_ctx* c, void* x1, int slot1, void* x2, int slot2) {
   return check_launch();
 }
 
+// Measured at I = J = 100,000: the matrix-core update_q pass is faster from
+// K = 7 (7.3 vs 7.5 ms; 7.9 vs 9.2 at K = 8, 13.8 vs 46.6 at K = 16) and slower
+// below (6.9 vs 6.0 ms at K = 4), where s and the gradient are few FMAs.
+bool qside_on_matrix_cores(const admx_ctx* c) {
+  return c->qside_mfma == 1 || (c->qside_mfma == -1 && c->K >= 7);
+}
+
+// BF16 form, Q side: s, t, the Hessian and the gradient on the matrix cores
+// (the split table of q, one image slot per 32-SNP block of p), then the FP64
+// finalize + QP + projection of the other mixed passes.
+int mixed_update_q_mfma(admx_ctx* c, const double* q, const double* p, const double* x_in, double* x_out,
+                        int32_t* status_dev) {
+  const int64_t I = c->I, J = c->J, K = c->K;
+  const int KD = split_table_cols((int)K);
+  const int64_t rows_q = (I + 255) / 256 * 256, nblk = (J + 31) / 32;
+  CHK(c->lqt.grow(sizeof(uint16_t) * KD * rows_q));
+  CHK(c->qimg.grow((size_t)(nblk + 1) * qimg_bytes((int)K)));
+  const int64_t nbx = (I + 127) / 128;  // 128 individuals (4 waves) per workgroup
+  int64_t nchunk = std::max<int64_t>(1, (c->target_waves / 4 + nbx - 1) / nbx);
+  nchunk = std::min<int64_t>(nchunk, nblk);
+  const int64_t jchunk = ((J + nchunk - 1) / nchunk + 31) / 32 * 32;
+  nchunk = (J + jchunk - 1) / jchunk;
+  const int na = (int)(K * (K + 1) / 2 + K);
+  CHK(c->part.grow((size_t)nchunk * na * I * sizeof(float)));
+  {
+    ProfScope ps(c, "mixed_prep");
+    launch_split_tables((int)K, q, I, rows_q, 0, c->lqt.p, nullptr, c->s);
+    launch_qimg((int)K, p, J, c->qimg.p, c->s);
+  }
+  CHK(check_launch());
+  {
+    ProfScope ps(c, "qside_update_bf16");
+    launch_qside_mfma((int)K, c->bed, c->pitch, I, J, jchunk, (int)nchunk, c->lqt.p, c->qimg.p,
+                      static_cast<float*>(c->part.p), c->s);
+  }
+  CHK(check_launch());
+  FinArgs f{c->part.p, I, (int)nchunk, 0, J, x_in, x_out, nullptr, c->keep_hess ? c->XtX_q : nullptr,
+            c->keep_hess ? c->Xtz_q : nullptr, 0, 1, status_dev, c->XtX_q, c->Xtz_q};
+  {
+    ProfScope ps(c, "fin_q_update_qp");
+    launch_fin_q_update_mixed((int)K, f, c->s);
+  }
+  return check_launch();
+}
+
 // Mixed-precision update passes of a FP64 context: FP32 copies of (q, p), the
 // BF16 pair tables, the FP32/BF16 pass, then the FP64 finalize + QP + projection.
 int mixed_update(admx_ctx* c, bool qside, const double* q, const double* p, const double* x_in, double* x_out,
                  int32_t* status_dev) {
   if (c->impute) return bad("the mean-imputation loops run with the FP64/FP32 passes (ADMX_PREC_F64)");
   CHK(need_resident(c, "the mixed-precision passes"));
+  if (qside && c->precision == ADMX_PREC_BF16 && qside_on_matrix_cores(c))
+    return mixed_update_q_mfma(c, q, p, x_in, x_out, status_dev);
   const int64_t I = c->I, J = c->J, K = c->K;
   if (c->fq.n < sizeof(float) * K * (I + 16)) {
     CHK(c->fq.grow(sizeof(float) * K * (I + 16)));
@@ -771,7 +818,7 @@ uint64_t layout_sig(const admx_ctx* c) {
   };
   const DevBuf* bufs[] = {&c->part,    &c->llpart,  &c->zal,     &c->fq,  &c->fp,  &c->ppt,
                           &c->mmt,     &c->qqt,     &c->wide[0], &c->wide[1], &c->wide[2], &c->wide[3],
-                          &c->llflag,  &c->lqt,     &c->lpt,     &c->lmt, &c->lps, &c->lqc};
+                          &c->llflag,  &c->lqt,     &c->lpt,     &c->lmt, &c->lps, &c->lqc, &c->qimg};
   for (const DevBuf* b : bufs) {
     mix((uint64_t)(uintptr_t)b->p);
     mix((uint64_t)b->n);
@@ -783,6 +830,7 @@ uint64_t layout_sig(const admx_ctx* c) {
   mix((uint64_t)c->impute);
   mix((uint64_t)(c->split_q + 2));
   mix((uint64_t)(c->split_p + 2));
+  mix((uint64_t)(c->qside_mfma + 2));
   mix((uint64_t)(uintptr_t)c->snp_mu.p);
   return h;
 }
@@ -1239,6 +1287,9 @@ int admx_destroy(admx_ctx* c) {
     c->lqt.release();
     c->lpt.release();
     c->lmt.release();
+    c->lps.release();
+    c->lqc.release();
+    c->qimg.release();
     c->feat.release();
     c->featpart.release();
     c->snp_mu.release();
@@ -1892,6 +1943,9 @@ int admx_set_tuning(admx_ctx* c, int key, int64_t value) {
     } else if (key == ADMX_TUNE_SPLIT_Q || key == ADMX_TUNE_SPLIT_P) {
       if (value < -1 || value > 1) return bad("Hessian split must be -1 (default), 0 or 1");
       (key == ADMX_TUNE_SPLIT_Q ? c->split_q : c->split_p) = (int)value;
+    } else if (key == ADMX_TUNE_QSIDE_MFMA) {
+      if (value < -1 || value > 1) return bad("matrix-core update_q must be -1 (default), 0 or 1");
+      c->qside_mfma = (int)value;
     } else if (key == ADMX_TUNE_SKIPMISSING) {
       c->impute = value == 0;
     } else if (key == ADMX_TUNE_QN_MODE) {

@@ -119,6 +119,16 @@ void launch_emp_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_
                      const float* qc, const void* QT, const void* PT, const void* MT, const void* QS, int64_t ldqs,
                      double* part, hipStream_t s);
 
+// BF16 form's update_q pass with s, t, the Hessian and the gradient on the
+// matrix cores: ceil(J/32) + 1 image slots (qimg_bytes(K) bytes each, built
+// from p by launch_qimg) of the SNP-side operands in LDS layout; QT is the individual
+// split table of launch_split_tables (role 0).  jchunk: a multiple of 32.
+// Partials [chunk][K(K+1)/2 + K][I] in FP32, as launch_qside_bf16 writes them.
+int qimg_bytes(int K);
+void launch_qimg(int K, const double* p, int64_t J, void* img, hipStream_t s);
+void launch_qside_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
+                       const void* QT, const void* img, float* part, hipStream_t s);
+
 // Batched QP on host-provided tableau inputs (XtX, negated Xtz, x): delta out.
 template <typename T> void launch_qp_batch(int K, bool simplex, int64_t n, const void* XtX, const void* Xtz_neg,
                                            const void* x, void* delta, int32_t* status, hipStream_t s);

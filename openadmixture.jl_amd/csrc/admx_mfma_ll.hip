@@ -20,29 +20,6 @@ namespace admx {
 // s is close to 1; t then takes 1 - sum_k q_ik on top, so it is 1 - s also when
 // q's column does not sum to 1.  The dropped products and the FP32
 // accumulation leave s and t with relative errors of a few 1e-7.
-template <int K> struct SplitDim {
-  static constexpr int KD = ((6 * K + 15) / 16) * 16;
-  static constexpr int NM = KD / 16;
-};
-
-__device__ __forceinline__ void split3(double v, __bf16& h1, __bf16& h2, __bf16& h3) {
-  h1 = (__bf16)(float)v;
-  const double r1 = v - (double)(float)h1;
-  h2 = (__bf16)(float)r1;
-  const double r2 = r1 - (double)(float)h2;
-  h3 = (__bf16)(float)r2;
-}
-
-// 1 - sum_k q_ik in FP64, rounded to FP32: q.(1-p) + (1 - sum q) = 1 - q.p is the
-// reference's t also where the columns of q do not sum to 1 (after EM steps,
-// whose q sums to the observed fraction, src/algorithms_inner.jl:76-89).
-template <int K>
-__device__ __forceinline__ float sum_complement(const double* __restrict__ q, int64_t i) {
-  double sq = 0.0;
-#pragma unroll
-  for (int k = 0; k < K; ++k) sq += q[(int64_t)K * i + k];
-  return (float)(1.0 - sq);
-}
 
 // Individual table (role 0) or SNP tables (role 1: T1 from p, T2 from 1 - p),
 // `rows` rows of KD BF16 each, zero past column n.
@@ -299,16 +276,6 @@ __device__ __forceinline__ void em_weights(float gm, float hm, float sv, float t
   dd = hm * (sv * r);
 }
 
-// Leading and residual BF16 of two FP32 weights, packed.
-__device__ __forceinline__ void split_pack(float w0, float w1, uint32_t& hi, uint32_t& lo) {
-  hi = pack_bf16x2(w0, w1);
-  const float r0 = w0 - __builtin_bit_cast(float, hi << 16), r1 = w1 - __builtin_bit_cast(float, hi & 0xffff0000u);
-  lo = pack_bf16x2(r0, r1);
-}
-
-__device__ __forceinline__ bf16x8 b8(const uint32_t* w) {
-  return __builtin_bit_cast(bf16x8, (u32x4){w[0], w[1], w[2], w[3]});
-}
 
 // em_q: the wave and tile layout of k_ll_mfma (64 individuals, 32-SNP tiles);
 // partials [chunk][K+1][I]: sum c p_k + d (1 - p_k) (k < K) and 0 (the FP64

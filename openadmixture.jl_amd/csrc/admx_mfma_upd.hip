@@ -1,0 +1,386 @@
+// admx_mfma_upd.hip -- the BF16 form's update_q! pass (src/loops.jl:867-894)
+// with every per-SNP product on the matrix cores:
+//
+//   s = q.p                                    three-term BF16 splits, as in the
+//                                              matrix-core log-likelihood
+//                                              (v_mfma_f32_32x32x16_bf16), t = 1 - s
+//   XtX_q = sum_j a_ij pp_j + b_ij mm_j        a = g/s^2, b = (2-g)/t^2 in BF16,
+//                                              pp, mm the BF16 pair products
+//   Xtz_q = sum_j c_ij p_j + d_ij (1-p_j)      c = g/s, d = (2-g)/t split into a
+//                                              leading and a residual BF16 term,
+//                                              p and 1-p into two-row tables
+//
+// (the Hessian and, for K <= 8, the gradient on v_mfma_f32_16x16x32_bf16),
+// leaving the VALU the per-genotype weights only (about 20 instructions per
+// genotype instead of 33 in k_qside_bf16, whose s and gradient are FP32 FMAs).
+// t = 1 - s in FP32, as in k_qside_bf16: the reference's t is 1 - s.
+//
+// A wave owns 32 individuals and a workgroup (4 waves) 128; the workgroup walks
+// its SNP chunk in blocks of 32.  Every block's SNP-side operands (split rows
+// of p, the pair tables, the gradient tables) are one contiguous image in HBM,
+// built per pass by k_qimg in exactly the LDS layout the MFMA operand loads
+// read, so the workgroup copies it with global_load_lds_dwordx4 (no registers,
+// no LDS writes) into one of two LDS buffers while it computes on the other;
+// one barrier per block publishes the copy.  Each wave reads the block's
+// operands from LDS (16 bytes per lane per operand), instead of every wave
+// re-reading them from L2.
+//
+// Tile layouts (checked on gfx950):
+//   32x32x16: A[m][k] in lane m + 32 (k/8), element k%8;  B[k][n] in lane
+//             n + 32 (k/8);  D[m][n] in lane n + 32 ((m/4)%2), register
+//             (m%4) + 4 (m/8).
+//   16x16x32: A[m][k] in lane m + 16 (k/8), element k%8;  B[k][n] in lane
+//             n + 16 (k/8);  D[m][n] in lane n + 16 (m/4), register m%4.
+// s is D[j][i] = sum A[j][.] (SNP split rows) B[.][i] (individual split rows):
+// lane i + 32h holds individual i and SNPs 8a + 4h + b in register 4a + b, and
+// its packed weights of registers 8 half .. 8 half + 7 (SNPs 16 half ..
+// 16 half + 15) are 8 BF16 per register quadruple, element e <-> SNP
+// 16 half + 8 (e/4) + 4h + e%4.  v_permlane16_swap of two such quadruples X, Y
+// (rows of 16 lanes: X' = [X0 Y0 X2 Y2], Y' = [X1 Y1 X3 Y3]) gives, for the
+// individuals 0-15 (X') and 16-31 (Y') of the wave, the 16x16x32 operand whose
+// reduction groups k/8 = 0..3 are (X, h = 0), (Y, h = 0), (X, h = 1), (Y, h = 1):
+// X, Y = a, b for the Hessian, c_hi, c_lo (d_hi, d_lo) for the gradient.  The
+// pair and gradient tables store their 32 SNP columns in that order, so each
+// table operand is one 16-byte LDS read.
+#include "admx_internal.hpp"
+#include "admx_mixed.hpp"
+
+namespace admx {
+
+// 16 bytes per lane from global memory straight into LDS (global_load_lds_dwordx4):
+// lane l's bytes land at dst + 16 l, dst uniform across the wave.
+__device__ __forceinline__ void copy_to_lds16(const uint8_t* src, uint8_t* dst) {
+  __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)dst, 16, 0, 0);
+}
+
+// Byte layout of one 32-SNP block image (and of each LDS buffer).
+template <int K> struct QImg {
+  static constexpr int KD = SplitDim<K>::KD, NM = SplitDim<K>::NM;
+  static constexpr int NP = K * (K + 1) / 2, NT16 = (NP + 15) / 16;  // 16-pair Hessian tiles
+  // Rows are padded by 16 bytes so that the 16-byte reads of lanes 0-15 fall
+  // on 16 different bank groups.
+  static constexpr int RS = 2 * KD + 16;  // split row: KD BF16
+  static constexpr int RT = 80;           // pair / gradient row: 32 BF16
+  static constexpr int SP = 0;            // 32 split rows of p
+  static constexpr int PP = SP + 32 * RS; // NP pair rows p_a p_b
+  static constexpr int MM = PP + NP * RT; // NP pair rows (1-p_a)(1-p_b)
+  static constexpr int GP = MM + NP * RT; // 2K rows: bf16(p_k), then the residual p_k - bf16(p_k)
+  static constexpr int GM = GP + 2 * K * RT;  // the same for 1 - p_k
+  static constexpr int ZR = GM + 2 * K * RT;  // one zero row (padding pairs and gradient rows)
+  static constexpr int BYTES = (ZR + RT + 1023) / 1024 * 1024;
+  static constexpr int NCH = BYTES / 1024;  // 1 KiB copies (one wave instruction each)
+};
+
+// SNP (within its block) of column `pos` of the pair and gradient tables.
+__host__ __device__ constexpr int qimg_snp(int pos) {
+  return 16 * (pos >> 4) + 8 * ((pos & 7) >> 2) + 4 * ((pos & 15) >> 3) + (pos & 3);
+}
+
+// ------------------------------------------------------------------ block images
+// Slot b (0 <= b <= ceil(J / 32)) holds the split rows of SNP block b and the
+// pair and gradient tables of block b - 1.  One workgroup per slot, each thread
+// writing whole 16-byte chunks, so the image is written with coalesced stores.
+// SNPs outside [0, J) take p = 0.5 in the split rows (s and t stay inside
+// (0, 1), and their missing genotype codes give zero weights) and 0 in the pair
+// and gradient tables.
+template <int K>
+__global__ __launch_bounds__(256) void k_qimg(const double* __restrict__ p, int64_t J, uint8_t* __restrict__ img) {
+  using G = QImg<K>;
+  const int64_t slot = blockIdx.x;
+  uint8_t* __restrict__ out = img + slot * G::BYTES;
+  const int64_t sblk = slot, tblk = slot - 1;  // blocks of the split rows and of the tables
+  auto pv = [&](int64_t blk, int s, int k) -> double {  // p of SNP s of a block (0.5 outside [0, J))
+    const int64_t j = blk * 32 + s;
+    return (j >= 0 && j < J) ? p[(int64_t)K * j + k] : 0.5;
+  };
+  auto in_range = [&](int64_t blk, int s) { return blk >= 0 && blk * 32 + s < J; };
+  for (int c = threadIdx.x; c < G::BYTES / 16; c += 256) {
+    const int off = 16 * c;
+    __bf16 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (__bf16)0.0f;
+    if (off < G::PP) {  // split rows: entry e of slot e / K is term PO[slot] of value k = e % K
+      const int r = (off - G::SP) / G::RS, col = ((off - G::SP) % G::RS) / 16;
+      if (col < G::KD / 8) {
+        constexpr int PO[6] = {0, 1, 0, 2, 1, 0};
+        for (int e = 0; e < 8; ++e) {
+          const int x = 8 * col + e, slot = x / K, k = x % K;
+          if (slot < 6) {
+            __bf16 t[3];
+            split3(pv(sblk, r, k), t[0], t[1], t[2]);
+            v[e] = t[PO[slot]];
+          }
+        }
+      }
+    } else if (off < G::GP) {  // pair rows, columns in operand order
+      const bool m = off >= G::MM;
+      const int r = (off - (m ? G::MM : G::PP)) / G::RT, col = ((off - (m ? G::MM : G::PP)) % G::RT) / 16;
+      if (col < 4) {
+        int a = 0, base = 0;
+        while (r >= base + (K - a)) base += K - (a++);
+        const int b = a + (r - base);
+        for (int e = 0; e < 8; ++e) {
+          const int s = qimg_snp(8 * col + e);
+          if (in_range(tblk, s)) {
+            const double xa = pv(tblk, s, a), xb = pv(tblk, s, b);
+            v[e] = (__bf16)(m ? (float)((1.0 - xa) * (1.0 - xb)) : (float)(xa * xb));
+          }
+        }
+      }
+    } else if (off < G::ZR) {  // gradient rows: bf16(x_k), then x_k - bf16(x_k), x = p or 1 - p
+      const bool m = off >= G::GM;
+      const int r = (off - (m ? G::GM : G::GP)) / G::RT, col = ((off - (m ? G::GM : G::GP)) % G::RT) / 16;
+      if (col < 4) {
+        const int k = r % K;
+        for (int e = 0; e < 8; ++e) {
+          const int s = qimg_snp(8 * col + e);
+          if (in_range(tblk, s)) {
+            const double x = m ? 1.0 - pv(tblk, s, k) : pv(tblk, s, k);
+            const __bf16 hi = (__bf16)(float)x;
+            v[e] = r < K ? hi : (__bf16)(float)(x - (double)(float)hi);
+          }
+        }
+      }
+    }
+    u32x4 w;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) w[e] = __builtin_bit_cast(uint32_t, (bf16x2){v[2 * e], v[2 * e + 1]});
+    *reinterpret_cast<u32x4*>(out + off) = w;
+  }
+}
+
+__device__ __forceinline__ bf16x8 lds_b8(const uint8_t* p) {
+  return __builtin_bit_cast(bf16x8, *reinterpret_cast<const u32x4*>(p));
+}
+
+// ------------------------------------------------------------------ the pass
+// Partials [chunk][NP + K][I] in FP32, the layout of k_qside_bf16, reduced in
+// fixed order by the FP64 finalize.  jchunk is a multiple of 32.  Individuals
+// past I have zero split rows (s = 0): their weights are not finite, but a
+// weight only reaches its own individual's row (Hessian) or column (gradient)
+// of the products, and those are not stored.
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ void swap16(uint32_t (&x)[4], uint32_t (&y)[4]) {
+#pragma unroll
+  for (int e = 0; e < 4; ++e) {
+    const auto r = __builtin_amdgcn_permlane16_swap(x[e], y[e], false, false);
+    x[e] = r[0];
+    y[e] = r[1];
+  }
+}
+
+template <int K>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_qside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk,
+                  const __bf16* __restrict__ QT, const uint8_t* __restrict__ img, float* __restrict__ part) {
+  using G = QImg<K>;
+  constexpr int KD = G::KD, NM = G::NM, NP = G::NP, NT = G::NT16, NA = NP + K;
+  constexpr bool G16 = K <= 8;  // gradient rows (k leading, k residual) fit one 16-row tile
+  __shared__ __align__(16) uint8_t sbuf0[G::BYTES];
+  __shared__ __align__(16) uint8_t sbuf1[G::BYTES];
+  __shared__ __align__(16) uint8_t tbuf[4][256];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: the copies' LDS base goes to M0
+  const int lane = threadIdx.x & 63, h = lane >> 5, cn = lane & 31;
+  const int n16 = lane & 15, g16 = lane >> 4;  // 16x16x32 operand lane: row/column n16, reduction group g16
+  uint8_t* __restrict__ tb = tbuf[wave];
+  const int64_t nbx = (I + 127) / 128;
+  const Tile2 tile = xcd_tile(nbx);  // neighbouring 128-individual blocks share BED lines: keep them on one XCD
+  const int64_t i0 = tile.x * 128 + 32 * wave;
+  const int64_t jb = tile.y * jchunk;
+  const int64_t je = (jb + jchunk < J) ? jb + jchunk : J;
+
+  bf16x8 bq[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) bq[m] = load_b8(QT, KD, (int)0, (i0 + cn) * KD + 16 * m + 8 * h);
+  const uint32_t shift = 2u * (uint32_t)(cn & 3);
+  const int bi = cn >> 2;
+
+  // per-lane operand offsets inside a buffer
+  const int o_sp = G::SP + cn * G::RS + 16 * h;
+  int o_h[NT];  // Hessian B: pair row 16 t + n16 of pp (even groups) or mm (odd groups), SNPs of h = g16 / 2
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int n = 16 * t + n16;
+    o_h[t] = (n < NP ? ((g16 & 1) ? G::MM : G::PP) + n * G::RT : G::ZR) + 16 * (g16 >> 1);
+  }
+  // gradient A: rows k (leading) and k + 8 (residual) of a 16-row tile (K <= 8),
+  // else rows k and 16 + k of a 32-row tile (lanes as the 32x32x16 A operand)
+  const int gm_ = G16 ? (n16 & 7) : (cn & 15);
+  const int grow = (G16 ? (n16 < 8 ? 0 : K) : (cn < 16 ? 0 : K)) + gm_;
+  const int gsh = G16 ? 16 * (g16 >> 1) : 16 * h;
+  const int o_gp = (gm_ < K ? G::GP + grow * G::RT : G::ZR) + gsh;
+  const int o_gm = (gm_ < K ? G::GM + grow * G::RT : G::ZR) + gsh;
+
+  const uint8_t* __restrict__ gbase = bed + (i0 >> 2) + 4 * (lane & 1);
+  auto gload = [&](int64_t j0) {
+    const int64_t j = j0 + (lane >> 1);
+    uint32_t w = 0x55555555u;
+    if (j < je) w = *reinterpret_cast<const uint32_t*>(gbase + pitch * j);
+    return w;
+  };
+  const uint8_t* __restrict__ isrc = img + lane * 16;
+  auto copy_block = [&](int64_t blk, uint8_t* dst) {  // this wave's 1 KiB pieces of image slot blk
+    for (int c = wave; c < G::NCH; c += 4)
+      copy_to_lds16(isrc + blk * G::BYTES + c * 1024, dst + c * 1024);
+  };
+  auto s_mfma = [&](const uint8_t* __restrict__ buf) {  // s from a buffer's split rows
+    f32x16 sv = (f32x16){};
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+      sv = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b8(buf + o_sp + 32 * m), bq[m], sv, 0, 0, 0);
+    return sv;
+  };
+
+  // Slot b of the image holds the split rows of block b and the pair and
+  // gradient tables of block b - 1, so the buffer of block b's products also
+  // holds what s of block b + 1 needs: those MFMAs are issued first and run
+  // under block b's weights on the vector ALU.
+  f32x4 acc[2][NT];  // Hessian, individuals 0-15 / 16-31 of the wave
+  f32x4 z16[2] = {(f32x4){}, (f32x4){}};
+  f32x16 z32 = (f32x16){};
+#pragma unroll
+  for (int gi = 0; gi < 2; ++gi)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[gi][t] = (f32x4){};
+  copy_block(jb / 32, sbuf0);
+  uint32_t wn = gload(jb);
+  __syncthreads();  // waits for the copy (vmcnt) before the barrier
+  f32x16 sa = s_mfma(sbuf0);
+  copy_block(jb / 32 + 1, sbuf1);
+  __syncthreads();  // the second copy has landed and every wave is done with sbuf0
+  int nb = 1;
+  for (int64_t js = jb; js < je; js += 32, nb ^= 1) {
+    const uint8_t* __restrict__ cur = nb ? sbuf1 : sbuf0;
+    // The other buffer was last read in the previous block, before the barrier
+    // that ended it.
+    if (js + 32 < je) copy_block(js / 32 + 2, nb ? sbuf0 : sbuf1);
+    const f32x16 sn = s_mfma(cur);  // block js + 32 (past the chunk on its last block: not used)
+    // genotype bytes: byte e of this lane's word is byte 4(l%2)+e of SNP row l/2 -> tb[byte][row]
+#pragma unroll
+    for (int e = 0; e < 4; ++e) tb[(4 * (lane & 1) + e) * 32 + (lane >> 1)] = (uint8_t)(wn >> (8 * e));
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    uint32_t rd[4];
+#pragma unroll
+    for (int a = 0; a < 4; ++a) rd[a] = *reinterpret_cast<const uint32_t*>(tb + bi * 32 + 8 * a + 4 * h);
+    __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+    if (js + 32 < je) wn = gload(js + 32);
+
+#pragma unroll
+    for (int half = 0; half < 2; ++half) {  // SNPs 16 half .. 16 half + 15: registers 8 half .. 8 half + 7
+      uint32_t wa[4], wb[4], ch[4], cl[4], dh[4], dl[4];
+#pragma unroll
+      for (int a = 2 * half; a < 2 * half + 2; ++a) {
+        const uint32_t codes = (rd[a] >> shift) & 0x03030303u;  // SNP 8a + 4h + b in byte b
+        const uint32_t gb = g_bytes(codes), hb = h_bytes(codes);
+#pragma unroll
+        for (int b = 0; b < 4; b += 2) {
+          float av[2], bv[2], cv[2], dv[2];
+#pragma unroll
+          for (int x = 0; x < 2; ++x) {
+            const int r = 4 * a + b + x;
+            const float sv = sa[r], tv = 1.0f - sv;
+            const float rr = __builtin_amdgcn_rcpf(sv * tv);  // 1 ulp: the weights are FP32-grade
+            const float is = tv * rr, it = sv * rr;
+            cv[x] = byte_f(gb, b + x) * is;
+            dv[x] = byte_f(hb, b + x) * it;
+            av[x] = cv[x] * is;
+            bv[x] = dv[x] * it;
+          }
+          const int w = ((4 * a + b) >> 1) & 3;
+          wa[w] = pack_bf16x2(av[0], av[1]);
+          wb[w] = pack_bf16x2(bv[0], bv[1]);
+          split_pack(cv[0], cv[1], ch[w], cl[w]);
+          split_pack(dv[0], dv[1], dh[w], dl[w]);
+        }
+      }
+      if constexpr (!G16) {  // gradient on 32x32x16 tiles, from the lanes' own weights
+        const bf16x8 gp = lds_b8(cur + o_gp + 32 * half), gm = lds_b8(cur + o_gm + 32 * half);
+        z32 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(gp, b8(ch), z32, 0, 0, 0);
+        z32 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(gp, b8(cl), z32, 0, 0, 0);
+        z32 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(gm, b8(dh), z32, 0, 0, 0);
+        z32 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(gm, b8(dl), z32, 0, 0, 0);
+      }
+      swap16(wa, wb);  // wa: individuals 0-15, wb: 16-31; groups (a h0) (b h0) (a h1) (b h1)
+#pragma unroll
+      for (int t = 0; t < NT; ++t) {
+        const bf16x8 bt = lds_b8(cur + o_h[t] + 32 * half);
+        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b8(wa), bt, acc[0][t], 0, 0, 0);
+        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b8(wb), bt, acc[1][t], 0, 0, 0);
+      }
+      if constexpr (G16) {
+        swap16(ch, cl);  // ch: individuals 0-15, cl: 16-31; groups (hi h0) (lo h0) (hi h1) (lo h1)
+        swap16(dh, dl);
+        const bf16x8 gp = lds_b8(cur + o_gp + 32 * half), gm = lds_b8(cur + o_gm + 32 * half);
+        z16[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gp, b8(ch), z16[0], 0, 0, 0);
+        z16[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gp, b8(cl), z16[1], 0, 0, 0);
+        z16[0] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gm, b8(dh), z16[0], 0, 0, 0);
+        z16[1] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(gm, b8(dl), z16[1], 0, 0, 0);
+      }
+    }
+    sa = sn;
+    // The next block's image and genotype word have landed (the barrier waits
+    // for them), and every wave is done reading `cur` before it is refilled.
+    __syncthreads();
+  }
+  float* __restrict__ out = part + tile.y * NA * I;
+#pragma unroll
+  for (int gi = 0; gi < 2; ++gi)  // D[individual][pair]: lane (pair n16, g16) holds individuals 4 g16 + r
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int pn = 16 * t + n16;
+      if (pn < NP) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t row = i0 + 16 * gi + 4 * g16 + r;
+          if (row < I) out[(int64_t)pn * I + row] = acc[gi][t][r];
+        }
+      }
+    }
+  if constexpr (G16) {  // D[row][individual]: lane (individual n16, g16) holds rows 4 g16 + r
+#pragma unroll
+    for (int gi = 0; gi < 2; ++gi) {
+      const int64_t col = i0 + 16 * gi + n16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float v = z16[gi][r] + __shfl_xor(z16[gi][r], 32);  // leading (rows < 8) + residual (rows 8..15)
+        const int k = 4 * g16 + r;
+        if (g16 < 2 && k < K && col < I) out[(int64_t)(NP + k) * I + col] = v;
+      }
+    }
+  } else if (i0 + cn < I) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int row = 8 * (r >> 2) + 4 * h + (r & 3);
+      if (row < K) out[(int64_t)(NP + row) * I + i0 + cn] = z32[r] + z32[r + 8];
+    }
+  }
+}
+
+// ------------------------------------------------------------------ launchers
+template <int K>
+static void qimg_k(const double* p, int64_t J, void* img, hipStream_t s) {
+  hipLaunchKernelGGL((k_qimg<K>), dim3((unsigned)((J + 31) / 32 + 1)), dim3(256), 0, s, p, J, (uint8_t*)img);
+}
+template <int K>
+static void qside_mfma_k(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
+                         const void* QT, const void* img, float* part, hipStream_t s) {
+  const dim3 grid((unsigned)((I + 127) / 128 * nchunk));  // 1-D, tiles ordered per XCD (xcd_tile)
+  hipLaunchKernelGGL((k_qside_mfma<K>), grid, dim3(256), 0, s, bed, pitch, I, J, jchunk, (const __bf16*)QT,
+                     (const uint8_t*)img, part);
+}
+
+int qimg_bytes(int K) {
+  int b = 0;
+  ADMX_DISPATCH_K(K, (b = QImg<KK>::BYTES));
+  return b;
+}
+void launch_qimg(int K, const double* p, int64_t J, void* img, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (qimg_k<KK>(p, J, img, s)));
+}
+void launch_qside_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
+                       const void* QT, const void* img, float* part, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (qside_mfma_k<KK>(bed, pitch, I, J, jchunk, nchunk, QT, img, part, s)));
+}
+
+}  // namespace admx

@@ -52,5 +52,43 @@ __device__ __forceinline__ void to_tiles(const uint32_t (&lo)[4], const uint32_t
   t1 = __builtin_bit_cast(bf16x8, b);
 }
 
+// Split tables: every FP64 value v becomes three BF16 terms v1 + v2 + v3
+// (v1 = bf16(v), v2 = bf16(v - v1), v3 = bf16(v - v1 - v2), a residual of about
+// 2^-24 |v|); the six products of order < 3 of two such values stack along an
+// MFMA's reduction dimension, KD = 16 ceil(6K / 16) entries per row.
+template <int K> struct SplitDim {
+  static constexpr int KD = ((6 * K + 15) / 16) * 16;
+  static constexpr int NM = KD / 16;
+};
+
+__device__ __forceinline__ void split3(double v, __bf16& h1, __bf16& h2, __bf16& h3) {
+  h1 = (__bf16)(float)v;
+  const double r1 = v - (double)(float)h1;
+  h2 = (__bf16)(float)r1;
+  const double r2 = r1 - (double)(float)h2;
+  h3 = (__bf16)(float)r2;
+}
+
+// 1 - sum_k q_ik in FP64, rounded to FP32: q.(1-p) + (1 - sum q) = 1 - q.p is the
+// reference's t also where the columns of q do not sum to 1 (after EM steps,
+// whose q sums to the observed fraction, src/algorithms_inner.jl:76-89).
+template <int K>
+__device__ __forceinline__ float sum_complement(const double* __restrict__ q, int64_t i) {
+  double sq = 0.0;
+#pragma unroll
+  for (int k = 0; k < K; ++k) sq += q[(int64_t)K * i + k];
+  return (float)(1.0 - sq);
+}
+
+// Leading and residual BF16 of two FP32 weights, packed.
+__device__ __forceinline__ void split_pack(float w0, float w1, uint32_t& hi, uint32_t& lo) {
+  hi = pack_bf16x2(w0, w1);
+  const float r0 = w0 - __builtin_bit_cast(float, hi << 16), r1 = w1 - __builtin_bit_cast(float, hi & 0xffff0000u);
+  lo = pack_bf16x2(r0, r1);
+}
+
+__device__ __forceinline__ bf16x8 b8(const uint32_t* w) {
+  return __builtin_bit_cast(bf16x8, (u32x4){w[0], w[1], w[2], w[3]});
+}
 
 }  // namespace admx
