@@ -487,6 +487,8 @@ def main():
     ap.add_argument("--graphs", action="store_true", help="replay steady-state iterations from hipGraphs")
     ap.add_argument("--qside-mfma", type=int, default=None, choices=[-1, 0, 1],
                     help="BF16 form's update_q pass: 1 all products on the matrix cores, 0 Hessian only")
+    ap.add_argument("--pside-mfma", type=int, default=None, choices=[-1, 0, 1],
+                    help="BF16 form's update_p pass: 1 all products on the matrix cores, 0 Hessian only")
     ap.add_argument("--autotune", action="store_true",
                     help="pick waves per pass and the Hessian split with the autotuner before timing")
     ap.add_argument("--stream-snps", type=int, default=0,
@@ -574,6 +576,8 @@ def main():
         ctx.set_precision(args.form)
     if args.qside_mfma is not None:
         ctx.set_tuning(oa._native.ADMX_TUNE_QSIDE_MFMA, args.qside_mfma)
+    if args.pside_mfma is not None:
+        ctx.set_tuning(oa._native.ADMX_TUNE_PSIDE_MFMA, args.pside_mfma)
     d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(args.seed + 1), T=T)
     oa.init_em(d, None, 5, d_cu=ctx, log=None)
     ctx.set_uv(d.U, d.V)

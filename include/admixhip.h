@@ -73,6 +73,8 @@ extern "C" {
                                    * (s and the gradient in FP32 on the vector ALU); -1 (default): 1 from
                                    * K = 7 (faster there, slower below).  Both within the BF16 form's error
                                    * bars against the FP64 oracle. */
+#define ADMX_TUNE_PSIDE_MFMA 10   /* the same for the BF16 form's update_p pass (w = a + b Hessian weight,
+                                   * gradient u = c - d from two-term BF16 products) */
 
 /* values of ADMX_TUNE_QN_MODE: admixture_qn!'s `mode` (src/algorithms_outer.jl:127-136) */
 #define ADMX_QN_ZAL 0

@@ -22,7 +22,7 @@ ADMX_TUNE_QN_MODE = 4
 ADMX_TUNE_SKIPMISSING = 5
 ADMX_TUNE_RESIDENT_SNPS = 6
 ADMX_TUNE_SPLIT_Q, ADMX_TUNE_SPLIT_P = 7, 8
-ADMX_TUNE_QSIDE_MFMA = 9
+ADMX_TUNE_QSIDE_MFMA, ADMX_TUNE_PSIDE_MFMA = 9, 10
 ADMX_QN_ZAL, ADMX_QN_LBQN = 0, 1
 ADMX_PREC_F64, ADMX_PREC_F32, ADMX_PREC_BF16 = 0, 1, 2
 ADMX_SKFR_GLOBAL, ADMX_SKFR_LOCAL = 0, 1

@@ -523,6 +523,50 @@ int mixed_update_q_mfma(admx_ctx* c, const double* q, const double* p, const dou
   return check_launch();
 }
 
+bool pside_on_matrix_cores(const admx_ctx* c) {
+  return c->pside_mfma == 1 || (c->pside_mfma == -1 && c->K >= 7);
+}
+
+// BF16 form, P side: the mirror of mixed_update_q_mfma (split table of p, one
+// image slot per 32-individual block of q).
+int mixed_update_p_mfma(admx_ctx* c, const double* q, const double* p, const double* x_in, double* x_out,
+                        int32_t* status_dev) {
+  const int64_t I = c->I, J = c->J, K = c->K;
+  const int KD = split_table_cols((int)K);
+  // the kernel reads split rows up to the last 128-SNP workgroup's end
+  const int64_t rows_p = (J + 127) / 128 * 128 + 32, nblk = (I + 31) / 32;
+  CHK(c->lpt.grow(sizeof(uint16_t) * KD * rows_p));
+  CHK(c->lmt.grow(sizeof(uint16_t) * KD * rows_p));
+  CHK(c->pimg.grow((size_t)(nblk + 1) * pimg_bytes((int)K)));
+  const int64_t nbx = (J + 127) / 128;  // 128 SNPs (4 waves) per workgroup
+  int64_t nchunk = std::max<int64_t>(1, (c->target_waves / 4 + nbx - 1) / nbx);
+  nchunk = std::min<int64_t>(nchunk, std::max<int64_t>(1, I / 512));
+  // chunks of whole 128-byte BED lines (512 individuals) where I allows
+  const int64_t ichunk = ((I + nchunk - 1) / nchunk + 511) / 512 * 512;
+  nchunk = (I + ichunk - 1) / ichunk;
+  const int na = (int)(K * (K + 1) / 2 + K);
+  CHK(c->part.grow((size_t)nchunk * na * J * sizeof(float)));
+  {
+    ProfScope ps(c, "mixed_prep");
+    launch_split_tables((int)K, p, J, rows_p, 1, c->lpt.p, c->lmt.p, c->s);
+    launch_pimg((int)K, q, I, c->pimg.p, c->s);
+  }
+  CHK(check_launch());
+  {
+    ProfScope ps(c, "pside_update_bf16");
+    launch_pside_mfma((int)K, c->bed, c->pitch, I, J, ichunk, (int)nchunk, c->lpt.p, c->pimg.p,
+                      static_cast<float*>(c->part.p), c->s);
+  }
+  CHK(check_launch());
+  FinArgs f{c->part.p, J, (int)nchunk, 0, J, x_in, x_out, nullptr, c->keep_hess ? c->XtX_p : nullptr,
+            c->keep_hess ? c->Xtz_p : nullptr, 0, 1, status_dev, c->XtX_p, c->Xtz_p};
+  {
+    ProfScope ps(c, "fin_p_update_qp");
+    launch_fin_p_update_mixed((int)K, f, c->s);
+  }
+  return check_launch();
+}
+
 // Mixed-precision update passes of a FP64 context: FP32 copies of (q, p), the
 // BF16 pair tables, the FP32/BF16 pass, then the FP64 finalize + QP + projection.
 int mixed_update(admx_ctx* c, bool qside, const double* q, const double* p, const double* x_in, double* x_out,
@@ -531,6 +575,8 @@ int mixed_update(admx_ctx* c, bool qside, const double* q, const double* p, cons
   CHK(need_resident(c, "the mixed-precision passes"));
   if (qside && c->precision == ADMX_PREC_BF16 && qside_on_matrix_cores(c))
     return mixed_update_q_mfma(c, q, p, x_in, x_out, status_dev);
+  if (!qside && c->precision == ADMX_PREC_BF16 && pside_on_matrix_cores(c))
+    return mixed_update_p_mfma(c, q, p, x_in, x_out, status_dev);
   const int64_t I = c->I, J = c->J, K = c->K;
   if (c->fq.n < sizeof(float) * K * (I + 16)) {
     CHK(c->fq.grow(sizeof(float) * K * (I + 16)));
@@ -818,7 +864,7 @@ uint64_t layout_sig(const admx_ctx* c) {
   };
   const DevBuf* bufs[] = {&c->part,    &c->llpart,  &c->zal,     &c->fq,  &c->fp,  &c->ppt,
                           &c->mmt,     &c->qqt,     &c->wide[0], &c->wide[1], &c->wide[2], &c->wide[3],
-                          &c->llflag,  &c->lqt,     &c->lpt,     &c->lmt, &c->lps, &c->lqc, &c->qimg};
+                          &c->llflag,  &c->lqt,     &c->lpt,     &c->lmt, &c->lps, &c->lqc, &c->qimg, &c->pimg};
   for (const DevBuf* b : bufs) {
     mix((uint64_t)(uintptr_t)b->p);
     mix((uint64_t)b->n);
@@ -831,6 +877,7 @@ uint64_t layout_sig(const admx_ctx* c) {
   mix((uint64_t)(c->split_q + 2));
   mix((uint64_t)(c->split_p + 2));
   mix((uint64_t)(c->qside_mfma + 2));
+  mix((uint64_t)(c->pside_mfma + 2));
   mix((uint64_t)(uintptr_t)c->snp_mu.p);
   return h;
 }
@@ -1290,6 +1337,7 @@ int admx_destroy(admx_ctx* c) {
     c->lps.release();
     c->lqc.release();
     c->qimg.release();
+    c->pimg.release();
     c->feat.release();
     c->featpart.release();
     c->snp_mu.release();
@@ -1943,9 +1991,9 @@ int admx_set_tuning(admx_ctx* c, int key, int64_t value) {
     } else if (key == ADMX_TUNE_SPLIT_Q || key == ADMX_TUNE_SPLIT_P) {
       if (value < -1 || value > 1) return bad("Hessian split must be -1 (default), 0 or 1");
       (key == ADMX_TUNE_SPLIT_Q ? c->split_q : c->split_p) = (int)value;
-    } else if (key == ADMX_TUNE_QSIDE_MFMA) {
-      if (value < -1 || value > 1) return bad("matrix-core update_q must be -1 (default), 0 or 1");
-      c->qside_mfma = (int)value;
+    } else if (key == ADMX_TUNE_QSIDE_MFMA || key == ADMX_TUNE_PSIDE_MFMA) {
+      if (value < -1 || value > 1) return bad("matrix-core update pass must be -1 (default), 0 or 1");
+      (key == ADMX_TUNE_QSIDE_MFMA ? c->qside_mfma : c->pside_mfma) = (int)value;
     } else if (key == ADMX_TUNE_SKIPMISSING) {
       c->impute = value == 0;
     } else if (key == ADMX_TUNE_QN_MODE) {

@@ -115,12 +115,14 @@ struct admx_ctx {
   int64_t target_waves = 65536;
   int split_q = -1, split_p = -1;  // ADMX_TUNE_SPLIT_Q / _P (-1: the default K thresholds)
   int qside_mfma = -1;             // ADMX_TUNE_QSIDE_MFMA (-1: on from K = 7)
+  int pside_mfma = -1;             // ADMX_TUNE_PSIDE_MFMA (-1: on from K = 7)
   int keep_hess = 1;
   int precision = ADMX_PREC_F64;
   admx_host::DevBuf fq, fp, ppt, mmt, qqt;  // mixed-precision pass inputs
   admx_host::DevBuf lqt, lpt, lmt;           // BF16 split tables of the matrix-core log-likelihood
   admx_host::DevBuf lps, lqc;                // BF16 form's EM: two-term split rows of q or p, 1 - sum_k q_ik
   admx_host::DevBuf qimg;                    // BF16 form's matrix-core update_q: 32-SNP block images of p
+  admx_host::DevBuf pimg;                    // ... and update_p: 32-individual block images of q
   admx_host::DevBuf feat, featpart;          // warm-start features (I x 64) and their chunk partials
   // J-chunk streaming: the BED lives in pinned host memory, or in a PLINK .bed
   // file read chunk by chunk, and every pass streams it through two device

@@ -129,6 +129,14 @@ void launch_qimg(int K, const double* p, int64_t J, void* img, hipStream_t s);
 void launch_qside_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
                        const void* QT, const void* img, float* part, hipStream_t s);
 
+// The P-side mirror: ceil(I/32) + 1 individual-side image slots (pimg_bytes(K)
+// each, from q by launch_pimg), PT the SNP split table of launch_split_tables
+// (role 1); ichunk a multiple of 32; partials [chunk][K(K+1)/2 + K][J] in FP32.
+int pimg_bytes(int K);
+void launch_pimg(int K, const double* q, int64_t I, void* img, hipStream_t s);
+void launch_pside_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t ichunk, int nchunk,
+                       const void* PT, const void* img, float* part, hipStream_t s);
+
 // Batched QP on host-provided tableau inputs (XtX, negated Xtz, x): delta out.
 template <typename T> void launch_qp_batch(int K, bool simplex, int64_t n, const void* XtX, const void* Xtz_neg,
                                            const void* x, void* delta, int32_t* status, hipStream_t s);

@@ -357,6 +357,302 @@ void k_qside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int
   }
 }
 
+// ------------------------------------------------------------------ P side
+// update_p! (src/loops.jl:923-950) on the matrix cores, the mirror of
+// k_qside_mfma with the roles of individuals and SNPs exchanged:
+//   s = q.p (three-term splits; D[i][j], individuals as rows), t = 1 - s,
+//   XtX_p = sum_i w_ij qq_i, w = a + b = g/s^2 + (2-g)/t^2 in BF16,
+//   Xtz_p = sum_i u_ij q_i,  u = c - d = g/s - (2-g)/t split into a leading
+//   and a residual FP16 term, q into two-row FP16 tables.
+// A wave owns 32 SNPs (lanes as the columns of the s tile), a workgroup 128;
+// the workgroup walks its individual chunk in blocks of 32, each block's
+// individual-side operands one image slot in LDS layout (slot b: split rows of
+// block b, tables of block b - 1).  With one Hessian weight per genotype, the
+// 16x16x32 operand of one block comes from swapping the weights of its two
+// halves: reduction groups (half 0, h 0) (half 1, h 0) (half 0, h 1)
+// (half 1, h 1), the column order of the qq rows.  Each lane reads its own
+// SNP row's 8 genotype bytes of a block (no transpose).
+// The P-side gradient weight u = c - d has either sign, so its sum over the
+// individuals cancels: it is split into FP16 terms (2^-22 relative, against
+// 2^-17 for BF16) on v_mfma_f32_*_f16, with u scaled by 2^-8 and the q rows by
+// 2^8 (|u| <= 2 / min(s, t) stays inside FP16's range for s, t >= 1e-5, and the
+// smallest q stays a normal number).
+constexpr double PGRAD_SCALE = 256.0;
+typedef _Float16 f16x2 __attribute__((ext_vector_type(2)));
+typedef _Float16 f16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ void split_pack_f16(float w0, float w1, uint32_t& hi, uint32_t& lo) {
+  const f16x2 h = __builtin_convertvector((f32x2){w0, w1}, f16x2);
+  hi = __builtin_bit_cast(uint32_t, h);
+  lo = __builtin_bit_cast(uint32_t, __builtin_convertvector((f32x2){w0 - (float)h[0], w1 - (float)h[1]}, f16x2));
+}
+__device__ __forceinline__ f16x8 h8(const uint32_t* w) {
+  return __builtin_bit_cast(f16x8, (u32x4){w[0], w[1], w[2], w[3]});
+}
+__device__ __forceinline__ f16x8 lds_h8(const uint8_t* p) {
+  return __builtin_bit_cast(f16x8, *reinterpret_cast<const u32x4*>(p));
+}
+
+template <int K> struct PImg {
+  static constexpr int KD = SplitDim<K>::KD, NM = SplitDim<K>::NM;
+  static constexpr int NP = K * (K + 1) / 2, NT16 = (NP + 15) / 16;
+  static constexpr int RS = 2 * KD + 16, RT = 80;
+  static constexpr int SQ = 0;                // 32 split rows of q
+  static constexpr int QQ = SQ + 32 * RS;     // NP pair rows q_a q_b
+  static constexpr int GQ = QQ + NP * RT;     // 2K FP16 rows: 2^8 q_k leading, then residual
+  static constexpr int ZR = GQ + 2 * K * RT;  // one zero row
+  static constexpr int BYTES = (ZR + RT + 1023) / 1024 * 1024;
+  static constexpr int NCH = BYTES / 1024;
+};
+
+// Individual (within its block) of column `pos` of the qq rows: pos = 8 g + e
+// holds half g % 2, h = g / 2, element e.
+__host__ __device__ constexpr int pimg_hcol(int pos) {
+  return 16 * ((pos >> 3) & 1) + 8 * ((pos & 7) >> 2) + 4 * (pos >> 4) + (pos & 3);
+}
+
+// Slot b (0 <= b <= ceil(I / 32)): split rows of individual block b (individuals
+// past I take q_k = 1/K, so that s stays inside (0, 1) where p does; their
+// genotype codes are masked to missing), the qq and gradient rows of block
+// b - 1 (0 outside [0, I)).
+template <int K>
+__global__ __launch_bounds__(256) void k_pimg(const double* __restrict__ q, int64_t I, uint8_t* __restrict__ img) {
+  using G = PImg<K>;
+  const int64_t slot = blockIdx.x;
+  uint8_t* __restrict__ out = img + slot * G::BYTES;
+  const int64_t sblk = slot, tblk = slot - 1;
+  auto in_range = [&](int64_t blk, int s) { return blk >= 0 && blk * 32 + s < I; };
+  auto qv = [&](int64_t blk, int s, int k) -> double {
+    return in_range(blk, s) ? q[(int64_t)K * (blk * 32 + s) + k] : 1.0 / K;
+  };
+  for (int c = threadIdx.x; c < G::BYTES / 16; c += 256) {
+    const int off = 16 * c;
+    __bf16 v[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) v[e] = (__bf16)0.0f;
+    if (off < G::QQ) {  // split rows
+      const int r = (off - G::SQ) / G::RS, col = ((off - G::SQ) % G::RS) / 16;
+      if (col < G::KD / 8) {
+        constexpr int QO[6] = {0, 0, 1, 0, 1, 2};
+        for (int e = 0; e < 8; ++e) {
+          const int x = 8 * col + e, slot6 = x / K, k = x % K;
+          if (slot6 < 6) {
+            __bf16 t[3];
+            split3(qv(sblk, r, k), t[0], t[1], t[2]);
+            v[e] = t[QO[slot6]];
+          }
+        }
+      }
+    } else if (off < G::GQ) {  // qq rows, columns in the Hessian operand order
+      const int r = (off - G::QQ) / G::RT, col = ((off - G::QQ) % G::RT) / 16;
+      if (col < 4) {
+        int a = 0, base = 0;
+        while (r >= base + (K - a)) base += K - (a++);
+        const int b = a + (r - base);
+        for (int e = 0; e < 8; ++e) {
+          const int i = pimg_hcol(8 * col + e);
+          if (in_range(tblk, i)) v[e] = (__bf16)(float)(qv(tblk, i, a) * qv(tblk, i, b));
+        }
+      }
+    } else if (off < G::ZR) {  // gradient rows in FP16, columns in the order of the weights' halves
+      const int r = (off - G::GQ) / G::RT, col = ((off - G::GQ) % G::RT) / 16;
+      if (col < 4) {
+        const int k = r % K;
+        _Float16 f[8];
+        for (int e = 0; e < 8; ++e) {
+          f[e] = (_Float16)0.0f;
+          const int i = qimg_snp(8 * col + e);
+          if (in_range(tblk, i)) {
+            const double x = qv(tblk, i, k) * PGRAD_SCALE;
+            const _Float16 hi = (_Float16)(float)x;
+            f[e] = r < K ? hi : (_Float16)(float)(x - (double)(float)hi);
+          }
+        }
+        u32x4 w;
+#pragma unroll
+        for (int e = 0; e < 4; ++e) w[e] = __builtin_bit_cast(uint32_t, (f16x2){f[2 * e], f[2 * e + 1]});
+        *reinterpret_cast<u32x4*>(out + off) = w;
+        continue;
+      }
+    }
+    u32x4 w;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) w[e] = __builtin_bit_cast(uint32_t, (bf16x2){v[2 * e], v[2 * e + 1]});
+    *reinterpret_cast<u32x4*>(out + off) = w;
+  }
+}
+
+// Partials [chunk][NP + K][J] in FP32, the layout of k_pside_bf16.  ichunk is a
+// multiple of 32.  SNPs past J have zero split rows (s = 0): their weights are
+// not finite but reach only their own SNP's row (Hessian) or column (gradient).
+template <int K>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
+void k_pside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int64_t J, int64_t ichunk,
+                  const __bf16* __restrict__ PT, const uint8_t* __restrict__ img, float* __restrict__ part) {
+  using G = PImg<K>;
+  constexpr int KD = G::KD, NM = G::NM, NP = G::NP, NT = G::NT16, NA = NP + K;
+  constexpr bool G16 = K <= 8;
+  __shared__ __align__(16) uint8_t sbuf0[G::BYTES];
+  __shared__ __align__(16) uint8_t sbuf1[G::BYTES];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);  // uniform: the copies' LDS base goes to M0
+  const int lane = threadIdx.x & 63, h = lane >> 5, cn = lane & 31;
+  const int n16 = lane & 15, g16 = lane >> 4;
+  const int64_t nbx = (J + 127) / 128;
+  const Tile2 tile = xcd_tile(nbx);  // workgroups of one individual chunk share its image slots: one XCD
+  const int64_t j0 = tile.x * 128 + 32 * wave;
+  const int64_t ib = tile.y * ichunk;
+  const int64_t ie = (ib + ichunk < I) ? ib + ichunk : I;
+  const bool okj = j0 + cn < J;
+
+  bf16x8 bp[NM];
+#pragma unroll
+  for (int m = 0; m < NM; ++m) bp[m] = load_b8(PT, KD, (int)0, (j0 + cn) * KD + 16 * m + 8 * h);
+
+  const int o_sq = G::SQ + cn * G::RS + 16 * h;
+  int o_h[NT];  // Hessian B: qq row 16 t + n16, columns of group g16
+#pragma unroll
+  for (int t = 0; t < NT; ++t) {
+    const int n = 16 * t + n16;
+    o_h[t] = (n < NP ? G::QQ + n * G::RT : G::ZR) + 16 * g16;
+  }
+  const int gm_ = G16 ? (n16 & 7) : (cn & 15);
+  const int grow = (G16 ? (n16 < 8 ? 0 : K) : (cn < 16 ? 0 : K)) + gm_;
+  const int o_gq = (gm_ < K ? G::GQ + grow * G::RT : G::ZR) + (G16 ? 16 * (g16 >> 1) : 16 * h);
+
+  // genotypes: this lane's SNP row, 8 bytes (32 individuals) per block
+  const uint8_t* __restrict__ grow_p = bed + pitch * (okj ? j0 + cn : 0);
+  auto gload = [&](int64_t i) {
+    uint2 w = {0x55555555u, 0x55555555u};
+    if (okj && i < ie) w = *reinterpret_cast<const uint2*>(grow_p + (i >> 2));
+    return w;
+  };
+  const uint8_t* __restrict__ isrc = img + lane * 16;
+  auto copy_block = [&](int64_t blk, uint8_t* dst) {
+    for (int c = wave; c < G::NCH; c += 4)
+      copy_to_lds16(isrc + blk * G::BYTES + c * 1024, dst + c * 1024);
+  };
+  auto s_mfma = [&](const uint8_t* __restrict__ buf) {  // s from a buffer's individual split rows
+    f32x16 sv = (f32x16){};
+#pragma unroll
+    for (int m = 0; m < NM; ++m)
+      sv = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b8(buf + o_sq + 32 * m), bp[m], sv, 0, 0, 0);
+    return sv;
+  };
+
+  f32x4 acc[2][NT];  // Hessian, SNPs 0-15 / 16-31 of the wave
+  f32x4 z16[2] = {(f32x4){}, (f32x4){}};
+  f32x16 z32 = (f32x16){};
+#pragma unroll
+  for (int gi = 0; gi < 2; ++gi)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) acc[gi][t] = (f32x4){};
+  copy_block(ib / 32, sbuf0);
+  uint2 wn = gload(ib);
+  __syncthreads();
+  f32x16 sa = s_mfma(sbuf0);
+  copy_block(ib / 32 + 1, sbuf1);
+  __syncthreads();
+  int nb = 1;
+  for (int64_t is = ib; is < ie; is += 32, nb ^= 1) {
+    const uint8_t* __restrict__ cur = nb ? sbuf1 : sbuf0;
+    if (is + 32 < ie) copy_block(is / 32 + 2, nb ? sbuf0 : sbuf1);
+    const f32x16 sn = s_mfma(cur);
+    uint2 w = wn;
+    if (is + 32 < ie) wn = gload(is + 32);
+    if (is + 32 > ie) {  // uniform: individuals past the chunk end are missing
+      uint32_t keep[2] = {0u, 0u};
+#pragma unroll
+      for (int t = 0; t < 32; ++t)
+        if (is + t < ie) keep[t >> 4] |= 3u << (2 * (t & 15));
+      w.x = (w.x & keep[0]) | (0x55555555u & ~keep[0]);
+      w.y = (w.y & keep[1]) | (0x55555555u & ~keep[1]);
+    }
+    uint32_t wh[2][4], uh[2][4], ul[2][4];  // per half: packed weights w, u leading, u residual
+#pragma unroll
+    for (int a = 0; a < 4; ++a) {
+      const int xb = 2 * a + h;  // byte of individuals 8a + 4h .. 8a + 4h + 3
+      const uint32_t byte = ((xb < 4 ? w.x : w.y) >> (8 * (xb & 3))) & 0xffu;
+      const uint32_t codes = (byte | (byte << 6) | (byte << 12) | (byte << 18)) & 0x03030303u;
+      const uint32_t gb = g_bytes(codes), hb = h_bytes(codes);
+#pragma unroll
+      for (int b = 0; b < 4; b += 2) {
+        float wv[2], uv[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+          const int r = 4 * a + b + x;
+          const float sv = sa[r], tv = 1.0f - sv;
+          const float rr = __builtin_amdgcn_rcpf(sv * tv);  // 1 ulp: the weights are FP32-grade
+          const float is_ = tv * rr, it = sv * rr;
+          const float cc = byte_f(gb, b + x) * is_, dd = byte_f(hb, b + x) * it;
+          wv[x] = cc * is_ + dd * it;
+          uv[x] = (cc - dd) * (float)(1.0 / PGRAD_SCALE);
+        }
+        const int half = a >> 1, e = ((4 * a + b) >> 1) & 3;
+        wh[half][e] = pack_bf16x2(wv[0], wv[1]);
+        split_pack_f16(uv[0], uv[1], uh[half][e], ul[half][e]);
+      }
+    }
+    if constexpr (!G16) {
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        const f16x8 gq = lds_h8(cur + o_gq + 32 * half);
+        z32 = __builtin_amdgcn_mfma_f32_32x32x16_f16(gq, h8(uh[half]), z32, 0, 0, 0);
+        z32 = __builtin_amdgcn_mfma_f32_32x32x16_f16(gq, h8(ul[half]), z32, 0, 0, 0);
+      }
+    }
+    swap16(wh[0], wh[1]);  // wh[0]: SNPs 0-15, wh[1]: 16-31; groups (half 0, h 0) (half 1, h 0) (half 0, h 1) (half 1, h 1)
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const bf16x8 bt = lds_b8(cur + o_h[t]);
+      acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b8(wh[0]), bt, acc[0][t], 0, 0, 0);
+      acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b8(wh[1]), bt, acc[1][t], 0, 0, 0);
+    }
+    if constexpr (G16) {
+#pragma unroll
+      for (int half = 0; half < 2; ++half) {
+        swap16(uh[half], ul[half]);  // uh: SNPs 0-15, ul: 16-31; groups (hi h0) (lo h0) (hi h1) (lo h1)
+        const f16x8 gq = lds_h8(cur + o_gq + 32 * half);
+        z16[0] = __builtin_amdgcn_mfma_f32_16x16x32_f16(gq, h8(uh[half]), z16[0], 0, 0, 0);
+        z16[1] = __builtin_amdgcn_mfma_f32_16x16x32_f16(gq, h8(ul[half]), z16[1], 0, 0, 0);
+      }
+    }
+    sa = sn;
+    __syncthreads();
+  }
+  float* __restrict__ out = part + tile.y * NA * J;
+#pragma unroll
+  for (int gi = 0; gi < 2; ++gi)  // D[SNP][pair]: lane (pair n16, g16) holds SNPs 4 g16 + r
+#pragma unroll
+    for (int t = 0; t < NT; ++t) {
+      const int pn = 16 * t + n16;
+      if (pn < NP) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int64_t col = j0 + 16 * gi + 4 * g16 + r;
+          if (col < J) out[(int64_t)pn * J + col] = acc[gi][t][r];
+        }
+      }
+    }
+  if constexpr (G16) {
+#pragma unroll
+    for (int gi = 0; gi < 2; ++gi) {
+      const int64_t col = j0 + 16 * gi + n16;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const float v = z16[gi][r] + __shfl_xor(z16[gi][r], 32);
+        const int k = 4 * g16 + r;
+        if (g16 < 2 && k < K && col < J) out[(int64_t)(NP + k) * J + col] = v;
+      }
+    }
+  } else if (okj) {
+#pragma unroll
+    for (int r = 0; r < 8; ++r) {
+      const int row = 8 * (r >> 2) + 4 * h + (r & 3);
+      if (row < K) out[(int64_t)(NP + row) * J + j0 + cn] = z32[r] + z32[r + 8];
+    }
+  }
+}
+
 // ------------------------------------------------------------------ launchers
 template <int K>
 static void qimg_k(const double* p, int64_t J, void* img, hipStream_t s) {
@@ -368,6 +664,31 @@ static void qside_mfma_k(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J
   const dim3 grid((unsigned)((I + 127) / 128 * nchunk));  // 1-D, tiles ordered per XCD (xcd_tile)
   hipLaunchKernelGGL((k_qside_mfma<K>), grid, dim3(256), 0, s, bed, pitch, I, J, jchunk, (const __bf16*)QT,
                      (const uint8_t*)img, part);
+}
+
+template <int K>
+static void pimg_k(const double* q, int64_t I, void* img, hipStream_t s) {
+  hipLaunchKernelGGL((k_pimg<K>), dim3((unsigned)((I + 31) / 32 + 1)), dim3(256), 0, s, q, I, (uint8_t*)img);
+}
+template <int K>
+static void pside_mfma_k(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t ichunk, int nchunk,
+                         const void* PT, const void* img, float* part, hipStream_t s) {
+  const dim3 grid((unsigned)((J + 127) / 128 * nchunk));  // 1-D, tiles ordered per XCD (xcd_tile)
+  hipLaunchKernelGGL((k_pside_mfma<K>), grid, dim3(256), 0, s, bed, pitch, I, J, ichunk, (const __bf16*)PT,
+                     (const uint8_t*)img, part);
+}
+
+int pimg_bytes(int K) {
+  int b = 0;
+  ADMX_DISPATCH_K(K, (b = PImg<KK>::BYTES));
+  return b;
+}
+void launch_pimg(int K, const double* q, int64_t I, void* img, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (pimg_k<KK>(q, I, img, s)));
+}
+void launch_pside_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t ichunk, int nchunk,
+                       const void* PT, const void* img, float* part, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (pside_mfma_k<KK>(bed, pitch, I, J, ichunk, nchunk, PT, img, part, s)));
 }
 
 int qimg_bytes(int K) {

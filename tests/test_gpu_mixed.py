@@ -64,18 +64,18 @@ def test_mixed_hessians_match_oracle(oa, O, K, form):
 
 
 @pytest.mark.parametrize("I,J,K", [(1, 1, 1), (37, 45, 3), (129, 70, 5), (1500, 4100, 8), (700, 333, 12),
-                                   (300, 260, 16)])
-def test_matrix_core_update_q(oa, O, I, J, K):
-    """BF16 form's update_q! with s, t, the Hessian and the gradient on the
-    matrix cores (ADMX_TUNE_QSIDE_MFMA = 1) and with the Hessian only (0),
-    against the FP64 oracle's update_q! on the same state: Hessian 5e-3,
-    gradient 1e-5 relative to the column maximum.  Ragged I and J leave
-    partial 128-individual workgroups and partial 32-SNP blocks; a quarter of
-    the SNPs have p close to 1 (t = 1 - s down to 1e-5), and q's columns sum
-    to 0.98 (t = q.(1-p) + 1 - sum q)."""
+                                   (300, 260, 16), (1100, 130, 7)])
+def test_matrix_core_update_passes(oa, O, I, J, K):
+    """BF16 form's update_q! and update_p! with s, the Hessian and the gradient
+    on the matrix cores (ADMX_TUNE_QSIDE_MFMA / _PSIDE_MFMA = 1) and with the
+    Hessian only (0), against the FP64 oracle's update_q! / update_p! on the
+    same state: Hessian 5e-3, gradient 1e-5 relative to the column maximum.
+    Ragged I and J leave partial 128-individual (128-SNP) workgroups and
+    partial 32-SNP (32-individual) blocks; a quarter of the SNPs have p close
+    to 1 (t = 1 - s down to 1e-5), and q's columns sum to 0.98."""
     from openadmixture_jl_amd import _native as N
 
-    print("matrix-core update_q", I, J, K, flush=True)
+    print("matrix-core update passes", I, J, K, flush=True)
     Qt, Pt, bed, pitch = make_cohort(O, I, J, K, alpha=0.3, miss=0.03, seed=I + J + K, p_lo=0.01, p_hi=0.99,
                                      beta=True)
     rng = np.random.default_rng(K + 1)
@@ -85,19 +85,28 @@ def test_matrix_core_update_q(oa, O, I, J, K):
     x = np.asfortranarray(np.hstack([q, p]))
     ref = O.OracleAdmix(bed, pitch, I, J, K, 3, x)
     ref.update_q(False)
+    ref.x_next[:, :I] = q  # update_p! reads q_next
+    ref.update_p(False)
     c = oa.HIPAdmixData(I, J, K, 3)
     c.upload_bed(bed.reshape((pitch, J), order="F"))
     c.set_precision("bf16")
     htol, gtol = TOL["bf16"]
     for mc in (1, 0):
         c.set_tuning(N.ADMX_TUNE_QSIDE_MFMA, mc)
+        c.set_tuning(N.ADMX_TUNE_PSIDE_MFMA, mc)
         c.set_x(0, x)
         c.update_q(False)
-        XtX, Xtz = c.hessians(0)
-        eh, eg = _colrel(XtX, ref.XtX_q), _colrel(Xtz, ref.Xtz_q)
-        print(f"  {'matrix cores' if mc else 'vector ALU gradient'}: XtX_q {eh:.2e} Xtz_q {eg:.2e}", flush=True)
-        assert np.isfinite(XtX).all() and np.isfinite(Xtz).all()
-        assert eh < htol and eg < gtol, (mc, eh, eg)
+        XtXq, Xtzq = c.hessians(0)
+        c.set_x(1, x)
+        c.update_p(False)
+        XtXp, Xtzp = c.hessians(1)
+        eq, gq = _colrel(XtXq, ref.XtX_q), _colrel(Xtzq, ref.Xtz_q)
+        ep, gp = _colrel(XtXp, ref.XtX_p), _colrel(Xtzp, ref.Xtz_p)
+        print(f"  {'matrix cores' if mc else 'vector ALU gradient'}: XtX_q {eq:.2e} Xtz_q {gq:.2e} "
+              f"XtX_p {ep:.2e} Xtz_p {gp:.2e}", flush=True)
+        for a in (XtXq, Xtzq, XtXp, Xtzp):
+            assert np.isfinite(a).all()
+        assert eq < htol and gq < gtol and ep < htol and gp < gtol, (mc, eq, gq, ep, gp)
     c.close()
 
 
