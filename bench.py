@@ -77,6 +77,24 @@ REF_FLOPS = {
 # MI355X vector-ALU peaks in TFLOP/s: FP64 78.6; FP32 78.6 without packed FP32
 # (every build here disables v_pk_*_f32, which the 157.3 TF FP32 figure needs).
 PEAK_VALU_TFLOPS = {"f64": 78.6, "f32": 78.6}
+PEAK_MFMA_BF16_TFLOPS = 2500.0  # MI355X dense BF16 / FP16 matrix peak
+
+
+def mfma_pass_flops(kernel: str, K: int) -> int:
+    """Executed matrix-core FLOPs per genotype of the BF16 form's matrix-core
+    update passes (padding of the tiles included): per 32 x 32 block, s takes
+    ceil(6K/16) v_mfma_f32_32x32x16 (32768 FLOPs each); the Hessian
+    ceil(K(K+1)/32) 16x16x32 tiles (16384 FLOPs) per (half, group) on the Q
+    side (weights a, b), one per group on the P side (weight a + b); the
+    gradient, with two-term weights against two-row tables, 8 16x16x32 (K <= 8)
+    or 8 32x32x16 (Q side) and half that on the P side."""
+    nm = (6 * K + 15) // 16
+    nt16 = (K * (K + 1) // 2 + 15) // 16
+    if kernel == "qside_update_bf16":
+        eq = nm + 2 * nt16 + (4 if K <= 8 else 8)
+    else:
+        eq = nm + nt16 + (2 if K <= 8 else 4)
+    return 32 * eq  # 32768 FLOPs per equivalent 32x32x16 MFMA / 1024 genotypes
 
 WORKLOADS = {
     "c3": dict(I=100_000, J=100_000, K=8, alpha=0.3, p_dist="beta", p_lo=0.01, p_hi=0.99, miss_rate=0.02),
@@ -607,10 +625,18 @@ def main():
     n_launch, ms = rep[dom]
     per_launch_s = ms / n_launch / 1e3
     flops = EXEC_FLOPS[dom](K)
-    achieved = flops * I * J / per_launch_s / 1e12
     # the plain form's genotype passes run in FP64 for FP64 and FP32 contexts alike
     vunit = "f64" if args.form == "plain" else "f32"
     peak = PEAK_VALU_TFLOPS[vunit]
+    bound = "valu"
+    if args.form == "bf16" and dom in ("qside_update_bf16", "pside_update_bf16"):
+        side = "q" if dom.startswith("q") else "p"
+        flag = (tuning or {}).get("mfma_" + side, -1)
+        if flag == -1:
+            flag = args.qside_mfma if side == "q" else args.pside_mfma
+        if flag == 1 or (flag in (None, -1) and K >= 7):  # the library's default rule
+            flops, peak, bound, vunit = mfma_pass_flops(dom, K), PEAK_MFMA_BF16_TFLOPS, "mfma", "bf16 matrix"
+    achieved = flops * I * J / per_launch_s / 1e12
     frac = achieved / peak
     if not frac <= 1.0:
         sys.exit(f"roofline fraction {frac:.3f} > 1 for {dom}: the executed-FLOP count is wrong")
@@ -632,7 +658,7 @@ def main():
         "data": "synthetic cohort generated on device (Dirichlet Q, Beta/Uniform P, Binomial(2, q.p) genotypes)",
         "config": {"workload": f"{args.workload}: I={I}, J={J}, K={K}, Q=3, ZAL, {args.form} {args.dtype}{stream_note}",
                    "I": I, "J": J, "K": K, "genotypes": I * J, "bed_bytes": ((I + 3) // 4) * J},
-        "roofline": {"bound": "valu", "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
+        "roofline": {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": frac,
                      "traffic": None if args.stream_snps else
                      measured_traffic(dom, args.workload, args.dtype if args.form == "plain" else args.form),
@@ -640,7 +666,9 @@ def main():
                      "note": f"{flops} executed {vunit} flops/genotype (FMA = 2; "
                              f"{'rank-1 upper-triangle form' if dom == 'qside_update' else 'kernel algebra'}) x "
                              f"{I * J} genotypes per launch / the kernel's mean HIP-event time; "
-                             f"{vunit} vector-ALU peak (MI355X FP64 78.6 TF; FP32 78.6 TF without packed FP32)"
+                             + (f"{vunit} vector-ALU peak (MI355X FP64 78.6 TF; FP32 78.6 TF without packed FP32)"
+                                if bound == "valu" else "MI355X dense BF16 matrix peak 2.5 PF; the weights "
+                                "(about 20 vector instructions per genotype) run beside the matrix cores")
                              + (f"; the reference's full two-term count is {REF_FLOPS[dom](K)} flops/genotype"
                                 if dom in REF_FLOPS else "")},
         "kernels": kernels,

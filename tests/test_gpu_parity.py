@@ -494,5 +494,11 @@ def test_autotune_sets_waves_and_split(oa, O):
     c.set_x(0, d.x)
     t = oa.autotune(c, candidates=(8192, 65536), reps=1)
     assert t["waves"] in (8192, 65536) and t["split_q"] in (0, 1) and t["split_p"] in (0, 1)
+    assert t["mfma_q"] == -1 and t["mfma_p"] == -1
     assert oa.autotune(c, candidates=(8192, 65536), reps=1) == t
+    # the BF16 form tunes which kernel runs each side instead of the split
+    c.set_precision("bf16")
+    tb = oa.autotune(c, candidates=(8192, 65536), reps=1)
+    assert tb["mfma_q"] in (0, 1) and tb["mfma_p"] in (0, 1) and tb["split_q"] == -1
+    assert oa.autotune(c, candidates=(8192, 65536), reps=1) == tb
     c.close()
