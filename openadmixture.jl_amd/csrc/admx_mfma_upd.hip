@@ -280,8 +280,9 @@ void k_qside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int
           for (int x = 0; x < 2; ++x) {
             const int r = 4 * a + b + x;
             const float sv = sa[r], tv = 1.0f - sv;
-            const float rr = __builtin_amdgcn_rcpf(sv * tv);  // 1 ulp: the weights are FP32-grade
-            const float is = tv * rr, it = sv * rr;
+            // 1-ulp reciprocals (the weights are FP32-grade): two of them cost fewer
+            // vector instructions than one of s t and three products
+            const float is = __builtin_amdgcn_rcpf(sv), it = __builtin_amdgcn_rcpf(tv);
             cv[x] = byte_f(gb, b + x) * is;
             dv[x] = byte_f(hb, b + x) * it;
             av[x] = cv[x] * is;
@@ -581,8 +582,7 @@ void k_pside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int
         for (int x = 0; x < 2; ++x) {
           const int r = 4 * a + b + x;
           const float sv = sa[r], tv = 1.0f - sv;
-          const float rr = __builtin_amdgcn_rcpf(sv * tv);  // 1 ulp: the weights are FP32-grade
-          const float is_ = tv * rr, it = sv * rr;
+          const float is_ = __builtin_amdgcn_rcpf(sv), it = __builtin_amdgcn_rcpf(tv);  // as on the Q side
           const float cc = byte_f(gb, b + x) * is_, dd = byte_f(hb, b + x) * it;
           wv[x] = cc * is_ + dd * it;
           uv[x] = (cc - dd) * (float)(1.0 / PGRAD_SCALE);
