@@ -18,7 +18,10 @@ def _pair(oa, O, I, J, K, seed, form):
         c = oa.HIPAdmixData(I, J, K, 3, T)
         c.upload_bed(bed.reshape((pitch, J), order="F"))
         if T == np.float64:
-            c.set_precision(form)
+            c.set_precision("bf16" if form == "bf16_mfma" else form)
+        if form == "bf16_mfma":  # the matrix-core update passes (LDS image copies, barriers) under capture
+            c.set_tuning(oa._native.ADMX_TUNE_QSIDE_MFMA, 1)
+            c.set_tuning(oa._native.ADMX_TUNE_PSIDE_MFMA, 1)
         c.set_graphs(graphs)
         d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(seed + 1), T=T)
         oa.init_em(d, None, 3, d_cu=c, log=None)
@@ -28,7 +31,7 @@ def _pair(oa, O, I, J, K, seed, form):
     return out
 
 
-@pytest.mark.parametrize("form", ["f64", "bf16", "ctx_f32"])
+@pytest.mark.parametrize("form", ["f64", "bf16", "bf16_mfma", "ctx_f32"])
 def test_graph_replay_is_bit_identical(oa, O, form):
     print("graph replay", form, flush=True)
     I, J, K = 900, 2500, 4
