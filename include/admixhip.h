@@ -72,7 +72,8 @@ extern "C" {
                                    * matrix cores (gradient from two-term BF16 products), 0 the Hessian only
                                    * (s and the gradient in FP32 on the vector ALU); -1 (default): 1 from
                                    * K = 7 (faster there, slower below).  Both within the BF16 form's error
-                                   * bars against the FP64 oracle. */
+                                   * bars against the FP64 oracle.  The BF16 form's em_q sums take the same
+                                   * block-image kernel unless this is 0. */
 #define ADMX_TUNE_PSIDE_MFMA 10   /* the same for the BF16 form's update_p pass (w = a + b Hessian weight,
                                    * gradient u = c - d from two-term BF16 products) */
 

@@ -124,10 +124,16 @@ void launch_emp_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_
 // from p by launch_qimg) of the SNP-side operands in LDS layout; QT is the individual
 // split table of launch_split_tables (role 0).  jchunk: a multiple of 32.
 // Partials [chunk][K(K+1)/2 + K][I] in FP32, as launch_qside_bf16 writes them.
-int qimg_bytes(int K);
-void launch_qimg(int K, const double* p, int64_t J, void* img, hipStream_t s);
+int qimg_bytes(int K, bool em = false);
+void launch_qimg(int K, const double* p, int64_t J, void* img, bool em, hipStream_t s);
 void launch_qside_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
                        const void* QT, const void* img, float* part, hipStream_t s);
+
+// em_q! on the same design (em = true images: split rows of p and 1 - p, no pair
+// rows): sum_j c p_k + d (1 - p_k) into FP64 partials [chunk][K+1][I] (row K: 0),
+// those of the FP64 EM pass; q the FP64 state (t's 1 - sum_k q_ik).
+void launch_emq_img(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
+                    const double* q, const void* QT, const void* img, double* part, hipStream_t s);
 
 // The P-side mirror: ceil(I/32) + 1 individual-side image slots (pimg_bytes(K)
 // each, from q by launch_pimg), PT the SNP split table of launch_split_tables

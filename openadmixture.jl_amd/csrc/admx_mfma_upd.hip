@@ -54,7 +54,10 @@ __device__ __forceinline__ void copy_to_lds16(const uint8_t* src, uint8_t* dst) 
 }
 
 // Byte layout of one 32-SNP block image (and of each LDS buffer).
-template <int K> struct QImg {
+// EM (em_q!, src/loops.jl:610-632): the split rows of 1 - p as well (t from
+// its own product, accurate where s is close to 1: FRAPPE EM states are not
+// projected), no pair rows.
+template <int K, bool EM = false> struct QImg {
   static constexpr int KD = SplitDim<K>::KD, NM = SplitDim<K>::NM;
   static constexpr int NP = K * (K + 1) / 2, NT16 = (NP + 15) / 16;  // 16-pair Hessian tiles
   // Rows are padded by 16 bytes so that the 16-byte reads of lanes 0-15 fall
@@ -62,9 +65,10 @@ template <int K> struct QImg {
   static constexpr int RS = 2 * KD + 16;  // split row: KD BF16
   static constexpr int RT = 80;           // pair / gradient row: 32 BF16
   static constexpr int SP = 0;            // 32 split rows of p
-  static constexpr int PP = SP + 32 * RS; // NP pair rows p_a p_b
-  static constexpr int MM = PP + NP * RT; // NP pair rows (1-p_a)(1-p_b)
-  static constexpr int GP = MM + NP * RT; // 2K rows: bf16(p_k), then the residual p_k - bf16(p_k)
+  static constexpr int SM = SP + 32 * RS; // EM: 32 split rows of 1 - p
+  static constexpr int PP = SM + (EM ? 32 * RS : 0);  // NP pair rows p_a p_b (update)
+  static constexpr int MM = PP + (EM ? 0 : NP * RT);  // NP pair rows (1-p_a)(1-p_b) (update)
+  static constexpr int GP = MM + (EM ? 0 : NP * RT);  // 2K rows: bf16(p_k), then the residual p_k - bf16(p_k)
   static constexpr int GM = GP + 2 * K * RT;  // the same for 1 - p_k
   static constexpr int ZR = GM + 2 * K * RT;  // one zero row (padding pairs and gradient rows)
   static constexpr int BYTES = (ZR + RT + 1023) / 1024 * 1024;
@@ -83,9 +87,9 @@ __host__ __device__ constexpr int qimg_snp(int pos) {
 // SNPs outside [0, J) take p = 0.5 in the split rows (s and t stay inside
 // (0, 1), and their missing genotype codes give zero weights) and 0 in the pair
 // and gradient tables.
-template <int K>
+template <int K, bool EM>
 __global__ __launch_bounds__(256) void k_qimg(const double* __restrict__ p, int64_t J, uint8_t* __restrict__ img) {
-  using G = QImg<K>;
+  using G = QImg<K, EM>;
   const int64_t slot = blockIdx.x;
   uint8_t* __restrict__ out = img + slot * G::BYTES;
   const int64_t sblk = slot, tblk = slot - 1;  // blocks of the split rows and of the tables
@@ -99,15 +103,18 @@ __global__ __launch_bounds__(256) void k_qimg(const double* __restrict__ p, int6
     __bf16 v[8];
 #pragma unroll
     for (int e = 0; e < 8; ++e) v[e] = (__bf16)0.0f;
-    if (off < G::PP) {  // split rows: entry e of slot e / K is term PO[slot] of value k = e % K
-      const int r = (off - G::SP) / G::RS, col = ((off - G::SP) % G::RS) / 16;
+    if (off < G::PP) {  // split rows (of p, then EM's of 1 - p): entry e of slot e / K is term PO[slot] of value k = e % K
+      const bool m = EM && off >= G::SM;
+      const int base = m ? G::SM : G::SP;
+      const int r = (off - base) / G::RS, col = ((off - base) % G::RS) / 16;
       if (col < G::KD / 8) {
         constexpr int PO[6] = {0, 1, 0, 2, 1, 0};
         for (int e = 0; e < 8; ++e) {
           const int x = 8 * col + e, slot = x / K, k = x % K;
           if (slot < 6) {
             __bf16 t[3];
-            split3(pv(sblk, r, k), t[0], t[1], t[2]);
+            const double pk = pv(sblk, r, k);
+            split3(m ? 1.0 - pk : pk, t[0], t[1], t[2]);
             v[e] = t[PO[slot]];
           }
         }
@@ -170,11 +177,15 @@ __device__ __forceinline__ void swap16(uint32_t (&x)[4], uint32_t (&y)[4]) {
   }
 }
 
-template <int K>
+// EM: em_q!'s sums sum_j c p_k + d (1 - p_k) (the gradient's algebra) with
+// t = q.(1-p) + (1 - sum_k q_k) from its own product, no Hessian, written as
+// the FP64 partials [chunk][K+1][I] of the FP64 EM pass (row K: 0).
+template <int K, bool EM>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2)))
 void k_qside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk,
-                  const __bf16* __restrict__ QT, const uint8_t* __restrict__ img, float* __restrict__ part) {
-  using G = QImg<K>;
+                  const __bf16* __restrict__ QT, const uint8_t* __restrict__ img, void* __restrict__ part,
+                  const double* __restrict__ q64) {
+  using G = QImg<K, EM>;
   constexpr int KD = G::KD, NM = G::NM, NP = G::NP, NT = G::NT16, NA = NP + K;
   constexpr bool G16 = K <= 8;  // gradient rows (k leading, k residual) fit one 16-row tile
   __shared__ __align__(16) uint8_t sbuf0[G::BYTES];
@@ -195,9 +206,11 @@ void k_qside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int
   for (int m = 0; m < NM; ++m) bq[m] = load_b8(QT, KD, (int)0, (i0 + cn) * KD + 16 * m + 8 * h);
   const uint32_t shift = 2u * (uint32_t)(cn & 3);
   const int bi = cn >> 2;
+  float corr = 0.f;  // EM: t = q.(1-p) + 1 - sum_k q_k
+  if constexpr (EM) corr = (i0 + cn < I) ? sum_complement<K>(q64, i0 + cn) : 0.f;
 
   // per-lane operand offsets inside a buffer
-  const int o_sp = G::SP + cn * G::RS + 16 * h;
+  const int o_sp = G::SP + cn * G::RS + 16 * h, o_sm = G::SM + cn * G::RS + 16 * h;
   int o_h[NT];  // Hessian B: pair row 16 t + n16 of pp (even groups) or mm (odd groups), SNPs of h = g16 / 2
 #pragma unroll
   for (int t = 0; t < NT; ++t) {
@@ -224,12 +237,14 @@ void k_qside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int
     for (int c = wave; c < G::NCH; c += 4)
       copy_to_lds16(isrc + blk * G::BYTES + c * 1024, dst + c * 1024);
   };
-  auto s_mfma = [&](const uint8_t* __restrict__ buf) {  // s from a buffer's split rows
-    f32x16 sv = (f32x16){};
+  auto st_mfma = [&](const uint8_t* __restrict__ buf, f32x16& sv, f32x16& tv) {  // s (and EM's t) from the split rows
+    sv = (f32x16){};
+    tv = (f32x16){};
 #pragma unroll
-    for (int m = 0; m < NM; ++m)
+    for (int m = 0; m < NM; ++m) {
       sv = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b8(buf + o_sp + 32 * m), bq[m], sv, 0, 0, 0);
-    return sv;
+      if constexpr (EM) tv = __builtin_amdgcn_mfma_f32_32x32x16_bf16(lds_b8(buf + o_sm + 32 * m), bq[m], tv, 0, 0, 0);
+    }
   };
 
   // Slot b of the image holds the split rows of block b and the pair and
@@ -246,7 +261,8 @@ void k_qside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int
   copy_block(jb / 32, sbuf0);
   uint32_t wn = gload(jb);
   __syncthreads();  // waits for the copy (vmcnt) before the barrier
-  f32x16 sa = s_mfma(sbuf0);
+  f32x16 sa, ta;
+  st_mfma(sbuf0, sa, ta);
   copy_block(jb / 32 + 1, sbuf1);
   __syncthreads();  // the second copy has landed and every wave is done with sbuf0
   int nb = 1;
@@ -255,7 +271,8 @@ void k_qside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int
     // The other buffer was last read in the previous block, before the barrier
     // that ended it.
     if (js + 32 < je) copy_block(js / 32 + 2, nb ? sbuf0 : sbuf1);
-    const f32x16 sn = s_mfma(cur);  // block js + 32 (past the chunk on its last block: not used)
+    f32x16 sn, tn;
+    st_mfma(cur, sn, tn);  // block js + 32 (past the chunk on its last block: not used)
     // genotype bytes: byte e of this lane's word is byte 4(l%2)+e of SNP row l/2 -> tb[byte][row]
 #pragma unroll
     for (int e = 0; e < 4; ++e) tb[(4 * (lane & 1) + e) * 32 + (lane >> 1)] = (uint8_t)(wn >> (8 * e));
@@ -279,7 +296,7 @@ void k_qside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int
 #pragma unroll
           for (int x = 0; x < 2; ++x) {
             const int r = 4 * a + b + x;
-            const float sv = sa[r], tv = 1.0f - sv;
+            const float sv = sa[r], tv = EM ? ta[r] + corr : 1.0f - sv;
             // 1-ulp reciprocals (the weights are FP32-grade): two of them cost fewer
             // vector instructions than one of s t and three products
             const float is = __builtin_amdgcn_rcpf(sv), it = __builtin_amdgcn_rcpf(tv);
@@ -289,8 +306,10 @@ void k_qside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int
             bv[x] = dv[x] * it;
           }
           const int w = ((4 * a + b) >> 1) & 3;
-          wa[w] = pack_bf16x2(av[0], av[1]);
-          wb[w] = pack_bf16x2(bv[0], bv[1]);
+          if constexpr (!EM) {
+            wa[w] = pack_bf16x2(av[0], av[1]);
+            wb[w] = pack_bf16x2(bv[0], bv[1]);
+          }
           split_pack(cv[0], cv[1], ch[w], cl[w]);
           split_pack(dv[0], dv[1], dh[w], dl[w]);
         }
@@ -302,12 +321,14 @@ void k_qside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int
         z32 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(gm, b8(dh), z32, 0, 0, 0);
         z32 = __builtin_amdgcn_mfma_f32_32x32x16_bf16(gm, b8(dl), z32, 0, 0, 0);
       }
-      swap16(wa, wb);  // wa: individuals 0-15, wb: 16-31; groups (a h0) (b h0) (a h1) (b h1)
+      if constexpr (!EM) {
+        swap16(wa, wb);  // wa: individuals 0-15, wb: 16-31; groups (a h0) (b h0) (a h1) (b h1)
 #pragma unroll
-      for (int t = 0; t < NT; ++t) {
-        const bf16x8 bt = lds_b8(cur + o_h[t] + 32 * half);
-        acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b8(wa), bt, acc[0][t], 0, 0, 0);
-        acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b8(wb), bt, acc[1][t], 0, 0, 0);
+        for (int t = 0; t < NT; ++t) {
+          const bf16x8 bt = lds_b8(cur + o_h[t] + 32 * half);
+          acc[0][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b8(wa), bt, acc[0][t], 0, 0, 0);
+          acc[1][t] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(b8(wb), bt, acc[1][t], 0, 0, 0);
+        }
       }
       if constexpr (G16) {
         swap16(ch, cl);  // ch: individuals 0-15, cl: 16-31; groups (hi h0) (lo h0) (hi h1) (lo h1)
@@ -320,11 +341,36 @@ void k_qside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int
       }
     }
     sa = sn;
+    ta = tn;
     // The next block's image and genotype word have landed (the barrier waits
     // for them), and every wave is done reading `cur` before it is refilled.
     __syncthreads();
   }
-  float* __restrict__ out = part + tile.y * NA * I;
+  if constexpr (EM) {
+    double* __restrict__ o64 = static_cast<double*>(part) + tile.y * (K + 1) * I;
+    if constexpr (G16) {
+#pragma unroll
+      for (int gi = 0; gi < 2; ++gi) {
+        const int64_t col = i0 + 16 * gi + n16;
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const double v = (double)z16[gi][r] + (double)__shfl_xor(z16[gi][r], 32);
+          const int k = 4 * g16 + r;
+          if (g16 < 2 && k < K && col < I) o64[(int64_t)k * I + col] = v;
+        }
+        if (g16 == 2 && col < I) o64[(int64_t)K * I + col] = 0.0;
+      }
+    } else if (i0 + cn < I) {
+#pragma unroll
+      for (int r = 0; r < 8; ++r) {
+        const int row = 8 * (r >> 2) + 4 * h + (r & 3);
+        if (row < K) o64[(int64_t)row * I + i0 + cn] = (double)z32[r] + (double)z32[r + 8];
+      }
+      if (h == 0) o64[(int64_t)K * I + i0 + cn] = 0.0;
+    }
+    return;
+  }
+  float* __restrict__ out = static_cast<float*>(part) + tile.y * NA * I;
 #pragma unroll
   for (int gi = 0; gi < 2; ++gi)  // D[individual][pair]: lane (pair n16, g16) holds individuals 4 g16 + r
 #pragma unroll
@@ -654,16 +700,20 @@ void k_pside_mfma(const uint8_t* __restrict__ bed, int64_t pitch, int64_t I, int
 }
 
 // ------------------------------------------------------------------ launchers
-template <int K>
+template <int K, bool EM>
 static void qimg_k(const double* p, int64_t J, void* img, hipStream_t s) {
-  hipLaunchKernelGGL((k_qimg<K>), dim3((unsigned)((J + 31) / 32 + 1)), dim3(256), 0, s, p, J, (uint8_t*)img);
+  hipLaunchKernelGGL((k_qimg<K, EM>), dim3((unsigned)((J + 31) / 32 + 1)), dim3(256), 0, s, p, J, (uint8_t*)img);
 }
 template <int K>
 static void qside_mfma_k(const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
-                         const void* QT, const void* img, float* part, hipStream_t s) {
+                         const void* QT, const void* img, void* part, const double* q64, hipStream_t s) {
   const dim3 grid((unsigned)((I + 127) / 128 * nchunk));  // 1-D, tiles ordered per XCD (xcd_tile)
-  hipLaunchKernelGGL((k_qside_mfma<K>), grid, dim3(256), 0, s, bed, pitch, I, J, jchunk, (const __bf16*)QT,
-                     (const uint8_t*)img, part);
+  if (q64)
+    hipLaunchKernelGGL((k_qside_mfma<K, true>), grid, dim3(256), 0, s, bed, pitch, I, J, jchunk, (const __bf16*)QT,
+                       (const uint8_t*)img, part, q64);
+  else
+    hipLaunchKernelGGL((k_qside_mfma<K, false>), grid, dim3(256), 0, s, bed, pitch, I, J, jchunk, (const __bf16*)QT,
+                       (const uint8_t*)img, part, q64);
 }
 
 template <int K>
@@ -691,17 +741,25 @@ void launch_pside_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int6
   ADMX_DISPATCH_K(K, (pside_mfma_k<KK>(bed, pitch, I, J, ichunk, nchunk, PT, img, part, s)));
 }
 
-int qimg_bytes(int K) {
+int qimg_bytes(int K, bool em) {
   int b = 0;
-  ADMX_DISPATCH_K(K, (b = QImg<KK>::BYTES));
+  ADMX_DISPATCH_K(K, (b = em ? QImg<KK, true>::BYTES : QImg<KK, false>::BYTES));
   return b;
 }
-void launch_qimg(int K, const double* p, int64_t J, void* img, hipStream_t s) {
-  ADMX_DISPATCH_K(K, (qimg_k<KK>(p, J, img, s)));
+void launch_qimg(int K, const double* p, int64_t J, void* img, bool em, hipStream_t s) {
+  if (em) {
+    ADMX_DISPATCH_K(K, (qimg_k<KK, true>(p, J, img, s)));
+  } else {
+    ADMX_DISPATCH_K(K, (qimg_k<KK, false>(p, J, img, s)));
+  }
 }
 void launch_qside_mfma(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
                        const void* QT, const void* img, float* part, hipStream_t s) {
-  ADMX_DISPATCH_K(K, (qside_mfma_k<KK>(bed, pitch, I, J, jchunk, nchunk, QT, img, part, s)));
+  ADMX_DISPATCH_K(K, (qside_mfma_k<KK>(bed, pitch, I, J, jchunk, nchunk, QT, img, part, nullptr, s)));
+}
+void launch_emq_img(int K, const uint8_t* bed, int64_t pitch, int64_t I, int64_t J, int64_t jchunk, int nchunk,
+                    const double* q, const void* QT, const void* img, double* part, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (qside_mfma_k<KK>(bed, pitch, I, J, jchunk, nchunk, QT, img, part, q, s)));
 }
 
 }  // namespace admx

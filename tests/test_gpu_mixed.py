@@ -198,18 +198,24 @@ def test_matrix_core_em_step(oa, O, I, J, K, qsum):
     c = oa.HIPAdmixData(I, J, K, 3)
     c.upload_bed(bed.reshape((pitch, J), order="F"))
     c.set_precision("bf16")
-    c.set_x(0, x)
-    c.em_q()
-    c.em_p()
-    got = c.get_x(1)
     st = O.OracleAdmix(bed, pitch, I, J, K, 3, x)
     st.em_q()
     st.em_p()
     ref = st.x_next
-    eq = (np.abs(got[:, :I] - ref[:, :I]).max(0) / np.abs(ref[:, :I]).max(0)).max()
-    ep = np.abs(got[:, I:] - ref[:, I:]).max()
-    print(f"q_next {eq:.2e} relative, p_next {ep:.2e} absolute", flush=True)
-    assert eq <= 2e-5 and ep <= 2e-5, (eq, ep)
+    from openadmixture_jl_amd import _native as N
+
+    # em_q on the block-image kernel (default) and on k_emq_mfma (QSIDE_MFMA = 0)
+    for mc in (-1, 0):
+        c.set_tuning(N.ADMX_TUNE_QSIDE_MFMA, mc)
+        c.set_x(0, x)
+        c.em_q()
+        c.em_p()
+        got = c.get_x(1)
+        eq = (np.abs(got[:, :I] - ref[:, :I]).max(0) / np.abs(ref[:, :I]).max(0)).max()
+        ep = np.abs(got[:, I:] - ref[:, I:]).max()
+        print(f"em_q kernel {'block images' if mc else 'k_emq_mfma'}: q_next {eq:.2e} relative, "
+              f"p_next {ep:.2e} absolute", flush=True)
+        assert eq <= 2e-5 and ep <= 2e-5, (mc, eq, ep)
     # the FP64 form on the same context still matches the oracle to 1e-10
     c.set_precision("f64")
     c.em_q()
