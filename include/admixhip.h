@@ -207,9 +207,12 @@ int admx_features(admx_ctx *ctx, const float *R, const float *mu, const float *i
 int admx_warm_q(admx_ctx *ctx, admx_mlp *m);
 /* p of the state x <- every SNP's observed allele frequency (alt / 2 nonmissing,
  * clipped to [1e-5, 1 - 1e-5]) for all K populations, computed on the device
- * from the resident BED; FP64 context.  No reference counterpart (the warm
+ * from the resident BED; FP64 context.  With ll_hw non-null, *ll_hw <- the
+ * log-likelihood at that p, which every q on the simplex gives (s = q.p = f):
+ * sum over SNPs of alt log f + (2 nonmissing - alt) log(1 - f), from the same
+ * counts instead of a genotype pass.  No reference counterpart (the warm
  * start's first p). */
-int admx_warm_p(admx_ctx *ctx);
+int admx_warm_p(admx_ctx *ctx, double *ll_hw);
 
 /* ---- BED decode and counts (decode tables src/loops.jl:2-5).  No count
  * function exists in the reference; these are exact integer counts. */

@@ -55,7 +55,7 @@ SIGNATURES = {
     "admx_mlp_train": ([_vp, _vp, _vp, _i64, C.c_int32, C.c_int32, C.c_float, C.c_uint64, _vp], _int),
     "admx_features": ([_vp, _vp, _vp, _vp, _i64, _vp], _int),
     "admx_warm_q": ([_vp, _vp], _int),
-    "admx_warm_p": ([_vp], _int),
+    "admx_warm_p": ([_vp, _vp], _int),
     "admx_copy_x": ([_vp, _int, _int], _int),
     "admx_heldout_deviance": ([_vp, _vp, C.c_int32, C.c_int32, C.c_uint64, _vp, _vp], _int),
     "admx_skfr_init": ([_vp, C.c_int32, C.c_uint64, C.c_int32, _vp], _int),

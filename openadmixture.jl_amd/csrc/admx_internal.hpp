@@ -223,9 +223,11 @@ void launch_gather_rows(const float* X, int ld, const int32_t* idx, int n, float
 void launch_adam(int64_t n, float* w, const float* g, float* m, float* v, float lr, float b1, float b2, float c1,
                  float c2, float eps, hipStream_t s);
 void launch_probs_to_q(const float* Y, int64_t n, double* q, hipStream_t s);
-// p (K x J, FP64) <- clipped observed allele frequencies from per-SNP counts.
+// p (K x J, FP64) <- clipped observed allele frequencies from per-SNP counts;
+// with ll_part (one double per 256 SNPs, or null), the per-workgroup partial
+// sums of the log-likelihood at that p, which any q on the simplex shares.
 void launch_freq_to_p(const unsigned long long* nm, const unsigned long long* alt, int64_t J, int K, double* p,
-                      hipStream_t s);
+                      double* ll_part, hipStream_t s);
 
 // sparse k-means feature ranking (admx_skfr.hip)
 void launch_skfr_masks(const int32_t* lab, int64_t I, int k, int64_t nwords, uint32_t* cm, unsigned* nk,

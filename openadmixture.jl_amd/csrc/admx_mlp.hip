@@ -257,23 +257,36 @@ __global__ __launch_bounds__(256) void k_probs_to_q(const float* __restrict__ Y,
 
 // p[k,j] = observed allele frequency of SNP j, alt / max(2 nonmissing, 1),
 // clipped to [1e-5, 1 - 1e-5], for every k (the warm start's first p).
+// Also the log-likelihood of every SNP at p_kj = f_j for all k, which any q on
+// the simplex gives (s_ij = q_i . p_j = f_j): over SNP j's nonmissing
+// genotypes, sum g log f + (2 - g) log(1 - f) = alt log f + (2 nm - alt) log(1 - f).
+// One partial per workgroup, summed in a fixed order by the caller.
 __global__ __launch_bounds__(256) void k_freq_to_p(const unsigned long long* __restrict__ nm,
                                                    const unsigned long long* __restrict__ alt, int64_t J, int K,
-                                                   double* __restrict__ p) {
+                                                   double* __restrict__ p, double* __restrict__ ll_part) {
+  __shared__ double sh[256];
   const int64_t j = (int64_t)blockIdx.x * 256 + threadIdx.x;
-  if (j >= J) return;
-  const unsigned long long d = 2ull * nm[j];
-  double f = (double)alt[j] / (double)(d > 0 ? d : 1ull);
-  f = fmin(fmax(f, 1e-5), 1.0 - 1e-5);
-  for (int k = 0; k < K; ++k) p[(int64_t)K * j + k] = f;
+  double term = 0.0;
+  if (j < J) {
+    const unsigned long long d = 2ull * nm[j];
+    double f = (double)alt[j] / (double)(d > 0 ? d : 1ull);
+    f = fmin(fmax(f, 1e-5), 1.0 - 1e-5);
+    for (int k = 0; k < K; ++k) p[(int64_t)K * j + k] = f;
+    const double a = (double)alt[j], b = (double)(d - alt[j]);
+    term = a * log(f) + b * log(1.0 - f);
+  }
+  if (ll_part) {
+    const double tot = block_sum<256>(term, sh);
+    if (threadIdx.x == 0) ll_part[blockIdx.x] = tot;
+  }
 }
 
 // ------------------------------------------------------------------ launchers
 static unsigned blocks_for(int64_t n) { return (unsigned)((n + 255) / 256); }
 
 void launch_freq_to_p(const unsigned long long* nm, const unsigned long long* alt, int64_t J, int K, double* p,
-                      hipStream_t s) {
-  hipLaunchKernelGGL(k_freq_to_p, dim3(blocks_for(J)), dim3(256), 0, s, nm, alt, J, K, p);
+                      double* ll_part, hipStream_t s) {
+  hipLaunchKernelGGL(k_freq_to_p, dim3(blocks_for(J)), dim3(256), 0, s, nm, alt, J, K, p, ll_part);
 }
 
 void launch_features(const uint8_t* bed, int64_t pitch, int64_t I, int64_t Jp, const float* R, const float* mu,

@@ -343,7 +343,7 @@ int admx_warm_q(admx_ctx* c, admx_mlp* m) {
   });
 }
 
-int admx_warm_p(admx_ctx* c) {
+int admx_warm_p(admx_ctx* c, double* ll_hw) {
   return guarded([&]() -> int {
     CHK(check_ctx(c));
     Guard g(c);
@@ -351,16 +351,22 @@ int admx_warm_p(admx_ctx* c) {
     CHK(need_bed(c));
     CHK(need_resident(c, "the warm start's allele frequencies"));
     const size_t jb = sizeof(unsigned long long) * (size_t)c->J;
+    const int64_t nblk = (c->J + 255) / 256;
     CHK(c->st[0].grow(jb));
     CHK(c->st[1].grow(jb));
+    CHK(c->st[2].grow(sizeof(double) * (size_t)(nblk + 1)));
     auto* nm = static_cast<unsigned long long*>(c->st[0].p);
     auto* alt = static_cast<unsigned long long*>(c->st[1].p);
+    double* part = static_cast<double*>(c->st[2].p);
     {
       ProfScope ps(c, "warm_p");
       launch_counts_snp(c->bed, c->pitch, c->I, c->J, nm, alt, c->s);
-      launch_freq_to_p(nm, alt, c->J, (int)c->K, static_cast<double*>(c->pp(c->x[ADMX_X])), c->s);
+      launch_freq_to_p(nm, alt, c->J, (int)c->K, static_cast<double*>(c->pp(c->x[ADMX_X])), ll_hw ? part : nullptr,
+                       c->s);
+      if (ll_hw) launch_reduce_sum(part, nblk, part + nblk, 0, c->s);
     }
     CHK(check_launch());
+    if (ll_hw) HIPCHK(hipMemcpyAsync(ll_hw, part + nblk, sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     return ADMX_OK;
   });

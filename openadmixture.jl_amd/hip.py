@@ -171,9 +171,13 @@ class HIPAdmixData:
         """q of the state x <- projected network output on the stored features."""
         N.check(N.lib().admx_warm_q(self._h, net.handle))
 
-    def warm_p(self):
-        """p of the state x <- observed allele frequencies (clipped), on the device."""
-        N.check(N.lib().admx_warm_p(self._h))
+    def warm_p(self) -> float:
+        """p of the state x <- observed allele frequencies (clipped), on the device.
+        Returns the log-likelihood at that p, which every q on the simplex gives
+        (Hardy-Weinberg: s = q.p = f), summed from the same per-SNP counts."""
+        out = C.c_double()
+        N.check(N.lib().admx_warm_p(self._h, C.byref(out)))
+        return out.value
 
     def copy_x(self, dst: int, src: int):
         """State vector dst <- state vector src, on the device."""

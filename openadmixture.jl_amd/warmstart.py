@@ -194,14 +194,18 @@ def training_set(P_world, panel: Panel, n_cohorts: int, I: int = 5000, J: int = 
 
 
 def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_iter: int = 1000,
-                     rtol: float = 1e-7, p_steps: int = 1, judge: bool = True, em_iters: int = 5, Q: int = 3,
+                     rtol: float = 1e-7, p_steps: int = 2, judge: bool = True, em_iters: int = 5, Q: int = 3,
                      stop_at=None, min_gain: float = 1.0):
     """admixture_qn! from the network's proposal (with the EM judge/fallback).
 
-    p starts at the observed allele frequencies and gets p_steps fixed-Q
-    iterations (one: on the c3 and c5 workloads a second or third step cost
-    an iteration's time each and saved no quasi-Newton iteration; none
-    doubled the iterations on c5).
+    p starts at the observed allele frequencies and gets p_steps fixed-Q MM
+    steps: update_p! with q_next = q (as admixture_qn! sets it under fix_q,
+    src/algorithms_outer.jl:98-101), p = p_next after each, and one
+    log-likelihood pass at the end.  Two steps reach the P of a fixed-Q
+    quasi-Newton iteration (2 update_p! + 2 log-likelihoods + extrapolation)
+    for one log-likelihood pass less; one step left the fit an iteration
+    longer at I = J = 100,000, K = 8, and none doubled the iterations at
+    I = 200,000, J = 50,000, K = 10.
 
     The judge keeps the proposal when its log-likelihood (after the P
     refinement) is at least that of one FRAPPE EM step from the random start
@@ -228,15 +232,20 @@ def warm_started_fit(ctx: HIPAdmixData, net: MLP, panel: Panel, seed: int, max_i
     t0 = time.perf_counter()
     panel.features(ctx, to_host=False)
     ctx.warm_q(net)
-    ctx.warm_p()  # observed allele frequencies, on the device
+    # observed allele frequencies, on the device, and the log-likelihood there
+    # (p_k = f for every k: s = f whatever q is), from the same counts
+    ll_hw = ctx.warm_p()
     # admixture_qn! iterations from iter = 1 write every quasi-Newton history
     # column before reading it (src/algorithms_outer.jl:125-133), so the
     # device's U and V need no upload for these fresh runs
-    ll_hw = ctx.loglikelihood(N.ADMX_X)  # p_k = f for every k: s = f whatever q is
-    ctx.set_ll(float("nan"), ll_hw)
-    for it in range(1, p_steps + 1):
-        ctx.qn_iteration(it, fix_q=True)
-    ll_warm = ctx.get_ll()[1]
+    ll_warm = ll_hw
+    if p_steps > 0:
+        ctx.copy_block(N.ADMX_X_NEXT, N.ADMX_X, N.ADMX_SIDE_Q)  # q_next = q: Q fixed
+        for _ in range(p_steps):
+            ctx.update_p()  # p_next from (q_next, p)
+            ctx.copy_block(N.ADMX_X, N.ADMX_X_NEXT, N.ADMX_SIDE_P)
+        ll_warm = ctx.loglikelihood(N.ADMX_X)
+    ctx.set_ll(float("nan"), ll_warm)
     gain = (ll_warm - ll_hw) / max((K - 1) * J, 1)
     chosen, ll_em = "warm", None
     if judge:

@@ -133,7 +133,10 @@ def test_warm_p_and_state_copy(oa, O):
     """admx_warm_p: every p[k, j] is SNP j's observed allele frequency
     alt / max(2 nonmissing, 1), clipped to [1e-5, 1 - 1e-5] (checked against the
     oracle's counts, bit for bit, including an all-missing and a monomorphic
-    SNP); q is untouched.  admx_copy_x copies a state vector exactly."""
+    SNP); q is untouched.  The log-likelihood it returns equals the closed form
+    sum alt log f + (2 nm - alt) log(1 - f) on the oracle's counts and a
+    genotype pass at the new state (q on the simplex, so s = f), both within
+    1e-12.  admx_copy_x copies a state vector exactly."""
     print("warm p", flush=True)
     I, J, K = 333, 700, 4
     _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.05, seed=71)
@@ -145,15 +148,64 @@ def test_warm_p_and_state_copy(oa, O):
     c.upload_bed(B)
     nm, alt, _ = O.counts(np.asfortranarray(B).ravel(order="F"), pitch, I, J)
     x0 = np.asfortranarray(np.random.default_rng(2).random((K, I + J)))
+    x0[:, :I] /= x0[:, :I].sum(0)
     c.set_x(0, x0)
-    c.warm_p()
+    ll_hw = c.warm_p()
     x = c.get_x(0)
     f = np.clip(alt / np.maximum(2 * nm, 1), 1e-5, 1 - 1e-5)
     assert np.array_equal(x[:, I:], np.tile(f, (K, 1)))
     assert np.array_equal(x[:, :I], x0[:, :I])
     assert f[10] == 1e-5 and f[11] == 1 - 1e-5
+    nm64, alt64 = nm.astype(np.float64), alt.astype(np.float64)
+    ll_ref = float(np.sum(alt64 * np.log(f) + (2 * nm64 - alt64) * np.log1p(-f)))
+    ll_pass = c.loglikelihood(0)
+    print(f"warm_p LL {ll_hw:.15e}, closed form {ll_ref:.15e}, pass {ll_pass:.15e}", flush=True)
+    assert abs(ll_hw - ll_ref) <= 1e-12 * abs(ll_ref)
+    assert abs(ll_hw - ll_pass) <= 1e-12 * abs(ll_pass)
     c.copy_x(3, 0)
     assert np.array_equal(c.get_x(3), x)
+    c.close()
+
+
+@pytest.mark.parametrize("prec", ["f64", "bf16"])
+def test_warm_p_refinement_matches_oracle(oa, O, prec):
+    """The warm start's P refinement as warm_started_fit drives it: q_next = q
+    (Q fixed), then two MM steps update_p! with p = p_next after each, from p at
+    the observed allele frequencies, against the oracle's update_p! on the same
+    state (including an all-missing and a monomorphic SNP).  Plain FP64: p
+    within 1e-9 max-abs and the log-likelihood within 1e-10; BF16 form: 1e-3
+    and 1e-6 (the per-step bar of the BF16 form's update passes, whose Hessian
+    and gradient carry its error)."""
+    print(f"warm P refinement {prec}", flush=True)
+    I, J, K = 600, 900, 4
+    _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.03, seed=73)
+    B = bed.reshape((pitch, J), order="F").copy()
+    B[: (I + 3) // 4, 10] = 0x55  # all missing
+    B[: (I + 3) // 4, 11] = 0xFF  # every genotype 2
+    B = np.asfortranarray(B)
+    flat = B.ravel(order="F")
+    c = oa.HIPAdmixData(I, J, K, 3)
+    c.upload_bed(B)
+    c.set_precision(prec)
+    x0 = np.asfortranarray(np.random.default_rng(4).dirichlet(np.ones(K), size=I + J).T)
+    c.set_x(0, x0)
+    c.warm_p()
+    x = c.get_x(0)
+    o = O.OracleAdmix(flat, pitch, I, J, K, 3, x)
+    c.copy_block(1, 0, 0)
+    o.x_next[:, :I] = o.x[:, :I]
+    for _ in range(2):
+        c.update_p()
+        c.copy_block(0, 1, 1)
+        o.update_p()
+        o.x[:, I:] = o.x_next[:, I:]
+    xg = c.get_x(0)
+    dp = float(np.max(np.abs(xg[:, I:] - o.x[:, I:])))
+    ll_g, ll_o = c.loglikelihood(0), o.loglikelihood()
+    print(f"  max|dp| {dp:.2e}, LL {ll_g:.12e} vs {ll_o:.12e}", flush=True)
+    assert np.array_equal(xg[:, :I], x[:, :I])
+    assert dp <= (1e-9 if prec == "f64" else 1e-3)
+    assert abs(ll_g - ll_o) <= (1e-10 if prec == "f64" else 1e-6) * abs(ll_o)
     c.close()
 
 

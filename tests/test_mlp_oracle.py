@@ -112,9 +112,20 @@ def _scripted(oa, ll_warm, ll_em, ll_hw):
         def warm_p(self):
             self.calls.append("warm_p")
             self.xs[0][:, 4:] = 0.375
+            return ll_hw  # the log-likelihood at p_k = f for every k
 
         def copy_x(self, dst, src):
             self.xs[dst] = self.xs[src].copy(order="F")
+
+        def copy_block(self, dst, src, side):
+            cols = slice(0, 4) if side == 0 else slice(4, 10)
+            self.xs[dst][:, cols] = self.xs[src][:, cols]
+
+        def update_p(self, update2=False, status=False):
+            # the MM step reads q_next, which the warm start sets to the proposal
+            assert np.all(self.xs[1][:, :4] == 0.9)
+            self.calls.append("update_p")
+            self.xs[1][:, 4:] = self.xs[0][:, 4:] + 0.0625
 
         def get_x(self, which, out=None):
             x = self.xs[which]
@@ -133,12 +144,10 @@ def _scripted(oa, ll_warm, ll_em, ll_hw):
             return self.ll
 
         def loglikelihood(self, which=0):
-            return ll_hw  # the state right after warm_p: p_k = f for every k
+            return ll_warm  # the state after the P refinement
 
         def qn_iteration(self, it, fix_q=False, fix_p=False):
-            if fix_q:
-                self.ll = (self.ll[1], ll_warm)
-                return dict(prev=0.0, basic=0.0, qn=0.0, new=ll_warm, reldiff=1.0)
+            assert not fix_q and not fix_p
             self.calls.append("qn")
             return dict(prev=-1.0, basic=-1.0, qn=-1.0, new=-1.0, reldiff=0.0)
 
@@ -168,11 +177,12 @@ def test_judge_keeps_the_better_start(oa, ll_warm, ll_em, ll_hw, chosen):
     assert r["chosen"] == chosen
     assert r["ll_hw"] == ll_hw and r["gain"] == (ll_warm - ll_hw) / 6
     if chosen == "warm":
-        assert ctx.calls == ["features", "warm_q", "warm_p", "em1", "qn"]
+        assert ctx.calls == ["features", "warm_q", "warm_p", "update_p", "update_p", "em1", "qn"]
         assert np.all(ctx.x[:, :4] == 0.9)  # the proposal is restored after the judge's EM step
-        assert np.all(ctx.x[:, 4:] == 0.375)
+        assert np.all(ctx.x[:, 4:] == 0.5)  # with its P after two MM steps
         assert r["history"][0][1] == ll_warm
     else:
-        assert ctx.calls == ["features", "warm_q", "warm_p", "em1", "em4", "qn"]  # fallback: EM(5) from the random start
+        # fallback: EM(5) from the random start
+        assert ctx.calls == ["features", "warm_q", "warm_p", "update_p", "update_p", "em1", "em4", "qn"]
         assert np.all(ctx.x[:, :4] == 0.1)
         assert r["history"][0][1] == ll_em
