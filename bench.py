@@ -44,9 +44,10 @@ sys.path.insert(0, ROOT)
 # FLOPs per genotype of the algebra each pass kernel executes (FMA = 2;
 # multiply, add, reciprocal and log = 1), the basis of `roofline`:
 #   qside_update  FP64 rank-1 form over the upper triangle: s = q.p (2K-1),
-#                 t, s*t, 1/(s t), 1/s, 1/t, c, d, a, b, a+b, c-d, sum b, sum d (13),
-#                 (a+b) p_k, z_k += (c-d) p_k, hb_k += b p_k (5K), K(K+1)/2 FMAs
-#                 of the triangle (K^2+K)                      -> K^2 + 8K + 12
+#                 t, s*t, 1/(s t), 1/s, 1/t, c, d, a, b, a+b, sum b, sum d (12),
+#                 (a+b) p_k, hb_k += b p_k (3K), K(K+1)/2 FMAs of the triangle
+#                 (K^2+K); the gradient is formed per individual from the
+#                 triangle, sum b p and sum d after the pass   -> K^2 + 6K + 11
 #   pside_update  s (2K-1), t, s*t, 1/(s t), 1/s, 1/t, c, d, a+b as c/s + d/t (3),
 #                 c-d (11), (a+b) q_k, z_k += (c-d) q_k (3K), triangle (K^2+K)
 #                                                              -> K^2 + 6K + 10
@@ -61,7 +62,7 @@ sys.path.insert(0, ROOT)
 # 2K^2+5K+8 P side, 2K+7 log-likelihood) is larger than what these kernels
 # execute, so it is kept as the algorithmic reference figure, not the roofline.
 EXEC_FLOPS = {
-    "qside_update": lambda K: K * K + 8 * K + 12,
+    "qside_update": lambda K: K * K + 6 * K + 11,
     "pside_update": lambda K: K * K + 6 * K + 10,
     "qside_ll": lambda K: 2 * K + 2,
     "qside_update_f32": lambda K: 2 * K * K + 11 * K + 8,

@@ -66,7 +66,7 @@ extern "C" {
                                    * the free HBM holds, less max(4 GiB, 10%) kept for pass scratch */
 #define ADMX_TUNE_SPLIT_Q 7       /* FP64 update_q pass, K >= 9: 1 runs the Hessian triangle in two launches
                                    * of half the accumulators each (higher occupancy, the genotypes read
-                                   * twice), 0 in one; -1 (default): two from K = 11.  Same results. */
+                                   * twice), 0 in one; -1 (default): two from K = 13.  Same results. */
 #define ADMX_TUNE_SPLIT_P 8       /* the same for the FP64 update_p pass; -1: two from K = 13 */
 #define ADMX_TUNE_QSIDE_MFMA 9    /* BF16 form's update_q pass: 1 forms s, the Hessian and the gradient on the
                                    * matrix cores (gradient from two-term BF16 products), 0 the Hessian only

@@ -30,8 +30,8 @@ struct PassArgs {
   // term instead of the product form.  nullptr: product form.
   const int* strict = nullptr;
   // FP64 update passes, K >= 9: 1 runs the Hessian triangle in two launches of
-  // half the accumulators each, 0 in one; -1 the default (Q side K >= 11, P side
-  // K >= 13).  Results are identical either way.
+  // half the accumulators each, 0 in one; -1 the default (from K = 13 on both
+  // sides).  Results are identical either way.
   int split = -1;
 };
 
@@ -61,7 +61,7 @@ inline int pass_na(int mode, bool qside, int K, bool f64) {
   const int tri = K * (K + 1) / 2;
   if (mode == MODE_EM) return qside ? (f64 ? K + 1 : K) : 2 * K;  // FP64 EM-q: sum (c-d) p and sum d
   if (mode == MODE_UPDATE) {
-    if (qside) return f64 ? tri + 2 * K + 2 : tri + K;
+    if (qside) return f64 ? tri + K + 2 : tri + K;  // FP64: W triangle, sum b p, sum b, sum d
     return tri + K;
   }
   return 1;

@@ -29,12 +29,17 @@ namespace admx {
 //   c = gm/s, d = hm/t, a = c/s, b = d/t
 //   Xtz_q[k,i]    += c p_k + d (1 - p_k)
 //   XtX_q[k2,k,i] += a p_k p_k2 + b (1 - p_k)(1 - p_k2)
-// FP64 accumulates the equivalent rank-1 form
-//   XtX = sum (a+b) p p' + (sum b) 11' - 1 (sum b p)' - (sum b p) 1'
-//   Xtz = sum (c-d) p + (sum d) 1
-// (one FMA per Hessian entry instead of two; its cancellation costs at most a
-// factor 1/min(p,1-p)^2 <= 1e10 on a 1e-16 unit roundoff).  FP32 keeps the
-// direct two-term form, where that factor would matter.
+// FP64 accumulates W = sum (a+b) p p', hb = sum b p, sb = sum b and sd = sum d,
+// from which the finalize forms the equivalent rank-1 form
+//   XtX = W + sb 11' - 1 hb' - hb 1'
+//   Xtz = W q - hb + sd 1
+// (one FMA per Hessian entry instead of two).  The gradient needs no
+// accumulators of its own: c = a s, d = b t and s = q.p, t = 1 - s give
+// sum c p = sum a p (p.q) and sum d (1-p) = sum d - sum b p + sum b s p, so
+// sum c p + d (1-p) = W q - hb + sd exactly, whatever sum(q) is.  Both forms
+// cancel at most a factor 1/min(p,1-p)^2 <= 1e10 of the largest b term on a
+// 1e-16 unit roundoff.  FP32 keeps the direct two-term form, where that factor
+// would matter.
 // PART (update mode): 0 accumulates everything; at large K the pass runs twice,
 // PART 1 with the first half of the Hessian triangle plus the vector sums and
 // PART 2 with the second half, so that each launch holds half the accumulators
@@ -50,7 +55,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
   }
   constexpr int NT = Tri<K>::N;
   constexpr int N0 = (PART == 2) ? NT / 2 : 0, N1 = (PART == 1) ? NT / 2 : NT;  // owned triangle entries
-  constexpr bool VEC = PART != 2;                                               // owns z, hb, sb, sd
+  constexpr bool VEC = PART != 2;                                               // owns hb, sb, sd (z)
   const T* __restrict__ q = static_cast<const T*>(a.q);
   const T* __restrict__ p = static_cast<const T*>(a.p);
   const int64_t Ilen = a.i1 - a.i0;
@@ -128,7 +133,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
         for (int k = 0; k < K; ++k) z[k] = __builtin_fma(cc, pv[k], __builtin_fma(dd, (T)1 - pv[k], z[k]));
       } else if constexpr (TRICK) {
         const T aa = cc * is, bb = dd * it;
-        const T w = aa + bb, u = cc - dd;
+        const T w = aa + bb;
         if constexpr (VEC) {
           sb += bb;
           sd += dd;
@@ -137,10 +142,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
 #pragma unroll
         for (int k = 0; k < K; ++k) {
           const T wp = w * pv[k];
-          if constexpr (VEC) {
-            z[k] = __builtin_fma(u, pv[k], z[k]);
-            hb[k] = __builtin_fma(bb, pv[k], hb[k]);
-          }
+          if constexpr (VEC) hb[k] = __builtin_fma(bb, pv[k], hb[k]);
 #pragma unroll
           for (int k2 = k; k2 < K; ++k2, ++n)
             if (n >= N0 && n < N1) H[n - N0] = __builtin_fma(wp, pv[k2], H[n - N0]);
@@ -248,7 +250,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
   } else {
     if (!valid) return;
     T* __restrict__ part = static_cast<T*>(a.part);
-    const int NA = (MODE == MODE_EM) ? (sizeof(T) == 8 ? K + 1 : K) : (TRICK ? NT + 2 * K + 2 : NT + K);
+    const int NA = (MODE == MODE_EM) ? (sizeof(T) == 8 ? K + 1 : K) : (TRICK ? NT + K + 2 : NT + K);
     T* out = part + tile.y * NA * Ilen + il;
     int o = 0;
     if constexpr (MODE == MODE_UPDATE) {
@@ -257,15 +259,16 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
       o = NT;
     }
     if constexpr (VEC) {
-#pragma unroll
-      for (int k = 0; k < K; ++k) out[(o++) * Ilen] = z[k];
       if constexpr (MODE == MODE_UPDATE && TRICK) {
 #pragma unroll
         for (int k = 0; k < K; ++k) out[(o++) * Ilen] = hb[k];
         out[(o++) * Ilen] = sb;
         out[(o++) * Ilen] = sd;
+      } else {
+#pragma unroll
+        for (int k = 0; k < K; ++k) out[(o++) * Ilen] = z[k];
+        if constexpr (MODE == MODE_EM && sizeof(T) == 8) out[(o++) * Ilen] = sd;
       }
-      if constexpr (MODE == MODE_EM && sizeof(T) == 8) out[(o++) * Ilen] = sd;
     }
   }
 }
@@ -424,11 +427,14 @@ static void qside_ki(int mode, const PassArgs& a, int nchunk, hipStream_t s) {
   dim3 grid((unsigned)(((n + 255) / 256) * nchunk));  // 1-D, tiles ordered per XCD (xcd_tile)
   constexpr bool trick = sizeof(T) == 8;
   // FP64 from K = 9 on, the triangle can run in two launches (see k_qside);
-  // a.split < 0 takes the default threshold K >= 11
+  // a.split < 0 takes the default threshold K >= 13.  One launch holds
+  // K(K+1)/2 + K + 2 FP64 accumulators per lane: 235 / 255 VGPRs at K = 11 / 12
+  // (two waves per SIMD; I = J = 100,000: 41.6 / 45.2 ms against 47.6 / 58.1 ms
+  // for two launches), 296 at K = 13 (one wave).
   if (mode == MODE_UPDATE) {
     bool done = false;
     if constexpr (sizeof(T) == 8 && K >= 9) {
-      if (a.split > 0 || (a.split < 0 && K >= 11)) {
+      if (a.split > 0 || (a.split < 0 && K >= 13)) {
         hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick, IMP, 1>), grid, dim3(256), 0, s, a);
         hipLaunchKernelGGL((k_qside<T, K, MODE_UPDATE, trick, IMP, 2>), grid, dim3(256), 0, s, a);
         done = true;

@@ -185,12 +185,16 @@ __device__ __forceinline__ int qp_from_hessian(T* __restrict__ L, int t, const T
 // ------------------------------------------------------------------ Q side finalize
 // Partials of type PT are summed, and the rank-1 form recombined, in the wider
 // of T and PT (an FP32 context's passes run in FP64); the QP runs in T.
+// Rank-1 (TRICK) partials carry W = sum (a+b) p p' (upper triangle), hb = sum b p,
+// sb = sum b and sd = sum d; with q the pass's own state (x_in),
+//   XtX = W + sb 11' - 1 hb' - hb 1',  Xtz = W q - hb + sd 1
+// (sum c p + d (1-p) over the same genotypes, by c = a s, d = b t, s = q.p).
 template <typename T, int K, bool TRICK, bool DO_QP, typename PT = T>
 __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_q_update(FinArgs f) {
   using AT = typename std::conditional<(sizeof(PT) > sizeof(T)), PT, T>::type;
   constexpr int NT = QPLanes<K>::N;
   constexpr int NTRI = Tri<K>::N;
-  constexpr int NA = TRICK ? NTRI + 2 * K + 2 : NTRI + K;
+  constexpr int NA = TRICK ? NTRI + K + 2 : NTRI + K;
   using LY = QPLayout<T, K, 1, NT>;
   __shared__ T lds[DO_QP ? LY::TOTAL * NT : 1];
   const int t = threadIdx.x;
@@ -205,22 +209,44 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_q_update(FinArgs f) {
 #pragma unroll
     for (int a = 0; a < NA; ++a) acc[a] += (AT)src[(int64_t)a * f.n];
   }
+  const int64_t g = f.off + col;
   T Hf[K * K], xtz[K];
-  {
+  if constexpr (TRICK) {
+    const T* __restrict__ qin = static_cast<const T*>(f.x_in) + (int64_t)K * g;
+    AT qv[K], wq[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      qv[k] = (AT)qin[k];
+      wq[k] = 0;
+    }
+    const AT sb = acc[NTRI + K], sd = acc[NTRI + K + 1];
     int n = 0;
 #pragma unroll
     for (int k = 0; k < K; ++k)
 #pragma unroll
       for (int k2 = k; k2 < K; ++k2, ++n) {
-        AT h = acc[n];
-        if constexpr (TRICK) h = h + acc[NTRI + 2 * K] - acc[NTRI + K + k] - acc[NTRI + K + k2];
+        const AT w = acc[n];
+        // (W q)_k summed over ascending columns: entries (k', k), k' < k, come from earlier rows
+        wq[k] = __builtin_fma(w, qv[k2], wq[k]);
+        if (k2 != k) wq[k2] = __builtin_fma(w, qv[k], wq[k2]);
+        const AT h = w + sb - acc[NTRI + k] - acc[NTRI + k2];
         Hf[k2 + K * k] = (T)h;
         Hf[k + K * k2] = (T)h;
       }
 #pragma unroll
-    for (int k = 0; k < K; ++k) xtz[k] = (T)(TRICK ? acc[NTRI + k] + acc[NTRI + 2 * K + 1] : acc[NTRI + k]);
+    for (int k = 0; k < K; ++k) xtz[k] = (T)(wq[k] - acc[NTRI + k] + sd);
+  } else {
+    int n = 0;
+#pragma unroll
+    for (int k = 0; k < K; ++k)
+#pragma unroll
+      for (int k2 = k; k2 < K; ++k2, ++n) {
+        Hf[k2 + K * k] = (T)acc[n];
+        Hf[k + K * k2] = (T)acc[n];
+      }
+#pragma unroll
+    for (int k = 0; k < K; ++k) xtz[k] = (T)acc[NTRI + k];
   }
-  const int64_t g = f.off + col;
   if (f.XtX) {
     T* X = static_cast<T*>(f.XtX) + (int64_t)K * K * g;
 #pragma unroll
