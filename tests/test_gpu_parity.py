@@ -131,6 +131,46 @@ def test_update_loops(oa, O, I, J, K, seed, T):
     c.close()
 
 
+def test_update_q_gradient_near_the_bounds(oa, O):
+    """The FP64 update_q pass forms Xtz = W q - sum b p + sum d from its Hessian
+    accumulators (W = sum (a+b) p p').  Where s is within 1e-5 of 1 and a
+    genotype is below 2, b = (2-g)/t^2 reaches 1e10 and the terms of size b
+    cancel to one of size d (1-p): the difference from the direct two-term sums
+    is then bounded by a few unit roundoffs of sum_j b_ij, as for the rank-1
+    Hessian.  A quarter of the SNPs sit at the projection's bound 1 - 1e-5 in
+    every population (a state the fits never reach; it maximises the
+    cancellation)."""
+    note("update_q gradient near the bounds")
+    I, J, K = 300, 2000, 6
+    Qt, Pt, bed, pitch = make_cohort(O, I, J, K, seed=21)
+    q, p = random_state(I, J, K, seed=21)
+    p[:, : J // 4] = 1 - 1e-5
+    c = ctx_for(oa, bed, pitch, I, J, K)
+    gX = np.zeros((K, K, I), order="F")
+    gZ = np.zeros((K, I), order="F")
+    c.update_q_loop(gX, gZ, q, p, 0, I, 0, J)
+    rX = np.zeros((K, K, I), order="F")
+    rZ = np.zeros((K, I), order="F")
+    O.update_q_loop(rX, rZ, bed, pitch, q, p, (0, I), (0, J))
+    g = O.decode(bed, pitch, I, J).astype(np.float64)
+    t = 1.0 - q.T @ p
+    b = np.where(g >= 0, 2.0 - g, 0.0) / t**2
+    bound = 64 * np.finfo(np.float64).eps * b.sum(1) + 1e-9 * np.abs(rZ).max(0)
+    err = np.abs(gZ - rZ).max(0)
+    assert np.all(err <= bound), (err / bound).max()
+    # the same bound holds the rank-1 Hessian
+    errX = np.abs(gX - rX).max((0, 1))
+    assert np.all(errX <= bound + 1e-9 * np.abs(rX).max((0, 1))), (errX / bound).max()
+    # away from the bound (p in [0.02, 0.98]) the gradient matches the direct sums closely
+    j0 = J // 4
+    gZ[...] = 0
+    rZ[...] = 0
+    c.update_q_loop(gX, gZ, q, p, 0, I, j0, J)
+    O.update_q_loop(rX, rZ, bed, pitch, q, p, (0, I), (j0, J))
+    assert (np.abs(gZ - rZ).max(0) / np.abs(rZ).max(0)).max() < 1e-11
+    c.close()
+
+
 @pytest.mark.parametrize("T", [F64, F32])
 @pytest.mark.parametrize("I,J,K,seed", CASES[:3])
 def test_em_loops(oa, O, I, J, K, seed, T):
@@ -479,6 +519,10 @@ def test_hessian_split_switch_is_exact(oa, O, K):
     assert np.array_equal(out[0][2], out[1][2])
     ref.update_q(False)
     ok, e = _colmax_close(out[1][0][0], ref.XtX_q, 1e-9)
+    assert ok, e
+    # the gradient formed from the triangle, sum b p and sum d after the pass,
+    # at an EM state whose q columns do not sum to 1
+    ok, e = _colmax_close(out[1][0][1], ref.Xtz_q, 1e-9)
     assert ok, e
     c.close()
 
