@@ -116,6 +116,9 @@ KERNEL_SYMBOLS = {  # bench kernel name -> rocprof symbol prefix of its K=8 (ski
     "qside_ll": "void admx::k_qside<double, 8, 2, false, false",
     "qside_update_bf16": "_ZN4admx12k_qside_bf16ILi8E",
     "pside_update_bf16": "void admx::k_pside_bf16<8>",
+    # the BF16 form's matrix-core update passes (the default from K = 7)
+    "qside_update_bf16_mfma": "void admx::k_qside_mfma<8, false>",
+    "pside_update_bf16_mfma": "void admx::k_pside_mfma<8>",
 }
 
 
@@ -662,7 +665,8 @@ def main():
         "roofline": {"bound": bound, "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
                      "frac": frac,
                      "traffic": None if args.stream_snps else
-                     measured_traffic(dom, args.workload, args.dtype if args.form == "plain" else args.form),
+                     measured_traffic(dom + ("_mfma" if bound == "mfma" else ""), args.workload,
+                                      args.dtype if args.form == "plain" else args.form),
                      "flops_per_genotype": flops, "avg_ms": per_launch_s * 1e3,
                      "note": f"{flops} executed {vunit} flops/genotype (FMA = 2; "
                              f"{'rank-1 upper-triangle form' if dom == 'qside_update' else 'kernel algebra'}) x "
