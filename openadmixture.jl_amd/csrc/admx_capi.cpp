@@ -270,11 +270,12 @@ int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
   if (mode == MODE_LL && !c->impute) {
     CHK(c->llflag.grow(2 * sizeof(int)));
     flags = static_cast<int*>(c->llflag.p);
+    const bool f32 = c->dtype == ADMX_F32;  // widened FP32 state
     launch_ll_state_check((int)c->K, static_cast<const double*>(q), c->I, static_cast<const double*>(p), c->J,
-                          flags, c->s);
+                          flags, c->s, f32);
     if (q2)
       launch_ll_state_check((int)c->K, static_cast<const double*>(q2), c->I, static_cast<const double*>(p2), c->J,
-                            flags + 1, c->s);
+                            flags + 1, c->s, f32);
     CHK(check_launch());
   }
   if (mode == MODE_LL) CHK(c->llpart.grow((size_t)(nbx * total) * (q2 ? 2 : 1) * sizeof(double)));

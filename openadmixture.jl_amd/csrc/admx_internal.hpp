@@ -196,7 +196,8 @@ void launch_gen_bed_rows(const double* Qt, const double* Pt, int64_t I, int64_t 
 // exactly 0 or 1, or holds a non-finite value; else 0.  Safe states: every p
 // in [1e-12, 1 - 1e-12], every q finite and >= 0, every column sum of q in
 // [1e-200, 1 + 1e-13]; then every s and 1 - s lies strictly inside (0, 1).
-void launch_ll_state_check(int K, const double* q, int64_t I, const double* p, int64_t J, int* flag, hipStream_t s);
+void launch_ll_state_check(int K, const double* q, int64_t I, const double* p, int64_t J, int* flag, hipStream_t s,
+                           bool f32_state = false);
 // y = (double)x, n values.
 void launch_widen(const float* x, int64_t n, double* y, hipStream_t s);
 // `rows` rows of a .bed file's payload (ibytes each, back to back; `real` of

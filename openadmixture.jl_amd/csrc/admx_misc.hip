@@ -548,8 +548,15 @@ void launch_gen_bed(const double* Qt, const double* Pt, int64_t I, int64_t J, in
 // the term-by-term form whenever this check cannot rule that out.
 __global__ void k_flag_clear(int* flag) { *flag = 0; }
 
+// The product form needs 0 < s < 1 for every s = q.p: q >= 0 with column sums
+// in [1e-200, 1 + eps] and p in [1e-12, 1 - delta] give s <= (1 + eps)(1 - delta)
+// < 1 for eps < delta.  FP64 states: eps = 1e-13, delta = 1e-12.  FP32 states
+// (an FP32 context's q and p, widened): a projected column's FP32 sum is 1 only
+// to a few 1e-7 (up to K = 16 roundings), so eps = 4e-6 and delta = 8e-6 (the
+// FP32 projection's bound 1 - 1e-5 is 1.0014e-5 from 1).
 __global__ __launch_bounds__(256) void k_ll_state_check(int K, const double* __restrict__ q, int64_t I,
-                                                        const double* __restrict__ p, int64_t J, int* flag) {
+                                                        const double* __restrict__ p, int64_t J, double eps,
+                                                        double delta, int* flag) {
   bool bad = false;
   const int64_t stride = (int64_t)gridDim.x * 256;
   for (int64_t i = (int64_t)blockIdx.x * 256 + threadIdx.x; i < I; i += stride) {
@@ -559,21 +566,23 @@ __global__ __launch_bounds__(256) void k_ll_state_check(int K, const double* __r
       bad |= !(v >= 0.0 && v <= 1e300);  // also NaN
       sum += v;
     }
-    bad |= !(sum >= 1e-200 && sum <= 1.0 + 1e-13);
+    bad |= !(sum >= 1e-200 && sum <= 1.0 + eps);
   }
   const int64_t np = (int64_t)K * J;
   for (int64_t e = (int64_t)blockIdx.x * 256 + threadIdx.x; e < np; e += stride) {
     const double v = p[e];
-    bad |= !(v >= 1e-12 && v <= 1.0 - 1e-12);
+    bad |= !(v >= 1e-12 && v <= 1.0 - delta);
   }
   if (__syncthreads_or(bad) && threadIdx.x == 0) atomicOr(flag, 1);
 }
 
-void launch_ll_state_check(int K, const double* q, int64_t I, const double* p, int64_t J, int* flag, hipStream_t s) {
+void launch_ll_state_check(int K, const double* q, int64_t I, const double* p, int64_t J, int* flag, hipStream_t s,
+                           bool f32_state) {
   hipLaunchKernelGGL(k_flag_clear, dim3(1), dim3(1), 0, s, flag);
   const int64_t n = std::max<int64_t>(I, (int64_t)K * J);
   const unsigned blocks = (unsigned)std::min<int64_t>(std::max<int64_t>((n + 255) / 256, 1), 1024);
-  hipLaunchKernelGGL(k_ll_state_check, dim3(blocks), dim3(256), 0, s, K, q, I, p, J, flag);
+  const double eps = f32_state ? 4e-6 : 1e-13, delta = f32_state ? 8e-6 : 1e-12;
+  hipLaunchKernelGGL(k_ll_state_check, dim3(blocks), dim3(256), 0, s, K, q, I, p, J, eps, delta, flag);
 }
 
 // FP64 copy of an FP32 array (an FP32 context's q or p for the FP64 passes).
