@@ -41,6 +41,22 @@ void launch_reduce_sum_if(const double* parts, int64_t nparts, double* out, cons
   hipLaunchKernelGGL(k_reduce_sum_if, dim3(1), dim3(256), 0, s, parts, nparts, out, flag, want);
 }
 
+// Block sums of NP per-thread values of a 256-thread block, stored to out[0, NP):
+// a fixed shuffle tree per wave, then the four wave sums in wave order.
+template <int NP>
+__device__ __forceinline__ void block_store_sums(const double (&acc)[NP], double* sh, double* __restrict__ out) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+#pragma unroll
+  for (int a = 0; a < NP; ++a) {
+    double x = acc[a];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if (lane == 0) sh[wave * NP + a] = x;
+  }
+  __syncthreads();
+  for (int a = threadIdx.x; a < NP; a += 256) out[a] = ((sh[a] + sh[NP + a]) + sh[2 * NP + a]) + sh[3 * NP + a];
+}
+
 // ------------------------------------------------------------------ ZAL quasi-Newton
 // update_UV!: U[:,col] = x_next - x, V[:,col] = x_next2 - x_next, and the
 // partial inner products A[a,b] = U[:,a]'(U[:,b] - V[:,b]) and
@@ -49,8 +65,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_zal_dots(int64_t N, const T* __restrict__ x, const T* __restrict__ xn,
                                                   const T* __restrict__ xn2, T* __restrict__ U, T* __restrict__ V,
                                                   int col, int m, double* __restrict__ part) {
-  __shared__ double sh[256];
   constexpr int NP = kMaxQ * kMaxQ + kMaxQ;
+  __shared__ double sh[4 * NP];
   double acc[NP];
 #pragma unroll
   for (int a = 0; a < NP; ++a) acc[a] = 0.0;
@@ -78,25 +94,42 @@ __global__ __launch_bounds__(256) void k_zal_dots(int64_t N, const T* __restrict
       for (int b = 0; b < kMaxQ; ++b) acc[a + kMaxQ * b] += (double)uu[a] * (double)(uu[b] - vv[b]);
     }
   }
-#pragma unroll
-  for (int a = 0; a < NP; ++a) {
-    const double tot = block_sum<256>(acc[a], sh);
-    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * NP + a] = tot;
-  }
+  block_store_sums<NP>(acc, sh, part + (int64_t)blockIdx.x * NP);
 }
 
-// Sum the block partials in block order, then coef = inv(A) * r with the
-// inverse by Gauss-Jordan elimination with partial pivoting (Julia's inv).
-__global__ __launch_bounds__(64) void k_zal_solve(const double* __restrict__ part, int nblocks, int m,
-                                                  double* __restrict__ coef) {
-  constexpr int NP = kMaxQ * kMaxQ + kMaxQ;
-  __shared__ double tot[NP];
-  for (int a = threadIdx.x; a < NP; a += 64) {
-    double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += part[(int64_t)b * NP + a];
-    tot[a] = s;
+// Sum of the NP per-block partials of a dots kernel over nblocks blocks into
+// tot (shared), by a 256-thread block in a fixed order: wave w owns the values
+// a = w, w + 4, ..., its lane l adds blocks l, l + 64, ... of all of them at
+// once (every load independent), and a fixed shuffle tree combines the lanes.
+template <int NP>
+__device__ __forceinline__ void sum_partials(const double* __restrict__ part, int nblocks, double* tot) {
+  constexpr int NV = (NP + 3) / 4;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double acc[NV];
+#pragma unroll
+  for (int v = 0; v < NV; ++v) acc[v] = 0.0;
+  for (int b = lane; b < nblocks; b += 64) {
+#pragma unroll
+    for (int v = 0; v < NV; ++v)
+      if (wave + 4 * v < NP) acc[v] += part[(int64_t)b * NP + wave + 4 * v];
+  }
+#pragma unroll
+  for (int v = 0; v < NV; ++v) {
+    double x = acc[v];
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+    if (lane == 0 && wave + 4 * v < NP) tot[wave + 4 * v] = x;
   }
   __syncthreads();
+}
+
+// Sum the block partials (sum_partials), then coef = inv(A) * r with the
+// inverse by Gauss-Jordan elimination with partial pivoting (Julia's inv).
+__global__ __launch_bounds__(256) void k_zal_solve(const double* __restrict__ part, int nblocks, int m,
+                                                   double* __restrict__ coef) {
+  constexpr int NP = kMaxQ * kMaxQ + kMaxQ;
+  __shared__ double tot[NP];
+  sum_partials<NP>(part, nblocks, tot);
   if (threadIdx.x != 0) return;
   double W[kMaxQ][2 * kMaxQ];
   const int w = 2 * m;
@@ -194,7 +227,7 @@ void launch_zal(int K, int64_t I, int64_t J, const void* x, const void* xn, cons
   double* coef = scratch + nb * NP;
   hipLaunchKernelGGL((k_zal_dots<T>), dim3((unsigned)nb), dim3(256), 0, s, N, (const T*)x, (const T*)xn,
                      (const T*)xn2, (T*)U, (T*)V, col, m, part);
-  hipLaunchKernelGGL(k_zal_solve, dim3(1), dim3(64), 0, s, part, (int)nb, m, coef);
+  hipLaunchKernelGGL(k_zal_solve, dim3(1), dim3(256), 0, s, part, (int)nb, m, coef);
   ADMX_DISPATCH_K(K, (zal_apply_q_k<T, KK>(I, N, xn, V, coef, m, x_out, s)));
   const int64_t np = N - (int64_t)K * I;
   hipLaunchKernelGGL((k_zal_apply_p<T>), dim3(grid_for(np, 4096)), dim3(256), 0, s, (int64_t)K * I, N,
@@ -210,8 +243,8 @@ template <typename T>
 __global__ __launch_bounds__(256) void k_lbqn_dots(int64_t N, const T* __restrict__ x, const T* __restrict__ xn,
                                                    const T* __restrict__ xn2, T* __restrict__ U, T* __restrict__ V,
                                                    int m, double* __restrict__ part) {
-  __shared__ double sh[256];
   constexpr int NP = kMaxQ * kMaxQ + kMaxQ;
+  __shared__ double sh[4 * NP];
   double acc[NP];
 #pragma unroll
   for (int a = 0; a < NP; ++a) acc[a] = 0.0;
@@ -232,11 +265,7 @@ __global__ __launch_bounds__(256) void k_lbqn_dots(int64_t N, const T* __restric
       for (int b = 0; b < kMaxQ; ++b) acc[a + kMaxQ * b] += (double)vv[a] * (double)vv[b];
     }
   }
-#pragma unroll
-  for (int a = 0; a < NP; ++a) {
-    const double tot = block_sum<256>(acc[a], sh);
-    if (threadIdx.x == 0) part[(int64_t)blockIdx.x * NP + a] = tot;
-  }
+  block_store_sums<NP>(acc, sh, part + (int64_t)blockIdx.x * NP);
 }
 
 // The two-loop recursion of update_QN_LBQN! (src/quasi_newton.jl:37-58) in
@@ -244,16 +273,11 @@ __global__ __launch_bounds__(256) void k_lbqn_dots(int64_t N, const T* __restric
 //   alpha_i = (r_i - sum_{k<i} alpha_k G[i,k]) / G[i,i],  gamma = r_0 / G[0,0],
 // and x_next = x - gamma u + sum_i gamma alpha_i V_i - sum_i alpha_i U_i:
 // coef[0..kMaxQ) multiplies U, coef[kMaxQ..2kMaxQ) multiplies V.
-__global__ __launch_bounds__(64) void k_lbqn_solve(const double* __restrict__ part, int nblocks, int m,
-                                                   double* __restrict__ coef) {
+__global__ __launch_bounds__(256) void k_lbqn_solve(const double* __restrict__ part, int nblocks, int m,
+                                                    double* __restrict__ coef) {
   constexpr int NP = kMaxQ * kMaxQ + kMaxQ;
   __shared__ double tot[NP];
-  for (int a = threadIdx.x; a < NP; a += 64) {
-    double s = 0.0;
-    for (int b = 0; b < nblocks; ++b) s += part[(int64_t)b * NP + a];
-    tot[a] = s;
-  }
-  __syncthreads();
+  sum_partials<NP>(part, nblocks, tot);
   if (threadIdx.x != 0) return;
   const double* G = tot;
   const double* r = tot + kMaxQ * kMaxQ;
@@ -321,7 +345,7 @@ void launch_lbqn(int K, int64_t I, int64_t J, const void* x, const void* xn, con
   double* coef = scratch + nb * NP;
   hipLaunchKernelGGL((k_lbqn_dots<T>), dim3((unsigned)nb), dim3(256), 0, s, N, (const T*)x, (const T*)xn,
                      (const T*)xn2, (T*)U, (T*)V, m, part);
-  hipLaunchKernelGGL(k_lbqn_solve, dim3(1), dim3(64), 0, s, part, (int)nb, m, coef);
+  hipLaunchKernelGGL(k_lbqn_solve, dim3(1), dim3(256), 0, s, part, (int)nb, m, coef);
   ADMX_DISPATCH_K(K, (lbqn_apply_q_k<T, KK>(I, N, x, U, V, coef, m, x_out, s)));
   const int64_t np = N - (int64_t)K * I;
   hipLaunchKernelGGL((k_lbqn_apply_p<T>), dim3(grid_for(np, 4096)), dim3(256), 0, s, (int64_t)K * I, N,
