@@ -488,6 +488,80 @@ def calibrate_copy(nbytes=1 << 30):
     print(json.dumps({"calibrate_copy_bytes": nbytes, "copies": 3}), flush=True)
 
 
+def roofline_of(rep, form, K, I, J, tuning, args):
+    """Dominant pass kernel of a profiled run and the FLOP rate of the algebra
+    it executes against the peak of the unit that runs it (FP64 / non-packed
+    FP32 vector ALU, or the BF16 matrix cores for the BF16 form's matrix-core
+    update passes)."""
+    dom = max((k for k in rep if k in EXEC_FLOPS), key=lambda k: rep[k][1])
+    n_launch, ms = rep[dom]
+    per_launch_s = ms / n_launch / 1e3
+    flops = EXEC_FLOPS[dom](K)
+    # the plain form's genotype passes run in FP64 for FP64 and FP32 contexts alike
+    vunit = "f64" if form == "plain" else "f32"
+    peak = PEAK_VALU_TFLOPS[vunit]
+    bound = "valu"
+    if form == "bf16" and dom in ("qside_update_bf16", "pside_update_bf16"):
+        side = "q" if dom.startswith("q") else "p"
+        flag = (tuning or {}).get("mfma_" + side, -1)
+        if flag == -1:
+            flag = args.qside_mfma if side == "q" else args.pside_mfma
+        if flag == 1 or (flag in (None, -1) and K >= 7):  # the library's default rule
+            flops, peak, bound, vunit = mfma_pass_flops(dom, K), PEAK_MFMA_BF16_TFLOPS, "mfma", "bf16 matrix"
+    achieved = flops * I * J / per_launch_s / 1e12
+    frac = achieved / peak
+    if not frac <= 1.0:
+        sys.exit(f"roofline fraction {frac:.3f} > 1 for {dom}: the executed-FLOP count is wrong")
+    return dict(kernel=dom, flops=flops, peak=peak, bound=bound, vunit=vunit, achieved=achieved, frac=frac,
+                per_launch_s=per_launch_s)
+
+
+def bf16_form_block(oa, ctx, wl, args):
+    """The BF16 form (update passes, log-likelihood and EM sums on the matrix
+    cores; FP64 log-likelihood reduction, QP and convergence test) on the same
+    cohort and context: EM(5) priming and warmup + steps iterations from the
+    same seeded start as the plain line, timed the same way, with the roofline
+    of its dominant kernel against the BF16 matrix peak."""
+    I, J, K = wl["I"], wl["J"], wl["K"]
+    ctx.set_precision("bf16")
+    try:
+        d = oa.AdmixData(I, J, K, 3, rng=np.random.default_rng(args.seed + 1))
+        oa.init_em(d, None, 5, d_cu=ctx, log=None)
+        ctx.set_uv(d.U, d.V)
+        it = 0
+        for _ in range(args.warmup):
+            it += 1
+            ctx.qn_iteration(it)
+        ctx.profile(True)
+        ctx.profile_reset()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            it += 1
+            ctx.qn_iteration(it)
+        ctx.synchronize()
+        el = time.perf_counter() - t0
+        rep = ctx.profile_report()
+        ctx.profile(False)
+    finally:
+        ctx.set_precision("f64")
+    roof = roofline_of(rep, "bf16", K, I, J, None, args)
+    dom = roof["kernel"]
+    return {
+        "value": I * J * args.steps / el, "unit": "genotype*iter/s", "ms_per_step": el / args.steps * 1e3,
+        "steps": args.steps, "dtype": "bf16 passes, f64 LL/QP",
+        "roofline": {"bound": roof["bound"], "kernel": dom, "achieved": roof["achieved"], "peak": roof["peak"],
+                     "unit": "TFLOP/s", "frac": roof["frac"],
+                     "traffic": measured_traffic(dom + ("_mfma" if roof["bound"] == "mfma" else ""), args.workload,
+                                                 "bf16"),
+                     "flops_per_genotype": roof["flops"], "avg_ms": roof["per_launch_s"] * 1e3,
+                     "note": f"{roof['flops']} executed {roof['vunit']} flops/genotype of the dominant kernel x "
+                             f"{I * J} genotypes per launch / its mean HIP-event time"},
+        "kernels": {k: {"launches": n, "avg_ms": round(t / n, 4), "share": round(t / (el * 1e3), 4)}
+                    for k, (n, t) in sorted(rep.items(), key=lambda kv: -kv[1][1])[:8]},
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -506,6 +580,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true", help="only run a 1 GiB device copy (counter scaling)")
     ap.add_argument("--target-waves", type=int, default=0, help="waves per genotype pass (0: library default)")
     ap.add_argument("--no-ttc", action="store_true", help="skip the time-to-converge runs")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="skip the BF16 form's throughput block of the plain line")
     ap.add_argument("--graphs", action="store_true", help="replay steady-state iterations from hipGraphs")
     ap.add_argument("--qside-mfma", type=int, default=None, choices=[-1, 0, 1],
                     help="BF16 form's update_q pass: 1 all products on the matrix cores, 0 Hessian only")
@@ -624,26 +700,9 @@ def main():
     ctx.profile(False)
 
     value = I * J * args.steps / el
-    # dominant kernel and the FLOP rate of the algebra it executes
-    dom = max((k for k in rep if k in EXEC_FLOPS), key=lambda k: rep[k][1])
-    n_launch, ms = rep[dom]
-    per_launch_s = ms / n_launch / 1e3
-    flops = EXEC_FLOPS[dom](K)
-    # the plain form's genotype passes run in FP64 for FP64 and FP32 contexts alike
-    vunit = "f64" if args.form == "plain" else "f32"
-    peak = PEAK_VALU_TFLOPS[vunit]
-    bound = "valu"
-    if args.form == "bf16" and dom in ("qside_update_bf16", "pside_update_bf16"):
-        side = "q" if dom.startswith("q") else "p"
-        flag = (tuning or {}).get("mfma_" + side, -1)
-        if flag == -1:
-            flag = args.qside_mfma if side == "q" else args.pside_mfma
-        if flag == 1 or (flag in (None, -1) and K >= 7):  # the library's default rule
-            flops, peak, bound, vunit = mfma_pass_flops(dom, K), PEAK_MFMA_BF16_TFLOPS, "mfma", "bf16 matrix"
-    achieved = flops * I * J / per_launch_s / 1e12
-    frac = achieved / peak
-    if not frac <= 1.0:
-        sys.exit(f"roofline fraction {frac:.3f} > 1 for {dom}: the executed-FLOP count is wrong")
+    roof = roofline_of(rep, args.form, K, I, J, tuning, args)
+    dom, flops, peak, bound, vunit, achieved, frac, per_launch_s = (
+        roof[k] for k in ("kernel", "flops", "peak", "bound", "vunit", "achieved", "frac", "per_launch_s"))
     kernels = {k: {"launches": n, "avg_ms": round(t / n, 4), "share": round(t / (el * 1e3), 4)}
                for k, (n, t) in sorted(rep.items(), key=lambda kv: -kv[1][1])}
     out = {
@@ -681,6 +740,8 @@ def main():
         "gen_seconds": t_gen,
         "tuning": tuning,
     }
+    if args.form == "plain" and args.dtype == "f64" and not args.stream_snps and not args.no_forms:
+        out["bf16_form"] = bf16_form_block(oa, ctx, wl, args)
     if not args.no_ttc and args.dtype == "f64":
         tol = 1e-4
         hist, its = converge_run(oa, ctx, wl, T, args.seed, "f64", args.ttc_max_iter)
