@@ -269,7 +269,10 @@ def converge_run(oa, ctx, wl, T, seed, form, max_iter, rtol=1e-7, stop_at=None):
 
 
 def _warm_kw(args):
-    return {} if args.warm_p_steps is None else {"p_steps": args.warm_p_steps}
+    kw = {} if args.warm_p_steps is None else {"p_steps": args.warm_p_steps}
+    if args.warm_q_steps is not None:
+        kw["q_steps"] = args.warm_q_steps
+    return kw
 
 
 def first_within(hist, ll_star, tol):
@@ -609,6 +612,8 @@ def main():
                          "evaluation cohort's (0: the same populations)")
     ap.add_argument("--warm-p-steps", type=int, default=None,
                     help="fixed-Q P refinement iterations of the warm start (default: the library's)")
+    ap.add_argument("--warm-q-steps", type=int, default=None,
+                    help="fixed-P update_q! steps of an accepted warm start (default: the library's)")
     ap.add_argument("--K", type=int, default=0, help="override the workload's K (throughput runs)")
     ap.add_argument("--skfr-iters", type=int, default=10, help="SKFR iterations of the AIM pre-selection block")
     ap.add_argument("--no-skfr", action="store_true", help="skip the SKFR AIM pre-selection block")

@@ -143,8 +143,15 @@ def _scripted(oa, ll_warm, ll_em, ll_hw):
         def get_ll(self):
             return self.ll
 
+        def update_q(self, update2=False, status=False):
+            # the fixed-P step of an accepted proposal reads the restored proposal
+            assert np.all(self.xs[0][:, :4] == 0.9) and np.all(self.xs[0][:, 4:] == 0.5)
+            self.calls.append("update_q")
+            self.xs[1][:, :4] = 0.95
+
         def loglikelihood(self, which=0):
-            return ll_warm  # the state after the P refinement
+            # the state after the P refinement, then after the fixed-P step
+            return ll_warm + (1.0 if "update_q" in self.calls else 0.0)
 
         def qn_iteration(self, it, fix_q=False, fix_p=False):
             assert not fix_q and not fix_p
@@ -177,10 +184,12 @@ def test_judge_keeps_the_better_start(oa, ll_warm, ll_em, ll_hw, chosen):
     assert r["chosen"] == chosen
     assert r["ll_hw"] == ll_hw and r["gain"] == (ll_warm - ll_hw) / 6
     if chosen == "warm":
-        assert ctx.calls == ["features", "warm_q", "warm_p", "update_p", "update_p", "em1", "qn"]
-        assert np.all(ctx.x[:, :4] == 0.9)  # the proposal is restored after the judge's EM step
-        assert np.all(ctx.x[:, 4:] == 0.5)  # with its P after two MM steps
-        assert r["history"][0][1] == ll_warm
+        # the proposal is restored after the judge's EM step, then takes one fixed-P step
+        assert ctx.calls == ["features", "warm_q", "warm_p", "update_p", "update_p", "em1", "update_q", "qn"]
+        assert np.all(ctx.x[:, :4] == 0.95)  # q after the fixed-P step
+        assert np.all(ctx.x[:, 4:] == 0.5)  # P after two MM steps
+        assert r["ll_warm"] == ll_warm  # the judge saw the P-refined proposal
+        assert r["history"][0][1] == ll_warm + 1.0  # the fit starts after the fixed-P step
     else:
         # fallback: EM(5) from the random start
         assert ctx.calls == ["features", "warm_q", "warm_p", "update_p", "update_p", "em1", "em4", "qn"]
