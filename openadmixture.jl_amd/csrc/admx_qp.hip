@@ -26,8 +26,7 @@ struct QPLayout {
   static constexpr int SZ = K + C + 1;
   static constexpr int TAB = 0;
   static constexpr int DG = SZ * SZ;
-  static constexpr int TMP = DG + SZ;
-  static constexpr int DEL = TMP + SZ;
+  static constexpr int DEL = DG + SZ;
   static constexpr int PAR = DEL + K;
   static constexpr int TOTAL = PAR + K;
 };
@@ -37,12 +36,13 @@ template <int K> struct QPLanes { static constexpr int N = K <= 12 ? 64 : 32; };
 
 #define QP_TB(r, c) L[((r) + SZ * (c)) * NT + t]
 #define QP_DG(i) L[(LY::DG + (i)) * NT + t]
-#define QP_TMP(i) L[(LY::TMP + (i)) * NT + t]
 #define QP_DEL(i) L[(LY::DEL + (i)) * NT + t]
 #define QP_PAR(i) L[(LY::PAR + (i)) * NT + t]
 
 // sweep!(A, k, tmp, inverse)  (src/qp.jl:297-317).  The saved column lives in
-// registers (constant index, inner loop) and in LDS (runtime index, outer loop).
+// registers; both loops are unrolled, so the SZ^2 read-modify-writes of the
+// tableau have compile-time LDS offsets and are issued back to back (their
+// arithmetic, element by element, is the reference's: A[j,i] -= (p tmp[i]) tmp[j]).
 template <typename T, int K, int C, int NT>
 __device__ __forceinline__ void qp_sweep(T* __restrict__ L, int t, int k, bool inverse) {
 #pragma clang fp contract(off)
@@ -59,12 +59,10 @@ __device__ __forceinline__ void qp_sweep(T* __restrict__ L, int t, int k, bool i
   }
   const T sg = inverse ? (T)1 : (T)-1;
 #pragma unroll
+  for (int i = 0; i < SZ; ++i) tmp[i] = (i == k) ? sg : tmp[i];
+#pragma unroll
   for (int i = 0; i < SZ; ++i) {
-    tmp[i] = (i == k) ? sg : tmp[i];
-    QP_TMP(i) = tmp[i];
-  }
-  for (int i = 0; i < SZ; ++i) {
-    const T pv = pk * QP_TMP(i);
+    const T pv = pk * tmp[i];
 #pragma unroll
     for (int j = 0; j < SZ; ++j) QP_TB(j, i) = QP_TB(j, i) - pv * tmp[j];
   }
