@@ -477,14 +477,18 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_qp_apply(FinArgs f, const T* 
   }
 }
 
-// Split the finalize into a reduction kernel and k_qp_apply when the fused
-// kernel's tableau takes more than 80 KiB of LDS per 64 problems, i.e. when
-// only one fused workgroup fits a CU (I = J = 100,000, K = 12: fin_q 12.1 ->
-// 2.4 ms, fin_p 10 -> 1.3 ms; at K = 8, two workgroups per CU, the fused
-// kernel is faster: 0.73 vs 0.86 ms).
+// Split the finalize into a reduction kernel and k_qp_apply from K = 9
+// (simplex) / 10 (box).  The fused kernel holds two 64-problem workgroups per
+// CU (one from about 80 KiB of tableau per workgroup, K = 11 / 12), so it
+// reads the partials at low occupancy; the reduction kernel reads them at full
+// occupancy and k_qp_apply reads the sums.  I = J = 100,000, K = 12: fin_q
+// 12.1 -> 2.4 ms, fin_p 10 -> 1.3 ms; I = 200,000, J = 50,000, K = 10, P side
+// (84 chunks x 65 partials per SNP): 1.68 ms fused against 0.28 split; at
+// K = 8 (c3: 40 chunks x 44 partials) the fused kernel is faster, 0.73 vs
+// 0.86 ms.
 template <typename T, int K, int C>
 constexpr bool qp_split() {
-  return (size_t)QPLayout<T, K, C, QPLanes<K>::N>::TOTAL * QPLanes<K>::N * sizeof(T) > 81920;
+  return C == 1 ? K >= 9 : K >= 10;
 }
 
 // Launch the split path if it applies; false leaves the fused kernel to the caller.

@@ -303,7 +303,7 @@ def _pair(oa, O, I, J, K, Q=3, T=F64, seed=0, miss=0.01, beta=False, p_lo=0.05, 
 
 @pytest.mark.parametrize("K", [4, 8, 10, 12, 13, 16])
 def test_update_q_p_device(oa, O, K):
-    # K >= 10 (Q side) / 11 (P side) runs the finalize as a reduction kernel
+    # K >= 9 (Q side) / 10 (P side) runs the finalize as a reduction kernel
     # followed by the separate QP kernel; K >= 13 splits the Hessian triangle
     # over two pass launches on both sides; K = 16 is the largest K
     note("update_q!/update_p!", K)
