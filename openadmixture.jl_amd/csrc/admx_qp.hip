@@ -7,10 +7,11 @@
 //   update_p!: per SNP create_tableau!(simplex=false) + quadratic_program!(c=0);
 //              p_next = p + delta; project_p!  (src/algorithms_inner.jl:275-305)
 //
-// One lane owns one problem.  Its (K+c+1)^2 sweep tableau, diagonal copy,
-// sweep column, step and parameters sit in LDS, interleaved across the
-// workgroup (element e of lane t at e*NT + t), so every LDS access of a wave
-// touches consecutive addresses.  The active-set loop is the reference's
+// One lane owns one problem.  Its (K+c+1)^2 sweep tableau sits in LDS,
+// interleaved across the workgroup (element e of lane t at e*NT + t), so every
+// LDS access of a wave touches consecutive addresses; the step, the parameters,
+// the initial diagonal and the swept column live in registers (every loop over
+// them is unrolled, so their indices are compile-time).  The active-set loop is the reference's
 // Jennrich-Sampson method step for step; arithmetic is evaluated without FMA
 // contraction, as Julia evaluates it outside @turbo.
 #include <type_traits>
@@ -25,19 +26,13 @@ template <typename T, int K, int C, int NT>
 struct QPLayout {
   static constexpr int SZ = K + C + 1;
   static constexpr int TAB = 0;
-  static constexpr int DG = SZ * SZ;
-  static constexpr int DEL = DG + SZ;
-  static constexpr int PAR = DEL + K;
-  static constexpr int TOTAL = PAR + K;
+  static constexpr int TOTAL = SZ * SZ;
 };
 
 // Lanes per workgroup of the QP kernels: the LDS footprint grows as K^2.
 template <int K> struct QPLanes { static constexpr int N = K <= 12 ? 64 : 32; };
 
 #define QP_TB(r, c) L[((r) + SZ * (c)) * NT + t]
-#define QP_DG(i) L[(LY::DG + (i)) * NT + t]
-#define QP_DEL(i) L[(LY::DEL + (i)) * NT + t]
-#define QP_PAR(i) L[(LY::PAR + (i)) * NT + t]
 
 // sweep!(A, k, tmp, inverse)  (src/qp.jl:297-317).  The saved column lives in
 // registers; both loops are unrolled, so the SZ^2 read-modify-writes of the
@@ -70,73 +65,82 @@ __device__ __forceinline__ void qp_sweep(T* __restrict__ L, int t, int k, bool i
 
 // quadratic_program!(delta, tableau, par, pmin=0, pmax=1, p=K, c)  (src/qp.jl:171-292)
 template <typename T, int K, int C, int NT>
-__device__ int qp_solve(T* __restrict__ L, int t) {
+__device__ int qp_solve(T* __restrict__ L, int t, const T (&par)[K], T (&del)[K]) {
 #pragma clang fp contract(off)
   using LY = QPLayout<T, K, C, NT>;
   constexpr int SZ = LY::SZ, E = SZ - 1;
   const T small = (T)1e-5, tol = (T)1e-8;
-  for (int i = 0; i < K; ++i) QP_DEL(i) = 0;
-  for (int i = 0; i < SZ; ++i) QP_DG(i) = QP_TB(i, i);
+  T dg[K];  // the diagonal before any sweep
+#pragma unroll
   for (int i = 0; i < K; ++i) {
-    if (QP_DG(i) <= (T)0 || QP_TB(i, i) < QP_DG(i) * tol) return 0;
+    del[i] = 0;
+    dg[i] = QP_TB(i, i);
+  }
+  // Sweep the parameters (rejected when a pivot is not positive or has lost
+  // all but tol of its initial value), then the constraint (rejected when its
+  // pivot is not negative).  One sweep instance serves every step (rolled loop;
+  // dg by an unrolled select, no register indexing).
+#pragma unroll 1
+  for (int i = 0; i < K + C; ++i) {
+    const T di = QP_TB(i, i);
+    if (i < K) {
+      T d0 = dg[0];
+#pragma unroll
+      for (int k = 1; k < K; ++k) d0 = (i == k) ? dg[k] : d0;
+      if (d0 <= (T)0 || di < d0 * tol) return 0;
+    } else if (di >= (T)0) {
+      return 0;
+    }
     qp_sweep<T, K, C, NT>(L, t, i, false);
   }
   uint32_t swept = (K >= 32) ? 0xffffffffu : ((1u << K) - 1u);
-  for (int i = K; i < K + C; ++i) {
-    if (QP_TB(i, i) >= (T)0) return 0;
-    qp_sweep<T, K, C, NT>(L, t, i, false);
-  }
   for (int iteration = 1; iteration <= 1000; ++iteration) {
     T a = 1;
+#pragma unroll
     for (int i = 0; i < K; ++i) {
       if ((swept >> i) & 1u) {
         const T ui = QP_TB(i, E);
         T ai;
-        if (ui > (T)0) ai = (T)1 - QP_PAR(i) - QP_DEL(i);
-        else ai = (T)0 - QP_PAR(i) - QP_DEL(i);
+        if (ui > (T)0) ai = (T)1 - par[i] - del[i];
+        else ai = (T)0 - par[i] - del[i];
         if ((T)fabs((double)ui) > (T)1e-10) {
           const T r = ai / ui;
           a = r < a ? r : a;
         }
       }
     }
+#pragma unroll
     for (int i = 0; i < K; ++i) {
       if ((swept >> i) & 1u) {
         const T ui = QP_TB(i, E);
-        QP_DEL(i) = QP_DEL(i) + a * ui;
-        QP_TB(i, E) = ((T)1 - a) * ui;
-        QP_TB(E, i) = QP_TB(i, E);
+        del[i] = del[i] + a * ui;
+        const T nu = ((T)1 - a) * ui;
+        QP_TB(i, E) = nu;
+        QP_TB(E, i) = nu;
       }
     }
+    // a swept parameter that is critical is inverse-swept; otherwise an unswept
+    // one that violates the KKT conditions is swept; otherwise solved
     int pick = -1;
+#pragma unroll
     for (int i = 0; i < K; ++i) {
-      const T xi = QP_PAR(i) + QP_DEL(i);
+      const T xi = par[i] + del[i];
       const bool critical = ((T)0 >= xi - small) || ((T)1 <= xi + small);
-      if (((swept >> i) & 1u) && fabs((double)QP_TB(i, i)) > 1e-10 && critical) {
-        pick = i;
-        break;
+      if (pick < 0 && ((swept >> i) & 1u) && fabs((double)QP_TB(i, i)) > 1e-10 && critical) pick = i;
+    }
+    const bool inverse = pick >= 0;
+    if (!inverse) {
+#pragma unroll
+      for (int i = 0; i < K; ++i) {
+        const T ui = QP_TB(i, E);
+        const T xi = par[i] + del[i];
+        const bool violation = (ui > (T)0 && (T)0 >= xi - small) || (ui < (T)0 && (T)1 <= xi + small);
+        if (pick < 0 && !((swept >> i) & 1u) && violation) pick = i;
       }
+      if (pick < 0) return iteration;
     }
-    if (pick >= 0) {
-      qp_sweep<T, K, C, NT>(L, t, pick, true);
-      swept &= ~(1u << pick);
-      continue;
-    }
-    for (int i = 0; i < K; ++i) {
-      const T ui = QP_TB(i, E);
-      const T xi = QP_PAR(i) + QP_DEL(i);
-      const bool violation = (ui > (T)0 && (T)0 >= xi - small) || (ui < (T)0 && (T)1 <= xi + small);
-      if (!((swept >> i) & 1u) && violation) {
-        pick = i;
-        break;
-      }
-    }
-    if (pick >= 0) {
-      qp_sweep<T, K, C, NT>(L, t, pick, false);
-      swept |= 1u << pick;
-      continue;
-    }
-    return iteration;
+    qp_sweep<T, K, C, NT>(L, t, pick, inverse);
+    swept ^= 1u << pick;  // inverse: the bit was set; forward: it was clear
   }
   return 0;
 }
@@ -147,7 +151,7 @@ __device__ int qp_solve(T* __restrict__ L, int t) {
 // beta = 0 clears y before reading x, so the reference executes mu = 0.
 template <typename T, int K, int C, int NT>
 __device__ __forceinline__ int qp_from_hessian(T* __restrict__ L, int t, const T (&Hf)[K * K], const T (&xtz)[K],
-                                               const T (&par)[K]) {
+                                               const T (&par)[K], T (&del)[K]) {
 #pragma clang fp contract(off)
   using LY = QPLayout<T, K, C, NT>;
   constexpr int SZ = LY::SZ;
@@ -163,7 +167,6 @@ __device__ __forceinline__ int qp_from_hessian(T* __restrict__ L, int t, const T
   for (int i = 0; i < K; ++i) {
     QP_TB(SZ - 1, i) = xtz[i];  // -r with r = -Xtz
     QP_TB(i, SZ - 1) = xtz[i];
-    QP_PAR(i) = par[i];
   }
   if constexpr (C == 1) {
 #pragma unroll
@@ -177,7 +180,7 @@ __device__ __forceinline__ int qp_from_hessian(T* __restrict__ L, int t, const T
     QP_TB(K + 1, K) = (T)1 - sx;
     QP_TB(K, K + 1) = (T)1 - sx;
   }
-  return qp_solve<T, K, C, NT>(L, t);
+  return qp_solve<T, K, C, NT>(L, t, par, del);
 }
 
 // ------------------------------------------------------------------ Q side finalize
@@ -264,11 +267,12 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_q_update(FinArgs f) {
 #pragma unroll
     for (int k = 0; k < K; ++k) par[k] = xin[k];
     T* L = lds;
-    const int rc = qp_from_hessian<T, K, 1, NT>(L, t, Hf, xtz, par);
+    T del[K];
+    const int rc = qp_from_hessian<T, K, 1, NT>(L, t, Hf, xtz, par, del);
     if (f.status) f.status[g] = rc;
     T v[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = par[k] + QP_DEL(k);
+    for (int k = 0; k < K; ++k) v[k] = par[k] + del[k];
     project_simplex<T, K>(v);
     T* xo = static_cast<T*>(f.x_out) + (int64_t)K * g;
 #pragma unroll
@@ -362,11 +366,12 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_fin_p_update(FinArgs f) {
 #pragma unroll
     for (int k = 0; k < K; ++k) par[k] = xin[k];
     T* L = lds;
-    const int rc = qp_from_hessian<T, K, 0, NT>(L, t, Hf, xtz, par);
+    T del[K];
+    const int rc = qp_from_hessian<T, K, 0, NT>(L, t, Hf, xtz, par, del);
     if (f.status) f.status[g] = rc;
     T* xo = static_cast<T*>(f.x_out) + (int64_t)K * g;
 #pragma unroll
-    for (int k = 0; k < K; ++k) xo[k] = project_box<T>(par[k] + QP_DEL(k));
+    for (int k = 0; k < K; ++k) xo[k] = project_box<T>(par[k] + del[k]);
   }
 }
 
@@ -429,10 +434,11 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_qp_batch(int64_t n, const T* 
     par[k] = x[(int64_t)K * col + k];
   }
   T* L = lds;
-  const int rc = qp_from_hessian<T, K, C, NT>(L, t, Hf, xtz, par);
+  T del[K];
+  const int rc = qp_from_hessian<T, K, C, NT>(L, t, Hf, xtz, par, del);
   if (status) status[col] = rc;
 #pragma unroll
-  for (int k = 0; k < K; ++k) delta[(int64_t)K * col + k] = QP_DEL(k);
+  for (int k = 0; k < K; ++k) delta[(int64_t)K * col + k] = del[k];
 }
 
 // QP + step + projection of the reduced Hessians: the second half of the
@@ -461,27 +467,28 @@ __global__ __launch_bounds__(QPLanes<K>::N) void k_qp_apply(FinArgs f, const T* 
     par[k] = xin[k];
   }
   T* L = lds;
-  const int rc = qp_from_hessian<T, K, C, NT>(L, t, Hf, xtz, par);
+  T del[K];
+  const int rc = qp_from_hessian<T, K, C, NT>(L, t, Hf, xtz, par, del);
   if (f.status) f.status[g] = rc;
   T* xo = static_cast<T*>(f.x_out) + (int64_t)K * g;
   if constexpr (C == 1) {
     T v[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) v[k] = par[k] + QP_DEL(k);
+    for (int k = 0; k < K; ++k) v[k] = par[k] + del[k];
     project_simplex<T, K>(v);
 #pragma unroll
     for (int k = 0; k < K; ++k) xo[k] = v[k];
   } else {
 #pragma unroll
-    for (int k = 0; k < K; ++k) xo[k] = project_box<T>(par[k] + QP_DEL(k));
+    for (int k = 0; k < K; ++k) xo[k] = project_box<T>(par[k] + del[k]);
   }
 }
 
 // Split the finalize into a reduction kernel and k_qp_apply from K = 9
-// (simplex) / 10 (box).  The fused kernel holds two 64-problem workgroups per
-// CU (one from about 80 KiB of tableau per workgroup, K = 11 / 12), so it
-// reads the partials at low occupancy; the reduction kernel reads them at full
-// occupancy and k_qp_apply reads the sums.  I = J = 100,000, K = 12: fin_q
+// (simplex) / 10 (box).  The fused kernel holds two or three 64-problem
+// workgroups per CU (the tableau's LDS: 50 KiB per workgroup at K = 8 simplex,
+// 100 KiB at K = 12), so it reads the partials at low occupancy; the reduction
+// kernel reads them at full occupancy and k_qp_apply reads the sums.  I = J = 100,000, K = 12: fin_q
 // 12.1 -> 2.4 ms, fin_p 10 -> 1.3 ms; I = 200,000, J = 50,000, K = 10, P side
 // (84 chunks x 65 partials per SNP): 1.68 ms fused against 0.28 split; at
 // K = 8 (c3: 40 chunks x 44 partials) the fused kernel is faster, 0.73 vs
