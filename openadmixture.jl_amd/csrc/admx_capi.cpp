@@ -1193,6 +1193,7 @@ int create_impl(admx_ctx** out, int64_t I, int64_t J, int64_t K, int64_t Q, int 
   if (!out) return bad("null output pointer");
   *out = nullptr;
   if (I < 1 || J < 1) return bad("I and J must be >= 1");
+  if (I >= (1LL << 31) || J >= (1LL << 31)) return bad("I and J must be below 2^31");
   if (K < 1 || K > kMaxK) return bad("K must be in 1..16");
   if (Q < 1 || Q > kMaxQ) return bad("Q must be in 1..8");
   if (dtype != ADMX_F64 && dtype != ADMX_F32) return bad("dtype must be ADMX_F64 or ADMX_F32");

@@ -40,6 +40,15 @@ __device__ __forceinline__ Tile2 xcd_tile(int64_t nx) {
 __device__ __forceinline__ int code_g(uint32_t c) { return (0x02010000u >> (c * 8u)) & 0xffu; }
 __device__ __forceinline__ int code_h(uint32_t c) { return (0x00010002u >> (c * 8u)) & 0xffu; }
 
+// (m*g, m*(2-g)) of the four codes as one 8- or 16-byte LDS entry: a pass
+// reads the pair for a code with one LDS load instead of two bit-field
+// extracts and two integer-to-float conversions on the vector ALU.
+template <typename T> struct __align__(2 * sizeof(T)) CodeGH { T g, h; };
+template <typename T> __device__ __forceinline__ void code_table_init(CodeGH<T>* tab) {
+  if (threadIdx.x < 4) tab[threadIdx.x] = {(T)code_g(threadIdx.x), (T)code_h(threadIdx.x)};
+  __syncthreads();
+}
+
 // Reciprocal: hardware estimate refined to full precision.  FP64: the estimate
 // r = (1 + d)/x carries |d| up to about 2^-24 (measured on gfx950: 2.6e8 ulp);
 // one third-order step r (1 + e + e^2), e = 1 - x r = -d, leaves (1 + d^3)/x,

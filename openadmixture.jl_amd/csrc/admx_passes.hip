@@ -84,10 +84,13 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
   for (int k = 0; k < ((MODE == MODE_UPDATE && TRICK) ? K : 1); ++k) hb[k] = 0;
 
   const T* __restrict__ mu = static_cast<const T*>(a.mu);
-  // (m*g, m*(2-g)) of code c in SNP j; imputed mean for a missing code under IMP.
-  auto weights = [&](uint32_t c, int64_t j, T& gm, T& hm) {
-    gm = (T)code_g(c);
-    hm = (T)code_h(c);
+  __shared__ CodeGH<T> ctab[4];
+  code_table_init(ctab);
+  // (m*g, m*(2-g)) of code c in SNP j (gh = ctab[c], read one genotype ahead);
+  // imputed mean for a missing code under IMP.
+  auto weights = [&](uint32_t c, const CodeGH<T>& gh, int64_t j, T& gm, T& hm) {
+    gm = gh.g;
+    hm = gh.h;
     if constexpr (IMP) {
       if (c == 1u) {
         const T mj = mu[j];
@@ -97,7 +100,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
     }
   };
   // Per-genotype accumulation for genotype code c of SNP j and P column pv.
-  auto step = [&](uint32_t c, int64_t j, const T (&pv)[K]) {
+  auto step = [&](uint32_t c, const CodeGH<T>& gh, int64_t j, const T (&pv)[K]) {
     T s = qi[0] * pv[0];
 #pragma unroll
     for (int k = 1; k < K; ++k) s = __builtin_fma(qi[k], pv[k], s);
@@ -112,13 +115,13 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
         // Terms in T, accumulated in Float64 (src/loops.jl:523, 535); a missing
         // entry adds 0 log s + 0 log(1-s), NaN where s is 0 or 1, as there.
         T gm, hm;
-        weights(c, j, gm, hm);
+        weights(c, gh, j, gm, hm);
         const T term = gm * tlog<T>(s) + hm * tlog<T>(t);
         acc += (double)term;
       }
     } else {
       T gm, hm;
-      weights(c, j, gm, hm);
+      weights(c, gh, j, gm, hm);
       const T r = rcp(s * t);
       const T is = t * r, it = s * r;
       const T cc = gm * is, dd = hm * it;
@@ -213,6 +216,8 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
         w0 = *reinterpret_cast<const uint32_t*>(wrow + a.pitch * (js + 16));
         w1 = *reinterpret_cast<const uint32_t*>(wrow + a.pitch * (js + 16) + 16);
       }
+      auto code = [&](int u) { return (cur[u >> 2] >> (8 * (u & 3) + shw)) & 3u; };
+      CodeGH<T> ghn = ctab[code(0)];
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
         T pv[K];
@@ -221,7 +226,9 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
         const int64_t jn = (u < 15) ? js + u + 1 : ((js + 16 < je) ? js + 16 : js + 15);
 #pragma unroll
         for (int k = 0; k < K; ++k) pn[k] = p[(int64_t)K * jn + k];
-        step((cur[u >> 2] >> (8 * (u & 3) + shw)) & 3u, js + u, pv);
+        const CodeGH<T> gh = ghn;
+        if (u < 15) ghn = ctab[code(u + 1)];
+        step(code(u), gh, js + u, pv);
         if ((u & 7) == 7) renorm();
       }
     }
@@ -235,7 +242,8 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
       T pv[K];
 #pragma unroll
       for (int k = 0; k < K; ++k) pv[k] = p[(int64_t)K * js + k];
-      step((gp[a.pitch * js] >> sh) & 3u, js, pv);
+      const uint32_t c = (gp[a.pitch * js] >> sh) & 3u;
+      step(c, ctab[c], js, pv);
       renorm();
     }
   }
@@ -297,6 +305,8 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
 #pragma unroll
   for (int k = 0; k < K; ++k) pj[k] = p[k + (int64_t)K * j];
   const uint8_t* __restrict__ col = a.bed + a.pitch * j;
+  __shared__ CodeGH<T> ctab[4];
+  code_table_init(ctab);
   T mj = 0;
   if constexpr (IMP) mj = static_cast<const T*>(a.mu)[j];
 
@@ -315,13 +325,18 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
   // then computes, so each wait covers one load issued a genotype earlier.
   // inr = 0 marks an individual outside [i0, i1) whose code was set to missing:
   // it adds nothing, also under IMP (only genuinely missing entries are imputed).
+  // Individual indices are below 2^31 (checked at context creation): the clamp
+  // compares 32-bit values, which the scalar unit does (a 64-bit signed compare
+  // would take a vector instruction per genotype).
   T qn[K];
+  const int32_t I32 = (int32_t)a.I;
   auto loadq = [&](int64_t i) {
-    const T* __restrict__ src = q + (int64_t)K * (i < a.I ? i : a.I - 1);
+    const int32_t i32 = (int32_t)i;
+    const T* __restrict__ src = q + (int64_t)K * (i32 < I32 ? i32 : I32 - 1);
 #pragma unroll
     for (int k = 0; k < K; ++k) qn[k] = src[k];
   };
-  auto body = [&](uint32_t c, int64_t next, uint32_t inr) {
+  auto body = [&](uint32_t c, const CodeGH<T>& gh, int64_t next, uint32_t inr) {
     T qv[K];
 #pragma unroll
     for (int k = 0; k < K; ++k) qv[k] = qn[k];
@@ -330,7 +345,7 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
 #pragma unroll
     for (int k = 1; k < K; ++k) s = __builtin_fma(qv[k], pj[k], s);
     const T tt = (T)1 - s;
-    T gm = (T)code_g(c), hm = (T)code_h(c);
+    T gm = gh.g, hm = gh.h;
     if constexpr (IMP) {
       if (c == 1u && inr) {
         gm = mj;
@@ -395,8 +410,13 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
           if (ib + t >= a.i0 && ib + t < a.i1) keep |= 3u << (2 * t);
         w = (w & keep) | (0x55555555u & ~keep);
       }
+      CodeGH<T> ghn = ctab[w & 3u];
 #pragma unroll
-      for (int t = 0; t < 16; ++t) body((w >> (2 * t)) & 3u, ib + t + 1, (keep >> (2 * t)) & 1u);
+      for (int t = 0; t < 16; ++t) {
+        const CodeGH<T> gh = ghn;
+        if (t < 15) ghn = ctab[(w >> (2 * t + 2)) & 3u];
+        body((w >> (2 * t)) & 3u, gh, ib + t + 1, (keep >> (2 * t)) & 1u);
+      }
     }
   }
   if (!valid) return;
