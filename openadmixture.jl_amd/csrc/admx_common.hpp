@@ -49,6 +49,18 @@ template <typename T> __device__ __forceinline__ void code_table_init(CodeGH<T>*
   __syncthreads();
 }
 
+// Factors of the product-form log-likelihood term s^g (1-s)^(2-g) of a code as
+// x = ax s + bx, y = ay s + by (two FMAs, each rounding exactly as 1 - s or s):
+//   00 -> (1-s)(1-s)   01 (missing) -> 1 * 1   10 -> s (1-s)   11 -> s s
+struct __align__(16) CodeLL { double ax, bx, ay, by; };
+__device__ __forceinline__ void code_table_init(CodeLL* tab) {
+  const uint32_t c = threadIdx.x;
+  if (c < 4)
+    tab[c] = {c >= 2u ? 1.0 : (c == 1u ? 0.0 : -1.0), c >= 2u ? 0.0 : 1.0,
+              c == 3u ? 1.0 : (c == 1u ? 0.0 : -1.0), c == 3u ? 0.0 : 1.0};
+  __syncthreads();
+}
+
 // Reciprocal: hardware estimate refined to full precision.  FP64: the estimate
 // r = (1 + d)/x carries |d| up to about 2^-24 (measured on gfx950: 2.6e8 ulp);
 // one third-order step r (1 + e + e^2), e = 1 - x r = -d, leaves (1 + d^3)/x,

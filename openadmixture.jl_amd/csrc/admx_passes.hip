@@ -19,6 +19,7 @@
 // Every chunk writes its partial sums to HBM; a second kernel reduces them in
 // a fixed order, so results are bitwise reproducible run to run (the
 // reference's CUDA path uses float atomics).
+#include <type_traits>
 #include "admx_common.hpp"
 #include "admx_internal.hpp"
 
@@ -84,7 +85,11 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
   for (int k = 0; k < ((MODE == MODE_UPDATE && TRICK) ? K : 1); ++k) hb[k] = 0;
 
   const T* __restrict__ mu = static_cast<const T*>(a.mu);
-  __shared__ CodeGH<T> ctab[4];
+  // per-code constants in LDS: the product-form log-likelihood's factors, or
+  // (m*g, m*(2-g)) for every other instance
+  constexpr bool LLPROD = MODE == MODE_LL && sizeof(T) == 8 && !IMP && !STRICT;
+  using CE = typename std::conditional<LLPROD, CodeLL, CodeGH<T>>::type;
+  __shared__ CE ctab[4];
   code_table_init(ctab);
   // (m*g, m*(2-g)) of code c in SNP j (gh = ctab[c], read one genotype ahead);
   // imputed mean for a missing code under IMP.
@@ -100,17 +105,17 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
     }
   };
   // Per-genotype accumulation for genotype code c of SNP j and P column pv.
-  auto step = [&](uint32_t c, const CodeGH<T>& gh, int64_t j, const T (&pv)[K]) {
+  auto step = [&](uint32_t c, const CE& gh, int64_t j, const T (&pv)[K]) {
     T s = qi[0] * pv[0];
 #pragma unroll
     for (int k = 1; k < K; ++k) s = __builtin_fma(qi[k], pv[k], s);
     const T t = (T)1 - s;
     if constexpr (MODE == MODE_LL) {
-      if constexpr (sizeof(T) == 8 && !IMP && !STRICT) {
+      if constexpr (LLPROD) {
         // prod_j s^g (1-s)^(2-g), renormalised every 8 SNPs; one log per lane.
-        const double x = (c >= 2u) ? (double)s : (double)t;
-        const double y = (c == 3u) ? (double)s : (double)t;
-        prod *= (c == 1u) ? 1.0 : x * y;
+        const double x = __builtin_fma(gh.ax, (double)s, gh.bx);
+        const double y = __builtin_fma(gh.ay, (double)s, gh.by);
+        prod *= x * y;
       } else {
         // Terms in T, accumulated in Float64 (src/loops.jl:523, 535); a missing
         // entry adds 0 log s + 0 log(1-s), NaN where s is 0 or 1, as there.
@@ -217,7 +222,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
         w1 = *reinterpret_cast<const uint32_t*>(wrow + a.pitch * (js + 16) + 16);
       }
       auto code = [&](int u) { return (cur[u >> 2] >> (8 * (u & 3) + shw)) & 3u; };
-      CodeGH<T> ghn = ctab[code(0)];
+      CE ghn = ctab[code(0)];
 #pragma unroll
       for (int u = 0; u < 16; ++u) {
         T pv[K];
@@ -226,7 +231,7 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
         const int64_t jn = (u < 15) ? js + u + 1 : ((js + 16 < je) ? js + 16 : js + 15);
 #pragma unroll
         for (int k = 0; k < K; ++k) pn[k] = p[(int64_t)K * jn + k];
-        const CodeGH<T> gh = ghn;
+        const CE gh = ghn;
         if (u < 15) ghn = ctab[code(u + 1)];
         step(code(u), gh, js + u, pv);
         if ((u & 7) == 7) renorm();
