@@ -233,6 +233,9 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
         for (int k = 0; k < K; ++k) pn[k] = p[(int64_t)K * jn + k];
         const CE gh = ghn;
         if (u < 15) ghn = ctab[code(u + 1)];
+        // the loads above (the next P column by scalar loads, the next code's
+        // constants from LDS) stay ahead of this genotype's arithmetic
+        __builtin_amdgcn_sched_barrier(0);
         step(code(u), gh, js + u, pv);
         if ((u & 7) == 7) renorm();
       }
