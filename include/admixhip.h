@@ -297,6 +297,13 @@ int admx_grad_hessian(admx_ctx *ctx, int side);
 /* ---- algorithms on the device-resident state */
 /* loglikelihood(g, q, p, ...) of state vector `which` (src/algorithms_inner.jl:27-35) */
 int admx_loglikelihood(admx_ctx *ctx, int which, double *out);
+/* The log-likelihoods of state vectors which1 and which2 into out[0], out[1],
+ * each identical to admx_loglikelihood's, from one pass over the genotypes
+ * where admixture_qn! pairs them too (FP64 loop up to K = 8, any streamed
+ * context); otherwise one pass per state.  No single counterpart in the
+ * reference: it fuses the two loglikelihood calls of admixture_qn! (ll_basic
+ * and ll_qn, src/algorithms_outer.jl:109-114, 134-139). */
+int admx_loglikelihood2(admx_ctx *ctx, int which1, int which2, double *out);
 /* em_q!(d, g): x_next.q from x  (src/algorithms_inner.jl:68-92) */
 int admx_em_q(admx_ctx *ctx);
 /* em_p!(d, g): x_next.p from x  (src/algorithms_inner.jl:108-141) */

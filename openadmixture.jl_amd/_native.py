@@ -79,6 +79,7 @@ SIGNATURES = {
     "admx_get_ll": ([_vp, _vp, _vp], _int),
     "admx_get_hessians": ([_vp, _int, _vp, _vp], _int),
     "admx_loglikelihood": ([_vp, _int, _vp], _int),
+    "admx_loglikelihood2": ([_vp, _int, _int, _vp], _int),
     "admx_em_q": ([_vp], _int),
     "admx_em_p": ([_vp], _int),
     "admx_init_em": ([_vp, _i64, _vp], _int),

@@ -292,6 +292,14 @@ int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
       if (mode == MODE_LL) {
         a.part = static_cast<double*>(c->llpart.p) + nbx * first[k];
         a.strict = flags;
+        if (q2 && flags && c->K <= 8) {  // both states in one pass over the genotypes
+          a.q2 = q2;
+          a.p2 = p2;
+          a.part2 = static_cast<double*>(c->llpart.p) + nbx * (total + first[k]);
+          a.strict2 = flags + 1;
+          launch_qside_ll2((int)c->K, a, ns[k], c->s);
+          return check_launch();
+        }
         if (q2) {
           PassArgs a2 = a;
           a2.q = q2;
@@ -479,8 +487,10 @@ int ll_to_slot(admx_ctx* c, const void* q, const void* p, int64_t i0, int64_t i1
 }
 
 // Full-cohort LLs of two states into two slots from one pass over the
-// genotypes (FP64/FP32 loop): for a streamed context, one BED transfer instead
-// of two; each LL is summed exactly as ll_to_slot sums it.
+// genotypes (FP64/FP32 loop; one fused kernel for the product form, see
+// launch_qside_ll2): each genotype's code is read once for both states, and a
+// streamed context transfers the BED once instead of twice; each LL is summed
+// exactly as ll_to_slot sums it.
 template <typename T>
 int ll2_to_slots(admx_ctx* c, void* x1, int slot1, void* x2, int slot2) {
   int nchunk = 0;
@@ -836,6 +846,17 @@ int dev_lbqn(admx_ctx* c, int64_t iter, const void* x, const void* xn, const voi
   return check_launch();
 }
 
+// Whether an iteration's two log-likelihoods share one pass (ll2_to_slots):
+// always on a streamed context (one BED transfer instead of two), and where
+// ll_to_slot would run the FP64 product form for both up to K = 8 (the fused
+// kernel holds both states' q: measured at I = J = 100,000, 8.6 ms against
+// 2 x 4.7 at K = 8, 6.1 against 2 x 3.7 at K = 4, even at K = 9, and
+// 14.3 against 2 x 6.0 at K = 12, where it drops to four waves per SIMD).
+bool ll_pairs(const admx_ctx* c) {
+  const bool mixed_ll = c->dtype == ADMX_F64 && c->precision == ADMX_PREC_BF16 && !c->impute && !c->stream_snps;
+  return c->stream_snps || (!c->impute && !mixed_ll && c->K <= 8);
+}
+
 // Everything of one admixture_qn! iteration up to the accept kernel, enqueued
 // on the context stream without host synchronisation (capturable).
 template <typename T>
@@ -854,12 +875,14 @@ int enqueue_qn_iteration(admx_ctx* c, int64_t iter, int fix_q, int fix_p) {
   else HIPCHK(hipMemcpyAsync(XN2, X, qbytes, hipMemcpyDeviceToDevice, c->s));
   if (!fix_p) CHK(dev_update_p<T>(c, true, nullptr));
   else HIPCHK(hipMemcpyAsync(c->pp(XN2), c->pp(X), pbytes, hipMemcpyDeviceToDevice, c->s));
-  // A streamed context evaluates ll_basic after the quasi-Newton step (which
-  // does not read it), in the same pass over the genotypes as ll_qn.
-  if (!c->stream_snps) CHK(ll_to_slot<T>(c, XN2, c->pp(XN2), 0, c->I, 0, c->J, LL_BASIC));
+  // ll_basic is evaluated after the quasi-Newton step (which does not read
+  // it), in the same pass over the genotypes as ll_qn, wherever both run the
+  // FP64 loop: the matrix-core log-likelihood of the BF16 form and the
+  // mean-imputation loops keep one pass per state.
+  if (!ll_pairs(c)) CHK(ll_to_slot<T>(c, XN2, c->pp(XN2), 0, c->I, 0, c->J, LL_BASIC));
   if (c->qn_mode == ADMX_QN_LBQN) CHK(dev_lbqn<T>(c, iter, X, XN, XN2, c->U, c->V, XT));
   else CHK(dev_zal<T>(c, iter, X, XN, XN2, c->U, c->V, XT));
-  if (c->stream_snps) CHK(ll2_to_slots<T>(c, XN2, LL_BASIC, XT, LL_QN));
+  if (ll_pairs(c)) CHK(ll2_to_slots<T>(c, XN2, LL_BASIC, XT, LL_QN));
   else CHK(ll_to_slot<T>(c, XT, c->pp(XT), 0, c->I, 0, c->J, LL_QN));
   {
     ProfScope ps(c, "accept");
@@ -1851,6 +1874,26 @@ int admx_loglikelihood(admx_ctx* c, int which, double* out) {
     void* X = c->x[which];
     CHK(BY_DTYPE(c, ll_to_slot, c, X, c->pp(X), 0, c->I, 0, c->J, LL_TMP));
     HIPCHK(hipMemcpyAsync(out, c->lls + LL_TMP, sizeof(double), hipMemcpyDeviceToHost, c->s));
+    HIPCHK(hipStreamSynchronize(c->s));
+    return ADMX_OK;
+  });
+}
+
+int admx_loglikelihood2(admx_ctx* c, int which1, int which2, double* out) {
+  return guarded([&]() -> int {
+    CHK(check_ctx(c));
+    Guard g(c);
+    CHK(need_bed(c));
+    if (which1 < 0 || which1 > 3 || which2 < 0 || which2 > 3 || !out) return bad("bad state vector");
+    void* X1 = c->x[which1];
+    void* X2 = c->x[which2];
+    if (ll_pairs(c)) {
+      CHK(BY_DTYPE(c, ll2_to_slots, c, X1, LL_TMP, X2, LL_TMP2));
+    } else {
+      CHK(BY_DTYPE(c, ll_to_slot, c, X1, c->pp(X1), 0, c->I, 0, c->J, LL_TMP));
+      CHK(BY_DTYPE(c, ll_to_slot, c, X2, c->pp(X2), 0, c->I, 0, c->J, LL_TMP2));
+    }
+    HIPCHK(hipMemcpyAsync(out, c->lls + LL_TMP, 2 * sizeof(double), hipMemcpyDeviceToHost, c->s));
     HIPCHK(hipStreamSynchronize(c->s));
     return ADMX_OK;
   });

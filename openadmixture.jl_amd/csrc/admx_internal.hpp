@@ -29,6 +29,14 @@ struct PassArgs {
   // launch_ll_state_check); the pass then sums g log s + (2-g) log(1-s) term by
   // term instead of the product form.  nullptr: product form.
   const int* strict = nullptr;
+  // second state of the fused two-state log-likelihood pass (launch_qside_ll2):
+  // its q, p, partials and state flag; strict_other (product-form instance)
+  // names the other state's flag: the instance runs only where that flag is set
+  const void* q2 = nullptr;
+  const void* p2 = nullptr;
+  void* part2 = nullptr;
+  const int* strict2 = nullptr;
+  const int* strict_other = nullptr;
   // FP64 update passes, K >= 9: 1 runs the Hessian triangle in two launches of
   // half the accumulators each, 0 in one; -1 the default (from K = 13 on both
   // sides).  Results are identical either way.
@@ -158,6 +166,13 @@ void launch_reduce_sum_if(const double* parts, int64_t nparts, double* out, cons
 // (a.strict non-null): the fallback of a product-form or matrix-core pass.
 void launch_qside_ll_strict(int K, const PassArgs& a, int nchunk, hipStream_t s);
 
+// FP64 product-form log-likelihoods of (a.q, a.p) and (a.q2, a.p2) in one pass
+// over the genotypes (partials at a.part and a.part2, laid out as one
+// single-state pass lays them out), each bit-identical to its single-state
+// pass.  a.strict and a.strict2 must be the two states' check flags: a flagged
+// state is summed by the single-state instances instead.
+void launch_qside_ll2(int K, const PassArgs& a, int nchunk, hipStream_t s);
+
 // ZAL quasi-Newton (src/quasi_newton.jl:5-31) on flat x of length N.
 template <typename T> void launch_zal(int K, int64_t I, int64_t J, const void* x, const void* xn, const void* xn2,
                                       void* U, void* V, void* x_out, int col, int m, int Qcols,
@@ -210,7 +225,7 @@ void launch_repitch(const uint8_t* raw, int64_t ibytes, int64_t real, int64_t ro
 int noalloc_setup(int device, double** out);
 
 // Slots of the device log-likelihood vector.
-enum LLSlot : int { LL_PREV = 0, LL_NEW = 1, LL_BASIC = 2, LL_QN = 3, LL_TMP = 4, LL_NSLOTS = 8 };
+enum LLSlot : int { LL_PREV = 0, LL_NEW = 1, LL_BASIC = 2, LL_QN = 3, LL_TMP = 4, LL_TMP2 = 5, LL_NSLOTS = 8 };
 
 // learned warm start (admx_mlp.hip)
 void launch_features(const uint8_t* bed, int64_t pitch, int64_t I, int64_t Jp, const float* R, const float* mu,

@@ -53,6 +53,11 @@ template <typename T, int K, int MODE, bool TRICK, bool IMP, int PART = 0, bool 
 __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
   if constexpr (MODE == MODE_LL) {
     if (a.strict && ((*a.strict != 0) != STRICT)) return;  // uniform: the other instance runs
+    // product form next to k_qside_ll2: runs only where that pass stepped aside
+    // (the other state was flagged)
+    if constexpr (!STRICT) {
+      if (a.strict_other && *a.strict_other == 0) return;
+    }
   }
   constexpr int NT = Tri<K>::N;
   constexpr int N0 = (PART == 2) ? NT / 2 : 0, N1 = (PART == 1) ? NT / 2 : NT;  // owned triangle entries
@@ -286,6 +291,144 @@ __global__ __launch_bounds__(256) void k_qside(PassArgs a) {
         if constexpr (MODE == MODE_EM && sizeof(T) == 8) out[(o++) * Ilen] = sd;
       }
     }
+  }
+}
+
+// ------------------------------------------------------------------ two log-likelihoods
+// The product-form log-likelihoods of two states (q, p) and (q2, p2) in one
+// pass over the genotypes: each genotype's code and its factor constants are
+// read once for both, and each state's sum is formed exactly as k_qside's
+// product-form instance forms it (same order of products, renormalisation,
+// log and block sum; partials of the second state at a.part2).  Runs only
+// when both state flags are clear; a flagged state takes the single-state
+// instances (see launch_qside_ll2).
+template <int K>
+__global__ __launch_bounds__(256) void k_qside_ll2(PassArgs a) {
+  if (*a.strict != 0 || *a.strict2 != 0) return;  // uniform
+  const double* __restrict__ q = static_cast<const double*>(a.q);
+  const double* __restrict__ p = static_cast<const double*>(a.p);
+  const double* __restrict__ q2 = static_cast<const double*>(a.q2);
+  const double* __restrict__ p2 = static_cast<const double*>(a.p2);
+  const int64_t Ilen = a.i1 - a.i0;
+  const int64_t nbx = (Ilen + 255) / 256;
+  const Tile2 tile = xcd_tile(nbx);
+  const int64_t il = tile.x * 256 + threadIdx.x;
+  const bool valid = il < Ilen;
+  const int64_t i = a.i0 + (valid ? il : 0);
+  const int64_t jb = a.j0 + tile.y * a.chunk;
+  const int64_t je = (a.j1 < jb + a.chunk) ? a.j1 : jb + a.chunk;
+
+  double qa[K], qb[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    qa[k] = q[k + (int64_t)K * i];
+    qb[k] = q2[k + (int64_t)K * i];
+  }
+  double prod = 1.0, prod2 = 1.0;
+  int esum = 0, esum2 = 0;
+  __shared__ CodeLL ctab[4];
+  code_table_init(ctab);
+  auto step = [&](const CodeLL& e, const double (&pv)[K], const double (&pw)[K]) {
+    double s = qa[0] * pv[0], s2 = qb[0] * pw[0];
+#pragma unroll
+    for (int k = 1; k < K; ++k) {
+      s = __builtin_fma(qa[k], pv[k], s);
+      s2 = __builtin_fma(qb[k], pw[k], s2);
+    }
+    prod *= __builtin_fma(e.ax, s, e.bx) * __builtin_fma(e.ay, s, e.by);
+    prod2 *= __builtin_fma(e.ax, s2, e.bx) * __builtin_fma(e.ay, s2, e.by);
+  };
+  auto renorm = [&]() {
+    int ex;
+    prod = __builtin_frexp(prod, &ex);
+    esum += ex;
+    prod2 = __builtin_frexp(prod2, &ex);
+    esum2 += ex;
+  };
+
+  // Genotype codes 16 SNPs per step through a per-wave LDS transpose, as in k_qside.
+  __shared__ __align__(16) uint8_t tbuf[4][512];
+  uint8_t* __restrict__ tb = tbuf[threadIdx.x >> 6];
+  const int lane = threadIdx.x & 63;
+  const int64_t wrow0 = tile.x * 256 + (threadIdx.x & ~63);
+  const bool wave_live = wrow0 < Ilen;  // wave-uniform
+  const int64_t ifirst = a.i0 + wrow0;
+  const int64_t wbase = (ifirst >> 2) & ~(int64_t)15;
+  const int bsel = (int)(((ifirst + lane) >> 2) - wbase);
+  const uint32_t shw = (uint32_t)((ifirst + lane) & 3) * 2u;
+  const uint8_t* __restrict__ wrow = a.bed + wbase + 4 * (lane & 3) + a.pitch * (lane >> 2);
+
+  int64_t js = jb;
+  if (wave_live && je - jb >= 16) {
+    uint32_t w0 = *reinterpret_cast<const uint32_t*>(wrow + a.pitch * jb);
+    uint32_t w1 = *reinterpret_cast<const uint32_t*>(wrow + a.pitch * jb + 16);
+    double pn[K], pn2[K];
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+      pn[k] = p[(int64_t)K * jb + k];
+      pn2[k] = p2[(int64_t)K * jb + k];
+    }
+    for (; js + 16 <= je; js += 16) {
+#pragma unroll
+      for (int e = 0; e < 4; ++e) {
+        tb[(4 * (lane & 3) + e) * 16 + (lane >> 2)] = (uint8_t)(w0 >> (8 * e));
+        tb[(16 + 4 * (lane & 3) + e) * 16 + (lane >> 2)] = (uint8_t)(w1 >> (8 * e));
+      }
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      const u32x4 cur = *reinterpret_cast<const u32x4*>(tb + 16 * bsel);
+      __builtin_amdgcn_fence(__ATOMIC_SEQ_CST, "wavefront");
+      if (js + 32 <= je) {
+        w0 = *reinterpret_cast<const uint32_t*>(wrow + a.pitch * (js + 16));
+        w1 = *reinterpret_cast<const uint32_t*>(wrow + a.pitch * (js + 16) + 16);
+      }
+      auto code = [&](int u) { return (cur[u >> 2] >> (8 * (u & 3) + shw)) & 3u; };
+      CodeLL en = ctab[code(0)];
+#pragma unroll
+      for (int u = 0; u < 16; ++u) {
+        double pv[K], pw[K];
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          pv[k] = pn[k];
+          pw[k] = pn2[k];
+        }
+        const int64_t jn = (u < 15) ? js + u + 1 : ((js + 16 < je) ? js + 16 : js + 15);
+#pragma unroll
+        for (int k = 0; k < K; ++k) {
+          pn[k] = p[(int64_t)K * jn + k];
+          pn2[k] = p2[(int64_t)K * jn + k];
+        }
+        const CodeLL e = en;
+        if (u < 15) en = ctab[code(u + 1)];
+        __builtin_amdgcn_sched_barrier(0);  // the loads above stay a genotype ahead
+        step(e, pv, pw);
+        if ((u & 7) == 7) renorm();
+      }
+    }
+  }
+  if (wave_live) {
+    const uint8_t* __restrict__ gp = a.bed + (i >> 2);
+    const uint32_t sh = (uint32_t)(i & 3) * 2u;
+    for (; js < je; ++js) {
+      double pv[K], pw[K];
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        pv[k] = p[(int64_t)K * js + k];
+        pw[k] = p2[(int64_t)K * js + k];
+      }
+      step(ctab[(gp[a.pitch * js] >> sh) & 3u], pv, pw);
+      renorm();
+    }
+  }
+
+  __shared__ double sh_red[256];
+  double acc = 0.0, acc2 = 0.0;
+  acc += log(prod) + (double)esum * 0.69314718055994530942;
+  acc2 += log(prod2) + (double)esum2 * 0.69314718055994530942;
+  const double tot = block_sum<256>(valid ? acc : 0.0, sh_red);
+  const double tot2 = block_sum<256>(valid ? acc2 : 0.0, sh_red);
+  if (threadIdx.x == 0) {
+    static_cast<double*>(a.part)[tile.y * nbx + tile.x] = tot;
+    static_cast<double*>(a.part2)[tile.y * nbx + tile.x] = tot2;
   }
 }
 
@@ -538,6 +681,26 @@ void launch_qside_ll_strict(int K, const PassArgs& a, int nchunk, hipStream_t s)
   ADMX_DISPATCH_K(K, (qside_ll_strict_k<KK>(a, nchunk, s)));
 }
 template void launch_pside<ADMX_T>(int, int, const PassArgs&, int, hipStream_t);
+
+template <int K>
+static void qside_ll2_k(const PassArgs& a, int nchunk, hipStream_t s) {
+  const dim3 grid((unsigned)(((a.i1 - a.i0 + 255) / 256) * nchunk));
+  hipLaunchKernelGGL((k_qside_ll2<K>), grid, dim3(256), 0, s, a);
+  // a flagged state: single-state instances (each returns at once where the
+  // fused pass ran or where its own flag selects the other form)
+  PassArgs a1 = a, a2 = a;
+  a1.strict_other = a.strict2;
+  a2.q = a.q2;
+  a2.p = a.p2;
+  a2.part = a.part2;
+  a2.strict = a.strict2;
+  a2.strict_other = a.strict;
+  qside_ki<double, K, false>(MODE_LL, a1, nchunk, s);
+  qside_ki<double, K, false>(MODE_LL, a2, nchunk, s);
+}
+void launch_qside_ll2(int K, const PassArgs& a, int nchunk, hipStream_t s) {
+  ADMX_DISPATCH_K(K, (qside_ll2_k<KK>(a, nchunk, s)));
+}
 #endif
 
 }  // namespace admx

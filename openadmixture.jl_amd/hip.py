@@ -254,6 +254,14 @@ class HIPAdmixData:
         N.check(N.lib().admx_loglikelihood(self._h, which, C.byref(out)))
         return out.value
 
+    def loglikelihood2(self, which1: int, which2: int):
+        """Log-likelihoods of two state vectors, from one pass over the
+        genotypes where admixture_qn! pairs them too (each identical to
+        loglikelihood's)."""
+        out = (C.c_double * 2)()
+        N.check(N.lib().admx_loglikelihood2(self._h, which1, which2, out))
+        return out[0], out[1]
+
     EM_MODES = {"base": N.ADMX_EM_BASE, "fallback": N.ADMX_EM_FALLBACK, "fallback2": N.ADMX_EM_FALLBACK2}
 
     def em_q(self, mode: str = "base"):

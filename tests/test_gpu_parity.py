@@ -468,6 +468,62 @@ def test_loglikelihood_degenerate_states(oa, O):
     c.close()
 
 
+def _same(a, b):
+    return (np.isnan(a) and np.isnan(b)) or a == b
+
+
+@pytest.mark.parametrize("K", [3, 8])
+def test_loglikelihood2_equals_two_single_passes(oa, O, K):
+    """admx_loglikelihood2 (both states' log-likelihoods in one pass over the
+    genotypes, as admixture_qn! pairs ll_basic and ll_qn) returns exactly what
+    two admx_loglikelihood calls return, bit for bit, for every pairing of
+    plain and degenerate states: both clear (the fused product-form kernel),
+    one flagged (the other state falls back to its single-state product form),
+    both flagged (term by term)."""
+    note("loglikelihood2 pairs", K)
+    I, J = 300, 64
+    _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.02, seed=61)
+    B = bed.reshape((pitch, J), order="F").copy()
+    B[: I // 4, 21] = 0x00  # SNP 21: monomorphic, every genotype 0
+    bed2 = np.asfortranarray(B).ravel(order="F")
+    c = ctx_for(oa, bed2, pitch, I, J, K)
+    q, p = random_state(I, J, K, 7)
+    q2, p2 = random_state(I, J, K, 8)
+    states = {"plain": (q, p), "plain2": (q2, p2)}
+    for name, (cols, val) in {"zero_p_mono_snp": ([21], 0.0), "one_p": ([7], 1.0), "nan_p": ([9], np.nan)}.items():
+        pp = p2.copy()
+        pp[:, cols] = val
+        states[name] = (q2, np.asfortranarray(pp))
+    for n1, (qa, pa) in states.items():
+        for n2, (qb, pb) in states.items():
+            c.set_x(2, np.asfortranarray(np.hstack([qa, pa])))
+            c.set_x(3, np.asfortranarray(np.hstack([qb, pb])))
+            r1, r2 = c.loglikelihood(2), c.loglikelihood(3)
+            f1, f2 = c.loglikelihood2(2, 3)
+            print(n1, n2, r1, r2, f1, f2, flush=True)
+            assert _same(f1, r1) and _same(f2, r2), (n1, n2)
+    c.close()
+
+
+@pytest.mark.parametrize("I,J,K", [(1003, 2021, 8), (4099, 257, 5), (300, 40, 12)])
+def test_loglikelihood2_random_states(oa, O, I, J, K):
+    """Two-state log-likelihood on ragged cohorts (fused up to K = 8): equal to
+    the two single passes bit for bit and to the oracle within 1e-10."""
+    note("loglikelihood2", I, J, K)
+    _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.03, seed=I + K)
+    c = ctx_for(oa, bed, pitch, I, J, K)
+    q, p = random_state(I, J, K, 3)
+    q2, p2 = random_state(I, J, K, 4)
+    c.set_x(0, np.asfortranarray(np.hstack([q, p])))
+    c.set_x(1, np.asfortranarray(np.hstack([q2, p2])))
+    f1, f2 = c.loglikelihood2(0, 1)
+    assert f1 == c.loglikelihood(0) and f2 == c.loglikelihood(1)
+    for got, (qq, pp) in ((f1, (q, p)), (f2, (q2, p2))):
+        ref = O.loglikelihood(bed, pitch, qq, pp)
+        assert abs(got - ref) <= 1e-10 * abs(ref)
+    c.close()
+
+
 def test_em_on_a_monomorphic_snp_matches_reference(oa, O):
     """FRAPPE EM from a random start on a cohort with a SNP whose genotypes
     are all 0: p of that SNP becomes exactly 0 after the first step, so the
