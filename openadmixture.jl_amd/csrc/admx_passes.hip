@@ -496,10 +496,12 @@ __global__ __launch_bounds__(256) void k_pside(PassArgs a) {
 #pragma unroll
     for (int k = 0; k < K; ++k) qv[k] = qn[k];
     loadq(next);
-    // EM: the next q column's scalar loads and the next code's LDS read stay a
-    // genotype ahead of this genotype's arithmetic (I = J = 100,000, K = 8:
-    // 12.25 -> 11.0 ms; the update pass is no faster with it)
-    if constexpr (MODE == MODE_EM) __builtin_amdgcn_sched_barrier(0);
+    // the next q column's scalar loads and the next code's LDS read stay a
+    // genotype ahead of this genotype's arithmetic: EM, and the update pass
+    // from K = 12 (I = J = 100,000: em_p 12.25 -> 11.0 ms at K = 8, update_p
+    // 39.5 -> 37.8 ms at K = 12; the update pass no faster at K = 4, 8 and 11,
+    // about 1% slower at 9 and 10)
+    if constexpr (MODE == MODE_EM || K >= 12) __builtin_amdgcn_sched_barrier(0);
     T s = qv[0] * pj[0];
 #pragma unroll
     for (int k = 1; k < K; ++k) s = __builtin_fma(qv[k], pj[k], s);
