@@ -292,7 +292,7 @@ int pass_q(admx_ctx* c, int mode, const void* q, const void* p, int64_t i0, int6
       if (mode == MODE_LL) {
         a.part = static_cast<double*>(c->llpart.p) + nbx * first[k];
         a.strict = flags;
-        if (q2 && flags && c->K <= 8) {  // both states in one pass over the genotypes
+        if (q2 && flags) {  // both states (one fused pass over the genotypes up to K = 8)
           a.q2 = q2;
           a.p2 = p2;
           a.part2 = static_cast<double*>(c->llpart.p) + nbx * (total + first[k]);

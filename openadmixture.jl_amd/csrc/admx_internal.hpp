@@ -166,11 +166,12 @@ void launch_reduce_sum_if(const double* parts, int64_t nparts, double* out, cons
 // (a.strict non-null): the fallback of a product-form or matrix-core pass.
 void launch_qside_ll_strict(int K, const PassArgs& a, int nchunk, hipStream_t s);
 
-// FP64 product-form log-likelihoods of (a.q, a.p) and (a.q2, a.p2) in one pass
-// over the genotypes (partials at a.part and a.part2, laid out as one
-// single-state pass lays them out), each bit-identical to its single-state
-// pass.  a.strict and a.strict2 must be the two states' check flags: a flagged
-// state is summed by the single-state instances instead.
+// FP64 log-likelihoods of (a.q, a.p) and (a.q2, a.p2) (partials at a.part and
+// a.part2, laid out as one single-state pass lays them out), each bit-identical
+// to its single-state pass: up to K = 8 one fused pass over the genotypes for
+// both product forms, from K = 9 on two passes.  a.strict and a.strict2 must be
+// the two states' check flags: a flagged state is summed by the single-state
+// instances.
 void launch_qside_ll2(int K, const PassArgs& a, int nchunk, hipStream_t s);
 
 // ZAL quasi-Newton (src/quasi_newton.jl:5-31) on flat x of length N.

@@ -694,17 +694,21 @@ template void launch_pside<ADMX_T>(int, int, const PassArgs&, int, hipStream_t);
 
 template <int K>
 static void qside_ll2_k(const PassArgs& a, int nchunk, hipStream_t s) {
-  const dim3 grid((unsigned)(((a.i1 - a.i0 + 255) / 256) * nchunk));
-  hipLaunchKernelGGL((k_qside_ll2<K>), grid, dim3(256), 0, s, a);
-  // a flagged state: single-state instances (each returns at once where the
-  // fused pass ran or where its own flag selects the other form)
   PassArgs a1 = a, a2 = a;
-  a1.strict_other = a.strict2;
   a2.q = a.q2;
   a2.p = a.p2;
   a2.part = a.part2;
   a2.strict = a.strict2;
-  a2.strict_other = a.strict;
+  // Up to K = 8 the fused kernel sums both states; the single-state instances
+  // then run only for a flagged state (each returns at once where the fused
+  // pass ran or where its own flag selects the other form).  From K = 9 on
+  // (where the fused kernel holds too few waves) they run as two passes.
+  if constexpr (K <= 8) {
+    const dim3 grid((unsigned)(((a.i1 - a.i0 + 255) / 256) * nchunk));
+    hipLaunchKernelGGL((k_qside_ll2<K>), grid, dim3(256), 0, s, a);
+    a1.strict_other = a.strict2;
+    a2.strict_other = a.strict;
+  }
   qside_ki<double, K, false>(MODE_LL, a1, nchunk, s);
   qside_ki<double, K, false>(MODE_LL, a2, nchunk, s);
 }
