@@ -201,6 +201,16 @@ def test_c3_loglikelihood_f64(oa, O, c3):
     note(f"c3 LL oracle {ref:.12e} device {dev:.12e} loop {loop:.12e}")
     assert abs(dev - ref) <= 1e-10 * abs(ref)
     assert abs(loop - ref) <= 1e-10 * abs(ref)
+    # both log-likelihoods of an iteration from one pass (as admixture_qn! pairs
+    # them): bit-identical to the single passes, the second state within 1e-10
+    p2 = np.asfortranarray(0.9 * p + 0.05)
+    x2 = np.asfortranarray(np.hstack([q, p2]))
+    c.set_x(1, x2)
+    f1, f2 = c.loglikelihood2(0, 1)
+    ref2 = O.loglikelihood(bed.ravel(order="F"), pitch, q, p2)
+    note(f"c3 LL pair {f1:.12e} {f2:.12e} oracle {ref2:.12e}")
+    assert f1 == dev and f2 == c.loglikelihood(1)
+    assert abs(f2 - ref2) <= 1e-10 * abs(ref2)
 
 
 def _sub_oracle_q(O, bed, x, n):
