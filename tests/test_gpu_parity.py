@@ -505,22 +505,25 @@ def test_loglikelihood2_equals_two_single_passes(oa, O, K):
     c.close()
 
 
-@pytest.mark.parametrize("I,J,K", [(1003, 2021, 8), (4099, 257, 5), (300, 40, 12)])
-def test_loglikelihood2_random_states(oa, O, I, J, K):
+@pytest.mark.parametrize("I,J,K,T", [(1003, 2021, 8, F64), (4099, 257, 5, F64), (300, 40, 12, F64), (777, 1500, 6, F32)])
+def test_loglikelihood2_random_states(oa, O, I, J, K, T):
     """Two-state log-likelihood on ragged cohorts (fused up to K = 8): equal to
-    the two single passes bit for bit and to the oracle within 1e-10."""
-    note("loglikelihood2", I, J, K)
+    the two single passes bit for bit and to the oracle within 1e-10 (Float64)
+    or 1e-5 (a Float32 context, whose passes run on FP64 copies of its state)."""
+    note("loglikelihood2", I, J, K, T.__name__)
     _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.03, seed=I + K)
-    c = ctx_for(oa, bed, pitch, I, J, K)
+    c = ctx_for(oa, bed, pitch, I, J, K, T=T)
     q, p = random_state(I, J, K, 3)
     q2, p2 = random_state(I, J, K, 4)
-    c.set_x(0, np.asfortranarray(np.hstack([q, p])))
-    c.set_x(1, np.asfortranarray(np.hstack([q2, p2])))
+    c.set_x(0, np.asfortranarray(np.hstack([q, p]), dtype=T))
+    c.set_x(1, np.asfortranarray(np.hstack([q2, p2]), dtype=T))
     f1, f2 = c.loglikelihood2(0, 1)
     assert f1 == c.loglikelihood(0) and f2 == c.loglikelihood(1)
+    tol = 1e-10 if T is F64 else 1e-5
     for got, (qq, pp) in ((f1, (q, p)), (f2, (q2, p2))):
-        ref = O.loglikelihood(bed, pitch, qq, pp)
-        assert abs(got - ref) <= 1e-10 * abs(ref)
+        ref = O.loglikelihood(bed, pitch, np.asfortranarray(qq.astype(T).astype(F64)),
+                              np.asfortranarray(pp.astype(T).astype(F64)))
+        assert abs(got - ref) <= tol * abs(ref)
     c.close()
 
 
