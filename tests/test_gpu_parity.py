@@ -505,7 +505,8 @@ def test_loglikelihood2_equals_two_single_passes(oa, O, K):
     c.close()
 
 
-@pytest.mark.parametrize("I,J,K,T", [(1003, 2021, 8, F64), (4099, 257, 5, F64), (300, 40, 12, F64), (777, 1500, 6, F32)])
+@pytest.mark.parametrize("I,J,K,T", [(1003, 2021, 8, F64), (4099, 257, 5, F64), (300, 40, 12, F64), (777, 1500, 6, F32),
+                                     (13, 7, 3, F64), (1, 1, 2, F64)])
 def test_loglikelihood2_random_states(oa, O, I, J, K, T):
     """Two-state log-likelihood on ragged cohorts (fused up to K = 8): equal to
     the two single passes bit for bit and to the oracle within 1e-10 (Float64)
