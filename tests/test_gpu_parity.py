@@ -505,6 +505,23 @@ def test_loglikelihood2_equals_two_single_passes(oa, O, K):
     c.close()
 
 
+def test_loglikelihood2_bf16_form_takes_one_pass_per_state(oa, O):
+    """In the BF16 form each state's log-likelihood runs on the matrix cores in
+    its own pass: loglikelihood2 returns exactly the two single calls."""
+    note("loglikelihood2 bf16")
+    I, J, K = 1500, 4100, 8
+    _, _, bed, pitch = make_cohort(O, I, J, K, miss=0.02, seed=17)
+    c = ctx_for(oa, bed, pitch, I, J, K)
+    c.set_precision("bf16")
+    q, p = random_state(I, J, K, 5)
+    q2, p2 = random_state(I, J, K, 6)
+    c.set_x(0, np.asfortranarray(np.hstack([q, p])))
+    c.set_x(1, np.asfortranarray(np.hstack([q2, p2])))
+    f1, f2 = c.loglikelihood2(0, 1)
+    assert f1 == c.loglikelihood(0) and f2 == c.loglikelihood(1)
+    c.close()
+
+
 @pytest.mark.parametrize("I,J,K,T", [(1003, 2021, 8, F64), (4099, 257, 5, F64), (300, 40, 12, F64), (777, 1500, 6, F32),
                                      (13, 7, 3, F64), (1, 1, 2, F64)])
 def test_loglikelihood2_random_states(oa, O, I, J, K, T):
